@@ -1,0 +1,199 @@
+"""Benchmark of the massless FFEA time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1] [--no-cpu]
+
+Workload (default "c1"): one dynamic blob, a cube of 48^3 unit cells (h = 1 nm) split 6 tets
+per cell (Kuhn), 117,649 linear nodes / 912,673 nodes with midpoints / 663,552 tets; material
+1500 kg/m^3, shear/bulk viscosity 1e-3 Pa s, shear modulus 3.7e8 Pa, bulk modulus 1.1e9 Pa,
+Stokes radius 0.5 nm on linear nodes, solvent viscosity 1e-3 Pa s; T = 300 K, dt = 1e-14 s,
+Jacobi-PCG to relative eps = 1e-6, thermal noise on (RngStream seeded by rng_seed 44),
+initial positions rest + N(0, (0.02 h)^2).  A step = one World::run iteration: element phase,
+assembly, force gather + noise, CG solve, integration -- all on the GPU, inputs resident in HBM.
+
+`value` is FP64-path steps/s (the reference's arithmetic); the mixed path (FP32 element,
+assembly and SpMV storage) is reported beside it.  Kernel durations come from HIP events the
+library records on its own stream around every launch in the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: (cells, sigma_h, description)
+    "c0": ((8, 8, 8), "cube 8^3 cells, Kuhn 6-tet split: 729 linear nodes, 3,072 tets"),
+    "c1": ((48, 48, 48), "cube 48^3 cells, Kuhn 6-tet split: 117,649 linear nodes, 663,552 tets"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(cfg):
+    from ffea_amd import mesh as M, units as U
+    cells, desc = CONFIGS[cfg]
+    m = M.box_mesh(*cells)
+    x0 = M.perturb(m, 0.02, seed=1)
+    prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
+    return m, prm, U.blob_scale(1e-9), x0, desc
+
+
+def algorithmic_bytes(kernel, sizes, bm):
+    """Bytes each launch must move (each operand once), per kernel; bm = matrix value bytes."""
+    N, E, B, S = sizes["num_linear"], sizes["num_elements"], sizes["num_blocks"], sizes["num_slots"]
+    v = 24 * N  # one FP64 3-vector over the linear nodes
+    return {
+        "element": E * (16 + 72 + 8 + 32 + 16 + 12 * bm + 90 * bm) + v,
+        "assemble": E * 16 * (9 * bm + 4) + 4 * (S + 1) + 5 * S + 9 * bm * S + v,
+        "rhs": E * (12 * bm + 16) + 4 * (N + 1) + 8 * N + 1 * N + 4 * N + 6 * v,
+        "cg_spmv": 9 * bm * B + 4 * B + 5 * v,
+        "cg_update": 7 * v,
+        "integrate": 4 * v,
+    }[kernel]
+
+
+def run_gpu(m, prm, sc, x0, precision, steps, warmup):
+    from ffea_amd.blob import Blob, package_seed
+    t0 = time.time()
+    b = Blob(m, prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision, pos=x0)
+    log(f"[{precision}] blob created in {time.time() - t0:.1f}s: {b.sizes()}")
+    b.step(warmup)
+    b.synchronize()
+    b.set_profiling(True)
+    iters = []
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step(1)
+        iters.append(b.last_cg()[0])
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    kt = b.kernel_times()
+    b.set_profiling(False)
+    sizes = b.sizes()
+    log(f"[{precision}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s, CG iterations "
+        f"mean {np.mean(iters):.2f}")
+    b.close()
+    return steps / dt, dt, kt, sizes, float(np.mean(iters))
+
+
+def kernel_table(kt, sizes, bm):
+    out = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        byts = algorithmic_bytes(k, sizes, bm)
+        out[k] = {"launches": cnt, "avg_us": avg_s * 1e6, "total_ms": ms,
+                  "alg_bytes": int(byts), "achieved_GBs": byts / avg_s / 1e9}
+    return out
+
+
+def cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
+    """The reference's own compiled objects (oracle/_ref, -O3 -ffast-math, OpenMP within the
+    blob) on this host, falling back to the FP64 restatement; a bounded sample of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    kind, lib = None, None
+    if os.path.exists(pyoracle.REF_FAST_SO):
+        kind, lib = "reference", pyoracle.Reference(pyoracle.REF_FAST_SO)
+    elif os.path.exists(pyoracle.ORACLE_SO):
+        kind, lib = "port", pyoracle.Oracle()
+    else:
+        return None
+    cores = os.cpu_count() or 1
+    if kind == "reference":
+        cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+    seed = lib.package_seed(prm.rng_seed)
+    lib.set_package_seed(seed)
+    s = lib.stream()
+    t0 = time.time()
+    blob = lib.blob(m, prm, sc, pos=x0)
+    setup = time.time() - t0
+    log(f"[cpu {kind}] setup {setup:.1f}s")
+    assert pyoracle.world_step(lib, [blob], s) == 0  # warm-up step
+    n, t0 = 0, time.perf_counter()
+    while True:
+        assert pyoracle.world_step(lib, [blob], s) == 0
+        n += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds or n >= 50:
+            break
+    blob.close()
+    log(f"[cpu {kind}] {n} steps in {el:.2f}s")
+    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": kind,
+            "sample": f"{n} noise-on steps of the same {cfg} workload after 1 warm-up step "
+                      f"({'reference objects -O3 -ffast-math, OpenMP within blob' if kind == 'reference' else 'FP64 restatement, 1 thread'}); "
+                      f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(independent blobs shard by blob, one large blob by node slabs: see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    m, prm, sc, x0, desc = make_case(a.config)
+    log(f"workload {a.config}: {desc}")
+    sps, wall, kt, sizes, iters = run_gpu(m, prm, sc, x0, "fp64", a.steps, a.warmup)
+    ktab = kernel_table(kt, sizes, 8)
+    mixed = None
+    if not a.no_mixed:
+        sps_m, wall_m, kt_m, sizes_m, iters_m = run_gpu(m, prm, sc, x0, "mixed", a.steps, a.warmup)
+        mixed = {"steps_per_s": sps_m, "ns_per_day": sps_m * 1e-5 * 86400, "cg_iters_mean": iters_m,
+                 "kernels": kernel_table(kt_m, sizes_m, 4)}
+    dom = max(ktab, key=lambda k: ktab[k]["total_ms"])
+    d = ktab[dom]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
+    cpu = None if a.no_cpu else cpu_baseline(a.config, m, prm, sc, x0, a.cpu_seconds)
+    out = {
+        "metric": "steps/s at CG eps=1e-6 (FP64 path), 48^3 cube",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Kuhn box mesh, seeded Gaussian initial displacement, RngStream noise",
+        "config": {"workload": f"{a.config}: {desc}; eps 1e-6, noise on, dt 1e-14 s",
+                   "linear_nodes": sizes["num_linear"], "elements": sizes["num_elements"],
+                   "blocks": sizes["num_blocks"], "parallelism": "single GPU"},
+        "ns_per_day": sps * 1e-5 * 86400,
+        "cg_iters_mean": iters,
+        "kernels": ktab,
+        "mixed": mixed,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's Blob / NoMassCGSolver interface over the C-ABI.
+
+`Blob` follows include/Blob.h (init :124, update_internal_forces :130,
+update_positions :140) and `GpuNoMassCGSolver` follows include/Solver.h:30-55
+(init / solve / apply_matrix).  Arrays use FFEA's full node numbering (N_t nodes
+including quadratic midpoints) and meso units, as the reference holds them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .units import SimParams
+
+PRECISIONS = {"fp64": _lib.FFH_PRECISION_FP64, "mixed": _lib.FFH_PRECISION_MIXED}
+
+
+def package_seed(rng_seed: int) -> np.ndarray:
+    """World::init RNG.2 (src/World.cpp:299-305): six RngStream seeds drawn from the host
+    libc's srand/rand; the int sum rand()+rand() wraps before the unsigned conversion.
+    With one thread the first RngStream starts from exactly these seeds."""
+    libc = C.CDLL(None)
+    libc.srand(C.c_uint(rng_seed & 0xffffffff))
+    two_max = (4294967087, 4294944443)
+    out = np.zeros(6, np.uint64)
+    for i in range(6):
+        s = (libc.rand() + libc.rand()) & 0xffffffff
+        if s >= 1 << 31:
+            s -= 1 << 32
+        out[i] = (s % (1 << 64)) % two_max[i // 3]
+    return out
+
+
+class Blob:
+    """One dynamic blob on the GPU (Blob + NoMassCGSolver + its RngStream).
+
+    mesh: an object with pos [N_t,3], topo [E,10], material [E,6] (SI), stokes_radius [N_t],
+    pinned (node ids), in the blob's input units; scale: the blob `scale` already divided by
+    the meso length (units.blob_scale).
+    """
+
+    def __init__(self, mesh, params: SimParams, scale: float, rng_state=None, precision="fp64",
+                 pos=None):
+        self._L = _lib.load()
+        self.num_nodes = int(mesh.pos.shape[0])
+        self.num_elements = int(mesh.topo.shape[0])
+        self.params = params
+        rest = np.ascontiguousarray(np.asarray(mesh.pos, np.float64) * scale).ravel()
+        topo = np.ascontiguousarray(mesh.topo, np.int32).ravel()
+        mat = np.ascontiguousarray(mesh.material, np.float64).ravel()
+        rad = np.ascontiguousarray(np.asarray(mesh.stokes_radius, np.float64) * scale)
+        pinned = np.ascontiguousarray(getattr(mesh, "pinned", np.zeros(0)), np.int32)
+        if rng_state is None:
+            rng_state = package_seed(params.rng_seed)
+        st = np.ascontiguousarray(rng_state, np.uint64)
+        self._h = self._L.ffh_blob_create(
+            self.num_nodes, self.num_elements, rest, topo, mat, rad, int(pinned.size),
+            pinned if pinned.size else np.zeros(1, np.int32), params.dt, params.kT,
+            params.epsilon2, params.max_iterations_cg, params.calc_noise, params.calc_stokes,
+            params.stokes_visc, st, PRECISIONS[precision])
+        if not self._h:
+            raise _lib.FFHError(-1, _lib.last_error())
+        self.precision = precision
+        if pos is not None:
+            self.set_positions(np.asarray(pos, np.float64) * scale)
+
+    # -- lifetime -----------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ffh_blob_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- the step -------------------------------------------------------------------
+    def update_internal_forces(self) -> int:
+        n = C.c_int(0)
+        _lib.check(self._L.ffh_blob_update_internal_forces(self._h, C.byref(n)))
+        return n.value
+
+    def update_positions(self) -> None:
+        _lib.check(self._L.ffh_blob_update_positions(self._h))
+
+    def step(self, nsteps: int = 1) -> None:
+        _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
+
+    # -- solver interface -------------------------------------------------------------
+    def solve(self, force) -> np.ndarray:
+        x = np.ascontiguousarray(force, np.float64).copy().ravel()
+        _lib.check(self._L.ffh_blob_solve(self._h, x))
+        return x
+
+    def apply_matrix(self, vec) -> np.ndarray:
+        out = np.zeros(3 * self.num_nodes, np.float64)
+        _lib.check(self._L.ffh_blob_apply_matrix(
+            self._h, np.ascontiguousarray(vec, np.float64).ravel(), out))
+        return out
+
+    # -- state ----------------------------------------------------------------------------
+    def set_positions(self, pos_meso) -> None:
+        _lib.check(self._L.ffh_blob_set_positions(
+            self._h, np.ascontiguousarray(pos_meso, np.float64).ravel()))
+
+    def _get(self, fn) -> np.ndarray:
+        out = np.zeros(3 * self.num_nodes, np.float64)
+        _lib.check(fn(self._h, out))
+        return out.reshape(-1, 3)
+
+    def positions(self) -> np.ndarray:
+        return self._get(self._L.ffh_blob_get_positions)
+
+    def velocities(self) -> np.ndarray:
+        return self._get(self._L.ffh_blob_get_velocities)
+
+    def rhs(self) -> np.ndarray:
+        return self._get(self._L.ffh_blob_get_rhs)
+
+    def rng_state(self) -> np.ndarray:
+        out = np.zeros(6, np.uint64)
+        _lib.check(self._L.ffh_blob_get_rng_state(self._h, out))
+        return out
+
+    def set_rng_state(self, st) -> None:
+        _lib.check(self._L.ffh_blob_set_rng_state(self._h, np.ascontiguousarray(st, np.uint64)))
+
+    def last_cg(self) -> tuple[int, float]:
+        it, res = C.c_int(0), C.c_double(0)
+        _lib.check(self._L.ffh_blob_last_cg(self._h, C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    def sizes(self) -> dict:
+        n, e, b, s = C.c_int(), C.c_int(), C.c_longlong(), C.c_longlong()
+        _lib.check(self._L.ffh_blob_sizes(self._h, C.byref(n), C.byref(e), C.byref(b), C.byref(s)))
+        return {"num_linear": n.value, "num_elements": e.value, "num_blocks": b.value,
+                "num_slots": s.value}
+
+    def stream(self) -> int:
+        return int(self._L.ffh_blob_stream(self._h) or 0)
+
+    def synchronize(self) -> None:
+        _lib.check(self._L.ffh_blob_synchronize(self._h))
+
+    def set_profiling(self, enable: bool) -> None:
+        _lib.check(self._L.ffh_blob_set_profiling(self._h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        ms = np.zeros(_lib.FFH_NUM_KERNELS, np.float64)
+        cnt = np.zeros(_lib.FFH_NUM_KERNELS, np.int64)
+        _lib.check(self._L.ffh_blob_kernel_times(self._h, ms, cnt))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_lib.KERNEL_NAMES)}
+
+    def strain_energy(self) -> float:
+        e = C.c_double(0)
+        _lib.check(self._L.ffh_blob_strain_energy(self._h, C.byref(e)))
+        return e.value
+
+
+class GpuNoMassCGSolver:
+    """The Solver interface (include/Solver.h:30-55) over the GPU blob.
+
+    init() mirrors NoMassCGSolver::init's role (pattern + matrix at current positions);
+    solve(x) takes the force [N_t,3] and returns the velocity; apply_matrix(in) returns K*in
+    (a real product, where the reference's NoMassCGSolver stub does nothing,
+    include/NoMassCGSolver.h:103-104).
+    """
+
+    def __init__(self):
+        self.blob: Blob | None = None
+
+    def init(self, mesh, params: SimParams, scale: float, rng_state=None, precision="fp64"):
+        self.blob = Blob(mesh, params, scale, rng_state=rng_state, precision=precision)
+        return _lib.FFH_OK
+
+    def solve(self, x) -> np.ndarray:
+        return self.blob.solve(x).reshape(-1, 3)
+
+    def apply_matrix(self, vec) -> np.ndarray:
+        return self.blob.apply_matrix(vec)

@@ -1,0 +1,1347 @@
+// ffh_kernels.hip -- gfx950 kernels of the massless FFEA time step and the C-ABI
+// (include/ffea_hip.h) that drives them.
+//
+// Kernels (one blob, everything resident in HBM, one HIP stream):
+//   k_element     one element per lane: Jacobian, shape gradients, volume + inversion,
+//                 Kelvin-Voigt viscosity blocks, elastic + fluctuating stress, nodal forces
+//                 (src/tetra_element_linear.cpp:193-490, src/Blob.cpp:675-724)
+//   k_assemble    one SELL slot per lane: sums the element blocks of its (row, col) in
+//                 ascending element order (the reference's source-list order,
+//                 src/SparseMatrixFixedPattern.cpp:85-93, src/SparseMatrixTypes.cpp:52-58),
+//                 adds Stokes drag on the diagonal, writes the Jacobi inverse diagonal
+//   k_rhs         one node per lane: force gather in element order, Stokes noise, pinned
+//                 rows, CG initialisation (src/Blob.cpp:3826-3923, src/NoMassCGSolver.cpp:205-230)
+//   k_cg_spmv     p = beta p + D^-1 r and q = K p (3x3-block SELL SpMV), p.q, alpha
+//   k_cg_update   x += alpha p, r -= alpha q, ||r||^2/||f||^2 check, beta
+//   k_integrate   v = x, pos += v dt, RngStream advanced by one step's draws
+//
+// The FP64 path rounds exactly as the reference's strict build (no FMA contraction: the
+// file is compiled with -ffp-contract=off); only the order of the CG dot-product sums
+// differs (tree instead of sequential), deterministically.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ffea_hip.h"
+#include "ffh_plan.h"
+
+using ffh::Mat3u;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(FFH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kMaxReduceBlocks = 1 << 16;
+
+// ----------------------------------------------------------------------------------------
+// MRG32k3a on the device: exact integer arithmetic mod m = 2^32 - c.
+// ----------------------------------------------------------------------------------------
+constexpr uint32_t kC1 = 4294967296ull - ffh::kM1;  // 209
+constexpr uint32_t kC2 = 4294967296ull - ffh::kM2;  // 22853
+constexpr double kNorm = 1.0 / (4294967087.0 + 1.0);
+
+template <uint32_t M, uint32_t Cc>
+__device__ __forceinline__ uint32_t mred(uint64_t x) {
+    x = (x >> 32) * Cc + (x & 0xffffffffull);
+    x = (x >> 32) * Cc + (x & 0xffffffffull);
+    return (uint32_t)(x >= M ? x - M : x);
+}
+
+template <uint32_t M, uint32_t Cc>
+__device__ __forceinline__ void matvec(const Mat3u &A, uint32_t s[3]) {
+    uint32_t v[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        uint64_t acc = (uint64_t)mred<M, Cc>((uint64_t)A.m[i][0] * s[0]) +
+                       (uint64_t)mred<M, Cc>((uint64_t)A.m[i][1] * s[1]) +
+                       (uint64_t)mred<M, Cc>((uint64_t)A.m[i][2] * s[2]);
+        v[i] = mred<M, Cc>(acc);
+    }
+    s[0] = v[0];
+    s[1] = v[1];
+    s[2] = v[2];
+}
+
+struct Stream6 {
+    uint32_t s[6];
+};
+
+// State of the stream `off` draws after `base`, via the base-256 digit tables.
+__device__ __forceinline__ Stream6 rng_jump(const uint32_t *__restrict__ base, const Mat3u *__restrict__ tbl,
+                                            uint32_t off) {
+    Stream6 g;
+#pragma unroll
+    for (int k = 0; k < 6; k++) g.s[k] = base[k];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        uint32_t v = (off >> (8 * d)) & 255u;
+        if (v) {
+            matvec<ffh::kM1, kC1>(tbl[(d * 256 + v) * 2 + 0], &g.s[0]);
+            matvec<ffh::kM2, kC2>(tbl[(d * 256 + v) * 2 + 1], &g.s[3]);
+        }
+    }
+    return g;
+}
+
+// RngStream::U01 (src/RngStream.cpp:266-289) in exact integers; u rounds as the reference.
+__device__ __forceinline__ double rng_u01(Stream6 &g) {
+    uint32_t p1 = mred<ffh::kM1, kC1>((uint64_t)1403580u * g.s[1]) +
+                  0u;  // a12 * s1
+    uint32_t t1 = mred<ffh::kM1, kC1>((uint64_t)(ffh::kM1 - 810728u) * g.s[0]);
+    uint64_t q1 = (uint64_t)p1 + t1;
+    p1 = (uint32_t)(q1 >= ffh::kM1 ? q1 - ffh::kM1 : q1);
+    g.s[0] = g.s[1];
+    g.s[1] = g.s[2];
+    g.s[2] = p1;
+    uint32_t p2 = mred<ffh::kM2, kC2>((uint64_t)527612u * g.s[5]);
+    uint32_t t2 = mred<ffh::kM2, kC2>((uint64_t)(ffh::kM2 - 1370589u) * g.s[3]);
+    uint64_t q2 = (uint64_t)p2 + t2;
+    p2 = (uint32_t)(q2 >= ffh::kM2 ? q2 - ffh::kM2 : q2);
+    g.s[3] = g.s[4];
+    g.s[4] = g.s[5];
+    g.s[5] = p2;
+    double d = (double)p1 - (double)p2;
+    return (p1 > p2) ? d * kNorm : (d + 4294967087.0) * kNorm;
+}
+
+// RAND(-.5,.5) (include/tetra_element_linear.h:118)
+__device__ __forceinline__ double rand_half(Stream6 &g) { return (-.5) + ((.5) - (-.5)) * rng_u01(g); }
+
+// ----------------------------------------------------------------------------------------
+// Reductions: deterministic block tree + last-block-done final sum in fixed order.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Sums NV values per thread over the block; thread 0 gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[kBlock / 64]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int i = 0; i < kBlock / 64; i++) s += sh[k][i];
+            v[k] = s;
+        }
+    }
+}
+
+// Publishes this block's NV partial sums; returns true in (all threads of) the last block,
+// whose thread 0 then holds the grid totals in v.
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], double *__restrict__ partials, unsigned *__restrict__ ticket) {
+    __shared__ double sh[NV][kBlock / 64];
+    __shared__ bool last;
+    block_sum<NV>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[(size_t)k * kMaxReduceBlocks + blockIdx.x] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
+            s += __builtin_nontemporal_load(&partials[(size_t)k * kMaxReduceBlocks + i]);
+        acc[k] = s;
+    }
+    __syncthreads();
+    block_sum<NV>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = acc[k];
+        *ticket = 0u;
+    }
+    return true;
+}
+
+// CG control block (device memory).
+struct CgCtl {
+    double delta, beta, alpha, f2, res, eps2;
+    int iter, done, status, maxit;
+    int inversions, pad_;
+};
+
+// ----------------------------------------------------------------------------------------
+// Element kernel
+// ----------------------------------------------------------------------------------------
+struct ElemArgs {
+    const double *X, *Y, *Z;
+    const int4 *enodes;
+    const double *jinv0;  // SoA [9][E]
+    const double *vol0, *A, *B, *G, *Em;
+    double *last_det;
+    void *eforce;  // T SoA [12][E], index (3a+i)*E + e
+    void *eblk;    // T AoS [E][90]
+    int E;
+    int calc_noise;
+    double kT, dt;
+    const uint32_t *rng_state;
+    const Mat3u *jump;
+    uint32_t elem_base;
+    CgCtl *ctl;
+};
+
+// Entry (R, C) of the element viscosity matrix (create_viscosity_matrix,
+// src/tetra_element_linear.cpp:299-328, with the macros of include/tetra_element_linear.h:55-116
+// and the diffusion term :599-638): the upper triangle as built, the lower one its mirror.
+template <typename R>
+__device__ __forceinline__ R visc_entry(const R (&dpsi)[12], R A, R B, R vol, int Rr, int Cc) {
+    if (Rr > Cc) {
+        int t = Rr;
+        Rr = Cc;
+        Cc = t;
+    }
+    const int i = Rr >> 2, a = Rr & 3, j = Cc >> 2, b = Cc & 3;
+    R v;
+    if (i == j) {
+        R del2 = dpsi[a] * dpsi[b] + dpsi[4 + a] * dpsi[4 + b] + dpsi[8 + a] * dpsi[8 + b];
+        v = dpsi[Rr] * dpsi[Cc] * (A + B);
+        v += del2 * A;
+    } else if (a == b) {
+        v = (B + A) * (dpsi[Rr] * dpsi[Cc]);
+    } else {
+        v = B * (dpsi[Rr] * dpsi[Cc]) + A * (dpsi[4 * i + b] * dpsi[4 * j + a]);
+    }
+    return v * vol;
+}
+
+template <typename R, typename T>
+__global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= g.E) return;
+    const int4 n = g.enodes[e];
+    const int nn[4] = {n.x, n.y, n.z, n.w};
+    // calculate_jacobian (:193-205), in FP64 from FP64 positions
+    R J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        J[r][0] = (R)(g.X[nn[r + 1]] - g.X[nn[0]]);
+        J[r][1] = (R)(g.Y[nn[r + 1]] - g.Y[nn[0]]);
+        J[r][2] = (R)(g.Z[nn[r + 1]] - g.Z[nn[0]]);
+    }
+    // calc_shape_function_derivatives_and_volume (:223-293)
+    R dpsi[12];
+    dpsi[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+    dpsi[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    dpsi[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+    dpsi[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+    dpsi[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    dpsi[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+    dpsi[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+    dpsi[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    dpsi[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+    R det = J[0][0] * dpsi[1] + J[1][0] * dpsi[2] + J[2][0] * dpsi[3];
+    const double last = g.last_det[e];
+    if (last * (double)det < 0) atomicAdd(&g.ctl->inversions, 1);
+    g.last_det[e] = (double)det;
+    const R vol = (R)(1.0 / 6.0) * fabs(det);
+    const R dinv = (R)1.0 / det;
+    dpsi[1] *= dinv;
+    dpsi[2] *= dinv;
+    dpsi[3] *= dinv;
+    dpsi[5] *= dinv;
+    dpsi[6] *= dinv;
+    dpsi[7] *= dinv;
+    dpsi[9] *= dinv;
+    dpsi[10] *= dinv;
+    dpsi[11] *= dinv;
+    dpsi[0] = -(dpsi[1] + dpsi[2] + dpsi[3]);
+    dpsi[4] = -(dpsi[5] + dpsi[6] + dpsi[7]);
+    dpsi[8] = -(dpsi[9] + dpsi[10] + dpsi[11]);
+
+    const R A = (R)g.A[e], B = (R)g.B[e], G = (R)g.G[e], Em = (R)g.Em[e], vol0 = (R)g.vol0[e];
+
+    // viscosity blocks (a<=b): blk[p][3i+j] = V[a+4i][b+4j]
+    T *blk = reinterpret_cast<T *>(g.eblk) + (size_t)e * 90;
+    {
+        int p = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = a; b < 4; b++, p++)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) blk[p * 9 + 3 * i + j] = (T)visc_entry<R>(dpsi, A, B, vol, a + 4 * i, b + 4 * j);
+    }
+
+    // add_shear_elastic_stress (:378-396): F_ij = sum_k J[k][i] Jinv0[j][k]; S = F F^T
+    R F[3][3], S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            R f = (R)0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) f += J[k][i] * (R)g.jinv0[(size_t)(3 * j + k) * g.E + e];
+            F[i][j] = f;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            R s = (R)0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) s += F[i][k] * F[j][k];
+            S[i][j] = s;
+        }
+    const R sc = (G * vol0 / vol);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[i][j] *= sc;
+    S[0][0] -= G;
+    S[1][1] -= G;
+    S[2][2] -= G;
+    // add_bulk_elastic_stress (:401-408)
+    {
+        const R c_2 = Em - G * (R)2.0 / (R)3.0;
+        const R c = G * ((R)1.0 - (vol0 / vol)) + (R)0.5 * c_2 * ((vol / vol0) - (vol0 / vol));
+        S[0][0] += c;
+        S[1][1] += c;
+        S[2][2] += c;
+    }
+    // add_fluctuating_stress (:417-436): 7 draws at this element's stream offset
+    if (g.calc_noise) {
+        Stream6 st = rng_jump(g.rng_state, g.jump, g.elem_base + 7u * (uint32_t)e);
+        const R c = sqrt(((R)24 * (R)g.kT) / (vol * (R)g.dt));
+        const R bf = sqrt(B) * (R)rand_half(st);
+        S[0][0] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
+        S[1][1] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
+        S[2][2] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
+        S[0][1] += c * sqrt(A) * (R)rand_half(st);
+        S[0][2] += c * sqrt(A) * (R)rand_half(st);
+        S[1][2] += c * sqrt(A) * (R)rand_half(st);
+        S[1][0] = S[0][1];
+        S[2][0] = S[0][2];
+        S[2][1] = S[1][2];
+    }
+    // apply_stress_tensor (:441-449) + add_element_force_vector (:474-490)
+    T *ef = reinterpret_cast<T *>(g.eforce);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            R du = (R)0 + vol * (dpsi[a] * S[i][0] + dpsi[4 + a] * S[i][1] + dpsi[8 + a] * S[i][2]);
+            ef[(size_t)(3 * a + i) * g.E + e] = (T)((R)0 - du);
+        }
+}
+
+// ----------------------------------------------------------------------------------------
+// Assembly
+// ----------------------------------------------------------------------------------------
+struct AsmArgs {
+    const void *eblk;  // T [E][90]
+    const uint32_t *ctb_off, *ctb;
+    const int32_t *col;
+    const uint8_t *flags;
+    const double *zeta;
+    void *val;  // T tiles
+    double *dinv;
+    int64_t nslots;
+};
+
+template <typename T, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
+    const int64_t slot = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= g.nslots) return;
+    const uint8_t fl = g.flags[slot];
+    const int col = g.col[slot];
+    Acc acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) acc[k] = (Acc)0.0;
+    const T *eb = reinterpret_cast<const T *>(g.eblk);
+    const uint32_t c0 = g.ctb_off[slot], c1 = g.ctb_off[slot + 1];
+    for (uint32_t c = c0; c < c1; c++) {
+        const uint32_t code = g.ctb[c];
+        const T *b = eb + (size_t)(code >> 5) * 90 + ((code >> 1) & 15u) * 9;
+        if (code & 1u) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc[3 * i + j] += (Acc)b[3 * j + i];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) acc[k] += (Acc)b[k];
+        }
+    }
+    if (fl & 1) {
+        if (fl & 2) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) acc[k] = (Acc)0.0;
+            acc[0] = acc[4] = acc[8] = (Acc)1.0;
+        } else {
+            const Acc z = (Acc)g.zeta[col];
+            acc[0] += z;
+            acc[4] += z;
+            acc[8] += z;
+        }
+        // calc_inverse_diagonal (src/SparseMatrixFixedPattern.cpp:325-333)
+        g.dinv[3 * (size_t)col + 0] = 1.0 / (double)(T)acc[0];
+        g.dinv[3 * (size_t)col + 1] = 1.0 / (double)(T)acc[4];
+        g.dinv[3 * (size_t)col + 2] = 1.0 / (double)(T)acc[8];
+    }
+    T *v = reinterpret_cast<T *>(g.val);
+    const int64_t tile = slot & ~(int64_t)63;
+    const int lane = (int)(slot & 63);
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[tile * 9 + k * 64 + lane] = (T)acc[k];
+}
+
+// ----------------------------------------------------------------------------------------
+// RHS: force gather, Stokes noise, pinned rows, CG initialisation
+// ----------------------------------------------------------------------------------------
+struct VecArgs {
+    int N;
+    double *f, *r, *x, *p0, *p1, *q, *vel;
+    const double *dinv;
+    double *partials;
+    unsigned *ticket;
+    CgCtl *ctl;
+};
+
+struct RhsArgs {
+    const void *eforce;  // T SoA [12][E]
+    int E;
+    const uint32_t *ctb_off, *ctb;
+    const double *zeta;
+    const uint8_t *pinned;
+    const uint32_t *rng_off;
+    const uint32_t *rng_state;
+    const Mat3u *jump;
+    uint32_t node_base;
+    int noise;
+    double kT, dt;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rhs(RhsArgs g, VecArgs v) {
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    double part[2] = {0.0, 0.0};
+    if (n < v.N) {
+        const T *ef = reinterpret_cast<const T *>(g.eforce);
+        double f[3] = {0.0, 0.0, 0.0};
+        for (uint32_t c = g.ctb_off[n]; c < g.ctb_off[n + 1]; c++) {
+            const uint32_t code = g.ctb[c];
+            const uint32_t e = code >> 2, a = code & 3u;
+#pragma unroll
+            for (int i = 0; i < 3; i++) f[i] += (double)ef[(size_t)(3 * a + i) * g.E + e];
+        }
+        if (g.noise) {
+            Stream6 st = rng_jump(g.rng_state, g.jump, g.node_base + g.rng_off[n]);
+            const double zeta = g.zeta[n];
+#pragma unroll
+            for (int i = 0; i < 3; i++) f[i] -= rand_half(st) * sqrt((24 * g.kT * zeta) / (g.dt));
+        }
+        if (g.pinned[n]) f[0] = f[1] = f[2] = 0;
+        double z[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            v.f[3 * n + i] = f[i];
+            v.r[3 * n + i] = f[i];
+            v.x[3 * n + i] = 0.0;
+            v.p0[3 * n + i] = 0.0;
+            v.p1[3 * n + i] = 0.0;
+            z[i] = v.dinv[3 * n + i] * f[i];
+        }
+        part[0] = f[0] * z[0] + f[1] * z[1] + f[2] * z[2];
+        part[1] = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    }
+    if (grid_sum<2>(part, v.partials, v.ticket) && threadIdx.x == 0) {
+        CgCtl *c = v.ctl;
+        c->delta = part[0];
+        c->f2 = part[1];
+        c->beta = 0.0;
+        c->alpha = 0.0;
+        c->res = 0.0;
+        c->iter = 0;
+        c->status = 0;
+        c->done = (part[1] == 0.0) ? 1 : 0;  // zero force: velocity 0 (see DESIGN notes)
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// CG iteration
+// ----------------------------------------------------------------------------------------
+struct SpmvArgs {
+    const int64_t *slice_off;
+    const int32_t *slice_width;
+    const int32_t *col;
+    const void *val;
+};
+
+// p_new = beta p_old + D^-1 r (vec3_scale_and_add, src/mat_vec_fns.cpp:265-275, with
+// parallel_apply_preconditioner :278-292), q = K p_new (SparseMatrixFixedPattern::apply,
+// :148-191: per scalar row, ascending column), pTq and alpha = delta / pTq (:261-276, :176).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs m, VecArgs v, const double *__restrict__ pold,
+                                                    double *__restrict__ pnew) {
+    if (v.ctl->done) return;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const double beta = v.ctl->beta;
+    double part[1] = {0.0};
+    if (row < v.N) {
+        const int s = row >> 6, lane = row & 63;
+        const int w = m.slice_width[s];
+        const int64_t base = m.slice_off[s];
+        const T *val = reinterpret_cast<const T *>(m.val);
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = m.col[tile + lane];
+            const T *vt = val + tile * 9 + lane;
+            double pc[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) pc[j] = beta * pold[3 * c + j] + v.dinv[3 * c + j] * v.r[3 * c + j];
+            acc0 += (double)vt[0 * 64] * pc[0];
+            acc0 += (double)vt[1 * 64] * pc[1];
+            acc0 += (double)vt[2 * 64] * pc[2];
+            acc1 += (double)vt[3 * 64] * pc[0];
+            acc1 += (double)vt[4 * 64] * pc[1];
+            acc1 += (double)vt[5 * 64] * pc[2];
+            acc2 += (double)vt[6 * 64] * pc[0];
+            acc2 += (double)vt[7 * 64] * pc[1];
+            acc2 += (double)vt[8 * 64] * pc[2];
+        }
+        double pn[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            pn[j] = beta * pold[3 * row + j] + v.dinv[3 * row + j] * v.r[3 * row + j];
+            pnew[3 * row + j] = pn[j];
+        }
+        v.q[3 * row + 0] = acc0;
+        v.q[3 * row + 1] = acc1;
+        v.q[3 * row + 2] = acc2;
+        part[0] = pn[0] * acc0 + pn[1] * acc1 + pn[2] * acc2;
+    }
+    if (grid_sum<1>(part, v.partials, v.ticket) && threadIdx.x == 0) {
+        v.ctl->alpha = v.ctl->delta / part[0];
+    }
+}
+
+// x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260); convergence on
+// ||r||^2/||f||^2 < eps^2 (src/NoMassCGSolver.cpp:182-191, :233-248); delta_new = r.D^-1 r.
+__global__ __launch_bounds__(kBlock) void k_cg_update(VecArgs v, const double *__restrict__ p) {
+    if (v.ctl->done) return;
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    const double alpha = v.ctl->alpha;
+    double part[2] = {0.0, 0.0};
+    if (n < v.N) {
+        double r[3], z[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            v.x[3 * n + i] += alpha * p[3 * n + i];
+            r[i] = v.r[3 * n + i] + (-alpha) * v.q[3 * n + i];
+            v.r[3 * n + i] = r[i];
+            z[i] = v.dinv[3 * n + i] * r[i];
+        }
+        part[0] = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        part[1] = r[0] * z[0] + r[1] * z[1] + r[2] * z[2];
+    }
+    if (grid_sum<2>(part, v.partials, v.ticket) && threadIdx.x == 0) {
+        CgCtl *c = v.ctl;
+        const double res = (c->f2 == 0.0) ? 0.0 : part[0] / c->f2;
+        c->res = res;
+        c->iter += 1;
+        if (res < c->eps2) {
+            c->done = 1;
+            c->status = 0;
+        } else if (c->iter >= c->maxit) {
+            c->done = 1;
+            c->status = -1;
+        } else {
+            c->beta = part[1] / c->delta;
+            c->delta = part[1];
+        }
+    }
+}
+
+// euler_integrate (src/Blob.cpp:3941-3953): vel = v; pos += v * dt.  Skipped after a CG
+// failure, as the reference skips it.  Thread 0 advances the stream by one step's draws.
+__global__ __launch_bounds__(kBlock) void k_integrate(VecArgs v, double *X, double *Y, double *Z, double dt,
+                                                     uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance) {
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n == 0 && advance) {
+        uint32_t s[6];
+        for (int k = 0; k < 6; k++) s[k] = rng_state[k];
+        matvec<ffh::kM1, kC1>(j1, &s[0]);
+        matvec<ffh::kM2, kC2>(j2, &s[3]);
+        for (int k = 0; k < 6; k++) rng_state[k] = s[k];
+    }
+    if (n >= v.N || v.ctl->status != 0) return;
+    const double x0 = v.x[3 * n], x1 = v.x[3 * n + 1], x2 = v.x[3 * n + 2];
+    v.vel[3 * n] = x0;
+    v.vel[3 * n + 1] = x1;
+    v.vel[3 * n + 2] = x2;
+    X[n] += x0 * dt;
+    Y[n] += x1 * dt;
+    Z[n] += x2 * dt;
+}
+
+// result = K in for Solver::apply_matrix (linear rows; midpoint rows handled on the host).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply(SpmvArgs m, int N, const double *__restrict__ in,
+                                                  double *__restrict__ out) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= N) return;
+    const int s = row >> 6, lane = row & 63;
+    const int w = m.slice_width[s];
+    const int64_t base = m.slice_off[s];
+    const T *val = reinterpret_cast<const T *>(m.val);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < w; t++) {
+        const int64_t tile = base + (int64_t)t * 64;
+        const int c = m.col[tile + lane];
+        const T *vt = val + tile * 9 + lane;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc[i] += (double)vt[(3 * i + j) * 64] * in[3 * c + j];
+    }
+    out[3 * row] = acc[0];
+    out[3 * row + 1] = acc[1];
+    out[3 * row + 2] = acc[2];
+}
+
+// Strain energy (src/Blob.cpp:1424-1432) recomputed from current positions.
+__global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *partials, unsigned *ticket,
+                                                          double *out) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double part[1] = {0.0};
+    if (e < g.E) {
+        const int4 n = g.enodes[e];
+        const int nn[4] = {n.x, n.y, n.z, n.w};
+        double J[3][3];
+        for (int r = 0; r < 3; r++) {
+            J[r][0] = g.X[nn[r + 1]] - g.X[nn[0]];
+            J[r][1] = g.Y[nn[r + 1]] - g.Y[nn[0]];
+            J[r][2] = g.Z[nn[r + 1]] - g.Z[nn[0]];
+        }
+        double c1 = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+        double c2 = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+        double c3 = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+        double det = J[0][0] * c1 + J[1][0] * c2 + J[2][0] * c3;
+        double vol = (1.0 / 6.0) * fabs(det);
+        double F[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double f = 0;
+                for (int k = 0; k < 3; k++) f += J[k][i] * g.jinv0[(size_t)(3 * j + k) * g.E + e];
+                F[i][j] = f;
+            }
+        double dc = F[0][0] * F[0][0] + F[1][1] * F[1][1] + F[2][2] * F[2][2] + F[0][1] * F[0][1] +
+                    F[1][0] * F[1][0] + F[0][2] * F[0][2] + F[2][0] * F[2][0] + F[1][2] * F[1][2] +
+                    F[2][1] * F[2][1];
+        double G = g.G[e], C = g.Em[e] - (2.0 / 3.0) * G, vol0 = g.vol0[e];
+        double t = vol / vol0;
+        part[0] = vol0 * (G * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * G) * log(t));
+    }
+    if (grid_sum<1>(part, partials, ticket) && threadIdx.x == 0) *out = 0.5 * part[0];
+}
+
+// ----------------------------------------------------------------------------------------
+// Device buffers
+// ----------------------------------------------------------------------------------------
+template <typename X>
+struct DBuf {
+    X *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct ffh_blob {
+    ffh::Plan plan;
+    int precision = FFH_PRECISION_FP64;
+    double dt = 0, kT = 0, eps2 = 0, stokes_visc = 0;
+    int maxit = 1000, calc_noise = 0, calc_stokes = 0;
+    int N = 0, E = 0;
+    size_t tsize = 8;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    // device arrays
+    double *X = nullptr, *Y = nullptr, *Z = nullptr;
+    int4 *enodes = nullptr;
+    double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
+           *last_det = nullptr;
+    void *eforce = nullptr, *eblk = nullptr, *val = nullptr;
+    uint32_t *sctb_off = nullptr, *sctb = nullptr, *nctb_off = nullptr, *nctb = nullptr, *rng_off = nullptr;
+    int32_t *slot_col = nullptr, *slice_width = nullptr;
+    int64_t *slice_off = nullptr;
+    uint8_t *slot_flags = nullptr, *pinned = nullptr;
+    double *zeta = nullptr, *dinv = nullptr;
+    double *f = nullptr, *r = nullptr, *x = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *vel = nullptr;
+    double *partials = nullptr, *tmp_in = nullptr, *tmp_out = nullptr, *scalar_out = nullptr;
+    unsigned *ticket = nullptr;
+    CgCtl *ctl = nullptr;
+    CgCtl *ctl_host = nullptr;  // pinned host mirror
+    uint32_t *rng_state = nullptr;
+    Mat3u *jump = nullptr;
+    // profiling
+    bool profiling = false;
+    double ktime[FFH_NUM_KERNELS] = {0};
+    long long kcount[FFH_NUM_KERNELS] = {0};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    int last_iters = 0;
+    double last_res = 0;
+    int cg_chunk = 16;
+
+    ~ffh_blob() {
+        if (stream) hipStreamSynchronize(stream);
+        for (auto &pe : pending) {
+            hipEventDestroy(pe.second.first);
+            hipEventDestroy(pe.second.second);
+        }
+        for (auto ev : event_pool) hipEventDestroy(ev);
+        for (void *p : allocs) hipFree(p);
+        if (ctl_host) hipHostFree(ctl_host);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    template <typename X>
+    int alloc(X *&p, size_t n) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(X));
+        if (e != hipSuccess) return fail(FFH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        p = static_cast<X *>(q);
+        return FFH_OK;
+    }
+    template <typename X>
+    int upload(X *&p, const std::vector<X> &h) {
+        int rc = alloc(p, h.size());
+        if (rc) return rc;
+        if (!h.empty()) HIP_OR_FAIL(hipMemcpy(p, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice));
+        return FFH_OK;
+    }
+
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    void collect_events(bool wait) {
+        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> keep;
+        for (auto &pe : pending) {
+            if (!wait && hipEventQuery(pe.second.second) != hipSuccess) {
+                keep.push_back(pe);
+                continue;
+            }
+            hipEventSynchronize(pe.second.second);
+            float ms = 0;
+            hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+            ktime[pe.first] += ms;
+            kcount[pe.first] += 1;
+            event_pool.push_back(pe.second.first);
+            event_pool.push_back(pe.second.second);
+        }
+        pending.swap(keep);
+    }
+    template <typename F>
+    int launch(int kid, F &&fn) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (profiling) {
+            a = get_event();
+            b = get_event();
+            hipEventRecord(a, stream);
+        }
+        fn();
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(FFH_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        if (profiling) {
+            hipEventRecord(b, stream);
+            pending.push_back({kid, {a, b}});
+            if (pending.size() > 4096) collect_events(true);
+        }
+        return FFH_OK;
+    }
+
+    VecArgs vec_args() const {
+        VecArgs v;
+        v.N = N;
+        v.f = f;
+        v.r = r;
+        v.x = x;
+        v.p0 = p0;
+        v.p1 = p1;
+        v.q = q;
+        v.vel = vel;
+        v.dinv = dinv;
+        v.partials = partials;
+        v.ticket = ticket;
+        v.ctl = ctl;
+        return v;
+    }
+    SpmvArgs spmv_args() const {
+        SpmvArgs m;
+        m.slice_off = slice_off;
+        m.slice_width = slice_width;
+        m.col = slot_col;
+        m.val = val;
+        return m;
+    }
+    ElemArgs elem_args() const {
+        ElemArgs g;
+        g.X = X;
+        g.Y = Y;
+        g.Z = Z;
+        g.enodes = enodes;
+        g.jinv0 = jinv0;
+        g.vol0 = vol0;
+        g.A = A;
+        g.B = B;
+        g.G = G;
+        g.Em = Em;
+        g.last_det = last_det;
+        g.eforce = eforce;
+        g.eblk = eblk;
+        g.E = E;
+        g.calc_noise = calc_noise;
+        g.kT = kT;
+        g.dt = dt;
+        g.rng_state = rng_state;
+        g.jump = jump;
+        g.elem_base = 0;
+        g.ctl = ctl;
+        return g;
+    }
+    int grid(int64_t n) const { return (int)((n + kBlock - 1) / kBlock); }
+
+    int run_element() {
+        ElemArgs g = elem_args();
+        if (precision == FFH_PRECISION_FP64)
+            return launch(0, [&] { hipLaunchKernelGGL((k_element<double, double>), grid(E), kBlock, 0, stream, g); });
+        return launch(0, [&] { hipLaunchKernelGGL((k_element<float, float>), grid(E), kBlock, 0, stream, g); });
+    }
+    int run_assemble() {
+        AsmArgs a;
+        a.eblk = eblk;
+        a.ctb_off = sctb_off;
+        a.ctb = sctb;
+        a.col = slot_col;
+        a.flags = slot_flags;
+        a.zeta = zeta;
+        a.val = val;
+        a.dinv = dinv;
+        a.nslots = plan.num_slots;
+        if (precision == FFH_PRECISION_FP64)
+            return launch(1, [&] {
+                hipLaunchKernelGGL((k_assemble<double, double>), grid(plan.num_slots), kBlock, 0, stream, a);
+            });
+        return launch(1, [&] {
+            hipLaunchKernelGGL((k_assemble<float, float>), grid(plan.num_slots), kBlock, 0, stream, a);
+        });
+    }
+    RhsArgs rhs_args(bool noise) const {
+        RhsArgs g;
+        g.eforce = eforce;
+        g.E = E;
+        g.ctb_off = nctb_off;
+        g.ctb = nctb;
+        g.zeta = zeta;
+        g.pinned = pinned;
+        g.rng_off = rng_off;
+        g.rng_state = rng_state;
+        g.jump = jump;
+        g.node_base = (uint32_t)plan.elem_draws;
+        g.noise = noise ? 1 : 0;
+        g.kT = kT;
+        g.dt = dt;
+        return g;
+    }
+    int run_rhs() {
+        RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1);
+        VecArgs v = vec_args();
+        if (precision == FFH_PRECISION_FP64)
+            return launch(2, [&] { hipLaunchKernelGGL((k_rhs<double>), grid(N), kBlock, 0, stream, g, v); });
+        return launch(2, [&] { hipLaunchKernelGGL((k_rhs<float>), grid(N), kBlock, 0, stream, g, v); });
+    }
+    // Runs CG iterations until converged or failed; polls the control block every chunk.
+    int run_cg() {
+        VecArgs v = vec_args();
+        SpmvArgs m = spmv_args();
+        int it = 0;
+        for (;;) {
+            int chunk = std::max(1, std::min(cg_chunk, maxit - it + 1));
+            for (int k = 0; k < chunk; k++, it++) {
+                double *pold = (it & 1) ? p1 : p0;
+                double *pnew = (it & 1) ? p0 : p1;
+                int rc;
+                if (precision == FFH_PRECISION_FP64)
+                    rc = launch(3, [&] {
+                        hipLaunchKernelGGL((k_cg_spmv<double>), grid(N), kBlock, 0, stream, m, v, pold, pnew);
+                    });
+                else
+                    rc = launch(3, [&] {
+                        hipLaunchKernelGGL((k_cg_spmv<float>), grid(N), kBlock, 0, stream, m, v, pold, pnew);
+                    });
+                if (rc) return rc;
+                rc = launch(4, [&] { hipLaunchKernelGGL(k_cg_update, grid(N), kBlock, 0, stream, v, pnew); });
+                if (rc) return rc;
+            }
+            HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
+            HIP_OR_FAIL(hipStreamSynchronize(stream));
+            if (ctl_host->done) break;
+            if (it > maxit + 1) break;
+        }
+        last_iters = ctl_host->iter;
+        last_res = ctl_host->res;
+        // next solve: one chunk covering this solve's iterations (+1 slack)
+        cg_chunk = std::max(2, std::min(ctl_host->iter + 1, 64));
+        if (ctl_host->status != 0)
+            return fail(FFH_E_CG, "Conjugate gradient solver: could not converge after " + std::to_string(maxit) +
+                                      " iterations");
+        return FFH_OK;
+    }
+    int run_integrate() {
+        VecArgs v = vec_args();
+        int adv = plan.draws_per_step > 0 ? 1 : 0;
+        Mat3u j1 = plan.step_jump[0], j2 = plan.step_jump[1];
+        return launch(5, [&] {
+            hipLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, v, X, Y, Z, dt, rng_state, j1, j2, adv);
+        });
+    }
+    int reset_inversions() {
+        HIP_OR_FAIL(hipMemsetAsync(&ctl->inversions, 0, sizeof(int), stream));
+        return FFH_OK;
+    }
+    int read_inversions(int *ninv) {
+        int h = 0;
+        HIP_OR_FAIL(hipMemcpyAsync(&ctl_host->inversions, &ctl->inversions, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OR_FAIL(hipStreamSynchronize(stream));
+        h = ctl_host->inversions;
+        if (ninv) *ninv = h;
+        if (h > 0) return fail(FFH_E_INVERSION, std::to_string(h) + " element(s) have inverted since the last step");
+        return FFH_OK;
+    }
+};
+
+extern "C" {
+
+const char *ffh_last_error(void) { return g_err.c_str(); }
+
+int ffh_version(void) { return (0 << 16) | 1; }
+
+ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
+                          const double *material_si, const double *stokes_radius, int num_pinned,
+                          const int *pinned, double dt, double kT, double epsilon2,
+                          int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                          const unsigned long long *rng_state, int precision) {
+    g_err.clear();
+    if (!pos || !topo || !material_si || !stokes_radius || (num_pinned > 0 && !pinned) || !rng_state) {
+        fail(FFH_E_ARG, "null argument");
+        return nullptr;
+    }
+    if (precision != FFH_PRECISION_FP64 && precision != FFH_PRECISION_MIXED) {
+        fail(FFH_E_ARG, "unknown precision mode");
+        return nullptr;
+    }
+    if (max_iterations_cg < 1 || !(dt > 0) || !(epsilon2 >= 0)) {
+        fail(FFH_E_ARG, "invalid solver parameters");
+        return nullptr;
+    }
+    for (int i = 0; i < 6; i++)
+        if (rng_state[i] >= (i < 3 ? ffh::kM1 : ffh::kM2)) {
+            fail(FFH_E_ARG, "RngStream state out of range");
+            return nullptr;
+        }
+    std::unique_ptr<ffh_blob> b(new ffh_blob());
+    std::string msg = ffh::build_plan(b->plan, num_nodes, num_elements, pos, topo, material_si, stokes_radius,
+                                      num_pinned, pinned, stokes_visc, calc_noise, calc_stokes);
+    if (!msg.empty()) {
+        fail(FFH_E_ARG, msg);
+        return nullptr;
+    }
+    const ffh::Plan &P = b->plan;
+    b->precision = precision;
+    b->dt = dt;
+    b->kT = kT;
+    b->eps2 = epsilon2;
+    b->maxit = max_iterations_cg;
+    b->calc_noise = calc_noise;
+    b->calc_stokes = calc_stokes;
+    b->stokes_visc = stokes_visc;
+    b->N = P.num_linear;
+    b->E = P.num_elements;
+    b->tsize = precision == FFH_PRECISION_FP64 ? 8 : 4;
+    if (b->grid(b->N) > kMaxReduceBlocks || b->grid(b->E) > kMaxReduceBlocks) {
+        fail(FFH_E_ARG, "blob too large for the reduction buffer");
+        return nullptr;
+    }
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        fail(FFH_E_HIP, "hipStreamCreate failed");
+        return nullptr;
+    }
+    const int N = b->N, E = b->E;
+    int rc = 0;
+    std::vector<double> pos3(P.pos);
+    rc |= b->alloc(b->X, 3 * (size_t)N);
+    if (!rc) {
+        b->Y = b->X + N;
+        b->Z = b->X + 2 * (size_t)N;
+        if (hipMemcpy(b->X, pos3.data(), pos3.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = 1;
+    }
+    {
+        std::vector<int4> en(E);
+        for (int e = 0; e < E; e++)
+            en[e] = make_int4(P.enodes[4 * (size_t)e], P.enodes[4 * (size_t)e + 1], P.enodes[4 * (size_t)e + 2],
+                              P.enodes[4 * (size_t)e + 3]);
+        rc |= b->upload(b->enodes, en);
+    }
+    rc |= b->upload(b->jinv0, P.jinv0);
+    rc |= b->upload(b->vol0, P.vol0);
+    rc |= b->upload(b->A, P.matA);
+    rc |= b->upload(b->B, P.matB);
+    rc |= b->upload(b->G, P.matG);
+    rc |= b->upload(b->Em, P.matE);
+    rc |= b->upload(b->last_det, P.last_det);
+    {
+        char *ef = nullptr, *eb = nullptr, *vv = nullptr;
+        rc |= b->alloc(ef, 12 * (size_t)E * b->tsize);
+        rc |= b->alloc(eb, 90 * (size_t)E * b->tsize);
+        rc |= b->alloc(vv, 9 * (size_t)P.num_slots * b->tsize);
+        b->eforce = ef;
+        b->eblk = eb;
+        b->val = vv;
+    }
+    rc |= b->upload(b->sctb_off, P.sctb_off);
+    rc |= b->upload(b->sctb, P.sctb);
+    rc |= b->upload(b->nctb_off, P.nctb_off);
+    rc |= b->upload(b->nctb, P.nctb);
+    rc |= b->upload(b->rng_off, P.rng_node_off);
+    rc |= b->upload(b->slot_col, P.slot_col);
+    rc |= b->upload(b->slice_width, P.slice_width);
+    rc |= b->upload(b->slice_off, P.slice_off);
+    rc |= b->upload(b->slot_flags, P.slot_flags);
+    rc |= b->upload(b->pinned, P.pinned);
+    rc |= b->upload(b->zeta, P.zeta);
+    rc |= b->alloc(b->dinv, 3 * (size_t)N);
+    rc |= b->alloc(b->f, 3 * (size_t)N);
+    rc |= b->alloc(b->r, 3 * (size_t)N);
+    rc |= b->alloc(b->x, 3 * (size_t)N);
+    rc |= b->alloc(b->p0, 3 * (size_t)N);
+    rc |= b->alloc(b->p1, 3 * (size_t)N);
+    rc |= b->alloc(b->q, 3 * (size_t)N);
+    rc |= b->alloc(b->vel, 3 * (size_t)N);
+    rc |= b->alloc(b->tmp_in, 3 * (size_t)N);
+    rc |= b->alloc(b->tmp_out, 3 * (size_t)N);
+    rc |= b->alloc(b->partials, 4 * (size_t)kMaxReduceBlocks);
+    rc |= b->alloc(b->scalar_out, 4);
+    rc |= b->alloc(b->ticket, 1);
+    rc |= b->alloc(b->ctl, 1);
+    rc |= b->alloc(b->rng_state, 6);
+    rc |= b->upload(b->jump, P.jump);
+    if (rc) {
+        if (g_err.empty()) fail(FFH_E_HIP, "device allocation or upload failed");
+        return nullptr;
+    }
+    if (hipHostMalloc((void **)&b->ctl_host, sizeof(CgCtl), hipHostMallocDefault) != hipSuccess) {
+        fail(FFH_E_HIP, "hipHostMalloc failed");
+        return nullptr;
+    }
+    CgCtl c{};
+    c.eps2 = epsilon2;
+    c.maxit = max_iterations_cg;
+    uint32_t st[6];
+    for (int i = 0; i < 6; i++) st[i] = (uint32_t)rng_state[i];
+    if (hipMemcpy(b->ctl, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rng_state, st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(b->ticket, 0, sizeof(unsigned)) != hipSuccess ||
+        hipMemset(b->vel, 0, 3 * (size_t)N * sizeof(double)) != hipSuccess ||
+        hipMemset(b->x, 0, 3 * (size_t)N * sizeof(double)) != hipSuccess) {
+        fail(FFH_E_HIP, "device initialisation failed");
+        return nullptr;
+    }
+    // NoMassCGSolver::init builds the element matrices at the current positions; so does the
+    // first assembly here (no noise draws are consumed: the element kernel's forces are unused).
+    {
+        int cn = b->calc_noise;
+        b->calc_noise = 0;
+        int r1 = b->run_element();
+        b->calc_noise = cn;
+        if (r1 || b->run_assemble() || hipStreamSynchronize(b->stream) != hipSuccess) {
+            if (g_err.empty()) fail(FFH_E_HIP, "initial assembly failed");
+            return nullptr;
+        }
+    }
+    return b.release();
+}
+
+void ffh_blob_destroy(ffh_blob *b) { delete b; }
+
+int ffh_blob_update_internal_forces(ffh_blob *b, int *num_inversions) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    int rc;
+    if ((rc = b->reset_inversions())) return rc;
+    if ((rc = b->run_element())) return rc;
+    if ((rc = b->run_assemble())) return rc;
+    return b->read_inversions(num_inversions);
+}
+
+int ffh_blob_update_positions(ffh_blob *b) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    int rc;
+    if ((rc = b->run_rhs())) return rc;
+    int cg = b->run_cg();
+    if ((rc = b->run_integrate())) return rc;
+    return cg;
+}
+
+int ffh_blob_step(ffh_blob *b, int nsteps) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    for (int s = 0; s < nsteps; s++) {
+        int rc;
+        if ((rc = b->reset_inversions())) return rc;
+        if ((rc = b->run_element())) return rc;
+        if ((rc = b->run_assemble())) return rc;
+        if ((rc = b->run_rhs())) return rc;
+        int cg = b->run_cg();  // synchronises: the inversion count is then final too
+        if (b->ctl_host->inversions > 0) {
+            // the reference aborts before the position phase (src/World.cpp:2025-2028)
+            int ninv = 0;
+            return b->read_inversions(&ninv);
+        }
+        if ((rc = b->run_integrate())) return rc;
+        if (cg) return cg;
+    }
+    return FFH_OK;
+}
+
+int ffh_blob_solve(ffh_blob *b, double *xio) {
+    if (!b || !xio) return fail(FFH_E_ARG, "null argument");
+    const ffh::Plan &P = b->plan;
+    for (size_t k = 0; k < P.mid_index.size(); k++) {
+        int m = P.mid_index[k];
+        if (xio[3 * m] != 0.0 || xio[3 * m + 1] != 0.0 || xio[3 * m + 2] != 0.0)
+            return fail(FFH_E_ARG, "midpoint rows of the force vector must be zero");
+    }
+    std::vector<double> fl(3 * (size_t)b->N);
+    for (int i = 0; i < b->N; i++)
+        for (int c = 0; c < 3; c++) fl[3 * i + c] = xio[3 * (size_t)P.orig_of[i] + c];
+    // Load the RHS as the "element forces" would: run a CG with f given directly.
+    HIP_OR_FAIL(hipMemcpyAsync(b->f, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    // reuse k_rhs with an empty gather: emulate by a tiny init on the host side
+    {
+        VecArgs v = b->vec_args();
+        // r = f, x = 0, p = 0, delta and f2
+        HIP_OR_FAIL(hipMemcpyAsync(b->r, b->f, fl.size() * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->x, 0, fl.size() * sizeof(double), b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->p0, 0, fl.size() * sizeof(double), b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->p1, 0, fl.size() * sizeof(double), b->stream));
+        std::vector<double> dinv(fl.size());
+        HIP_OR_FAIL(hipMemcpyAsync(dinv.data(), b->dinv, fl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+        (void)v;
+        double delta = 0, f2 = 0;
+        for (int i = 0; i < b->N; i++) {
+            double z0 = dinv[3 * i] * fl[3 * i], z1 = dinv[3 * i + 1] * fl[3 * i + 1], z2 = dinv[3 * i + 2] * fl[3 * i + 2];
+            delta += fl[3 * i] * z0 + fl[3 * i + 1] * z1 + fl[3 * i + 2] * z2;
+            f2 += fl[3 * i] * fl[3 * i] + fl[3 * i + 1] * fl[3 * i + 1] + fl[3 * i + 2] * fl[3 * i + 2];
+        }
+        CgCtl c{};
+        c.delta = delta;
+        c.f2 = f2;
+        c.eps2 = b->eps2;
+        c.maxit = b->maxit;
+        c.done = f2 == 0.0 ? 1 : 0;
+        HIP_OR_FAIL(hipMemcpy(b->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+    }
+    int rc = b->run_cg();
+    std::vector<double> xl(fl.size());
+    HIP_OR_FAIL(hipMemcpyAsync(xl.data(), b->x, xl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < b->N; i++)
+        for (int c = 0; c < 3; c++) xio[3 * (size_t)P.orig_of[i] + c] = xl[3 * i + c];
+    return rc;
+}
+
+int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
+    if (!b || !in || !result) return fail(FFH_E_ARG, "null argument");
+    const ffh::Plan &P = b->plan;
+    std::vector<double> il(3 * (size_t)b->N), ol(3 * (size_t)b->N);
+    for (int i = 0; i < b->N; i++)
+        for (int c = 0; c < 3; c++) il[3 * i + c] = in[3 * (size_t)P.orig_of[i] + c];
+    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    SpmvArgs m = b->spmv_args();
+    double *ti = b->tmp_in, *to = b->tmp_out;
+    int N = b->N;
+    if (b->precision == FFH_PRECISION_FP64)
+        hipLaunchKernelGGL((k_apply<double>), b->grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
+    else
+        hipLaunchKernelGGL((k_apply<float>), b->grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipMemcpyAsync(ol.data(), b->tmp_out, ol.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    // midpoint rows are identity (one + zero drag); linear rows from the device
+    for (int i = 0; i < P.num_nodes_total; i++)
+        for (int c = 0; c < 3; c++) result[3 * (size_t)i + c] = in[3 * (size_t)i + c];
+    for (int i = 0; i < b->N; i++)
+        for (int c = 0; c < 3; c++) result[3 * (size_t)P.orig_of[i] + c] = ol[3 * i + c];
+    return FFH_OK;
+}
+
+int ffh_blob_set_positions(ffh_blob *b, const double *pos) {
+    if (!b || !pos) return fail(FFH_E_ARG, "null argument");
+    const int N = b->N;
+    std::vector<double> h(3 * (size_t)N);
+    for (int i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) h[(size_t)c * N + i] = pos[3 * (size_t)b->plan.orig_of[i] + c];
+    HIP_OR_FAIL(hipMemcpyAsync(b->X, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
+}
+
+int ffh_blob_get_positions(ffh_blob *b, double *pos) {
+    if (!b || !pos) return fail(FFH_E_ARG, "null argument");
+    const int N = b->N;
+    const ffh::Plan &P = b->plan;
+    std::vector<double> h(3 * (size_t)N);
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->X, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) pos[3 * (size_t)P.orig_of[i] + c] = h[(size_t)c * N + i];
+    // linearise_element (src/tetra_element_linear.cpp:567-591)
+    for (size_t k = 0; k < P.mid_index.size(); k++)
+        for (int c = 0; c < 3; c++)
+            pos[3 * (size_t)P.mid_index[k] + c] = .5 * (pos[3 * (size_t)P.mid_a[k] + c] + pos[3 * (size_t)P.mid_b[k] + c]);
+    return FFH_OK;
+}
+
+static int get_node_vector(ffh_blob *b, const double *dev, double *out) {
+    const ffh::Plan &P = b->plan;
+    std::vector<double> h(3 * (size_t)b->N);
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    std::memset(out, 0, sizeof(double) * 3 * (size_t)P.num_nodes_total);
+    for (int i = 0; i < b->N; i++)
+        for (int c = 0; c < 3; c++) out[3 * (size_t)P.orig_of[i] + c] = h[3 * i + c];
+    return FFH_OK;
+}
+
+int ffh_blob_get_velocities(ffh_blob *b, double *vel) {
+    if (!b || !vel) return fail(FFH_E_ARG, "null argument");
+    return get_node_vector(b, b->vel, vel);
+}
+
+int ffh_blob_get_rhs(ffh_blob *b, double *f) {
+    if (!b || !f) return fail(FFH_E_ARG, "null argument");
+    return get_node_vector(b, b->f, f);
+}
+
+int ffh_blob_get_rng_state(ffh_blob *b, unsigned long long *state6) {
+    if (!b || !state6) return fail(FFH_E_ARG, "null argument");
+    uint32_t s[6];
+    HIP_OR_FAIL(hipMemcpyAsync(s, b->rng_state, sizeof(s), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < 6; i++) state6[i] = s[i];
+    return FFH_OK;
+}
+
+int ffh_blob_set_rng_state(ffh_blob *b, const unsigned long long *state6) {
+    if (!b || !state6) return fail(FFH_E_ARG, "null argument");
+    uint32_t s[6];
+    for (int i = 0; i < 6; i++) {
+        if (state6[i] >= (i < 3 ? ffh::kM1 : ffh::kM2)) return fail(FFH_E_ARG, "RngStream state out of range");
+        s[i] = (uint32_t)state6[i];
+    }
+    HIP_OR_FAIL(hipMemcpyAsync(b->rng_state, s, sizeof(s), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
+}
+
+int ffh_blob_last_cg(ffh_blob *b, int *iterations, double *residual2) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (iterations) *iterations = b->last_iters;
+    if (residual2) *residual2 = b->last_res;
+    return FFH_OK;
+}
+
+int ffh_blob_sizes(ffh_blob *b, int *num_linear, int *num_elements, long long *num_blocks, long long *num_slots) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (num_linear) *num_linear = b->N;
+    if (num_elements) *num_elements = b->E;
+    if (num_blocks) *num_blocks = b->plan.num_blocks;
+    if (num_slots) *num_slots = b->plan.num_slots;
+    return FFH_OK;
+}
+
+void *ffh_blob_stream(ffh_blob *b) { return b ? (void *)b->stream : nullptr; }
+
+int ffh_blob_synchronize(ffh_blob *b) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
+}
+
+int ffh_blob_set_profiling(ffh_blob *b, int enable) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    b->collect_events(true);
+    b->profiling = enable != 0;
+    for (int k = 0; k < FFH_NUM_KERNELS; k++) {
+        b->ktime[k] = 0;
+        b->kcount[k] = 0;
+    }
+    return FFH_OK;
+}
+
+int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    b->collect_events(true);
+    for (int k = 0; k < FFH_NUM_KERNELS; k++) {
+        if (ms) ms[k] = b->ktime[k];
+        if (counts) counts[k] = b->kcount[k];
+    }
+    return FFH_OK;
+}
+
+int ffh_blob_strain_energy(ffh_blob *b, double *energy) {
+    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
+    ElemArgs g = b->elem_args();
+    hipLaunchKernelGGL(k_strain_energy, b->grid(b->E), kBlock, 0, b->stream, g, b->partials, b->ticket, b->scalar_out);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipMemcpyAsync(energy, b->scalar_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
+}
+
+}  // extern "C"

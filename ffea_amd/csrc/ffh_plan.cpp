@@ -1,0 +1,345 @@
+// ffh_plan.cpp -- host setup of a blob for the GPU step (compiled with -ffp-contract=off:
+// the rest-state arithmetic rounds exactly as the reference's strict build).
+//
+// Follows: Blob::init order (src/Blob.cpp:335-412), calc_rest_state_info (:3760-3822) with
+// mat3_invert (src/mat_vec_fns.cpp:148-177), load_material_params conversions
+// (src/Blob.cpp:2861-2866), stokes drag (:408-412), calculate_node_element_connectivity
+// (:3974-4019) and the pattern registration of NoMassCGSolver::init
+// (src/NoMassCGSolver.cpp:93-139) restricted to linear nodes.
+#include "ffh_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ffh {
+
+namespace {
+
+// mesoDimensions (include/dimensions.h:30-41)
+constexpr double kMesoVolume = 4.913e-30;
+constexpr double kMesoMass = 1.994307387553024e-26;
+constexpr double kMesoPressure = 8.4305832688784826e8;
+constexpr double kMesoTime = 3.7302915560886126e-13;
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// MRG32k3a one-step transition matrices (reduced mod m: -a13n -> m1 - a13n).
+const uint64_t kA[2][3][3] = {
+    {{0, 1, 0}, {0, 0, 1}, {kM1 - 810728ull, 1403580ull, 0}},
+    {{0, 1, 0}, {0, 0, 1}, {kM2 - 1370589ull, 0, 527612ull}},
+};
+
+struct M3 {
+    uint64_t a[3][3];
+};
+
+M3 mul(const M3 &x, const M3 &y, uint64_t m) {
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            unsigned __int128 s = 0;
+            for (int k = 0; k < 3; k++) s += (unsigned __int128)x.a[i][k] * y.a[k][j];
+            r.a[i][j] = (uint64_t)(s % m);
+        }
+    return r;
+}
+
+M3 ident() {
+    M3 r{};
+    for (int i = 0; i < 3; i++) r.a[i][i] = 1;
+    return r;
+}
+
+M3 base(int comp) {
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.a[i][j] = kA[comp][i][j];
+    return r;
+}
+
+Mat3u to32(const M3 &x) {
+    Mat3u r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.m[i][j] = (uint32_t)x.a[i][j];
+    return r;
+}
+
+M3 powm(int comp, uint64_t n) {
+    uint64_t m = comp ? kM2 : kM1;
+    M3 r = ident(), w = base(comp);
+    while (n) {
+        if (n & 1) r = mul(w, r, m);
+        w = mul(w, w, m);
+        n >>= 1;
+    }
+    return r;
+}
+
+// Jacobian + cofactors + determinant in the reference's operation order
+// (src/tetra_element_linear.cpp:193-205, :223-258).
+void jacobian(const double *x, const double *y, const double *z, const int *n, double J[3][3]) {
+    const double *p[3] = {x, y, z};
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) J[r][c] = p[c][n[r + 1]] - p[c][n[0]];
+}
+
+}  // namespace
+
+Mat3u mrg_pow(int comp, uint64_t n) { return to32(powm(comp, n)); }
+
+std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
+                       const double *material_si, const double *stokes_radius, int num_pinned,
+                       const int *pinned, double stokes_visc, int calc_noise, int calc_stokes) {
+    if (num_nodes <= 0 || num_elements <= 0) return "mesh must have nodes and elements";
+    if ((uint64_t)num_elements >= (1u << 27)) return "more than 2^27 elements per blob is not supported";
+    const int NT = num_nodes, E = num_elements;
+    P.num_nodes_total = NT;
+    P.num_elements = E;
+
+    // --- linear nodes (load_topology marks n[0..3] linear, src/Blob.cpp:2618-2621) ---
+    std::vector<uint8_t> is_lin(NT, 0);
+    for (int e = 0; e < E; e++)
+        for (int j = 0; j < 10; j++) {
+            int v = topo[10 * e + j];
+            if (v < 0 || v >= NT) return "element " + std::to_string(e) + " references an out of bounds node index";
+            if (j < 4) is_lin[v] = 1;
+        }
+    for (int e = 0; e < E; e++)
+        for (int j = 4; j < 10; j++)
+            if (is_lin[topo[10 * e + j]])
+                return "node " + std::to_string(topo[10 * e + j]) + " is both a vertex and an edge midpoint";
+    P.lin_of.assign(NT, -1);
+    P.orig_of.clear();
+    for (int i = 0; i < NT; i++)
+        if (is_lin[i]) {
+            P.lin_of[i] = (int)P.orig_of.size();
+            P.orig_of.push_back(i);
+        }
+    const int N = (int)P.orig_of.size();
+    P.num_linear = N;
+    if (calc_stokes == 1)
+        for (int i = 0; i < NT; i++)
+            if (!is_lin[i] && stokes_radius[i] != 0.0)
+                return "midpoint node " + std::to_string(i) +
+                       " has a nonzero Stokes radius; the GPU path requires zero drag on midpoints";
+
+    // --- midpoints for N_t output ---
+    {
+        std::vector<uint8_t> seen(NT, 0);
+        static const int ends[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+        P.mid_index.clear();
+        P.mid_a.clear();
+        P.mid_b.clear();
+        for (int e = 0; e < E; e++)
+            for (int m = 0; m < 6; m++) {
+                int v = topo[10 * e + 4 + m];
+                if (seen[v]) continue;
+                seen[v] = 1;
+                P.mid_index.push_back(v);
+                P.mid_a.push_back(topo[10 * e + ends[m][0]]);
+                P.mid_b.push_back(topo[10 * e + ends[m][1]]);
+            }
+    }
+
+    // --- positions (linear nodes, SoA) ---
+    P.pos.assign(3 * (size_t)N, 0.0);
+    for (int i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) P.pos[(size_t)c * N + i] = pos[3 * (size_t)P.orig_of[i] + c];
+    const double *X = P.pos.data(), *Y = X + N, *Z = Y + N;
+
+    // --- elements: compact connectivity, materials, rest state ---
+    P.enodes.resize(4 * (size_t)E);
+    P.jinv0.resize(9 * (size_t)E);
+    P.vol0.resize(E);
+    P.last_det.resize(E);
+    P.matA.resize(E);
+    P.matB.resize(E);
+    P.matG.resize(E);
+    P.matE.resize(E);
+    for (int e = 0; e < E; e++) {
+        int n[4];
+        for (int a = 0; a < 4; a++) n[a] = P.enodes[4 * (size_t)e + a] = P.lin_of[topo[10 * e + a]];
+        const double *m = &material_si[6 * (size_t)e];
+        P.matA[e] = m[1] / (kMesoPressure * kMesoTime);
+        P.matB[e] = m[2] / (kMesoPressure * kMesoTime) - (2.0 / 3.0) * m[1];
+        P.matG[e] = m[3] / kMesoPressure;
+        P.matE[e] = m[4] / kMesoPressure;
+        (void)kMesoVolume;
+        (void)kMesoMass;
+        double J[3][3];
+        jacobian(X, Y, Z, n, J);
+        // mat3_invert (src/mat_vec_fns.cpp:148-177)
+        double mi[3][3];
+        mi[0][0] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+        mi[0][1] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+        mi[0][2] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+        mi[1][0] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+        mi[1][1] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+        mi[1][2] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+        mi[2][0] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+        mi[2][1] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+        mi[2][2] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+        double det = J[0][0] * mi[0][0] + J[1][0] * mi[0][1] + J[2][0] * mi[0][2];
+        if (det == 0.0) return "element " + std::to_string(e) + " has zero rest volume";
+        double inv = 1.0 / det;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) P.jinv0[(size_t)(3 * i + j) * E + e] = mi[i][j] * inv;
+        // calc_shape_function_derivatives_and_volume: the same cofactors give det and vol.
+        P.vol0[e] = (1.0 / 6.0) * std::fabs(det);
+        P.last_det[e] = det;
+    }
+
+    // --- nodes: drag, pins, RNG offsets ---
+    P.zeta.assign(N, 0.0);
+    P.pinned.assign(N, 0);
+    for (int i = 0; i < num_pinned; i++) {
+        int v = pinned[i];
+        if (v < 0 || v >= NT) return "pinned node index out of bounds";
+        if (P.lin_of[v] >= 0) P.pinned[P.lin_of[v]] = 1;
+        // pinned midpoints carry no force in the reference either (linearise_force zeroes them)
+    }
+    if (calc_stokes == 1)
+        for (int i = 0; i < N; i++)
+            if (!P.pinned[i]) P.zeta[i] = 6.0 * kPi * stokes_visc * stokes_radius[P.orig_of[i]];
+    P.rng_node_off.resize(N);
+    for (int i = 0; i < N; i++) P.rng_node_off[i] = 3u * (uint32_t)P.orig_of[i];
+
+    // --- node force gather (element order), src/Blob.cpp:3974-4019 ---
+    P.nctb_off.assign((size_t)N + 1, 0);
+    for (int e = 0; e < E; e++)
+        for (int a = 0; a < 4; a++) P.nctb_off[P.enodes[4 * (size_t)e + a] + 1]++;
+    for (int i = 0; i < N; i++) P.nctb_off[i + 1] += P.nctb_off[i];
+    P.nctb.resize(P.nctb_off[N]);
+    {
+        std::vector<uint32_t> cur(P.nctb_off.begin(), P.nctb_off.end() - 1);
+        for (int e = 0; e < E; e++)
+            for (int a = 0; a < 4; a++) {
+                int n = P.enodes[4 * (size_t)e + a];
+                P.nctb[cur[n]++] = ((uint32_t)e << 2) | (uint32_t)a;
+            }
+    }
+
+    // --- block pattern (linear, unpinned pairs + every diagonal), rows sorted by column ---
+    std::vector<uint64_t> keys;
+    keys.reserve(16 * (size_t)E + N);
+    for (int e = 0; e < E; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
+                if (P.pinned[r] || P.pinned[c]) continue;
+                keys.push_back(((uint64_t)r << 32) | (uint32_t)c);
+            }
+    for (int i = 0; i < N; i++) keys.push_back(((uint64_t)i << 32) | (uint32_t)i);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int64_t NB = (int64_t)keys.size();
+    P.num_blocks = NB;
+    std::vector<int64_t> row_off((size_t)N + 1, 0);
+    std::vector<int32_t> bcol(NB);
+    for (int64_t k = 0; k < NB; k++) {
+        row_off[(keys[k] >> 32) + 1]++;
+        bcol[k] = (int32_t)(keys[k] & 0xffffffffu);
+    }
+    for (int i = 0; i < N; i++) row_off[i + 1] += row_off[i];
+    keys.clear();
+    keys.shrink_to_fit();
+    auto find_block = [&](int r, int c) -> int64_t {
+        auto b = bcol.begin() + row_off[r], en = bcol.begin() + row_off[r + 1];
+        auto it = std::lower_bound(b, en, c);
+        return (it != en && *it == c) ? (int64_t)(it - bcol.begin()) : -1;
+    };
+    std::vector<uint32_t> bctb_off((size_t)NB + 1, 0);
+    for (int e = 0; e < E; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
+                if (P.pinned[r] || P.pinned[c]) continue;
+                bctb_off[find_block(r, c) + 1]++;
+            }
+    for (int64_t k = 0; k < NB; k++) bctb_off[k + 1] += bctb_off[k];
+    std::vector<uint32_t> bctb(bctb_off[NB]);
+    {
+        std::vector<uint32_t> cur(bctb_off.begin(), bctb_off.end() - 1);
+        for (int e = 0; e < E; e++)
+            for (int a = 0; a < 4; a++)
+                for (int b = 0; b < 4; b++) {
+                    int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
+                    if (P.pinned[r] || P.pinned[c]) continue;
+                    int64_t k = find_block(r, c);
+                    uint32_t code = a <= b ? ((uint32_t)pair_index(a, b) << 1) : (((uint32_t)pair_index(b, a) << 1) | 1u);
+                    bctb[cur[k]++] = ((uint32_t)e << 5) | code;
+                }
+    }
+
+    // --- SELL-64 layout ---
+    P.num_slices = (N + kSlice - 1) / kSlice;
+    P.slice_off.assign((size_t)P.num_slices + 1, 0);
+    P.slice_width.assign(P.num_slices, 0);
+    for (int s = 0; s < P.num_slices; s++) {
+        int w = 0;
+        for (int l = 0; l < kSlice; l++) {
+            int r = s * kSlice + l;
+            if (r < N) w = std::max<int>(w, (int)(row_off[r + 1] - row_off[r]));
+        }
+        P.slice_width[s] = w;
+        P.slice_off[s + 1] = P.slice_off[s] + (int64_t)w * kSlice;
+    }
+    P.num_slots = P.slice_off[P.num_slices];
+    if (P.num_slots >= (int64_t)1 << 31) return "matrix pattern too large";
+    P.slot_col.assign(P.num_slots, 0);
+    P.slot_flags.assign(P.num_slots, 4);
+    P.sctb_off.assign((size_t)P.num_slots + 1, 0);
+    for (int s = 0; s < P.num_slices; s++)
+        for (int t = 0; t < P.slice_width[s]; t++)
+            for (int l = 0; l < kSlice; l++) {
+                int64_t slot = P.slice_off[s] + (int64_t)t * kSlice + l;
+                int r = s * kSlice + l;
+                if (r < N && t < row_off[r + 1] - row_off[r]) {
+                    int64_t k = row_off[r] + t;
+                    P.slot_col[slot] = bcol[k];
+                    uint8_t f = 0;
+                    if (bcol[k] == r) f |= 1;
+                    if (P.pinned[r]) f |= 2;
+                    P.slot_flags[slot] = f;
+                    P.sctb_off[slot + 1] = bctb_off[k + 1] - bctb_off[k];
+                } else {
+                    P.slot_col[slot] = r < N ? r : N - 1;  // zero block on a valid column
+                }
+            }
+    for (int64_t k = 0; k < P.num_slots; k++) P.sctb_off[k + 1] += P.sctb_off[k];
+    P.sctb.resize(P.sctb_off[P.num_slots]);
+    for (int s = 0; s < P.num_slices; s++)
+        for (int t = 0; t < P.slice_width[s]; t++)
+            for (int l = 0; l < kSlice; l++) {
+                int64_t slot = P.slice_off[s] + (int64_t)t * kSlice + l;
+                int r = s * kSlice + l;
+                if (r < N && t < row_off[r + 1] - row_off[r]) {
+                    int64_t k = row_off[r] + t;
+                    std::copy(bctb.begin() + bctb_off[k], bctb.begin() + bctb_off[k + 1],
+                              P.sctb.begin() + P.sctb_off[slot]);
+                }
+            }
+
+    // --- RNG: draws per step and digit jump tables ---
+    P.elem_draws = calc_noise == 1 ? 7ull * (uint64_t)E : 0;
+    uint64_t node_draws = (calc_noise == 1 && calc_stokes == 1) ? 3ull * (uint64_t)NT : 0;
+    P.draws_per_step = P.elem_draws + node_draws;
+    if (P.draws_per_step >= (1ull << 32)) return "more than 2^32 random draws per step is not supported";
+    P.jump.resize(4 * 256 * 2);
+    for (int comp = 0; comp < 2; comp++) {
+        uint64_t m = comp ? kM2 : kM1;
+        M3 step = base(comp);  // A^(256^d)
+        for (int d = 0; d < 4; d++) {
+            M3 acc = ident();
+            for (int v = 0; v < 256; v++) {
+                P.jump[((size_t)d * 256 + v) * 2 + comp] = to32(acc);
+                acc = mul(step, acc, m);
+            }
+            step = acc;  // A^(256^(d+1))
+        }
+        P.step_jump[comp] = to32(powm(comp, P.draws_per_step));
+    }
+    return "";
+}
+
+}  // namespace ffh

@@ -1,0 +1,75 @@
+// ffh_plan.h -- host-side setup of a blob for the GPU path: linear-node compaction,
+// rest state, the block sparsity pattern in sliced-ELL (SELL-64) order with its
+// element-to-slot gather maps, the node force gather maps and the RngStream jump tables.
+// Built once per blob (the work NoMassCGSolver::init and Blob::init do on the CPU).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ffh {
+
+constexpr int kSlice = 64;  // rows per SELL slice = one wavefront
+constexpr uint32_t kM1 = 4294967087u;
+constexpr uint32_t kM2 = 4294944443u;
+
+// Index of the 3x3 block (a,b), a<=b, of an element's 12x12 viscosity matrix.
+inline int pair_index(int a, int b) {
+    static const int tbl[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+    return tbl[a][b];
+}
+
+struct Mat3u {
+    uint32_t m[3][3];
+};
+
+struct Plan {
+    // sizes
+    int num_nodes_total = 0;  // N_t
+    int num_linear = 0;       // N
+    int num_elements = 0;     // E
+    // node maps
+    std::vector<int> lin_of;   // [N_t] -> compact index or -1
+    std::vector<int> orig_of;  // [N]   -> original index
+    // per element (compact node ids), rest data and material in meso units
+    std::vector<int32_t> enodes;   // [4*E] AoS (int4 per element)
+    std::vector<double> jinv0;     // [9*E] SoA: jinv0[k*E + e]
+    std::vector<double> vol0;      // [E]
+    std::vector<double> last_det;  // [E]
+    std::vector<double> matA, matB, matG, matE;  // [E]
+    // per linear node
+    std::vector<double> pos;     // [3*N] SoA x|y|z of the current positions
+    std::vector<double> zeta;    // [N] Stokes drag (0 when calc_stokes == 0)
+    std::vector<uint8_t> pinned; // [N]
+    std::vector<uint32_t> rng_node_off;  // [N] 3*original index (offset inside the node draws)
+    // midpoint reconstruction for N_t output: for each midpoint, its two edge ends (original ids)
+    std::vector<int32_t> mid_index, mid_a, mid_b;
+    // force gather: per linear node, contributions (e << 2 | a) in ascending element order
+    std::vector<uint32_t> nctb_off, nctb;
+    // block pattern in SELL-64 order
+    int num_slices = 0;
+    int64_t num_blocks = 0;  // real 3x3 blocks
+    int64_t num_slots = 0;   // padded slots = sum over slices of 64 * width
+    std::vector<int64_t> slice_off;  // [num_slices+1] first slot of each slice
+    std::vector<int32_t> slice_width;
+    std::vector<int32_t> slot_col;    // [num_slots]
+    std::vector<uint8_t> slot_flags;  // bit0 diagonal block, bit1 identity (pinned row), bit2 padding
+    std::vector<uint32_t> sctb_off;   // [num_slots+1]
+    std::vector<uint32_t> sctb;       // e << 5 | pair << 1 | transpose, ascending e per slot
+    // RNG jump tables: tbl[((d*256 + v)*2 + comp)] = A_comp^(v * 256^d), d=0..3
+    std::vector<Mat3u> jump;
+    Mat3u step_jump[2];        // A^(draws per step)
+    uint64_t draws_per_step = 0;
+    uint64_t elem_draws = 0;   // 7*E when calc_noise
+};
+
+// Builds the plan; returns "" on success or an error message.
+std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
+                       const double *material_si, const double *stokes_radius, int num_pinned,
+                       const int *pinned, double stokes_visc, int calc_noise, int calc_stokes);
+
+// Exact jump matrices of MRG32k3a component c (0: m1, 1: m2) raised to n.
+Mat3u mrg_pow(int comp, uint64_t n);
+
+}  // namespace ffh

@@ -1,0 +1,252 @@
+"""FFEA text formats on the massless path: .node .top .surf .mat .stokes .pin and the
+.ffea script, read with the reference loaders' rules and written as ffeatools writes them.
+
+Readers follow Blob::load_nodes (src/Blob.cpp:2415-2513), load_topology (:2517-2640),
+load_surface (:2644-2739), load_material_params (:2821-2876), load_stokes_params
+(:2880-2947), load_pinned_nodes (:3691-3755); the script parser follows the
+<param>/<system><blob><conformation> layout of FFEA_input_reader
+(include/FFEA_input_reader.h:38-76) for the keys this path reads.
+"""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+
+from .mesh import Mesh
+
+
+class FormatError(ValueError):
+    pass
+
+
+def _lines(path):
+    with open(path) as f:
+        return f.read().split("\n")
+
+
+def _header(lines, path, kind):
+    if lines[0] not in (f"ffea {kind}", f"walrus {kind}"):
+        raise FormatError(f"{path}: this is not a 'ffea {kind}' (read '{lines[0]}')")
+
+
+def _count(line, key, path):
+    m = re.match(rf"{key}\s+(-?\d+)\s*$", line)
+    if not m:
+        raise FormatError(f"{path}: expected '{key} <n>', read '{line}'")
+    return int(m.group(1))
+
+
+def read_node(path):
+    """Returns (pos [N,3] float64, num_surface_nodes)."""
+    L = _lines(path)
+    _header(L, path, "node file")
+    n = _count(L[1], "num_nodes", path)
+    ns = _count(L[2], "num_surface_nodes", path)
+    _count(L[3], "num_interior_nodes", path)
+    if L[4] != "surface nodes:":
+        raise FormatError(f"{path}: could not find 'surface nodes:' line")
+    pos = np.zeros((n, 3))
+    k = 5
+    for i in range(n):
+        if i == ns:
+            if L[k] != "interior nodes:":
+                raise FormatError(f"{path}: could not find 'interior nodes:' line")
+            k += 1
+        v = L[k].split()
+        if len(v) < 3:
+            raise FormatError(f"{path}: error reading node {i}")
+        pos[i] = [float(v[0]), float(v[1]), float(v[2])]
+        k += 1
+    return pos, ns
+
+
+def read_topology(path, num_nodes=None):
+    """Returns (topo [E,10] int32, num_surface_elements)."""
+    L = _lines(path)
+    _header(L, path, "topology file")
+    e = _count(L[1], "num_elements", path)
+    es = _count(L[2], "num_surface_elements", path)
+    _count(L[3], "num_interior_elements", path)
+    if L[4] != "surface elements:":
+        raise FormatError(f"{path}: could not find 'surface elements:' line")
+    topo = np.zeros((e, 10), np.int32)
+    k = 5
+    for i in range(e):
+        if i == es:
+            if L[k] != "interior elements:":
+                raise FormatError(f"{path}: could not find 'interior elements:' line")
+            k += 1
+        v = L[k].split()
+        if len(v) < 10:
+            raise FormatError(f"{path}: error reading element {i}")
+        topo[i] = [int(x) for x in v[:10]]
+        k += 1
+    if num_nodes is not None and (topo.min() < 0 or topo.max() >= num_nodes):
+        raise FormatError(f"{path}: element references an out of bounds node index")
+    return topo, es
+
+
+def read_material(path, num_elements=None):
+    L = _lines(path)
+    _header(L, path, "material params file")
+    e = _count(L[1], "num_elements", path)
+    if num_elements is not None and e != num_elements:
+        raise FormatError(f"{path}: {e} elements, topology has {num_elements}")
+    mat = np.array([[float(x) for x in L[2 + i].split()[:6]] for i in range(e)], np.float64)
+    if mat.shape != (e, 6):
+        raise FormatError(f"{path}: there should be 6 values per element")
+    return mat
+
+
+def read_stokes(path, num_nodes=None):
+    L = _lines(path)
+    _header(L, path, "stokes radii file")
+    n = _count(L[1], "num_nodes", path)
+    if num_nodes is not None and n != num_nodes:
+        raise FormatError(f"{path}: {n} nodes, node file has {num_nodes}")
+    return np.array([float(L[2 + i].split()[0]) for i in range(n)], np.float64)
+
+
+def read_pin(path, num_nodes=None):
+    L = _lines(path)
+    _header(L, path, "pinned nodes file")
+    n = _count(L[1], "num_pinned_nodes", path)
+    if L[2] != "pinned nodes:":
+        raise FormatError(f"{path}: could not find 'pinned nodes:' line")
+    pins = np.array([int(L[3 + i].split()[0]) for i in range(n)], np.int32)
+    if num_nodes is not None and n and (pins.min() < 0 or pins.max() >= num_nodes):
+        raise FormatError(f"{path}: pinned node references an out of bounds node index")
+    return pins
+
+
+def read_surface(path):
+    L = _lines(path)
+    _header(L, path, "surface file")
+    n = _count(L[1], "num_surface_faces", path)
+    if L[2] != "faces:":
+        raise FormatError(f"{path}: could not find 'faces:' line")
+    return np.array([[int(x) for x in L[3 + i].split()[:4]] for i in range(n)], np.int32)
+
+
+def load_blob(node, top, mat, stokes, pin=None) -> Mesh:
+    pos, _ = read_node(node)
+    topo, _ = read_topology(top, pos.shape[0])
+    material = read_material(mat, topo.shape[0])
+    rad = read_stokes(stokes, pos.shape[0])
+    pins = read_pin(pin, pos.shape[0]) if pin else np.zeros(0, np.int32)
+    lin = np.unique(topo[:, :4])
+    return Mesh(pos=pos, topo=topo, material=material, stokes_radius=rad, num_linear=int(lin.size),
+                pinned=pins)
+
+
+# ---------------------------------------------------------------------------------------
+# writers
+# ---------------------------------------------------------------------------------------
+def write_node(path, pos, num_surface=None):
+    pos = np.asarray(pos)
+    n = pos.shape[0]
+    ns = n if num_surface is None else int(num_surface)
+    with open(path, "w") as f:
+        f.write(f"ffea node file\nnum_nodes {n}\nnum_surface_nodes {ns}\nnum_interior_nodes {n - ns}\n")
+        f.write("surface nodes:\n")
+        for i in range(n):
+            if i == ns:
+                f.write("interior nodes:\n")
+            f.write("%.17g %.17g %.17g\n" % tuple(pos[i]))
+        if ns == n:
+            f.write("interior nodes:\n")
+
+
+def write_topology(path, topo):
+    topo = np.asarray(topo)
+    e = topo.shape[0]
+    with open(path, "w") as f:
+        f.write(f"ffea topology file\nnum_elements {e}\nnum_surface_elements {e}\n"
+                f"num_interior_elements 0\nsurface elements:\n")
+        for row in topo:
+            f.write(" ".join(str(int(v)) for v in row) + "\n")
+        f.write("interior elements:\n")
+
+
+def write_material(path, material):
+    material = np.asarray(material)
+    with open(path, "w") as f:
+        f.write(f"ffea material params file\nnum_elements {material.shape[0]}\n")
+        for row in material:
+            f.write(" ".join("%.17g" % v for v in row) + "\n")
+
+
+def write_stokes(path, radius):
+    radius = np.asarray(radius)
+    with open(path, "w") as f:
+        f.write(f"ffea stokes radii file\nnum_nodes {radius.shape[0]}\n")
+        for r in radius:
+            f.write("%.17g\n" % r)
+
+
+def write_pin(path, pins):
+    pins = np.asarray(pins, np.int64)
+    with open(path, "w") as f:
+        f.write(f"ffea pinned nodes file\nnum_pinned_nodes {pins.size}\npinned nodes:\n")
+        for p in pins:
+            f.write(f"{int(p)}\n")
+
+
+def write_blob(prefix, mesh: Mesh):
+    write_node(prefix + ".node", mesh.pos)
+    write_topology(prefix + ".top", mesh.topo)
+    write_material(prefix + ".mat", mesh.material)
+    write_stokes(prefix + ".stokes", mesh.stokes_radius)
+    write_pin(prefix + ".pin", mesh.pinned)
+
+
+# ---------------------------------------------------------------------------------------
+# .ffea script (the keys of the CG_nomass path)
+# ---------------------------------------------------------------------------------------
+def read_script(path) -> dict:
+    """Parses <param> key/values and the first conformation of each <blob>.
+
+    Returns {"params": {key: str}, "blobs": [{"nodes":..., "topology":..., "material":...,
+    "stokes":..., "pin":..., "solver":..., "scale":...}]}; file paths resolved relative to
+    the script.
+    """
+    text = open(path).read()
+    base = os.path.dirname(os.path.abspath(path))
+    params = {}
+    m = re.search(r"<param>(.*?)</param>", text, re.S)
+    if not m:
+        raise FormatError(f"{path}: no <param> block")
+    for k, v in re.findall(r"<\s*([A-Za-z_0-9]+)\s*=\s*([^>]*)>", m.group(1)):
+        params[k.strip()] = v.strip()
+    blobs = []
+    for bm in re.finditer(r"<blob>(.*?)</blob>", text, re.S):
+        body = bm.group(1)
+        b = {}
+        for k, v in re.findall(r"<\s*([A-Za-z_0-9]+)\s*=\s*([^>]*)>", body):
+            k, v = k.strip(), v.strip()
+            if k in b:
+                continue  # first conformation only
+            b[k] = v
+        for k in ("nodes", "topology", "surface", "material", "stokes", "pin", "vdw"):
+            if k in b:
+                b[k] = os.path.normpath(os.path.join(base, b[k]))
+        blobs.append(b)
+    return {"params": params, "blobs": blobs}
+
+
+def params_from_script(params: dict):
+    """SimParams from a <param> block with SimulationParams::assign's conversions and
+    defaults (src/SimulationParams.cpp:26-60, :211-540)."""
+    from .units import SimParams, KB
+
+    def g(k, d):
+        return float(params[k]) if k in params else d
+
+    eps = g("epsilon", 0.1)  # default epsilon2 = 0.01 (src/SimulationParams.cpp:36)
+    stokes = int(float(params.get("calc_stokes", params.get("do_stokes", 1))))
+    return SimParams.from_si(dt=g("dt", 1e-14), kT=g("kT", 4.11e-21), epsilon=eps,
+                             max_iterations_cg=int(g("max_iterations_cg", 1000)),
+                             calc_noise=int(g("calc_noise", 1)), calc_stokes=stokes,
+                             stokes_visc=g("stokes_visc", 1e-3), rng_seed=int(g("rng_seed", 0)))

@@ -1,0 +1,153 @@
+/*
+ * ffea_hip.h -- C-ABI of the MI355X (gfx950) implementation of FFEA's massless
+ * (CG_nomass) fluctuating-FEM time step.
+ *
+ * One handle = one dynamic blob plus the RngStream that feeds it, resident on one
+ * GPU with its own HIP stream.  A handle is single-owner and not reentrant.
+ * Every call returns FFH_OK (0) or a negative FFH_E* code; ffh_last_error() gives
+ * the message of the last failure on the calling thread.
+ *
+ * Host arrays use FFEA's full node numbering (N_t nodes: linear nodes and the
+ * quadratic edge midpoints of the 10-node topology), positions and radii in meso
+ * units after `scale`, materials in SI as written in a .mat file.  On the device
+ * the blob is compacted to its linear nodes (midpoint rows of the reference's
+ * matrix are identity and carry no force; their positions are edge averages).
+ */
+#ifndef FFEA_HIP_H
+#define FFEA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFH_OK 0
+#define FFH_E_ARG -1         /* invalid argument or unsupported mesh */
+#define FFH_E_HIP -2         /* HIP runtime failure */
+#define FFH_E_INVERSION -3   /* an element changed orientation (run must stop) */
+#define FFH_E_CG -4          /* CG did not converge in max_iterations_cg */
+#define FFH_E_NOMEM -5
+
+#define FFH_PRECISION_FP64 0  /* element, assembly, SpMV and CG all FP64 */
+#define FFH_PRECISION_MIXED 1 /* FP32 element/assembly/SpMV, FP64 CG scalars, RHS and integration */
+
+typedef struct ffh_blob ffh_blob;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ffh_last_error(void);
+
+/* Library version, (major << 16) | minor. */
+int ffh_version(void);
+
+/*
+ * Create a blob on the current HIP device.
+ * Replaces Blob::init's post-load setup (src/Blob.cpp:335-412: linearise_element,
+ * calc_rest_state_info :3760-3822, calculate_node_element_connectivity :3974-4019,
+ * stokes_drag :408-412) and Solver::init (include/Solver.h:45;
+ * NoMassCGSolver::init, src/NoMassCGSolver.cpp:63-162).
+ *   num_nodes        N_t, all nodes of the .node file
+ *   num_elements     E
+ *   pos              [3*N_t] rest positions, meso units (Blob::load_nodes, :2480-2495)
+ *   topo             [10*E] node indices per element, FFEA order (load_topology :2606-2622)
+ *   material_si      [6*E] density, shear_visc, bulk_visc, shear_mod, bulk_mod, dielectric
+ *                    (load_material_params, src/Blob.cpp:2821-2876)
+ *   stokes_radius    [N_t] meso units (load_stokes_params :2880-2947); must be 0 on midpoints
+ *   pinned           [num_pinned] node indices (load_pinned_nodes :3691-3755)
+ *   dt, kT, epsilon2, stokes_visc   meso units as SimulationParams holds them
+ *   rng_state        [6] RngStream Cg state the blob's stream starts from (World::init
+ *                    RNG.2/.3, src/World.cpp:287-323; or a .fcp checkpoint line)
+ *   precision        FFH_PRECISION_FP64 or FFH_PRECISION_MIXED
+ * Returns NULL on failure (see ffh_last_error).
+ */
+ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
+                          const double *material_si, const double *stokes_radius, int num_pinned,
+                          const int *pinned, double dt, double kT, double epsilon2,
+                          int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                          const unsigned long long *rng_state, int precision);
+
+/* Release all device memory of the blob (Blob::~Blob / NoMassCGSolver::~NoMassCGSolver). */
+void ffh_blob_destroy(ffh_blob *b);
+
+/*
+ * Blob::update_internal_forces (src/Blob.cpp:638-735; include/Blob.h:130): the element
+ * loop (Jacobian, shape gradients, volume + inversion check, Kelvin-Voigt viscosity
+ * matrix, elastic + fluctuating stress, nodal forces) followed by the assembly of the
+ * viscosity matrix into its fixed pattern (SparseMatrixFixedPattern::build,
+ * src/SparseMatrixFixedPattern.cpp:85-93).  Returns FFH_OK, or FFH_E_INVERSION with the
+ * number of inverted elements in *num_inversions (may be NULL).
+ */
+int ffh_blob_update_internal_forces(ffh_blob *b, int *num_inversions);
+
+/*
+ * Blob::update_positions (src/Blob.cpp:737-764): aggregate_forces_and_solve (:3826-3932:
+ * force gather, Stokes noise, pinned rows, NoMassCGSolver::solve) then euler_integrate
+ * (:3937-3953) and linearise_element.  Returns FFH_OK or FFH_E_CG (then, as in the
+ * reference, positions are not integrated).
+ */
+int ffh_blob_update_positions(ffh_blob *b);
+
+/*
+ * nsteps iterations of World::run for this blob (src/World.cpp:1811-2043): element phase,
+ * then position phase, per step.  Stops at the first error.  Asynchronous with respect to
+ * the host except for the convergence checks of the CG solve.
+ */
+int ffh_blob_step(ffh_blob *b, int nsteps);
+
+/*
+ * Solver::solve (include/Solver.h:51; NoMassCGSolver::solve, src/NoMassCGSolver.cpp:165-197):
+ * x is [3*N_t] host memory, the force on entry, the velocity on exit.  Uses the viscosity
+ * matrix of the last update_internal_forces (or of creation).
+ */
+int ffh_blob_solve(ffh_blob *b, double *x);
+
+/*
+ * Solver::apply_matrix (include/Solver.h:54): result = K * in over [3*N_t] host vectors
+ * (K = assembled viscosity matrix with Stokes drag; midpoint and pinned rows identity).
+ */
+int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result);
+
+/* Node state over all N_t nodes, host memory [3*N_t], meso units.  get_positions re-averages
+ * midpoints (linearise_element); get_velocities reports 0 on midpoints, like the reference. */
+int ffh_blob_set_positions(ffh_blob *b, const double *pos);
+int ffh_blob_get_positions(ffh_blob *b, double *pos);
+int ffh_blob_get_velocities(ffh_blob *b, double *vel);
+
+/* The force vector handed to the last solve (after gather, noise and pinning), [3*N_t]. */
+int ffh_blob_get_rhs(ffh_blob *b, double *f);
+
+/* RngStream Cg state (RngStream::GetState / SetSeed, src/RngStream.cpp:388-395, :441-445),
+ * i.e. one line of a .fcp checkpoint (src/World.cpp:4234-4280). */
+int ffh_blob_get_rng_state(ffh_blob *b, unsigned long long *state6);
+int ffh_blob_set_rng_state(ffh_blob *b, const unsigned long long *state6);
+
+/* Iterations of the last CG solve and its final ||r||^2/||f||^2. */
+int ffh_blob_last_cg(ffh_blob *b, int *iterations, double *residual2);
+
+/* Sizes: linear nodes, elements, 3x3 blocks of the matrix pattern, padded SpMV slots. */
+int ffh_blob_sizes(ffh_blob *b, int *num_linear, int *num_elements, long long *num_blocks,
+                   long long *num_slots);
+
+/* The HIP stream (hipStream_t) all work of the blob is issued on. */
+void *ffh_blob_stream(ffh_blob *b);
+
+/* Wait for all work issued on the blob's stream. */
+int ffh_blob_synchronize(ffh_blob *b);
+
+/*
+ * Per-kernel timing with HIP events on the blob's stream.  When enabled, every launch is
+ * bracketed by events; ffh_blob_kernel_times returns, for each kernel id, the summed
+ * milliseconds and launch counts since the last reset.  Kernel ids:
+ *   0 element   1 assemble   2 rhs+CG init   3 CG spmv (p update, q = K p, p.q)
+ *   4 CG update (x, r, residual)   5 integrate
+ */
+#define FFH_NUM_KERNELS 6
+int ffh_blob_set_profiling(ffh_blob *b, int enable);
+int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts);
+
+/* Strain energy of Blob::make_measurements (src/Blob.cpp:1424-1432) from the last element
+ * update, in meso energy units. */
+int ffh_blob_strain_energy(ffh_blob *b, double *energy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,949 @@
+/*
+ * ffea_oracle.c -- FP64 CPU restatement of FFEA's massless (CG_nomass) time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path in
+ * ffea_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links it and has no CPU
+ * fallback through it.
+ *
+ * It restates, in plain C over flat arrays, the reference's algorithm in the
+ * reference's own evaluation order (build with -O2 -ffp-contract=off, no
+ * fast-math, so every sum and product rounds exactly as the reference's
+ * strict-FP build does):
+ *
+ *   RngStream (MRG32k3a)       src/RngStream.cpp:45-105 (constants),
+ *                              :112-217 (mod-m matrix ops), :266-310 (U01/U01d),
+ *                              :327-435 (streams, substreams, AdvanceState),
+ *                              :508-524 (RandU01/RandInt)
+ *   package seeding            src/World.cpp:287-323
+ *   element kernels            src/tetra_element_linear.cpp:193-205 (jacobian),
+ *                              :223-293 (dpsi + volume + inversion),
+ *                              :299-328 + :599-638 (viscosity matrix),
+ *                              :378-408 (elastic stress), :417-436 (noise),
+ *                              :441-449 + :474-490 (nodal forces),
+ *                              :567-591 (midpoint re-linearisation)
+ *                              include/tetra_element_linear.h:55-118 (macros)
+ *   math helpers               src/mat_vec_fns.cpp:93-177, :250-275
+ *   Blob glue                  src/Blob.cpp:335-412 (init order), :638-735
+ *                              (update_internal_forces), :737-764
+ *                              (update_positions), :1066-1085 (linearise_force),
+ *                              :3760-3822 (rest state), :3826-3932
+ *                              (aggregate_forces_and_solve), :3937-3953 (euler),
+ *                              :3974-4019 (connectivity), :2861-2866 (material
+ *                              unit conversion), :1424-1432 (strain energy)
+ *   solver                     src/NoMassCGSolver.cpp:63-162 (init / pattern),
+ *                              :165-197 (solve), :205-292 (vector kernels)
+ *   sparse pattern / matrix    src/SparsityPattern.cpp:57-143,
+ *                              src/SparseMatrixFixedPattern.cpp:48-68, :85-93,
+ *                              :148-191, :325-333; src/SparseMatrixTypes.cpp:52-58
+ *   step order                 src/World.cpp:1811-2043
+ *
+ * Layout is the reference's: all N_t nodes (linear + quadratic midpoints),
+ * 10-node elements, a scalar CSR over 3*N_t rows whose every entry keeps the
+ * list of element-matrix locations it sums.
+ *
+ * Parity pins (tests/): RngStream known-answer sum 39.697547445251
+ * (tests/consistency/rngstream/testRngStream.cpp); sequential-vs-jump stream
+ * equality; bitwise equality against the reference's own compiled objects
+ * (oracle/_ref) for elements, assembled matrix, CG solutions and trajectories;
+ * closed forms (unit tetrahedron, rest state stress-free, rigid motions).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* RngStream restatement (MRG32k3a, L'Ecuyer et al. 2002)                      */
+/* ------------------------------------------------------------------------- */
+
+static const double m1 = 4294967087.0;
+static const double m2 = 4294944443.0;
+static const double norm_ = 1.0 / (4294967087.0 + 1.0);
+static const double a12 = 1403580.0;
+static const double a13n = 810728.0;
+static const double a21 = 527612.0;
+static const double a23n = 1370589.0;
+static const double two17 = 131072.0;
+static const double two53 = 9007199254740992.0;
+static const double fact = 5.9604644775390625e-8;
+
+static const double InvA1[3][3] = {{184888585.0, 0.0, 1945170933.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}};
+static const double InvA2[3][3] = {{0.0, 360363334.0, 4225571728.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}};
+static const double A1p0[3][3] = {{0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}, {-810728.0, 1403580.0, 0.0}};
+static const double A2p0[3][3] = {{0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}, {-1370589.0, 0.0, 527612.0}};
+static const double A1p76[3][3] = {{82758667.0, 1871391091.0, 4127413238.0},
+                                   {3672831523.0, 69195019.0, 1871391091.0},
+                                   {3672091415.0, 3528743235.0, 69195019.0}};
+static const double A2p76[3][3] = {{1511326704.0, 3759209742.0, 1610795712.0},
+                                   {4292754251.0, 1511326704.0, 3889917532.0},
+                                   {3859662829.0, 4292754251.0, 3708466080.0}};
+static const double A1p127[3][3] = {{2427906178.0, 3580155704.0, 949770784.0},
+                                    {226153695.0, 1230515664.0, 3580155704.0},
+                                    {1988835001.0, 986791581.0, 1230515664.0}};
+static const double A2p127[3][3] = {{1464411153.0, 277697599.0, 1610723613.0},
+                                    {32183930.0, 1464411153.0, 1022607788.0},
+                                    {2824425944.0, 32183930.0, 2093834863.0}};
+
+/* (a*s + c) mod m for |a|,|s|,|c| < 2^35, exact in doubles (src/RngStream.cpp:112-130). */
+static double mult_mod_m(double a, double s, double c, double m) {
+    double v = a * s + c;
+    long a1;
+    if (v >= two53 || v <= -two53) {
+        a1 = (long)(a / two17);
+        a -= a1 * two17;
+        v = a1 * s;
+        a1 = (long)(v / m);
+        v -= a1 * m;
+        v = v * two17 + a * s + c;
+    }
+    a1 = (long)(v / m);
+    if ((v -= a1 * m) < 0.0) return v += m;
+    return v;
+}
+
+static void mat_vec_mod_m(const double A[3][3], const double s[3], double v[3], double m) {
+    double x[3];
+    for (int i = 0; i < 3; ++i) {
+        x[i] = mult_mod_m(A[i][0], s[0], 0.0, m);
+        x[i] = mult_mod_m(A[i][1], s[1], x[i], m);
+        x[i] = mult_mod_m(A[i][2], s[2], x[i], m);
+    }
+    for (int i = 0; i < 3; ++i) v[i] = x[i];
+}
+
+static void mat_mat_mod_m(const double A[3][3], const double B[3][3], double C[3][3], double m) {
+    double V[3], W[3][3];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) V[j] = B[j][i];
+        mat_vec_mod_m(A, V, V, m);
+        for (int j = 0; j < 3; ++j) W[j][i] = V[j];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[i][j] = W[i][j];
+}
+
+static void mat_two_pow_mod_m(const double A[3][3], double B[3][3], double m, long e) {
+    if (A != (const double(*)[3])B)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) B[i][j] = A[i][j];
+    for (long i = 0; i < e; i++) mat_mat_mod_m((const double(*)[3])B, (const double(*)[3])B, B, m);
+}
+
+static void mat_pow_mod_m(const double A[3][3], double B[3][3], double m, long n) {
+    double W[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            W[i][j] = A[i][j];
+            B[i][j] = 0.0;
+        }
+    for (int j = 0; j < 3; ++j) B[j][j] = 1.0;
+    while (n > 0) {
+        if (n % 2) mat_mat_mod_m((const double(*)[3])W, (const double(*)[3])B, B, m);
+        mat_mat_mod_m((const double(*)[3])W, (const double(*)[3])W, W, m);
+        n /= 2;
+    }
+}
+
+typedef struct {
+    double Cg[6], Bg[6], Ig[6];
+    int anti, inc_prec;
+} orc_stream;
+
+static double next_seed[6] = {12345.0, 12345.0, 12345.0, 12345.0, 12345.0, 12345.0};
+
+static int check_seed(const unsigned long seed[6]) {
+    for (int i = 0; i < 3; ++i)
+        if (seed[i] >= 4294967087UL) return -1;
+    for (int i = 3; i < 6; ++i)
+        if (seed[i] >= 4294944443UL) return -1;
+    if (seed[0] == 0 && seed[1] == 0 && seed[2] == 0) return -1;
+    if (seed[3] == 0 && seed[4] == 0 && seed[5] == 0) return -1;
+    return 0;
+}
+
+/* RngStream::SetPackageSeed (src/RngStream.cpp:378-385); returns 1 on success. */
+int orc_rng_set_package_seed(const unsigned long seed[6]) {
+    if (check_seed(seed)) return 0;
+    for (int i = 0; i < 6; ++i) next_seed[i] = (double)seed[i];
+    return 1;
+}
+
+/* Constructor RngStream::RngStream (src/RngStream.cpp:327-345). */
+void orc_rng_stream_init(orc_stream *g) {
+    g->anti = 0;
+    g->inc_prec = 0;
+    for (int i = 0; i < 6; ++i) g->Bg[i] = g->Cg[i] = g->Ig[i] = next_seed[i];
+    mat_vec_mod_m(A1p127, next_seed, next_seed, m1);
+    mat_vec_mod_m(A2p127, &next_seed[3], &next_seed[3], m2);
+}
+
+/* RngStream::U01 (src/RngStream.cpp:266-289). */
+double orc_rng_u01(orc_stream *g) {
+    long k;
+    double p1, p2, u;
+    p1 = a12 * g->Cg[1] - a13n * g->Cg[0];
+    k = (long)(p1 / m1);
+    p1 -= k * m1;
+    if (p1 < 0.0) p1 += m1;
+    g->Cg[0] = g->Cg[1];
+    g->Cg[1] = g->Cg[2];
+    g->Cg[2] = p1;
+    p2 = a21 * g->Cg[5] - a23n * g->Cg[3];
+    k = (long)(p2 / m2);
+    p2 -= k * m2;
+    if (p2 < 0.0) p2 += m2;
+    g->Cg[3] = g->Cg[4];
+    g->Cg[4] = g->Cg[5];
+    g->Cg[5] = p2;
+    u = ((p1 > p2) ? (p1 - p2) * norm_ : (p1 - p2 + m1) * norm_);
+    return (g->anti == 0) ? u : (1 - u);
+}
+
+static double orc_rng_u01d(orc_stream *g) {
+    double u = orc_rng_u01(g);
+    if (g->anti) {
+        u += (orc_rng_u01(g) - 1.0) * fact;
+        return (u < 0.0) ? u + 1.0 : u;
+    }
+    u += orc_rng_u01(g) * fact;
+    return (u < 1.0) ? u : (u - 1.0);
+}
+
+double orc_rng_randu01(orc_stream *g) { return g->inc_prec ? orc_rng_u01d(g) : orc_rng_u01(g); }
+
+int orc_rng_randint(orc_stream *g, int low, int high) {
+    return low + (int)((high - low + 1.0) * orc_rng_randu01(g));
+}
+
+void orc_rng_reset_start_stream(orc_stream *g) {
+    for (int i = 0; i < 6; ++i) g->Cg[i] = g->Bg[i] = g->Ig[i];
+}
+void orc_rng_reset_start_substream(orc_stream *g) {
+    for (int i = 0; i < 6; ++i) g->Cg[i] = g->Bg[i];
+}
+void orc_rng_reset_next_substream(orc_stream *g) {
+    mat_vec_mod_m(A1p76, g->Bg, g->Bg, m1);
+    mat_vec_mod_m(A2p76, &g->Bg[3], &g->Bg[3], m2);
+    for (int i = 0; i < 6; ++i) g->Cg[i] = g->Bg[i];
+}
+void orc_rng_set_antithetic(orc_stream *g, int a) { g->anti = a != 0; }
+void orc_rng_increased_precis(orc_stream *g, int p) { g->inc_prec = p != 0; }
+
+int orc_rng_set_seed(orc_stream *g, const unsigned long seed[6]) {
+    if (check_seed(seed)) return 0;
+    for (int i = 0; i < 6; ++i) g->Cg[i] = g->Bg[i] = g->Ig[i] = (double)seed[i];
+    return 1;
+}
+
+void orc_rng_get_state(const orc_stream *g, unsigned long seed[6]) {
+    for (int i = 0; i < 6; ++i) seed[i] = (unsigned long)g->Cg[i];
+}
+
+/* RngStream::AdvanceState (src/RngStream.cpp:408-435): n = 2^e + c (e>0), -2^-e + c (e<0), c (e=0). */
+void orc_rng_advance_state(orc_stream *g, long e, long c) {
+    double B1[3][3], C1[3][3], B2[3][3], C2[3][3];
+    if (e > 0) {
+        mat_two_pow_mod_m(A1p0, B1, m1, e);
+        mat_two_pow_mod_m(A2p0, B2, m2, e);
+    } else if (e < 0) {
+        mat_two_pow_mod_m(InvA1, B1, m1, -e);
+        mat_two_pow_mod_m(InvA2, B2, m2, -e);
+    }
+    if (c >= 0) {
+        mat_pow_mod_m(A1p0, C1, m1, c);
+        mat_pow_mod_m(A2p0, C2, m2, c);
+    } else {
+        mat_pow_mod_m(InvA1, C1, m1, -c);
+        mat_pow_mod_m(InvA2, C2, m2, -c);
+    }
+    if (e) {
+        mat_mat_mod_m((const double(*)[3])B1, (const double(*)[3])C1, C1, m1);
+        mat_mat_mod_m((const double(*)[3])B2, (const double(*)[3])C2, C2, m2);
+    }
+    mat_vec_mod_m((const double(*)[3])C1, g->Cg, g->Cg, m1);
+    mat_vec_mod_m((const double(*)[3])C2, &g->Cg[3], &g->Cg[3], m2);
+}
+
+/* World::init RNG.2 (src/World.cpp:299-305): six seeds from glibc srand/rand.  The int
+ * sum rand()+rand() wraps (two's complement), then converts to unsigned long. */
+void orc_package_seed_from_rng_seed(int rng_seed, unsigned long sixseed[6]) {
+    const unsigned long two_max[2] = {4294967087UL, 4294944443UL};
+    srand((unsigned)rng_seed);
+    for (int i = 0; i < 6; i++) {
+        unsigned r1 = (unsigned)rand();
+        unsigned r2 = (unsigned)rand();
+        int s = (int)(r1 + r2);
+        sixseed[i] = ((unsigned long)(long)s) % two_max[i / 3];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Element kernels (src/tetra_element_linear.cpp)                              */
+/* ------------------------------------------------------------------------- */
+
+typedef double m3[3][3];
+
+/* calculate_jacobian (:193-205): J[r][c] = pos(n_{r+1})[c] - pos(n_0)[c]. */
+static void elem_jacobian(const double *pos, const int *n, m3 J) {
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) J[r][c] = pos[3 * n[r + 1] + c] - pos[3 * n[0] + c];
+}
+
+/* calc_shape_function_derivatives_and_volume (:223-293).  Returns -1 on a sign change
+ * of det relative to *last_det, leaving dpsi half-built exactly as the reference does. */
+static int elem_dpsi_vol(m3 J, double dpsi[12], double *vol, double *last_det) {
+    double det;
+    dpsi[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+    dpsi[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    dpsi[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+    dpsi[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+    dpsi[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    dpsi[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+    dpsi[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+    dpsi[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    dpsi[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+    det = J[0][0] * dpsi[1] + J[1][0] * dpsi[2] + J[2][0] * dpsi[3];
+    if (*last_det * det < 0) return -1;
+    *last_det = det;
+    *vol = (1.0 / 6.0) * fabs(det);
+    det = 1.0 / det;
+    dpsi[1] *= det;
+    dpsi[2] *= det;
+    dpsi[3] *= det;
+    dpsi[5] *= det;
+    dpsi[6] *= det;
+    dpsi[7] *= det;
+    dpsi[9] *= det;
+    dpsi[10] *= det;
+    dpsi[11] *= det;
+    dpsi[0] = -(dpsi[1] + dpsi[2] + dpsi[3]);
+    dpsi[4] = -(dpsi[5] + dpsi[6] + dpsi[7]);
+    dpsi[8] = -(dpsi[9] + dpsi[10] + dpsi[11]);
+    return 0;
+}
+
+/* create_viscosity_matrix (:299-328) with BULK_VISCOUS_SUBMATRIX_* (header :55-116),
+ * calc_del2_matrix (:599-619) and add_diffusion_matrix (:621-638). */
+static void elem_viscosity(const double dpsi[12], double A, double B, double vol, double V[12][12]) {
+    /* diagonal 4x4 submatrices, upper triangle */
+    for (int I = 0; I < 12; I += 4) {
+        for (int a = 0; a < 4; a++)
+            for (int b = a; b < 4; b++) V[I + a][I + b] = dpsi[I + a] * dpsi[I + b] * (A + B);
+    }
+    /* off-diagonal submatrices (I,J) = (0,4), (0,8), (4,8) */
+    const int IJ[3][2] = {{0, 4}, {0, 8}, {4, 8}};
+    for (int s = 0; s < 3; s++) {
+        int I = IJ[s][0], J = IJ[s][1];
+        double K[4][4];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) K[a][b] = dpsi[I + a] * dpsi[J + b];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++)
+                V[I + a][J + b] = (a == b) ? (B + A) * K[a][a] : B * K[a][b] + A * K[b][a];
+    }
+    /* del2 (upper triangle) and diffusion term on the three diagonal blocks */
+    double del2[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = a; b < 4; b++)
+            del2[a][b] = (dpsi[a] * dpsi[b] + dpsi[4 + a] * dpsi[4 + b] + dpsi[8 + a] * dpsi[8 + b]);
+    for (int i = 0; i < 3; i++)
+        for (int a = 0; a < 4; a++)
+            for (int b = a; b < 4; b++) V[i * 4 + a][i * 4 + b] += del2[a][b] * A;
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j <= i; j++) V[j][i] *= vol;
+    for (int i = 1; i < 12; i++)
+        for (int j = 0; j < i; j++) V[i][j] = V[j][i];
+}
+
+/* add_shear_elastic_stress (:378-396) + mat3_mult_both_transposed / mat3_mult_transpose
+ * / mat3_scale (src/mat_vec_fns.cpp:93-126).  stress must be zero on entry. */
+static void elem_shear_stress(m3 J, const double Jinv0[9], double G, double vol_0, double vol, m3 F,
+                              m3 stress) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) F[i][j] = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) F[i][j] += J[k][i] * Jinv0[3 * j + k];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) stress[i][j] += F[i][k] * F[j][k];
+    double s = (G * vol_0 / vol);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) stress[i][j] *= s;
+    stress[0][0] -= G;
+    stress[1][1] -= G;
+    stress[2][2] -= G;
+}
+
+/* add_bulk_elastic_stress (:401-408). */
+static void elem_bulk_stress(double G, double E, double vol_0, double vol, m3 stress) {
+    double c_2 = E - G * 2.0 / 3.0;
+    double c = G * (1.0 - (vol_0 / vol)) + 0.5 * c_2 * ((vol / vol_0) - (vol_0 / vol));
+    stress[0][0] += c;
+    stress[1][1] += c;
+    stress[2][2] += c;
+}
+
+/* RAND(-.5,.5) (include/tetra_element_linear.h:118). */
+static double rand_half(orc_stream *g) { return ((-.5) + ((.5) - (-.5)) * (orc_rng_randu01(g))); }
+
+/* add_fluctuating_stress (:417-436): 7 draws, order bf, s00, s11, s22, s01, s02, s12. */
+static void elem_fluct_stress(double kT, double dt, double A, double B, double vol, orc_stream *g, m3 stress) {
+    double c = sqrt((24 * kT) / (vol * dt));
+    double bf = sqrt(B) * rand_half(g);
+    stress[0][0] += c * (sqrt(2 * A) * rand_half(g) + bf);
+    stress[1][1] += c * (sqrt(2 * A) * rand_half(g) + bf);
+    stress[2][2] += c * (sqrt(2 * A) * rand_half(g) + bf);
+    stress[0][1] += c * sqrt(A) * rand_half(g);
+    stress[0][2] += c * sqrt(A) * rand_half(g);
+    stress[1][2] += c * sqrt(A) * rand_half(g);
+    stress[1][0] = stress[0][1];
+    stress[2][0] = stress[0][2];
+    stress[2][1] = stress[1][2];
+}
+
+/* linearise_element (:567-591). */
+static void elem_linearise(double *pos, const int *n) {
+    const int ends[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    for (int m = 0; m < 6; m++)
+        for (int c = 0; c < 3; c++)
+            pos[3 * n[4 + m] + c] = .5 * (pos[3 * n[ends[m][0]] + c] + pos[3 * n[ends[m][1]] + c]);
+}
+
+/* mat3_invert (src/mat_vec_fns.cpp:148-177). */
+static void mat3_invert(m3 m, double inv[9]) {
+    double mi[3][3], det;
+    mi[0][0] = m[2][2] * m[1][1] - m[2][1] * m[1][2];
+    mi[0][1] = m[2][1] * m[0][2] - m[2][2] * m[0][1];
+    mi[0][2] = m[1][2] * m[0][1] - m[1][1] * m[0][2];
+    mi[1][0] = m[2][0] * m[1][2] - m[2][2] * m[1][0];
+    mi[1][1] = m[2][2] * m[0][0] - m[2][0] * m[0][2];
+    mi[1][2] = m[1][0] * m[0][2] - m[1][2] * m[0][0];
+    mi[2][0] = m[2][1] * m[1][0] - m[2][0] * m[1][1];
+    mi[2][1] = m[2][0] * m[0][1] - m[2][1] * m[0][0];
+    mi[2][2] = m[1][1] * m[0][0] - m[1][0] * m[0][1];
+    det = m[0][0] * mi[0][0] + m[1][0] * mi[0][1] + m[2][0] * mi[0][2];
+    det = 1.0 / det;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) inv[3 * i + j] = mi[i][j] * det;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Blob (N_t layout) + NoMassCGSolver restatement                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    int col;
+    int nsrc, cap;
+    const double **src;
+} orc_entry;
+
+typedef struct {
+    int n, cap;
+    orc_entry *e;
+} orc_row;
+
+typedef struct {
+    /* sizes */
+    int num_nodes, num_elements, num_pinned;
+    /* parameters in meso units (as held by SimulationParams after assign()) */
+    double dt, kT, epsilon2, stokes_visc;
+    int max_iterations_cg, calc_noise, calc_stokes;
+    /* nodes */
+    double *pos, *vel, *stokes_radius, *stokes_drag, *force;
+    int *is_pinned, *pinned;
+    /* elements */
+    int *topo;
+    double *rho, *A, *B, *G, *E, *mass;
+    double *J_inv_0, *vol_0, *vol, *last_det, *dpsi, *F_ij, *visc, *node_force, *stress_mag;
+    /* node -> element force contributions (elem*10 + slot) */
+    int *contrib_off, *contrib;
+    /* fixed-pattern CSR with source lists */
+    int num_rows, nnz;
+    int *key, *col, *src_off;
+    const double **src;
+    double *val, *diag_inv;
+    int *diag_idx;
+    double one;
+    /* CG work vectors (vector3 per node in the reference) */
+    double *r, *p, *z, *q, *f;
+    int last_cg_iterations, last_cg_status, last_inversions;
+} orc_blob;
+
+static void *xcalloc(size_t n, size_t s) {
+    void *p = calloc(n ? n : 1, s);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+
+/* SparsityPattern::register_contribution (src/SparsityPattern.cpp:57-80). */
+static void pattern_register(orc_row *rows, int i, int j, const double *loc) {
+    orc_row *R = &rows[i];
+    int k;
+    for (k = 0; k < R->n; k++) {
+        if (R->e[k].col == j) {
+            orc_entry *E = &R->e[k];
+            if (E->nsrc == E->cap) {
+                E->cap = E->cap ? 2 * E->cap : 4;
+                E->src = realloc(E->src, sizeof(*E->src) * E->cap);
+            }
+            E->src[E->nsrc++] = loc;
+            return;
+        }
+        if (R->e[k].col > j) break;
+    }
+    if (R->n == R->cap) {
+        R->cap = R->cap ? 2 * R->cap : 8;
+        R->e = realloc(R->e, sizeof(*R->e) * R->cap);
+    }
+    memmove(&R->e[k + 1], &R->e[k], sizeof(*R->e) * (R->n - k));
+    R->n++;
+    R->e[k].col = j;
+    R->e[k].cap = 4;
+    R->e[k].nsrc = 1;
+    R->e[k].src = malloc(sizeof(*R->e[k].src) * 4);
+    R->e[k].src[0] = loc;
+}
+
+static int pattern_check(const orc_row *rows, int i, int j) {
+    for (int k = 0; k < rows[i].n; k++)
+        if (rows[i].e[k].col == j) return 1;
+    return 0;
+}
+
+/* Create a blob.  pos: [3*num_nodes] meso units (already scaled); topo: [10*E];
+ * mat_si: [6*E] density, shear_visc, bulk_visc, shear_mod, bulk_mod, dielectric in SI;
+ * stokes_radius: [num_nodes] meso units; params in meso units. */
+orc_blob *orc_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
+                          const double *mat_si, const double *stokes_radius, int num_pinned,
+                          const int *pinned, double dt, double kT, double epsilon2, int max_iterations_cg,
+                          int calc_noise, int calc_stokes, double stokes_visc) {
+    /* mesoDimensions (include/dimensions.h:30-41) */
+    const double meso_volume = 4.913e-30, meso_mass = 1.994307387553024e-26;
+    const double meso_pressure = 8.4305832688784826e8, meso_time = 3.7302915560886126e-13;
+    const double pi = 3.141592653589793238462643383279502884197169399375105820974944592307816406286;
+
+    orc_blob *b = xcalloc(1, sizeof(orc_blob));
+    int N = num_nodes, E = num_elements;
+    b->num_nodes = N;
+    b->num_elements = E;
+    b->dt = dt;
+    b->kT = kT;
+    b->epsilon2 = epsilon2;
+    b->max_iterations_cg = max_iterations_cg;
+    b->calc_noise = calc_noise;
+    b->calc_stokes = calc_stokes;
+    b->stokes_visc = stokes_visc;
+    b->pos = xcalloc(3 * (size_t)N, sizeof(double));
+    b->vel = xcalloc(3 * (size_t)N, sizeof(double));
+    b->force = xcalloc(3 * (size_t)N, sizeof(double));
+    b->stokes_radius = xcalloc(N, sizeof(double));
+    b->stokes_drag = xcalloc(N, sizeof(double));
+    b->is_pinned = xcalloc(N, sizeof(int));
+    memcpy(b->pos, pos, sizeof(double) * 3 * (size_t)N);
+    memcpy(b->stokes_radius, stokes_radius, sizeof(double) * N);
+    b->num_pinned = num_pinned;
+    b->pinned = xcalloc(num_pinned, sizeof(int));
+    for (int i = 0; i < num_pinned; i++) {
+        b->pinned[i] = pinned[i];
+        b->is_pinned[pinned[i]] = 1;
+    }
+    b->topo = xcalloc(10 * (size_t)E, sizeof(int));
+    memcpy(b->topo, topo, sizeof(int) * 10 * (size_t)E);
+    b->rho = xcalloc(E, sizeof(double));
+    b->A = xcalloc(E, sizeof(double));
+    b->B = xcalloc(E, sizeof(double));
+    b->G = xcalloc(E, sizeof(double));
+    b->E = xcalloc(E, sizeof(double));
+    b->mass = xcalloc(E, sizeof(double));
+    /* Blob::load_material_params unit conversion (src/Blob.cpp:2861-2866); the
+     * (2/3)*shear_visc term is subtracted in SI units, as the reference does. */
+    for (int e = 0; e < E; e++) {
+        const double *m = &mat_si[6 * e];
+        b->rho[e] = m[0] * meso_volume / meso_mass;
+        b->A[e] = m[1] / (meso_pressure * meso_time);
+        b->B[e] = m[2] / (meso_pressure * meso_time) - (2.0 / 3.0) * m[1];
+        b->G[e] = m[3] / meso_pressure;
+        b->E[e] = m[4] / meso_pressure;
+    }
+    b->J_inv_0 = xcalloc(9 * (size_t)E, sizeof(double));
+    b->vol_0 = xcalloc(E, sizeof(double));
+    b->vol = xcalloc(E, sizeof(double));
+    b->last_det = xcalloc(E, sizeof(double));
+    b->dpsi = xcalloc(12 * (size_t)E, sizeof(double));
+    b->F_ij = xcalloc(9 * (size_t)E, sizeof(double));
+    b->visc = xcalloc(144 * (size_t)E, sizeof(double));
+    b->node_force = xcalloc(30 * (size_t)E, sizeof(double));
+    b->stress_mag = xcalloc(E, sizeof(double));
+
+    /* Blob::init order (src/Blob.cpp:335-412): linearise, rest state, connectivity,
+     * solver init, then stokes drag. */
+    for (int e = 0; e < E; e++) elem_linearise(b->pos, &b->topo[10 * e]);
+
+    /* calc_rest_state_info (src/Blob.cpp:3760-3822) */
+    for (int e = 0; e < E; e++) {
+        m3 J;
+        elem_jacobian(b->pos, &b->topo[10 * e], J);
+        mat3_invert(J, &b->J_inv_0[9 * e]);
+        elem_dpsi_vol(J, &b->dpsi[12 * e], &b->vol[e], &b->last_det[e]);
+        b->vol_0[e] = b->vol[e];
+        b->mass[e] = b->vol_0[e] * b->rho[e];
+    }
+
+    /* calculate_node_element_connectivity (src/Blob.cpp:3974-4019) */
+    b->contrib_off = xcalloc((size_t)N + 1, sizeof(int));
+    for (int e = 0; e < E; e++)
+        for (int j = 0; j < 10; j++) b->contrib_off[b->topo[10 * e + j] + 1]++;
+    for (int i = 0; i < N; i++) b->contrib_off[i + 1] += b->contrib_off[i];
+    b->contrib = xcalloc((size_t)b->contrib_off[N], sizeof(int));
+    {
+        int *cnt = xcalloc(N, sizeof(int));
+        for (int e = 0; e < E; e++)
+            for (int j = 0; j < 10; j++) {
+                int ni = b->topo[10 * e + j];
+                b->contrib[b->contrib_off[ni] + cnt[ni]++] = 10 * e + j;
+            }
+        free(cnt);
+    }
+
+    /* NoMassCGSolver::init (src/NoMassCGSolver.cpp:63-162) */
+    b->num_rows = 3 * N;
+    b->one = 1;
+    orc_row *rows = xcalloc(b->num_rows, sizeof(orc_row));
+    for (int e = 0; e < E; e++) {
+        m3 J;
+        elem_jacobian(b->pos, &b->topo[10 * e], J);
+        elem_dpsi_vol(J, &b->dpsi[12 * e], &b->vol[e], &b->last_det[e]);
+        elem_viscosity(&b->dpsi[12 * e], b->A[e], b->B[e], b->vol[e], (double(*)[12]) & b->visc[144 * e]);
+        for (int ni = 0; ni < 10; ++ni)
+            for (int nj = 0; nj < 10; ++nj) {
+                int ni_index = b->topo[10 * e + ni], nj_index = b->topo[10 * e + nj];
+                int ni_row = 3 * ni_index, nj_row = 3 * nj_index;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        if (b->is_pinned[ni_index] == 0 && b->is_pinned[nj_index] == 0 && ni < 4 && nj < 4) {
+                            pattern_register(rows, ni_row + i, nj_row + j,
+                                             &b->visc[144 * e + 12 * (ni + 4 * i) + (nj + 4 * j)]);
+                        } else if (ni == nj && i == j) {
+                            if (!pattern_check(rows, ni_row + i, nj_row + j))
+                                pattern_register(rows, ni_row + i, nj_row + j, &b->one);
+                        }
+                    }
+            }
+    }
+    if (calc_stokes == 1)
+        for (int ni = 0; ni < N; ++ni)
+            if (b->is_pinned[ni] == 0)
+                for (int nj = 0; nj < 3; ++nj) pattern_register(rows, 3 * ni + nj, 3 * ni + nj, &b->stokes_drag[ni]);
+
+    /* SparsityPattern::create_sparse_matrix (src/SparsityPattern.cpp:97-143) */
+    int nnz = 0;
+    size_t nsrc = 0;
+    for (int i = 0; i < b->num_rows; i++) {
+        nnz += rows[i].n;
+        for (int k = 0; k < rows[i].n; k++) nsrc += rows[i].e[k].nsrc;
+    }
+    b->nnz = nnz;
+    b->key = xcalloc((size_t)b->num_rows + 1, sizeof(int));
+    b->col = xcalloc(nnz, sizeof(int));
+    b->val = xcalloc(nnz, sizeof(double));
+    b->src_off = xcalloc((size_t)nnz + 1, sizeof(int));
+    b->src = xcalloc(nsrc, sizeof(double *));
+    b->diag_idx = xcalloc(b->num_rows, sizeof(int));
+    int pos_ = 0;
+    size_t sp = 0;
+    for (int i = 0; i < b->num_rows; i++) {
+        b->key[i] = pos_;
+        for (int k = 0; k < rows[i].n; k++) {
+            b->col[pos_] = rows[i].e[k].col;
+            b->src_off[pos_] = (int)sp;
+            for (int s = 0; s < rows[i].e[k].nsrc; s++) b->src[sp++] = rows[i].e[k].src[s];
+            if (b->col[pos_] == i) b->diag_idx[i] = pos_;
+            free(rows[i].e[k].src);
+            pos_++;
+        }
+        free(rows[i].e);
+    }
+    b->key[b->num_rows] = pos_;
+    b->src_off[nnz] = (int)sp;
+    free(rows);
+    b->diag_inv = xcalloc(b->num_rows, sizeof(double));
+    b->r = xcalloc(3 * (size_t)N, sizeof(double));
+    b->p = xcalloc(3 * (size_t)N, sizeof(double));
+    b->z = xcalloc(3 * (size_t)N, sizeof(double));
+    b->q = xcalloc(3 * (size_t)N, sizeof(double));
+    b->f = xcalloc(3 * (size_t)N, sizeof(double));
+
+    /* stokes drag (src/Blob.cpp:408-412) */
+    if (calc_stokes == 1)
+        for (int i = 0; i < N; ++i) b->stokes_drag[i] = 6.0 * pi * stokes_visc * b->stokes_radius[i];
+    return b;
+}
+
+void orc_blob_destroy(orc_blob *b) {
+    if (!b) return;
+    void *ptrs[] = {b->pos, b->vel, b->stokes_radius, b->stokes_drag, b->force, b->is_pinned, b->pinned,
+                    b->topo, b->rho, b->A, b->B, b->G, b->E, b->mass, b->J_inv_0, b->vol_0, b->vol,
+                    b->last_det, b->dpsi, b->F_ij, b->visc, b->node_force, b->stress_mag, b->contrib_off,
+                    b->contrib, b->key, b->col, b->src_off, (void *)b->src, b->val, b->diag_inv, b->diag_idx,
+                    b->r, b->p, b->z, b->q, b->f};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    free(b);
+}
+
+/* Overwrite all N_t node positions (the restart frame of src/World.cpp:647-763). */
+void orc_blob_set_positions(orc_blob *b, const double *pos) {
+    memcpy(b->pos, pos, sizeof(double) * 3 * (size_t)b->num_nodes);
+}
+
+/* Blob::zero_force + set_forces_to_zero (src/World.cpp:1844-1909). */
+static void blob_zero_forces(orc_blob *b) {
+    memset(b->node_force, 0, sizeof(double) * 30 * (size_t)b->num_elements);
+    memset(b->force, 0, sizeof(double) * 3 * (size_t)b->num_nodes);
+}
+
+/* Blob::update_internal_forces (src/Blob.cpp:638-735), single thread. Returns #inversions. */
+int orc_blob_update_internal_forces(orc_blob *b, orc_stream *g) {
+    int inversions = 0;
+    for (int n = 0; n < b->num_elements; n++) {
+        m3 J, stress, F;
+        double du[12];
+        const int *t = &b->topo[10 * n];
+        double *dpsi = &b->dpsi[12 * n];
+        elem_jacobian(b->pos, t, J);
+        if (elem_dpsi_vol(J, dpsi, &b->vol[n], &b->last_det[n]) != 0) inversions++;
+        elem_viscosity(dpsi, b->A[n], b->B[n], b->vol[n], (double(*)[12]) & b->visc[144 * n]);
+        memset(stress, 0, sizeof(stress));
+        elem_shear_stress(J, &b->J_inv_0[9 * n], b->G[n], b->vol_0[n], b->vol[n], F, stress);
+        memcpy(&b->F_ij[9 * n], F, sizeof(F));
+        elem_bulk_stress(b->G[n], b->E[n], b->vol_0[n], b->vol[n], stress);
+        if (b->calc_noise == 1) elem_fluct_stress(b->kT, b->dt, b->A[n], b->B[n], b->vol[n], g, stress);
+        /* mat3_double_contraction_symmetric (src/mat_vec_fns.cpp:131-134) */
+        b->stress_mag[n] = sqrt(stress[0][0] * stress[0][0] + stress[1][1] * stress[1][1] +
+                                stress[2][2] * stress[2][2] +
+                                2 * (stress[0][1] * stress[0][1] + stress[0][2] * stress[0][2] +
+                                     stress[1][2] * stress[1][2]));
+        for (int k = 0; k < 12; k++) du[k] = 0;
+        /* apply_stress_tensor (:441-449) */
+        double vol = b->vol[n];
+        for (int i = 0; i < 3; i++)
+            for (int a = 0; a < 4; a++)
+                du[4 * i + a] +=
+                    vol * (dpsi[a] * stress[i][0] + dpsi[4 + a] * stress[i][1] + dpsi[8 + a] * stress[i][2]);
+        /* add_element_force_vector (:474-490): node_force[a][i] -= du[4i+a] */
+        double *nf = &b->node_force[30 * n];
+        for (int i = 0; i < 3; i++)
+            for (int a = 0; a < 4; a++) nf[3 * a + i] -= du[4 * i + a];
+    }
+    b->last_inversions = inversions;
+    return inversions;
+}
+
+/* SparseMatrixFixedPattern::build + calc_inverse_diagonal. */
+static void blob_build_matrix(orc_blob *b) {
+    for (int k = 0; k < b->nnz; k++) {
+        double sum = 0.0;
+        for (int s = b->src_off[k]; s < b->src_off[k + 1]; s++) sum += *(b->src[s]);
+        b->val[k] = sum;
+    }
+    for (int i = 0; i < b->num_rows; i++) b->diag_inv[i] = 1.0 / b->val[b->diag_idx[i]];
+}
+
+/* SparseMatrixFixedPattern::apply(vector3*, vector3*) (src/SparseMatrixFixedPattern.cpp:148-191). */
+static void blob_spmv(const orc_blob *b, const double *in, double *out) {
+    for (int i = 0; i < b->num_rows; i++) {
+        double w = 0.0;
+        for (int j = b->key[i]; j < b->key[i + 1]; ++j) w += b->val[j] * in[b->col[j]];
+        out[i] = w;
+    }
+}
+
+/* NoMassCGSolver::solve (src/NoMassCGSolver.cpp:165-197) with its vector kernels (:205-292)
+ * and vec3_add_to_scaled / vec3_scale_and_add (src/mat_vec_fns.cpp:250-275). */
+static int blob_solve(orc_blob *b, double *x) {
+    int N = b->num_nodes;
+    double *r = b->r, *p = b->p, *z = b->z, *q = b->q, *f = b->f, *D = b->diag_inv;
+    blob_build_matrix(b);
+    double delta_new = 0, delta_old, pTq, alpha;
+    for (int i = 0; i < N; i++) {
+        for (int c = 0; c < 3; c++) {
+            r[3 * i + c] = x[3 * i + c];
+            f[3 * i + c] = x[3 * i + c];
+            x[3 * i + c] = 0;
+            z[3 * i + c] = D[3 * i + c] * r[3 * i + c];
+            p[3 * i + c] = z[3 * i + c];
+        }
+        delta_new += r[3 * i] * z[3 * i] + r[3 * i + 1] * z[3 * i + 1] + r[3 * i + 2] * z[3 * i + 2];
+    }
+    for (int it = 0; it < b->max_iterations_cg; it++) {
+        blob_spmv(b, p, q);
+        pTq = 0;
+        for (int i = 0; i < N; ++i) pTq += p[3 * i] * q[3 * i] + p[3 * i + 1] * q[3 * i + 1] + p[3 * i + 2] * q[3 * i + 2];
+        alpha = delta_new / pTq;
+        for (int i = 0; i < 3 * N; i++) x[i] += alpha * p[i];
+        for (int i = 0; i < 3 * N; i++) r[i] += -alpha * q[i];
+        double r2 = 0, f2 = 0, res;
+        for (int i = 0; i < N; i++) {
+            r2 += r[3 * i] * r[3 * i] + r[3 * i + 1] * r[3 * i + 1] + r[3 * i + 2] * r[3 * i + 2];
+            f2 += f[3 * i] * f[3 * i] + f[3 * i + 1] * f[3 * i + 1] + f[3 * i + 2] * f[3 * i + 2];
+        }
+        res = (f2 == 0.0) ? 0.0 : r2 / f2;
+        if (res < b->epsilon2) {
+            b->last_cg_iterations = it + 1;
+            b->last_cg_status = 0;
+            return 0;
+        }
+        delta_old = delta_new;
+        delta_new = 0;
+        for (int i = 0; i < N; i++) {
+            z[3 * i] = D[3 * i] * r[3 * i];
+            z[3 * i + 1] = D[3 * i + 1] * r[3 * i + 1];
+            z[3 * i + 2] = D[3 * i + 2] * r[3 * i + 2];
+            delta_new += r[3 * i] * z[3 * i] + r[3 * i + 1] * z[3 * i + 1] + r[3 * i + 2] * z[3 * i + 2];
+        }
+        double beta = delta_new / delta_old;
+        for (int i = 0; i < 3 * N; i++) p[i] = beta * p[i] + z[i];
+    }
+    b->last_cg_iterations = b->max_iterations_cg;
+    b->last_cg_status = -1;
+    return -1;
+}
+
+/* Blob::aggregate_forces_and_solve, nomass branch (src/Blob.cpp:3826-3932).  Leaves the
+ * RHS in b->f (copied by the solver) and the velocity solution in b->force. */
+static int blob_aggregate_forces(orc_blob *b, orc_stream *g) {
+    int N = b->num_nodes;
+    for (int n = 0; n < N; n++)
+        for (int m = b->contrib_off[n]; m < b->contrib_off[n + 1]; m++) {
+            const double *c = &b->node_force[3 * b->contrib[m]];
+            b->force[3 * n] += c[0];
+            b->force[3 * n + 1] += c[1];
+            b->force[3 * n + 2] += c[2];
+        }
+    if (b->calc_stokes == 1 && b->calc_noise == 1)
+        for (int i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++)
+                b->force[3 * i + c] -= rand_half(g) * sqrt((24 * b->kT * b->stokes_drag[i]) / (b->dt));
+    /* Blob::linearise_force (src/Blob.cpp:1066-1085) */
+    for (int e = 0; e < b->num_elements; e++) {
+        const int *nI = &b->topo[10 * e];
+        double *F = b->force;
+        for (int j = 0; j < 3; j++) {
+            F[3 * nI[0] + j] += 0.5 * (F[3 * nI[4] + j] + F[3 * nI[5] + j] + F[3 * nI[6] + j]);
+            F[3 * nI[1] + j] += 0.5 * (F[3 * nI[4] + j] + F[3 * nI[7] + j] + F[3 * nI[8] + j]);
+            F[3 * nI[2] + j] += 0.5 * (F[3 * nI[5] + j] + F[3 * nI[7] + j] + F[3 * nI[9] + j]);
+            F[3 * nI[3] + j] += 0.5 * (F[3 * nI[6] + j] + F[3 * nI[8] + j] + F[3 * nI[9] + j]);
+        }
+        for (int j = 4; j < 10; j++) F[3 * nI[j]] = F[3 * nI[j] + 1] = F[3 * nI[j] + 2] = 0.;
+    }
+    for (int n = 0; n < b->num_pinned; n++) {
+        int pn = b->pinned[n];
+        b->force[3 * pn] = b->force[3 * pn + 1] = b->force[3 * pn + 2] = 0;
+    }
+    return 0;
+}
+
+/* Blob::update_positions (src/Blob.cpp:737-764) with euler_integrate (:3937-3953). */
+int orc_blob_update_positions(orc_blob *b, orc_stream *g) {
+    blob_aggregate_forces(b, g);
+    int st = blob_solve(b, b->force);
+    if (st != 0) return st; /* the reference skips integration on a solver error */
+    for (int i = 0; i < b->num_nodes; i++)
+        for (int c = 0; c < 3; c++) {
+            b->vel[3 * i + c] = b->force[3 * i + c];
+            b->pos[3 * i + c] += b->force[3 * i + c] * b->dt;
+        }
+    for (int e = 0; e < b->num_elements; e++) elem_linearise(b->pos, &b->topo[10 * e]);
+    return 0;
+}
+
+/* One World::run iteration over nblobs blobs sharing one stream (src/World.cpp:1844-2043):
+ * zero forces, element phase for every blob, then position phase for every blob.
+ * Returns the total inversion count (the reference aborts the run if > 0), or -1 if a
+ * solve failed. */
+int orc_world_step(orc_blob **blobs, int nblobs, orc_stream *g) {
+    int inv = 0, err = 0;
+    for (int i = 0; i < nblobs; i++) blob_zero_forces(blobs[i]);
+    for (int i = 0; i < nblobs; i++) inv += orc_blob_update_internal_forces(blobs[i], g);
+    if (inv > 0) return inv;
+    for (int i = 0; i < nblobs; i++)
+        if (orc_blob_update_positions(blobs[i], g) != 0) err = 1;
+    return err ? -1 : 0;
+}
+
+/* ---- element/assembly/solver probes for tests ---------------------------- */
+
+/* Element phase + aggregation only (no solve): fills node_force, dpsi, vol, visc and
+ * leaves the linear-system RHS in out_rhs [3*N_t]. */
+int orc_blob_forces_only(orc_blob *b, orc_stream *g, double *out_rhs) {
+    blob_zero_forces(b);
+    int inv = orc_blob_update_internal_forces(b, g);
+    blob_aggregate_forces(b, g);
+    memcpy(out_rhs, b->force, sizeof(double) * 3 * (size_t)b->num_nodes);
+    return inv;
+}
+
+/* NoMassCGSolver::solve on a caller RHS (in/out) with the element matrices currently held. */
+int orc_blob_solve(orc_blob *b, double *x) { return blob_solve(b, x); }
+
+/* V * in with the matrix as last built (call after a solve or orc_blob_build). */
+void orc_blob_build(orc_blob *b) { blob_build_matrix(b); }
+void orc_blob_apply_matrix(const orc_blob *b, const double *in, double *out) { blob_spmv(b, in, out); }
+
+int orc_blob_nnz(const orc_blob *b) { return b->nnz; }
+int orc_blob_num_rows(const orc_blob *b) { return b->num_rows; }
+void orc_blob_get_csr(const orc_blob *b, int *key, int *col, double *val) {
+    memcpy(key, b->key, sizeof(int) * ((size_t)b->num_rows + 1));
+    memcpy(col, b->col, sizeof(int) * (size_t)b->nnz);
+    memcpy(val, b->val, sizeof(double) * (size_t)b->nnz);
+}
+int orc_blob_last_cg_iterations(const orc_blob *b) { return b->last_cg_iterations; }
+
+/* Copy out arrays by name for golden comparisons. */
+void orc_blob_get(const orc_blob *b, const char *what, double *out) {
+    size_t N = b->num_nodes, E = b->num_elements;
+    if (!strcmp(what, "pos")) memcpy(out, b->pos, sizeof(double) * 3 * N);
+    else if (!strcmp(what, "vel")) memcpy(out, b->vel, sizeof(double) * 3 * N);
+    else if (!strcmp(what, "force")) memcpy(out, b->force, sizeof(double) * 3 * N);
+    else if (!strcmp(what, "rhs")) memcpy(out, b->f, sizeof(double) * 3 * N);
+    else if (!strcmp(what, "dpsi")) memcpy(out, b->dpsi, sizeof(double) * 12 * E);
+    else if (!strcmp(what, "vol")) memcpy(out, b->vol, sizeof(double) * E);
+    else if (!strcmp(what, "vol_0")) memcpy(out, b->vol_0, sizeof(double) * E);
+    else if (!strcmp(what, "J_inv_0")) memcpy(out, b->J_inv_0, sizeof(double) * 9 * E);
+    else if (!strcmp(what, "F_ij")) memcpy(out, b->F_ij, sizeof(double) * 9 * E);
+    else if (!strcmp(what, "visc")) memcpy(out, b->visc, sizeof(double) * 144 * E);
+    else if (!strcmp(what, "node_force")) memcpy(out, b->node_force, sizeof(double) * 30 * E);
+    else if (!strcmp(what, "stress_mag")) memcpy(out, b->stress_mag, sizeof(double) * E);
+    else if (!strcmp(what, "stokes_drag")) memcpy(out, b->stokes_drag, sizeof(double) * N);
+    else if (!strcmp(what, "last_det")) memcpy(out, b->last_det, sizeof(double) * E);
+    else if (!strcmp(what, "material")) {
+        for (size_t e = 0; e < E; e++) {
+            out[5 * e] = b->rho[e];
+            out[5 * e + 1] = b->A[e];
+            out[5 * e + 2] = b->B[e];
+            out[5 * e + 3] = b->G[e];
+            out[5 * e + 4] = b->E[e];
+        }
+    }
+}
+
+/* Strain energy of Blob::make_measurements (src/Blob.cpp:1424-1432), from F_ij of the
+ * last element update: 1/2 sum vol0 [G(F:F - 3) + 1/2 C (J^2 - 1) - (C + 2G) ln J]. */
+double orc_blob_strain_energy(const orc_blob *b) {
+    double se = 0.0;
+    for (int n = 0; n < b->num_elements; n++) {
+        const double *F = &b->F_ij[9 * n];
+        /* mat3_double_contraction (src/mat_vec_fns.cpp:139-143) order */
+        double dc = F[0] * F[0] + F[4] * F[4] + F[8] * F[8] + F[1] * F[1] + F[3] * F[3] + F[2] * F[2] +
+                    F[6] * F[6] + F[5] * F[5] + F[7] * F[7];
+        double C = b->E[n] - (2.0 / 3.0) * b->G[n];
+        double t = b->vol[n] / b->vol_0[n];
+        se += b->vol_0[n] * (b->G[n] * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * b->G[n]) * log(t));
+    }
+    return 0.5 * se;
+}

@@ -1,0 +1,293 @@
+"""ctypes front end for the CPU checkers (test infrastructure only).
+
+`Oracle` loads oracle/liboracle.so (the FP64 restatement, ffea_oracle.c);
+`Reference` loads oracle/_ref/libffea_ref.so (the reference's own compiled objects,
+see ref_harness.cpp).  Both expose the same Blob / stream API so a test can run one
+scenario through each and compare.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libffea_ref.so")
+REF_FAST_SO = os.path.join(HERE, "_ref", "libffea_ref_fast.so")
+
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_ulp = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def build(with_ref: bool = True) -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference sources are present)."""
+    targets = ["all"] if with_ref else ["oracle"]
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
+
+
+class _Stream(C.Structure):
+    _fields_ = [("Cg", C.c_double * 6), ("Bg", C.c_double * 6), ("Ig", C.c_double * 6),
+                ("anti", C.c_int), ("inc_prec", C.c_int)]
+
+
+class _Lib:
+    prefix = ""
+    path = ""
+
+    def __init__(self, path=None):
+        path = path or self.path
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
+        self.lib = C.CDLL(path)
+        p = self.prefix
+        L = self.lib
+        f = getattr(L, p + "blob_create")
+        f.restype = C.c_void_p
+        f.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, _ip, C.c_double, C.c_double,
+                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_double]
+        getattr(L, p + "blob_destroy").argtypes = [C.c_void_p]
+        getattr(L, p + "blob_set_positions").argtypes = [C.c_void_p, _dp]
+        f = getattr(L, p + "world_step")
+        f.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_void_p]
+        f.restype = C.c_int
+        f = getattr(L, p + "blob_forces_only")
+        f.argtypes = [C.c_void_p, C.c_void_p, _dp]
+        f.restype = C.c_int
+        f = getattr(L, p + "blob_solve")
+        f.argtypes = [C.c_void_p, _dp]
+        f.restype = C.c_int
+        getattr(L, p + "blob_get").argtypes = [C.c_void_p, C.c_char_p, _dp]
+        getattr(L, p + "blob_build").argtypes = [C.c_void_p]
+        for n in ("blob_nnz", "blob_num_rows"):
+            getattr(L, p + n).argtypes = [C.c_void_p]
+            getattr(L, p + n).restype = C.c_int
+        getattr(L, p + "blob_get_csr").argtypes = [C.c_void_p, _ip, _ip, _dp]
+        getattr(L, p + "blob_apply_matrix").argtypes = [C.c_void_p, _dp, _dp]
+        getattr(L, p + "package_seed_from_rng_seed").argtypes = [C.c_int, _ulp]
+        getattr(L, p + "rng_set_package_seed").argtypes = [_ulp]
+        getattr(L, p + "rng_set_package_seed").restype = C.c_int
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    # ---- RNG ---------------------------------------------------------------
+    def package_seed(self, rng_seed: int) -> np.ndarray:
+        out = np.zeros(6, np.uint64)
+        self.fn("package_seed_from_rng_seed")(int(rng_seed), out)
+        return out
+
+    def set_package_seed(self, seed) -> bool:
+        return bool(self.fn("rng_set_package_seed")(np.ascontiguousarray(seed, np.uint64)))
+
+    # ---- Blob --------------------------------------------------------------
+    def blob(self, mesh, params, scale: float, pos=None) -> "Blob":
+        return Blob(self, mesh, params, scale, pos)
+
+
+class Blob:
+    def __init__(self, lib: _Lib, mesh, params, scale: float, pos=None):
+        self.lib = lib
+        self.N = mesh.num_nodes
+        self.E = mesh.num_elements
+        rest = np.ascontiguousarray(mesh.pos * scale, np.float64).ravel()
+        topo = np.ascontiguousarray(mesh.topo, np.int32).ravel()
+        mat = np.ascontiguousarray(mesh.material, np.float64).ravel()
+        rad = np.ascontiguousarray(mesh.stokes_radius * scale, np.float64)
+        pinned = np.ascontiguousarray(mesh.pinned, np.int32)
+        self.h = lib.fn("blob_create")(self.N, self.E, rest, topo, mat, rad, int(pinned.size),
+                                       pinned if pinned.size else np.zeros(1, np.int32),
+                                       params.dt, params.kT, params.epsilon2,
+                                       params.max_iterations_cg, params.calc_noise,
+                                       params.calc_stokes, params.stokes_visc)
+        if pos is not None:
+            self.set_positions(np.asarray(pos) * scale)
+
+    def close(self):
+        if self.h:
+            self.lib.fn("blob_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_positions(self, pos_meso):
+        self.lib.fn("blob_set_positions")(self.h, np.ascontiguousarray(pos_meso, np.float64).ravel())
+
+    def get(self, what: str) -> np.ndarray:
+        sizes = {"pos": 3 * self.N, "vel": 3 * self.N, "force": 3 * self.N, "rhs": 3 * self.N,
+                 "dpsi": 12 * self.E, "vol": self.E, "vol_0": self.E, "J_inv_0": 9 * self.E,
+                 "F_ij": 9 * self.E, "visc": 144 * self.E, "node_force": 30 * self.E,
+                 "stress_mag": self.E, "stokes_drag": self.N, "material": 5 * self.E,
+                 "last_det": self.E}
+        out = np.zeros(sizes[what], np.float64)
+        self.lib.fn("blob_get")(self.h, what.encode(), out)
+        return out
+
+    def forces_only(self, stream) -> tuple[int, np.ndarray]:
+        out = np.zeros(3 * self.N, np.float64)
+        inv = self.lib.fn("blob_forces_only")(self.h, stream.ptr, out)
+        return inv, out
+
+    def solve(self, rhs) -> tuple[int, np.ndarray]:
+        x = np.ascontiguousarray(rhs, np.float64).copy().ravel()
+        st = self.lib.fn("blob_solve")(self.h, x)
+        return st, x
+
+    def csr(self):
+        nr = self.lib.fn("blob_num_rows")(self.h)
+        nnz = self.lib.fn("blob_nnz")(self.h)
+        key = np.zeros(nr + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz, np.float64)
+        self.lib.fn("blob_get_csr")(self.h, key, col, val)
+        return key, col, val
+
+    def build_matrix(self):
+        self.lib.fn("blob_build")(self.h)
+
+    def apply_matrix(self, x) -> np.ndarray:
+        out = np.zeros(3 * self.N, np.float64)
+        self.lib.fn("blob_apply_matrix")(self.h, np.ascontiguousarray(x, np.float64).ravel(), out)
+        return out
+
+
+def world_step(lib: _Lib, blobs, stream) -> int:
+    arr = (C.c_void_p * len(blobs))(*[b.h for b in blobs])
+    return lib.fn("world_step")(arr, len(blobs), stream.ptr)
+
+
+class Oracle(_Lib):
+    prefix = "orc_"
+    path = ORACLE_SO
+
+    def __init__(self, path=None):
+        super().__init__(path)
+        L = self.lib
+        for n in ("rng_u01", "rng_randu01"):
+            getattr(L, "orc_" + n).argtypes = [C.c_void_p]
+            getattr(L, "orc_" + n).restype = C.c_double
+        L.orc_rng_randint.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_rng_randint.restype = C.c_int
+        L.orc_rng_advance_state.argtypes = [C.c_void_p, C.c_long, C.c_long]
+        for n in ("rng_stream_init", "rng_reset_start_stream", "rng_reset_start_substream",
+                  "rng_reset_next_substream"):
+            getattr(L, "orc_" + n).argtypes = [C.c_void_p]
+        L.orc_rng_set_antithetic.argtypes = [C.c_void_p, C.c_int]
+        L.orc_rng_increased_precis.argtypes = [C.c_void_p, C.c_int]
+        L.orc_rng_get_state.argtypes = [C.c_void_p, _ulp]
+        L.orc_rng_set_seed.argtypes = [C.c_void_p, _ulp]
+        L.orc_rng_set_seed.restype = C.c_int
+        L.orc_blob_strain_energy.argtypes = [C.c_void_p]
+        L.orc_blob_strain_energy.restype = C.c_double
+        L.orc_blob_last_cg_iterations.argtypes = [C.c_void_p]
+        L.orc_blob_last_cg_iterations.restype = C.c_int
+
+    def stream(self) -> "OracleStream":
+        return OracleStream(self)
+
+    def strain_energy(self, blob: Blob) -> float:
+        return self.lib.orc_blob_strain_energy(blob.h)
+
+    def cg_iterations(self, blob: Blob) -> int:
+        return self.lib.orc_blob_last_cg_iterations(blob.h)
+
+
+class OracleStream:
+    """An RngStream of the restatement (constructor semantics: takes the package seed)."""
+
+    def __init__(self, o: Oracle):
+        self.o = o
+        self.s = _Stream()
+        self.ptr = C.cast(C.pointer(self.s), C.c_void_p)
+        o.lib.orc_rng_stream_init(self.ptr)
+
+    def u01(self):
+        return self.o.lib.orc_rng_u01(self.ptr)
+
+    def randu01(self):
+        return self.o.lib.orc_rng_randu01(self.ptr)
+
+    def randint(self, lo, hi):
+        return self.o.lib.orc_rng_randint(self.ptr, lo, hi)
+
+    def advance(self, e, c):
+        self.o.lib.orc_rng_advance_state(self.ptr, e, c)
+
+    def reset_start_stream(self):
+        self.o.lib.orc_rng_reset_start_stream(self.ptr)
+
+    def reset_start_substream(self):
+        self.o.lib.orc_rng_reset_start_substream(self.ptr)
+
+    def reset_next_substream(self):
+        self.o.lib.orc_rng_reset_next_substream(self.ptr)
+
+    def set_antithetic(self, a):
+        self.o.lib.orc_rng_set_antithetic(self.ptr, int(a))
+
+    def increased_precis(self, p):
+        self.o.lib.orc_rng_increased_precis(self.ptr, int(p))
+
+    def state(self) -> np.ndarray:
+        out = np.zeros(6, np.uint64)
+        self.o.lib.orc_rng_get_state(self.ptr, out)
+        return out
+
+    def set_seed(self, seed) -> bool:
+        return bool(self.o.lib.orc_rng_set_seed(self.ptr, np.ascontiguousarray(seed, np.uint64)))
+
+
+class Reference(_Lib):
+    prefix = "ref_"
+    path = REF_SO
+
+    def __init__(self, path=None):
+        super().__init__(path)
+        L = self.lib
+        L.ref_rng_new.restype = C.c_void_p
+        L.ref_rng_delete.argtypes = [C.c_void_p]
+        L.ref_rng_randu01.argtypes = [C.c_void_p]
+        L.ref_rng_randu01.restype = C.c_double
+        L.ref_rng_get_state.argtypes = [C.c_void_p, _ulp]
+        L.ref_rng_set_seed.argtypes = [C.c_void_p, _ulp]
+        L.ref_rng_advance_state.argtypes = [C.c_void_p, C.c_long, C.c_long]
+
+    def stream(self) -> "RefStream":
+        return RefStream(self)
+
+
+class RefStream:
+    def __init__(self, r: Reference):
+        self.r = r
+        self.ptr = C.c_void_p(r.lib.ref_rng_new())
+
+    def __del__(self):
+        try:
+            self.r.lib.ref_rng_delete(self.ptr)
+        except Exception:
+            pass
+
+    def randu01(self):
+        return self.r.lib.ref_rng_randu01(self.ptr)
+
+    def advance(self, e, c):
+        self.r.lib.ref_rng_advance_state(self.ptr, e, c)
+
+    def state(self) -> np.ndarray:
+        out = np.zeros(6, np.uint64)
+        self.r.lib.ref_rng_get_state(self.ptr, out)
+        return out
+
+    def set_seed(self, seed) -> bool:
+        return bool(self.r.lib.ref_rng_set_seed(self.ptr, np.ascontiguousarray(seed, np.uint64)))
+
+
+def have_reference() -> bool:
+    return os.path.exists(REF_SO)

@@ -1,0 +1,367 @@
+/*
+ * ref_harness.cpp -- drives the reference's OWN compiled hot-path objects
+ * (tetra_element_linear, NoMassCGSolver, SparsityPattern, SparseMatrixFixedPattern,
+ * SparseMatrixTypes, mat_vec_fns, RngStream, mesh_node) from /root/reference/src.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the checker).  Built by oracle/Makefile
+ * into oracle/_ref/libffea_ref.so; nothing in the product links it.
+ *
+ * Blob.cpp / World.cpp cannot be compiled here (they include Eigen, which the
+ * reference's build downloads), so the short Blob-level loops that call the
+ * element and solver objects are restated below, each citing the lines it follows.
+ * Every numerical kernel (element math, pattern, matrix build, SpMV, CG, RNG) is the
+ * reference's own code.  The C API mirrors oracle/ffea_oracle.c (ref_* for orc_*), so
+ * the tests run one scenario through both and compare bit for bit.
+ */
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <set>
+
+#include "RngStream.h"
+#include "mat_vec_fns.h"
+#include "mesh_node.h"
+#include "tetra_element_linear.h"
+#define private public /* read NoMassCGSolver::V for matrix comparisons */
+#include "NoMassCGSolver.h"
+#undef private
+
+namespace {
+const double meso_volume = 4.913e-30, meso_mass = 1.994307387553024e-26;
+const double meso_pressure = 8.4305832688784826e8, meso_time = 3.7302915560886126e-13;
+}  // namespace
+
+struct ref_blob {
+    int N, E, num_pinned;
+    mesh_node *node;
+    tetra_element_linear *elem;
+    NoMassCGSolver *solver;
+    SimulationParams *params;
+    unsigned char *params_storage;
+    vector3 *force;
+    int *pinned;
+    double *rhs;
+    int last_inversions;
+};
+
+extern "C" {
+
+/* ---- RngStream ---------------------------------------------------------- */
+void *ref_rng_new(void) { return new RngStream(); }
+void ref_rng_delete(void *g) { delete static_cast<RngStream *>(g); }
+int ref_rng_set_package_seed(const unsigned long seed[6]) { return RngStream::SetPackageSeed(seed) ? 1 : 0; }
+double ref_rng_randu01(void *g) { return static_cast<RngStream *>(g)->RandU01(); }
+void ref_rng_get_state(void *g, unsigned long out[6]) { static_cast<RngStream *>(g)->GetState(out); }
+int ref_rng_set_seed(void *g, const unsigned long seed[6]) { return static_cast<RngStream *>(g)->SetSeed(seed) ? 1 : 0; }
+void ref_rng_advance_state(void *g, long e, long c) { static_cast<RngStream *>(g)->AdvanceState(e, c); }
+
+/* World::init RNG.2 (src/World.cpp:299-305), verbatim semantics with the host libc rand(). */
+void ref_package_seed_from_rng_seed(int rng_seed, unsigned long sixseed[6]) {
+    unsigned long twoMax[2] = {4294967087UL, 4294944443UL};
+    srand(rng_seed);
+    for (int i = 0; i < 6; i++) {
+        int s = (int)((unsigned)rand() + (unsigned)rand());
+        sixseed[i] = ((unsigned long)s) % twoMax[i / 3];
+    }
+}
+
+/* ---- Blob --------------------------------------------------------------- */
+ref_blob *ref_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
+                          const double *mat_si, const double *stokes_radius, int num_pinned,
+                          const int *pinned, double dt, double kT, double epsilon2, int max_iterations_cg,
+                          int calc_noise, int calc_stokes, double stokes_visc) {
+    ref_blob *b = new ref_blob();
+    b->N = num_nodes;
+    b->E = num_elements;
+    /* SimulationParams.o cannot be linked (its TU needs a generated header); the solver
+     * and element objects only read these plain fields, so a zeroed instance suffices. */
+    b->params_storage = new unsigned char[sizeof(SimulationParams)]();
+    b->params = reinterpret_cast<SimulationParams *>(b->params_storage);
+    b->params->dt = dt;
+    b->params->kT = kT;
+    b->params->epsilon2 = epsilon2;
+    b->params->max_iterations_cg = max_iterations_cg;
+    b->params->calc_noise = calc_noise;
+    b->params->calc_stokes = calc_stokes;
+    b->params->stokes_visc = stokes_visc;
+    b->params->calc_es = 0;
+
+    /* Blob::load_nodes (src/Blob.cpp:2480-2495) */
+    b->node = new mesh_node[num_nodes];
+    for (int i = 0; i < num_nodes; i++) {
+        b->node[i].pos.x = pos[3 * i];
+        b->node[i].pos.y = pos[3 * i + 1];
+        b->node[i].pos.z = pos[3 * i + 2];
+        b->node[i].vel.x = b->node[i].vel.y = b->node[i].vel.z = 0;
+        b->node[i].index = i;
+        b->node[i].stokes_radius = stokes_radius[i];
+    }
+    /* Blob::load_topology (src/Blob.cpp:2606-2622) + load_material_params (:2861-2866) */
+    b->elem = new tetra_element_linear[num_elements];
+    for (int e = 0; e < num_elements; e++) {
+        for (int j = 0; j < 10; j++) b->elem[e].n[j] = &b->node[topo[10 * e + j]];
+        for (int j = 0; j < 4; j++) b->node[topo[10 * e + j]].set_linear();
+        b->elem[e].index = e;
+        const double *m = &mat_si[6 * e];
+        b->elem[e].rho = m[0] * meso_volume / meso_mass;
+        b->elem[e].A = m[1] / (meso_pressure * meso_time);
+        b->elem[e].B = m[2] / (meso_pressure * meso_time) - (2.0 / 3.0) * m[1];
+        b->elem[e].G = m[3] / meso_pressure;
+        b->elem[e].E = m[4] / meso_pressure;
+        b->elem[e].dielectric = m[5];
+    }
+    b->num_pinned = num_pinned;
+    b->pinned = new int[num_pinned > 0 ? num_pinned : 1];
+    for (int i = 0; i < num_pinned; i++) b->pinned[i] = pinned[i];
+
+    /* Blob::init order (src/Blob.cpp:335-412) */
+    for (int e = 0; e < num_elements; e++) b->elem[e].linearise_element();
+    /* calc_rest_state_info (src/Blob.cpp:3768-3794) */
+    for (int e = 0; e < num_elements; e++) {
+        matrix3 J;
+        scalar temp;
+        b->elem[e].calculate_jacobian(J);
+        mat3_invert(J, b->elem[e].J_inv_0, &temp);
+        b->elem[e].calc_shape_function_derivatives_and_volume(J);
+        b->elem[e].vol_0 = b->elem[e].vol;
+        b->elem[e].mass = b->elem[e].vol_0 * b->elem[e].rho;
+    }
+    /* calculate_node_element_connectivity (src/Blob.cpp:3974-4019) */
+    for (int i = 0; i < num_nodes; i++) b->node[i].num_element_contributors = 0;
+    for (int e = 0; e < num_elements; e++)
+        for (int j = 0; j < 10; j++) b->node[b->elem[e].n[j]->index].num_element_contributors++;
+    for (int i = 0; i < num_nodes; i++)
+        b->node[i].force_contributions = new vector3 *[b->node[i].num_element_contributors + 1];
+    int *cnt = new int[num_nodes]();
+    for (int e = 0; e < num_elements; e++)
+        for (int j = 0; j < 10; j++) {
+            int ni = b->elem[e].n[j]->index;
+            b->node[ni].force_contributions[cnt[ni]++] = &b->elem[e].node_force[j];
+        }
+    delete[] cnt;
+    b->solver = new NoMassCGSolver();
+    std::set<int> bsite;
+    b->solver->init(num_nodes, num_elements, b->node, b->elem, b->params, num_pinned, b->pinned, bsite);
+    b->force = new vector3[num_nodes];
+    for (int i = 0; i < num_nodes; i++) b->force[i].x = b->force[i].y = b->force[i].z = 0;
+    if (calc_stokes == 1)
+        for (int i = 0; i < num_nodes; ++i)
+            b->node[i].stokes_drag = 6.0 * ffea_const::pi * stokes_visc * b->node[i].stokes_radius;
+    b->rhs = new double[3 * (size_t)num_nodes]();
+    return b;
+}
+
+void ref_blob_destroy(ref_blob *b) {
+    if (!b) return;
+    delete b->solver;
+    delete[] b->elem;
+    delete[] b->node;
+    delete[] b->force;
+    delete[] b->pinned;
+    delete[] b->rhs;
+    delete[] b->params_storage;
+    delete b;
+}
+
+void ref_blob_set_positions(ref_blob *b, const double *pos) {
+    for (int i = 0; i < b->N; i++) {
+        b->node[i].pos.x = pos[3 * i];
+        b->node[i].pos.y = pos[3 * i + 1];
+        b->node[i].pos.z = pos[3 * i + 2];
+    }
+}
+
+static void zero_forces(ref_blob *b) {
+    for (int e = 0; e < b->E; e++) b->elem[e].zero_force();
+    for (int i = 0; i < b->N; i++) b->force[i].x = b->force[i].y = b->force[i].z = 0;
+}
+
+/* Blob::update_internal_forces, nomass, one thread (src/Blob.cpp:675-724). */
+int ref_blob_update_internal_forces(ref_blob *b, void *rngp) {
+    RngStream *rng = static_cast<RngStream *>(rngp);
+    int inversions = 0;
+    for (int n = 0; n < b->E; n++) {
+        matrix3 J, stress;
+        vector12 du;
+        b->elem[n].calculate_jacobian(J);
+        if (b->elem[n].calc_shape_function_derivatives_and_volume(J) == FFEA_ERROR) inversions++;
+        b->elem[n].create_viscosity_matrix();
+        mat3_set_zero(stress);
+        b->elem[n].add_shear_elastic_stress(J, stress);
+        b->elem[n].add_bulk_elastic_stress(stress);
+        if (b->params->calc_noise == 1) b->elem[n].add_fluctuating_stress(b->params, rng, stress, 0);
+        b->elem[n].internal_stress_mag = sqrt(mat3_double_contraction_symmetric(stress));
+        vec12_set_zero(du);
+        b->elem[n].apply_stress_tensor(stress, du);
+        b->elem[n].add_element_force_vector(du);
+    }
+    b->last_inversions = inversions;
+    return inversions;
+}
+
+/* Blob::aggregate_forces_and_solve up to the solve (src/Blob.cpp:3826-3923). */
+static void aggregate(ref_blob *b, RngStream *rng) {
+    const int thread_id = 0;
+    for (int n = 0; n < b->N; n++)
+        for (int m = 0; m < b->node[n].num_element_contributors; m++) {
+            b->force[n][0] += b->node[n].force_contributions[m]->x;
+            b->force[n][1] += b->node[n].force_contributions[m]->y;
+            b->force[n][2] += b->node[n].force_contributions[m]->z;
+        }
+    if (b->params->calc_stokes == 1 && b->params->calc_noise == 1)
+        for (int i = 0; i < b->N; i++) {
+            b->force[i].x -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
+            b->force[i].y -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
+            b->force[i].z -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
+        }
+    /* Blob::linearise_force (src/Blob.cpp:1066-1085) */
+    int nIdx[10];
+    for (int i = 0; i < b->E; i++) {
+        for (int j = 0; j < 10; j++) nIdx[j] = b->elem[i].n[j]->index;
+        vector3 *force = b->force;
+        for (int j = 0; j < 3; j++) {
+            force[nIdx[0]][j] += 0.5 * (force[nIdx[4]][j] + force[nIdx[5]][j] + force[nIdx[6]][j]);
+            force[nIdx[1]][j] += 0.5 * (force[nIdx[4]][j] + force[nIdx[7]][j] + force[nIdx[8]][j]);
+            force[nIdx[2]][j] += 0.5 * (force[nIdx[5]][j] + force[nIdx[7]][j] + force[nIdx[9]][j]);
+            force[nIdx[3]][j] += 0.5 * (force[nIdx[6]][j] + force[nIdx[8]][j] + force[nIdx[9]][j]);
+        }
+        for (int j = 4; j < 10; j++) force[nIdx[j]][0] = force[nIdx[j]][1] = force[nIdx[j]][2] = 0.;
+    }
+    for (int n = 0; n < b->num_pinned; n++) {
+        int pn = b->pinned[n];
+        b->force[pn].x = b->force[pn].y = b->force[pn].z = 0;
+    }
+    for (int i = 0; i < b->N; i++) {
+        b->rhs[3 * i] = b->force[i].x;
+        b->rhs[3 * i + 1] = b->force[i].y;
+        b->rhs[3 * i + 2] = b->force[i].z;
+    }
+}
+
+/* Blob::update_positions (src/Blob.cpp:737-764) + euler_integrate (:3937-3953). */
+int ref_blob_update_positions(ref_blob *b, void *rngp) {
+    aggregate(b, static_cast<RngStream *>(rngp));
+    if (b->solver->solve(b->force) == FFEA_ERROR) return -1;
+    for (int i = 0; i < b->N; i++) {
+        b->node[i].vel[0] = b->force[i][0];
+        b->node[i].vel[1] = b->force[i][1];
+        b->node[i].vel[2] = b->force[i][2];
+        b->node[i].pos[0] += b->force[i][0] * b->params->dt;
+        b->node[i].pos[1] += b->force[i][1] * b->params->dt;
+        b->node[i].pos[2] += b->force[i][2] * b->params->dt;
+    }
+    for (int n = 0; n < b->E; n++) b->elem[n].linearise_element();
+    return 0;
+}
+
+/* One World::run iteration (src/World.cpp:1844-2043). */
+int ref_world_step(ref_blob **blobs, int nblobs, void *rng) {
+    int inv = 0, err = 0;
+    for (int i = 0; i < nblobs; i++) zero_forces(blobs[i]);
+    for (int i = 0; i < nblobs; i++) inv += ref_blob_update_internal_forces(blobs[i], rng);
+    if (inv > 0) return inv;
+    for (int i = 0; i < nblobs; i++)
+        if (ref_blob_update_positions(blobs[i], rng) != 0) err = 1;
+    return err ? -1 : 0;
+}
+
+int ref_blob_forces_only(ref_blob *b, void *rng, double *out_rhs) {
+    zero_forces(b);
+    int inv = ref_blob_update_internal_forces(b, rng);
+    aggregate(b, static_cast<RngStream *>(rng));
+    memcpy(out_rhs, b->rhs, sizeof(double) * 3 * (size_t)b->N);
+    return inv;
+}
+
+/* NoMassCGSolver::solve on a caller RHS (in/out, [3*N_t]). */
+int ref_blob_solve(ref_blob *b, double *x) {
+    vector3 *v = new vector3[b->N];
+    for (int i = 0; i < b->N; i++) v[i].assign(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    int st = b->solver->solve(v);
+    for (int i = 0; i < b->N; i++) {
+        x[3 * i] = v[i].x;
+        x[3 * i + 1] = v[i].y;
+        x[3 * i + 2] = v[i].z;
+    }
+    delete[] v;
+    return st == FFEA_OK ? 0 : -1;
+}
+
+void ref_blob_build(ref_blob *b) { b->solver->V->build(); }
+int ref_blob_nnz(ref_blob *b) { return b->solver->V->get_num_nonzero_elements(); }
+int ref_blob_num_rows(ref_blob *b) { return b->solver->V->get_num_rows(); }
+void ref_blob_get_csr(ref_blob *b, int *key, int *col, double *val) {
+    SparseMatrixFixedPattern *V = b->solver->V;
+    int nr = V->get_num_rows(), nnz = V->get_num_nonzero_elements();
+    sparse_entry *en = V->get_entries();
+    int *k = V->get_key();
+    for (int i = 0; i <= nr; i++) key[i] = k[i];
+    for (int i = 0; i < nnz; i++) {
+        col[i] = en[i].column_index;
+        val[i] = en[i].val;
+    }
+}
+void ref_blob_apply_matrix(ref_blob *b, const double *in, double *out) {
+    vector3 *vi = new vector3[b->N], *vo = new vector3[b->N];
+    for (int i = 0; i < b->N; i++) vi[i].assign(in[3 * i], in[3 * i + 1], in[3 * i + 2]);
+    b->solver->V->apply(vi, vo);
+    for (int i = 0; i < b->N; i++) {
+        out[3 * i] = vo[i].x;
+        out[3 * i + 1] = vo[i].y;
+        out[3 * i + 2] = vo[i].z;
+    }
+    delete[] vi;
+    delete[] vo;
+}
+
+void ref_blob_get(ref_blob *b, const char *what, double *out) {
+    size_t N = b->N, E = b->E;
+    if (!strcmp(what, "pos")) {
+        for (size_t i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++) out[3 * i + c] = b->node[i].pos[c];
+    } else if (!strcmp(what, "vel")) {
+        for (size_t i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++) out[3 * i + c] = b->node[i].vel[c];
+    } else if (!strcmp(what, "force")) {
+        for (size_t i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++) out[3 * i + c] = b->force[i][c];
+    } else if (!strcmp(what, "rhs")) {
+        memcpy(out, b->rhs, sizeof(double) * 3 * N);
+    } else if (!strcmp(what, "dpsi")) {
+        for (size_t e = 0; e < E; e++)
+            for (int k = 0; k < 12; k++) out[12 * e + k] = b->elem[e].dpsi[k];
+    } else if (!strcmp(what, "vol")) {
+        for (size_t e = 0; e < E; e++) out[e] = b->elem[e].vol;
+    } else if (!strcmp(what, "vol_0")) {
+        for (size_t e = 0; e < E; e++) out[e] = b->elem[e].vol_0;
+    } else if (!strcmp(what, "J_inv_0")) {
+        for (size_t e = 0; e < E; e++)
+            for (int k = 0; k < 9; k++) out[9 * e + k] = b->elem[e].J_inv_0[k / 3][k % 3];
+    } else if (!strcmp(what, "F_ij")) {
+        for (size_t e = 0; e < E; e++)
+            for (int k = 0; k < 9; k++) out[9 * e + k] = b->elem[e].F_ij[k / 3][k % 3];
+    } else if (!strcmp(what, "visc")) {
+        for (size_t e = 0; e < E; e++)
+            for (int k = 0; k < 144; k++) out[144 * e + k] = b->elem[e].viscosity_matrix[k / 12][k % 12];
+    } else if (!strcmp(what, "node_force")) {
+        for (size_t e = 0; e < E; e++)
+            for (int j = 0; j < 10; j++)
+                for (int c = 0; c < 3; c++) out[30 * e + 3 * j + c] = b->elem[e].node_force[j][c];
+    } else if (!strcmp(what, "stress_mag")) {
+        for (size_t e = 0; e < E; e++) out[e] = b->elem[e].internal_stress_mag;
+    } else if (!strcmp(what, "stokes_drag")) {
+        for (size_t i = 0; i < N; i++) out[i] = b->node[i].stokes_drag;
+    } else if (!strcmp(what, "material")) {
+        for (size_t e = 0; e < E; e++) {
+            out[5 * e] = b->elem[e].rho;
+            out[5 * e + 1] = b->elem[e].A;
+            out[5 * e + 2] = b->elem[e].B;
+            out[5 * e + 3] = b->elem[e].G;
+            out[5 * e + 4] = b->elem[e].E;
+        }
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""GPU parity: the HIP path (through the C-ABI) against the reference's golden vectors and the
+oracle.
+
+Bars: FP64 path -- element forces, Stokes noise, gathered force vector and RngStream state
+bit-exact; velocities (CG solutions, whose dot products are summed in tree order instead of the
+reference's sequential order) within 1e-9 relative in norm; positions within 1e-12 relative.
+Mixed path (FP32 element/assembly/SpMV storage, FP64 CG scalars, RHS and integration) --
+force vector within 1e-5 relative in norm, velocities within 1e-4.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, box_case
+from test_golden import CASES, load_case
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_blob(m, prm, sc, seed, x0, precision="fp64"):
+    from ffea_amd.blob import Blob
+    return Blob(m, prm, sc, rng_state=seed, precision=precision, pos=x0)
+
+
+def rel(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def seq_csr_apply(key, col, val, x):
+    """SparseMatrixFixedPattern::apply's sequential row sums, in plain Python doubles."""
+    out = np.zeros(key.size - 1)
+    xl = x.tolist()
+    vl, cl = val.tolist(), col.tolist()
+    for i in range(key.size - 1):
+        w = 0.0
+        for j in range(key[i], key[i + 1]):
+            w += vl[j] * xl[cl[j]]
+        out[i] = w
+    return out
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_first_step_bitwise(name):
+    g, m, prm, sc = load_case(name)
+    print(f"case {name}", flush=True)
+    b = gpu_blob(m, prm, sc, g["seed"], g["x0"])
+    assert b.update_internal_forces() == 0
+    b.update_positions()
+    assert np.array_equal(b.rhs().ravel(), g["rhs"]), "force vector (element forces + noise) not bit-exact"
+    # RNG consumed exactly one step's draws
+    assert np.array_equal(b.rng_state(), g["rng_after_forces"])
+    assert rel(b.positions(), g["traj"][0]) < 1e-12
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_three_steps(name):
+    g, m, prm, sc = load_case(name)
+    b = gpu_blob(m, prm, sc, g["seed"], g["x0"])
+    x0 = g["x0"].ravel() * sc
+    for k in range(g["traj"].shape[0]):
+        b.step(1)
+        p = b.positions().ravel()
+        # error relative to the displacement accumulated so far, and to |x|
+        assert np.linalg.norm(p - g["traj"][k]) <= 1e-9 * np.linalg.norm(g["traj"][k] - x0) + 1e-15 * np.linalg.norm(x0)
+    assert rel(b.velocities(), g["vel"]) < 1e-9
+    assert np.array_equal(b.rng_state(), g["rng_after_steps"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_matrix_and_solve(name):
+    g, m, prm, sc = load_case(name)
+    b = gpu_blob(m, prm, sc, g["seed"], g["x0"])
+    b.update_internal_forces()  # element matrices at x0, as in the golden
+    x = np.random.default_rng(3).normal(size=3 * m.num_nodes)
+    want = seq_csr_apply(g["csr_key"], g["csr_col"], g["csr_val"], x)
+    got = b.apply_matrix(x)
+    assert np.array_equal(got, want), "assembled matrix or SpMV order differs from the reference"
+    v = b.solve(g["rhs"])
+    assert rel(v, g["solve_v"]) < 1e-9
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_precision_against_golden(name):
+    g, m, prm, sc = load_case(name)
+    b = gpu_blob(m, prm, sc, g["seed"], g["x0"], precision="mixed")
+    b.update_internal_forces()
+    b.update_positions()
+    assert rel(b.rhs(), g["rhs"]) < 1e-5
+    assert np.array_equal(b.rng_state(), g["rng_after_forces"])
+    b.step(2)
+    assert rel(b.velocities(), g["vel"]) < 1e-4
+    x0 = g["x0"].ravel() * sc
+    assert np.linalg.norm(b.positions().ravel() - g["traj"][2]) <= 1e-4 * np.linalg.norm(g["traj"][2] - x0)
+
+
+def test_fp64_c0_trajectory_against_oracle(oracle):
+    """C0 cube (8^3 cells, 729 linear nodes, 3072 tets), noise on, 20 steps."""
+    import pyoracle
+    m, prm, sc, x0 = box_case((8, 8, 8), noise=1)
+    seed = oracle.package_seed(44)
+    oracle.set_package_seed(seed)
+    s = oracle.stream()
+    bo = oracle.blob(m, prm, sc, pos=x0)
+    b = gpu_blob(m, prm, sc, seed, x0)
+    xs = x0.ravel() * sc
+    for k in range(20):
+        assert pyoracle.world_step(oracle, [bo], s) == 0
+        b.step(1)
+        if k == 0:
+            assert np.array_equal(b.rhs().ravel(), bo.get("rhs"))
+        po = bo.get("pos")
+        assert np.linalg.norm(b.positions().ravel() - po) <= 1e-9 * np.linalg.norm(po - xs)
+        assert b.last_cg()[0] == oracle.cg_iterations(bo)
+    assert np.array_equal(b.rng_state(), s.state())
+
+
+def test_fp64_noise_off_long_trajectory(oracle):
+    """Noise-off relaxation, 1000 steps: FP64 GPU within 1e-10 relative of the oracle."""
+    import pyoracle
+    m, prm, sc, x0 = box_case((4, 4, 3), noise=0, sigma=0.05, seed=2)
+    seed = oracle.package_seed(44)
+    oracle.set_package_seed(seed)
+    s = oracle.stream()
+    bo = oracle.blob(m, prm, sc, pos=x0)
+    b = gpu_blob(m, prm, sc, seed, x0)
+    for _ in range(1000):
+        assert pyoracle.world_step(oracle, [bo], s) == 0
+    b.step(1000)
+    po = bo.get("pos")
+    xs = x0.ravel() * sc
+    assert np.linalg.norm(b.positions().ravel() - po) <= 1e-10 * np.linalg.norm(po - xs)
+    assert np.linalg.norm(b.positions().ravel() - po) <= 1e-10 * np.linalg.norm(po)
+
+
+def test_inversion_is_reported():
+    """Moving one node through the opposite face flips an element (src/tetra_element_linear.cpp:260)."""
+    from ffea_amd import _lib
+    m, prm, sc, x0 = box_case((2, 2, 2), noise=0)
+    b = gpu_blob(m, prm, sc, np.array([1, 2, 3, 4, 5, 6], np.uint64), x0)
+    p = b.positions()
+    p[0] += 3.0 * 5.9  # node 0 pushed through its elements
+    b.set_positions(p.ravel())
+    with pytest.raises(_lib.InversionError):
+        b.step(1)
+
+
+def test_cg_nonconvergence_is_reported():
+    from ffea_amd import _lib
+    m, prm, sc, x0 = box_case((3, 3, 3), noise=1)
+    prm.max_iterations_cg = 1
+    prm.epsilon2 = 1e-30
+    b = gpu_blob(m, prm, sc, np.array([1, 2, 3, 4, 5, 6], np.uint64), x0)
+    before = b.positions()
+    with pytest.raises(_lib.CGError):
+        b.step(1)
+    assert np.array_equal(b.positions(), before)  # integration skipped, as the reference does
