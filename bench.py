@@ -102,6 +102,21 @@ def kernel_table(kt, sizes, bm):
 
 
 def cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
+    """Runs _cpu_baseline in a thread with a 2 GiB stack: the reference's SpMV keeps two
+    3*N_t-double arrays on the stack (src/SparseMatrixFixedPattern.cpp:153-154), 44 MB at c1."""
+    import threading
+    res = {}
+    old = threading.stack_size(2 << 30)
+    try:
+        t = threading.Thread(target=lambda: res.update(out=_cpu_baseline(cfg, m, prm, sc, x0, max_seconds)))
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    return res.get("out")
+
+
+def _cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
     """The reference's own compiled objects (oracle/_ref, -O3 -ffast-math, OpenMP within the
     blob) on this host, falling back to the FP64 restatement; a bounded sample of steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -115,10 +130,10 @@ def cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
         return None
     cores = os.cpu_count() or 1
     if kind == "reference":
-        cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+        cores = lib.num_threads()
     seed = lib.package_seed(prm.rng_seed)
     lib.set_package_seed(seed)
-    s = lib.stream()
+    s = lib.stream(per_thread=True) if kind == "reference" else lib.stream()
     t0 = time.time()
     blob = lib.blob(m, prm, sc, pos=x0)
     setup = time.time() - t0
@@ -132,6 +147,7 @@ def cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
         if el > max_seconds or n >= 50:
             break
     blob.close()
+    del s
     log(f"[cpu {kind}] {n} steps in {el:.2f}s")
     return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": kind,
             "sample": f"{n} noise-on steps of the same {cfg} workload after 1 warm-up step "

@@ -252,6 +252,9 @@ class Reference(_Lib):
         super().__init__(path)
         L = self.lib
         L.ref_rng_new.restype = C.c_void_p
+        L.ref_rng_new_array.restype = C.c_void_p
+        L.ref_rng_new_array.argtypes = [C.c_int]
+        L.ref_num_threads.restype = C.c_int
         L.ref_rng_delete.argtypes = [C.c_void_p]
         L.ref_rng_randu01.argtypes = [C.c_void_p]
         L.ref_rng_randu01.restype = C.c_double
@@ -259,14 +262,18 @@ class Reference(_Lib):
         L.ref_rng_set_seed.argtypes = [C.c_void_p, _ulp]
         L.ref_rng_advance_state.argtypes = [C.c_void_p, C.c_long, C.c_long]
 
-    def stream(self) -> "RefStream":
-        return RefStream(self)
+    def stream(self, per_thread: bool = False) -> "RefStream":
+        """One RngStream, or (per_thread) one per OpenMP thread as World::init makes them."""
+        return RefStream(self, self.lib.ref_num_threads() if per_thread else 1)
+
+    def num_threads(self) -> int:
+        return self.lib.ref_num_threads()
 
 
 class RefStream:
-    def __init__(self, r: Reference):
+    def __init__(self, r: Reference, n: int = 1):
         self.r = r
-        self.ptr = C.c_void_p(r.lib.ref_rng_new())
+        self.ptr = C.c_void_p(r.lib.ref_rng_new_array(n))
 
     def __del__(self):
         try:

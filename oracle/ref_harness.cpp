@@ -19,6 +19,9 @@
 #include <cstring>
 #include <new>
 #include <set>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "RngStream.h"
 #include "mat_vec_fns.h"
@@ -49,8 +52,17 @@ struct ref_blob {
 extern "C" {
 
 /* ---- RngStream ---------------------------------------------------------- */
-void *ref_rng_new(void) { return new RngStream(); }
-void ref_rng_delete(void *g) { delete static_cast<RngStream *>(g); }
+void *ref_rng_new(void) { return new RngStream[1]; }
+/* one stream per OpenMP thread, as World::init allocates them (src/World.cpp:313) */
+void *ref_rng_new_array(int n) { return new RngStream[n]; }
+int ref_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void ref_rng_delete(void *g) { delete[] static_cast<RngStream *>(g); }
 int ref_rng_set_package_seed(const unsigned long seed[6]) { return RngStream::SetPackageSeed(seed) ? 1 : 0; }
 double ref_rng_randu01(void *g) { return static_cast<RngStream *>(g)->RandU01(); }
 void ref_rng_get_state(void *g, unsigned long out[6]) { static_cast<RngStream *>(g)->GetState(out); }
@@ -178,10 +190,20 @@ static void zero_forces(ref_blob *b) {
     for (int i = 0; i < b->N; i++) b->force[i].x = b->force[i].y = b->force[i].z = 0;
 }
 
-/* Blob::update_internal_forces, nomass, one thread (src/Blob.cpp:675-724). */
+/* Blob::update_internal_forces, nomass (src/Blob.cpp:660-724); OpenMP over elements with one
+ * stream per thread in the FFEA_PARALLEL_WITHIN_BLOB build, exactly one thread otherwise. */
 int ref_blob_update_internal_forces(ref_blob *b, void *rngp) {
     RngStream *rng = static_cast<RngStream *>(rngp);
     int inversions = 0;
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+#pragma omp parallel reduction(+ : inversions)
+    {
+        const int tid = omp_get_thread_num();
+#pragma omp for schedule(guided)
+#else
+    {
+        const int tid = 0;
+#endif
     for (int n = 0; n < b->E; n++) {
         matrix3 J, stress;
         vector12 du;
@@ -191,11 +213,12 @@ int ref_blob_update_internal_forces(ref_blob *b, void *rngp) {
         mat3_set_zero(stress);
         b->elem[n].add_shear_elastic_stress(J, stress);
         b->elem[n].add_bulk_elastic_stress(stress);
-        if (b->params->calc_noise == 1) b->elem[n].add_fluctuating_stress(b->params, rng, stress, 0);
+        if (b->params->calc_noise == 1) b->elem[n].add_fluctuating_stress(b->params, rng, stress, tid);
         b->elem[n].internal_stress_mag = sqrt(mat3_double_contraction_symmetric(stress));
         vec12_set_zero(du);
         b->elem[n].apply_stress_tensor(stress, du);
         b->elem[n].add_element_force_vector(du);
+    }
     }
     b->last_inversions = inversions;
     return inversions;
@@ -203,7 +226,9 @@ int ref_blob_update_internal_forces(ref_blob *b, void *rngp) {
 
 /* Blob::aggregate_forces_and_solve up to the solve (src/Blob.cpp:3826-3923). */
 static void aggregate(ref_blob *b, RngStream *rng) {
-    const int thread_id = 0;
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+#pragma omp parallel for schedule(guided)
+#endif
     for (int n = 0; n < b->N; n++)
         for (int m = 0; m < b->node[n].num_element_contributors; m++) {
             b->force[n][0] += b->node[n].force_contributions[m]->x;
@@ -211,11 +236,22 @@ static void aggregate(ref_blob *b, RngStream *rng) {
             b->force[n][2] += b->node[n].force_contributions[m]->z;
         }
     if (b->params->calc_stokes == 1 && b->params->calc_noise == 1)
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+#pragma omp parallel
+#endif
+    {
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+        const int thread_id = omp_get_thread_num();
+#pragma omp for schedule(guided)
+#else
+        const int thread_id = 0;
+#endif
         for (int i = 0; i < b->N; i++) {
             b->force[i].x -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
             b->force[i].y -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
             b->force[i].z -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
         }
+    }
     /* Blob::linearise_force (src/Blob.cpp:1066-1085) */
     int nIdx[10];
     for (int i = 0; i < b->E; i++) {
@@ -244,6 +280,9 @@ static void aggregate(ref_blob *b, RngStream *rng) {
 int ref_blob_update_positions(ref_blob *b, void *rngp) {
     aggregate(b, static_cast<RngStream *>(rngp));
     if (b->solver->solve(b->force) == FFEA_ERROR) return -1;
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+#pragma omp parallel for schedule(static)
+#endif
     for (int i = 0; i < b->N; i++) {
         b->node[i].vel[0] = b->force[i][0];
         b->node[i].vel[1] = b->force[i][1];
