@@ -51,42 +51,58 @@ def make_case(cfg):
 
 
 def algorithmic_bytes(kernel, sizes, bm):
-    """Bytes each launch must move (each operand once), per kernel; bm = matrix value bytes."""
-    N, E, B, S = sizes["num_linear"], sizes["num_elements"], sizes["num_blocks"], sizes["num_slots"]
+    """Bytes each launch must move, each operand counted once (bm = bytes per matrix/element
+    value: 8 on the FP64 path, 4 on the mixed path; CG vectors are FP64 on both).
+
+    N linear nodes, E elements, B 3x3 blocks, U = (B + N) / 2 upper-triangle blocks,
+    C = 10 E element-block contributions to them (4 diagonal + 6 off-diagonal per element)."""
+    N, E, B = sizes["num_linear"], sizes["num_elements"], sizes["num_blocks"]
+    U = (B + N) // 2
+    C = 10 * E
     v = 24 * N  # one FP64 3-vector over the linear nodes
     return {
-        "element": E * (16 + 72 + 8 + 32 + 16 + 12 * bm + 90 * bm) + v,
-        "assemble": E * 16 * (9 * bm + 4) + 4 * (S + 1) + 5 * S + 9 * bm * S + v,
-        "rhs": E * (12 * bm + 16) + 4 * (N + 1) + 8 * N + 1 * N + 4 * N + 6 * v,
+        # enodes 16, J0^-1 72, vol0 8, A B G E 32, last_det r+w 16; writes forces 12 + record 20
+        "element": E * (16 + 72 + 8 + 32 + 16 + (12 + 20) * bm) + v,
+        # records + contribution codes + per-entry tables, writes every block once + D^-1
+        "assemble": 20 * bm * E + 4 * C + 15 * U + 9 * bm * B + 32 * N,
+        # element forces + codes + per-node data; writes f r x p q z
+        "rhs": 12 * bm * E + 16 * E + 39 * N + 6 * v,
+        # matrix + column indices, z gathered once, p q read and written
         "cg_spmv": 9 * bm * B + 4 * B + 5 * v,
-        "cg_update": 7 * v,
+        "cg_update": 8 * v,
         "integrate": 4 * v,
     }[kernel]
 
 
 def run_gpu(m, prm, sc, x0, precision, steps, warmup):
+    """Warm-up, then two timed regions of `steps` steps: the first without per-kernel events
+    (the throughput), the second with start/stop events carried in every dispatch packet on
+    the blob's stream (the per-kernel durations)."""
     from ffea_amd.blob import Blob, package_seed
     t0 = time.time()
     b = Blob(m, prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision, pos=x0)
     log(f"[{precision}] blob created in {time.time() - t0:.1f}s: {b.sizes()}")
     b.step(warmup)
     b.synchronize()
-    b.set_profiling(True)
     iters = []
-    b.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         b.step(1)
         iters.append(b.last_cg()[0])
     b.synchronize()
     dt = time.perf_counter() - t0
+    b.set_profiling(True)
+    t1 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    dt_prof = time.perf_counter() - t1
     kt = b.kernel_times()
     b.set_profiling(False)
     sizes = b.sizes()
-    log(f"[{precision}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s, CG iterations "
-        f"mean {np.mean(iters):.2f}")
+    log(f"[{precision}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s (with per-kernel events "
+        f"{steps / dt_prof:.1f} steps/s), CG iterations mean {np.mean(iters):.2f}")
     b.close()
-    return steps / dt, dt, kt, sizes, float(np.mean(iters))
+    return steps / dt, dt, kt, sizes, float(np.mean(iters)), dt_prof / steps
 
 
 def kernel_table(kt, sizes, bm):
@@ -172,13 +188,13 @@ def main():
         sys.exit(2)
     m, prm, sc, x0, desc = make_case(a.config)
     log(f"workload {a.config}: {desc}")
-    sps, wall, kt, sizes, iters = run_gpu(m, prm, sc, x0, "fp64", a.steps, a.warmup)
+    sps, wall, kt, sizes, iters, prof_step = run_gpu(m, prm, sc, x0, "fp64", a.steps, a.warmup)
     ktab = kernel_table(kt, sizes, 8)
     mixed = None
     if not a.no_mixed:
-        sps_m, wall_m, kt_m, sizes_m, iters_m = run_gpu(m, prm, sc, x0, "mixed", a.steps, a.warmup)
-        mixed = {"steps_per_s": sps_m, "ns_per_day": sps_m * 1e-5 * 86400, "cg_iters_mean": iters_m,
-                 "kernels": kernel_table(kt_m, sizes_m, 4)}
+        sps_m, wall_m, kt_m, sizes_m, iters_m, prof_m = run_gpu(m, prm, sc, x0, "mixed", a.steps, a.warmup)
+        mixed = {"steps_per_s": sps_m, "ns_per_day": sps_m * 1e-5 * 86400, "ms_per_step": 1e3 * wall_m / a.steps,
+                 "cg_iters_mean": iters_m, "kernels": kernel_table(kt_m, sizes_m, 4)}
     dom = max(ktab, key=lambda k: ktab[k]["total_ms"])
     d = ktab[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
@@ -203,6 +219,7 @@ def main():
                    "blocks": sizes["num_blocks"], "parallelism": "single GPU"},
         "ns_per_day": sps * 1e-5 * 86400,
         "cg_iters_mean": iters,
+        "ms_per_step_with_kernel_events": 1e3 * prof_step,
         "kernels": ktab,
         "mixed": mixed,
         "roofline": roof,

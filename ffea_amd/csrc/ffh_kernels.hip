@@ -18,6 +18,7 @@
 // The FP64 path rounds exactly as the reference's strict build (no FMA contraction: the
 // file is compiled with -ffp-contract=off); only the order of the CG dot-product sums
 // differs (tree instead of sequential), deterministically.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -48,6 +49,7 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kLaneTable = 4 * 256 * 2;  // jump[kLaneTable + 2 l + comp] = A_comp^(7 l), l < 64
 constexpr int kMaxReduceBlocks = 1 << 16;
 
 // ----------------------------------------------------------------------------------------
@@ -83,7 +85,21 @@ struct Stream6 {
     uint32_t s[6];
 };
 
-// State of the stream `off` draws after `base`, via the base-256 digit tables.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+    typedef double4 type;
+    __device__ static double4 make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
+};
+template <>
+struct Vec4<float> {
+    typedef float4 type;
+    __device__ static float4 make(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+};
+
+// State of the stream `off` draws after `base`, via the base-256 digit tables
+// tbl[(d*256 + v)*2 + comp] = A_comp^(v 256^d).
 __device__ __forceinline__ Stream6 rng_jump(const uint32_t *__restrict__ base, const Mat3u *__restrict__ tbl,
                                             uint32_t off) {
     Stream6 g;
@@ -98,6 +114,11 @@ __device__ __forceinline__ Stream6 rng_jump(const uint32_t *__restrict__ base, c
         }
     }
     return g;
+}
+
+__device__ __forceinline__ void rng_apply(Stream6 &g, const Mat3u &m1, const Mat3u &m2) {
+    matvec<ffh::kM1, kC1>(m1, &g.s[0]);
+    matvec<ffh::kM2, kC2>(m2, &g.s[3]);
 }
 
 // RngStream::U01 (src/RngStream.cpp:266-289) in exact integers; u rounds as the reference.
@@ -125,7 +146,9 @@ __device__ __forceinline__ double rng_u01(Stream6 &g) {
 __device__ __forceinline__ double rand_half(Stream6 &g) { return (-.5) + ((.5) - (-.5)) * rng_u01(g); }
 
 // ----------------------------------------------------------------------------------------
-// Reductions: deterministic block tree + last-block-done final sum in fixed order.
+// Reductions.  A producing kernel writes one partial sum per block (block tree, fixed order);
+// every block of the consuming kernel re-adds those partials in the same fixed order, so all
+// blocks see bitwise the same scalar and runs are reproducible -- no atomics, no fences.
 // ----------------------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -133,66 +156,36 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sums NV values per thread over the block; thread 0 gets the totals.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[kBlock / 64]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) sh[k][w] = s;
-    }
+// Block sum of v; the total is returned to every thread.  sh: >= blockDim.x/64 doubles.
+__device__ __forceinline__ double block_sum_all(double v, double *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum(v);
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0.0;
-            for (int i = 0; i < kBlock / 64; i++) s += sh[k][i];
-            v[k] = s;
-        }
-    }
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; i++) t += sh[i];
+    return t;
 }
 
-// Publishes this block's NV partial sums; returns true in (all threads of) the last block,
-// whose thread 0 then holds the grid totals in v.
-template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double *__restrict__ partials, unsigned *__restrict__ ticket) {
-    __shared__ double sh[NV][kBlock / 64];
-    __shared__ bool last;
-    block_sum<NV>(v, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) partials[(size_t)k * kMaxReduceBlocks + blockIdx.x] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
-            s += __builtin_nontemporal_load(&partials[(size_t)k * kMaxReduceBlocks + i]);
-        acc[k] = s;
-    }
-    __syncthreads();
-    block_sum<NV>(acc, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) v[k] = acc[k];
-        *ticket = 0u;
-    }
-    return true;
+// Writes this block's sum of v to parts[blockIdx.x].
+__device__ __forceinline__ void block_partial(double v, double *__restrict__ parts, double *sh) {
+    double t = block_sum_all(v, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = t;
 }
 
-// CG control block (device memory).
+// Fixed-order total of P partials, identical in every block.
+__device__ __forceinline__ double parts_total(const double *__restrict__ parts, int P, double *sh) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) s += parts[i];
+    return block_sum_all(s, sh);
+}
+
+// CG control block (device memory), written by one thread of the deciding kernel.
 struct CgCtl {
-    double delta, beta, alpha, f2, res, eps2;
-    int iter, done, status, maxit;
-    int inversions, pad_;
+    double res, f2;
+    double delta[2];  // delta_k = r_k . D^-1 r_k, slot k & 1
+    int iter, done, status, inversions;
 };
 
 // ----------------------------------------------------------------------------------------
@@ -204,8 +197,8 @@ struct ElemArgs {
     const double *jinv0;  // SoA [9][E]
     const double *vol0, *A, *B, *G, *Em;
     double *last_det;
-    void *eforce;  // T SoA [12][E], index (3a+i)*E + e
-    void *eblk;    // T AoS [E][90]
+    void *eforce;  // T SoA [12][E]: node_force[a][i] at (3a+i)*E + e
+    void *erec;    // T [E][20]: (vol, A, B, 0), then (d psi_a/dx, /dy, /dz, 0) for a = 0..3
     int E;
     int calc_noise;
     double kT, dt;
@@ -219,7 +212,7 @@ struct ElemArgs {
 // src/tetra_element_linear.cpp:299-328, with the macros of include/tetra_element_linear.h:55-116
 // and the diffusion term :599-638): the upper triangle as built, the lower one its mirror.
 template <typename R>
-__device__ __forceinline__ R visc_entry(const R (&dpsi)[12], R A, R B, R vol, int Rr, int Cc) {
+__device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, int Cc) {
     if (Rr > Cc) {
         int t = Rr;
         Rr = Cc;
@@ -285,18 +278,14 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
 
     const R A = (R)g.A[e], B = (R)g.B[e], G = (R)g.G[e], Em = (R)g.Em[e], vol0 = (R)g.vol0[e];
 
-    // viscosity blocks (a<=b): blk[p][3i+j] = V[a+4i][b+4j]
-    T *blk = reinterpret_cast<T *>(g.eblk) + (size_t)e * 90;
+    // record for the assembly: everything create_viscosity_matrix reads, node-major so that a
+    // node's gradient is one aligned 4-wide load
     {
-        int p = 0;
+        typedef typename Vec4<T>::type V4;
+        V4 *rec = reinterpret_cast<V4 *>(reinterpret_cast<T *>(g.erec) + (size_t)e * 20);
+        rec[0] = Vec4<T>::make((T)vol, (T)A, (T)B, (T)0);
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = a; b < 4; b++, p++)
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++) blk[p * 9 + 3 * i + j] = (T)visc_entry<R>(dpsi, A, B, vol, a + 4 * i, b + 4 * j);
+        for (int a = 0; a < 4; a++) rec[1 + a] = Vec4<T>::make((T)dpsi[a], (T)dpsi[4 + a], (T)dpsi[8 + a], (T)0);
     }
 
     // add_shear_elastic_stress (:378-396): F_ij = sum_k J[k][i] Jinv0[j][k]; S = F F^T
@@ -337,7 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     }
     // add_fluctuating_stress (:417-436): 7 draws at this element's stream offset
     if (g.calc_noise) {
-        Stream6 st = rng_jump(g.rng_state, g.jump, g.elem_base + 7u * (uint32_t)e);
+        // the wave's first element's state (uniform), then A^(7 lane) for this lane
+        const int lane = threadIdx.x & 63;
+        const uint32_t e0 = __builtin_amdgcn_readfirstlane((uint32_t)(e - lane));
+        Stream6 st = rng_jump(g.rng_state, g.jump, g.elem_base + 7u * e0);
+        rng_apply(st, g.jump[kLaneTable + 2 * lane], g.jump[kLaneTable + 2 * lane + 1]);
         const R c = sqrt(((R)24 * (R)g.kT) / (vol * (R)g.dt));
         const R bf = sqrt(B) * (R)rand_half(st);
         S[0][0] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
@@ -351,93 +344,144 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
         S[2][1] = S[1][2];
     }
     // apply_stress_tensor (:441-449) + add_element_force_vector (:474-490)
-    T *ef = reinterpret_cast<T *>(g.eforce);
+    T fo[12];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             R du = (R)0 + vol * (dpsi[a] * S[i][0] + dpsi[4 + a] * S[i][1] + dpsi[8 + a] * S[i][2]);
-            ef[(size_t)(3 * a + i) * g.E + e] = (T)((R)0 - du);
+            fo[3 * a + i] = (T)((R)0 - du);
         }
+    T *ef = reinterpret_cast<T *>(g.eforce);
+#pragma unroll
+    for (int k = 0; k < 12; k++) ef[(size_t)k * g.E + e] = fo[k];
 }
 
 // ----------------------------------------------------------------------------------------
-// Assembly
+// Assembly over the upper triangle of the block pattern
 // ----------------------------------------------------------------------------------------
 struct AsmArgs {
-    const void *eblk;  // T [E][90]
-    const uint32_t *ctb_off, *ctb;
-    const int32_t *col;
+    const void *erec;  // T [E][20]
+    const int64_t *toff;
+    const uint16_t *twidth, *cnt;
+    const uint32_t *ctb;
+    const int32_t *slot, *mirror, *node;
     const uint8_t *flags;
     const double *zeta;
     void *val;  // T tiles
     double *dinv;
-    int64_t nslots;
+    int64_t nupper;
 };
 
-template <typename T, typename Acc>
+// SparseMatrixFixedPattern::build (src/SparseMatrixFixedPattern.cpp:85-93): each entry is the
+// sum of its sources in registration order -- ascending element, Stokes drag last
+// (src/NoMassCGSolver.cpp:93-139, src/SparseMatrixTypes.cpp:52-58).  K is symmetric entry by
+// entry (each element block pair is an exact mirror), so the lower block is the transposed
+// upper one.  Diagonal blocks also give calc_inverse_diagonal (:325-333).
+//
+// Block (a,b) of an element, entry (i,j) = V[a+4i][b+4j] with g_a = grad psi_a, g_b = grad psi_b
+// (the element's record holds dpsi[4i+a] = g_a[i]); these are the expressions of visc_entry with
+// every product written in an order that rounds identically:
+//   i == j : ((g_a[i] g_b[i]) (A+B) + (g_a.g_b) A) vol
+//   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
-    const int64_t slot = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= g.nslots) return;
-    const uint8_t fl = g.flags[slot];
-    const int col = g.col[slot];
-    Acc acc[9];
+    const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (u >= g.nupper) return;
+    const uint8_t fl = g.flags[u];
+    const bool diag = (fl & 1) != 0;
+    T acc[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) acc[k] = (Acc)0.0;
-    const T *eb = reinterpret_cast<const T *>(g.eblk);
-    const uint32_t c0 = g.ctb_off[slot], c1 = g.ctb_off[slot + 1];
-    for (uint32_t c = c0; c < c1; c++) {
-        const uint32_t code = g.ctb[c];
-        const T *b = eb + (size_t)(code >> 5) * 90 + ((code >> 1) & 15u) * 9;
-        if (code & 1u) {
+    for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
+    const T *rec = reinterpret_cast<const T *>(g.erec);
+    const int64_t grp = u >> 6;
+    const int lane = (int)(u & 63);
+    const int cnt = g.cnt[u];
+    const uint32_t *tl = g.ctb + g.toff[grp] + lane;
+#pragma unroll 2
+    for (int k = 0; k < cnt; k++) {
+        const uint32_t code = tl[64 * k];
+        const int a = (code >> 2) & 3, b = code & 3;
+        typedef typename Vec4<T>::type V4;
+        const V4 *rp = reinterpret_cast<const V4 *>(rec + (size_t)(code >> 4) * 20);
+        const V4 h = rp[0], va = rp[1 + a], vb = rp[1 + b];
+        const T ga[3] = {va.x, va.y, va.z};
+        const T gb[3] = {vb.x, vb.y, vb.z};
+        const T vol = h.x, A = h.y, B = h.z;
+        const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) acc[3 * i + j] += (Acc)b[3 * j + i];
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] += (Acc)b[k];
-        }
+            for (int j = 0; j < 3; j++) {
+                T v;
+                if (i == j) {
+                    v = ga[i] * gb[i] * (A + B);
+                    v += del2 * A;
+                } else if (diag) {
+                    v = (B + A) * (ga[i] * gb[j]);
+                } else {
+                    v = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
+                }
+                acc[3 * i + j] += v * vol;
+            }
     }
-    if (fl & 1) {
+    const int node = g.node[u];
+    if (diag) {
         if (fl & 2) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] = (Acc)0.0;
-            acc[0] = acc[4] = acc[8] = (Acc)1.0;
+            for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
+            acc[0] = acc[4] = acc[8] = (T)1.0;
         } else {
-            const Acc z = (Acc)g.zeta[col];
+            const T z = (T)g.zeta[node];
             acc[0] += z;
             acc[4] += z;
             acc[8] += z;
         }
-        // calc_inverse_diagonal (src/SparseMatrixFixedPattern.cpp:325-333)
-        g.dinv[3 * (size_t)col + 0] = 1.0 / (double)(T)acc[0];
-        g.dinv[3 * (size_t)col + 1] = 1.0 / (double)(T)acc[4];
-        g.dinv[3 * (size_t)col + 2] = 1.0 / (double)(T)acc[8];
+        g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[0];
+        g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[4];
+        g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[8];
     }
     T *v = reinterpret_cast<T *>(g.val);
-    const int64_t tile = slot & ~(int64_t)63;
-    const int lane = (int)(slot & 63);
+    {
+        const int64_t slot = g.slot[u];
+        const int64_t tile = slot & ~(int64_t)63;
+        const int lane = (int)(slot & 63);
 #pragma unroll
-    for (int k = 0; k < 9; k++) v[tile * 9 + k * 64 + lane] = (T)acc[k];
+        for (int k = 0; k < 9; k++) v[tile * 9 + k * 64 + lane] = acc[k];
+    }
+    const int32_t mslot = g.mirror[u];
+    if (mslot >= 0) {
+        const int64_t tile = (int64_t)mslot & ~(int64_t)63;
+        const int lane = mslot & 63;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) v[tile * 9 + (3 * i + j) * 64 + lane] = acc[3 * j + i];
+    }
 }
 
 // ----------------------------------------------------------------------------------------
-// RHS: force gather, Stokes noise, pinned rows, CG initialisation
+// RHS: force gather, Stokes noise, pinned rows, CG initialisation.  64 nodes x 3 components
+// per block (wave = component).
 // ----------------------------------------------------------------------------------------
-struct VecArgs {
+struct CgArgs {
     int N;
-    double *f, *r, *x, *p0, *p1, *q, *vel;
+    double *f, *r, *x, *p, *q, *z, *vel;
     const double *dinv;
-    double *partials;
-    unsigned *ticket;
+    double *parts_rz, *parts_rr, *parts_ff, *parts_pq;
+    int P_rhs, P_upd, P_spmv;
+    double eps2;
+    int maxit;
     CgCtl *ctl;
 };
 
 struct RhsArgs {
     const void *eforce;  // T SoA [12][E]
     int E;
-    const uint32_t *ctb_off, *ctb;
+    const double *f_in;  // solve() RHS given directly ([3N]); nullptr in the time step
+    const int64_t *toff;
+    const uint16_t *cnt;
+    const uint32_t *ctb;
     const double *zeta;
     const uint8_t *pinned;
     const uint32_t *rng_off;
@@ -449,53 +493,70 @@ struct RhsArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rhs(RhsArgs g, VecArgs v) {
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    double part[2] = {0.0, 0.0};
-    if (n < v.N) {
-        const T *ef = reinterpret_cast<const T *>(g.eforce);
-        double f[3] = {0.0, 0.0, 0.0};
-        for (uint32_t c = g.ctb_off[n]; c < g.ctb_off[n + 1]; c++) {
-            const uint32_t code = g.ctb[c];
-            const uint32_t e = code >> 2, a = code & 3u;
-#pragma unroll
-            for (int i = 0; i < 3; i++) f[i] += (double)ef[(size_t)(3 * a + i) * g.E + e];
-        }
-        if (g.noise) {
+__global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
+    __shared__ double sh[4];
+    __shared__ uint32_t rs[6][64];
+    const int i = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = blockIdx.x * 64 + lane;
+    const bool live = n < v.N;
+    // Stokes noise (src/Blob.cpp:3895-3898): draws 3*node + {x,y,z} of the node phase.  Wave 0
+    // jumps to the node's first draw; waves 1, 2 step on from the shared state.
+    const bool noise = g.noise && !g.f_in;
+    if (noise) {
+        if (i == 0 && live) {
             Stream6 st = rng_jump(g.rng_state, g.jump, g.node_base + g.rng_off[n]);
-            const double zeta = g.zeta[n];
 #pragma unroll
-            for (int i = 0; i < 3; i++) f[i] -= rand_half(st) * sqrt((24 * g.kT * zeta) / (g.dt));
+            for (int k = 0; k < 6; k++) rs[k][lane] = st.s[k];
         }
-        if (g.pinned[n]) f[0] = f[1] = f[2] = 0;
-        double z[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            v.f[3 * n + i] = f[i];
-            v.r[3 * n + i] = f[i];
-            v.x[3 * n + i] = 0.0;
-            v.p0[3 * n + i] = 0.0;
-            v.p1[3 * n + i] = 0.0;
-            z[i] = v.dinv[3 * n + i] * f[i];
-        }
-        part[0] = f[0] * z[0] + f[1] * z[1] + f[2] * z[2];
-        part[1] = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        __syncthreads();
     }
-    if (grid_sum<2>(part, v.partials, v.ticket) && threadIdx.x == 0) {
-        CgCtl *c = v.ctl;
-        c->delta = part[0];
-        c->f2 = part[1];
-        c->beta = 0.0;
-        c->alpha = 0.0;
-        c->res = 0.0;
-        c->iter = 0;
-        c->status = 0;
-        c->done = (part[1] == 0.0) ? 1 : 0;  // zero force: velocity 0 (see DESIGN notes)
+    double prz = 0.0, pff = 0.0;
+    if (live) {
+        double f;
+        if (g.f_in) {
+            f = g.f_in[3 * (size_t)n + i];
+        } else {
+            const T *ef = reinterpret_cast<const T *>(g.eforce);
+            f = 0.0;
+            const int cnt = g.cnt[n];
+            const uint32_t *tl = g.ctb + g.toff[blockIdx.x] + lane;
+#pragma unroll 4
+            for (int k = 0; k < cnt; k++) {
+                const uint32_t code = tl[64 * k];
+                f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
+            }
+            if (noise) {
+                Stream6 st;
+#pragma unroll
+                for (int k = 0; k < 6; k++) st.s[k] = rs[k][lane];
+                for (int k = 0; k < i; k++) (void)rng_u01(st);
+                f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
+            }
+            if (g.pinned[n]) f = 0;
+        }
+        const size_t k = 3 * (size_t)n + i;
+        const double z = v.dinv[k] * f;
+        v.f[k] = f;
+        v.r[k] = f;
+        v.x[k] = 0.0;
+        v.p[k] = 0.0;
+        v.q[k] = 0.0;
+        v.z[k] = z;
+        prz = f * z;
+        pff = f * f;
+    }
+    block_partial(prz, v.parts_rz, sh);
+    block_partial(pff, v.parts_ff, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        v.ctl->done = 0;
+        v.ctl->status = 0;
+        v.ctl->iter = 0;
+        v.ctl->res = 0.0;
     }
 }
 
 // ----------------------------------------------------------------------------------------
-// CG iteration
+// CG iteration `it` (the reference's loop index, src/NoMassCGSolver.cpp:175-194)
 // ----------------------------------------------------------------------------------------
 struct SpmvArgs {
     const int64_t *slice_off;
@@ -504,95 +565,101 @@ struct SpmvArgs {
     const void *val;
 };
 
-// p_new = beta p_old + D^-1 r (vec3_scale_and_add, src/mat_vec_fns.cpp:265-275, with
-// parallel_apply_preconditioner :278-292), q = K p_new (SparseMatrixFixedPattern::apply,
-// :148-191: per scalar row, ascending column), pTq and alpha = delta / pTq (:261-276, :176).
+// Convergence test of update(it-1) (||r||^2/||f||^2 < eps^2, src/NoMassCGSolver.cpp:182-191,
+// :233-248); beta = delta_it / delta_(it-1); p = z + beta p (vec3_scale_and_add,
+// src/mat_vec_fns.cpp:265-275); q = K p, formed as K z + beta q_old so that only z is gathered
+// (K is applied block by block, each scalar row summed in ascending column order like
+// SparseMatrixFixedPattern::apply, src/SparseMatrixFixedPattern.cpp:148-191); p.q partials.
+// One block = one 64-row slice; wave w computes component w of every row.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs m, VecArgs v, const double *__restrict__ pold,
-                                                    double *__restrict__ pnew) {
+__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it) {
+    __shared__ double sh[4];
     if (v.ctl->done) return;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
-    const double beta = v.ctl->beta;
-    double part[1] = {0.0};
+    double beta = 0.0;
+    if (it == 0) {
+        const double f2 = parts_total(v.parts_ff, v.P_rhs, sh);
+        const double d0 = parts_total(v.parts_rz, v.P_rhs, sh);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            v.ctl->f2 = f2;
+            v.ctl->delta[0] = d0;
+            if (f2 == 0.0) {  // zero force: zero velocity
+                v.ctl->done = 1;
+                v.ctl->status = 0;
+                v.ctl->iter = 0;
+            }
+        }
+        if (f2 == 0.0) return;
+    } else {
+        const double f2 = v.ctl->f2;
+        const double r2 = parts_total(v.parts_rr, v.P_upd, sh);
+        const double res = r2 / f2;
+        if (res < v.eps2 || it >= v.maxit) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                v.ctl->done = 1;
+                v.ctl->status = res < v.eps2 ? 0 : -1;
+                v.ctl->iter = it;
+                v.ctl->res = res;
+            }
+            return;
+        }
+        const double dnew = parts_total(v.parts_rz, v.P_upd, sh);
+        beta = dnew / v.ctl->delta[(it - 1) & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) v.ctl->delta[it & 1] = dnew;
+    }
+    const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 64 + lane;
+    double part = 0.0;
     if (row < v.N) {
-        const int s = row >> 6, lane = row & 63;
-        const int w = m.slice_width[s];
-        const int64_t base = m.slice_off[s];
+        const int w = m.slice_width[blockIdx.x];
+        const int64_t base = m.slice_off[blockIdx.x];
         const T *val = reinterpret_cast<const T *>(m.val);
-        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        const double *__restrict__ z = v.z;
+        double acc = 0.0;
+#pragma unroll 4
         for (int t = 0; t < w; t++) {
             const int64_t tile = base + (int64_t)t * 64;
             const int c = m.col[tile + lane];
-            const T *vt = val + tile * 9 + lane;
-            double pc[3];
-#pragma unroll
-            for (int j = 0; j < 3; j++) pc[j] = beta * pold[3 * c + j] + v.dinv[3 * c + j] * v.r[3 * c + j];
-            acc0 += (double)vt[0 * 64] * pc[0];
-            acc0 += (double)vt[1 * 64] * pc[1];
-            acc0 += (double)vt[2 * 64] * pc[2];
-            acc1 += (double)vt[3 * 64] * pc[0];
-            acc1 += (double)vt[4 * 64] * pc[1];
-            acc1 += (double)vt[5 * 64] * pc[2];
-            acc2 += (double)vt[6 * 64] * pc[0];
-            acc2 += (double)vt[7 * 64] * pc[1];
-            acc2 += (double)vt[8 * 64] * pc[2];
+            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
+            const T v0 = vt[0], v1 = vt[64], v2 = vt[128];
+            acc += (double)v0 * z[3 * c + 0];
+            acc += (double)v1 * z[3 * c + 1];
+            acc += (double)v2 * z[3 * c + 2];
         }
-        double pn[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            pn[j] = beta * pold[3 * row + j] + v.dinv[3 * row + j] * v.r[3 * row + j];
-            pnew[3 * row + j] = pn[j];
-        }
-        v.q[3 * row + 0] = acc0;
-        v.q[3 * row + 1] = acc1;
-        v.q[3 * row + 2] = acc2;
-        part[0] = pn[0] * acc0 + pn[1] * acc1 + pn[2] * acc2;
+        const size_t k = 3 * (size_t)row + ci;
+        const double pn = z[k] + beta * v.p[k];
+        const double qn = acc + beta * v.q[k];
+        v.p[k] = pn;
+        v.q[k] = qn;
+        part = pn * qn;
     }
-    if (grid_sum<1>(part, v.partials, v.ticket) && threadIdx.x == 0) {
-        v.ctl->alpha = v.ctl->delta / part[0];
-    }
+    block_partial(part, v.parts_pq, sh);
 }
 
-// x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260); convergence on
-// ||r||^2/||f||^2 < eps^2 (src/NoMassCGSolver.cpp:182-191, :233-248); delta_new = r.D^-1 r.
-__global__ __launch_bounds__(kBlock) void k_cg_update(VecArgs v, const double *__restrict__ p) {
+// alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
+// z = D^-1 r (parallel_apply_preconditioner, src/NoMassCGSolver.cpp:278-292); partials of r.r
+// and r.z.
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs v, int it) {
+    __shared__ double sh[4];
     if (v.ctl->done) return;
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    const double alpha = v.ctl->alpha;
-    double part[2] = {0.0, 0.0};
-    if (n < v.N) {
-        double r[3], z[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            v.x[3 * n + i] += alpha * p[3 * n + i];
-            r[i] = v.r[3 * n + i] + (-alpha) * v.q[3 * n + i];
-            v.r[3 * n + i] = r[i];
-            z[i] = v.dinv[3 * n + i] * r[i];
-        }
-        part[0] = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-        part[1] = r[0] * z[0] + r[1] * z[1] + r[2] * z[2];
+    const double alpha = v.ctl->delta[it & 1] / parts_total(v.parts_pq, v.P_spmv, sh);
+    double prr = 0.0, prz = 0.0;
+    const int n3 = 3 * v.N;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n3; k += gridDim.x * kBlock) {
+        v.x[k] += alpha * v.p[k];
+        const double r = v.r[k] + (-alpha) * v.q[k];
+        v.r[k] = r;
+        const double z = v.dinv[k] * r;
+        v.z[k] = z;
+        prr += r * r;
+        prz += r * z;
     }
-    if (grid_sum<2>(part, v.partials, v.ticket) && threadIdx.x == 0) {
-        CgCtl *c = v.ctl;
-        const double res = (c->f2 == 0.0) ? 0.0 : part[0] / c->f2;
-        c->res = res;
-        c->iter += 1;
-        if (res < c->eps2) {
-            c->done = 1;
-            c->status = 0;
-        } else if (c->iter >= c->maxit) {
-            c->done = 1;
-            c->status = -1;
-        } else {
-            c->beta = part[1] / c->delta;
-            c->delta = part[1];
-        }
-    }
+    block_partial(prr, v.parts_rr, sh);
+    block_partial(prz, v.parts_rz, sh);
 }
 
 // euler_integrate (src/Blob.cpp:3941-3953): vel = v; pos += v * dt.  Skipped after a CG
 // failure, as the reference skips it.  Thread 0 advances the stream by one step's draws.
-__global__ __launch_bounds__(kBlock) void k_integrate(VecArgs v, double *X, double *Y, double *Z, double dt,
+__global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double *X, double *Y, double *Z, double dt,
                                                      uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance) {
     const int n = blockIdx.x * kBlock + threadIdx.x;
     if (n == 0 && advance) {
@@ -637,11 +704,12 @@ __global__ __launch_bounds__(kBlock) void k_apply(SpmvArgs m, int N, const doubl
     out[3 * row + 2] = acc[2];
 }
 
-// Strain energy (src/Blob.cpp:1424-1432) recomputed from current positions.
-__global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *partials, unsigned *ticket,
-                                                          double *out) {
+// Strain energy terms (src/Blob.cpp:1424-1432) recomputed from current positions; one partial
+// per block, summed on the host.
+__global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *parts) {
+    __shared__ double sh[4];
     const int e = blockIdx.x * kBlock + threadIdx.x;
-    double part[1] = {0.0};
+    double part = 0.0;
     if (e < g.E) {
         const int4 n = g.enodes[e];
         const int nn[4] = {n.x, n.y, n.z, n.w};
@@ -668,9 +736,9 @@ __global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *pa
                     F[2][1] * F[2][1];
         double G = g.G[e], C = g.Em[e] - (2.0 / 3.0) * G, vol0 = g.vol0[e];
         double t = vol / vol0;
-        part[0] = vol0 * (G * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * G) * log(t));
+        part = vol0 * (G * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * G) * log(t));
     }
-    if (grid_sum<1>(part, partials, ticket) && threadIdx.x == 0) *out = 0.5 * part[0];
+    block_partial(part, parts, sh);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -698,19 +766,22 @@ struct ffh_blob {
     int4 *enodes = nullptr;
     double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
            *last_det = nullptr;
-    void *eforce = nullptr, *eblk = nullptr, *val = nullptr;
-    uint32_t *sctb_off = nullptr, *sctb = nullptr, *nctb_off = nullptr, *nctb = nullptr, *rng_off = nullptr;
-    int32_t *slot_col = nullptr, *slice_width = nullptr;
+    void *eforce = nullptr, *erec = nullptr, *val = nullptr;
+    uint32_t *uctb = nullptr, *nctb = nullptr, *rng_off = nullptr;
+    int64_t *ut_off = nullptr, *nt_off = nullptr;
+    uint16_t *ut_width = nullptr, *ut_cnt = nullptr, *nt_cnt = nullptr;
+    int32_t *slot_col = nullptr, *slice_width = nullptr, *up_slot = nullptr, *up_mirror = nullptr,
+            *up_node = nullptr;
     int64_t *slice_off = nullptr;
-    uint8_t *slot_flags = nullptr, *pinned = nullptr;
+    uint8_t *up_flags = nullptr, *pinned = nullptr;
     double *zeta = nullptr, *dinv = nullptr;
-    double *f = nullptr, *r = nullptr, *x = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *vel = nullptr;
-    double *partials = nullptr, *tmp_in = nullptr, *tmp_out = nullptr, *scalar_out = nullptr;
-    unsigned *ticket = nullptr;
+    double *f = nullptr, *r = nullptr, *x = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *vel = nullptr;
+    double *parts = nullptr, *tmp_in = nullptr, *tmp_out = nullptr;
     CgCtl *ctl = nullptr;
     CgCtl *ctl_host = nullptr;  // pinned host mirror
     uint32_t *rng_state = nullptr;
     Mat3u *jump = nullptr;
+    int P_rhs = 0, P_upd = 0, P_spmv = 0, P_elem = 0;
     // profiling
     bool profiling = false;
     double ktime[FFH_NUM_KERNELS] = {0};
@@ -722,15 +793,15 @@ struct ffh_blob {
     int cg_chunk = 16;
 
     ~ffh_blob() {
-        if (stream) hipStreamSynchronize(stream);
+        if (stream) (void)hipStreamSynchronize(stream);
         for (auto &pe : pending) {
-            hipEventDestroy(pe.second.first);
-            hipEventDestroy(pe.second.second);
+            (void)hipEventDestroy(pe.second.first);
+            (void)hipEventDestroy(pe.second.second);
         }
-        for (auto ev : event_pool) hipEventDestroy(ev);
-        for (void *p : allocs) hipFree(p);
-        if (ctl_host) hipHostFree(ctl_host);
-        if (stream) hipStreamDestroy(stream);
+        for (auto ev : event_pool) (void)hipEventDestroy(ev);
+        for (void *p : allocs) (void)hipFree(p);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 
     template <typename X>
@@ -756,8 +827,8 @@ struct ffh_blob {
             event_pool.pop_back();
             return e;
         }
-        hipEvent_t e;
-        hipEventCreate(&e);
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
         return e;
     }
     void collect_events(bool wait) {
@@ -767,9 +838,9 @@ struct ffh_blob {
                 keep.push_back(pe);
                 continue;
             }
-            hipEventSynchronize(pe.second.second);
+            (void)hipEventSynchronize(pe.second.second);
             float ms = 0;
-            hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+            (void)hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
             ktime[pe.first] += ms;
             kcount[pe.first] += 1;
             event_pool.push_back(pe.second.first);
@@ -777,41 +848,49 @@ struct ffh_blob {
         }
         pending.swap(keep);
     }
+    // Launches through fn(start, stop); with profiling on, the two events travel in the
+    // dispatch packet itself (hipExtLaunchKernelGGL), so timing adds no extra packets.
     template <typename F>
     int launch(int kid, F &&fn) {
         hipEvent_t a = nullptr, b = nullptr;
         if (profiling) {
             a = get_event();
             b = get_event();
-            hipEventRecord(a, stream);
         }
-        fn();
+        fn(a, b);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(FFH_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
         if (profiling) {
-            hipEventRecord(b, stream);
             pending.push_back({kid, {a, b}});
             if (pending.size() > 4096) collect_events(true);
         }
         return FFH_OK;
     }
 
-    VecArgs vec_args() const {
-        VecArgs v;
+    CgArgs cg_args() const {
+        CgArgs v;
         v.N = N;
         v.f = f;
         v.r = r;
         v.x = x;
-        v.p0 = p0;
-        v.p1 = p1;
+        v.p = p;
         v.q = q;
+        v.z = z;
         v.vel = vel;
         v.dinv = dinv;
-        v.partials = partials;
-        v.ticket = ticket;
+        v.parts_rz = parts;
+        v.parts_rr = parts + 2 * (size_t)kMaxReduceBlocks;
+        v.parts_ff = parts + 3 * (size_t)kMaxReduceBlocks;
+        v.parts_pq = parts + 4 * (size_t)kMaxReduceBlocks;
+        v.P_rhs = P_rhs;
+        v.P_upd = P_upd;
+        v.P_spmv = P_spmv;
+        v.eps2 = eps2;
+        v.maxit = maxit;
         v.ctl = ctl;
         return v;
     }
+    double *parts_energy() const { return parts + 5 * (size_t)kMaxReduceBlocks; }
     SpmvArgs spmv_args() const {
         SpmvArgs m;
         m.slice_off = slice_off;
@@ -834,7 +913,7 @@ struct ffh_blob {
         g.Em = Em;
         g.last_det = last_det;
         g.eforce = eforce;
-        g.eblk = eblk;
+        g.erec = erec;
         g.E = E;
         g.calc_noise = calc_noise;
         g.kT = kT;
@@ -845,38 +924,42 @@ struct ffh_blob {
         g.ctl = ctl;
         return g;
     }
-    int grid(int64_t n) const { return (int)((n + kBlock - 1) / kBlock); }
+    static int grid(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
     int run_element() {
         ElemArgs g = elem_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(0, [&] { hipLaunchKernelGGL((k_element<double, double>), grid(E), kBlock, 0, stream, g); });
-        return launch(0, [&] { hipLaunchKernelGGL((k_element<float, float>), grid(E), kBlock, 0, stream, g); });
+            return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<double, double>), grid(E), kBlock, 0, stream, e0, e1, 0, g); });
+        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), grid(E), kBlock, 0, stream, e0, e1, 0, g); });
     }
     int run_assemble() {
         AsmArgs a;
-        a.eblk = eblk;
-        a.ctb_off = sctb_off;
-        a.ctb = sctb;
-        a.col = slot_col;
-        a.flags = slot_flags;
+        a.erec = erec;
+        a.toff = ut_off;
+        a.twidth = ut_width;
+        a.cnt = ut_cnt;
+        a.ctb = uctb;
+        a.slot = up_slot;
+        a.mirror = up_mirror;
+        a.node = up_node;
+        a.flags = up_flags;
         a.zeta = zeta;
         a.val = val;
         a.dinv = dinv;
-        a.nslots = plan.num_slots;
+        a.nupper = plan.num_upper;
         if (precision == FFH_PRECISION_FP64)
-            return launch(1, [&] {
-                hipLaunchKernelGGL((k_assemble<double, double>), grid(plan.num_slots), kBlock, 0, stream, a);
+            return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
+                hipExtLaunchKernelGGL((k_assemble<double>), grid(plan.num_upper), kBlock, 0, stream, e0, e1, 0, a);
             });
-        return launch(1, [&] {
-            hipLaunchKernelGGL((k_assemble<float, float>), grid(plan.num_slots), kBlock, 0, stream, a);
-        });
+        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_assemble<float>), grid(plan.num_upper), kBlock, 0, stream, e0, e1, 0, a); });
     }
-    RhsArgs rhs_args(bool noise) const {
+    RhsArgs rhs_args(bool noise, const double *f_in) const {
         RhsArgs g;
         g.eforce = eforce;
         g.E = E;
-        g.ctb_off = nctb_off;
+        g.f_in = f_in;
+        g.toff = nt_off;
+        g.cnt = nt_cnt;
         g.ctb = nctb;
         g.zeta = zeta;
         g.pinned = pinned;
@@ -889,56 +972,56 @@ struct ffh_blob {
         g.dt = dt;
         return g;
     }
-    int run_rhs() {
-        RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1);
-        VecArgs v = vec_args();
+    int run_rhs(const double *f_in = nullptr) {
+        RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1, f_in);
+        CgArgs v = cg_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(2, [&] { hipLaunchKernelGGL((k_rhs<double>), grid(N), kBlock, 0, stream, g, v); });
-        return launch(2, [&] { hipLaunchKernelGGL((k_rhs<float>), grid(N), kBlock, 0, stream, g, v); });
+            return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_rhs, 192, 0, stream, e0, e1, 0, g, v); });
+        return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_rhs, 192, 0, stream, e0, e1, 0, g, v); });
     }
-    // Runs CG iterations until converged or failed; polls the control block every chunk.
-    int run_cg() {
-        VecArgs v = vec_args();
+    int launch_spmv(int it) {
+        CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
+        if (precision == FFH_PRECISION_FP64)
+            return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_spmv, 192, 0, stream, e0, e1, 0, m, v, it); });
+        return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_spmv, 192, 0, stream, e0, e1, 0, m, v, it); });
+    }
+    int launch_update(int it) {
+        CgArgs v = cg_args();
+        return launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_upd, kBlock, 0, stream, e0, e1, 0, v, it); });
+    }
+    // NoMassCGSolver::solve's loop: spmv(0), then [update(it), spmv(it+1)] until a spmv kernel
+    // finds convergence (or it == maxit).  The control block is polled once per chunk.
+    int run_cg() {
+        int rc = launch_spmv(0);
+        if (rc) return rc;
         int it = 0;
         for (;;) {
-            int chunk = std::max(1, std::min(cg_chunk, maxit - it + 1));
-            for (int k = 0; k < chunk; k++, it++) {
-                double *pold = (it & 1) ? p1 : p0;
-                double *pnew = (it & 1) ? p0 : p1;
-                int rc;
-                if (precision == FFH_PRECISION_FP64)
-                    rc = launch(3, [&] {
-                        hipLaunchKernelGGL((k_cg_spmv<double>), grid(N), kBlock, 0, stream, m, v, pold, pnew);
-                    });
-                else
-                    rc = launch(3, [&] {
-                        hipLaunchKernelGGL((k_cg_spmv<float>), grid(N), kBlock, 0, stream, m, v, pold, pnew);
-                    });
-                if (rc) return rc;
-                rc = launch(4, [&] { hipLaunchKernelGGL(k_cg_update, grid(N), kBlock, 0, stream, v, pnew); });
-                if (rc) return rc;
+            int chunk = std::max(1, std::min(cg_chunk, maxit - it));
+            for (int k = 0; k < chunk && it < maxit; k++) {
+                if ((rc = launch_update(it))) return rc;
+                it++;
+                if ((rc = launch_spmv(it))) return rc;
             }
             HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
             HIP_OR_FAIL(hipStreamSynchronize(stream));
-            if (ctl_host->done) break;
-            if (it > maxit + 1) break;
+            if (ctl_host->done || it >= maxit) break;
         }
+        if (!ctl_host->done) return fail(FFH_E_HIP, "CG control block not finalised");
         last_iters = ctl_host->iter;
         last_res = ctl_host->res;
-        // next solve: one chunk covering this solve's iterations (+1 slack)
-        cg_chunk = std::max(2, std::min(ctl_host->iter + 1, 64));
+        cg_chunk = std::max(1, std::min(ctl_host->iter, 64));
         if (ctl_host->status != 0)
             return fail(FFH_E_CG, "Conjugate gradient solver: could not converge after " + std::to_string(maxit) +
                                       " iterations");
         return FFH_OK;
     }
     int run_integrate() {
-        VecArgs v = vec_args();
+        CgArgs v = cg_args();
         int adv = plan.draws_per_step > 0 ? 1 : 0;
         Mat3u j1 = plan.step_jump[0], j2 = plan.step_jump[1];
-        return launch(5, [&] {
-            hipLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, v, X, Y, Z, dt, rng_state, j1, j2, adv);
+        return launch(5, [&](hipEvent_t e0, hipEvent_t e1) {
+            hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, X, Y, Z, dt, rng_state, j1, j2, adv);
         });
     }
     int reset_inversions() {
@@ -946,10 +1029,9 @@ struct ffh_blob {
         return FFH_OK;
     }
     int read_inversions(int *ninv) {
-        int h = 0;
         HIP_OR_FAIL(hipMemcpyAsync(&ctl_host->inversions, &ctl->inversions, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_OR_FAIL(hipStreamSynchronize(stream));
-        h = ctl_host->inversions;
+        int h = ctl_host->inversions;
         if (ninv) *ninv = h;
         if (h > 0) return fail(FFH_E_INVERSION, std::to_string(h) + " element(s) have inverted since the last step");
         return FFH_OK;
@@ -1004,8 +1086,12 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     b->N = P.num_linear;
     b->E = P.num_elements;
     b->tsize = precision == FFH_PRECISION_FP64 ? 8 : 4;
-    if (b->grid(b->N) > kMaxReduceBlocks || b->grid(b->E) > kMaxReduceBlocks) {
-        fail(FFH_E_ARG, "blob too large for the reduction buffer");
+    b->P_rhs = ffh_blob::grid(b->N, 64);
+    b->P_spmv = P.num_slices;
+    b->P_upd = std::max(1, std::min(128, ffh_blob::grid(3 * (int64_t)b->N)));
+    b->P_elem = ffh_blob::grid(b->E);
+    if (b->P_rhs > kMaxReduceBlocks || b->P_spmv > kMaxReduceBlocks || b->P_elem > kMaxReduceBlocks) {
+        fail(FFH_E_ARG, "blob too large for the reduction buffers");
         return nullptr;
     }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
@@ -1014,12 +1100,11 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     }
     const int N = b->N, E = b->E;
     int rc = 0;
-    std::vector<double> pos3(P.pos);
     rc |= b->alloc(b->X, 3 * (size_t)N);
     if (!rc) {
         b->Y = b->X + N;
         b->Z = b->X + 2 * (size_t)N;
-        if (hipMemcpy(b->X, pos3.data(), pos3.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = 1;
+        if (hipMemcpy(b->X, P.pos.data(), P.pos.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = 1;
     }
     {
         std::vector<int4> en(E);
@@ -1036,38 +1121,44 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     rc |= b->upload(b->Em, P.matE);
     rc |= b->upload(b->last_det, P.last_det);
     {
-        char *ef = nullptr, *eb = nullptr, *vv = nullptr;
+        char *ef = nullptr, *er = nullptr, *vv = nullptr;
         rc |= b->alloc(ef, 12 * (size_t)E * b->tsize);
-        rc |= b->alloc(eb, 90 * (size_t)E * b->tsize);
+        rc |= b->alloc(er, 20 * (size_t)E * b->tsize);
         rc |= b->alloc(vv, 9 * (size_t)P.num_slots * b->tsize);
         b->eforce = ef;
-        b->eblk = eb;
+        b->erec = er;
         b->val = vv;
+        // padding slots stay zero for good; real slots are rewritten by every assembly
+        if (!rc && hipMemset(vv, 0, 9 * (size_t)P.num_slots * b->tsize) != hipSuccess) rc = 1;
     }
-    rc |= b->upload(b->sctb_off, P.sctb_off);
-    rc |= b->upload(b->sctb, P.sctb);
-    rc |= b->upload(b->nctb_off, P.nctb_off);
-    rc |= b->upload(b->nctb, P.nctb);
+    rc |= b->upload(b->ut_off, P.utile.off);
+    rc |= b->upload(b->ut_width, P.utile.width);
+    rc |= b->upload(b->ut_cnt, P.utile.cnt);
+    rc |= b->upload(b->uctb, P.utile.code);
+    rc |= b->upload(b->nt_off, P.ntile.off);
+    rc |= b->upload(b->nt_cnt, P.ntile.cnt);
+    rc |= b->upload(b->nctb, P.ntile.code);
+    rc |= b->upload(b->up_slot, P.up_slot);
+    rc |= b->upload(b->up_mirror, P.up_mirror);
+    rc |= b->upload(b->up_node, P.up_node);
+    rc |= b->upload(b->up_flags, P.up_flags);
     rc |= b->upload(b->rng_off, P.rng_node_off);
     rc |= b->upload(b->slot_col, P.slot_col);
     rc |= b->upload(b->slice_width, P.slice_width);
     rc |= b->upload(b->slice_off, P.slice_off);
-    rc |= b->upload(b->slot_flags, P.slot_flags);
     rc |= b->upload(b->pinned, P.pinned);
     rc |= b->upload(b->zeta, P.zeta);
     rc |= b->alloc(b->dinv, 3 * (size_t)N);
     rc |= b->alloc(b->f, 3 * (size_t)N);
     rc |= b->alloc(b->r, 3 * (size_t)N);
     rc |= b->alloc(b->x, 3 * (size_t)N);
-    rc |= b->alloc(b->p0, 3 * (size_t)N);
-    rc |= b->alloc(b->p1, 3 * (size_t)N);
+    rc |= b->alloc(b->p, 3 * (size_t)N);
+    rc |= b->alloc(b->z, 3 * (size_t)N);
     rc |= b->alloc(b->q, 3 * (size_t)N);
     rc |= b->alloc(b->vel, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_in, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_out, 3 * (size_t)N);
-    rc |= b->alloc(b->partials, 4 * (size_t)kMaxReduceBlocks);
-    rc |= b->alloc(b->scalar_out, 4);
-    rc |= b->alloc(b->ticket, 1);
+    rc |= b->alloc(b->parts, 6 * (size_t)kMaxReduceBlocks);
     rc |= b->alloc(b->ctl, 1);
     rc |= b->alloc(b->rng_state, 6);
     rc |= b->upload(b->jump, P.jump);
@@ -1080,13 +1171,10 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
         return nullptr;
     }
     CgCtl c{};
-    c.eps2 = epsilon2;
-    c.maxit = max_iterations_cg;
     uint32_t st[6];
     for (int i = 0; i < 6; i++) st[i] = (uint32_t)rng_state[i];
     if (hipMemcpy(b->ctl, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->rng_state, st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(b->ticket, 0, sizeof(unsigned)) != hipSuccess ||
         hipMemset(b->vel, 0, 3 * (size_t)N * sizeof(double)) != hipSuccess ||
         hipMemset(b->x, 0, 3 * (size_t)N * sizeof(double)) != hipSuccess) {
         fail(FFH_E_HIP, "device initialisation failed");
@@ -1158,34 +1246,8 @@ int ffh_blob_solve(ffh_blob *b, double *xio) {
     std::vector<double> fl(3 * (size_t)b->N);
     for (int i = 0; i < b->N; i++)
         for (int c = 0; c < 3; c++) fl[3 * i + c] = xio[3 * (size_t)P.orig_of[i] + c];
-    // Load the RHS as the "element forces" would: run a CG with f given directly.
-    HIP_OR_FAIL(hipMemcpyAsync(b->f, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
-    // reuse k_rhs with an empty gather: emulate by a tiny init on the host side
-    {
-        VecArgs v = b->vec_args();
-        // r = f, x = 0, p = 0, delta and f2
-        HIP_OR_FAIL(hipMemcpyAsync(b->r, b->f, fl.size() * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
-        HIP_OR_FAIL(hipMemsetAsync(b->x, 0, fl.size() * sizeof(double), b->stream));
-        HIP_OR_FAIL(hipMemsetAsync(b->p0, 0, fl.size() * sizeof(double), b->stream));
-        HIP_OR_FAIL(hipMemsetAsync(b->p1, 0, fl.size() * sizeof(double), b->stream));
-        std::vector<double> dinv(fl.size());
-        HIP_OR_FAIL(hipMemcpyAsync(dinv.data(), b->dinv, fl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-        HIP_OR_FAIL(hipStreamSynchronize(b->stream));
-        (void)v;
-        double delta = 0, f2 = 0;
-        for (int i = 0; i < b->N; i++) {
-            double z0 = dinv[3 * i] * fl[3 * i], z1 = dinv[3 * i + 1] * fl[3 * i + 1], z2 = dinv[3 * i + 2] * fl[3 * i + 2];
-            delta += fl[3 * i] * z0 + fl[3 * i + 1] * z1 + fl[3 * i + 2] * z2;
-            f2 += fl[3 * i] * fl[3 * i] + fl[3 * i + 1] * fl[3 * i + 1] + fl[3 * i + 2] * fl[3 * i + 2];
-        }
-        CgCtl c{};
-        c.delta = delta;
-        c.f2 = f2;
-        c.eps2 = b->eps2;
-        c.maxit = b->maxit;
-        c.done = f2 == 0.0 ? 1 : 0;
-        HIP_OR_FAIL(hipMemcpy(b->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
-    }
+    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (int rc0 = b->run_rhs(b->tmp_in)) return rc0;
     int rc = b->run_cg();
     std::vector<double> xl(fl.size());
     HIP_OR_FAIL(hipMemcpyAsync(xl.data(), b->x, xl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -1206,9 +1268,9 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     double *ti = b->tmp_in, *to = b->tmp_out;
     int N = b->N;
     if (b->precision == FFH_PRECISION_FP64)
-        hipLaunchKernelGGL((k_apply<double>), b->grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
+        hipLaunchKernelGGL((k_apply<double>), ffh_blob::grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
     else
-        hipLaunchKernelGGL((k_apply<float>), b->grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
+        hipLaunchKernelGGL((k_apply<float>), ffh_blob::grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
     HIP_OR_FAIL(hipGetLastError());
     HIP_OR_FAIL(hipMemcpyAsync(ol.data(), b->tmp_out, ol.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
@@ -1337,10 +1399,14 @@ int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts) {
 int ffh_blob_strain_energy(ffh_blob *b, double *energy) {
     if (!b || !energy) return fail(FFH_E_ARG, "null argument");
     ElemArgs g = b->elem_args();
-    hipLaunchKernelGGL(k_strain_energy, b->grid(b->E), kBlock, 0, b->stream, g, b->partials, b->ticket, b->scalar_out);
+    hipLaunchKernelGGL(k_strain_energy, b->P_elem, kBlock, 0, b->stream, g, b->parts_energy());
     HIP_OR_FAIL(hipGetLastError());
-    HIP_OR_FAIL(hipMemcpyAsync(energy, b->scalar_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    std::vector<double> h(b->P_elem);
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->parts_energy(), h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *energy = 0.5 * s;
     return FFH_OK;
 }
 

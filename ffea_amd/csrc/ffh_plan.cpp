@@ -87,6 +87,26 @@ void jacobian(const double *x, const double *y, const double *z, const int *n, d
 
 Mat3u mrg_pow(int comp, uint64_t n) { return to32(powm(comp, n)); }
 
+std::string make_tiles(const std::vector<uint32_t> &off, const std::vector<uint32_t> &items, Tiles &t) {
+    const size_t n = off.size() - 1, groups = (n + kSlice - 1) / kSlice;
+    t.off.assign(groups + 1, 0);
+    t.width.assign(groups, 0);
+    t.cnt.assign(n, 0);
+    for (size_t u = 0; u < n; u++) {
+        uint32_t c = off[u + 1] - off[u];
+        if (c > 65535) return "a contribution list exceeds 65535 entries";
+        t.cnt[u] = (uint16_t)c;
+        t.width[u / kSlice] = std::max<uint16_t>(t.width[u / kSlice], (uint16_t)c);
+    }
+    for (size_t g = 0; g < groups; g++) t.off[g + 1] = t.off[g] + (int64_t)t.width[g] * kSlice;
+    t.code.assign((size_t)t.off[groups], 0u);
+    for (size_t u = 0; u < n; u++) {
+        const size_t g = u / kSlice, l = u % kSlice;
+        for (uint32_t k = 0; k < t.cnt[u]; k++) t.code[(size_t)t.off[g] + (size_t)k * kSlice + l] = items[off[u] + k];
+    }
+    return "";
+}
+
 std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
                        const double *material_si, const double *stokes_radius, int num_pinned,
                        const int *pinned, double stokes_visc, int calc_noise, int calc_stokes) {
@@ -248,12 +268,14 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         auto it = std::lower_bound(b, en, c);
         return (it != en && *it == c) ? (int64_t)(it - bcol.begin()) : -1;
     };
+    // contributions per block, ascending element (the source-list order of the reference's
+    // pattern, src/NoMassCGSolver.cpp:93-139): code e << 4 | a << 2 | b
     std::vector<uint32_t> bctb_off((size_t)NB + 1, 0);
     for (int e = 0; e < E; e++)
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++) {
                 int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
-                if (P.pinned[r] || P.pinned[c]) continue;
+                if (P.pinned[r] || P.pinned[c] || r > c) continue;
                 bctb_off[find_block(r, c) + 1]++;
             }
     for (int64_t k = 0; k < NB; k++) bctb_off[k + 1] += bctb_off[k];
@@ -264,10 +286,9 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
             for (int a = 0; a < 4; a++)
                 for (int b = 0; b < 4; b++) {
                     int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
-                    if (P.pinned[r] || P.pinned[c]) continue;
+                    if (P.pinned[r] || P.pinned[c] || r > c) continue;
                     int64_t k = find_block(r, c);
-                    uint32_t code = a <= b ? ((uint32_t)pair_index(a, b) << 1) : (((uint32_t)pair_index(b, a) << 1) | 1u);
-                    bctb[cur[k]++] = ((uint32_t)e << 5) | code;
+                    bctb[cur[k]++] = ((uint32_t)e << 4) | ((uint32_t)a << 2) | (uint32_t)b;
                 }
     }
 
@@ -286,9 +307,8 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     }
     P.num_slots = P.slice_off[P.num_slices];
     if (P.num_slots >= (int64_t)1 << 31) return "matrix pattern too large";
+    std::vector<int64_t> slot_of_block(NB, -1);
     P.slot_col.assign(P.num_slots, 0);
-    P.slot_flags.assign(P.num_slots, 4);
-    P.sctb_off.assign((size_t)P.num_slots + 1, 0);
     for (int s = 0; s < P.num_slices; s++)
         for (int t = 0; t < P.slice_width[s]; t++)
             for (int l = 0; l < kSlice; l++) {
@@ -297,35 +317,49 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
                 if (r < N && t < row_off[r + 1] - row_off[r]) {
                     int64_t k = row_off[r] + t;
                     P.slot_col[slot] = bcol[k];
-                    uint8_t f = 0;
-                    if (bcol[k] == r) f |= 1;
-                    if (P.pinned[r]) f |= 2;
-                    P.slot_flags[slot] = f;
-                    P.sctb_off[slot + 1] = bctb_off[k + 1] - bctb_off[k];
+                    slot_of_block[k] = slot;
                 } else {
                     P.slot_col[slot] = r < N ? r : N - 1;  // zero block on a valid column
                 }
             }
-    for (int64_t k = 0; k < P.num_slots; k++) P.sctb_off[k + 1] += P.sctb_off[k];
-    P.sctb.resize(P.sctb_off[P.num_slots]);
-    for (int s = 0; s < P.num_slices; s++)
-        for (int t = 0; t < P.slice_width[s]; t++)
-            for (int l = 0; l < kSlice; l++) {
-                int64_t slot = P.slice_off[s] + (int64_t)t * kSlice + l;
-                int r = s * kSlice + l;
-                if (r < N && t < row_off[r + 1] - row_off[r]) {
+    // upper-triangle assembly list: diagonal blocks first (long lists), then the off-diagonal
+    // ones, each in SELL slot order, so a wavefront's lists have similar lengths
+    P.up_slot.clear();
+    P.up_mirror.clear();
+    P.up_node.clear();
+    P.up_flags.clear();
+    P.uctb_off.assign(1, 0);
+    P.uctb.clear();
+    for (int pass = 0; pass < 2; pass++)
+        for (int s = 0; s < P.num_slices; s++)
+            for (int t = 0; t < P.slice_width[s]; t++)
+                for (int l = 0; l < kSlice; l++) {
+                    int r = s * kSlice + l;
+                    if (r >= N || t >= row_off[r + 1] - row_off[r]) continue;
                     int64_t k = row_off[r] + t;
-                    std::copy(bctb.begin() + bctb_off[k], bctb.begin() + bctb_off[k + 1],
-                              P.sctb.begin() + P.sctb_off[slot]);
+                    int c = bcol[k];
+                    if (c < r || (pass == 0) != (c == r)) continue;
+                    P.up_slot.push_back((int32_t)slot_of_block[k]);
+                    P.up_mirror.push_back(c == r ? -1 : (int32_t)slot_of_block[find_block(c, r)]);
+                    P.up_node.push_back(r);
+                    P.up_flags.push_back((uint8_t)((c == r ? 1 : 0) | (P.pinned[r] ? 2 : 0)));
+                    P.uctb.insert(P.uctb.end(), bctb.begin() + bctb_off[k], bctb.begin() + bctb_off[k + 1]);
+                    P.uctb_off.push_back((uint32_t)P.uctb.size());
                 }
-            }
+    P.num_upper = (int64_t)P.up_slot.size();
+    {
+        std::string m = make_tiles(P.uctb_off, P.uctb, P.utile);
+        if (!m.empty()) return m;
+        m = make_tiles(P.nctb_off, P.nctb, P.ntile);
+        if (!m.empty()) return m;
+    }
 
     // --- RNG: draws per step and digit jump tables ---
     P.elem_draws = calc_noise == 1 ? 7ull * (uint64_t)E : 0;
     uint64_t node_draws = (calc_noise == 1 && calc_stokes == 1) ? 3ull * (uint64_t)NT : 0;
     P.draws_per_step = P.elem_draws + node_draws;
     if (P.draws_per_step >= (1ull << 32)) return "more than 2^32 random draws per step is not supported";
-    P.jump.resize(4 * 256 * 2);
+    P.jump.resize(4 * 256 * 2 + 64 * 2);
     for (int comp = 0; comp < 2; comp++) {
         uint64_t m = comp ? kM2 : kM1;
         M3 step = base(comp);  // A^(256^d)
@@ -338,6 +372,8 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
             step = acc;  // A^(256^(d+1))
         }
         P.step_jump[comp] = to32(powm(comp, P.draws_per_step));
+        // A^(7 l): the lane offsets of a wavefront of consecutive elements
+        for (int l = 0; l < 64; l++) P.jump[4 * 256 * 2 + 2 * l + comp] = to32(powm(comp, 7ull * (uint64_t)l));
     }
     return "";
 }

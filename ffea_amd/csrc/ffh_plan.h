@@ -24,6 +24,16 @@ struct Mat3u {
     uint32_t m[3][3];
 };
 
+// Per-entry lists laid out in 64-wide tiles: entry u = 64 g + lane finds its k-th item at
+// code[off[g] + 64 k + lane] for k < cnt[u]; width[g] = max cnt over the tile.  Lanes of a
+// wavefront then read one list position with one coalesced load.
+struct Tiles {
+    std::vector<int64_t> off;     // [groups + 1]
+    std::vector<uint16_t> width;  // [groups]
+    std::vector<uint16_t> cnt;    // [entries]
+    std::vector<uint32_t> code;   // [off[groups]]
+};
+
 struct Plan {
     // sizes
     int num_nodes_total = 0;  // N_t
@@ -47,6 +57,7 @@ struct Plan {
     std::vector<int32_t> mid_index, mid_a, mid_b;
     // force gather: per linear node, contributions (e << 2 | a) in ascending element order
     std::vector<uint32_t> nctb_off, nctb;
+    Tiles ntile;  // the same lists, tiled per 64 nodes
     // block pattern in SELL-64 order
     int num_slices = 0;
     int64_t num_blocks = 0;  // real 3x3 blocks
@@ -54,15 +65,25 @@ struct Plan {
     std::vector<int64_t> slice_off;  // [num_slices+1] first slot of each slice
     std::vector<int32_t> slice_width;
     std::vector<int32_t> slot_col;    // [num_slots]
-    std::vector<uint8_t> slot_flags;  // bit0 diagonal block, bit1 identity (pinned row), bit2 padding
-    std::vector<uint32_t> sctb_off;   // [num_slots+1]
-    std::vector<uint32_t> sctb;       // e << 5 | pair << 1 | transpose, ascending e per slot
+    // assembly over the upper triangle (row <= col): each entry fills its SELL slot and,
+    // off the diagonal, the mirrored slot (col,row) with the transposed block
+    int64_t num_upper = 0;
+    std::vector<int32_t> up_slot;     // [num_upper] SELL slot of (row, col)
+    std::vector<int32_t> up_mirror;   // [num_upper] SELL slot of (col, row), -1 on the diagonal
+    std::vector<int32_t> up_node;     // [num_upper] row node (diagonal: its Stokes drag and D^-1)
+    std::vector<uint8_t> up_flags;    // bit0 diagonal, bit1 identity (pinned row)
+    std::vector<uint32_t> uctb_off;   // [num_upper+1]
+    std::vector<uint32_t> uctb;       // e << 4 | a << 2 | b (local row / col node), ascending e
+    Tiles utile;                      // the same lists, tiled per 64 upper entries
     // RNG jump tables: tbl[((d*256 + v)*2 + comp)] = A_comp^(v * 256^d), d=0..3
     std::vector<Mat3u> jump;
     Mat3u step_jump[2];        // A^(draws per step)
     uint64_t draws_per_step = 0;
     uint64_t elem_draws = 0;   // 7*E when calc_noise
 };
+
+// Tiles CSR lists (off[n+1], items); returns "" or an error message.
+std::string make_tiles(const std::vector<uint32_t> &off, const std::vector<uint32_t> &items, Tiles &t);
 
 // Builds the plan; returns "" on success or an error message.
 std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
