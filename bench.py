@@ -183,7 +183,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(independent blobs shard by blob, one large blob by node slabs: see DESIGN.md)",
+              "(independent blobs shard by blob with no communication; one large blob by node slabs with a "
+              "boundary exchange and two scalar all-reduces per CG iteration)",
               file=sys.stderr)
         sys.exit(2)
     m, prm, sc, x0, desc = make_case(a.config)

@@ -1088,7 +1088,7 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     b->tsize = precision == FFH_PRECISION_FP64 ? 8 : 4;
     b->P_rhs = ffh_blob::grid(b->N, 64);
     b->P_spmv = P.num_slices;
-    b->P_upd = std::max(1, std::min(128, ffh_blob::grid(3 * (int64_t)b->N)));
+    b->P_upd = std::max(1, std::min(512, ffh_blob::grid(3 * (int64_t)b->N)));
     b->P_elem = ffh_blob::grid(b->E);
     if (b->P_rhs > kMaxReduceBlocks || b->P_spmv > kMaxReduceBlocks || b->P_elem > kMaxReduceBlocks) {
         fail(FFH_E_ARG, "blob too large for the reduction buffers");
