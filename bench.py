@@ -171,6 +171,26 @@ def _cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
                       f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
 
 
+def copy_calibration(mbytes: int) -> dict:
+    """One device-to-device copy of `mbytes` MiB through the HIP runtime (a plain copy kernel of
+    known size): the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        return {"error": "hipMalloc failed"}
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"bytes": n, "seconds": dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +200,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -202,6 +224,7 @@ def main():
             "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
             "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
     cpu = None if a.no_cpu else cpu_baseline(a.config, m, prm, sc, x0, a.cpu_seconds)
+    calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
     out = {
         "metric": "steps/s at CG eps=1e-6 (FP64 path), 48^3 cube",
         "value": sps,
@@ -226,6 +249,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if calib:
+        out["copy_calibration"] = calib
     print(json.dumps(out), flush=True)
 
 

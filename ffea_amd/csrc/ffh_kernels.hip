@@ -174,11 +174,28 @@ __device__ __forceinline__ void block_partial(double v, double *__restrict__ par
     if (threadIdx.x == 0) parts[blockIdx.x] = t;
 }
 
+// Writes this block's sum of v to parts[idx].
+__device__ __forceinline__ void block_partial_at(double v, double *__restrict__ parts, int idx, double *sh) {
+    double t = block_sum_all(v, sh);
+    if (threadIdx.x == 0) parts[idx] = t;
+}
+
 // Fixed-order total of P partials, identical in every block.
 __device__ __forceinline__ double parts_total(const double *__restrict__ parts, int P, double *sh) {
     double s = 0.0;
     for (int i = threadIdx.x; i < P; i += blockDim.x) s += parts[i];
     return block_sum_all(s, sh);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The gathering
+// kernels remap the dispatch index so that XCD x works on the contiguous logical blocks
+// [x per, (x+1) per): neighbouring rows, elements and records then share one L2.  The grid is
+// launched with 8 per blocks; logical blocks >= n do no work.
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int xcd_grid(int n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+__device__ __forceinline__ int xcd_block(int n) {
+    const int per = (n + kXcds - 1) / kXcds;
+    return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
 }
 
 // CG control block (device memory), written by one thread of the deciding kernel.
@@ -234,7 +251,7 @@ __device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, 
 
 template <typename R, typename T>
 __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
-    const int e = blockIdx.x * kBlock + threadIdx.x;
+    const int e = xcd_block((g.E + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     if (e >= g.E) return;
     const int4 n = g.enodes[e];
     const int nn[4] = {n.x, n.y, n.z, n.w};
@@ -386,9 +403,10 @@ struct AsmArgs {
 //   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
-    const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t u = (int64_t)xcd_block((int)((g.nupper + kBlock - 1) / kBlock)) * kBlock + threadIdx.x;
     if (u >= g.nupper) return;
     const uint8_t fl = g.flags[u];
+    if (fl & 4) return;  // tile padding
     const bool diag = (fl & 1) != 0;
     T acc[9];
 #pragma unroll
@@ -497,7 +515,8 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
     __shared__ double sh[4];
     __shared__ uint32_t rs[6][64];
     const int i = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = blockIdx.x * 64 + lane;
+    const int blk = xcd_block((v.N + 63) / 64);
+    const int n = blk * 64 + lane;
     const bool live = n < v.N;
     // Stokes noise (src/Blob.cpp:3895-3898): draws 3*node + {x,y,z} of the node phase.  Wave 0
     // jumps to the node's first draw; waves 1, 2 step on from the shared state.
@@ -519,7 +538,7 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             const T *ef = reinterpret_cast<const T *>(g.eforce);
             f = 0.0;
             const int cnt = g.cnt[n];
-            const uint32_t *tl = g.ctb + g.toff[blockIdx.x] + lane;
+            const uint32_t *tl = g.ctb + g.toff[blk] + lane;
 #pragma unroll 4
             for (int k = 0; k < cnt; k++) {
                 const uint32_t code = tl[64 * k];
@@ -546,7 +565,7 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         pff = f * f;
     }
     block_partial(prz, v.parts_rz, sh);
-    block_partial(pff, v.parts_ff, sh);
+    block_partial_at(pff, v.parts_ff, blk, sh);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         v.ctl->done = 0;
         v.ctl->status = 0;
@@ -607,11 +626,12 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it) {
         if (blockIdx.x == 0 && threadIdx.x == 0) v.ctl->delta[it & 1] = dnew;
     }
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 64 + lane;
+    const int slice = xcd_block((v.N + 63) / 64);
+    const int row = slice * 64 + lane;
     double part = 0.0;
     if (row < v.N) {
-        const int w = m.slice_width[blockIdx.x];
-        const int64_t base = m.slice_off[blockIdx.x];
+        const int w = m.slice_width[slice];
+        const int64_t base = m.slice_off[slice];
         const T *val = reinterpret_cast<const T *>(m.val);
         const double *__restrict__ z = v.z;
         double acc = 0.0;
@@ -632,7 +652,7 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it) {
         v.q[k] = qn;
         part = pn * qn;
     }
-    block_partial(part, v.parts_pq, sh);
+    block_partial_at(part, v.parts_pq, slice, sh);
 }
 
 // alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
@@ -929,8 +949,8 @@ struct ffh_blob {
     int run_element() {
         ElemArgs g = elem_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<double, double>), grid(E), kBlock, 0, stream, e0, e1, 0, g); });
-        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), grid(E), kBlock, 0, stream, e0, e1, 0, g); });
+            return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<double, double>), xcd_grid(grid(E)), kBlock, 0, stream, e0, e1, 0, g); });
+        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), xcd_grid(grid(E)), kBlock, 0, stream, e0, e1, 0, g); });
     }
     int run_assemble() {
         AsmArgs a;
@@ -949,9 +969,9 @@ struct ffh_blob {
         a.nupper = plan.num_upper;
         if (precision == FFH_PRECISION_FP64)
             return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
-                hipExtLaunchKernelGGL((k_assemble<double>), grid(plan.num_upper), kBlock, 0, stream, e0, e1, 0, a);
+                hipExtLaunchKernelGGL((k_assemble<double>), xcd_grid(grid(plan.num_upper)), kBlock, 0, stream, e0, e1, 0, a);
             });
-        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_assemble<float>), grid(plan.num_upper), kBlock, 0, stream, e0, e1, 0, a); });
+        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_assemble<float>), xcd_grid(grid(plan.num_upper)), kBlock, 0, stream, e0, e1, 0, a); });
     }
     RhsArgs rhs_args(bool noise, const double *f_in) const {
         RhsArgs g;
@@ -1086,8 +1106,8 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     b->N = P.num_linear;
     b->E = P.num_elements;
     b->tsize = precision == FFH_PRECISION_FP64 ? 8 : 4;
-    b->P_rhs = ffh_blob::grid(b->N, 64);
-    b->P_spmv = P.num_slices;
+    b->P_rhs = xcd_grid(ffh_blob::grid(b->N, 64));
+    b->P_spmv = xcd_grid(P.num_slices);
     b->P_upd = std::max(1, std::min(512, ffh_blob::grid(3 * (int64_t)b->N)));
     b->P_elem = ffh_blob::grid(b->E);
     if (b->P_rhs > kMaxReduceBlocks || b->P_spmv > kMaxReduceBlocks || b->P_elem > kMaxReduceBlocks) {

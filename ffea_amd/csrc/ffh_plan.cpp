@@ -322,16 +322,26 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
                     P.slot_col[slot] = r < N ? r : N - 1;  // zero block on a valid column
                 }
             }
-    // upper-triangle assembly list: diagonal blocks first (long lists), then the off-diagonal
-    // ones, each in SELL slot order, so a wavefront's lists have similar lengths
+    // upper-triangle assembly list, slice by slice: the slice's diagonal blocks (long lists, one
+    // tile), then its off-diagonal blocks padded with empty entries to whole 64-wide tiles, so a
+    // tile's lists have similar lengths and consecutive entries share their elements
     P.up_slot.clear();
     P.up_mirror.clear();
     P.up_node.clear();
     P.up_flags.clear();
     P.uctb_off.assign(1, 0);
     P.uctb.clear();
-    for (int pass = 0; pass < 2; pass++)
-        for (int s = 0; s < P.num_slices; s++)
+    auto pad_to_tile = [&]() {
+        while (P.up_slot.size() % kSlice) {
+            P.up_slot.push_back(0);
+            P.up_mirror.push_back(-1);
+            P.up_node.push_back(0);
+            P.up_flags.push_back(4);  // empty entry
+            P.uctb_off.push_back((uint32_t)P.uctb.size());
+        }
+    };
+    for (int s = 0; s < P.num_slices; s++) {
+        for (int pass = 0; pass < 2; pass++) {
             for (int t = 0; t < P.slice_width[s]; t++)
                 for (int l = 0; l < kSlice; l++) {
                     int r = s * kSlice + l;
@@ -346,6 +356,9 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
                     P.uctb.insert(P.uctb.end(), bctb.begin() + bctb_off[k], bctb.begin() + bctb_off[k + 1]);
                     P.uctb_off.push_back((uint32_t)P.uctb.size());
                 }
+            pad_to_tile();
+        }
+    }
     P.num_upper = (int64_t)P.up_slot.size();
     {
         std::string m = make_tiles(P.uctb_off, P.uctb, P.utile);

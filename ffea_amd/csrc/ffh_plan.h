@@ -71,7 +71,7 @@ struct Plan {
     std::vector<int32_t> up_slot;     // [num_upper] SELL slot of (row, col)
     std::vector<int32_t> up_mirror;   // [num_upper] SELL slot of (col, row), -1 on the diagonal
     std::vector<int32_t> up_node;     // [num_upper] row node (diagonal: its Stokes drag and D^-1)
-    std::vector<uint8_t> up_flags;    // bit0 diagonal, bit1 identity (pinned row)
+    std::vector<uint8_t> up_flags;    // bit0 diagonal, bit1 identity (pinned row), bit2 empty (padding)
     std::vector<uint32_t> uctb_off;   // [num_upper+1]
     std::vector<uint32_t> uctb;       // e << 4 | a << 2 | b (local row / col node), ascending e
     Tiles utile;                      // the same lists, tiled per 64 upper entries
