@@ -171,6 +171,21 @@ def _cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
                       f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
 
 
+KERNEL_SYMBOL = {"element": "k_element<{t}, {t}>", "assemble": "k_assemble<{t}>", "rhs": "k_rhs<{t}>",
+                 "cg_spmv": "k_cg_spmv<{t}>", "cg_update": "k_cg_update", "integrate": "k_integrate"}
+
+
+def measured_traffic(cfg, kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (profiles/traffic_<cfg>.json, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{cfg}.json")
+    if not os.path.exists(path):
+        return None
+    t = "double" if precision == "fp64" else "float"
+    rec = json.load(open(path))["kernels"].get(KERNEL_SYMBOL[kernel].format(t=t))
+    return None if rec is None else rec["traffic_bytes"]
+
+
 def copy_calibration(mbytes: int) -> dict:
     """One device-to-device copy of `mbytes` MiB through the HIP runtime (a plain copy kernel of
     known size): the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
@@ -221,7 +236,8 @@ def main():
     dom = max(ktab, key=lambda k: ktab[k]["total_ms"])
     d = ktab[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(a.config, dom, "fp64"),
             "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
     cpu = None if a.no_cpu else cpu_baseline(a.config, m, prm, sc, x0, a.cpu_seconds)
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
