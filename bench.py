@@ -31,9 +31,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    # name: (cells, sigma_h, description)
-    "c0": ((8, 8, 8), "cube 8^3 cells, Kuhn 6-tet split: 729 linear nodes, 3,072 tets"),
-    "c1": ((48, 48, 48), "cube 48^3 cells, Kuhn 6-tet split: 117,649 linear nodes, 663,552 tets"),
+    # name: (cells, jitter (fraction of h), stiff-core factor, description)
+    "c0": ((8, 8, 8), 0.0, 1.0, "cube 8^3 cells, Kuhn 6-tet split: 729 linear nodes, 3,072 tets"),
+    "c1": ((48, 48, 48), 0.0, 1.0, "cube 48^3 cells, Kuhn 6-tet split: 117,649 linear nodes, 663,552 tets"),
+    "c4": ((192, 96, 48), 0.15, 10.0,
+           "box 192x96x48 cells, Kuhn 6-tet split, nodes jittered +-0.15h (seeded, no inverted tets), "
+           "central third along x 10x stiffer (shear and bulk moduli): 917,329 linear nodes, 5,308,416 tets"),
 }
 
 
@@ -43,8 +46,10 @@ def log(msg):
 
 def make_case(cfg):
     from ffea_amd import mesh as M, units as U
-    cells, desc = CONFIGS[cfg]
-    m = M.box_mesh(*cells)
+    cells, jitter, core, desc = CONFIGS[cfg]
+    m = M.box_mesh(*cells, jitter=jitter, seed=4)
+    if core != 1.0:
+        M.stiff_core(m, axis=0, factor=core)
     x0 = M.perturb(m, 0.02, seed=1)
     prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
     return m, prm, U.blob_scale(1e-9), x0, desc
@@ -242,7 +247,7 @@ def main():
     cpu = None if a.no_cpu else cpu_baseline(a.config, m, prm, sc, x0, a.cpu_seconds)
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
     out = {
-        "metric": "steps/s at CG eps=1e-6 (FP64 path), 48^3 cube",
+        "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,
