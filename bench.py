@@ -1,6 +1,6 @@
 """Benchmark of the massless FFEA time step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c0|c2|c4] [--no-cpu]
 
 Workload (default "c1"): one dynamic blob, a cube of 48^3 unit cells (h = 1 nm) split 6 tets
 per cell (Kuhn), 117,649 linear nodes / 912,673 nodes with midpoints / 663,552 tets; material
@@ -9,6 +9,11 @@ Stokes radius 0.5 nm on linear nodes, solvent viscosity 1e-3 Pa s; T = 300 K, dt
 Jacobi-PCG to relative eps = 1e-6, thermal noise on (RngStream seeded by rng_seed 44),
 initial positions rest + N(0, (0.02 h)^2).  A step = one World::run iteration: element phase,
 assembly, force gather + noise, CG solve, integration -- all on the GPU, inputs resident in HBM.
+
+Other workloads: "c0" (8^3 cells), "c4" (192x96x48 jittered box with a 10x stiffer core) and
+"c2", a batch of 256 independent blobs of uneven size (box meshes with nx, ny, nz uniform in
+[6, 32], nodes jittered +-0.15 h, every material parameter scaled by a per-blob log-uniform
+factor in [0.5, 2]) stepped together: one launch per phase for all blobs.
 
 `value` is FP64-path steps/s (the reference's arithmetic); the mixed path (FP32 element,
 assembly and SpMV storage) is reported beside it.  Kernel durations come from HIP events the
@@ -31,8 +36,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    # name: (cells, jitter (fraction of h), stiff-core factor, description)
+    # name: (cells, jitter (fraction of h), stiff-core factor, description); cells None: batch
     "c0": ((8, 8, 8), 0.0, 1.0, "cube 8^3 cells, Kuhn 6-tet split: 729 linear nodes, 3,072 tets"),
+    "c2": (None, 0.15, 1.0,
+           "batch of 256 blobs, box meshes with nx, ny, nz uniform in [6, 32] (seeded), Kuhn 6-tet split, "
+           "nodes jittered +-0.15h (no inverted tets), each material parameter scaled by a per-blob "
+           "log-uniform factor in [0.5, 2]: 2,111,146 linear nodes, 10,886,874 tets"),
     "c1": ((48, 48, 48), 0.0, 1.0, "cube 48^3 cells, Kuhn 6-tet split: 117,649 linear nodes, 663,552 tets"),
     "c4": ((192, 96, 48), 0.15, 10.0,
            "box 192x96x48 cells, Kuhn 6-tet split, nodes jittered +-0.15h (seeded, no inverted tets), "
@@ -45,14 +54,24 @@ def log(msg):
 
 
 def make_case(cfg):
+    """Meshes (one per blob), parameters, scale, initial positions and description."""
     from ffea_amd import mesh as M, units as U
     cells, jitter, core, desc = CONFIGS[cfg]
+    prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
+    if cells is None:
+        dims = np.random.default_rng(2).integers(6, 33, size=(256, 3))
+        ms, xs = [], []
+        for k, d in enumerate(dims):
+            m = M.box_mesh(*map(int, d), jitter=jitter, seed=1000 + k)
+            M.scale_material(m, 0.5, 2.0, seed=2000 + k)
+            ms.append(m)
+            xs.append(M.perturb(m, 0.02, seed=3000 + k))
+        return ms, prm, U.blob_scale(1e-9), xs, desc
     m = M.box_mesh(*cells, jitter=jitter, seed=4)
     if core != 1.0:
         M.stiff_core(m, axis=0, factor=core)
     x0 = M.perturb(m, 0.02, seed=1)
-    prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
-    return m, prm, U.blob_scale(1e-9), x0, desc
+    return [m], prm, U.blob_scale(1e-9), [x0], desc
 
 
 def algorithmic_bytes(kernel, sizes, bm):
@@ -79,14 +98,18 @@ def algorithmic_bytes(kernel, sizes, bm):
     }[kernel]
 
 
-def run_gpu(m, prm, sc, x0, precision, steps, warmup):
+def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
     """Warm-up, then two timed regions of `steps` steps: the first without per-kernel events
     (the throughput), the second with start/stop events carried in every dispatch packet on
     the blob's stream (the per-kernel durations)."""
-    from ffea_amd.blob import Blob, package_seed
+    from ffea_amd.blob import Batch, Blob, package_seed
     t0 = time.time()
-    b = Blob(m, prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision, pos=x0)
-    log(f"[{precision}] blob created in {time.time() - t0:.1f}s: {b.sizes()}")
+    if len(ms) == 1:
+        b = Blob(ms[0], prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision, pos=xs[0])
+    else:
+        b = Batch(ms, prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision)
+        b.set_positions(np.concatenate(xs) * sc)
+    log(f"[{precision}] {len(ms)} blob(s) created in {time.time() - t0:.1f}s: {b.sizes()}")
     b.step(warmup)
     b.synchronize()
     iters = []
@@ -122,14 +145,14 @@ def kernel_table(kt, sizes, bm):
     return out
 
 
-def cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
+def cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     """Runs _cpu_baseline in a thread with a 2 GiB stack: the reference's SpMV keeps two
     3*N_t-double arrays on the stack (src/SparseMatrixFixedPattern.cpp:153-154), 44 MB at c1."""
     import threading
     res = {}
     old = threading.stack_size(2 << 30)
     try:
-        t = threading.Thread(target=lambda: res.update(out=_cpu_baseline(cfg, m, prm, sc, x0, max_seconds)))
+        t = threading.Thread(target=lambda: res.update(out=_cpu_baseline(cfg, ms, prm, sc, xs, max_seconds)))
         t.start()
         t.join()
     finally:
@@ -137,9 +160,11 @@ def cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
     return res.get("out")
 
 
-def _cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
+def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     """The reference's own compiled objects (oracle/_ref, -O3 -ffast-math, OpenMP within the
-    blob) on this host, falling back to the FP64 restatement; a bounded sample of steps."""
+    blob) on this host, falling back to the FP64 restatement; a bounded sample of steps.  For a
+    batch, the sample is the first blobs up to 1/8 of the elements, and the rate is scaled by
+    the element ratio (the reference's step time is linear in the element count)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     kind, lib = None, None
@@ -155,23 +180,31 @@ def _cpu_baseline(cfg, m, prm, sc, x0, max_seconds=20.0):
     seed = lib.package_seed(prm.rng_seed)
     lib.set_package_seed(seed)
     s = lib.stream(per_thread=True) if kind == "reference" else lib.stream()
+    etot = sum(m.num_elements for m in ms)
+    nb, esub = 0, 0
+    while nb < len(ms) and (nb == 0 or esub < etot / 8):
+        esub += ms[nb].num_elements
+        nb += 1
     t0 = time.time()
-    blob = lib.blob(m, prm, sc, pos=x0)
+    blobs = [lib.blob(m, prm, sc, pos=x) for m, x in zip(ms[:nb], xs[:nb])]
     setup = time.time() - t0
     log(f"[cpu {kind}] setup {setup:.1f}s")
-    assert pyoracle.world_step(lib, [blob], s) == 0  # warm-up step
+    assert pyoracle.world_step(lib, blobs, s) == 0  # warm-up step
     n, t0 = 0, time.perf_counter()
     while True:
-        assert pyoracle.world_step(lib, [blob], s) == 0
+        assert pyoracle.world_step(lib, blobs, s) == 0
         n += 1
         el = time.perf_counter() - t0
         if el > max_seconds or n >= 50:
             break
-    blob.close()
+    for b in blobs:
+        b.close()
     del s
-    log(f"[cpu {kind}] {n} steps in {el:.2f}s")
-    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": kind,
-            "sample": f"{n} noise-on steps of the same {cfg} workload after 1 warm-up step "
+    log(f"[cpu {kind}] {n} steps of {nb} blob(s) in {el:.2f}s")
+    part = "" if nb == len(ms) else (f" restricted to its first {nb} of {len(ms)} blobs ({esub} of {etot} tets), "
+                                     f"rate scaled by {esub}/{etot}")
+    return {"value": n / el * esub / etot, "unit": "steps/s", "cores": cores, "kind": kind,
+            "sample": f"{n} noise-on steps of the same {cfg} workload{part} after 1 warm-up step "
                       f"({'reference objects -O3 -ffast-math, OpenMP within blob' if kind == 'reference' else 'FP64 restatement, 1 thread'}); "
                       f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
 
@@ -229,13 +262,13 @@ def main():
               "boundary exchange and two scalar all-reduces per CG iteration)",
               file=sys.stderr)
         sys.exit(2)
-    m, prm, sc, x0, desc = make_case(a.config)
+    ms, prm, sc, xs, desc = make_case(a.config)
     log(f"workload {a.config}: {desc}")
-    sps, wall, kt, sizes, iters, prof_step = run_gpu(m, prm, sc, x0, "fp64", a.steps, a.warmup)
+    sps, wall, kt, sizes, iters, prof_step = run_gpu(ms, prm, sc, xs, "fp64", a.steps, a.warmup)
     ktab = kernel_table(kt, sizes, 8)
     mixed = None
     if not a.no_mixed:
-        sps_m, wall_m, kt_m, sizes_m, iters_m, prof_m = run_gpu(m, prm, sc, x0, "mixed", a.steps, a.warmup)
+        sps_m, wall_m, kt_m, sizes_m, iters_m, prof_m = run_gpu(ms, prm, sc, xs, "mixed", a.steps, a.warmup)
         mixed = {"steps_per_s": sps_m, "ns_per_day": sps_m * 1e-5 * 86400, "ms_per_step": 1e3 * wall_m / a.steps,
                  "cg_iters_mean": iters_m, "kernels": kernel_table(kt_m, sizes_m, 4)}
     dom = max(ktab, key=lambda k: ktab[k]["total_ms"])
@@ -244,7 +277,7 @@ def main():
             "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS,
             "traffic": measured_traffic(a.config, dom, "fp64"),
             "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
-    cpu = None if a.no_cpu else cpu_baseline(a.config, m, prm, sc, x0, a.cpu_seconds)
+    cpu = None if a.no_cpu else cpu_baseline(a.config, ms, prm, sc, xs, a.cpu_seconds)
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
     out = {
         "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
@@ -259,7 +292,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: seeded Kuhn box mesh, seeded Gaussian initial displacement, RngStream noise",
-        "config": {"workload": f"{a.config}: {desc}; eps 1e-6, noise on, dt 1e-14 s",
+        "config": {"workload": f"{a.config}: {desc}; eps 1e-6, noise on, dt 1e-14 s", "blobs": len(ms),
                    "linear_nodes": sizes["num_linear"], "elements": sizes["num_elements"],
                    "blocks": sizes["num_blocks"], "parallelism": "single GPU"},
         "ns_per_day": sps * 1e-5 * 86400,

@@ -32,7 +32,8 @@ EXPORTS = (
     "ffh_blob_get_positions", "ffh_blob_get_velocities", "ffh_blob_get_rhs",
     "ffh_blob_get_rng_state", "ffh_blob_set_rng_state", "ffh_blob_last_cg", "ffh_blob_sizes",
     "ffh_blob_stream", "ffh_blob_synchronize", "ffh_blob_set_profiling",
-    "ffh_blob_kernel_times", "ffh_blob_strain_energy",
+    "ffh_blob_kernel_times", "ffh_blob_strain_energy", "ffh_batch_create",
+    "ffh_batch_num_blobs", "ffh_batch_last_cg",
 )
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -72,6 +73,12 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_create.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, _ip, C.c_double,
                                   C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                   _u64p, C.c_int]
+    L.ffh_batch_create.restype = C.c_void_p
+    L.ffh_batch_create.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _dp, _dp, _ip, _ip, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   _u64p, C.c_int]
+    L.ffh_batch_num_blobs.argtypes = [C.c_void_p]
+    L.ffh_batch_last_cg.argtypes = [C.c_void_p, _ip, _dp, _ip]
     L.ffh_blob_destroy.argtypes = [C.c_void_p]
     L.ffh_blob_destroy.restype = None
     L.ffh_blob_update_internal_forces.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
@@ -95,7 +102,7 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_strain_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     for n in EXPORTS:
         if n not in ("ffh_last_error", "ffh_version", "ffh_blob_create", "ffh_blob_destroy",
-                     "ffh_blob_stream"):
+                     "ffh_blob_stream", "ffh_batch_create"):
             getattr(L, n).restype = C.c_int
     if path is None:
         _lib = L

@@ -167,6 +167,67 @@ class Blob:
         return e.value
 
 
+class Batch(Blob):
+    """Several blobs of one World stepped together on the GPU (ffh_batch_create): one launch
+    per phase for all of them, one RngStream in World::run's draw order (the element phases
+    of every blob, then their position phases, src/World.cpp:2020-2041).
+
+    meshes: one mesh per blob (as for Blob); scales: one blob scale each (or one for all).
+    Node arrays (positions, velocities, rhs, solve/apply_matrix vectors) are the blobs'
+    arrays concatenated in blob order; `split` cuts such an array back into blobs.
+    """
+
+    def __init__(self, meshes, params: SimParams, scales, rng_state=None, precision="fp64"):
+        self._L = _lib.load()
+        meshes = list(meshes)
+        nb = len(meshes)
+        if np.ndim(scales) == 0:
+            scales = [float(scales)] * nb
+        if len(scales) != nb or nb == 0:
+            raise ValueError("need one scale per blob and at least one blob")
+        nn = np.array([m.pos.shape[0] for m in meshes], np.int32)
+        ne = np.array([m.topo.shape[0] for m in meshes], np.int32)
+        pins = [np.asarray(getattr(m, "pinned", np.zeros(0)), np.int32).ravel() for m in meshes]
+        npin = np.array([p.size for p in pins], np.int32)
+        pos = np.ascontiguousarray(np.concatenate(
+            [np.asarray(m.pos, np.float64) * sc for m, sc in zip(meshes, scales)])).ravel()
+        topo = np.ascontiguousarray(np.concatenate([m.topo for m in meshes]), np.int32).ravel()
+        mat = np.ascontiguousarray(np.concatenate([m.material for m in meshes]), np.float64).ravel()
+        rad = np.ascontiguousarray(np.concatenate(
+            [np.asarray(m.stokes_radius, np.float64) * sc for m, sc in zip(meshes, scales)]))
+        pin = np.ascontiguousarray(np.concatenate(pins) if npin.sum() else np.zeros(1), np.int32)
+        if rng_state is None:
+            rng_state = package_seed(params.rng_seed)
+        st = np.ascontiguousarray(rng_state, np.uint64)
+        self.num_blobs = nb
+        self.num_nodes = int(nn.sum())
+        self.num_elements = int(ne.sum())
+        self.node_off = np.concatenate([[0], np.cumsum(nn)]).astype(np.int64)
+        self.params = params
+        self._h = self._L.ffh_batch_create(
+            nb, nn, ne, pos, topo, mat, rad, npin, pin, params.dt, params.kT, params.epsilon2,
+            params.max_iterations_cg, params.calc_noise, params.calc_stokes, params.stokes_visc,
+            st, PRECISIONS[precision])
+        if not self._h:
+            raise _lib.FFHError(-1, _lib.last_error())
+        self.precision = precision
+
+    def split(self, arr) -> list:
+        """Cut a concatenated per-node array ([N_t,...] or [3 N_t]) into per-blob pieces."""
+        a = np.asarray(arr)
+        if a.ndim == 1:
+            a = a.reshape(-1, 3)
+        return [a[self.node_off[i]:self.node_off[i + 1]] for i in range(self.num_blobs)]
+
+    def last_cg_per_blob(self):
+        """(iterations, residual2, status) arrays of the last solve, one entry per blob."""
+        it = np.zeros(self.num_blobs, np.int32)
+        res = np.zeros(self.num_blobs, np.float64)
+        stat = np.zeros(self.num_blobs, np.int32)
+        _lib.check(self._L.ffh_batch_last_cg(self._h, it, res, stat))
+        return it, res, stat
+
+
 class GpuNoMassCGSolver:
     """The Solver interface (include/Solver.h:30-55) over the GPU blob.
 

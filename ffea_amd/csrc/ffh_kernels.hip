@@ -50,7 +50,6 @@ int fail(int code, const std::string &msg) {
 
 constexpr int kBlock = 256;
 constexpr int kLaneTable = 4 * 256 * 2;  // jump[kLaneTable + 2 l + comp] = A_comp^(7 l), l < 64
-constexpr int kMaxReduceBlocks = 1 << 16;
 
 // ----------------------------------------------------------------------------------------
 // MRG32k3a on the device: exact integer arithmetic mod m = 2^32 - c.
@@ -180,10 +179,10 @@ __device__ __forceinline__ void block_partial_at(double v, double *__restrict__ 
     if (threadIdx.x == 0) parts[idx] = t;
 }
 
-// Fixed-order total of P partials, identical in every block.
-__device__ __forceinline__ double parts_total(const double *__restrict__ parts, int P, double *sh) {
+// Fixed-order total of the partials [i0, i1): one blob's slices.
+__device__ __forceinline__ double parts_range(const double *__restrict__ parts, int i0, int i1, double *sh) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) s += parts[i];
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) s += parts[i];
     return block_sum_all(s, sh);
 }
 
@@ -483,14 +482,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
 // per block (wave = component).
 // ----------------------------------------------------------------------------------------
 struct CgArgs {
-    int N;
+    int N, nslices;
     double *f, *r, *x, *p, *q, *z, *vel;
     const double *dinv;
-    double *parts_rz, *parts_rr, *parts_ff, *parts_pq;
-    int P_rhs, P_upd, P_spmv;
+    double *parts_rz, *parts_rr, *parts_ff, *parts_pq;  // one partial per slice
+    const int32_t *slice_blob, *blob_slice_off;
     double eps2;
     int maxit;
-    CgCtl *ctl;
+    CgCtl *ctl;  // one per blob
 };
 
 struct RhsArgs {
@@ -515,11 +514,13 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
     __shared__ double sh[4];
     __shared__ uint32_t rs[6][64];
     const int i = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int blk = xcd_block((v.N + 63) / 64);
+    const int blk = xcd_block(v.nslices);
+    if (blk >= v.nslices) return;
     const int n = blk * 64 + lane;
-    const bool live = n < v.N;
+    const bool live = !g.pinned[n] || g.f_in;
     // Stokes noise (src/Blob.cpp:3895-3898): draws 3*node + {x,y,z} of the node phase.  Wave 0
-    // jumps to the node's first draw; waves 1, 2 step on from the shared state.
+    // jumps to the node's first draw; waves 1, 2 step on from the shared state.  Pinned rows
+    // are zeroed afterwards in the reference, so their draws are skipped (not evaluated).
     const bool noise = g.noise && !g.f_in;
     if (noise) {
         if (i == 0 && live) {
@@ -529,48 +530,43 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         }
         __syncthreads();
     }
-    double prz = 0.0, pff = 0.0;
-    if (live) {
-        double f;
-        if (g.f_in) {
-            f = g.f_in[3 * (size_t)n + i];
-        } else {
-            const T *ef = reinterpret_cast<const T *>(g.eforce);
-            f = 0.0;
-            const int cnt = g.cnt[n];
-            const uint32_t *tl = g.ctb + g.toff[blk] + lane;
+    double f = 0.0;
+    if (g.f_in) {
+        f = g.f_in[3 * (size_t)n + i];
+    } else if (live) {
+        const T *ef = reinterpret_cast<const T *>(g.eforce);
+        const int cnt = g.cnt[n];
+        const uint32_t *tl = g.ctb + g.toff[blk] + lane;
 #pragma unroll 4
-            for (int k = 0; k < cnt; k++) {
-                const uint32_t code = tl[64 * k];
-                f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
-            }
-            if (noise) {
-                Stream6 st;
-#pragma unroll
-                for (int k = 0; k < 6; k++) st.s[k] = rs[k][lane];
-                for (int k = 0; k < i; k++) (void)rng_u01(st);
-                f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
-            }
-            if (g.pinned[n]) f = 0;
+        for (int k = 0; k < cnt; k++) {
+            const uint32_t code = tl[64 * k];
+            f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
         }
-        const size_t k = 3 * (size_t)n + i;
-        const double z = v.dinv[k] * f;
-        v.f[k] = f;
-        v.r[k] = f;
-        v.x[k] = 0.0;
-        v.p[k] = 0.0;
-        v.q[k] = 0.0;
-        v.z[k] = z;
-        prz = f * z;
-        pff = f * f;
+        if (noise) {
+            Stream6 st;
+#pragma unroll
+            for (int k = 0; k < 6; k++) st.s[k] = rs[k][lane];
+            for (int k = 0; k < i; k++) (void)rng_u01(st);
+            f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
+        }
     }
-    block_partial(prz, v.parts_rz, sh);
-    block_partial_at(pff, v.parts_ff, blk, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        v.ctl->done = 0;
-        v.ctl->status = 0;
-        v.ctl->iter = 0;
-        v.ctl->res = 0.0;
+    const size_t k = 3 * (size_t)n + i;
+    const double z = v.dinv[k] * f;
+    v.f[k] = f;
+    v.r[k] = f;
+    v.x[k] = 0.0;
+    v.p[k] = 0.0;
+    v.q[k] = 0.0;
+    v.z[k] = z;
+    block_partial_at(f * z, v.parts_rz, blk, sh);
+    block_partial_at(f * f, v.parts_ff, blk, sh);
+    const int b = v.slice_blob[blk];
+    if (blk == v.blob_slice_off[b] && threadIdx.x == 0) {
+        CgCtl *c = &v.ctl[b];
+        c->done = 0;
+        c->status = 0;
+        c->iter = 0;
+        c->res = 0.0;
     }
 }
 
@@ -589,92 +585,94 @@ struct SpmvArgs {
 // src/mat_vec_fns.cpp:265-275); q = K p, formed as K z + beta q_old so that only z is gathered
 // (K is applied block by block, each scalar row summed in ascending column order like
 // SparseMatrixFixedPattern::apply, src/SparseMatrixFixedPattern.cpp:148-191); p.q partials.
-// One block = one 64-row slice; wave w computes component w of every row.
+// One block = one 64-row slice; wave w computes component w of every row.  Every scalar is
+// per blob: the blob's slices are a contiguous range of partials.
 template <typename T>
 __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it) {
     __shared__ double sh[4];
-    if (v.ctl->done) return;
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
+    const bool decider = slice == s0 && threadIdx.x == 0;
     double beta = 0.0;
     if (it == 0) {
-        const double f2 = parts_total(v.parts_ff, v.P_rhs, sh);
-        const double d0 = parts_total(v.parts_rz, v.P_rhs, sh);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            v.ctl->f2 = f2;
-            v.ctl->delta[0] = d0;
+        const double f2 = parts_range(v.parts_ff, s0, s1, sh);
+        const double d0 = parts_range(v.parts_rz, s0, s1, sh);
+        if (decider) {
+            ctl->f2 = f2;
+            ctl->delta[0] = d0;
             if (f2 == 0.0) {  // zero force: zero velocity
-                v.ctl->done = 1;
-                v.ctl->status = 0;
-                v.ctl->iter = 0;
+                ctl->done = 1;
+                ctl->status = 0;
+                ctl->iter = 0;
             }
         }
         if (f2 == 0.0) return;
     } else {
-        const double f2 = v.ctl->f2;
-        const double r2 = parts_total(v.parts_rr, v.P_upd, sh);
+        const double f2 = ctl->f2;
+        const double r2 = parts_range(v.parts_rr, s0, s1, sh);
         const double res = r2 / f2;
         if (res < v.eps2 || it >= v.maxit) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                v.ctl->done = 1;
-                v.ctl->status = res < v.eps2 ? 0 : -1;
-                v.ctl->iter = it;
-                v.ctl->res = res;
+            if (decider) {
+                ctl->done = 1;
+                ctl->status = res < v.eps2 ? 0 : -1;
+                ctl->iter = it;
+                ctl->res = res;
             }
             return;
         }
-        const double dnew = parts_total(v.parts_rz, v.P_upd, sh);
-        beta = dnew / v.ctl->delta[(it - 1) & 1];
-        if (blockIdx.x == 0 && threadIdx.x == 0) v.ctl->delta[it & 1] = dnew;
+        const double dnew = parts_range(v.parts_rz, s0, s1, sh);
+        beta = dnew / ctl->delta[(it - 1) & 1];
+        if (decider) ctl->delta[it & 1] = dnew;
     }
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int slice = xcd_block((v.N + 63) / 64);
     const int row = slice * 64 + lane;
-    double part = 0.0;
-    if (row < v.N) {
-        const int w = m.slice_width[slice];
-        const int64_t base = m.slice_off[slice];
-        const T *val = reinterpret_cast<const T *>(m.val);
-        const double *__restrict__ z = v.z;
-        double acc = 0.0;
+    const int w = m.slice_width[slice];
+    const int64_t base = m.slice_off[slice];
+    const T *val = reinterpret_cast<const T *>(m.val);
+    const double *__restrict__ z = v.z;
+    double acc = 0.0;
 #pragma unroll 4
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = m.col[tile + lane];
-            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
-            const T v0 = vt[0], v1 = vt[64], v2 = vt[128];
-            acc += (double)v0 * z[3 * c + 0];
-            acc += (double)v1 * z[3 * c + 1];
-            acc += (double)v2 * z[3 * c + 2];
-        }
-        const size_t k = 3 * (size_t)row + ci;
-        const double pn = z[k] + beta * v.p[k];
-        const double qn = acc + beta * v.q[k];
-        v.p[k] = pn;
-        v.q[k] = qn;
-        part = pn * qn;
+    for (int t = 0; t < w; t++) {
+        const int64_t tile = base + (int64_t)t * 64;
+        const int c = m.col[tile + lane];
+        const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
+        const T v0 = vt[0], v1 = vt[64], v2 = vt[128];
+        acc += (double)v0 * z[3 * c + 0];
+        acc += (double)v1 * z[3 * c + 1];
+        acc += (double)v2 * z[3 * c + 2];
     }
-    block_partial_at(part, v.parts_pq, slice, sh);
+    const size_t k = 3 * (size_t)row + ci;
+    const double pn = z[k] + beta * v.p[k];
+    const double qn = acc + beta * v.q[k];
+    v.p[k] = pn;
+    v.q[k] = qn;
+    block_partial_at(pn * qn, v.parts_pq, slice, sh);
 }
 
 // alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
 // z = D^-1 r (parallel_apply_preconditioner, src/NoMassCGSolver.cpp:278-292); partials of r.r
-// and r.z.
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs v, int it) {
+// and r.z.  One block = the 3 x 64 scalars of one slice.
+__global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     __shared__ double sh[4];
-    if (v.ctl->done) return;
-    const double alpha = v.ctl->delta[it & 1] / parts_total(v.parts_pq, v.P_spmv, sh);
-    double prr = 0.0, prz = 0.0;
-    const int n3 = 3 * v.N;
-    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n3; k += gridDim.x * kBlock) {
-        v.x[k] += alpha * v.p[k];
-        const double r = v.r[k] + (-alpha) * v.q[k];
-        v.r[k] = r;
-        const double z = v.dinv[k] * r;
-        v.z[k] = z;
-        prr += r * r;
-        prz += r * z;
-    }
-    block_partial(prr, v.parts_rr, sh);
-    block_partial(prz, v.parts_rz, sh);
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const double alpha =
+        ctl->delta[it & 1] / parts_range(v.parts_pq, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
+    const size_t k = (size_t)slice * 192 + threadIdx.x;
+    v.x[k] += alpha * v.p[k];
+    const double r = v.r[k] + (-alpha) * v.q[k];
+    v.r[k] = r;
+    const double z = v.dinv[k] * r;
+    v.z[k] = z;
+    block_partial_at(r * r, v.parts_rr, slice, sh);
+    block_partial_at(r * z, v.parts_rz, slice, sh);
 }
 
 // euler_integrate (src/Blob.cpp:3941-3953): vel = v; pos += v * dt.  Skipped after a CG
@@ -689,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double *X, doubl
         matvec<ffh::kM2, kC2>(j2, &s[3]);
         for (int k = 0; k < 6; k++) rng_state[k] = s[k];
     }
-    if (n >= v.N || v.ctl->status != 0) return;
+    if (n >= v.N || v.ctl[v.slice_blob[n >> 6]].status != 0) return;
     const double x0 = v.x[3 * n], x1 = v.x[3 * n + 1], x2 = v.x[3 * n + 2];
     v.vel[3 * n] = x0;
     v.vel[3 * n + 1] = x1;
@@ -797,11 +795,14 @@ struct ffh_blob {
     double *zeta = nullptr, *dinv = nullptr;
     double *f = nullptr, *r = nullptr, *x = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *vel = nullptr;
     double *parts = nullptr, *tmp_in = nullptr, *tmp_out = nullptr;
-    CgCtl *ctl = nullptr;
+    size_t pstride = 0;  // partials per reduction: max(slices, element blocks)
+    int32_t *slice_blob = nullptr, *blob_slice_off = nullptr;
+    int nblobs = 1;
+    CgCtl *ctl = nullptr;       // [nblobs]; ctl[0].inversions counts the whole batch
     CgCtl *ctl_host = nullptr;  // pinned host mirror
     uint32_t *rng_state = nullptr;
     Mat3u *jump = nullptr;
-    int P_rhs = 0, P_upd = 0, P_spmv = 0, P_elem = 0;
+    int P_slice = 0, P_elem = 0;
     // profiling
     bool profiling = false;
     double ktime[FFH_NUM_KERNELS] = {0};
@@ -810,6 +811,9 @@ struct ffh_blob {
     std::vector<hipEvent_t> event_pool;
     int last_iters = 0;
     double last_res = 0;
+    std::vector<int> blob_iters;
+    std::vector<double> blob_res;
+    std::vector<int> blob_status;
     int cg_chunk = 16;
 
     ~ffh_blob() {
@@ -890,6 +894,7 @@ struct ffh_blob {
     CgArgs cg_args() const {
         CgArgs v;
         v.N = N;
+        v.nslices = plan.num_slices;
         v.f = f;
         v.r = r;
         v.x = x;
@@ -899,18 +904,17 @@ struct ffh_blob {
         v.vel = vel;
         v.dinv = dinv;
         v.parts_rz = parts;
-        v.parts_rr = parts + 2 * (size_t)kMaxReduceBlocks;
-        v.parts_ff = parts + 3 * (size_t)kMaxReduceBlocks;
-        v.parts_pq = parts + 4 * (size_t)kMaxReduceBlocks;
-        v.P_rhs = P_rhs;
-        v.P_upd = P_upd;
-        v.P_spmv = P_spmv;
+        v.parts_rr = parts + 1 * pstride;
+        v.parts_ff = parts + 2 * pstride;
+        v.parts_pq = parts + 3 * pstride;
+        v.slice_blob = slice_blob;
+        v.blob_slice_off = blob_slice_off;
         v.eps2 = eps2;
         v.maxit = maxit;
         v.ctl = ctl;
         return v;
     }
-    double *parts_energy() const { return parts + 5 * (size_t)kMaxReduceBlocks; }
+    double *parts_energy() const { return parts + 4 * pstride; }
     SpmvArgs spmv_args() const {
         SpmvArgs m;
         m.slice_off = slice_off;
@@ -996,26 +1000,29 @@ struct ffh_blob {
         RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1, f_in);
         CgArgs v = cg_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_rhs, 192, 0, stream, e0, e1, 0, g, v); });
-        return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_rhs, 192, 0, stream, e0, e1, 0, g, v); });
+            return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+        return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
     }
     int launch_spmv(int it) {
         CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_spmv, 192, 0, stream, e0, e1, 0, m, v, it); });
-        return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_spmv, 192, 0, stream, e0, e1, 0, m, v, it); });
+            return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it); });
+        return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it); });
     }
     int launch_update(int it) {
         CgArgs v = cg_args();
-        return launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_upd, kBlock, 0, stream, e0, e1, 0, v, it); });
+        return launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_slice, 192, 0, stream, e0, e1, 0, v, it); });
     }
     // NoMassCGSolver::solve's loop: spmv(0), then [update(it), spmv(it+1)] until a spmv kernel
-    // finds convergence (or it == maxit).  The control block is polled once per chunk.
+    // finds convergence (or it == maxit).  Every blob iterates on its own scalars and stops on
+    // its own test; the launches go on until all have stopped.  The control blocks are polled
+    // once per chunk.
     int run_cg() {
         int rc = launch_spmv(0);
         if (rc) return rc;
         int it = 0;
+        bool all_done = false;
         for (;;) {
             int chunk = std::max(1, std::min(cg_chunk, maxit - it));
             for (int k = 0; k < chunk && it < maxit; k++) {
@@ -1023,17 +1030,28 @@ struct ffh_blob {
                 it++;
                 if ((rc = launch_spmv(it))) return rc;
             }
-            HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
+            HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, nblobs * sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
             HIP_OR_FAIL(hipStreamSynchronize(stream));
-            if (ctl_host->done || it >= maxit) break;
+            all_done = true;
+            for (int b = 0; b < nblobs && all_done; b++) all_done = ctl_host[b].done != 0;
+            if (all_done || it >= maxit) break;
         }
-        if (!ctl_host->done) return fail(FFH_E_HIP, "CG control block not finalised");
-        last_iters = ctl_host->iter;
-        last_res = ctl_host->res;
-        cg_chunk = std::max(1, std::min(ctl_host->iter, 64));
-        if (ctl_host->status != 0)
+        if (!all_done) return fail(FFH_E_HIP, "CG control block not finalised");
+        int worst = 0, failed = -1;
+        last_res = 0;
+        for (int b = 0; b < nblobs; b++) {
+            blob_iters[b] = ctl_host[b].iter;
+            blob_res[b] = ctl_host[b].res;
+            blob_status[b] = ctl_host[b].status;
+            worst = std::max(worst, ctl_host[b].iter);
+            last_res = std::max(last_res, ctl_host[b].res);
+            if (ctl_host[b].status != 0 && failed < 0) failed = b;
+        }
+        last_iters = worst;
+        cg_chunk = std::max(1, std::min(worst, 64));
+        if (failed >= 0)
             return fail(FFH_E_CG, "Conjugate gradient solver: could not converge after " + std::to_string(maxit) +
-                                      " iterations");
+                                      " iterations" + (nblobs > 1 ? " (blob " + std::to_string(failed) + ")" : ""));
         return FFH_OK;
     }
     int run_integrate() {
@@ -1064,12 +1082,16 @@ const char *ffh_last_error(void) { return g_err.c_str(); }
 
 int ffh_version(void) { return (0 << 16) | 1; }
 
-ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
-                          const double *material_si, const double *stokes_radius, int num_pinned,
-                          const int *pinned, double dt, double kT, double epsilon2,
-                          int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
-                          const unsigned long long *rng_state, int precision) {
-    g_err.clear();
+}  // extern "C"
+
+// Common creation path of ffh_blob_create (one blob) and ffh_batch_create (the disjoint union
+// of several blobs; blob_node_off marks their node ranges in the concatenated arrays).
+static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos, const int *topo,
+                             const double *material_si, const double *stokes_radius, int num_pinned,
+                             const int *pinned, double dt, double kT, double epsilon2,
+                             int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                             const unsigned long long *rng_state, int precision, int num_blobs,
+                             const int *blob_node_off) {
     if (!pos || !topo || !material_si || !stokes_radius || (num_pinned > 0 && !pinned) || !rng_state) {
         fail(FFH_E_ARG, "null argument");
         return nullptr;
@@ -1089,7 +1111,8 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
         }
     std::unique_ptr<ffh_blob> b(new ffh_blob());
     std::string msg = ffh::build_plan(b->plan, num_nodes, num_elements, pos, topo, material_si, stokes_radius,
-                                      num_pinned, pinned, stokes_visc, calc_noise, calc_stokes);
+                                      num_pinned, pinned, stokes_visc, calc_noise, calc_stokes, num_blobs,
+                                      blob_node_off);
     if (!msg.empty()) {
         fail(FFH_E_ARG, msg);
         return nullptr;
@@ -1106,14 +1129,13 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     b->N = P.num_linear;
     b->E = P.num_elements;
     b->tsize = precision == FFH_PRECISION_FP64 ? 8 : 4;
-    b->P_rhs = xcd_grid(ffh_blob::grid(b->N, 64));
-    b->P_spmv = xcd_grid(P.num_slices);
-    b->P_upd = std::max(1, std::min(512, ffh_blob::grid(3 * (int64_t)b->N)));
+    b->P_slice = xcd_grid(P.num_slices);
     b->P_elem = ffh_blob::grid(b->E);
-    if (b->P_rhs > kMaxReduceBlocks || b->P_spmv > kMaxReduceBlocks || b->P_elem > kMaxReduceBlocks) {
-        fail(FFH_E_ARG, "blob too large for the reduction buffers");
-        return nullptr;
-    }
+    b->pstride = (size_t)std::max(P.num_slices, b->P_elem);
+    b->nblobs = P.num_blobs;
+    b->blob_iters.assign(b->nblobs, 0);
+    b->blob_res.assign(b->nblobs, 0.0);
+    b->blob_status.assign(b->nblobs, 0);
     if (hipStreamCreate(&b->stream) != hipSuccess) {
         fail(FFH_E_HIP, "hipStreamCreate failed");
         return nullptr;
@@ -1178,22 +1200,25 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     rc |= b->alloc(b->vel, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_in, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_out, 3 * (size_t)N);
-    rc |= b->alloc(b->parts, 6 * (size_t)kMaxReduceBlocks);
-    rc |= b->alloc(b->ctl, 1);
+    rc |= b->alloc(b->parts, 5 * b->pstride);
+    rc |= b->upload(b->slice_blob, P.slice_blob);
+    rc |= b->upload(b->blob_slice_off, P.blob_slice_off);
+    rc |= b->alloc(b->ctl, (size_t)b->nblobs);
     rc |= b->alloc(b->rng_state, 6);
     rc |= b->upload(b->jump, P.jump);
     if (rc) {
         if (g_err.empty()) fail(FFH_E_HIP, "device allocation or upload failed");
         return nullptr;
     }
-    if (hipHostMalloc((void **)&b->ctl_host, sizeof(CgCtl), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&b->ctl_host, b->nblobs * sizeof(CgCtl), hipHostMallocDefault) != hipSuccess) {
         fail(FFH_E_HIP, "hipHostMalloc failed");
         return nullptr;
     }
-    CgCtl c{};
+    std::vector<CgCtl> c(b->nblobs);
+    std::memset(c.data(), 0, c.size() * sizeof(CgCtl));
     uint32_t st[6];
     for (int i = 0; i < 6; i++) st[i] = (uint32_t)rng_state[i];
-    if (hipMemcpy(b->ctl, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(b->ctl, c.data(), c.size() * sizeof(CgCtl), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->rng_state, st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->vel, 0, 3 * (size_t)N * sizeof(double)) != hipSuccess ||
         hipMemset(b->x, 0, 3 * (size_t)N * sizeof(double)) != hipSuccess) {
@@ -1213,6 +1238,89 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
         }
     }
     return b.release();
+}
+
+extern "C" {
+
+ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
+                          const double *material_si, const double *stokes_radius, int num_pinned,
+                          const int *pinned, double dt, double kT, double epsilon2,
+                          int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                          const unsigned long long *rng_state, int precision) {
+    g_err.clear();
+    return create_impl(num_nodes, num_elements, pos, topo, material_si, stokes_radius, num_pinned, pinned, dt,
+                       kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, stokes_visc, rng_state, precision,
+                       1, nullptr);
+}
+
+ffh_blob *ffh_batch_create(int num_blobs, const int *num_nodes, const int *num_elements, const double *pos,
+                           const int *topo, const double *material_si, const double *stokes_radius,
+                           const int *num_pinned, const int *pinned, double dt, double kT, double epsilon2,
+                           int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                           const unsigned long long *rng_state, int precision) {
+    g_err.clear();
+    if (num_blobs < 1 || !num_nodes || !num_elements || !num_pinned) {
+        fail(FFH_E_ARG, "null argument or no blobs");
+        return nullptr;
+    }
+    // concatenate: node indices of blob b's topology and pins are shifted by its node offset
+    std::vector<int> noff(num_blobs + 1, 0);
+    int64_t tn = 0, te = 0, tp = 0;
+    for (int b = 0; b < num_blobs; b++) {
+        if (num_nodes[b] <= 0 || num_elements[b] <= 0 || num_pinned[b] < 0) {
+            fail(FFH_E_ARG, "blob " + std::to_string(b) + ": needs nodes and elements");
+            return nullptr;
+        }
+        tn += num_nodes[b];
+        te += num_elements[b];
+        tp += num_pinned[b];
+        if (tn >= (int64_t)1 << 30 || te >= (int64_t)1 << 27) {
+            fail(FFH_E_ARG, "batch too large");
+            return nullptr;
+        }
+        noff[b + 1] = (int)tn;
+    }
+    if (!topo || (tp > 0 && !pinned)) {
+        fail(FFH_E_ARG, "null argument");
+        return nullptr;
+    }
+    std::vector<int> gtopo(10 * (size_t)te), gpin((size_t)tp);
+    int64_t e0 = 0, p0 = 0;
+    for (int b = 0; b < num_blobs; b++) {
+        for (int64_t k = 0; k < 10 * (int64_t)num_elements[b]; k++) {
+            int v = topo[10 * e0 + k];
+            if (v < 0 || v >= num_nodes[b]) {
+                fail(FFH_E_ARG, "blob " + std::to_string(b) + ": topology node index out of bounds");
+                return nullptr;
+            }
+            gtopo[10 * e0 + k] = v + noff[b];
+        }
+        for (int k = 0; k < num_pinned[b]; k++) {
+            int v = pinned[p0 + k];
+            if (v < 0 || v >= num_nodes[b]) {
+                fail(FFH_E_ARG, "blob " + std::to_string(b) + ": pinned node index out of bounds");
+                return nullptr;
+            }
+            gpin[p0 + k] = v + noff[b];
+        }
+        e0 += num_elements[b];
+        p0 += num_pinned[b];
+    }
+    return create_impl((int)tn, (int)te, pos, gtopo.data(), material_si, stokes_radius, (int)tp, gpin.data(), dt,
+                       kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, stokes_visc, rng_state, precision,
+                       num_blobs, noff.data());
+}
+
+int ffh_batch_num_blobs(ffh_blob *b) { return b ? b->nblobs : fail(FFH_E_ARG, "null blob"); }
+
+int ffh_batch_last_cg(ffh_blob *b, int *iterations, double *residual2, int *status) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    for (int k = 0; k < b->nblobs; k++) {
+        if (iterations) iterations[k] = b->blob_iters[k];
+        if (residual2) residual2[k] = b->blob_res[k];
+        if (status) status[k] = b->blob_status[k] == 0 ? FFH_OK : FFH_E_CG;
+    }
+    return FFH_OK;
 }
 
 void ffh_blob_destroy(ffh_blob *b) { delete b; }
@@ -1263,9 +1371,10 @@ int ffh_blob_solve(ffh_blob *b, double *xio) {
         if (xio[3 * m] != 0.0 || xio[3 * m + 1] != 0.0 || xio[3 * m + 2] != 0.0)
             return fail(FFH_E_ARG, "midpoint rows of the force vector must be zero");
     }
-    std::vector<double> fl(3 * (size_t)b->N);
+    std::vector<double> fl(3 * (size_t)b->N, 0.0);
     for (int i = 0; i < b->N; i++)
-        for (int c = 0; c < 3; c++) fl[3 * i + c] = xio[3 * (size_t)P.orig_of[i] + c];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) fl[3 * i + c] = xio[3 * (size_t)P.orig_of[i] + c];
     HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     if (int rc0 = b->run_rhs(b->tmp_in)) return rc0;
     int rc = b->run_cg();
@@ -1273,16 +1382,18 @@ int ffh_blob_solve(ffh_blob *b, double *xio) {
     HIP_OR_FAIL(hipMemcpyAsync(xl.data(), b->x, xl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     for (int i = 0; i < b->N; i++)
-        for (int c = 0; c < 3; c++) xio[3 * (size_t)P.orig_of[i] + c] = xl[3 * i + c];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) xio[3 * (size_t)P.orig_of[i] + c] = xl[3 * i + c];
     return rc;
 }
 
 int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     if (!b || !in || !result) return fail(FFH_E_ARG, "null argument");
     const ffh::Plan &P = b->plan;
-    std::vector<double> il(3 * (size_t)b->N), ol(3 * (size_t)b->N);
+    std::vector<double> il(3 * (size_t)b->N, 0.0), ol(3 * (size_t)b->N);
     for (int i = 0; i < b->N; i++)
-        for (int c = 0; c < 3; c++) il[3 * i + c] = in[3 * (size_t)P.orig_of[i] + c];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) il[3 * i + c] = in[3 * (size_t)P.orig_of[i] + c];
     HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     SpmvArgs m = b->spmv_args();
     double *ti = b->tmp_in, *to = b->tmp_out;
@@ -1298,16 +1409,18 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     for (int i = 0; i < P.num_nodes_total; i++)
         for (int c = 0; c < 3; c++) result[3 * (size_t)i + c] = in[3 * (size_t)i + c];
     for (int i = 0; i < b->N; i++)
-        for (int c = 0; c < 3; c++) result[3 * (size_t)P.orig_of[i] + c] = ol[3 * i + c];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) result[3 * (size_t)P.orig_of[i] + c] = ol[3 * i + c];
     return FFH_OK;
 }
 
 int ffh_blob_set_positions(ffh_blob *b, const double *pos) {
     if (!b || !pos) return fail(FFH_E_ARG, "null argument");
     const int N = b->N;
-    std::vector<double> h(3 * (size_t)N);
+    std::vector<double> h(3 * (size_t)N, 0.0);
     for (int i = 0; i < N; i++)
-        for (int c = 0; c < 3; c++) h[(size_t)c * N + i] = pos[3 * (size_t)b->plan.orig_of[i] + c];
+        if (b->plan.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) h[(size_t)c * N + i] = pos[3 * (size_t)b->plan.orig_of[i] + c];
     HIP_OR_FAIL(hipMemcpyAsync(b->X, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return FFH_OK;
@@ -1321,7 +1434,8 @@ int ffh_blob_get_positions(ffh_blob *b, double *pos) {
     HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->X, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     for (int i = 0; i < N; i++)
-        for (int c = 0; c < 3; c++) pos[3 * (size_t)P.orig_of[i] + c] = h[(size_t)c * N + i];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) pos[3 * (size_t)P.orig_of[i] + c] = h[(size_t)c * N + i];
     // linearise_element (src/tetra_element_linear.cpp:567-591)
     for (size_t k = 0; k < P.mid_index.size(); k++)
         for (int c = 0; c < 3; c++)
@@ -1336,7 +1450,8 @@ static int get_node_vector(ffh_blob *b, const double *dev, double *out) {
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     std::memset(out, 0, sizeof(double) * 3 * (size_t)P.num_nodes_total);
     for (int i = 0; i < b->N; i++)
-        for (int c = 0; c < 3; c++) out[3 * (size_t)P.orig_of[i] + c] = h[3 * i + c];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) out[3 * (size_t)P.orig_of[i] + c] = h[3 * i + c];
     return FFH_OK;
 }
 
@@ -1380,7 +1495,7 @@ int ffh_blob_last_cg(ffh_blob *b, int *iterations, double *residual2) {
 
 int ffh_blob_sizes(ffh_blob *b, int *num_linear, int *num_elements, long long *num_blocks, long long *num_slots) {
     if (!b) return fail(FFH_E_ARG, "null blob");
-    if (num_linear) *num_linear = b->N;
+    if (num_linear) *num_linear = b->plan.num_real_linear;
     if (num_elements) *num_elements = b->E;
     if (num_blocks) *num_blocks = b->plan.num_blocks;
     if (num_slots) *num_slots = b->plan.num_slots;

@@ -109,7 +109,8 @@ std::string make_tiles(const std::vector<uint32_t> &off, const std::vector<uint3
 
 std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
                        const double *material_si, const double *stokes_radius, int num_pinned,
-                       const int *pinned, double stokes_visc, int calc_noise, int calc_stokes) {
+                       const int *pinned, double stokes_visc, int calc_noise, int calc_stokes,
+                       int num_blobs, const int *blob_node_off) {
     if (num_nodes <= 0 || num_elements <= 0) return "mesh must have nodes and elements";
     if ((uint64_t)num_elements >= (1u << 27)) return "more than 2^27 elements per blob is not supported";
     const int NT = num_nodes, E = num_elements;
@@ -128,15 +129,47 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         for (int j = 4; j < 10; j++)
             if (is_lin[topo[10 * e + j]])
                 return "node " + std::to_string(topo[10 * e + j]) + " is both a vertex and an edge midpoint";
+    // blobs and their node ranges
+    std::vector<int> boff;
+    if (blob_node_off) {
+        if (num_blobs < 1) return "need at least one blob";
+        boff.assign(blob_node_off, blob_node_off + num_blobs + 1);
+        if (boff[0] != 0 || boff[num_blobs] != NT) return "blob node offsets must span all nodes";
+        for (int b = 0; b < num_blobs; b++)
+            if (boff[b + 1] < boff[b]) return "blob node offsets must be non-decreasing";
+    } else {
+        num_blobs = 1;
+        boff = {0, NT};
+    }
+    P.num_blobs = num_blobs;
+    {
+        std::vector<int> node_blob(NT);
+        for (int b = 0; b < num_blobs; b++)
+            for (int i = boff[b]; i < boff[b + 1]; i++) node_blob[i] = b;
+        for (int e = 0; e < E; e++)
+            for (int j = 1; j < 10; j++)
+                if (node_blob[topo[10 * e + j]] != node_blob[topo[10 * e]])
+                    return "element " + std::to_string(e) + " joins nodes of two blobs";
+    }
+    // compact linear numbering, ascending original index; each blob padded to whole slices
     P.lin_of.assign(NT, -1);
     P.orig_of.clear();
-    for (int i = 0; i < NT; i++)
-        if (is_lin[i]) {
-            P.lin_of[i] = (int)P.orig_of.size();
-            P.orig_of.push_back(i);
-        }
+    P.blob_slice_off.assign(num_blobs + 1, 0);
+    for (int b = 0; b < num_blobs; b++) {
+        P.blob_slice_off[b] = (int)(P.orig_of.size() / kSlice);
+        for (int i = boff[b]; i < boff[b + 1]; i++)
+            if (is_lin[i]) {
+                P.lin_of[i] = (int)P.orig_of.size();
+                P.orig_of.push_back(i);
+            }
+        while (P.orig_of.size() % kSlice) P.orig_of.push_back(-1);
+    }
+    P.blob_slice_off[num_blobs] = (int)(P.orig_of.size() / kSlice);
     const int N = (int)P.orig_of.size();
     P.num_linear = N;
+    P.num_real_linear = 0;
+    for (int i = 0; i < N; i++) P.num_real_linear += P.orig_of[i] >= 0;
+    if (P.num_real_linear == 0) return "mesh has no linear nodes";
     if (calc_stokes == 1)
         for (int i = 0; i < NT; i++)
             if (!is_lin[i] && stokes_radius[i] != 0.0)
@@ -164,7 +197,8 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     // --- positions (linear nodes, SoA) ---
     P.pos.assign(3 * (size_t)N, 0.0);
     for (int i = 0; i < N; i++)
-        for (int c = 0; c < 3; c++) P.pos[(size_t)c * N + i] = pos[3 * (size_t)P.orig_of[i] + c];
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) P.pos[(size_t)c * N + i] = pos[3 * (size_t)P.orig_of[i] + c];
     const double *X = P.pos.data(), *Y = X + N, *Z = Y + N;
 
     // --- elements: compact connectivity, materials, rest state ---
@@ -212,6 +246,8 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     // --- nodes: drag, pins, RNG offsets ---
     P.zeta.assign(N, 0.0);
     P.pinned.assign(N, 0);
+    for (int i = 0; i < N; i++)
+        if (P.orig_of[i] < 0) P.pinned[i] = 1;  // padding rows: identity, zero force
     for (int i = 0; i < num_pinned; i++) {
         int v = pinned[i];
         if (v < 0 || v >= NT) return "pinned node index out of bounds";
@@ -222,7 +258,7 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         for (int i = 0; i < N; i++)
             if (!P.pinned[i]) P.zeta[i] = 6.0 * kPi * stokes_visc * stokes_radius[P.orig_of[i]];
     P.rng_node_off.resize(N);
-    for (int i = 0; i < N; i++) P.rng_node_off[i] = 3u * (uint32_t)P.orig_of[i];
+    for (int i = 0; i < N; i++) P.rng_node_off[i] = P.orig_of[i] >= 0 ? 3u * (uint32_t)P.orig_of[i] : 0u;
 
     // --- node force gather (element order), src/Blob.cpp:3974-4019 ---
     P.nctb_off.assign((size_t)N + 1, 0);
@@ -306,6 +342,9 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         P.slice_off[s + 1] = P.slice_off[s] + (int64_t)w * kSlice;
     }
     P.num_slots = P.slice_off[P.num_slices];
+    P.slice_blob.assign(P.num_slices, 0);
+    for (int b = 0; b < num_blobs; b++)
+        for (int sl = P.blob_slice_off[b]; sl < P.blob_slice_off[b + 1]; sl++) P.slice_blob[sl] = b;
     if (P.num_slots >= (int64_t)1 << 31) return "matrix pattern too large";
     std::vector<int64_t> slot_of_block(NB, -1);
     P.slot_col.assign(P.num_slots, 0);

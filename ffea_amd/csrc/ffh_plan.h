@@ -37,11 +37,17 @@ struct Tiles {
 struct Plan {
     // sizes
     int num_nodes_total = 0;  // N_t
-    int num_linear = 0;       // N
+    int num_linear = 0;       // N, rows on the device (real linear nodes + slice padding)
+    int num_real_linear = 0;  // linear nodes of the mesh
     int num_elements = 0;     // E
+    // blobs: each blob's rows start on a 64-row slice boundary (padding rows are pinned
+    // dummies), so every slice -- and every per-slice partial sum -- belongs to one blob
+    int num_blobs = 1;
+    std::vector<int32_t> blob_slice_off;  // [num_blobs + 1]
+    std::vector<int32_t> slice_blob;      // [num_slices]
     // node maps
     std::vector<int> lin_of;   // [N_t] -> compact index or -1
-    std::vector<int> orig_of;  // [N]   -> original index
+    std::vector<int> orig_of;  // [N]   -> original index, -1 for padding rows
     // per element (compact node ids), rest data and material in meso units
     std::vector<int32_t> enodes;   // [4*E] AoS (int4 per element)
     std::vector<double> jinv0;     // [9*E] SoA: jinv0[k*E + e]
@@ -86,9 +92,12 @@ struct Plan {
 std::string make_tiles(const std::vector<uint32_t> &off, const std::vector<uint32_t> &items, Tiles &t);
 
 // Builds the plan; returns "" on success or an error message.
+// blob_node_off: [num_blobs + 1] offsets of each blob's nodes in the concatenated node arrays
+// (nullptr: one blob); elements must only reference nodes of their own blob.
 std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
                        const double *material_si, const double *stokes_radius, int num_pinned,
-                       const int *pinned, double stokes_visc, int calc_noise, int calc_stokes);
+                       const int *pinned, double stokes_visc, int calc_noise, int calc_stokes,
+                       int num_blobs = 1, const int *blob_node_off = nullptr);
 
 // Exact jump matrices of MRG32k3a component c (0: m1, 1: m2) raised to n.
 Mat3u mrg_pow(int comp, uint64_t n);

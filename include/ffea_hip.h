@@ -64,6 +64,36 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
                           int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
                           const unsigned long long *rng_state, int precision);
 
+/*
+ * Create a batch: several blobs of one World on the current HIP device, stepped together (one
+ * kernel launch per phase for all of them) and fed by one RngStream, in World::run's order
+ * (src/World.cpp:2020-2041): the element phases of blobs 0..B-1 (7 draws per element), then
+ * their position phases (3 draws per node).  With one OpenMP thread this is the reference's
+ * draw order for params.num_blobs > 1 (World::init creates the streams,
+ * src/World.cpp:298-317, and hands the same array to every Blob::config, :2618-2622).
+ *   num_blobs                 B
+ *   num_nodes[B], num_elements[B], num_pinned[B]   per-blob counts
+ *   pos, stokes_radius        blobs' node arrays concatenated (blob 0's N_t nodes first)
+ *   topo, material_si         blobs' element arrays concatenated; topology and pinned indices
+ *   pinned                    are local to their blob, as in each blob's .top / .pin file
+ * Every other argument is as for ffh_blob_create (SimulationParams is shared by the World).
+ * The handle works with every ffh_blob_* call; node arrays are then the concatenation over
+ * blobs, and each blob's CG runs on its own scalars and stops on its own test.
+ */
+ffh_blob *ffh_batch_create(int num_blobs, const int *num_nodes, const int *num_elements, const double *pos,
+                           const int *topo, const double *material_si, const double *stokes_radius,
+                           const int *num_pinned, const int *pinned, double dt, double kT, double epsilon2,
+                           int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                           const unsigned long long *rng_state, int precision);
+
+/* Number of blobs of a handle (1 for ffh_blob_create). */
+int ffh_batch_num_blobs(ffh_blob *b);
+
+/* Per-blob result of the last CG solve: iterations, final ||r||^2/||f||^2 and FFH_OK or
+ * FFH_E_CG (then that blob's positions were not integrated), arrays of num_blobs entries
+ * (NoMassCGSolver::solve's outcome per Blob, src/NoMassCGSolver.cpp:165-197). */
+int ffh_batch_last_cg(ffh_blob *b, int *iterations, double *residual2, int *status);
+
 /* Release all device memory of the blob (Blob::~Blob / NoMassCGSolver::~NoMassCGSolver). */
 void ffh_blob_destroy(ffh_blob *b);
 
@@ -119,7 +149,8 @@ int ffh_blob_get_rhs(ffh_blob *b, double *f);
 int ffh_blob_get_rng_state(ffh_blob *b, unsigned long long *state6);
 int ffh_blob_set_rng_state(ffh_blob *b, const unsigned long long *state6);
 
-/* Iterations of the last CG solve and its final ||r||^2/||f||^2. */
+/* Iterations of the last CG solve and its final ||r||^2/||f||^2 (for a batch: the largest
+ * over its blobs). */
 int ffh_blob_last_cg(ffh_blob *b, int *iterations, double *residual2);
 
 /* Sizes: linear nodes, elements, 3x3 blocks of the matrix pattern, padded SpMV slots. */
