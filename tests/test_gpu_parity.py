@@ -156,3 +156,16 @@ def test_cg_nonconvergence_is_reported():
     with pytest.raises(_lib.CGError):
         b.step(1)
     assert np.array_equal(b.positions(), before)  # integration skipped, as the reference does
+
+
+def test_strain_energy_against_oracle(oracle):
+    """k_strain_energy (Blob::make_measurements' sum, src/Blob.cpp:1424-1432) against the oracle
+    on a jittered, displaced box; the sum order differs (block tree), hence the tolerance."""
+    m, prm, sc, x0 = box_case((4, 3, 3), noise=0, sigma=0.05, seed=6, jitter=0.1)
+    oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+    s = oracle.stream()
+    bo = oracle.blob(m, prm, sc, pos=x0)
+    assert bo.forces_only(s)[0] == 0
+    b = gpu_blob(m, prm, sc, np.arange(1, 7, dtype=np.uint64), x0)
+    assert b.update_internal_forces() == 0
+    assert b.strain_energy() == pytest.approx(oracle.strain_energy(bo), rel=1e-12)

@@ -110,3 +110,34 @@ def test_cg_solution_matches_dense_solve(oracle):
     # midpoint rows are identity with zero force -> zero velocity
     mid = np.setdiff1d(np.arange(m.num_nodes), np.unique(m.topo[:, :4]))
     assert np.all(v.reshape(-1, 3)[mid] == 0)
+
+
+def test_strain_energy_is_the_potential_of_the_elastic_forces(oracle):
+    """The measured strain energy (src/Blob.cpp:1424-1432) and the elastic nodal forces
+    (src/tetra_element_linear.cpp:378-408, :441-490) are computed by separate code.  With noise
+    off the force vector is purely elastic and must be minus the energy's gradient; a dropped
+    term, a wrong factor or a wrong sign in either breaks this.  Also: zero at rest, invariant
+    under rigid motion."""
+    m, prm, sc, x0 = box_case((2, 2, 1), noise=0, sigma=0.05, seed=5, jitter=0.1)
+
+    def energy_and_rhs(pos):
+        b, rhs = rhs_of(oracle, m, prm, sc, pos)
+        return oracle.strain_energy(b), rhs
+
+    U0, f = energy_and_rhs(x0)
+    assert U0 > 0
+    U_rest, _ = energy_and_rhs(m.pos)
+    assert abs(U_rest) < 1e-12 * U0
+    th = 0.4
+    R = np.array([[1, 0, 0], [0, np.cos(th), -np.sin(th)], [0, np.sin(th), np.cos(th)]])
+    U_rot, _ = energy_and_rhs(x0 @ R.T + np.array([1.0, 2.0, -3.0]))
+    assert U_rot == pytest.approx(U0, rel=1e-11)
+    h = 1e-5  # input units; central differences, O(h^2) truncation
+    fmax = np.abs(f).max()
+    for node in range(m.num_linear):
+        for c in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[node, c] += h
+            xm[node, c] -= h
+            dU = (energy_and_rhs(xp)[0] - energy_and_rhs(xm)[0]) / (2 * h * sc)
+            assert -dU == pytest.approx(f[3 * node + c], abs=1e-6 * fmax), (node, c)
