@@ -33,8 +33,13 @@ EXPORTS = (
     "ffh_blob_get_rng_state", "ffh_blob_set_rng_state", "ffh_blob_last_cg", "ffh_blob_sizes",
     "ffh_blob_stream", "ffh_blob_synchronize", "ffh_blob_set_profiling",
     "ffh_blob_kernel_times", "ffh_blob_strain_energy", "ffh_batch_create",
-    "ffh_batch_num_blobs", "ffh_batch_last_cg",
+    "ffh_batch_num_blobs", "ffh_batch_last_cg", "ffh_mpnn_num_params", "ffh_mpnn_init",
+    "ffh_mpnn_set_weights", "ffh_mpnn_get_weights", "ffh_blob_set_preconditioner",
+    "ffh_mpnn_apply", "ffh_mpnn_harvest", "ffh_mpnn_harvest_count", "ffh_mpnn_train",
+    "ffh_mpnn_loss_grad",
 )
+FFH_PRECOND_JACOBI = 0
+FFH_PRECOND_LEARNED = 1
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -79,6 +84,17 @@ def load(path: str | None = None) -> C.CDLL:
                                    _u64p, C.c_int]
     L.ffh_batch_num_blobs.argtypes = [C.c_void_p]
     L.ffh_batch_last_cg.argtypes = [C.c_void_p, _ip, _dp, _ip]
+    _fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.ffh_mpnn_num_params.argtypes = []
+    L.ffh_mpnn_init.argtypes = [C.c_void_p, C.c_ulonglong]
+    L.ffh_mpnn_set_weights.argtypes = [C.c_void_p, _fp]
+    L.ffh_mpnn_get_weights.argtypes = [C.c_void_p, _fp]
+    L.ffh_blob_set_preconditioner.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ffh_mpnn_apply.argtypes = [C.c_void_p, _dp, _dp]
+    L.ffh_mpnn_harvest.argtypes = [C.c_void_p, C.c_int]
+    L.ffh_mpnn_harvest_count.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    L.ffh_mpnn_train.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_ulonglong, _fp]
+    L.ffh_mpnn_loss_grad.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_double), _fp]
     L.ffh_blob_destroy.argtypes = [C.c_void_p]
     L.ffh_blob_destroy.restype = None
     L.ffh_blob_update_internal_forces.argtypes = [C.c_void_p, C.POINTER(C.c_int)]

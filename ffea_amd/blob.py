@@ -167,6 +167,48 @@ class Blob:
         return e.value
 
 
+    # -- learned preconditioner (no counterpart in the reference) -------------------------
+    def mpnn_init(self, seed: int = 0) -> None:
+        _lib.check(self._L.ffh_mpnn_init(self._h, int(seed)))
+
+    def mpnn_weights(self) -> np.ndarray:
+        w = np.zeros(self._L.ffh_mpnn_num_params(), np.float32)
+        _lib.check(self._L.ffh_mpnn_get_weights(self._h, w))
+        return w
+
+    def mpnn_set_weights(self, w) -> None:
+        _lib.check(self._L.ffh_mpnn_set_weights(self._h, np.ascontiguousarray(w, np.float32)))
+
+    def set_preconditioner(self, kind: str = "jacobi", fallback_iterations: int = 0) -> None:
+        k = {"jacobi": _lib.FFH_PRECOND_JACOBI, "learned": _lib.FFH_PRECOND_LEARNED}[kind]
+        _lib.check(self._L.ffh_blob_set_preconditioner(self._h, k, int(fallback_iterations)))
+
+    def mpnn_apply(self, r) -> np.ndarray:
+        out = np.zeros(3 * self.num_nodes, np.float64)
+        _lib.check(self._L.ffh_mpnn_apply(self._h, np.ascontiguousarray(r, np.float64).ravel(), out))
+        return out.reshape(-1, 3)
+
+    def mpnn_harvest(self, max_snapshots: int) -> None:
+        _lib.check(self._L.ffh_mpnn_harvest(self._h, int(max_snapshots)))
+
+    def mpnn_harvest_count(self) -> tuple[int, int]:
+        n, p = C.c_int(0), C.c_longlong(0)
+        _lib.check(self._L.ffh_mpnn_harvest_count(self._h, C.byref(n), C.byref(p)))
+        return n.value, p.value
+
+    def mpnn_train(self, epochs: int, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        loss = np.zeros(int(epochs), np.float32)
+        _lib.check(self._L.ffh_mpnn_train(self._h, int(epochs), float(lr), int(seed), loss))
+        return loss
+
+    def mpnn_loss_grad(self, r, e) -> tuple[float, np.ndarray]:
+        g = np.zeros(self._L.ffh_mpnn_num_params(), np.float32)
+        loss = C.c_double(0)
+        _lib.check(self._L.ffh_mpnn_loss_grad(self._h, np.ascontiguousarray(r, np.float64).ravel(),
+                                               np.ascontiguousarray(e, np.float64).ravel(), C.byref(loss), g))
+        return loss.value, g
+
+
 class Batch(Blob):
     """Several blobs of one World stepped together on the GPU (ffh_batch_create): one launch
     per phase for all of them, one RngStream in World::run's draw order (the element phases

@@ -31,6 +31,9 @@
 #include "../../include/ffea_hip.h"
 #include "ffh_plan.h"
 
+#include <algorithm>
+#include <random>
+
 using ffh::Mat3u;
 
 namespace {
@@ -490,6 +493,9 @@ struct CgArgs {
     double eps2;
     int maxit;
     CgCtl *ctl;  // one per blob
+    // learned preconditioner (nullptr otherwise): partials of |D^-1 r|^2, r before the update,
+    // partials of z.r_prev for the flexible beta
+    double *parts_uu, *r_prev, *parts_zrp;
 };
 
 struct RhsArgs {
@@ -560,6 +566,7 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
     v.z[k] = z;
     block_partial_at(f * z, v.parts_rz, blk, sh);
     block_partial_at(f * f, v.parts_ff, blk, sh);
+    if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
     const int b = v.slice_blob[blk];
     if (blk == v.blob_slice_off[b] && threadIdx.x == 0) {
         CgCtl *c = &v.ctl[b];
@@ -587,8 +594,10 @@ struct SpmvArgs {
 // SparseMatrixFixedPattern::apply, src/SparseMatrixFixedPattern.cpp:148-191); p.q partials.
 // One block = one 64-row slice; wave w computes component w of every row.  Every scalar is
 // per blob: the blob's slices are a contiguous range of partials.
+// mode 0: Jacobi-PCG (Fletcher-Reeves beta); 1: flexible CG with the learned preconditioner
+// (beta = z.(r - r_prev) / delta_prev, Polak-Ribiere form); 2: restart (beta = 0: p = z).
 template <typename T>
-__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it) {
+__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
     __shared__ double sh[4];
     const int slice = xcd_block(v.nslices);
     if (slice >= v.nslices) return;
@@ -625,7 +634,10 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it) {
             return;
         }
         const double dnew = parts_range(v.parts_rz, s0, s1, sh);
-        beta = dnew / ctl->delta[(it - 1) & 1];
+        if (mode == 0)
+            beta = dnew / ctl->delta[(it - 1) & 1];
+        else if (mode == 1)
+            beta = (dnew - parts_range(v.parts_zrp, s0, s1, sh)) / ctl->delta[(it - 1) & 1];
         if (decider) ctl->delta[it & 1] = dnew;
     }
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -667,12 +679,17 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
         ctl->delta[it & 1] / parts_range(v.parts_pq, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
     const size_t k = (size_t)slice * 192 + threadIdx.x;
     v.x[k] += alpha * v.p[k];
-    const double r = v.r[k] + (-alpha) * v.q[k];
+    const double r0 = v.r[k];
+    const double r = r0 + (-alpha) * v.q[k];
     v.r[k] = r;
     const double z = v.dinv[k] * r;
     v.z[k] = z;
     block_partial_at(r * r, v.parts_rr, slice, sh);
     block_partial_at(r * z, v.parts_rz, slice, sh);
+    if (v.parts_uu) {  // learned preconditioner follows: z holds u = D^-1 r
+        v.r_prev[k] = r0;
+        block_partial_at(z * z, v.parts_uu, slice, sh);
+    }
 }
 
 // euler_integrate (src/Blob.cpp:3941-3953): vel = v; pos += v * dt.  Skipped after a CG
@@ -759,6 +776,8 @@ __global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *pa
     block_partial(part, parts, sh);
 }
 
+#include "ffh_mpnn.h"
+
 // ----------------------------------------------------------------------------------------
 // Device buffers
 // ----------------------------------------------------------------------------------------
@@ -811,6 +830,26 @@ struct ffh_blob {
     std::vector<hipEvent_t> event_pool;
     int last_iters = 0;
     double last_res = 0;
+    // learned preconditioner (allocated on first use)
+    int precond = FFH_PRECOND_JACOBI;
+    int fallback_iters = 0;  // learned iterations before the Jacobi restart (0: plain_count)
+    int plain_count = -1;    // iterations of the last plain solve
+    bool mp_alloced = false, mp_ready = false, mp_edges = false;
+    std::vector<int32_t> blob_nreal_h;
+    int32_t *blob_nreal = nullptr;
+    float *mp_w = nullptr, *mp_m1 = nullptr, *mp_m2 = nullptr, *mp_g = nullptr;
+    bf16_t *mp_wT = nullptr, *mp_wN = nullptr;
+    float *mp_aw = nullptr, *mp_cf = nullptr;
+    bf16_t *mp_H[4] = {nullptr, nullptr, nullptr, nullptr}, *mp_G[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *mp_feat = nullptr, *mp_u = nullptr, *mp_tgt = nullptr, *mp_dP[2] = {nullptr, nullptr}, *mp_dX = nullptr;
+    float *mp_part = nullptr, *mp_loss = nullptr;
+    double *r_prev = nullptr;
+    int adam_t = 0;
+    // harvested (r_k, x* - x_k) snapshots of plain solves
+    int hv_cap = 0, hv_count = 0;
+    bool hv_on = false;
+    long long hv_pairs = 0;
+    float *hv_r = nullptr, *hv_x = nullptr;
     std::vector<int> blob_iters;
     std::vector<double> blob_res;
     std::vector<int> blob_status;
@@ -912,9 +951,16 @@ struct ffh_blob {
         v.eps2 = eps2;
         v.maxit = maxit;
         v.ctl = ctl;
+        v.parts_uu = nullptr;
+        v.r_prev = nullptr;
+        v.parts_zrp = parts_zrp();
         return v;
     }
     double *parts_energy() const { return parts + 4 * pstride; }
+    double *parts_uu() const { return parts + 5 * pstride; }
+    double *parts_zrp() const { return parts + 6 * pstride; }
+    double *parts_den() const { return parts + 7 * pstride; }
+    double *parts_loss() const { return parts + 8 * pstride; }
     SpmvArgs spmv_args() const {
         SpmvArgs m;
         m.slice_off = slice_off;
@@ -957,6 +1003,7 @@ struct ffh_blob {
         return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), xcd_grid(grid(E)), kBlock, 0, stream, e0, e1, 0, g); });
     }
     int run_assemble() {
+        mp_edges = false;
         AsmArgs a;
         a.erec = erec;
         a.toff = ut_off;
@@ -999,36 +1046,201 @@ struct ffh_blob {
     int run_rhs(const double *f_in = nullptr) {
         RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1, f_in);
         CgArgs v = cg_args();
+        if (learned_solve()) v.parts_uu = parts_uu();
         if (precision == FFH_PRECISION_FP64)
             return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
         return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
     }
-    int launch_spmv(int it) {
+    int launch_spmv(int it, int mode) {
         CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it); });
-        return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it); });
+            return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
+        return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
     }
-    int launch_update(int it) {
+    int launch_update(int it, bool learned_next) {
         CgArgs v = cg_args();
+        if (learned_next) {
+            v.parts_uu = parts_uu();
+            v.r_prev = r_prev;
+        }
         return launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_slice, 192, 0, stream, e0, e1, 0, v, it); });
+    }
+    // ---- learned preconditioner ----
+    bool learned_solve() const { return precond == FFH_PRECOND_LEARNED && mp_ready && plain_count >= 0; }
+    int mp_alloc() {
+        if (mp_alloced) return FFH_OK;
+        const size_t n = (size_t)N;
+        int rc = 0;
+        blob_nreal_h.assign(nblobs, 0);
+        for (int b = 0; b < nblobs; b++)
+            for (int i = plan.blob_slice_off[b] * 64; i < plan.blob_slice_off[b + 1] * 64; i++)
+                blob_nreal_h[b] += plan.orig_of[i] >= 0;
+        rc |= upload(blob_nreal, blob_nreal_h);
+        rc |= alloc(mp_w, kNumParams);
+        rc |= alloc(mp_m1, kNumParams);
+        rc |= alloc(mp_m2, kNumParams);
+        rc |= alloc(mp_g, kNumParams);
+        rc |= alloc(mp_wT, (size_t)kNLay * 64 * kHid);
+        rc |= alloc(mp_wN, (size_t)kNLay * 64 * kHid);
+        rc |= alloc(mp_aw, (size_t)plan.num_slots);
+        rc |= alloc(mp_cf, n);
+        for (int l = 0; l < 4; l++) rc |= alloc(mp_H[l], n * kHid);
+        for (int l = 1; l < 4; l++) rc |= alloc(mp_G[l], n * kHid);
+        rc |= alloc(mp_feat, n * kNIn);
+        rc |= alloc(mp_u, 3 * n);
+        rc |= alloc(mp_tgt, 3 * n);
+        rc |= alloc(mp_dP[0], n * kHid);
+        rc |= alloc(mp_dP[1], n * kHid);
+        rc |= alloc(mp_dX, n * 64);
+        const size_t nw = (n + kWgradRows - 1) / kWgradRows, ne = (n + 255) / 256;
+        rc |= alloc(mp_part, std::max({(size_t)plan.num_slices * kOutGrad, nw * kLayGrad, ne * kEmbGrad}));
+        rc |= alloc(mp_loss, 4096);
+        rc |= alloc(r_prev, 3 * n);
+        if (rc) return rc;
+        mp_alloced = true;
+        return FFH_OK;
+    }
+    MpnnArgs mp_args(bool skip_done) const {
+        MpnnArgs a;
+        a.N = N;
+        a.nslices = plan.num_slices;
+        a.slice_blob = slice_blob;
+        a.blob_slice_off = blob_slice_off;
+        a.blob_nreal = blob_nreal;
+        a.pinned = pinned;
+        a.slice_off = slice_off;
+        a.slice_width = slice_width;
+        a.col = slot_col;
+        a.aw = mp_aw;
+        a.cf = mp_cf;
+        a.wf = mp_w;
+        a.wT = mp_wT;
+        a.wN = mp_wN;
+        a.ctl = skip_done ? ctl : nullptr;
+        return a;
+    }
+    int mp_edges_now() {
+        if (mp_edges) return FFH_OK;
+        SpmvArgs m = spmv_args();
+        const int g = (plan.num_slices + 3) / 4;
+        if (precision == FFH_PRECISION_FP64)
+            hipLaunchKernelGGL((k_mpnn_edges<double>), g, 256, 0, stream, m, plan.num_slices, dinv, zeta, mp_aw, mp_cf);
+        else
+            hipLaunchKernelGGL((k_mpnn_edges<float>), g, 256, 0, stream, m, plan.num_slices, dinv, zeta, mp_aw, mp_cf);
+        HIP_OR_FAIL(hipGetLastError());
+        mp_edges = true;
+        return FFH_OK;
+    }
+    int mp_pack() {
+        hipLaunchKernelGGL(k_mpnn_pack, (kNLay * 64 * kHid + 255) / 256, 256, 0, stream, mp_w, mp_wT, mp_wN);
+        HIP_OR_FAIL(hipGetLastError());
+        return FFH_OK;
+    }
+    // Forward pass to z (CG).  Input: z = u = D^-1 r and the partials of |u|^2.
+    int mpnn_apply(const double *rvec, const double *rprev, bool skip_done) {
+        int rc;
+        if ((rc = mp_edges_now())) return rc;
+        MpnnArgs a = mp_args(skip_done);
+        const int S = plan.num_slices;
+        hipLaunchKernelGGL((k_mpnn_embed<double>), P_slice, 256, 0, stream, a, (const double *)z, (const double *)parts_uu(),
+                           (float *)nullptr, mp_H[0], (const float *)nullptr, (float *)nullptr, (double *)nullptr);
+        bf16_t *h = mp_H[0], *hn = mp_H[1];
+        for (int l = 0; l < kNLay; l++) {
+            hipLaunchKernelGGL((k_mpnn_spmm<bf16_t, false>), P_slice, 256, 0, stream, a, (const bf16_t *)h, kHid, 0,
+                               (void *)mp_G[1], (const bf16_t *)nullptr);
+            hipLaunchKernelGGL(k_mpnn_dense, S, 128, 0, stream, a, l, (const bf16_t *)h, (const bf16_t *)mp_G[1], hn);
+            std::swap(h, hn);
+        }
+        hipLaunchKernelGGL(k_mpnn_out_cg, P_slice, 192, 0, stream, a, (const bf16_t *)h, (const double *)parts_uu(), z,
+                           rvec, rprev, parts, rprev ? parts_zrp() : (double *)nullptr);
+        HIP_OR_FAIL(hipGetLastError());
+        return FFH_OK;
+    }
+    // One training pass over a snapshot (residual rs, target es = x* - x_k, float [3N]):
+    // forward, loss into *loss_slot (device), gradient into mp_g.
+    int mpnn_forward_backward(const float *rs, const float *es, float *loss_slot) {
+        int rc;
+        if ((rc = mp_edges_now())) return rc;
+        MpnnArgs a = mp_args(false);
+        const int S = plan.num_slices;
+        hipLaunchKernelGGL(k_mpnn_prep, S, 192, 0, stream, S, rs, (const double *)dinv, mp_u, parts_uu());
+        hipLaunchKernelGGL((k_mpnn_embed<float>), P_slice, 256, 0, stream, a, (const float *)mp_u,
+                           (const double *)parts_uu(), mp_feat, mp_H[0], es, mp_tgt, parts_den());
+        for (int l = 1; l <= kNLay; l++) {
+            hipLaunchKernelGGL((k_mpnn_spmm<bf16_t, false>), P_slice, 256, 0, stream, a, (const bf16_t *)mp_H[l - 1],
+                               kHid, 0, (void *)mp_G[l], (const bf16_t *)nullptr);
+            hipLaunchKernelGGL(k_mpnn_dense, S, 128, 0, stream, a, l - 1, (const bf16_t *)mp_H[l - 1],
+                               (const bf16_t *)mp_G[l], mp_H[l]);
+        }
+        int cur = 0;
+        hipLaunchKernelGGL(k_mpnn_out_train, S, 64, 0, stream, a, (const bf16_t *)mp_H[kNLay], (const float *)mp_tgt,
+                           (const double *)parts_den(), 1.0f / (float)nblobs, mp_dP[cur], mp_part, parts_loss());
+        hipLaunchKernelGGL(k_mpnn_sum_loss, 1, 256, 0, stream, (const double *)parts_loss(), S, loss_slot);
+        hipLaunchKernelGGL(k_mpnn_reduce, 1, 256, 0, stream, (const float *)mp_part, S, kOutGrad, 1, 0, mp_g);
+        const int nw = (N + kWgradRows - 1) / kWgradRows, ne = (N + 255) / 256;
+        for (int l = kNLay; l >= 1; l--) {
+            hipLaunchKernelGGL(k_mpnn_wgrad, nw, 128, 0, stream, N, (const bf16_t *)mp_H[l - 1], (const bf16_t *)mp_G[l],
+                               (const float *)mp_dP[cur], mp_part);
+            hipLaunchKernelGGL(k_mpnn_reduce, (kLayGrad + 255) / 256, 256, 0, stream, (const float *)mp_part, nw,
+                               kLayGrad, 0, l - 1, mp_g);
+            hipLaunchKernelGGL(k_mpnn_dgrad, S, 128, 0, stream, a, l - 1, (const float *)mp_dP[cur], mp_dX);
+            hipLaunchKernelGGL((k_mpnn_spmm<float, true>), P_slice, 256, 0, stream, a, (const float *)mp_dX, 64, 32,
+                               (void *)mp_dP[1 - cur], (const bf16_t *)mp_H[l - 1]);
+            cur = 1 - cur;
+        }
+        hipLaunchKernelGGL(k_mpnn_embed_grad, ne, 256, 0, stream, N, (const float *)mp_dP[cur], (const float *)mp_feat,
+                           mp_part);
+        hipLaunchKernelGGL(k_mpnn_reduce, 1, 256, 0, stream, (const float *)mp_part, ne, kEmbGrad, 2, 0, mp_g);
+        HIP_OR_FAIL(hipGetLastError());
+        return FFH_OK;
+    }
+    int mpnn_adam(float lr) {
+        adam_t++;
+        const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)adam_t));
+        const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)adam_t));
+        hipLaunchKernelGGL(k_mpnn_adam, (kNumParams + 255) / 256, 256, 0, stream, mp_w, mp_m1, mp_m2,
+                           (const float *)mp_g, lr, c1, c2);
+        HIP_OR_FAIL(hipGetLastError());
+        return mp_pack();
+    }
+    int harvest_snap(int k) {
+        if (!hv_on || k >= hv_cap) return FFH_OK;
+        const size_t n3 = 3 * (size_t)N;
+        hipLaunchKernelGGL(k_mpnn_harvest, (int)((n3 + 255) / 256), 256, 0, stream, (int)n3, (const double *)r,
+                           (const double *)x, hv_r + (size_t)k * n3, hv_x + (size_t)k * n3);
+        HIP_OR_FAIL(hipGetLastError());
+        return FFH_OK;
     }
     // NoMassCGSolver::solve's loop: spmv(0), then [update(it), spmv(it+1)] until a spmv kernel
     // finds convergence (or it == maxit).  Every blob iterates on its own scalars and stops on
     // its own test; the launches go on until all have stopped.  The control blocks are polled
     // once per chunk.
+    //
+    // With the learned preconditioner (flexible CG), the first F iterations use it (F = the
+    // fallback count, by default the iteration count of the last plain solve); iteration F + 1
+    // restarts Jacobi-PCG from the current iterate (beta = 0), which then runs to the same
+    // stopping test.  Snapshots for training are taken from plain solves only.
     int run_cg() {
-        int rc = launch_spmv(0);
-        if (rc) return rc;
+        const bool learned = learned_solve();
+        const int F = learned ? std::max(1, fallback_iters > 0 ? fallback_iters : plain_count) : 0;
+        const int snap0 = hv_count;
+        const bool harvesting = hv_on && !learned;
+        int rc;
+        if (learned && (rc = mpnn_apply(r, nullptr, false))) return rc;
+        if ((rc = launch_spmv(0, 0))) return rc;
+        if (harvesting && (rc = harvest_snap(snap0))) return rc;
         int it = 0;
         bool all_done = false;
         for (;;) {
-            int chunk = std::max(1, std::min(cg_chunk, maxit - it));
+            int chunk = learned && it < F ? F - it : std::max(1, std::min(cg_chunk, maxit - it));
             for (int k = 0; k < chunk && it < maxit; k++) {
-                if ((rc = launch_update(it))) return rc;
+                const bool lz = learned && it < F;
+                if ((rc = launch_update(it, lz))) return rc;
                 it++;
-                if ((rc = launch_spmv(it))) return rc;
+                if (harvesting && (rc = harvest_snap(snap0 + it))) return rc;
+                if (lz && (rc = mpnn_apply(r, r_prev, true))) return rc;
+                if ((rc = launch_spmv(it, lz ? 1 : (learned && it == F + 1 ? 2 : 0)))) return rc;
             }
             HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, nblobs * sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
             HIP_OR_FAIL(hipStreamSynchronize(stream));
@@ -1048,7 +1260,21 @@ struct ffh_blob {
             if (ctl_host[b].status != 0 && failed < 0) failed = b;
         }
         last_iters = worst;
-        cg_chunk = std::max(1, std::min(worst, 64));
+        if (!learned) {
+            cg_chunk = std::max(1, std::min(worst, 64));
+            plain_count = worst;
+        }
+        if (harvesting) {
+            const int ns = std::min(hv_cap - snap0, worst);
+            for (int k = 0; k < ns; k++) {
+                hipLaunchKernelGGL(k_mpnn_harvest_finish, plan.num_slices, 192, 0, stream, plan.num_slices,
+                                   (const int32_t *)slice_blob, (const CgCtl *)ctl, k, (const double *)x,
+                                   hv_r + (size_t)(snap0 + k) * 3 * N, hv_x + (size_t)(snap0 + k) * 3 * N);
+                HIP_OR_FAIL(hipGetLastError());
+            }
+            for (int b = 0; b < nblobs; b++) hv_pairs += std::min(blob_iters[b], ns);
+            hv_count = snap0 + std::max(ns, 0);
+        }
         if (failed >= 0)
             return fail(FFH_E_CG, "Conjugate gradient solver: could not converge after " + std::to_string(maxit) +
                                       " iterations" + (nblobs > 1 ? " (blob " + std::to_string(failed) + ")" : ""));
@@ -1200,7 +1426,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->alloc(b->vel, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_in, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_out, 3 * (size_t)N);
-    rc |= b->alloc(b->parts, 5 * b->pstride);
+    rc |= b->alloc(b->parts, 9 * b->pstride);
     rc |= b->upload(b->slice_blob, P.slice_blob);
     rc |= b->upload(b->blob_slice_off, P.blob_slice_off);
     rc |= b->alloc(b->ctl, (size_t)b->nblobs);
@@ -1528,6 +1754,164 @@ int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts) {
         if (ms) ms[k] = b->ktime[k];
         if (counts) counts[k] = b->kcount[k];
     }
+    return FFH_OK;
+}
+
+// ---- learned preconditioner -----------------------------------------------------------------
+
+static int mp_ready_check(ffh_blob *b) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (!b->mp_ready) return fail(FFH_E_ARG, "no network weights: call ffh_mpnn_init or ffh_mpnn_set_weights");
+    return FFH_OK;
+}
+
+int ffh_mpnn_num_params(void) { return kNumParams; }
+
+int ffh_mpnn_set_weights(ffh_blob *b, const float *w) {
+    if (!b || !w) return fail(FFH_E_ARG, "null argument");
+    for (int i = 0; i < kNumParams; i++)
+        if (!std::isfinite(w[i])) return fail(FFH_E_ARG, "non-finite network weight");
+    int rc;
+    if ((rc = b->mp_alloc())) return rc;
+    HIP_OR_FAIL(hipMemcpyAsync(b->mp_w, w, kNumParams * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipMemsetAsync(b->mp_m1, 0, kNumParams * sizeof(float), b->stream));
+    HIP_OR_FAIL(hipMemsetAsync(b->mp_m2, 0, kNumParams * sizeof(float), b->stream));
+    b->adam_t = 0;
+    if ((rc = b->mp_pack())) return rc;
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    b->mp_ready = true;
+    return FFH_OK;
+}
+
+int ffh_mpnn_get_weights(ffh_blob *b, float *w) {
+    if (int rc = mp_ready_check(b)) return rc;
+    if (!w) return fail(FFH_E_ARG, "null argument");
+    HIP_OR_FAIL(hipMemcpyAsync(w, b->mp_w, kNumParams * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
+}
+
+int ffh_mpnn_init(ffh_blob *b, unsigned long long seed) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    // He-normal hidden weights, zero biases, zero output layer (the untrained network is Jacobi)
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<float> w(kNumParams, 0.0f);
+    for (int i = 0; i < kHid * kNIn; i++) w[kOffWe + i] = (float)(nd(gen) * std::sqrt(2.0 / kNIn));
+    for (int l = 0; l < kNLay; l++)
+        for (int i = 0; i < 2 * kHid * kHid; i++) w[kOffLay + l * kLayStride + i] = (float)(nd(gen) * std::sqrt(1.0 / kHid));
+    return ffh_mpnn_set_weights(b, w.data());
+}
+
+int ffh_blob_set_preconditioner(ffh_blob *b, int kind, int fallback_iterations) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (kind != FFH_PRECOND_JACOBI && kind != FFH_PRECOND_LEARNED) return fail(FFH_E_ARG, "unknown preconditioner");
+    if (kind == FFH_PRECOND_LEARNED && !b->mp_ready) return mp_ready_check(b);
+    if (fallback_iterations < 0) return fail(FFH_E_ARG, "negative fallback count");
+    b->precond = kind;
+    b->fallback_iters = fallback_iterations;
+    return FFH_OK;
+}
+
+int ffh_mpnn_apply(ffh_blob *b, const double *rin, double *zout) {
+    if (int rc = mp_ready_check(b)) return rc;
+    if (!rin || !zout) return fail(FFH_E_ARG, "null argument");
+    const ffh::Plan &P = b->plan;
+    std::vector<double> rl(3 * (size_t)b->N, 0.0), zl(3 * (size_t)b->N);
+    for (int i = 0; i < b->N; i++)
+        if (P.orig_of[i] >= 0 && !P.pinned[i])
+            for (int c = 0; c < 3; c++) rl[3 * i + c] = rin[3 * (size_t)P.orig_of[i] + c];
+    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, rl.data(), rl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    // u = D^-1 r into z, partials of |u|^2 -- the update kernel's work, here on a given r
+    CgArgs v = b->cg_args();
+    v.r = b->tmp_in;
+    hipLaunchKernelGGL((k_mpnn_u<CgArgs>), P.num_slices, 192, 0, b->stream, v, b->parts_uu());
+    HIP_OR_FAIL(hipGetLastError());
+    if (int rc = b->mpnn_apply(b->tmp_in, nullptr, false)) return rc;
+    HIP_OR_FAIL(hipMemcpyAsync(zl.data(), b->z, zl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    std::memset(zout, 0, sizeof(double) * 3 * (size_t)P.num_nodes_total);
+    for (int i = 0; i < b->N; i++)
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) zout[3 * (size_t)P.orig_of[i] + c] = zl[3 * i + c];
+    return FFH_OK;
+}
+
+int ffh_mpnn_harvest(ffh_blob *b, int max_snapshots) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (max_snapshots < 0) return fail(FFH_E_ARG, "negative snapshot count");
+    int rc;
+    if ((rc = b->mp_alloc())) return rc;
+    if (max_snapshots > b->hv_cap) {
+        const size_t n3 = 3 * (size_t)b->N;
+        float *r = nullptr, *x = nullptr;
+        if ((rc = b->alloc(r, n3 * max_snapshots)) || (rc = b->alloc(x, n3 * max_snapshots))) return rc;
+        b->hv_r = r;  // the old (smaller) buffers stay allocated until the handle is destroyed
+        b->hv_x = x;
+        b->hv_cap = max_snapshots;
+    }
+    b->hv_on = max_snapshots > 0;
+    b->hv_count = 0;
+    b->hv_pairs = 0;
+    return FFH_OK;
+}
+
+int ffh_mpnn_harvest_count(ffh_blob *b, int *snapshots, long long *pairs) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (snapshots) *snapshots = b->hv_count;
+    if (pairs) *pairs = b->hv_pairs;
+    return FFH_OK;
+}
+
+int ffh_mpnn_train(ffh_blob *b, int epochs, float lr, unsigned long long seed, float *loss_per_epoch) {
+    if (int rc = mp_ready_check(b)) return rc;
+    if (epochs < 1 || epochs > 4096 || !(lr > 0)) return fail(FFH_E_ARG, "invalid training parameters");
+    if (b->hv_count < 1) return fail(FFH_E_ARG, "no harvested snapshots: call ffh_mpnn_harvest, then solve");
+    b->hv_on = false;  // the training data are fixed from here on
+    HIP_OR_FAIL(hipMemsetAsync(b->mp_loss, 0, epochs * sizeof(float), b->stream));
+    std::mt19937_64 gen(seed);
+    std::vector<int> order(b->hv_count);
+    for (int i = 0; i < b->hv_count; i++) order[i] = i;
+    const size_t n3 = 3 * (size_t)b->N;
+    for (int e = 0; e < epochs; e++) {
+        std::shuffle(order.begin(), order.end(), gen);
+        for (int s : order) {
+            int rc;
+            if ((rc = b->mpnn_forward_backward(b->hv_r + (size_t)s * n3, b->hv_x + (size_t)s * n3, b->mp_loss + e)))
+                return rc;
+            if ((rc = b->mpnn_adam(lr))) return rc;
+        }
+    }
+    std::vector<float> h(epochs);
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->mp_loss, epochs * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    if (loss_per_epoch)
+        for (int e = 0; e < epochs; e++) loss_per_epoch[e] = h[e] / (float)b->hv_count;
+    return FFH_OK;
+}
+
+int ffh_mpnn_loss_grad(ffh_blob *b, const double *rin, const double *ein, double *loss, float *grad) {
+    if (int rc = mp_ready_check(b)) return rc;
+    if (!rin || !ein) return fail(FFH_E_ARG, "null argument");
+    const ffh::Plan &P = b->plan;
+    const size_t n3 = 3 * (size_t)b->N;
+    std::vector<float> rl(n3, 0.0f), el(n3, 0.0f);
+    for (int i = 0; i < b->N; i++)
+        if (P.orig_of[i] >= 0 && !P.pinned[i])
+            for (int c = 0; c < 3; c++) {
+                rl[3 * i + c] = (float)rin[3 * (size_t)P.orig_of[i] + c];
+                el[3 * i + c] = (float)ein[3 * (size_t)P.orig_of[i] + c];
+            }
+    float *dr = reinterpret_cast<float *>(b->tmp_in), *de = reinterpret_cast<float *>(b->tmp_out);
+    HIP_OR_FAIL(hipMemcpyAsync(dr, rl.data(), n3 * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(de, el.data(), n3 * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipMemsetAsync(b->mp_loss + 4095, 0, sizeof(float), b->stream));
+    if (int rc = b->mpnn_forward_backward(dr, de, b->mp_loss + 4095)) return rc;
+    float lh = 0.0f;
+    HIP_OR_FAIL(hipMemcpyAsync(&lh, b->mp_loss + 4095, sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    if (grad) HIP_OR_FAIL(hipMemcpyAsync(grad, b->mp_g, kNumParams * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    if (loss) *loss = lh;
     return FFH_OK;
 }
 

@@ -178,6 +178,43 @@ int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts);
  * update, in meso energy units. */
 int ffh_blob_strain_energy(ffh_blob *b, double *energy);
 
+/*
+ * Learned preconditioner.  No counterpart in the reference, whose massless solver is
+ * Jacobi-preconditioned CG only (src/NoMassCGSolver.cpp:165-197, :278-292); these calls extend
+ * the Solver behind the same solve().  A mesh message-passing network (features: the
+ * Jacobi-scaled residual normalised per blob and the node's drag share; embedding 4 -> 32; three
+ * layers h' = relu(W1 h + W2 A h + b) with A the trace-normalised blocks of the current matrix;
+ * output 32 -> 3) adds a correction to the Jacobi step: z = D^-1 r + s_b * net.  Its dense layers
+ * run on bf16 MFMA with fp32 accumulation.
+ *
+ * With FFH_PRECOND_LEARNED the solve is flexible CG (Polak-Ribiere beta) for at most
+ * fallback_iterations iterations (0: the iteration count of the last plain solve); blobs not
+ * converged by then restart Jacobi-PCG from their current iterate and run to the same test
+ * ||r||^2 / ||f||^2 < eps^2.  Until a plain solve has been made, solves stay plain.
+ *
+ * Training data: ffh_mpnn_harvest(b, S) records, for the next plain solves, the residual r_k of
+ * every iteration and its exact correction x* - x_k (device memory, up to S snapshots of all
+ * blobs); ffh_mpnn_train then runs Adam over the snapshots (one snapshot per step, seeded
+ * order) with the matrix current at training time, and returns the mean loss per epoch.  The
+ * loss of a snapshot is the mean over blobs of |net - t|^2 / |(x* - x_k) / s_b|^2 with
+ * t = (x* - x_k - D^-1 r_k) / s_b.
+ */
+#define FFH_PRECOND_JACOBI 0
+#define FFH_PRECOND_LEARNED 1
+int ffh_mpnn_num_params(void);
+int ffh_mpnn_init(ffh_blob *b, unsigned long long seed);
+int ffh_mpnn_set_weights(ffh_blob *b, const float *weights);
+int ffh_mpnn_get_weights(ffh_blob *b, float *weights);
+int ffh_blob_set_preconditioner(ffh_blob *b, int kind, int fallback_iterations);
+/* z = the learned preconditioner applied to the host residual r ([3*N_t]) on the current matrix. */
+int ffh_mpnn_apply(ffh_blob *b, const double *r, double *z);
+int ffh_mpnn_harvest(ffh_blob *b, int max_snapshots);
+int ffh_mpnn_harvest_count(ffh_blob *b, int *snapshots, long long *pairs);
+int ffh_mpnn_train(ffh_blob *b, int epochs, float learning_rate, unsigned long long seed, float *loss_per_epoch);
+/* Loss and gradient (ffh_mpnn_num_params floats) of one (residual, correction) pair, [3*N_t]
+ * each, without a weight update. */
+int ffh_mpnn_loss_grad(ffh_blob *b, const double *r, const double *correction, double *loss, float *grad);
+
 #ifdef __cplusplus
 }
 #endif
