@@ -38,6 +38,10 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     # name: (cells, jitter (fraction of h), stiff-core factor, description); cells None: batch
     "c0": ((8, 8, 8), 0.0, 1.0, "cube 8^3 cells, Kuhn 6-tet split: 729 linear nodes, 3,072 tets"),
+    "c3": (None, 0.15, 1.0,
+           "learned preconditioner: trained on 20,000 (residual, correction) pairs harvested from plain CG "
+           "solves of a 64-blob batch drawn like c2 (n in [6, 32]); evaluated on 64 held-out jittered box "
+           "meshes with n in [24, 64] per axis, flexible CG with the network against Jacobi-PCG"),
     "c2": (None, 0.15, 1.0,
            "batch of 256 blobs, box meshes with nx, ny, nz uniform in [6, 32] (seeded), Kuhn 6-tet split, "
            "nodes jittered +-0.15h (no inverted tets), each material parameter scaled by a per-blob "
@@ -53,11 +57,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def box_batch(count, lo, hi, seed, jitter=0.15):
+    """`count` box meshes, nx, ny, nz uniform in [lo, hi], jittered, per-blob material scaling."""
+    from ffea_amd import mesh as M
+    dims = np.random.default_rng(seed).integers(lo, hi + 1, size=(count, 3))
+    ms, xs = [], []
+    for k, d in enumerate(dims):
+        m = M.box_mesh(*map(int, d), jitter=jitter, seed=seed * 1000 + k)
+        M.scale_material(m, 0.5, 2.0, seed=seed * 1000 + 500 + k)
+        ms.append(m)
+        xs.append(M.perturb(m, 0.02, seed=seed * 1000 + 900 + k))
+    return ms, xs
+
+
 def make_case(cfg):
     """Meshes (one per blob), parameters, scale, initial positions and description."""
     from ffea_amd import mesh as M, units as U
     cells, jitter, core, desc = CONFIGS[cfg]
     prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
+    if cfg == "c3":
+        ms, xs = box_batch(64, 24, 64, seed=31)
+        return ms, prm, U.blob_scale(1e-9), xs, desc
     if cells is None:
         dims = np.random.default_rng(2).integers(6, 33, size=(256, 3))
         ms, xs = [], []
@@ -95,7 +115,7 @@ def algorithmic_bytes(kernel, sizes, bm):
         "cg_spmv": 9 * bm * B + 4 * B + 5 * v,
         "cg_update": 8 * v,
         "integrate": 4 * v,
-    }[kernel]
+    }.get(kernel)
 
 
 def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
@@ -140,8 +160,9 @@ def kernel_table(kt, sizes, bm):
             continue
         avg_s = ms / cnt / 1e3
         byts = algorithmic_bytes(k, sizes, bm)
-        out[k] = {"launches": cnt, "avg_us": avg_s * 1e6, "total_ms": ms,
-                  "alg_bytes": int(byts), "achieved_GBs": byts / avg_s / 1e9}
+        out[k] = {"launches": cnt, "avg_us": avg_s * 1e6, "total_ms": ms}
+        if byts is not None:
+            out[k].update(alg_bytes=int(byts), achieved_GBs=byts / avg_s / 1e9)
     return out
 
 
@@ -244,6 +265,93 @@ def copy_calibration(mbytes: int) -> dict:
     return {"bytes": n, "seconds": dt}
 
 
+def cg_phase(b, steps, kind):
+    """`steps` time steps with preconditioner `kind`; CG time per step from the kernel events
+    (force gather + CG init, SpMV, update, and the network's kernels), per-blob iterations."""
+    b.set_preconditioner(kind)
+    b.step(2)
+    b.set_profiling(True)
+    its = []
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step(1)
+        its.append(b.last_cg_per_blob()[0].copy() if hasattr(b, "last_cg_per_blob") else np.array([b.last_cg()[0]]))
+    b.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    kt = b.kernel_times()
+    b.set_profiling(False)
+    cg_ms = sum(v[0] for k, v in kt.items() if k not in ("element", "assemble", "integrate")) / steps
+    its = np.array(its)
+    return {"cg_ms_per_solve": cg_ms, "ms_per_step_with_events": 1e3 * wall,
+            "iterations_mean_per_blob": float(its.mean()), "iterations_max": int(its.max()),
+            "launched_iterations_mean": float(its.max(axis=1).mean()),
+            "kernels_us": {k: v[0] / v[1] * 1e3 for k, v in kt.items() if v[1]}}
+
+
+def run_c3(a):
+    """Learned preconditioner: harvest, train, evaluate on held-out meshes."""
+    from ffea_amd import units as U
+    from ffea_amd.blob import Batch, package_seed
+    prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
+    sc = U.blob_scale(1e-9)
+    t0 = time.time()
+    tm, tx = box_batch(64, 6, 32, seed=13)
+    tb = Batch(tm, prm, sc, rng_state=package_seed(prm.rng_seed), precision="mixed")
+    tb.set_positions(np.concatenate(tx) * sc)
+    tb.mpnn_init(seed=1)
+    tb.mpnn_harvest(1200)
+    steps = 0
+    while tb.mpnn_harvest_count()[1] < 20000 and steps < 200:
+        tb.step(1)
+        steps += 1
+    snaps, pairs = tb.mpnn_harvest_count()
+    log(f"[c3] harvested {pairs} pairs in {snaps} snapshots over {steps} steps ({time.time() - t0:.1f}s)")
+    t1 = time.perf_counter()
+    loss = tb.mpnn_train(epochs=a.epochs, lr=2e-3, seed=1)
+    train_s = time.perf_counter() - t1
+    log(f"[c3] trained {a.epochs} epochs in {train_s:.1f}s, loss {loss[0]:.4f} -> {loss[-1]:.4f}")
+    w = tb.mpnn_weights()
+    train_sizes = tb.sizes()
+    tb.close()
+    ms, prm, sc, xs, desc = make_case("c3")
+    res = {}
+    for prec in ("fp64", "mixed"):
+        t0 = time.time()
+        b = Batch(ms, prm, sc, rng_state=package_seed(prm.rng_seed), precision=prec)
+        b.set_positions(np.concatenate(xs) * sc)
+        log(f"[c3 {prec}] {len(ms)} held-out blobs created in {time.time() - t0:.1f}s: {b.sizes()}")
+        b.mpnn_set_weights(w)
+        b.step(a.warmup)
+        res[f"{prec}_jacobi"] = cg_phase(b, a.steps, "jacobi")
+        res[f"{prec}_learned"] = cg_phase(b, a.steps, "learned")
+        for k in ("jacobi", "learned"):
+            r = res[f"{prec}_{k}"]
+            log(f"[c3 {prec} {k}] CG {r['cg_ms_per_solve']:.2f} ms/solve, iterations per blob "
+                f"{r['iterations_mean_per_blob']:.2f} (max {r['iterations_max']})")
+        sizes = b.sizes()
+        b.close()
+    base = res["fp64_jacobi"]["cg_ms_per_solve"]
+    out = {
+        "metric": "c3: CG time-to-solution at eps=1e-6, learned-preconditioned mixed flexible CG vs plain FP64 "
+                  "Jacobi-PCG (ratio of CG ms per solve)",
+        "value": base / res["mixed_learned"]["cg_ms_per_solve"],
+        "unit": "x (higher is better)",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": res["mixed_learned"]["ms_per_step_with_events"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network, f32/f64 CG",
+        "data": "synthetic: seeded jittered box meshes, RngStream noise",
+        "config": {"workload": f"c3: {desc}", "held_out_linear_nodes": sizes["num_linear"],
+                   "held_out_elements": sizes["num_elements"], "train_linear_nodes": train_sizes["num_linear"],
+                   "parallelism": "single GPU"},
+        "training": {"pairs": pairs, "snapshots": snaps, "epochs": a.epochs, "seconds": train_s,
+                     "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1])},
+        "solvers": res,
+        "speedup_mixed_jacobi_vs_fp64_jacobi": base / res["mixed_jacobi"]["cg_ms_per_solve"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +361,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--epochs", type=int, default=20, help="c3: training epochs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -262,6 +371,8 @@ def main():
               "boundary exchange and two scalar all-reduces per CG iteration)",
               file=sys.stderr)
         sys.exit(2)
+    if a.config == "c3":
+        return run_c3(a)
     ms, prm, sc, xs, desc = make_case(a.config)
     log(f"workload {a.config}: {desc}")
     sps, wall, kt, sizes, iters, prof_step = run_gpu(ms, prm, sc, xs, "fp64", a.steps, a.warmup)

@@ -1124,11 +1124,13 @@ struct ffh_blob {
         if (mp_edges) return FFH_OK;
         SpmvArgs m = spmv_args();
         const int g = (plan.num_slices + 3) / 4;
+        const int S = plan.num_slices;
+        int rc;
         if (precision == FFH_PRECISION_FP64)
-            hipLaunchKernelGGL((k_mpnn_edges<double>), g, 256, 0, stream, m, plan.num_slices, dinv, zeta, mp_aw, mp_cf);
+            rc = launch(6, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_mpnn_edges<double>), g, 256, 0, stream, e0, e1, 0, m, S, (const double *)dinv, (const double *)zeta, mp_aw, mp_cf); });
         else
-            hipLaunchKernelGGL((k_mpnn_edges<float>), g, 256, 0, stream, m, plan.num_slices, dinv, zeta, mp_aw, mp_cf);
-        HIP_OR_FAIL(hipGetLastError());
+            rc = launch(6, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_mpnn_edges<float>), g, 256, 0, stream, e0, e1, 0, m, S, (const double *)dinv, (const double *)zeta, mp_aw, mp_cf); });
+        if (rc) return rc;
         mp_edges = true;
         return FFH_OK;
     }
@@ -1143,19 +1145,31 @@ struct ffh_blob {
         if ((rc = mp_edges_now())) return rc;
         MpnnArgs a = mp_args(skip_done);
         const int S = plan.num_slices;
-        hipLaunchKernelGGL((k_mpnn_embed<double>), P_slice, 256, 0, stream, a, (const double *)z, (const double *)parts_uu(),
-                           (float *)nullptr, mp_H[0], (const float *)nullptr, (float *)nullptr, (double *)nullptr);
+        const double *uu = parts_uu();
+        rc = launch(7, [&](hipEvent_t e0, hipEvent_t e1) {
+            hipExtLaunchKernelGGL((k_mpnn_embed<double>), P_slice, 256, 0, stream, e0, e1, 0, a, (const double *)z, uu,
+                                  (float *)nullptr, mp_H[0], (const float *)nullptr, (float *)nullptr, (double *)nullptr);
+        });
+        if (rc) return rc;
         bf16_t *h = mp_H[0], *hn = mp_H[1];
         for (int l = 0; l < kNLay; l++) {
-            hipLaunchKernelGGL((k_mpnn_spmm<bf16_t, false>), P_slice, 256, 0, stream, a, (const bf16_t *)h, kHid, 0,
-                               (void *)mp_G[1], (const bf16_t *)nullptr);
-            hipLaunchKernelGGL(k_mpnn_dense, S, 128, 0, stream, a, l, (const bf16_t *)h, (const bf16_t *)mp_G[1], hn);
+            rc = launch(8, [&](hipEvent_t e0, hipEvent_t e1) {
+                hipExtLaunchKernelGGL((k_mpnn_spmm<bf16_t, false>), P_slice, 256, 0, stream, e0, e1, 0, a, (const bf16_t *)h,
+                                      kHid, 0, (void *)mp_G[1], (const bf16_t *)nullptr);
+            });
+            if (rc) return rc;
+            rc = launch(9, [&](hipEvent_t e0, hipEvent_t e1) {
+                hipExtLaunchKernelGGL(k_mpnn_dense, S, 128, 0, stream, e0, e1, 0, a, l, (const bf16_t *)h,
+                                      (const bf16_t *)mp_G[1], hn);
+            });
+            if (rc) return rc;
             std::swap(h, hn);
         }
-        hipLaunchKernelGGL(k_mpnn_out_cg, P_slice, 192, 0, stream, a, (const bf16_t *)h, (const double *)parts_uu(), z,
-                           rvec, rprev, parts, rprev ? parts_zrp() : (double *)nullptr);
-        HIP_OR_FAIL(hipGetLastError());
-        return FFH_OK;
+        double *zrp = rprev ? parts_zrp() : (double *)nullptr;
+        return launch(10, [&](hipEvent_t e0, hipEvent_t e1) {
+            hipExtLaunchKernelGGL(k_mpnn_out_cg, P_slice, 192, 0, stream, e0, e1, 0, a, (const bf16_t *)h, uu, z, rvec, rprev,
+                                  parts, zrp);
+        });
     }
     // One training pass over a snapshot (residual rs, target es = x* - x_k, float [3N]):
     // forward, loss into *loss_slot (device), gradient into mp_g.

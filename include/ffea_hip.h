@@ -169,8 +169,10 @@ int ffh_blob_synchronize(ffh_blob *b);
  * milliseconds and launch counts since the last reset.  Kernel ids:
  *   0 element   1 assemble   2 rhs+CG init   3 CG spmv (p update, q = K p, p.q)
  *   4 CG update (x, r, residual)   5 integrate
+ *   learned preconditioner: 6 couplings A_ij   7 embedding   8 message (A h)
+ *   9 dense layer (MFMA)   10 output layer
  */
-#define FFH_NUM_KERNELS 6
+#define FFH_NUM_KERNELS 11
 int ffh_blob_set_profiling(ffh_blob *b, int enable);
 int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts);
 
