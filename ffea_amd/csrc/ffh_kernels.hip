@@ -189,6 +189,37 @@ __device__ __forceinline__ double parts_range(const double *__restrict__ parts, 
     return block_sum_all(s, sh);
 }
 
+// Per-blob totals.  A blob of many slices would make every consumer block re-add thousands of
+// partials (O(slices^2) work in all); above kTotSlicesMin slices per blob a small kernel adds
+// each blob's partials once (same fixed order) into tot[blob][slot] and consumers read that.
+constexpr int kTotFF = 0, kTotRZ = 1, kTotRR = 2, kTotPQ = 3, kTotUU = 4, kTotZRP = 5, kTotDEN = 6;
+constexpr int kTotSlots = 8;
+constexpr int kTotSlicesMin = 3072;
+
+__device__ __forceinline__ double blob_total(const double *__restrict__ parts, const double *__restrict__ tot, int slot,
+                                             int b, int s0, int s1, double *sh) {
+    if (tot) return tot[b * kTotSlots + slot];
+    return parts_range(parts, s0, s1, sh);
+}
+
+struct TotArgs {
+    const double *parts[kTotSlots];
+    const int32_t *blob_slice_off;
+    double *tot;
+    int mask;
+};
+
+__global__ __launch_bounds__(192) void k_blob_totals(TotArgs t) {
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
+    const int s0 = t.blob_slice_off[b], s1 = t.blob_slice_off[b + 1];
+    for (int k = 0; k < kTotSlots; k++)
+        if ((t.mask >> k) & 1) {
+            const double v = parts_range(t.parts[k], s0, s1, sh);
+            if (threadIdx.x == 0) t.tot[b * kTotSlots + k] = v;
+        }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The gathering
 // kernels remap the dispatch index so that XCD x works on the contiguous logical blocks
 // [x per, (x+1) per): neighbouring rows, elements and records then share one L2.  The grid is
@@ -496,6 +527,7 @@ struct CgArgs {
     // learned preconditioner (nullptr otherwise): partials of |D^-1 r|^2, r before the update,
     // partials of z.r_prev for the flexible beta
     double *parts_uu, *r_prev, *parts_zrp;
+    const double *tot;  // per-blob totals (large blobs) or nullptr: consumers add the partials
 };
 
 struct RhsArgs {
@@ -608,8 +640,8 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     const bool decider = slice == s0 && threadIdx.x == 0;
     double beta = 0.0;
     if (it == 0) {
-        const double f2 = parts_range(v.parts_ff, s0, s1, sh);
-        const double d0 = parts_range(v.parts_rz, s0, s1, sh);
+        const double f2 = blob_total(v.parts_ff, v.tot, kTotFF, b, s0, s1, sh);
+        const double d0 = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
         if (decider) {
             ctl->f2 = f2;
             ctl->delta[0] = d0;
@@ -622,7 +654,7 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
         if (f2 == 0.0) return;
     } else {
         const double f2 = ctl->f2;
-        const double r2 = parts_range(v.parts_rr, s0, s1, sh);
+        const double r2 = blob_total(v.parts_rr, v.tot, kTotRR, b, s0, s1, sh);
         const double res = r2 / f2;
         if (res < v.eps2 || it >= v.maxit) {
             if (decider) {
@@ -633,11 +665,11 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             }
             return;
         }
-        const double dnew = parts_range(v.parts_rz, s0, s1, sh);
+        const double dnew = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
         if (mode == 0)
             beta = dnew / ctl->delta[(it - 1) & 1];
         else if (mode == 1)
-            beta = (dnew - parts_range(v.parts_zrp, s0, s1, sh)) / ctl->delta[(it - 1) & 1];
+            beta = (dnew - blob_total(v.parts_zrp, v.tot, kTotZRP, b, s0, s1, sh)) / ctl->delta[(it - 1) & 1];
         if (decider) ctl->delta[it & 1] = dnew;
     }
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -676,7 +708,7 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     CgCtl *ctl = &v.ctl[b];
     if (ctl->done) return;
     const double alpha =
-        ctl->delta[it & 1] / parts_range(v.parts_pq, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
+        ctl->delta[it & 1] / blob_total(v.parts_pq, v.tot, kTotPQ, b, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
     const size_t k = (size_t)slice * 192 + threadIdx.x;
     v.x[k] += alpha * v.p[k];
     const double r0 = v.r[k];
@@ -817,6 +849,8 @@ struct ffh_blob {
     size_t pstride = 0;  // partials per reduction: max(slices, element blocks)
     int32_t *slice_blob = nullptr, *blob_slice_off = nullptr;
     int nblobs = 1;
+    bool use_tot = false;  // some blob has more than kTotSlicesMin slices
+    double *tot = nullptr;
     CgCtl *ctl = nullptr;       // [nblobs]; ctl[0].inversions counts the whole batch
     CgCtl *ctl_host = nullptr;  // pinned host mirror
     uint32_t *rng_state = nullptr;
@@ -954,9 +988,13 @@ struct ffh_blob {
         v.parts_uu = nullptr;
         v.r_prev = nullptr;
         v.parts_zrp = parts_zrp();
+        v.tot = use_tot ? tot : nullptr;
         return v;
     }
     double *parts_energy() const { return parts + 4 * pstride; }
+    double *parts_rr() const { return parts + 1 * pstride; }
+    double *parts_ff() const { return parts + 2 * pstride; }
+    double *parts_pq() const { return parts + 3 * pstride; }
     double *parts_uu() const { return parts + 5 * pstride; }
     double *parts_zrp() const { return parts + 6 * pstride; }
     double *parts_den() const { return parts + 7 * pstride; }
@@ -1047,16 +1085,24 @@ struct ffh_blob {
         RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1, f_in);
         CgArgs v = cg_args();
         if (learned_solve()) v.parts_uu = parts_uu();
+        int rc;
         if (precision == FFH_PRECISION_FP64)
-            return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
-        return launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+        else
+            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+        if (rc) return rc;
+        return totals((1 << kTotFF) | (1 << kTotRZ) | (v.parts_uu ? 1 << kTotUU : 0));
     }
     int launch_spmv(int it, int mode) {
         CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
+        int rc;
         if (precision == FFH_PRECISION_FP64)
-            return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
-        return launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
+            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
+        else
+            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
+        if (rc) return rc;
+        return totals(1 << kTotPQ);
     }
     int launch_update(int it, bool learned_next) {
         CgArgs v = cg_args();
@@ -1064,7 +1110,19 @@ struct ffh_blob {
             v.parts_uu = parts_uu();
             v.r_prev = r_prev;
         }
-        return launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_slice, 192, 0, stream, e0, e1, 0, v, it); });
+        int rc = launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_slice, 192, 0, stream, e0, e1, 0, v, it); });
+        if (rc) return rc;
+        return totals((1 << kTotRR) | (1 << kTotRZ) | (learned_next ? 1 << kTotUU : 0));
+    }
+    int totals(int mask) {
+        if (!use_tot) return FFH_OK;
+        TotArgs t;
+        double *pp[kTotSlots] = {parts_ff(), parts, parts_rr(), parts_pq(), parts_uu(), parts_zrp(), parts_den(), parts};
+        for (int k = 0; k < kTotSlots; k++) t.parts[k] = pp[k];
+        t.blob_slice_off = blob_slice_off;
+        t.tot = tot;
+        t.mask = mask;
+        return launch(11, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_blob_totals, nblobs, 192, 0, stream, e0, e1, 0, t); });
     }
     // ---- learned preconditioner ----
     bool learned_solve() const { return precond == FFH_PRECOND_LEARNED && mp_ready && plain_count >= 0; }
@@ -1118,6 +1176,7 @@ struct ffh_blob {
         a.wT = mp_wT;
         a.wN = mp_wN;
         a.ctl = skip_done ? ctl : nullptr;
+        a.tot = use_tot ? tot : nullptr;
         return a;
     }
     int mp_edges_now() {
@@ -1166,10 +1225,12 @@ struct ffh_blob {
             std::swap(h, hn);
         }
         double *zrp = rprev ? parts_zrp() : (double *)nullptr;
-        return launch(10, [&](hipEvent_t e0, hipEvent_t e1) {
+        rc = launch(10, [&](hipEvent_t e0, hipEvent_t e1) {
             hipExtLaunchKernelGGL(k_mpnn_out_cg, P_slice, 192, 0, stream, e0, e1, 0, a, (const bf16_t *)h, uu, z, rvec, rprev,
                                   parts, zrp);
         });
+        if (rc) return rc;
+        return totals((1 << kTotRZ) | (rprev ? 1 << kTotZRP : 0));
     }
     // One training pass over a snapshot (residual rs, target es = x* - x_k, float [3N]):
     // forward, loss into *loss_slot (device), gradient into mp_g.
@@ -1179,8 +1240,10 @@ struct ffh_blob {
         MpnnArgs a = mp_args(false);
         const int S = plan.num_slices;
         hipLaunchKernelGGL(k_mpnn_prep, S, 192, 0, stream, S, rs, (const double *)dinv, mp_u, parts_uu());
+        if ((rc = totals(1 << kTotUU))) return rc;
         hipLaunchKernelGGL((k_mpnn_embed<float>), P_slice, 256, 0, stream, a, (const float *)mp_u,
                            (const double *)parts_uu(), mp_feat, mp_H[0], es, mp_tgt, parts_den());
+        if ((rc = totals(1 << kTotDEN))) return rc;
         for (int l = 1; l <= kNLay; l++) {
             hipLaunchKernelGGL((k_mpnn_spmm<bf16_t, false>), P_slice, 256, 0, stream, a, (const bf16_t *)mp_H[l - 1],
                                kHid, 0, (void *)mp_G[l], (const bf16_t *)nullptr);
@@ -1376,6 +1439,8 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     b->blob_iters.assign(b->nblobs, 0);
     b->blob_res.assign(b->nblobs, 0.0);
     b->blob_status.assign(b->nblobs, 0);
+    for (int k = 0; k < b->nblobs; k++)
+        if (P.blob_slice_off[k + 1] - P.blob_slice_off[k] > kTotSlicesMin) b->use_tot = true;
     if (hipStreamCreate(&b->stream) != hipSuccess) {
         fail(FFH_E_HIP, "hipStreamCreate failed");
         return nullptr;
@@ -1444,6 +1509,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->upload(b->slice_blob, P.slice_blob);
     rc |= b->upload(b->blob_slice_off, P.blob_slice_off);
     rc |= b->alloc(b->ctl, (size_t)b->nblobs);
+    rc |= b->alloc(b->tot, (size_t)b->nblobs * kTotSlots);
     rc |= b->alloc(b->rng_state, 6);
     rc |= b->upload(b->jump, P.jump);
     if (rc) {
@@ -1841,6 +1907,7 @@ int ffh_mpnn_apply(ffh_blob *b, const double *rin, double *zout) {
     v.r = b->tmp_in;
     hipLaunchKernelGGL((k_mpnn_u<CgArgs>), P.num_slices, 192, 0, b->stream, v, b->parts_uu());
     HIP_OR_FAIL(hipGetLastError());
+    if (int rc = b->totals(1 << kTotUU)) return rc;
     if (int rc = b->mpnn_apply(b->tmp_in, nullptr, false)) return rc;
     HIP_OR_FAIL(hipMemcpyAsync(zl.data(), b->z, zl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));

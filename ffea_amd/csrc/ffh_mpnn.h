@@ -66,6 +66,7 @@ struct MpnnArgs {
     const bf16_t *wT;   // [3][32][64]: forward B operand, wT[l][j][k] = Wcat_l[k][j]
     const bf16_t *wN;   // [3][64][32]: backward B operand, wN[l][k][j] = Wcat_l[k][j]
     const CgCtl *ctl;   // nullptr in training: no blob is skipped
+    const double *tot;  // per-blob totals or nullptr
 };
 
 __device__ __forceinline__ bool blob_done(const MpnnArgs &a, int slice) {
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void k_mpnn_edges(SpmvArgs m, int nslices, con
 
 // Per-blob scale s_b from the per-slice partials of |u|^2 (identical in every block).
 __device__ __forceinline__ double blob_scale(const MpnnArgs &a, const double *parts_uu, int b, double *sh) {
-    const double uu = parts_range(parts_uu, a.blob_slice_off[b], a.blob_slice_off[b + 1], sh);
+    const double uu = blob_total(parts_uu, a.tot, kTotUU, b, a.blob_slice_off[b], a.blob_slice_off[b + 1], sh);
     const int n = a.blob_nreal[b];
     return n > 0 ? sqrt(uu / n) : 0.0;
 }
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(64) void k_mpnn_out_train(MpnnArgs a, const bf16_t 
     __shared__ float red[kOutGrad][65];
     const int slice = blockIdx.x;
     const int b = a.slice_blob[slice];
-    const double den = parts_range(parts_den, a.blob_slice_off[b], a.blob_slice_off[b + 1], sh);
+    const double den = blob_total(parts_den, a.tot, kTotDEN, b, a.blob_slice_off[b], a.blob_slice_off[b + 1], sh);
     const int row = slice * 64 + threadIdx.x;
     const bf16_t *h = H3 + (size_t)row * kHid;
     float hv[kHid];

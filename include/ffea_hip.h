@@ -171,8 +171,9 @@ int ffh_blob_synchronize(ffh_blob *b);
  *   4 CG update (x, r, residual)   5 integrate
  *   learned preconditioner: 6 couplings A_ij   7 embedding   8 message (A h)
  *   9 dense layer (MFMA)   10 output layer
+ *   11 per-blob totals of the CG reductions (blobs of more than 3072 64-row slices)
  */
-#define FFH_NUM_KERNELS 11
+#define FFH_NUM_KERNELS 12
 int ffh_blob_set_profiling(ffh_blob *b, int enable);
 int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts);
 

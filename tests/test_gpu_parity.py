@@ -169,3 +169,24 @@ def test_strain_energy_against_oracle(oracle):
     b = gpu_blob(m, prm, sc, np.arange(1, 7, dtype=np.uint64), x0)
     assert b.update_internal_forces() == 0
     assert b.strain_energy() == pytest.approx(oracle.strain_energy(bo), rel=1e-12)
+
+
+def test_large_blob_per_blob_totals(oracle):
+    """A blob of more than 3072 slices (61 x 59 x 59 nodes) takes the per-blob totals path of the
+    CG reductions: one noisy step against the oracle, same bars as the c0 trajectory."""
+    import pyoracle
+    m, prm, sc, x0 = box_case((60, 58, 58), noise=1)
+    seed = oracle.package_seed(44)
+    oracle.set_package_seed(seed)
+    s = oracle.stream()
+    bo = oracle.blob(m, prm, sc, pos=x0)
+    b = gpu_blob(m, prm, sc, seed, x0)
+    assert b.sizes()["num_linear"] > 3072 * 64
+    print("large blob", b.sizes(), flush=True)
+    assert pyoracle.world_step(oracle, [bo], s) == 0
+    b.step(1)
+    assert np.array_equal(b.rhs().ravel(), bo.get("rhs"))
+    po = bo.get("pos")
+    assert np.linalg.norm(b.positions().ravel() - po) <= 1e-9 * np.linalg.norm(po - x0.ravel() * sc)
+    assert b.last_cg()[0] == oracle.cg_iterations(bo)
+    assert np.array_equal(b.rng_state(), s.state())
