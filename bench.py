@@ -230,6 +230,65 @@ def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
                       f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
 
 
+def parity_vs_reference(cfg, ms, prm, sc, xs, steps):
+    """Noise-off parity of the GPU paths against the reference's own objects built strict
+    (oracle/_ref/libffea_ref.so, one thread; the FP64 restatement if absent): the CG solution
+    of the first solve (same force vector) and positions / velocities after `steps` steps."""
+    import threading
+    from ffea_amd import units as U
+    from ffea_amd.blob import Blob
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if len(ms) != 1:
+        return None
+    m, x0 = ms[0], xs[0]
+    p0 = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=0, calc_stokes=1, rng_seed=prm.rng_seed)
+    ref = {}
+
+    def cpu():
+        kind = "reference" if os.path.exists(pyoracle.REF_SO) else "port"
+        lib = pyoracle.Reference() if kind == "reference" else pyoracle.Oracle()
+        lib.set_package_seed(lib.package_seed(p0.rng_seed))
+        s = lib.stream()
+        b = lib.blob(m, p0, sc, pos=x0)
+        inv, rhs = b.forces_only(s)
+        st, v = b.solve(rhs)
+        b.close()
+        b = lib.blob(m, p0, sc, pos=x0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert pyoracle.world_step(lib, [b], s) == 0
+        ref.update(kind=kind, rhs=rhs, v=v, pos=b.get("pos"), vel=b.get("vel"), secs=time.perf_counter() - t0)
+        b.close()
+
+    old = threading.stack_size(2 << 30)
+    try:
+        t = threading.Thread(target=cpu)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if "pos" not in ref:
+        return None
+    xs0 = x0.ravel() * sc
+    out = {"reference": ref["kind"], "steps": steps, "noise": "off", "cpu_seconds": ref["secs"]}
+    for prec in ("fp64", "mixed"):
+        b = Blob(m, p0, sc, rng_state=np.arange(1, 7, dtype=np.uint64), precision=prec, pos=x0)
+        b.update_internal_forces()
+        v = b.solve(ref["rhs"]).ravel()
+        b.set_positions(x0 * sc)
+        b.step(steps)
+        p, vel = b.positions().ravel(), b.velocities().ravel()
+        b.close()
+        rel = lambda a, r: float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
+        out[prec] = {"cg_solution_rel": rel(v, ref["v"]), "velocity_rel": rel(vel, ref["vel"]),
+                     "position_rel_to_displacement": float(np.linalg.norm(p - ref["pos"]) /
+                                                           np.linalg.norm(ref["pos"] - xs0)),
+                     "position_rel": rel(p, ref["pos"])}
+    log(f"[parity {cfg}] vs {ref['kind']} over {steps} noise-off steps: {out}")
+    return out
+
+
 KERNEL_SYMBOL = {"element": "k_element<{t}, {t}>", "assemble": "k_assemble<{t}>", "rhs": "k_rhs<{t}>",
                  "cg_spmv": "k_cg_spmv<{t}>", "cg_update": "k_cg_update", "integrate": "k_integrate"}
 
@@ -362,6 +421,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--epochs", type=int, default=20, help="c3: training epochs")
+    ap.add_argument("--parity-steps", type=int, default=10,
+                    help="noise-off steps compared with the reference's objects (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -389,6 +450,9 @@ def main():
             "traffic": measured_traffic(a.config, dom, "fp64"),
             "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
     cpu = None if a.no_cpu else cpu_baseline(a.config, ms, prm, sc, xs, a.cpu_seconds)
+    parity = None
+    if not a.no_cpu and a.parity_steps > 0:
+        parity = parity_vs_reference(a.config, ms, prm, sc, xs, a.parity_steps)
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
     out = {
         "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
@@ -413,6 +477,7 @@ def main():
         "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "parity_vs_reference": parity,
     }
     if calib:
         out["copy_calibration"] = calib

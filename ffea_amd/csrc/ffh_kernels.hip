@@ -209,13 +209,25 @@ struct TotArgs {
     int mask;
 };
 
-__global__ __launch_bounds__(192) void k_blob_totals(TotArgs t) {
-    __shared__ double sh[4];
+// One block of 1024 threads per blob; each thread adds every 1024th partial, four loads in
+// flight, then a block tree -- a fixed order.
+__global__ __launch_bounds__(1024) void k_blob_totals(TotArgs t) {
+    __shared__ double sh[16];
     const int b = blockIdx.x;
     const int s0 = t.blob_slice_off[b], s1 = t.blob_slice_off[b + 1];
     for (int k = 0; k < kTotSlots; k++)
         if ((t.mask >> k) & 1) {
-            const double v = parts_range(t.parts[k], s0, s1, sh);
+            const double *p = t.parts[k];
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int i = s0 + (int)threadIdx.x;
+            for (; i + 3 * 1024 < s1; i += 4 * 1024) {
+                a0 += p[i];
+                a1 += p[i + 1024];
+                a2 += p[i + 2 * 1024];
+                a3 += p[i + 3 * 1024];
+            }
+            for (; i < s1; i += 1024) a0 += p[i];
+            const double v = block_sum_all((a0 + a1) + (a2 + a3), sh);
             if (threadIdx.x == 0) t.tot[b * kTotSlots + k] = v;
         }
 }
@@ -1122,7 +1134,7 @@ struct ffh_blob {
         t.blob_slice_off = blob_slice_off;
         t.tot = tot;
         t.mask = mask;
-        return launch(11, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_blob_totals, nblobs, 192, 0, stream, e0, e1, 0, t); });
+        return launch(11, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_blob_totals, nblobs, 1024, 0, stream, e0, e1, 0, t); });
     }
     // ---- learned preconditioner ----
     bool learned_solve() const { return precond == FFH_PRECOND_LEARNED && mp_ready && plain_count >= 0; }
