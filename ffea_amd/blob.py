@@ -96,6 +96,11 @@ class Blob:
     def step(self, nsteps: int = 1) -> None:
         _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
 
+    def steps_done(self) -> int:
+        n = C.c_longlong(0)
+        _lib.check(self._L.ffh_blob_steps_done(self._h, C.byref(n)))
+        return n.value
+
     # -- solver interface -------------------------------------------------------------
     def solve(self, force) -> np.ndarray:
         x = np.ascontiguousarray(force, np.float64).copy().ravel()

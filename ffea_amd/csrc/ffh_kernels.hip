@@ -876,6 +876,7 @@ struct ffh_blob {
     std::vector<hipEvent_t> event_pool;
     int last_iters = 0;
     double last_res = 0;
+    long long steps_done = 0;  // completed position phases
     // learned preconditioner (allocated on first use)
     int precond = FFH_PRECOND_JACOBI;
     int fallback_iters = 0;  // learned iterations before the Jacobi restart (0: plain_count)
@@ -1658,6 +1659,7 @@ int ffh_blob_update_positions(ffh_blob *b) {
     if ((rc = b->run_rhs())) return rc;
     int cg = b->run_cg();
     if ((rc = b->run_integrate())) return rc;
+    b->steps_done++;
     return cg;
 }
 
@@ -1676,8 +1678,15 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
             return b->read_inversions(&ninv);
         }
         if ((rc = b->run_integrate())) return rc;
+        b->steps_done++;
         if (cg) return cg;
     }
+    return FFH_OK;
+}
+
+int ffh_blob_steps_done(ffh_blob *b, long long *steps) {
+    if (!b || !steps) return fail(FFH_E_ARG, "null argument");
+    *steps = b->steps_done;
     return FFH_OK;
 }
 

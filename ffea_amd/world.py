@@ -1,0 +1,292 @@
+"""World-level driver of the massless path: a .ffea script's dynamic CG_nomass blobs stepped
+together on one GPU (ffea_amd.blob.Batch), with FFEA's output and checkpoint files.
+
+Follows World::run's order per step (src/World.cpp:1811-2043):
+  1. at check steps, the RngStream state is written to the checkpoint file (before the element
+     phase consumes draws; World::print_checkpoints, src/World.cpp:4234-4280);
+  2. element phase of every blob (Blob::update_internal_forces);
+  3. at check steps, a trajectory frame and a measurement line
+     (World::print_trajectory_and_measurement_files, src/World.cpp:4125-4231): node positions
+     at the start of the step, strain energy of this element phase, centroid and RMSD;
+  4. position phase of every blob (Blob::update_positions).
+Steps run from step_initial to num_steps inclusive.
+
+File formats (written as the reference writes them):
+  .ftj  header (src/World.cpp:532-548), frames of "Blob b, Conformation 0, step s" + DYNAMIC +
+        one "%e" x 10 line per node (positions in m, then 7 zeros on the massless path,
+        Blob::write_nodes_to_file src/Blob.cpp:1143-1180), "*", the conformation-change block, "*";
+  .fm   "Measurements:" then Time StrainEnergy Centroid.x/y/z RMSD in "%-14.6e" columns
+        (World::write_measurements_to_file, src/World.cpp:4342-4366);
+  .fcp  "RNGStreams dedicated to the thermal stress: K" then K lines of six integers.
+Restart (restart = 1, checkpoint_in; src/World.cpp:645-800): positions from the last complete
+trajectory frame (which is removed and written again), the last measurement line removed,
+RngStream state from checkpoint_in, stepping resumes at that frame's step.
+
+Deliberate differences: the checkpoint file is rewritten whole at each check (the reference
+rewinds without truncating); the .fm header lists this implementation's parameters; default
+output names are <script>.ftj / <script>.fm / <script>.fcp.
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from . import formats as F
+from . import units as U
+from .blob import Batch, package_seed
+
+
+# ---------------------------------------------------------------------------------------------
+# file formats
+# ---------------------------------------------------------------------------------------------
+def traj_header(nodes_per_blob) -> str:
+    s = "FFEA_trajectory_file\n\nInitialisation:\nNumber of Blobs %d\nNumber of Conformations" % len(nodes_per_blob)
+    s += "".join(" %d" % 1 for _ in nodes_per_blob) + "\n"
+    for b, n in enumerate(nodes_per_blob):
+        s += "Blob %d:\t" % b + "Conformation %d Nodes %d\t" % (0, n) + "\n"
+    return s + "\n*\n"
+
+
+def traj_frame(step: int, positions_m) -> str:
+    """One frame; positions_m: per blob [N_t, 3] in metres."""
+    out = []
+    for b, p in enumerate(positions_m):
+        out.append("Blob %d, Conformation %d, step %d\nDYNAMIC\n" % (b, 0, step))
+        out.append("".join("%e %e %e %e %e %e %e %e %e %e\n" % (x, y, z, 0., 0., 0., 0., 0., 0., 0.)
+                           for x, y, z in np.asarray(p, np.float64)))
+    out.append("*\nConformation Changes:\n")
+    out.append("".join("Blob %d: Conformation %d -> Conformation %d\n" % (b, 0, 0) for b in range(len(positions_m))))
+    out.append("*\n")
+    return "".join(out)
+
+
+MEAS_COLUMNS = ("Time", "StrainEnergy", "Centroid.x", "Centroid.y", "Centroid.z", "RMSD")
+
+
+def meas_line(values) -> str:
+    return "".join("%-14.6e" % v for v in values) + "\n"
+
+
+def checkpoint_text(states) -> str:
+    s = "RNGStreams dedicated to the thermal stress: %d\n" % len(states)
+    return s + "".join("%d %d %d %d %d %d\n" % tuple(int(v) for v in st) for st in states)
+
+
+def read_checkpoint(path):
+    """States of the thermal-stress streams of a .fcp file ([K, 6] uint64)."""
+    lines = open(path).read().split("\n")
+    head = lines[0].split()
+    if not lines[0].startswith("RNGStreams dedicated to the thermal stress:"):
+        raise F.FormatError(f"{path}: not an FFEA checkpoint file")
+    k = int(head[-1])
+    st = np.array([[int(v) for v in lines[1 + i].split()[:6]] for i in range(k)], np.uint64)
+    return st
+
+
+def read_trajectory(path):
+    """Complete frames of a .ftj file: list of (step, [per-blob [N_t, 10] float64], start offset).
+    A frame is complete when its conformation-change block is closed by '*'."""
+    data = open(path, "rb").read()
+    text = data.decode()
+    head_end = text.index("\n*\n") + 3
+    frames, pos = [], head_end
+    lines = text[head_end:].split("\n")
+    off = head_end
+    i = 0
+
+    def take():
+        nonlocal i, off
+        line = lines[i]
+        i += 1
+        off += len(line) + 1
+        return line
+
+    try:
+        while i < len(lines) and lines[i].startswith("Blob "):
+            start = off
+            blobs, step = [], None
+            while lines[i].startswith("Blob ") and ", step " in lines[i]:
+                h = take()
+                step = int(h.rsplit("step", 1)[1])
+                state = take()
+                rows = []
+                if state != "STATIC":
+                    while lines[i] and lines[i][0] not in "B*":
+                        rows.append([float(v) for v in take().split()])
+                blobs.append(np.array(rows))
+            if take() != "*" or take() != "Conformation Changes:":
+                break
+            while lines[i].startswith("Blob "):
+                take()
+            if take() != "*":
+                break
+            frames.append((step, blobs, start))
+    except (IndexError, ValueError):
+        pass  # a frame cut off part-way: not complete
+    return frames
+
+
+# ---------------------------------------------------------------------------------------------
+# the world
+# ---------------------------------------------------------------------------------------------
+class World:
+    """The dynamic CG_nomass blobs of a .ffea script on one GPU (World::init / World::run)."""
+
+    def __init__(self, script: str, precision: str = "fp64", quiet: bool = True):
+        self.script = os.path.abspath(script)
+        s = F.read_script(script)
+        self.raw = s["params"]
+        self.params = F.params_from_script(self.raw)
+        base = os.path.dirname(self.script)
+        stem = os.path.splitext(os.path.basename(self.script))[0]
+
+        def path(key, default):
+            v = self.raw.get(key)
+            return os.path.normpath(os.path.join(base, v)) if v else os.path.join(base, default)
+
+        self.traj_path = path("trajectory_out_fname", stem + ".ftj")
+        self.meas_path = path("measurement_out_fname", stem + ".fm")
+        self.ckpt_out = path("checkpoint_out", stem + ".fcp")
+        self.ckpt_in = path("checkpoint_in", "") if self.raw.get("checkpoint_in") else None
+        self.restart = int(float(self.raw.get("restart", 0)))
+        self.num_steps = int(float(self.raw.get("num_steps", 0)))
+        self.check = int(float(self.raw.get("check", 10000)))
+        self.dt_si = float(self.raw.get("dt", 1e-14))
+        if self.restart and not self.ckpt_in:
+            raise F.FormatError("Checkpoint input file required if restart is required.")
+        if self.ckpt_in and os.path.abspath(self.ckpt_in) == os.path.abspath(self.ckpt_out):
+            raise F.FormatError("checkpoint_in and checkpoint_out must differ")
+        self.meshes, self.scales = [], []
+        for b in s["blobs"]:
+            solver = b.get("solver", "CG_nomass")
+            if solver != "CG_nomass":
+                raise F.FormatError(f"blob solver {solver}: only CG_nomass is on this path")
+            if b.get("motion_state", "DYNAMIC") != "DYNAMIC":
+                raise F.FormatError("only DYNAMIC blobs are on this path")
+            self.meshes.append(F.load_blob(b["nodes"], b["topology"], b["material"], b["stokes"], b.get("pin")))
+            self.scales.append(U.blob_scale(float(b.get("scale", 1.0))))
+        self.nodes = [m.num_nodes for m in self.meshes]
+        self.pos0 = [m.pos * sc for m, sc in zip(self.meshes, self.scales)]  # meso units
+        if self.restart:
+            st = read_checkpoint(self.ckpt_in)[0]
+        else:
+            st = package_seed(self.params.rng_seed)
+        self.batch = Batch(self.meshes, self.params, self.scales, rng_state=st, precision=precision)
+        self.quiet = quiet
+        self.step_initial = 0
+        if self.restart:
+            self._restart_files()
+        else:
+            with open(self.traj_path, "w") as f:
+                f.write(traj_header(self.nodes))
+            with open(self.meas_path, "w") as f:
+                f.write(self._meas_header())
+
+    def _meas_header(self) -> str:
+        p = self.raw
+        s = "FFEA Global Measurement File\n\nSimulation Details:\n"
+        s += "\tImplementation: ffea_amd (MI355X massless path)\n"
+        s += f"\tScript Filename = {os.path.basename(self.script)}\n\tSimulation Type = Full\n\n"
+        s += "Parameters:\n\tSystem parameters:\n"
+        for k in ("restart", "dt", "num_steps", "check", "rng_seed", "kT", "max_iterations_cg", "epsilon",
+                  "stokes_visc", "num_blobs"):
+            if k in p:
+                s += f"\t{k} = {p[k]}\n"
+        s += "\n\tCalculations enabled:\n"
+        s += f"\tcalc_noise = {self.params.calc_noise}\n\tcalc_stokes = {self.params.calc_stokes}\n\n\n"
+        s += "Measurements:\n" + "".join("%-14s" % c for c in MEAS_COLUMNS) + "\n"
+        return s
+
+    def _restart_files(self):
+        frames = read_trajectory(self.traj_path)
+        if not frames:
+            raise F.FormatError(f"{self.traj_path}: no complete frame to restart from")
+        step, blobs, start = frames[-1]
+        if len(blobs) != len(self.meshes):
+            raise F.FormatError(f"{self.traj_path}: frame has {len(blobs)} blobs, script has {len(self.meshes)}")
+        pos = np.concatenate([b[:, :3] / U.MESO_LENGTH for b in blobs])  # read_nodes_from_file
+        self.batch.set_positions(pos)
+        with open(self.traj_path, "r+b") as f:
+            f.truncate(start)
+        with open(self.meas_path, "rb") as f:
+            text = f.read()
+        cut = text.rstrip(b"\n").rfind(b"\n") + 1  # drop the last measurement line
+        with open(self.meas_path, "r+b") as f:
+            f.truncate(cut)
+        self.step_initial = step
+
+    def measurements(self, step: int):
+        """Time, strain energy, centroid, RMSD (World::make_measurements, src/World.cpp:4282-4340;
+        Blob::make_measurements, src/Blob.cpp:1363-1528) at the positions of the current step."""
+        pos = self.batch.split(self.batch.positions())
+        tot = sum(self.nodes)
+        cog = np.zeros(3)
+        msd = 0.0
+        for p, p0, n in zip(pos, self.pos0, self.nodes):
+            c = np.zeros(3)
+            for k in range(3):  # sequential node sums, as the reference adds them
+                c[k] = float(np.add.accumulate(p[:, k])[-1]) / n
+            d = p - p0
+            r = np.sqrt(float(np.add.accumulate((d * d).sum(axis=1))[-1]) / n)
+            cog += c * n
+            msd += r * r * n
+        cog /= tot
+        return (step * self.dt_si, self.batch.strain_energy() * U.MESO_ENERGY, *(cog * U.MESO_LENGTH),
+                np.sqrt(msd / tot) * U.MESO_LENGTH), pos
+
+    def run(self):
+        """World::run from step_initial to num_steps inclusive."""
+        b = self.batch
+        t0 = time.time()
+        step = self.step_initial
+        while step <= self.num_steps:
+            if step % self.check == 0:
+                with open(self.ckpt_out, "w") as f:
+                    f.write(checkpoint_text([b.rng_state()]))
+                b.update_internal_forces()  # an inversion raises: the run stops, as the reference's does
+                vals, pos = self.measurements(step)
+                with open(self.traj_path, "a") as f:
+                    f.write(traj_frame(step, [p * U.MESO_LENGTH for p in pos]))
+                with open(self.meas_path, "a") as f:
+                    f.write(meas_line(vals))
+                self._positions_phase()
+                step += 1
+                if not self.quiet:
+                    print(f"step = {step - 1} ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+            else:
+                n = min(self.check - step % self.check, self.num_steps + 1 - step)
+                before = b.steps_done()
+                try:
+                    b.step(n)
+                except _lib.CGError as e:
+                    # the failing step completed without moving that blob; the reference carries
+                    # on after a failed solve (src/World.cpp:2042), so does this loop
+                    print(f"warning: {e}; the blob is not moved that step", file=sys.stderr)
+                step += b.steps_done() - before
+        b.synchronize()
+        return step - 1
+
+    def _positions_phase(self):
+        try:
+            self.batch.update_positions()
+        except _lib.CGError as e:
+            print(f"warning: {e}; the blob is not moved this step", file=sys.stderr)
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="Run a .ffea script's CG_nomass blobs on one GPU")
+    ap.add_argument("script")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    a = ap.parse_args(argv)
+    w = World(a.script, precision=a.precision, quiet=False)
+    w.run()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -122,6 +122,11 @@ int ffh_blob_update_positions(ffh_blob *b);
  */
 int ffh_blob_step(ffh_blob *b, int nsteps);
 
+/* Number of position phases completed by this handle (update_positions and ffh_blob_step);
+ * after ffh_blob_step returns FFH_E_CG it tells how many steps were made, the failing one
+ * included (World::run goes on after a failed solve, src/World.cpp:2042). */
+int ffh_blob_steps_done(ffh_blob *b, long long *steps);
+
 /*
  * Solver::solve (include/Solver.h:51; NoMassCGSolver::solve, src/NoMassCGSolver.cpp:165-197):
  * x is [3*N_t] host memory, the force on entry, the velocity on exit.  Uses the viscosity
