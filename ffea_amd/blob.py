@@ -96,6 +96,9 @@ class Blob:
     def step(self, nsteps: int = 1) -> None:
         _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
 
+    def set_graphs(self, enable: bool) -> None:
+        _lib.check(self._L.ffh_blob_set_graphs(self._h, int(bool(enable))))
+
     def steps_done(self) -> int:
         n = C.c_longlong(0)
         _lib.check(self._L.ffh_blob_steps_done(self._h, C.byref(n)))

@@ -758,6 +758,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double *X, doubl
     Z[n] += x2 * dt;
 }
 
+__global__ void k_reset_inversions(CgCtl *ctl) {
+    if (threadIdx.x == 0) ctl->inversions = 0;
+}
+
 // result = K in for Solver::apply_matrix (linear rows; midpoint rows handled on the host).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_apply(SpmvArgs m, int N, const double *__restrict__ in,
@@ -904,6 +908,7 @@ struct ffh_blob {
 
     ~ffh_blob() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto &pe : pending) {
             (void)hipEventDestroy(pe.second.first);
             (void)hipEventDestroy(pe.second.second);
@@ -1311,33 +1316,69 @@ struct ffh_blob {
     // fallback count, by default the iteration count of the last plain solve); iteration F + 1
     // restarts Jacobi-PCG from the current iterate (beta = 0), which then runs to the same
     // stopping test.  Snapshots for training are taken from plain solves only.
-    int run_cg() {
-        const bool learned = learned_solve();
-        const int F = learned ? std::max(1, fallback_iters > 0 ? fallback_iters : plain_count) : 0;
-        const int snap0 = hv_count;
-        const bool harvesting = hv_on && !learned;
+    struct CgRun {
+        bool learned, harvesting;
+        int F, snap0, it;
+    };
+    CgRun cg_setup() {
+        CgRun c;
+        c.learned = learned_solve();
+        c.F = c.learned ? std::max(1, fallback_iters > 0 ? fallback_iters : plain_count) : 0;
+        c.snap0 = hv_count;
+        c.harvesting = hv_on && !c.learned;
+        c.it = 0;
+        return c;
+    }
+    int cg_begin(CgRun &c) {
         int rc;
-        if (learned && (rc = mpnn_apply(r, nullptr, false))) return rc;
+        if (c.learned && (rc = mpnn_apply(r, nullptr, false))) return rc;
         if ((rc = launch_spmv(0, 0))) return rc;
-        if (harvesting && (rc = harvest_snap(snap0))) return rc;
-        int it = 0;
-        bool all_done = false;
-        for (;;) {
-            int chunk = learned && it < F ? F - it : std::max(1, std::min(cg_chunk, maxit - it));
-            for (int k = 0; k < chunk && it < maxit; k++) {
-                const bool lz = learned && it < F;
-                if ((rc = launch_update(it, lz))) return rc;
-                it++;
-                if (harvesting && (rc = harvest_snap(snap0 + it))) return rc;
-                if (lz && (rc = mpnn_apply(r, r_prev, true))) return rc;
-                if ((rc = launch_spmv(it, lz ? 1 : (learned && it == F + 1 ? 2 : 0)))) return rc;
-            }
-            HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, nblobs * sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
-            HIP_OR_FAIL(hipStreamSynchronize(stream));
-            all_done = true;
-            for (int b = 0; b < nblobs && all_done; b++) all_done = ctl_host[b].done != 0;
-            if (all_done || it >= maxit) break;
+        if (c.harvesting && (rc = harvest_snap(c.snap0))) return rc;
+        return FFH_OK;
+    }
+    int cg_iterations(CgRun &c, int count) {
+        int rc;
+        for (int k = 0; k < count && c.it < maxit; k++) {
+            const bool lz = c.learned && c.it < c.F;
+            if ((rc = launch_update(c.it, lz))) return rc;
+            c.it++;
+            if (c.harvesting && (rc = harvest_snap(c.snap0 + c.it))) return rc;
+            if (lz && (rc = mpnn_apply(r, r_prev, true))) return rc;
+            if ((rc = launch_spmv(c.it, lz ? 1 : (c.learned && c.it == c.F + 1 ? 2 : 0)))) return rc;
         }
+        return FFH_OK;
+    }
+    int cg_poll(bool &all_done) {
+        HIP_OR_FAIL(hipMemcpyAsync(ctl_host, ctl, nblobs * sizeof(CgCtl), hipMemcpyDeviceToHost, stream));
+        HIP_OR_FAIL(hipStreamSynchronize(stream));
+        all_done = true;
+        for (int b = 0; b < nblobs && all_done; b++) all_done = ctl_host[b].done != 0;
+        return FFH_OK;
+    }
+    int chunk_of(const CgRun &c) const {
+        return c.learned && c.it < c.F ? c.F - c.it : std::max(1, std::min(cg_chunk, maxit - c.it));
+    }
+    // Iterates and polls until every blob has stopped (or maxit), then the per-blob results.
+    // `polled`: the launches up to c.it were already made and polled once (graph replay).
+    int cg_run_rest(CgRun &c, bool polled, bool all_done) {
+        int rc;
+        if (!polled) all_done = false;
+        while (!(polled && (all_done || c.it >= maxit))) {
+            if ((rc = cg_iterations(c, chunk_of(c)))) return rc;
+            if ((rc = cg_poll(all_done))) return rc;
+            polled = true;
+        }
+        return cg_finish(c, all_done);
+    }
+    int run_cg() {
+        CgRun c = cg_setup();
+        int rc;
+        if ((rc = cg_begin(c))) return rc;
+        return cg_run_rest(c, false, false);
+    }
+    int cg_finish(CgRun &c, bool all_done) {
+        const bool learned = c.learned, harvesting = c.harvesting;
+        const int snap0 = c.snap0;
         if (!all_done) return fail(FFH_E_HIP, "CG control block not finalised");
         int worst = 0, failed = -1;
         last_res = 0;
@@ -1378,9 +1419,79 @@ struct ffh_blob {
             hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, X, Y, Z, dt, rng_state, j1, j2, adv);
         });
     }
-    int reset_inversions() {
-        HIP_OR_FAIL(hipMemsetAsync(&ctl->inversions, 0, sizeof(int), stream));
+    int reset_inversions() {  // a kernel, not a memset: it is also recorded into step graphs
+        hipLaunchKernelGGL(k_reset_inversions, 1, 64, 0, stream, ctl);
+        HIP_OR_FAIL(hipGetLastError());
         return FFH_OK;
+    }
+    // ---- one time step as a replayed hipGraph ----
+    // The graph holds: inversion reset, element, assembly, force gather + CG start, and K CG
+    // iterations (K = the last solve's count; blobs that stop earlier exit their kernels at
+    // once).  One replay per step, one poll; a blob that needs more iterations continues with
+    // ordinary launches, then the integration is launched.  Learned preconditioning, harvesting
+    // and per-kernel profiling take the ordinary launch path.
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;  // by K
+    bool use_graphs = true;
+    int graph_for(int K, hipGraphExec_t &ge) {
+        for (auto &g : graphs)
+            if (g.first == K) {
+                ge = g.second;
+                return FFH_OK;
+            }
+        hipGraph_t g = nullptr;
+        HIP_OR_FAIL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = reset_inversions();
+        if (!rc) rc = run_element();
+        if (!rc) rc = run_assemble();
+        if (!rc) rc = run_rhs();
+        CgRun c = cg_setup();
+        if (!rc) rc = cg_begin(c);
+        if (!rc) rc = cg_iterations(c, K);
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(FFH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(FFH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        if (graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(graphs.front().second);
+            graphs.erase(graphs.begin());
+        }
+        graphs.push_back({K, x});
+        ge = x;
+        return FFH_OK;
+    }
+    bool graph_step_ok() const {
+        return use_graphs && !profiling && !learned_solve() && !hv_on;
+    }
+    // One step through a graph replay; returns like the ordinary step (inversion, CG status).
+    int step_eager() {
+        int rc;
+        if ((rc = reset_inversions())) return rc;
+        if ((rc = run_element())) return rc;
+        if ((rc = run_assemble())) return rc;
+        if ((rc = run_rhs())) return rc;
+        return run_cg();  // synchronises: the inversion count is then final too
+    }
+    int step_graph() {
+        const int K = std::max(1, std::min(cg_chunk, maxit));
+        hipGraphExec_t ge = nullptr;
+        int rc;
+        if (graph_for(K, ge) != FFH_OK) {  // capture not available: ordinary launches from now on
+            use_graphs = false;
+            (void)hipGetLastError();
+            return step_eager();
+        }
+        HIP_OR_FAIL(hipGraphLaunch(ge, stream));
+        CgRun c = cg_setup();
+        c.it = K;
+        bool all_done = false;
+        if ((rc = cg_poll(all_done))) return rc;
+        return cg_run_rest(c, true, all_done);
     }
     int read_inversions(int *ninv) {
         HIP_OR_FAIL(hipMemcpyAsync(&ctl_host->inversions, &ctl->inversions, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1666,12 +1777,9 @@ int ffh_blob_update_positions(ffh_blob *b) {
 int ffh_blob_step(ffh_blob *b, int nsteps) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     for (int s = 0; s < nsteps; s++) {
-        int rc;
-        if ((rc = b->reset_inversions())) return rc;
-        if ((rc = b->run_element())) return rc;
-        if ((rc = b->run_assemble())) return rc;
-        if ((rc = b->run_rhs())) return rc;
-        int cg = b->run_cg();  // synchronises: the inversion count is then final too
+        int rc, cg;
+        cg = b->graph_step_ok() ? b->step_graph() : b->step_eager();  // both synchronise
+        if (cg != FFH_OK && cg != FFH_E_CG) return cg;
         if (b->ctl_host->inversions > 0) {
             // the reference aborts before the position phase (src/World.cpp:2025-2028)
             int ninv = 0;
@@ -1681,6 +1789,12 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
         b->steps_done++;
         if (cg) return cg;
     }
+    return FFH_OK;
+}
+
+int ffh_blob_set_graphs(ffh_blob *b, int enable) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    b->use_graphs = enable != 0;
     return FFH_OK;
 }
 

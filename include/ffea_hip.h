@@ -122,6 +122,12 @@ int ffh_blob_update_positions(ffh_blob *b);
  */
 int ffh_blob_step(ffh_blob *b, int nsteps);
 
+/* ffh_blob_step replays one captured hipGraph per step (element, assembly, force gather and
+ * the CG iterations of the last solve's count; the host polls once per step) unless disabled
+ * here, or the learned preconditioner, harvesting or profiling is on.  Results are the same
+ * either way: the same kernels run in the same order.  Enabled by default. */
+int ffh_blob_set_graphs(ffh_blob *b, int enable);
+
 /* Number of position phases completed by this handle (update_positions and ffh_blob_step);
  * after ffh_blob_step returns FFH_E_CG it tells how many steps were made, the failing one
  * included (World::run goes on after a failed solve, src/World.cpp:2042). */

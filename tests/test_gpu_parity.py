@@ -190,3 +190,25 @@ def test_large_blob_per_blob_totals(oracle):
     assert np.linalg.norm(b.positions().ravel() - po) <= 1e-9 * np.linalg.norm(po - x0.ravel() * sc)
     assert b.last_cg()[0] == oracle.cg_iterations(bo)
     assert np.array_equal(b.rng_state(), s.state())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_equals_ordinary_launches(precision):
+    """ffh_blob_step through a replayed step graph against the same steps launched one by one:
+    bit-identical positions and RngStream state after every step, with host reads and
+    synchronisations between the replays; then a stretch of steps in one call."""
+    m, prm, sc, x0 = box_case((5, 4, 3), noise=1, jitter=0.1)
+    seed = np.arange(3, 9, dtype=np.uint64)
+    a = gpu_blob(m, prm, sc, seed, x0, precision)
+    b = gpu_blob(m, prm, sc, seed, x0, precision)
+    b.set_graphs(False)
+    for k in range(6):
+        a.step(1)
+        b.step(1)
+        assert np.array_equal(a.positions(), b.positions()), k
+        assert np.array_equal(a.rng_state(), b.rng_state()), k
+        assert a.last_cg() == b.last_cg()
+    a.step(25)
+    b.step(25)
+    assert np.array_equal(a.positions(), b.positions())
+    assert a.steps_done() == b.steps_done() == 31
