@@ -129,7 +129,7 @@ def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
     else:
         b = Batch(ms, prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision)
         b.set_positions(np.concatenate(xs) * sc)
-    log(f"[{precision}] {len(ms)} blob(s) created in {time.time() - t0:.1f}s: {b.sizes()}")
+    log(f"[{precision}] {len(ms)} blob(s) created in {time.time() - t0:.1f}s: {b.sizes()}, tuned {b.tuning()}")
     b.step(warmup)
     b.synchronize()
     iters = []
@@ -147,6 +147,7 @@ def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
     kt = b.kernel_times()
     b.set_profiling(False)
     sizes = b.sizes()
+    sizes["tuning"] = b.tuning()
     log(f"[{precision}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s (with per-kernel events "
         f"{steps / dt_prof:.1f} steps/s), CG iterations mean {np.mean(iters):.2f}")
     b.close()
@@ -469,7 +470,8 @@ def main():
         "data": "synthetic: seeded Kuhn box mesh, seeded Gaussian initial displacement, RngStream noise",
         "config": {"workload": f"{a.config}: {desc}; eps 1e-6, noise on, dt 1e-14 s", "blobs": len(ms),
                    "linear_nodes": sizes["num_linear"], "elements": sizes["num_elements"],
-                   "blocks": sizes["num_blocks"], "parallelism": "single GPU"},
+                   "blocks": sizes["num_blocks"], "parallelism": "single GPU",
+                   "launch_shapes": sizes["tuning"]},
         "ns_per_day": sps * 1e-5 * 86400,
         "cg_iters_mean": iters,
         "ms_per_step_with_kernel_events": 1e3 * prof_step,

@@ -96,6 +96,15 @@ class Blob:
     def step(self, nsteps: int = 1) -> None:
         _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
 
+    def tuning(self) -> dict:
+        e, a, w = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(self._L.ffh_blob_tuning(self._h, C.byref(e), C.byref(a), C.byref(w)))
+        return {"element_block": e.value, "assembly_block": a.value, "spmv_waves_per_slice": w.value}
+
+    def set_tuning(self, element_block: int, assembly_block: int, spmv_waves_per_slice: int) -> None:
+        _lib.check(self._L.ffh_blob_set_tuning(self._h, int(element_block), int(assembly_block),
+                                               int(spmv_waves_per_slice)))
+
     def set_graphs(self, enable: bool) -> None:
         _lib.check(self._L.ffh_blob_set_graphs(self._h, int(bool(enable))))
 

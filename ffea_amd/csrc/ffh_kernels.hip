@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -296,7 +297,8 @@ __device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, 
 
 template <typename R, typename T>
 __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
-    const int e = xcd_block((g.E + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned); waves stay 64-aligned
+    const int e = xcd_block((g.E + bd - 1) / bd) * bd + threadIdx.x;
     if (e >= g.E) return;
     const int4 n = g.enodes[e];
     const int nn[4] = {n.x, n.y, n.z, n.w};
@@ -448,7 +450,8 @@ struct AsmArgs {
 //   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
-    const int64_t u = (int64_t)xcd_block((int)((g.nupper + kBlock - 1) / kBlock)) * kBlock + threadIdx.x;
+    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned)
+    const int64_t u = (int64_t)xcd_block((int)((g.nupper + bd - 1) / bd)) * bd + threadIdx.x;
     if (u >= g.nupper) return;
     const uint8_t fl = g.flags[u];
     if (fl & 4) return;  // tile padding
@@ -709,6 +712,109 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     block_partial_at(pn * qn, v.parts_pq, slice, sh);
 }
 
+// Sum of parts[i0, i1) in exactly the order parts_range gives with a 192-thread block, computed
+// by one 64-thread wave (virtual thread t + 64 w for w = 0, 1, 2).
+__device__ __forceinline__ double parts_range_as192(const double *__restrict__ parts, int i0, int i1) {
+    const int t = threadIdx.x & 63;
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        double s = 0.0;
+        for (int i = i0 + t + 64 * w; i < i1; i += 192) s += parts[i];
+        tot += __shfl(wave_sum(s), 0, 64);
+    }
+    return tot;
+}
+
+__device__ __forceinline__ double blob_total_as192(const double *parts, const double *tot, int slot, int b, int s0, int s1) {
+    if (tot) return tot[b * kTotSlots + slot];
+    return parts_range_as192(parts, s0, s1);
+}
+
+// k_cg_spmv with one 64-lane wave per slice: each lane computes the 3 components of its row, so
+// the column index and the gathered z block are loaded once instead of three times.  Row sums,
+// reductions and the partial of p.q are formed in k_cg_spmv's exact order, so both variants
+// give bit-identical results (the tuner may pick either).
+template <typename T>
+__global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, int mode) {
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
+    const bool decider = slice == s0 && threadIdx.x == 0;
+    double beta = 0.0;
+    if (it == 0) {
+        const double f2 = blob_total_as192(v.parts_ff, v.tot, kTotFF, b, s0, s1);
+        const double d0 = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
+        if (decider) {
+            ctl->f2 = f2;
+            ctl->delta[0] = d0;
+            if (f2 == 0.0) {
+                ctl->done = 1;
+                ctl->status = 0;
+                ctl->iter = 0;
+            }
+        }
+        if (f2 == 0.0) return;
+    } else {
+        const double f2 = ctl->f2;
+        const double r2 = blob_total_as192(v.parts_rr, v.tot, kTotRR, b, s0, s1);
+        const double res = r2 / f2;
+        if (res < v.eps2 || it >= v.maxit) {
+            if (decider) {
+                ctl->done = 1;
+                ctl->status = res < v.eps2 ? 0 : -1;
+                ctl->iter = it;
+                ctl->res = res;
+            }
+            return;
+        }
+        const double dnew = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
+        if (mode == 0)
+            beta = dnew / ctl->delta[(it - 1) & 1];
+        else if (mode == 1)
+            beta = (dnew - blob_total_as192(v.parts_zrp, v.tot, kTotZRP, b, s0, s1)) / ctl->delta[(it - 1) & 1];
+        if (decider) ctl->delta[it & 1] = dnew;
+    }
+    const int lane = threadIdx.x & 63;
+    const int row = slice * 64 + lane;
+    const int w = m.slice_width[slice];
+    const int64_t base = m.slice_off[slice];
+    const T *val = reinterpret_cast<const T *>(m.val);
+    const double *__restrict__ z = v.z;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 2
+    for (int t = 0; t < w; t++) {
+        const int64_t tile = base + (int64_t)t * 64;
+        const int c = m.col[tile + lane];
+        const T *vt = val + tile * 9 + lane;
+        const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
+        a0 += (double)vt[0] * z0;
+        a0 += (double)vt[64] * z1;
+        a0 += (double)vt[128] * z2;
+        a1 += (double)vt[192] * z0;
+        a1 += (double)vt[256] * z1;
+        a1 += (double)vt[320] * z2;
+        a2 += (double)vt[384] * z0;
+        a2 += (double)vt[448] * z1;
+        a2 += (double)vt[512] * z2;
+    }
+    const size_t k = 3 * (size_t)row;
+    const double acc[3] = {a0, a1, a2};
+    double part = 0.0;
+#pragma unroll
+    for (int ci = 0; ci < 3; ci++) {
+        const double pn = z[k + ci] + beta * v.p[k + ci];
+        const double qn = acc[ci] + beta * v.q[k + ci];
+        v.p[k + ci] = pn;
+        v.q[k + ci] = qn;
+        part += __shfl(wave_sum(pn * qn), 0, 64);  // ((0 + W0) + W1) + W2, as the 3-wave block
+    }
+    if (lane == 0) v.parts_pq[slice] = part;
+}
+
 // alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
 // z = D^-1 r (parallel_apply_preconditioner, src/NoMassCGSolver.cpp:278-292); partials of r.r
 // and r.z.  One block = the 3 x 64 scalars of one slice.
@@ -866,6 +972,8 @@ struct ffh_blob {
     int32_t *slice_blob = nullptr, *blob_slice_off = nullptr;
     int nblobs = 1;
     bool use_tot = false;  // some blob has more than kTotSlicesMin slices
+    // launch shapes picked by autotune(): element / assembly block size, SpMV variant
+    int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3;
     double *tot = nullptr;
     CgCtl *ctl = nullptr;       // [nblobs]; ctl[0].inversions counts the whole batch
     CgCtl *ctl_host = nullptr;  // pinned host mirror
@@ -1055,8 +1163,8 @@ struct ffh_blob {
     int run_element() {
         ElemArgs g = elem_args();
         if (precision == FFH_PRECISION_FP64)
-            return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<double, double>), xcd_grid(grid(E)), kBlock, 0, stream, e0, e1, 0, g); });
-        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), xcd_grid(grid(E)), kBlock, 0, stream, e0, e1, 0, g); });
+            return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<double, double>), xcd_grid(grid(E, elem_block)), elem_block, 0, stream, e0, e1, 0, g); });
+        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), xcd_grid(grid(E, elem_block)), elem_block, 0, stream, e0, e1, 0, g); });
     }
     int run_assemble() {
         mp_edges = false;
@@ -1076,9 +1184,9 @@ struct ffh_blob {
         a.nupper = plan.num_upper;
         if (precision == FFH_PRECISION_FP64)
             return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
-                hipExtLaunchKernelGGL((k_assemble<double>), xcd_grid(grid(plan.num_upper)), kBlock, 0, stream, e0, e1, 0, a);
+                hipExtLaunchKernelGGL((k_assemble<double>), xcd_grid(grid(plan.num_upper, asm_block)), asm_block, 0, stream, e0, e1, 0, a);
             });
-        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_assemble<float>), xcd_grid(grid(plan.num_upper)), kBlock, 0, stream, e0, e1, 0, a); });
+        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_assemble<float>), xcd_grid(grid(plan.num_upper, asm_block)), asm_block, 0, stream, e0, e1, 0, a); });
     }
     RhsArgs rhs_args(bool noise, const double *f_in) const {
         RhsArgs g;
@@ -1115,10 +1223,21 @@ struct ffh_blob {
         CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
         int rc;
+        const bool one = spmv_waves == 1;
         if (precision == FFH_PRECISION_FP64)
-            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
+            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) {
+                if (one)
+                    hipExtLaunchKernelGGL((k_cg_spmv1<double>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode);
+                else
+                    hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode);
+            });
         else
-            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); });
+            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) {
+                if (one)
+                    hipExtLaunchKernelGGL((k_cg_spmv1<float>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode);
+                else
+                    hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode);
+            });
         if (rc) return rc;
         return totals(1 << kTotPQ);
     }
@@ -1419,6 +1538,71 @@ struct ffh_blob {
             hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, X, Y, Z, dt, rng_state, j1, j2, adv);
         });
     }
+    // ---- autotuning of launch shapes (results do not depend on them) ----
+    template <typename F>
+    int time_ms(F &&fn, int reps, float &ms) {
+        hipEvent_t a = nullptr, b = nullptr;
+        HIP_OR_FAIL(hipEventCreate(&a));
+        HIP_OR_FAIL(hipEventCreate(&b));
+        int rc = fn();  // warm-up (and, for a solve, its own synchronisation)
+        HIP_OR_FAIL(hipEventRecord(a, stream));
+        for (int k = 0; k < reps && !rc; k++) rc = fn();
+        HIP_OR_FAIL(hipEventRecord(b, stream));
+        HIP_OR_FAIL(hipEventSynchronize(b));
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return rc;
+    }
+    // Element and assembly block sizes (elements / blocks per workgroup) and the SpMV variant
+    // (one wave per component vs one wave per slice), each the fastest of a few timed runs on
+    // this blob's own data.  Tuning runs change no state the step depends on: the element and
+    // assembly runs recompute what the next step recomputes; the solves use a synthetic force.
+    int autotune() {
+        int rc;
+        const int sizes[3] = {64, 128, 256};
+        float best = 1e30f, ms = 0;
+        int pick = elem_block;
+        for (int bs : sizes) {
+            elem_block = bs;
+            if ((rc = time_ms([&] { return run_element(); }, 3, ms))) return rc;
+            if (ms < best) best = ms, pick = bs;
+        }
+        elem_block = pick;
+        best = 1e30f;
+        pick = asm_block;
+        for (int bs : sizes) {
+            asm_block = bs;
+            if ((rc = time_ms([&] { return run_assemble(); }, 3, ms))) return rc;
+            if (ms < best) best = ms, pick = bs;
+        }
+        asm_block = pick;
+        std::vector<double> ones(3 * (size_t)N, 1.0);
+        HIP_OR_FAIL(hipMemcpyAsync(tmp_in, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        best = 1e30f;
+        pick = spmv_waves;
+        const int waves[2] = {3, 1};
+        for (int wv : waves) {
+            spmv_waves = wv;
+            auto solve = [&] {
+                int r1 = run_rhs(tmp_in);
+                int r2 = r1 ? r1 : run_cg();
+                return r2 == FFH_E_CG ? FFH_OK : r2;
+            };
+            if ((rc = time_ms(solve, 2, ms))) return rc;
+            if (ms < best) best = ms, pick = wv;
+        }
+        spmv_waves = pick;
+        plain_count = -1;
+        last_iters = 0;
+        last_res = 0;
+        cg_chunk = 16;
+        std::fill(blob_iters.begin(), blob_iters.end(), 0);
+        std::fill(blob_res.begin(), blob_res.end(), 0.0);
+        std::fill(blob_status.begin(), blob_status.end(), 0);
+        HIP_OR_FAIL(hipStreamSynchronize(stream));
+        return FFH_OK;
+    }
     int reset_inversions() {  // a kernel, not a memset: it is also recorded into step graphs
         hipLaunchKernelGGL(k_reset_inversions, 1, 64, 0, stream, ctl);
         HIP_OR_FAIL(hipGetLastError());
@@ -1667,6 +1851,11 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
             return nullptr;
         }
     }
+    const char *tune = std::getenv("FFH_AUTOTUNE");
+    if (!(tune && tune[0] == '0') && b->autotune() != FFH_OK) {
+        if (g_err.empty()) fail(FFH_E_HIP, "autotuning failed");
+        return nullptr;
+    }
     return b.release();
 }
 
@@ -1789,6 +1978,27 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
         b->steps_done++;
         if (cg) return cg;
     }
+    return FFH_OK;
+}
+
+int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *spmv_waves_per_slice) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (element_block) *element_block = b->elem_block;
+    if (assembly_block) *assembly_block = b->asm_block;
+    if (spmv_waves_per_slice) *spmv_waves_per_slice = b->spmv_waves;
+    return FFH_OK;
+}
+
+int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int spmv_waves_per_slice) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
+    if (!ok(element_block) || !ok(assembly_block) || (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3))
+        return fail(FFH_E_ARG, "block sizes must be 64, 128 or 256 and SpMV waves per slice 1 or 3");
+    b->elem_block = element_block;
+    b->asm_block = assembly_block;
+    b->spmv_waves = spmv_waves_per_slice;
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    b->graphs.clear();
     return FFH_OK;
 }
 

@@ -212,3 +212,36 @@ def test_graph_replay_equals_ordinary_launches(precision):
     b.step(25)
     assert np.array_equal(a.positions(), b.positions())
     assert a.steps_done() == b.steps_done() == 31
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("batch", [False, True])
+def test_tuned_launch_shapes_give_identical_results(precision, batch):
+    """Every launch shape the tuner may pick (element / assembly workgroup size, SpMV waves per
+    slice) gives bit-identical trajectories, for one blob and for a batch."""
+    from ffea_amd.blob import Batch
+    seed = np.arange(5, 11, dtype=np.uint64)
+    if batch:
+        from test_gpu_batch import uneven_blobs
+        meshes, x0s, prm, sc = uneven_blobs()
+
+        def make():
+            g = Batch(meshes, prm, sc, rng_state=seed, precision=precision)
+            g.set_positions(np.concatenate(x0s) * sc)
+            return g
+    else:
+        m, prm, sc, x0 = box_case((6, 5, 4), noise=1, jitter=0.1)
+
+        def make():
+            return gpu_blob(m, prm, sc, seed, x0, precision)
+    ref = None
+    for shape in ((256, 256, 3), (64, 128, 1), (128, 64, 3), (256, 64, 1)):
+        b = make()
+        b.set_tuning(*shape)
+        assert b.tuning() == dict(zip(("element_block", "assembly_block", "spmv_waves_per_slice"), shape))
+        b.step(6)
+        out = (b.positions(), b.rng_state(), b.last_cg())
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]) and out[2] == ref[2], shape
