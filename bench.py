@@ -105,10 +105,11 @@ def algorithmic_bytes(kernel, sizes, bm):
     C = 10 * E
     v = 24 * N  # one FP64 3-vector over the linear nodes
     return {
-        # enodes 16, J0^-1 72, vol0 8, A B G E 32, last_det r+w 16; writes forces 12 + record 20
-        "element": E * (16 + 72 + 8 + 32 + 16 + (12 + 20) * bm) + v,
-        # records + contribution codes + per-entry tables, writes every block once + D^-1
-        "assemble": 20 * bm * E + 4 * C + 15 * U + 9 * bm * B + 32 * N,
+        # enodes 16, J0^-1 72, vol0 8, A B G E 32, last_det r+w 16; writes the 12 nodal forces
+        "element": E * (16 + 72 + 8 + 32 + 16 + 12 * bm) + v,
+        # enodes + A, B and positions (the geometry is recomputed), contribution codes,
+        # per-entry tables; writes every block once + D^-1
+        "assemble": 32 * E + v + 4 * C + 15 * U + 9 * bm * B + 32 * N,
         # element forces + codes + per-node data; writes f r x p q z
         "rhs": 12 * bm * E + 16 * E + 39 * N + 6 * v,
         # matrix + column indices, z gathered once, p q read and written

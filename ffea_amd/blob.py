@@ -96,14 +96,17 @@ class Blob:
     def step(self, nsteps: int = 1) -> None:
         _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
 
-    def tuning(self) -> dict:
-        e, a, w = C.c_int(), C.c_int(), C.c_int()
-        _lib.check(self._L.ffh_blob_tuning(self._h, C.byref(e), C.byref(a), C.byref(w)))
-        return {"element_block": e.value, "assembly_block": a.value, "spmv_waves_per_slice": w.value}
+    TUNING_KEYS = ("element_block", "assembly_block", "assembly_unroll", "spmv_waves_per_slice")
 
-    def set_tuning(self, element_block: int, assembly_block: int, spmv_waves_per_slice: int) -> None:
+    def tuning(self) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        _lib.check(self._L.ffh_blob_tuning(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(self.TUNING_KEYS, (x.value for x in v)))
+
+    def set_tuning(self, element_block: int, assembly_block: int, assembly_unroll: int,
+                   spmv_waves_per_slice: int) -> None:
         _lib.check(self._L.ffh_blob_set_tuning(self._h, int(element_block), int(assembly_block),
-                                               int(spmv_waves_per_slice)))
+                                               int(assembly_unroll), int(spmv_waves_per_slice)))
 
     def set_graphs(self, enable: bool) -> None:
         _lib.check(self._L.ffh_blob_set_graphs(self._h, int(bool(enable))))

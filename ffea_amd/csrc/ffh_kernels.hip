@@ -88,19 +88,6 @@ struct Stream6 {
     uint32_t s[6];
 };
 
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<double> {
-    typedef double4 type;
-    __device__ static double4 make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
-};
-template <>
-struct Vec4<float> {
-    typedef float4 type;
-    __device__ static float4 make(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
-};
-
 // State of the stream `off` draws after `base`, via the base-256 digit tables
 // tbl[(d*256 + v)*2 + comp] = A_comp^(v 256^d).
 __device__ __forceinline__ Stream6 rng_jump(const uint32_t *__restrict__ base, const Mat3u *__restrict__ tbl,
@@ -261,7 +248,6 @@ struct ElemArgs {
     const double *vol0, *A, *B, *G, *Em;
     double *last_det;
     void *eforce;  // T SoA [12][E]: node_force[a][i] at (3a+i)*E + e
-    void *erec;    // T [E][20]: (vol, A, B, 0), then (d psi_a/dx, /dy, /dz, 0) for a = 0..3
     int E;
     int calc_noise;
     double kT, dt;
@@ -295,23 +281,21 @@ __device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, 
     return v * vol;
 }
 
-template <typename R, typename T>
-__global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
-    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned); waves stay 64-aligned
-    const int e = xcd_block((g.E + bd - 1) / bd) * bd + threadIdx.x;
-    if (e >= g.E) return;
-    const int4 n = g.enodes[e];
+// calculate_jacobian (src/tetra_element_linear.cpp:193-205), from FP64 positions, and
+// calc_shape_function_derivatives_and_volume (:223-293): J, det and the shape gradients
+// dpsi[a + 4 dim] (x-major).  Shared by the element kernel and the assembly, which recomputes
+// them per contribution: the same code rounds the same way, so the matrix is unchanged.
+template <typename R>
+__device__ __forceinline__ void element_geometry(const double *__restrict__ X, const double *__restrict__ Y,
+                                                 const double *__restrict__ Z, const int4 n, R J[3][3],
+                                                 R dpsi[12], R &det) {
     const int nn[4] = {n.x, n.y, n.z, n.w};
-    // calculate_jacobian (:193-205), in FP64 from FP64 positions
-    R J[3][3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        J[r][0] = (R)(g.X[nn[r + 1]] - g.X[nn[0]]);
-        J[r][1] = (R)(g.Y[nn[r + 1]] - g.Y[nn[0]]);
-        J[r][2] = (R)(g.Z[nn[r + 1]] - g.Z[nn[0]]);
+        J[r][0] = (R)(X[nn[r + 1]] - X[nn[0]]);
+        J[r][1] = (R)(Y[nn[r + 1]] - Y[nn[0]]);
+        J[r][2] = (R)(Z[nn[r + 1]] - Z[nn[0]]);
     }
-    // calc_shape_function_derivatives_and_volume (:223-293)
-    R dpsi[12];
     dpsi[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
     dpsi[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
     dpsi[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
@@ -321,11 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     dpsi[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
     dpsi[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
     dpsi[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
-    R det = J[0][0] * dpsi[1] + J[1][0] * dpsi[2] + J[2][0] * dpsi[3];
-    const double last = g.last_det[e];
-    if (last * (double)det < 0) atomicAdd(&g.ctl->inversions, 1);
-    g.last_det[e] = (double)det;
-    const R vol = (R)(1.0 / 6.0) * fabs(det);
+    det = J[0][0] * dpsi[1] + J[1][0] * dpsi[2] + J[2][0] * dpsi[3];
     const R dinv = (R)1.0 / det;
     dpsi[1] *= dinv;
     dpsi[2] *= dinv;
@@ -339,18 +319,21 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     dpsi[0] = -(dpsi[1] + dpsi[2] + dpsi[3]);
     dpsi[4] = -(dpsi[5] + dpsi[6] + dpsi[7]);
     dpsi[8] = -(dpsi[9] + dpsi[10] + dpsi[11]);
+}
+
+template <typename R, typename T>
+__global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
+    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned); waves stay 64-aligned
+    const int e = xcd_block((g.E + bd - 1) / bd) * bd + threadIdx.x;
+    if (e >= g.E) return;
+    R J[3][3], dpsi[12], det;
+    element_geometry<R>(g.X, g.Y, g.Z, g.enodes[e], J, dpsi, det);
+    const double last = g.last_det[e];
+    if (last * (double)det < 0) atomicAdd(&g.ctl->inversions, 1);
+    g.last_det[e] = (double)det;
+    const R vol = (R)(1.0 / 6.0) * fabs(det);
 
     const R A = (R)g.A[e], B = (R)g.B[e], G = (R)g.G[e], Em = (R)g.Em[e], vol0 = (R)g.vol0[e];
-
-    // record for the assembly: everything create_viscosity_matrix reads, node-major so that a
-    // node's gradient is one aligned 4-wide load
-    {
-        typedef typename Vec4<T>::type V4;
-        V4 *rec = reinterpret_cast<V4 *>(reinterpret_cast<T *>(g.erec) + (size_t)e * 20);
-        rec[0] = Vec4<T>::make((T)vol, (T)A, (T)B, (T)0);
-#pragma unroll
-        for (int a = 0; a < 4; a++) rec[1 + a] = Vec4<T>::make((T)dpsi[a], (T)dpsi[4 + a], (T)dpsi[8 + a], (T)0);
-    }
 
     // add_shear_elastic_stress (:378-396): F_ij = sum_k J[k][i] Jinv0[j][k]; S = F F^T
     R F[3][3], S[3][3];
@@ -425,7 +408,9 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
 // Assembly over the upper triangle of the block pattern
 // ----------------------------------------------------------------------------------------
 struct AsmArgs {
-    const void *erec;  // T [E][20]
+    const double *X, *Y, *Z;  // positions: the element geometry is recomputed per contribution
+    const int4 *enodes;
+    const double *A, *B;
     const int64_t *toff;
     const uint16_t *twidth, *cnt;
     const uint32_t *ctb;
@@ -444,11 +429,11 @@ struct AsmArgs {
 // upper one.  Diagonal blocks also give calc_inverse_diagonal (:325-333).
 //
 // Block (a,b) of an element, entry (i,j) = V[a+4i][b+4j] with g_a = grad psi_a, g_b = grad psi_b
-// (the element's record holds dpsi[4i+a] = g_a[i]); these are the expressions of visc_entry with
+// (dpsi[4i+a] = g_a[i], recomputed from the positions); these are the expressions of visc_entry with
 // every product written in an order that rounds identically:
 //   i == j : ((g_a[i] g_b[i]) (A+B) + (g_a.g_b) A) vol
 //   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
-template <typename T>
+template <typename T, int kUnroll>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
     const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned)
     const int64_t u = (int64_t)xcd_block((int)((g.nupper + bd - 1) / bd)) * bd + threadIdx.x;
@@ -459,21 +444,26 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
     T acc[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-    const T *rec = reinterpret_cast<const T *>(g.erec);
     const int64_t grp = u >> 6;
     const int lane = (int)(u & 63);
     const int cnt = g.cnt[u];
     const uint32_t *tl = g.ctb + g.toff[grp] + lane;
-#pragma unroll 2
+#pragma unroll kUnroll
     for (int k = 0; k < cnt; k++) {
         const uint32_t code = tl[64 * k];
         const int a = (code >> 2) & 3, b = code & 3;
-        typedef typename Vec4<T>::type V4;
-        const V4 *rp = reinterpret_cast<const V4 *>(rec + (size_t)(code >> 4) * 20);
-        const V4 h = rp[0], va = rp[1 + a], vb = rp[1 + b];
-        const T ga[3] = {va.x, va.y, va.z};
-        const T gb[3] = {vb.x, vb.y, vb.z};
-        const T vol = h.x, A = h.y, B = h.z;
+        const int e = (int)(code >> 4);
+        T J[3][3], dpsi[12], det;
+        element_geometry<T>(g.X, g.Y, g.Z, g.enodes[e], J, dpsi, det);
+        const T vol = (T)(1.0 / 6.0) * fabs(det);
+        const T A = (T)g.A[e], B = (T)g.B[e];
+        T ga[3], gb[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {  // selects, not a dynamic register index
+            const T *d = dpsi + 4 * i;
+            ga[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : d[3];
+            gb[i] = b == 0 ? d[0] : b == 1 ? d[1] : b == 2 ? d[2] : d[3];
+        }
         const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -957,7 +947,7 @@ struct ffh_blob {
     int4 *enodes = nullptr;
     double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
            *last_det = nullptr;
-    void *eforce = nullptr, *erec = nullptr, *val = nullptr;
+    void *eforce = nullptr, *val = nullptr;
     uint32_t *uctb = nullptr, *nctb = nullptr, *rng_off = nullptr;
     int64_t *ut_off = nullptr, *nt_off = nullptr;
     uint16_t *ut_width = nullptr, *ut_cnt = nullptr, *nt_cnt = nullptr;
@@ -973,7 +963,7 @@ struct ffh_blob {
     int nblobs = 1;
     bool use_tot = false;  // some blob has more than kTotSlicesMin slices
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
-    int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3;
+    int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
     double *tot = nullptr;
     CgCtl *ctl = nullptr;       // [nblobs]; ctl[0].inversions counts the whole batch
     CgCtl *ctl_host = nullptr;  // pinned host mirror
@@ -1147,7 +1137,6 @@ struct ffh_blob {
         g.Em = Em;
         g.last_det = last_det;
         g.eforce = eforce;
-        g.erec = erec;
         g.E = E;
         g.calc_noise = calc_noise;
         g.kT = kT;
@@ -1169,7 +1158,12 @@ struct ffh_blob {
     int run_assemble() {
         mp_edges = false;
         AsmArgs a;
-        a.erec = erec;
+        a.X = X;
+        a.Y = Y;
+        a.Z = Z;
+        a.enodes = enodes;
+        a.A = A;
+        a.B = B;
         a.toff = ut_off;
         a.twidth = ut_width;
         a.cnt = ut_cnt;
@@ -1182,11 +1176,24 @@ struct ffh_blob {
         a.val = val;
         a.dinv = dinv;
         a.nupper = plan.num_upper;
-        if (precision == FFH_PRECISION_FP64)
-            return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
-                hipExtLaunchKernelGGL((k_assemble<double>), xcd_grid(grid(plan.num_upper, asm_block)), asm_block, 0, stream, e0, e1, 0, a);
-            });
-        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_assemble<float>), xcd_grid(grid(plan.num_upper, asm_block)), asm_block, 0, stream, e0, e1, 0, a); });
+        const int gr = xcd_grid(grid(plan.num_upper, asm_block));
+        return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
+            if (precision == FFH_PRECISION_FP64) {
+                if (asm_unroll == 1)
+                    hipExtLaunchKernelGGL((k_assemble<double, 1>), gr, asm_block, 0, stream, e0, e1, 0, a);
+                else if (asm_unroll == 2)
+                    hipExtLaunchKernelGGL((k_assemble<double, 2>), gr, asm_block, 0, stream, e0, e1, 0, a);
+                else
+                    hipExtLaunchKernelGGL((k_assemble<double, 4>), gr, asm_block, 0, stream, e0, e1, 0, a);
+            } else {
+                if (asm_unroll == 1)
+                    hipExtLaunchKernelGGL((k_assemble<float, 1>), gr, asm_block, 0, stream, e0, e1, 0, a);
+                else if (asm_unroll == 2)
+                    hipExtLaunchKernelGGL((k_assemble<float, 2>), gr, asm_block, 0, stream, e0, e1, 0, a);
+                else
+                    hipExtLaunchKernelGGL((k_assemble<float, 4>), gr, asm_block, 0, stream, e0, e1, 0, a);
+            }
+        });
     }
     RhsArgs rhs_args(bool noise, const double *f_in) const {
         RhsArgs g;
@@ -1570,13 +1577,18 @@ struct ffh_blob {
         }
         elem_block = pick;
         best = 1e30f;
+        int pick_u = asm_unroll;
         pick = asm_block;
-        for (int bs : sizes) {
-            asm_block = bs;
-            if ((rc = time_ms([&] { return run_assemble(); }, 3, ms))) return rc;
-            if (ms < best) best = ms, pick = bs;
-        }
+        const int unrolls[3] = {1, 2, 4};
+        for (int bs : sizes)
+            for (int un : unrolls) {
+                asm_block = bs;
+                asm_unroll = un;
+                if ((rc = time_ms([&] { return run_assemble(); }, 3, ms))) return rc;
+                if (ms < best) best = ms, pick = bs, pick_u = un;
+            }
         asm_block = pick;
+        asm_unroll = pick_u;
         std::vector<double> ones(3 * (size_t)N, 1.0);
         HIP_OR_FAIL(hipMemcpyAsync(tmp_in, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         best = 1e30f;
@@ -1776,12 +1788,10 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->upload(b->Em, P.matE);
     rc |= b->upload(b->last_det, P.last_det);
     {
-        char *ef = nullptr, *er = nullptr, *vv = nullptr;
+        char *ef = nullptr, *vv = nullptr;
         rc |= b->alloc(ef, 12 * (size_t)E * b->tsize);
-        rc |= b->alloc(er, 20 * (size_t)E * b->tsize);
         rc |= b->alloc(vv, 9 * (size_t)P.num_slots * b->tsize);
         b->eforce = ef;
-        b->erec = er;
         b->val = vv;
         // padding slots stay zero for good; real slots are rewritten by every assembly
         if (!rc && hipMemset(vv, 0, 9 * (size_t)P.num_slots * b->tsize) != hipSuccess) rc = 1;
@@ -1981,21 +1991,27 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
     return FFH_OK;
 }
 
-int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *spmv_waves_per_slice) {
+int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_unroll,
+                    int *spmv_waves_per_slice) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     if (element_block) *element_block = b->elem_block;
     if (assembly_block) *assembly_block = b->asm_block;
+    if (assembly_unroll) *assembly_unroll = b->asm_unroll;
     if (spmv_waves_per_slice) *spmv_waves_per_slice = b->spmv_waves;
     return FFH_OK;
 }
 
-int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int spmv_waves_per_slice) {
+int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_unroll,
+                        int spmv_waves_per_slice) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
-    if (!ok(element_block) || !ok(assembly_block) || (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3))
-        return fail(FFH_E_ARG, "block sizes must be 64, 128 or 256 and SpMV waves per slice 1 or 3");
+    if (!ok(element_block) || !ok(assembly_block) || (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3) ||
+        (assembly_unroll != 1 && assembly_unroll != 2 && assembly_unroll != 4))
+        return fail(FFH_E_ARG, "block sizes must be 64, 128 or 256, the unroll depth 1, 2 or 4 and SpMV waves "
+                               "per slice 1 or 3");
     b->elem_block = element_block;
     b->asm_block = assembly_block;
+    b->asm_unroll = assembly_unroll;
     b->spmv_waves = spmv_waves_per_slice;
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
     b->graphs.clear();

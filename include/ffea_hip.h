@@ -123,11 +123,14 @@ int ffh_blob_update_positions(ffh_blob *b);
 int ffh_blob_step(ffh_blob *b, int nsteps);
 
 /* Launch shapes, tuned at creation by timing this blob's own kernels (set FFH_AUTOTUNE=0 in
- * the environment to keep the defaults 256 / 256 / 3): threads per workgroup of the element
- * and assembly kernels (64, 128 or 256) and waves per 64-row SpMV slice (3: one per component;
- * 1: one wave computes all three).  Results are bit-identical for every choice. */
-int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *spmv_waves_per_slice);
-int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int spmv_waves_per_slice);
+ * the environment to keep the defaults 256 / 256 / 2 / 3): threads per workgroup of the element
+ * and assembly kernels (64, 128 or 256), the assembly loop's unroll depth (1, 2 or 4) and waves
+ * per 64-row SpMV slice (3: one per component; 1: one wave computes all three).  Results are
+ * bit-identical for every choice. */
+int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_unroll,
+                    int *spmv_waves_per_slice);
+int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_unroll,
+                        int spmv_waves_per_slice);
 
 /* ffh_blob_step replays one captured hipGraph per step (element, assembly, force gather and
  * the CG iterations of the last solve's count; the host polls once per step) unless disabled

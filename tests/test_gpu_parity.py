@@ -217,8 +217,8 @@ def test_graph_replay_equals_ordinary_launches(precision):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("batch", [False, True])
 def test_tuned_launch_shapes_give_identical_results(precision, batch):
-    """Every launch shape the tuner may pick (element / assembly workgroup size, SpMV waves per
-    slice) gives bit-identical trajectories, for one blob and for a batch."""
+    """Every launch shape the tuner may pick (element / assembly workgroup size, assembly unroll
+    depth, SpMV waves per slice) gives bit-identical trajectories, for one blob and for a batch."""
     from ffea_amd.blob import Batch
     seed = np.arange(5, 11, dtype=np.uint64)
     if batch:
@@ -235,10 +235,10 @@ def test_tuned_launch_shapes_give_identical_results(precision, batch):
         def make():
             return gpu_blob(m, prm, sc, seed, x0, precision)
     ref = None
-    for shape in ((256, 256, 3), (64, 128, 1), (128, 64, 3), (256, 64, 1)):
+    for shape in ((256, 256, 2, 3), (64, 128, 1, 1), (128, 64, 4, 3), (256, 64, 4, 1)):
         b = make()
         b.set_tuning(*shape)
-        assert b.tuning() == dict(zip(("element_block", "assembly_block", "spmv_waves_per_slice"), shape))
+        assert b.tuning() == dict(zip(b.TUNING_KEYS, shape))
         b.step(6)
         out = (b.positions(), b.rng_state(), b.last_cg())
         if ref is None:
