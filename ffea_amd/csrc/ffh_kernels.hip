@@ -242,7 +242,7 @@ struct CgCtl {
 // Element kernel
 // ----------------------------------------------------------------------------------------
 struct ElemArgs {
-    const double *X, *Y, *Z;
+    const double4 *pos;  // (x, y, z, 0) per linear node: one 32-byte load per node
     const int4 *enodes;
     const double *jinv0;  // SoA [9][E]
     const double *vol0, *A, *B, *G, *Em;
@@ -286,15 +286,15 @@ __device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, 
 // dpsi[a + 4 dim] (x-major).  Shared by the element kernel and the assembly, which recomputes
 // them per contribution: the same code rounds the same way, so the matrix is unchanged.
 template <typename R>
-__device__ __forceinline__ void element_geometry(const double *__restrict__ X, const double *__restrict__ Y,
-                                                 const double *__restrict__ Z, const int4 n, R J[3][3],
+__device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos, const int4 n, R J[3][3],
                                                  R dpsi[12], R &det) {
-    const int nn[4] = {n.x, n.y, n.z, n.w};
+    const double4 p0 = pos[n.x], p1 = pos[n.y], p2 = pos[n.z], p3 = pos[n.w];
+    const double4 pr[3] = {p1, p2, p3};
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        J[r][0] = (R)(X[nn[r + 1]] - X[nn[0]]);
-        J[r][1] = (R)(Y[nn[r + 1]] - Y[nn[0]]);
-        J[r][2] = (R)(Z[nn[r + 1]] - Z[nn[0]]);
+        J[r][0] = (R)(pr[r].x - p0.x);
+        J[r][1] = (R)(pr[r].y - p0.y);
+        J[r][2] = (R)(pr[r].z - p0.z);
     }
     dpsi[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
     dpsi[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     const int e = xcd_block((g.E + bd - 1) / bd) * bd + threadIdx.x;
     if (e >= g.E) return;
     R J[3][3], dpsi[12], det;
-    element_geometry<R>(g.X, g.Y, g.Z, g.enodes[e], J, dpsi, det);
+    element_geometry<R>(g.pos, g.enodes[e], J, dpsi, det);
     const double last = g.last_det[e];
     if (last * (double)det < 0) atomicAdd(&g.ctl->inversions, 1);
     g.last_det[e] = (double)det;
@@ -408,7 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
 // Assembly over the upper triangle of the block pattern
 // ----------------------------------------------------------------------------------------
 struct AsmArgs {
-    const double *X, *Y, *Z;  // positions: the element geometry is recomputed per contribution
+    const double4 *pos;  // positions: the element geometry is recomputed per contribution
+    const uint32_t *flat, *eoff;  // contributions entry-major, [eoff[u], eoff[u+1]) for entry u
     const int4 *enodes;
     const double *A, *B;
     const int64_t *toff;
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
         const int a = (code >> 2) & 3, b = code & 3;
         const int e = (int)(code >> 4);
         T J[3][3], dpsi[12], det;
-        element_geometry<T>(g.X, g.Y, g.Z, g.enodes[e], J, dpsi, det);
+        element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
         const T vol = (T)(1.0 / 6.0) * fabs(det);
         const T A = (T)g.A[e], B = (T)g.B[e];
         T ga[3], gb[3];
@@ -513,6 +514,120 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) v[tile * 9 + (3 * i + j) * 64 + lane] = acc[3 * j + i];
+    }
+}
+
+// One contribution (element block (a,b) of code) of an upper block, entry by entry, premultiplied
+// by vol as k_assemble adds it.  a == b exactly for the diagonal blocks.
+template <typename T>
+__device__ __forceinline__ void contribution(const AsmArgs &g, uint32_t code, T w[9]) {
+    const int a = (code >> 2) & 3, b = code & 3;
+    const int e = (int)(code >> 4);
+    T J[3][3], dpsi[12], det;
+    element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
+    const T vol = (T)(1.0 / 6.0) * fabs(det);
+    const T A = (T)g.A[e], B = (T)g.B[e];
+    T ga[3], gb[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const T *d = dpsi + 4 * i;
+        ga[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : d[3];
+        gb[i] = b == 0 ? d[0] : b == 1 ? d[1] : b == 2 ? d[2] : d[3];
+    }
+    const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            T v;
+            if (i == j) {
+                v = ga[i] * gb[i] * (A + B);
+                v += del2 * A;
+            } else if (a == b) {
+                v = (B + A) * (ga[i] * gb[j]);
+            } else {
+                v = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
+            }
+            w[3 * i + j] = v * vol;
+        }
+}
+
+// Two-phase assembly: one workgroup per tile of 64 upper blocks.  Phase 1: all 256 threads
+// evaluate the tile's contributions (entry-major, each block's in ascending element order) in
+// chunks of 256, nine values each into LDS -- independent gathers in flight instead of one
+// dependent chain per lane.  Phase 2: the lane of each block adds its contributions of the
+// chunk in order.  The values and the order of additions are k_assemble's: bit-identical.
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble2(AsmArgs g) {
+    __shared__ T lv[9][256];
+    const int ntiles = (int)(g.nupper >> 6);
+    const int tile = xcd_block(ntiles);
+    if (tile >= ntiles) return;
+    const int t = threadIdx.x;
+    const int64_t u0 = (int64_t)tile * 64;
+    const uint32_t c0 = g.eoff[u0], c1 = g.eoff[u0 + 64];
+    uint32_t s = 0, en = 0;
+    if (t < 64) {
+        s = g.eoff[u0 + t];
+        en = g.eoff[u0 + t + 1];
+    }
+    T acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
+    for (uint32_t ch = c0; ch < c1; ch += 256) {
+        const uint32_t f = ch + t;
+        if (f < c1) {
+            T w[9];
+            contribution<T>(g, g.flat[f], w);
+#pragma unroll
+            for (int k = 0; k < 9; k++) lv[k][t] = w[k];
+        }
+        __syncthreads();
+        if (t < 64) {
+            const uint32_t lo = s > ch ? s : ch, hi = en < ch + 256 ? en : ch + 256;
+            for (uint32_t i = lo; i < hi; i++)
+#pragma unroll
+                for (int k = 0; k < 9; k++) acc[k] += lv[k][i - ch];
+        }
+        __syncthreads();
+    }
+    if (t >= 64) return;
+    const int64_t u = u0 + t;
+    const uint8_t fl = g.flags[u];
+    if (fl & 4) return;  // tile padding
+    const bool diag = (fl & 1) != 0;
+    const int node = g.node[u];
+    if (diag) {
+        if (fl & 2) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
+            acc[0] = acc[4] = acc[8] = (T)1.0;
+        } else {
+            const T z = (T)g.zeta[node];
+            acc[0] += z;
+            acc[4] += z;
+            acc[8] += z;
+        }
+        g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[0];
+        g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[4];
+        g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[8];
+    }
+    T *v = reinterpret_cast<T *>(g.val);
+    {
+        const int64_t slot = g.slot[u];
+        const int64_t tl = slot & ~(int64_t)63;
+        const int ln = (int)(slot & 63);
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[tl * 9 + k * 64 + ln] = acc[k];
+    }
+    const int32_t mslot = g.mirror[u];
+    if (mslot >= 0) {
+        const int64_t tl = (int64_t)mslot & ~(int64_t)63;
+        const int ln = mslot & 63;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) v[tl * 9 + (3 * i + j) * 64 + ln] = acc[3 * j + i];
     }
 }
 
@@ -834,7 +949,7 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
 
 // euler_integrate (src/Blob.cpp:3941-3953): vel = v; pos += v * dt.  Skipped after a CG
 // failure, as the reference skips it.  Thread 0 advances the stream by one step's draws.
-__global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double *X, double *Y, double *Z, double dt,
+__global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, double dt,
                                                      uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance) {
     const int n = blockIdx.x * kBlock + threadIdx.x;
     if (n == 0 && advance) {
@@ -849,9 +964,11 @@ __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double *X, doubl
     v.vel[3 * n] = x0;
     v.vel[3 * n + 1] = x1;
     v.vel[3 * n + 2] = x2;
-    X[n] += x0 * dt;
-    Y[n] += x1 * dt;
-    Z[n] += x2 * dt;
+    double4 p = pos[n];
+    p.x += x0 * dt;
+    p.y += x1 * dt;
+    p.z += x2 * dt;
+    pos[n] = p;
 }
 
 __global__ void k_reset_inversions(CgCtl *ctl) {
@@ -891,12 +1008,12 @@ __global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *pa
     double part = 0.0;
     if (e < g.E) {
         const int4 n = g.enodes[e];
-        const int nn[4] = {n.x, n.y, n.z, n.w};
+        const double4 p0 = g.pos[n.x], pr[3] = {g.pos[n.y], g.pos[n.z], g.pos[n.w]};
         double J[3][3];
         for (int r = 0; r < 3; r++) {
-            J[r][0] = g.X[nn[r + 1]] - g.X[nn[0]];
-            J[r][1] = g.Y[nn[r + 1]] - g.Y[nn[0]];
-            J[r][2] = g.Z[nn[r + 1]] - g.Z[nn[0]];
+            J[r][0] = pr[r].x - p0.x;
+            J[r][1] = pr[r].y - p0.y;
+            J[r][2] = pr[r].z - p0.z;
         }
         double c1 = J[2][2] * J[1][1] - J[2][1] * J[1][2];
         double c2 = J[2][1] * J[0][2] - J[2][2] * J[0][1];
@@ -943,12 +1060,12 @@ struct ffh_blob {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     // device arrays
-    double *X = nullptr, *Y = nullptr, *Z = nullptr;
+    double4 *pos = nullptr;
     int4 *enodes = nullptr;
     double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
            *last_det = nullptr;
     void *eforce = nullptr, *val = nullptr;
-    uint32_t *uctb = nullptr, *nctb = nullptr, *rng_off = nullptr;
+    uint32_t *uctb = nullptr, *nctb = nullptr, *rng_off = nullptr, *uflat = nullptr, *ueoff = nullptr;
     int64_t *ut_off = nullptr, *nt_off = nullptr;
     uint16_t *ut_width = nullptr, *ut_cnt = nullptr, *nt_cnt = nullptr;
     int32_t *slot_col = nullptr, *slice_width = nullptr, *up_slot = nullptr, *up_mirror = nullptr,
@@ -1125,9 +1242,7 @@ struct ffh_blob {
     }
     ElemArgs elem_args() const {
         ElemArgs g;
-        g.X = X;
-        g.Y = Y;
-        g.Z = Z;
+        g.pos = pos;
         g.enodes = enodes;
         g.jinv0 = jinv0;
         g.vol0 = vol0;
@@ -1158,12 +1273,12 @@ struct ffh_blob {
     int run_assemble() {
         mp_edges = false;
         AsmArgs a;
-        a.X = X;
-        a.Y = Y;
-        a.Z = Z;
+        a.pos = pos;
         a.enodes = enodes;
         a.A = A;
         a.B = B;
+        a.flat = uflat;
+        a.eoff = ueoff;
         a.toff = ut_off;
         a.twidth = ut_width;
         a.cnt = ut_cnt;
@@ -1177,6 +1292,15 @@ struct ffh_blob {
         a.dinv = dinv;
         a.nupper = plan.num_upper;
         const int gr = xcd_grid(grid(plan.num_upper, asm_block));
+        if (asm_unroll == 0) {  // two-phase: one workgroup per 64-entry tile
+            const int gt = xcd_grid((int)(plan.num_upper / 64));
+            return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
+                if (precision == FFH_PRECISION_FP64)
+                    hipExtLaunchKernelGGL((k_assemble2<double>), gt, 256, 0, stream, e0, e1, 0, a);
+                else
+                    hipExtLaunchKernelGGL((k_assemble2<float>), gt, 256, 0, stream, e0, e1, 0, a);
+            });
+        }
         return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
             if (precision == FFH_PRECISION_FP64) {
                 if (asm_unroll == 1)
@@ -1542,7 +1666,7 @@ struct ffh_blob {
         int adv = plan.draws_per_step > 0 ? 1 : 0;
         Mat3u j1 = plan.step_jump[0], j2 = plan.step_jump[1];
         return launch(5, [&](hipEvent_t e0, hipEvent_t e1) {
-            hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, X, Y, Z, dt, rng_state, j1, j2, adv);
+            hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, pos, dt, rng_state, j1, j2, adv);
         });
     }
     // ---- autotuning of launch shapes (results do not depend on them) ----
@@ -1579,9 +1703,10 @@ struct ffh_blob {
         best = 1e30f;
         int pick_u = asm_unroll;
         pick = asm_block;
-        const int unrolls[3] = {1, 2, 4};
+        const int unrolls[4] = {0, 1, 2, 4};  // 0: the two-phase kernel (block size unused)
         for (int bs : sizes)
             for (int un : unrolls) {
+                if (un == 0 && bs != 256) continue;
                 asm_block = bs;
                 asm_unroll = un;
                 if ((rc = time_ms([&] { return run_assemble(); }, 3, ms))) return rc;
@@ -1767,11 +1892,11 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     }
     const int N = b->N, E = b->E;
     int rc = 0;
-    rc |= b->alloc(b->X, 3 * (size_t)N);
+    rc |= b->alloc(b->pos, (size_t)N);
     if (!rc) {
-        b->Y = b->X + N;
-        b->Z = b->X + 2 * (size_t)N;
-        if (hipMemcpy(b->X, P.pos.data(), P.pos.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = 1;
+        std::vector<double4> p4(N);
+        for (int i = 0; i < N; i++) p4[i] = make_double4(P.pos[i], P.pos[(size_t)N + i], P.pos[2 * (size_t)N + i], 0.0);
+        if (hipMemcpy(b->pos, p4.data(), p4.size() * sizeof(double4), hipMemcpyHostToDevice) != hipSuccess) rc = 1;
     }
     {
         std::vector<int4> en(E);
@@ -1800,6 +1925,8 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->upload(b->ut_width, P.utile.width);
     rc |= b->upload(b->ut_cnt, P.utile.cnt);
     rc |= b->upload(b->uctb, P.utile.code);
+    rc |= b->upload(b->uflat, P.uctb);
+    rc |= b->upload(b->ueoff, P.uctb_off);
     rc |= b->upload(b->nt_off, P.ntile.off);
     rc |= b->upload(b->nt_cnt, P.ntile.cnt);
     rc |= b->upload(b->nctb, P.ntile.code);
@@ -2006,9 +2133,9 @@ int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int 
     if (!b) return fail(FFH_E_ARG, "null blob");
     auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
     if (!ok(element_block) || !ok(assembly_block) || (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3) ||
-        (assembly_unroll != 1 && assembly_unroll != 2 && assembly_unroll != 4))
-        return fail(FFH_E_ARG, "block sizes must be 64, 128 or 256, the unroll depth 1, 2 or 4 and SpMV waves "
-                               "per slice 1 or 3");
+        (assembly_unroll != 0 && assembly_unroll != 1 && assembly_unroll != 2 && assembly_unroll != 4))
+        return fail(FFH_E_ARG, "block sizes must be 64, 128 or 256, the assembly unroll depth 0 (two-phase), 1, 2 "
+                               "or 4 and SpMV waves per slice 1 or 3");
     b->elem_block = element_block;
     b->asm_block = assembly_block;
     b->asm_unroll = assembly_unroll;
@@ -2084,11 +2211,13 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
 int ffh_blob_set_positions(ffh_blob *b, const double *pos) {
     if (!b || !pos) return fail(FFH_E_ARG, "null argument");
     const int N = b->N;
-    std::vector<double> h(3 * (size_t)N, 0.0);
+    std::vector<double4> h(N, make_double4(0.0, 0.0, 0.0, 0.0));
     for (int i = 0; i < N; i++)
-        if (b->plan.orig_of[i] >= 0)
-            for (int c = 0; c < 3; c++) h[(size_t)c * N + i] = pos[3 * (size_t)b->plan.orig_of[i] + c];
-    HIP_OR_FAIL(hipMemcpyAsync(b->X, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        if (b->plan.orig_of[i] >= 0) {
+            const double *q = pos + 3 * (size_t)b->plan.orig_of[i];
+            h[i] = make_double4(q[0], q[1], q[2], 0.0);
+        }
+    HIP_OR_FAIL(hipMemcpyAsync(b->pos, h.data(), h.size() * sizeof(double4), hipMemcpyHostToDevice, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return FFH_OK;
 }
@@ -2097,12 +2226,16 @@ int ffh_blob_get_positions(ffh_blob *b, double *pos) {
     if (!b || !pos) return fail(FFH_E_ARG, "null argument");
     const int N = b->N;
     const ffh::Plan &P = b->plan;
-    std::vector<double> h(3 * (size_t)N);
-    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->X, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    std::vector<double4> h(N);
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->pos, h.size() * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     for (int i = 0; i < N; i++)
-        if (P.orig_of[i] >= 0)
-            for (int c = 0; c < 3; c++) pos[3 * (size_t)P.orig_of[i] + c] = h[(size_t)c * N + i];
+        if (P.orig_of[i] >= 0) {
+            double *q = pos + 3 * (size_t)P.orig_of[i];
+            q[0] = h[i].x;
+            q[1] = h[i].y;
+            q[2] = h[i].z;
+        }
     // linearise_element (src/tetra_element_linear.cpp:567-591)
     for (size_t k = 0; k < P.mid_index.size(); k++)
         for (int c = 0; c < 3; c++)
