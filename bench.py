@@ -204,6 +204,18 @@ def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     lib.set_package_seed(seed)
     s = lib.stream(per_thread=True) if kind == "reference" else lib.stream()
     etot = sum(m.num_elements for m in ms)
+    note = ""
+    if len(ms) == 1 and etot > 2_000_000:
+        # one large blob: the reference's setup alone would take minutes, so its step is timed on
+        # a 48^3-cell box of the same construction and scaled by the element count (its step time
+        # is linear in the elements)
+        from ffea_amd import mesh as M
+        cells, jitter, core, _ = CONFIGS[cfg]
+        m1 = M.box_mesh(48, 48, 48, jitter=jitter, seed=4)
+        if core != 1.0:
+            M.stiff_core(m1, axis=0, factor=core)
+        ms, xs = [m1], [M.perturb(m1, 0.02, seed=1)]
+        note = f" on a 48^3-cell box of the same construction ({m1.num_elements} tets)"
     nb, esub = 0, 0
     while nb < len(ms) and (nb == 0 or esub < etot / 8):
         esub += ms[nb].num_elements
@@ -226,10 +238,71 @@ def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     log(f"[cpu {kind}] {n} steps of {nb} blob(s) in {el:.2f}s")
     part = "" if nb == len(ms) else (f" restricted to its first {nb} of {len(ms)} blobs ({esub} of {etot} tets), "
                                      f"rate scaled by {esub}/{etot}")
+    if note:
+        part = note + f", rate scaled by {esub}/{etot}"
     return {"value": n / el * esub / etot, "unit": "steps/s", "cores": cores, "kind": kind,
             "sample": f"{n} noise-on steps of the same {cfg} workload{part} after 1 warm-up step "
                       f"({'reference objects -O3 -ffast-math, OpenMP within blob' if kind == 'reference' else 'FP64 restatement, 1 thread'}); "
                       f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
+
+
+def equipartition(ms, prm, sc, xs, precision, steps=4000, start=2000, every=100):
+    """Noise-on run from the bench's initial state: mean strain energy over steps start..steps
+    (sampled every `every` steps, Blob::make_measurements' observable) against the equipartition
+    value (3 N - 6) kT / 2 of a free blob -- the check of tests/physics/sphere_63_120_nomass."""
+    from ffea_amd.blob import Batch, Blob, package_seed
+    if len(ms) == 1:
+        b = Blob(ms[0], prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision, pos=xs[0])
+        pinned = ms[0].pinned.size
+    else:
+        b = Batch(ms, prm, sc, rng_state=package_seed(prm.rng_seed), precision=precision)
+        b.set_positions(np.concatenate(xs) * sc)
+        pinned = sum(m.pinned.size for m in ms)
+    nlin = b.sizes()["num_linear"]
+    modes = 3 * (nlin - pinned) - 6 * len(ms)
+    e = []
+    for k in range(0, steps + 1, every):
+        if k >= start:
+            e.append(b.strain_energy())
+        if k < steps:
+            b.step(every)
+    b.close()
+    ratio = float(np.mean(e)) / (modes * prm.kT / 2)
+    log(f"[equipartition {precision}] <U> / ((3N-6) kT/2) = {ratio:.4f} over steps {start}..{steps}")
+    return {"steps": steps, "average_from_step": start, "sample_every": every, "modes": modes,
+            "mean_energy_over_theory": ratio}
+
+
+def mixed_vs_fp64(ms, prm, sc, xs, steps=20):
+    """The mixed path against the FP64 path (whose parity with the reference is established at
+    c0/c1 and in the tests): noise-off positions and velocities after `steps` steps from the
+    bench's initial state, and one CG solution of the same force vector."""
+    from ffea_amd import units as U
+    from ffea_amd.blob import Batch, Blob
+    p0 = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=0, calc_stokes=1, rng_seed=prm.rng_seed)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        if len(ms) == 1:
+            b = Blob(ms[0], p0, sc, rng_state=np.arange(1, 7, dtype=np.uint64), precision=prec, pos=xs[0])
+        else:
+            b = Batch(ms, p0, sc, rng_state=np.arange(1, 7, dtype=np.uint64), precision=prec)
+            b.set_positions(np.concatenate(xs) * sc)
+        b.update_internal_forces()
+        if prec == "fp64":
+            b.update_positions()
+            f = b.rhs()
+        v = b.solve(f).ravel()
+        b.set_positions(np.concatenate(xs) * sc)
+        b.step(steps)
+        out[prec] = (v, b.positions().ravel(), b.velocities().ravel())
+        b.close()
+    x0 = np.concatenate(xs).ravel() * sc
+    (v64, p64, w64), (vm, pm, wm) = out["fp64"], out["mixed"]
+    rel = lambda a, r: float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
+    res = {"steps": steps, "noise": "off", "cg_solution_rel": rel(vm, v64), "velocity_rel": rel(wm, w64),
+           "position_rel_to_displacement": float(np.linalg.norm(pm - p64) / np.linalg.norm(p64 - x0))}
+    log(f"[mixed vs fp64] {res}")
+    return res
 
 
 def parity_vs_reference(cfg, ms, prm, sc, xs, steps):
@@ -252,14 +325,17 @@ def parity_vs_reference(cfg, ms, prm, sc, xs, steps):
         lib = pyoracle.Reference() if kind == "reference" else pyoracle.Oracle()
         lib.set_package_seed(lib.package_seed(p0.rng_seed))
         s = lib.stream()
+        log(f"[parity {cfg}] {kind}: setting up")
         b = lib.blob(m, p0, sc, pos=x0)
         inv, rhs = b.forces_only(s)
         st, v = b.solve(rhs)
         b.close()
         b = lib.blob(m, p0, sc, pos=x0)
+        log(f"[parity {cfg}] {kind}: {steps} steps")
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             assert pyoracle.world_step(lib, [b], s) == 0
+            log(f"[parity {cfg}] step {k + 1} done")
         ref.update(kind=kind, rhs=rhs, v=v, pos=b.get("pos"), vel=b.get("vel"), secs=time.perf_counter() - t0)
         b.close()
 
@@ -423,8 +499,11 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--epochs", type=int, default=20, help="c3: training epochs")
-    ap.add_argument("--parity-steps", type=int, default=10,
-                    help="noise-off steps compared with the reference's objects (0: skip)")
+    ap.add_argument("--parity-steps", type=int, default=None,
+                    help="noise-off steps compared with the reference's strict objects (0: skip; default 10 "
+                         "for c0 and c1, 0 above: their setup alone takes minutes on the host)")
+    ap.add_argument("--equipartition-steps", type=int, default=4000,
+                    help="noise-on steps of the equipartition check, averaged over the second half (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -453,8 +532,14 @@ def main():
             "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
     cpu = None if a.no_cpu else cpu_baseline(a.config, ms, prm, sc, xs, a.cpu_seconds)
     parity = None
-    if not a.no_cpu and a.parity_steps > 0:
-        parity = parity_vs_reference(a.config, ms, prm, sc, xs, a.parity_steps)
+    psteps = a.parity_steps if a.parity_steps is not None else (10 if a.config in ("c0", "c1") else 0)
+    if not a.no_cpu and psteps > 0:
+        parity = parity_vs_reference(a.config, ms, prm, sc, xs, psteps)
+    mvf = None if a.no_mixed else mixed_vs_fp64(ms, prm, sc, xs)
+    equi = None
+    if a.equipartition_steps > 0:
+        equi = {p: equipartition(ms, prm, sc, xs, p, a.equipartition_steps, a.equipartition_steps // 2)
+                for p in (("fp64",) if a.no_mixed else ("fp64", "mixed"))}
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
     out = {
         "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
@@ -481,6 +566,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity_vs_reference": parity,
+        "mixed_vs_fp64": mvf,
+        "equipartition": equi,
     }
     if calib:
         out["copy_calibration"] = calib
