@@ -1480,22 +1480,17 @@ struct ffh_blob {
         if (rc) return rc;
         bf16_t *h = mp_H[0], *hn = mp_H[1];
         for (int l = 0; l < kNLay; l++) {
-            rc = launch(8, [&](hipEvent_t e0, hipEvent_t e1) {
-                hipExtLaunchKernelGGL((k_mpnn_spmm<bf16_t, false>), P_slice, 256, 0, stream, e0, e1, 0, a, (const bf16_t *)h,
-                                      kHid, 0, (void *)mp_G[1], (const bf16_t *)nullptr);
-            });
-            if (rc) return rc;
-            rc = launch(9, [&](hipEvent_t e0, hipEvent_t e1) {
-                hipExtLaunchKernelGGL(k_mpnn_dense, S, 128, 0, stream, e0, e1, 0, a, l, (const bf16_t *)h,
-                                      (const bf16_t *)mp_G[1], hn);
+            rc = launch(9, [&](hipEvent_t e0, hipEvent_t e1) {  // message + dense, fused
+                hipExtLaunchKernelGGL(k_mpnn_layer, P_slice, 256, 0, stream, e0, e1, 0, a, l, (const bf16_t *)h, hn);
             });
             if (rc) return rc;
             std::swap(h, hn);
         }
+        (void)S;
         double *zrp = rprev ? parts_zrp() : (double *)nullptr;
         rc = launch(10, [&](hipEvent_t e0, hipEvent_t e1) {
-            hipExtLaunchKernelGGL(k_mpnn_out_cg, P_slice, 192, 0, stream, e0, e1, 0, a, (const bf16_t *)h, uu, z, rvec, rprev,
-                                  parts, zrp);
+            hipExtLaunchKernelGGL(k_mpnn_out_cg1, P_slice, 64, 0, stream, e0, e1, 0, a, (const bf16_t *)h, uu, z, rvec,
+                                  rprev, parts, zrp);
         });
         if (rc) return rc;
         return totals((1 << kTotRZ) | (rprev ? 1 << kTotZRP : 0));

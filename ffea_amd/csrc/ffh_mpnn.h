@@ -234,6 +234,103 @@ __global__ __launch_bounds__(128) void k_mpnn_dense(MpnnArgs a, int layer, const
     }
 }
 
+// One message-passing layer fused: h_out = relu([h | A h] Wcat + b) for one 64-row slice per
+// block.  Threads 4r..4r+3 gather channels 8q..8q+7 of row r's message into LDS (bf16, as
+// k_mpnn_spmm stores it), then waves 0 and 1 run the dense layer on MFMA with the h half of the A
+// operand from memory and the message half from LDS -- the message never goes to HBM.
+__global__ __launch_bounds__(256) void k_mpnn_layer(MpnnArgs a, int layer, const bf16_t *__restrict__ H,
+                                                    bf16_t *__restrict__ Hout) {
+    __shared__ bf16_t gs[64][kHid + 8];  // +8: rows 80 bytes apart, 8-byte aligned reads
+    const int slice = xcd_block(a.nslices);
+    if (slice >= a.nslices || blob_done(a, slice)) return;
+    {
+        const int rloc = threadIdx.x >> 2, q = threadIdx.x & 3;
+        const int w = a.slice_width[slice];
+        const int64_t base = a.slice_off[slice];
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < w; t++) {
+            const int64_t slot = base + (int64_t)t * 64 + rloc;
+            const int c = a.col[slot];
+            const float wgt = a.aw[slot];
+            const uint4 v = *reinterpret_cast<const uint4 *>(H + (size_t)c * kHid + 8 * q);
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+            for (int k = 0; k < 4; k++) {
+                acc[2 * k] += wgt * bf2f((bf16_t)(vv[k] & 0xffffu));
+                acc[2 * k + 1] += wgt * bf2f((bf16_t)(vv[k] >> 16));
+            }
+        }
+        for (int k = 0; k < 8; k++) gs[rloc][8 * q + k] = f2bf(acc[k]);
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wv >= 2) return;
+    const int row0 = slice * 64 + wv * 32;
+    const int rl = wv * 32 + (lane & 31), kq = 4 * (lane >> 5);
+    const int r = slice * 64 + rl;
+    const bf16_t *wT = a.wT + (size_t)layer * kHid * 64;
+    f32x16_t acc = {0};
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        bf16x4_t av;
+        if (s < 4) {
+            av = *reinterpret_cast<const bf16x4_t *>(H + (size_t)r * kHid + s * 8 + kq);
+        } else {
+            const bf16_t *g = &gs[rl][(s - 4) * 8 + kq];
+            av[0] = (short)g[0];
+            av[1] = (short)g[1];
+            av[2] = (short)g[2];
+            av[3] = (short)g[3];
+        }
+        const bf16x4_t bv = *reinterpret_cast<const bf16x4_t *>(wT + (lane & 31) * 64 + s * 8 + kq);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    const int j = lane & 31;
+    const float bj = a.wf[kOffLay + layer * kLayStride + 2 * kHid * kHid + j];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int i = 8 * (v >> 2) + kq + (v & 3);
+        Hout[(size_t)(row0 + i) * kHid + j] = f2bf(fmaxf(acc[v] + bj, 0.0f));
+    }
+}
+
+// Output layer inside CG, one wave per slice (one lane per row): z = u + s_b (Wo h3 + bo), 0 on
+// pinned rows, with the partials of z.r and z.r_prev.
+__global__ __launch_bounds__(64) void k_mpnn_out_cg1(MpnnArgs a, const bf16_t *__restrict__ H3, const double *parts_uu,
+                                                     double *__restrict__ z, const double *__restrict__ r,
+                                                     const double *__restrict__ r_prev, double *parts_rz,
+                                                     double *parts_zrp) {
+    __shared__ double sh[4];
+    const int slice = xcd_block(a.nslices);
+    if (slice >= a.nslices || blob_done(a, slice)) return;
+    const int b = a.slice_blob[slice];
+    const double s = blob_scale(a, parts_uu, b, sh);
+    const int row = slice * 64 + (int)threadIdx.x;
+    float o[kNOut];
+    for (int c = 0; c < kNOut; c++) o[c] = a.wf[kOffBo + c];
+    const uint4 *hp = reinterpret_cast<const uint4 *>(H3 + (size_t)row * kHid);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = hp[q];
+        const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float h = bf2f((bf16_t)(k & 1 ? vv[k >> 1] >> 16 : vv[k >> 1] & 0xffffu));
+            for (int c = 0; c < kNOut; c++) o[c] += a.wf[kOffWo + c * kHid + 8 * q + k] * h;
+        }
+    }
+    const bool zero = a.pinned[row] || s == 0.0;
+    double rz = 0.0, zp = 0.0;
+    for (int c = 0; c < kNOut; c++) {
+        const size_t k = 3 * (size_t)row + c;
+        const double zz = zero ? 0.0 : z[k] + s * (double)o[c];
+        z[k] = zz;
+        rz += zz * r[k];
+        if (r_prev) zp += zz * r_prev[k];
+    }
+    block_partial_at(rz, parts_rz, slice, sh);
+    if (r_prev) block_partial_at(zp, parts_zrp, slice, sh);
+}
+
 // Output layer inside CG: z = u + s_b (Wo h3 + bo) (0 on pinned rows), with the partials of
 // z.r and z.r_prev for the flexible CG's beta.  One block = one slice, thread = (row, comp).
 __global__ __launch_bounds__(192) void k_mpnn_out_cg(MpnnArgs a, const bf16_t *__restrict__ H3, const double *parts_uu,
