@@ -273,6 +273,60 @@ def equipartition(ms, prm, sc, xs, precision, steps=4000, start=2000, every=100)
             "mean_energy_over_theory": ratio}
 
 
+def long_trajectory(cfg, ms, prm, sc, xs, steps):
+    """A long noise-off trajectory of the GPU paths against the reference's own objects in its
+    fast build (oracle/_ref/libffea_ref_fast.so: -O3 -ffast-math, OpenMP on every host core;
+    the strict build would take the better part of an hour for 1,000 c1 steps): positions and
+    velocities every `steps // 10` steps, relative to the displacement."""
+    import threading
+    from ffea_amd import units as U
+    from ffea_amd.blob import Blob
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if len(ms) != 1 or not os.path.exists(pyoracle.REF_FAST_SO):
+        return None
+    m, x0 = ms[0], xs[0]
+    p0 = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=0, calc_stokes=1, rng_seed=prm.rng_seed)
+    every = max(1, steps // 10)
+    ref = []
+
+    def cpu():
+        lib = pyoracle.Reference(pyoracle.REF_FAST_SO)
+        lib.set_package_seed(lib.package_seed(p0.rng_seed))
+        s = lib.stream(per_thread=True)
+        b = lib.blob(m, p0, sc, pos=x0)
+        for k in range(1, steps + 1):
+            assert pyoracle.world_step(lib, [b], s) == 0
+            if k % every == 0:
+                ref.append((k, b.get("pos"), b.get("vel")))
+                log(f"[trajectory {cfg}] reference step {k}")
+        b.close()
+
+    old = threading.stack_size(2 << 30)
+    try:
+        t = threading.Thread(target=cpu)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    xs0 = x0.ravel() * sc
+    out = {"reference": "reference objects, fast build (OpenMP)", "steps": steps, "noise": "off"}
+    for prec in ("fp64", "mixed"):
+        b = Blob(m, p0, sc, rng_state=np.arange(1, 7, dtype=np.uint64), precision=prec, pos=x0)
+        worst_p = worst_v = 0.0
+        done = 0
+        for k, pr, vr in ref:
+            b.step(k - done)
+            done = k
+            p, v = b.positions().ravel(), b.velocities().ravel()
+            worst_p = max(worst_p, float(np.linalg.norm(p - pr) / np.linalg.norm(pr - xs0)))
+            worst_v = max(worst_v, float(np.linalg.norm(v - vr) / np.linalg.norm(vr)))
+        b.close()
+        out[prec] = {"max_position_rel_to_displacement": worst_p, "max_velocity_rel": worst_v}
+    log(f"[trajectory {cfg}] {out}")
+    return out
+
+
 def mixed_vs_fp64(ms, prm, sc, xs, steps=20):
     """The mixed path against the FP64 path (whose parity with the reference is established at
     c0/c1 and in the tests): noise-off positions and velocities after `steps` steps from the
@@ -502,6 +556,8 @@ def main():
     ap.add_argument("--parity-steps", type=int, default=None,
                     help="noise-off steps compared with the reference's strict objects (0: skip; default 10 "
                          "for c0 and c1, 0 above: their setup alone takes minutes on the host)")
+    ap.add_argument("--trajectory-steps", type=int, default=0,
+                    help="noise-off steps compared with the reference's fast build (0: skip; c1 asks for 1000)")
     ap.add_argument("--equipartition-steps", type=int, default=4000,
                     help="noise-on steps of the equipartition check, averaged over the second half (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -536,6 +592,7 @@ def main():
     if not a.no_cpu and psteps > 0:
         parity = parity_vs_reference(a.config, ms, prm, sc, xs, psteps)
     mvf = None if a.no_mixed else mixed_vs_fp64(ms, prm, sc, xs)
+    traj = long_trajectory(a.config, ms, prm, sc, xs, a.trajectory_steps) if a.trajectory_steps > 0 else None
     equi = None
     if a.equipartition_steps > 0:
         equi = {p: equipartition(ms, prm, sc, xs, p, a.equipartition_steps, a.equipartition_steps // 2)
@@ -567,6 +624,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_vs_reference": parity,
         "mixed_vs_fp64": mvf,
+        "trajectory_vs_reference": traj,
         "equipartition": equi,
     }
     if calib:

@@ -38,7 +38,7 @@ EXPORTS = (
     "ffh_mpnn_set_weights", "ffh_mpnn_get_weights", "ffh_blob_set_preconditioner",
     "ffh_mpnn_apply", "ffh_mpnn_harvest", "ffh_mpnn_harvest_count", "ffh_mpnn_train",
     "ffh_mpnn_loss_grad", "ffh_blob_steps_done", "ffh_blob_set_graphs", "ffh_blob_tuning",
-    "ffh_blob_set_tuning",
+    "ffh_blob_set_tuning", "ffh_blob_set_external_forces",
 )
 FFH_PRECOND_JACOBI = 0
 FFH_PRECOND_LEARNED = 1
@@ -99,6 +99,7 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_mpnn_loss_grad.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_double), _fp]
     L.ffh_blob_steps_done.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.ffh_blob_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.ffh_blob_set_external_forces.argtypes = [C.c_void_p, C.c_void_p]
     L.ffh_blob_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
     L.ffh_blob_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
     L.ffh_blob_destroy.argtypes = [C.c_void_p]

@@ -108,6 +108,16 @@ class Blob:
         _lib.check(self._L.ffh_blob_set_tuning(self._h, int(element_block), int(assembly_block),
                                                int(assembly_unroll), int(spmv_waves_per_slice)))
 
+    def set_external_forces(self, force_meso=None) -> None:
+        """Forces added to every following position phase ([N_t, 3] meso units; None clears)."""
+        if force_meso is None:
+            _lib.check(self._L.ffh_blob_set_external_forces(self._h, None))
+            return
+        f = np.ascontiguousarray(force_meso, np.float64).ravel()
+        if f.size != 3 * self.num_nodes:
+            raise ValueError("external forces need 3 values per node")
+        _lib.check(self._L.ffh_blob_set_external_forces(self._h, f.ctypes.data_as(C.c_void_p)))
+
     def set_graphs(self, enable: bool) -> None:
         _lib.check(self._L.ffh_blob_set_graphs(self._h, int(bool(enable))))
 

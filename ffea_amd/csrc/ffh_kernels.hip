@@ -665,6 +665,7 @@ struct RhsArgs {
     uint32_t node_base;
     int noise;
     double kT, dt;
+    const double *ext;  // external forces per linear node [3N], midpoints folded; or nullptr
 };
 
 template <typename T>
@@ -700,6 +701,9 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             const uint32_t code = tl[64 * k];
             f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
         }
+        // surface / external forces come after the element forces and before the Stokes noise
+        // (src/Blob.cpp:3842-3851, then :3895-3898)
+        if (g.ext) f += g.ext[3 * (size_t)n + i];
         if (noise) {
             Stream6 st;
 #pragma unroll
@@ -1079,6 +1083,8 @@ struct ffh_blob {
     int32_t *slice_blob = nullptr, *blob_slice_off = nullptr;
     int nblobs = 1;
     bool use_tot = false;  // some blob has more than kTotSlicesMin slices
+    double *ext = nullptr;  // external forces (ffh_blob_set_external_forces)
+    bool ext_on = false;
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
     int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
     double *tot = nullptr;
@@ -1336,6 +1342,7 @@ struct ffh_blob {
         g.noise = noise ? 1 : 0;
         g.kT = kT;
         g.dt = dt;
+        g.ext = (ext_on && !f_in) ? ext : nullptr;
         return g;
     }
     int run_rhs(const double *f_in = nullptr) {
@@ -2137,6 +2144,48 @@ int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int 
     b->spmv_waves = spmv_waves_per_slice;
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
     b->graphs.clear();
+    return FFH_OK;
+}
+
+int ffh_blob_set_external_forces(ffh_blob *b, const double *force) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the force gather's arguments change
+    b->graphs.clear();
+    if (!force) {
+        b->ext_on = false;
+        return FFH_OK;
+    }
+    const ffh::Plan &P = b->plan;
+    const size_t NT = (size_t)P.num_nodes_total;
+    std::vector<double> full(force, force + 3 * NT);
+    for (size_t k = 0; k < 3 * NT; k++)
+        if (!std::isfinite(full[k])) return fail(FFH_E_ARG, "non-finite external force");
+    // linearise_force (src/Blob.cpp:1066-1085): element by element, each vertex takes half the
+    // sum of the forces on its three edge midpoints, then the element's midpoints are zeroed
+    for (int e = 0; e < P.num_elements; e++) {
+        int v[4];
+        for (int a = 0; a < 4; a++) v[a] = P.orig_of[P.enodes[4 * (size_t)e + a]];
+        const int32_t *m = &P.emid[6 * (size_t)e];
+        for (int j = 0; j < 3; j++) {
+            full[3 * (size_t)v[0] + j] += 0.5 * (full[3 * (size_t)m[0] + j] + full[3 * (size_t)m[1] + j] + full[3 * (size_t)m[2] + j]);
+            full[3 * (size_t)v[1] + j] += 0.5 * (full[3 * (size_t)m[0] + j] + full[3 * (size_t)m[3] + j] + full[3 * (size_t)m[4] + j]);
+            full[3 * (size_t)v[2] + j] += 0.5 * (full[3 * (size_t)m[1] + j] + full[3 * (size_t)m[3] + j] + full[3 * (size_t)m[5] + j]);
+            full[3 * (size_t)v[3] + j] += 0.5 * (full[3 * (size_t)m[2] + j] + full[3 * (size_t)m[4] + j] + full[3 * (size_t)m[5] + j]);
+        }
+        for (int k = 0; k < 6; k++)
+            for (int j = 0; j < 3; j++) full[3 * (size_t)m[k] + j] = 0.0;
+    }
+    std::vector<double> lin(3 * (size_t)b->N, 0.0);
+    for (int i = 0; i < b->N; i++)
+        if (P.orig_of[i] >= 0)
+            for (int j = 0; j < 3; j++) lin[3 * (size_t)i + j] = full[3 * (size_t)P.orig_of[i] + j];
+    if (!b->ext) {
+        int rc = b->alloc(b->ext, lin.size());
+        if (rc) return rc;
+    }
+    HIP_OR_FAIL(hipMemcpyAsync(b->ext, lin.data(), lin.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    b->ext_on = true;
     return FFH_OK;
 }
 

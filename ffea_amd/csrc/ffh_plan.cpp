@@ -183,9 +183,11 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         P.mid_index.clear();
         P.mid_a.clear();
         P.mid_b.clear();
+        P.emid.resize(6 * (size_t)E);
         for (int e = 0; e < E; e++)
             for (int m = 0; m < 6; m++) {
                 int v = topo[10 * e + 4 + m];
+                P.emid[6 * (size_t)e + m] = v;
                 if (seen[v]) continue;
                 seen[v] = 1;
                 P.mid_index.push_back(v);

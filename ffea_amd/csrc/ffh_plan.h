@@ -61,6 +61,7 @@ struct Plan {
     std::vector<uint32_t> rng_node_off;  // [N] 3*original index (offset inside the node draws)
     // midpoint reconstruction for N_t output: for each midpoint, its two edge ends (original ids)
     std::vector<int32_t> mid_index, mid_a, mid_b;
+    std::vector<int32_t> emid;     // [6*E] midpoint node ids of each element (original numbering)
     // force gather: per linear node, contributions (e << 2 | a) in ascending element order
     std::vector<uint32_t> nctb_off, nctb;
     Tiles ntile;  // the same lists, tiled per 64 nodes

@@ -133,6 +133,18 @@ int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *a
 int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_unroll,
                         int spmv_waves_per_slice);
 
+/*
+ * Forces from outside the element loop -- surface (face) forces of interactions, springs,
+ * constant forces -- as computed by the caller: [3*N_t] host array in meso units, or NULL to
+ * clear.  They are added to the force vector of every following position phase after the
+ * element forces and before the Stokes noise, as Blob::aggregate_forces_and_solve adds face
+ * forces (src/Blob.cpp:3842-3851); forces on edge midpoints are moved to the edge ends as
+ * linearise_force does (src/Blob.cpp:1066-1085), and pinned rows stay zero.  The caller
+ * refreshes them as often as its interaction model needs.  Additions are grouped as one
+ * pre-summed term per node, so the force vector can differ from the reference's in the last bit.
+ */
+int ffh_blob_set_external_forces(ffh_blob *b, const double *force);
+
 /* ffh_blob_step replays one captured hipGraph per step (element, assembly, force gather and
  * the CG iterations of the last solve's count; the host polls once per step) unless disabled
  * here, or the learned preconditioner, harvesting or profiling is on.  Results are the same

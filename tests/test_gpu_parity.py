@@ -245,3 +245,50 @@ def test_tuned_launch_shapes_give_identical_results(precision, batch):
             ref = out
         else:
             assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]) and out[2] == ref[2], shape
+
+
+def test_uniform_external_force_moves_blob_at_stokes_velocity():
+    """A uniform external force F on every vertex of a free blob at rest, noise off: the
+    viscosity matrix annihilates rigid translation and the rest state has no elastic force, so
+    every vertex moves at v = F / zeta (Stokes' law, zeta = 6 pi eta r), as a reference run
+    with the same face forces would."""
+    m, prm, sc, x0 = box_case((4, 3, 3), noise=0)
+    b = gpu_blob(m, prm, sc, np.arange(1, 7, dtype=np.uint64), m.pos)
+    nl = m.num_linear
+    F = np.zeros((m.num_nodes, 3))
+    F[:nl] = [1e-3, -2e-3, 5e-4]
+    b.set_external_forces(F)
+    b.step(1)
+    zeta = 6 * np.pi * prm.stokes_visc * m.stokes_radius[:nl] * sc
+    v = b.velocities()[:nl]
+    assert rel(v, F[:nl] / zeta[:, None]) < 1e-5
+    p = b.positions()[:nl]
+    # a displacement of ~1e-7 read off coordinates of ~20: ~1e-8 relative is the rounding floor
+    assert rel(p - m.pos[:nl] * sc, v * prm.dt) < 1e-7
+    b.set_external_forces(None)
+    b.step(1)
+    assert np.abs(b.velocities()[:nl]).max() < 1e-3 * np.abs(v).max()
+
+
+def test_midpoint_external_force_goes_to_edge_ends():
+    """A force on one edge midpoint reaches the solver as half of it on each end of the edge
+    (linearise_force, src/Blob.cpp:1066-1085), nowhere else."""
+    from ffea_amd.mesh import EDGES
+    m, prm, sc, x0 = box_case((3, 3, 2), noise=0)
+    seed = np.arange(1, 7, dtype=np.uint64)
+    a = gpu_blob(m, prm, sc, seed, x0)
+    b = gpu_blob(m, prm, sc, seed, x0)
+    e, slot = 7, 3  # element 7, its edge (1, 2)
+    mid = m.topo[e, 4 + slot]
+    ends = [m.topo[e, EDGES[slot][0]], m.topo[e, EDGES[slot][1]]]
+    F = np.zeros((m.num_nodes, 3))
+    F[mid] = [2e-3, -1e-3, 4e-3]
+    b.set_external_forces(F)
+    a.step(1)
+    b.step(1)
+    d = b.rhs() - a.rhs()
+    scale = np.abs(a.rhs()).max()
+    for n in ends:
+        assert np.allclose(d[n], 0.5 * F[mid], rtol=0, atol=1e-12 * scale)
+    others = np.setdiff1d(np.arange(m.num_nodes), ends)
+    assert np.abs(d[others]).max() <= 1e-12 * scale
