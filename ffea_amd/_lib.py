@@ -100,8 +100,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_steps_done.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.ffh_blob_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.ffh_blob_set_external_forces.argtypes = [C.c_void_p, C.c_void_p]
-    L.ffh_blob_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
-    L.ffh_blob_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.ffh_blob_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 5
+    L.ffh_blob_set_tuning.argtypes = [C.c_void_p] + [C.c_int] * 5
     L.ffh_blob_destroy.argtypes = [C.c_void_p]
     L.ffh_blob_destroy.restype = None
     L.ffh_blob_update_internal_forces.argtypes = [C.c_void_p, C.POINTER(C.c_int)]

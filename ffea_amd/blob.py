@@ -96,17 +96,19 @@ class Blob:
     def step(self, nsteps: int = 1) -> None:
         _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
 
-    TUNING_KEYS = ("element_block", "assembly_block", "assembly_unroll", "spmv_waves_per_slice")
+    TUNING_KEYS = ("element_block", "assembly_block", "assembly_unroll", "spmv_waves_per_slice",
+                   "spmv_nontemporal")
 
     def tuning(self) -> dict:
-        v = [C.c_int() for _ in range(4)]
+        v = [C.c_int() for _ in range(5)]
         _lib.check(self._L.ffh_blob_tuning(self._h, *[C.byref(x) for x in v]))
         return dict(zip(self.TUNING_KEYS, (x.value for x in v)))
 
     def set_tuning(self, element_block: int, assembly_block: int, assembly_unroll: int,
-                   spmv_waves_per_slice: int) -> None:
+                   spmv_waves_per_slice: int, spmv_nontemporal: int = 0) -> None:
         _lib.check(self._L.ffh_blob_set_tuning(self._h, int(element_block), int(assembly_block),
-                                               int(assembly_unroll), int(spmv_waves_per_slice)))
+                                               int(assembly_unroll), int(spmv_waves_per_slice),
+                                               int(spmv_nontemporal)))
 
     def set_external_forces(self, force_meso=None) -> None:
         """Forces added to every following position phase ([N_t, 3] meso units; None clears)."""

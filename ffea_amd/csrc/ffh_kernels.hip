@@ -220,6 +220,17 @@ __global__ __launch_bounds__(1024) void k_blob_totals(TotArgs t) {
         }
 }
 
+// Streamed-once loads of the SpMV (matrix values, column indices).  Non-temporal (kNt) they do
+// not push the gathered vector out of the XCD's L2 -- a gain when the matrix streams from HBM,
+// a loss when it fits the Infinity Cache and would be re-read from there; the tuner decides.
+template <bool kNt, typename X>
+__device__ __forceinline__ X stream_load(const X *p) {
+    if constexpr (kNt)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The gathering
 // kernels remap the dispatch index so that XCD x works on the contiguous logical blocks
 // [x per, (x+1) per): neighbouring rows, elements and records then share one L2.  The grid is
@@ -752,7 +763,7 @@ struct SpmvArgs {
 // per blob: the blob's slices are a contiguous range of partials.
 // mode 0: Jacobi-PCG (Fletcher-Reeves beta); 1: flexible CG with the learned preconditioner
 // (beta = z.(r - r_prev) / delta_prev, Polak-Ribiere form); 2: restart (beta = 0: p = z).
-template <typename T>
+template <typename T, bool kNt>
 __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
     __shared__ double sh[4];
     const int slice = xcd_block(v.nslices);
@@ -806,9 +817,9 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
 #pragma unroll 4
     for (int t = 0; t < w; t++) {
         const int64_t tile = base + (int64_t)t * 64;
-        const int c = m.col[tile + lane];
+        const int c = stream_load<kNt>(m.col + tile + lane);
         const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
-        const T v0 = vt[0], v1 = vt[64], v2 = vt[128];
+        const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
         acc += (double)v0 * z[3 * c + 0];
         acc += (double)v1 * z[3 * c + 1];
         acc += (double)v2 * z[3 * c + 2];
@@ -844,7 +855,7 @@ __device__ __forceinline__ double blob_total_as192(const double *parts, const do
 // the column index and the gathered z block are loaded once instead of three times.  Row sums,
 // reductions and the partial of p.q are formed in k_cg_spmv's exact order, so both variants
 // give bit-identical results (the tuner may pick either).
-template <typename T>
+template <typename T, bool kNt>
 __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, int mode) {
     const int slice = xcd_block(v.nslices);
     if (slice >= v.nslices) return;
@@ -897,18 +908,21 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
 #pragma unroll 2
     for (int t = 0; t < w; t++) {
         const int64_t tile = base + (int64_t)t * 64;
-        const int c = m.col[tile + lane];
+        const int c = stream_load<kNt>(m.col + tile + lane);
         const T *vt = val + tile * 9 + lane;
+        T v[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = stream_load<kNt>(vt + 64 * k);
         const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-        a0 += (double)vt[0] * z0;
-        a0 += (double)vt[64] * z1;
-        a0 += (double)vt[128] * z2;
-        a1 += (double)vt[192] * z0;
-        a1 += (double)vt[256] * z1;
-        a1 += (double)vt[320] * z2;
-        a2 += (double)vt[384] * z0;
-        a2 += (double)vt[448] * z1;
-        a2 += (double)vt[512] * z2;
+        a0 += (double)v[0] * z0;
+        a0 += (double)v[1] * z1;
+        a0 += (double)v[2] * z2;
+        a1 += (double)v[3] * z0;
+        a1 += (double)v[4] * z1;
+        a1 += (double)v[5] * z2;
+        a2 += (double)v[6] * z0;
+        a2 += (double)v[7] * z1;
+        a2 += (double)v[8] * z2;
     }
     const size_t k = 3 * (size_t)row;
     const double acc[3] = {a0, a1, a2};
@@ -1087,6 +1101,7 @@ struct ffh_blob {
     bool ext_on = false;
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
     int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
+    bool spmv_nt = false;  // non-temporal matrix loads in the SpMV
     double *tot = nullptr;
     CgCtl *ctl = nullptr;       // [nblobs]; ctl[0].inversions counts the whole batch
     CgCtl *ctl_host = nullptr;  // pinned host mirror
@@ -1360,22 +1375,19 @@ struct ffh_blob {
     int launch_spmv(int it, int mode) {
         CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
-        int rc;
-        const bool one = spmv_waves == 1;
-        if (precision == FFH_PRECISION_FP64)
-            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) {
-                if (one)
-                    hipExtLaunchKernelGGL((k_cg_spmv1<double>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode);
-                else
-                    hipExtLaunchKernelGGL((k_cg_spmv<double>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode);
-            });
-        else
-            rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) {
-                if (one)
-                    hipExtLaunchKernelGGL((k_cg_spmv1<float>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode);
-                else
-                    hipExtLaunchKernelGGL((k_cg_spmv<float>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode);
-            });
+        int rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) {
+            const int sel = (spmv_waves == 1 ? 1 : 0) | (spmv_nt ? 2 : 0) | (precision == FFH_PRECISION_FP64 ? 4 : 0);
+            switch (sel) {
+            case 0: hipExtLaunchKernelGGL((k_cg_spmv<float, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 1: hipExtLaunchKernelGGL((k_cg_spmv1<float, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 2: hipExtLaunchKernelGGL((k_cg_spmv<float, true>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 3: hipExtLaunchKernelGGL((k_cg_spmv1<float, true>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 4: hipExtLaunchKernelGGL((k_cg_spmv<double, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 5: hipExtLaunchKernelGGL((k_cg_spmv1<double, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 6: hipExtLaunchKernelGGL((k_cg_spmv<double, true>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            default: hipExtLaunchKernelGGL((k_cg_spmv1<double, true>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            }
+        });
         if (rc) return rc;
         return totals(1 << kTotPQ);
     }
@@ -1720,18 +1732,22 @@ struct ffh_blob {
         HIP_OR_FAIL(hipMemcpyAsync(tmp_in, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         best = 1e30f;
         pick = spmv_waves;
-        const int waves[2] = {3, 1};
-        for (int wv : waves) {
+        bool pick_nt = spmv_nt;
+        const int waves[4] = {3, 1, 3, 1};
+        for (int vi = 0; vi < 4; vi++) {
+            const int wv = waves[vi];
             spmv_waves = wv;
+            spmv_nt = vi >= 2;
             auto solve = [&] {
                 int r1 = run_rhs(tmp_in);
                 int r2 = r1 ? r1 : run_cg();
                 return r2 == FFH_E_CG ? FFH_OK : r2;
             };
             if ((rc = time_ms(solve, 2, ms))) return rc;
-            if (ms < best) best = ms, pick = wv;
+            if (ms < best) best = ms, pick = wv, pick_nt = spmv_nt;
         }
         spmv_waves = pick;
+        spmv_nt = pick_nt;
         plain_count = -1;
         last_iters = 0;
         last_res = 0;
@@ -2121,17 +2137,18 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
 }
 
 int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_unroll,
-                    int *spmv_waves_per_slice) {
+                    int *spmv_waves_per_slice, int *spmv_nontemporal) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     if (element_block) *element_block = b->elem_block;
     if (assembly_block) *assembly_block = b->asm_block;
     if (assembly_unroll) *assembly_unroll = b->asm_unroll;
     if (spmv_waves_per_slice) *spmv_waves_per_slice = b->spmv_waves;
+    if (spmv_nontemporal) *spmv_nontemporal = b->spmv_nt ? 1 : 0;
     return FFH_OK;
 }
 
 int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_unroll,
-                        int spmv_waves_per_slice) {
+                        int spmv_waves_per_slice, int spmv_nontemporal) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
     if (!ok(element_block) || !ok(assembly_block) || (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3) ||
@@ -2142,6 +2159,7 @@ int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int 
     b->asm_block = assembly_block;
     b->asm_unroll = assembly_unroll;
     b->spmv_waves = spmv_waves_per_slice;
+    b->spmv_nt = spmv_nontemporal != 0;
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
     b->graphs.clear();
     return FFH_OK;

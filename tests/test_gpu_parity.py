@@ -235,7 +235,8 @@ def test_tuned_launch_shapes_give_identical_results(precision, batch):
         def make():
             return gpu_blob(m, prm, sc, seed, x0, precision)
     ref = None
-    for shape in ((256, 256, 2, 3), (64, 128, 1, 1), (128, 64, 4, 3), (256, 64, 4, 1), (128, 256, 0, 1)):
+    for shape in ((256, 256, 2, 3, 0), (64, 128, 1, 1, 1), (128, 64, 4, 3, 1), (256, 64, 4, 1, 0),
+                  (128, 256, 0, 1, 1)):
         b = make()
         b.set_tuning(*shape)
         assert b.tuning() == dict(zip(b.TUNING_KEYS, shape))
