@@ -421,18 +421,25 @@ def parity_vs_reference(cfg, ms, prm, sc, xs, steps):
     return out
 
 
-KERNEL_SYMBOL = {"element": "k_element<{t}, {t}>", "assemble": "k_assemble<{t}>", "rhs": "k_rhs<{t}>",
-                 "cg_spmv": "k_cg_spmv<{t}>", "cg_update": "k_cg_update", "integrate": "k_integrate"}
+def kernel_symbol(kernel, precision, tuning):
+    """The kernel's name in rocprofv3 records for the launch shapes the tuner picked."""
+    t = "double" if precision == "fp64" else "float"
+    u = tuning.get("assembly_unroll", 2)
+    nt = "true" if tuning.get("spmv_nontemporal") else "false"
+    spmv = "k_cg_spmv1" if tuning.get("spmv_waves_per_slice") == 1 else "k_cg_spmv"
+    return {"element": f"k_element<{t}, {t}>",
+            "assemble": f"k_assemble2<{t}>" if u == 0 else f"k_assemble<{t}, {u}>",
+            "rhs": f"k_rhs<{t}>", "cg_spmv": f"{spmv}<{t}, {nt}>", "cg_update": "k_cg_update",
+            "integrate": "k_integrate"}.get(kernel)
 
 
-def measured_traffic(cfg, kernel, precision):
+def measured_traffic(cfg, kernel, precision, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (profiles/traffic_<cfg>.json, copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{cfg}.json")
     if not os.path.exists(path):
         return None
-    t = "double" if precision == "fp64" else "float"
-    rec = json.load(open(path))["kernels"].get(KERNEL_SYMBOL[kernel].format(t=t))
+    rec = json.load(open(path))["kernels"].get(kernel_symbol(kernel, precision, tuning))
     return None if rec is None else rec["traffic_bytes"]
 
 
@@ -584,7 +591,7 @@ def main():
     d = ktab[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": measured_traffic(a.config, dom, "fp64"),
+            "traffic": measured_traffic(a.config, dom, "fp64", sizes.get("tuning", {})),
             "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]}
     cpu = None if a.no_cpu else cpu_baseline(a.config, ms, prm, sc, xs, a.cpu_seconds)
     parity = None
