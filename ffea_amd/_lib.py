@@ -38,10 +38,13 @@ EXPORTS = (
     "ffh_mpnn_set_weights", "ffh_mpnn_get_weights", "ffh_blob_set_preconditioner",
     "ffh_mpnn_apply", "ffh_mpnn_harvest", "ffh_mpnn_harvest_count", "ffh_mpnn_train",
     "ffh_mpnn_loss_grad", "ffh_blob_steps_done", "ffh_blob_set_graphs", "ffh_blob_tuning",
-    "ffh_blob_set_tuning", "ffh_blob_set_external_forces",
+    "ffh_blob_set_tuning", "ffh_blob_set_external_forces", "ffh_blob_set_solver",
+    "ffh_blob_set_velocities", "ffh_blob_kinetic_energy",
 )
 FFH_PRECOND_JACOBI = 0
 FFH_PRECOND_LEARNED = 1
+# solver codes of include/FFEA_return_codes.h:30-33, by .ffea "solver" value
+FFH_SOLVERS = {"CG": 1, "masslumped": 2, "CG_nomass": 3}
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -123,6 +126,9 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.ffh_blob_kernel_times.argtypes = [C.c_void_p, _dp, _i64p]
     L.ffh_blob_strain_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.ffh_blob_set_solver.argtypes = [C.c_void_p, C.c_int]
+    L.ffh_blob_set_velocities.argtypes = [C.c_void_p, _dp]
+    L.ffh_blob_kinetic_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     for n in EXPORTS:
         if n not in ("ffh_last_error", "ffh_version", "ffh_blob_create", "ffh_blob_destroy",
                      "ffh_blob_stream", "ffh_batch_create"):

@@ -63,6 +63,7 @@ class Blob:
         if not self._h:
             raise _lib.FFHError(-1, _lib.last_error())
         self.precision = precision
+        self.solver = "CG_nomass"
         if pos is not None:
             self.set_positions(np.asarray(pos, np.float64) * scale)
 
@@ -129,6 +130,14 @@ class Blob:
         return n.value
 
     # -- solver interface -------------------------------------------------------------
+    def set_solver(self, name: str) -> None:
+        """The Blob's linear solver by its .ffea name (src/World.cpp:2545-2560): "CG_nomass"
+        (default), "CG" (mass matrix, ConjugateGradientSolver) or "masslumped"."""
+        if name not in _lib.FFH_SOLVERS:
+            raise ValueError(f"solver {name!r}: one of {sorted(_lib.FFH_SOLVERS)}")
+        _lib.check(self._L.ffh_blob_set_solver(self._h, _lib.FFH_SOLVERS[name]))
+        self.solver = name
+
     def solve(self, force) -> np.ndarray:
         x = np.ascontiguousarray(force, np.float64).copy().ravel()
         _lib.check(self._L.ffh_blob_solve(self._h, x))
@@ -144,6 +153,10 @@ class Blob:
     def set_positions(self, pos_meso) -> None:
         _lib.check(self._L.ffh_blob_set_positions(
             self._h, np.ascontiguousarray(pos_meso, np.float64).ravel()))
+
+    def set_velocities(self, vel_meso) -> None:
+        _lib.check(self._L.ffh_blob_set_velocities(
+            self._h, np.ascontiguousarray(vel_meso, np.float64).ravel()))
 
     def _get(self, fn) -> np.ndarray:
         out = np.zeros(3 * self.num_nodes, np.float64)
@@ -196,6 +209,11 @@ class Blob:
     def strain_energy(self) -> float:
         e = C.c_double(0)
         _lib.check(self._L.ffh_blob_strain_energy(self._h, C.byref(e)))
+        return e.value
+
+    def kinetic_energy(self) -> float:
+        e = C.c_double(0)
+        _lib.check(self._L.ffh_blob_kinetic_energy(self._h, C.byref(e)))
         return e.value
 
 
@@ -285,6 +303,7 @@ class Batch(Blob):
         if not self._h:
             raise _lib.FFHError(-1, _lib.last_error())
         self.precision = precision
+        self.solver = "CG_nomass"
 
     def split(self, arr) -> list:
         """Cut a concatenated per-node array ([N_t,...] or [3 N_t]) into per-blob pieces."""

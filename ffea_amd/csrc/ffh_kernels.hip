@@ -266,6 +266,7 @@ struct ElemArgs {
     const Mat3u *jump;
     uint32_t elem_base;
     CgCtl *ctl;
+    const double *vel;  // [3N] node velocities (mass-ful solvers: the viscous force K_e u_e)
 };
 
 // Entry (R, C) of the element viscosity matrix (create_viscosity_matrix,
@@ -332,7 +333,7 @@ __device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos
     dpsi[8] = -(dpsi[9] + dpsi[10] + dpsi[11]);
 }
 
-template <typename R, typename T>
+template <typename R, typename T, bool kMass>
 __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned); waves stay 64-aligned
     const int e = xcd_block((g.E + bd - 1) / bd) * bd + threadIdx.x;
@@ -401,13 +402,37 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
         S[2][0] = S[0][2];
         S[2][1] = S[1][2];
     }
+    // The mass-ful solvers start du from the viscous force K_e u_e (src/Blob.cpp:703-706):
+    // get_element_velocity_vector (:454-469, x-major over the vertices) and mat12_apply
+    // (src/mat_vec_fns.cpp:26-34, each row summed in column order from zero); the massless
+    // solver starts it from zero.
+    R kv[12];
+    if constexpr (kMass) {
+        const int4 nn = g.enodes[e];
+        const int ni[4] = {nn.x, nn.y, nn.z, nn.w};
+        R u[12];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int a = 0; a < 4; a++) u[4 * i + a] = (R)g.vel[3 * (size_t)ni[a] + i];
+#pragma unroll
+        for (int rr = 0; rr < 12; rr++) {
+            R acc = (R)0;
+#pragma unroll
+            for (int cc = 0; cc < 12; cc++) acc += visc_entry<R>(dpsi, A, B, vol, rr, cc) * u[cc];
+            kv[rr] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) kv[k] = (R)0;
+    }
     // apply_stress_tensor (:441-449) + add_element_force_vector (:474-490)
     T fo[12];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-            R du = (R)0 + vol * (dpsi[a] * S[i][0] + dpsi[4 + a] * S[i][1] + dpsi[8 + a] * S[i][2]);
+            R du = kv[4 * i + a] + vol * (dpsi[a] * S[i][0] + dpsi[4 + a] * S[i][1] + dpsi[8 + a] * S[i][2]);
             fo[3 * a + i] = (T)((R)0 - du);
         }
     T *ef = reinterpret_cast<T *>(g.eforce);
@@ -677,6 +702,8 @@ struct RhsArgs {
     int noise;
     double kT, dt;
     const double *ext;  // external forces per linear node [3N], midpoints folded; or nullptr
+    const double *vel;     // mass-ful solvers with Stokes drag: node velocities [3N]; or nullptr
+    const double *lumped;  // MassLumpedSolver: inverse masses [N] (the solve is x = f / m); or nullptr
 };
 
 template <typename T>
@@ -715,6 +742,8 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         // surface / external forces come after the element forces and before the Stokes noise
         // (src/Blob.cpp:3842-3851, then :3895-3898)
         if (g.ext) f += g.ext[3 * (size_t)n + i];
+        // mass-ful solvers: Stokes drag on the node's velocity, before its noise (:3854-3877)
+        if (g.vel) f -= g.vel[3 * (size_t)n + i] * g.zeta[n];
         if (noise) {
             Stream6 st;
 #pragma unroll
@@ -724,16 +753,21 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         }
     }
     const size_t k = 3 * (size_t)n + i;
-    const double z = v.dinv[k] * f;
-    v.f[k] = f;
-    v.r[k] = f;
-    v.x[k] = 0.0;
-    v.p[k] = 0.0;
-    v.q[k] = 0.0;
-    v.z[k] = z;
-    block_partial_at(f * z, v.parts_rz, blk, sh);
-    block_partial_at(f * f, v.parts_ff, blk, sh);
-    if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
+    if (g.lumped) {  // MassLumpedSolver::solve (src/MassLumpedSolver.cpp:82-90)
+        v.f[k] = f;
+        v.x[k] = f * g.lumped[n];
+    } else {
+        const double z = v.dinv[k] * f;
+        v.f[k] = f;
+        v.r[k] = f;
+        v.x[k] = 0.0;
+        v.p[k] = 0.0;
+        v.q[k] = 0.0;
+        v.z[k] = z;
+        block_partial_at(f * z, v.parts_rz, blk, sh);
+        block_partial_at(f * f, v.parts_ff, blk, sh);
+        if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
+    }
     const int b = v.slice_blob[blk];
     if (blk == v.blob_slice_off[b] && threadIdx.x == 0) {
         CgCtl *c = &v.ctl[b];
@@ -763,7 +797,22 @@ struct SpmvArgs {
 // per blob: the blob's slices are a contiguous range of partials.
 // mode 0: Jacobi-PCG (Fletcher-Reeves beta); 1: flexible CG with the learned preconditioner
 // (beta = z.(r - r_prev) / delta_prev, Polak-Ribiere form); 2: restart (beta = 0: p = z).
-template <typename T, bool kNt>
+//
+// kMass: ConjugateGradientSolver on the mass matrix (src/ConjugateGradientSolver.cpp:205-230),
+// whose matrix holds one double per block (the same for the three components) and whose loop
+// tests ||r||^2/||f||^2 < eps^2 before each product -- first on r = f, where the ratio is 1 --
+// and fails after max_iterations_cg updates without testing the last one.
+template <bool kMass>
+__device__ __forceinline__ bool cg_stop_first(double f2, double eps2) {
+    return f2 == 0.0 || (kMass && 1.0 < eps2);
+}
+template <bool kMass>
+__device__ __forceinline__ bool cg_stop(double res, int it, const CgArgs &v, bool &ok) {
+    ok = kMass ? (it < v.maxit && res < v.eps2) : res < v.eps2;
+    return ok || it >= v.maxit;
+}
+
+template <typename T, bool kNt, bool kMass>
 __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
     __shared__ double sh[4];
     const int slice = xcd_block(v.nslices);
@@ -777,24 +826,26 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     if (it == 0) {
         const double f2 = blob_total(v.parts_ff, v.tot, kTotFF, b, s0, s1, sh);
         const double d0 = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
+        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
         if (decider) {
             ctl->f2 = f2;
             ctl->delta[0] = d0;
-            if (f2 == 0.0) {  // zero force: zero velocity
+            if (stop) {  // zero force: zero solution
                 ctl->done = 1;
                 ctl->status = 0;
                 ctl->iter = 0;
             }
         }
-        if (f2 == 0.0) return;
+        if (stop) return;
     } else {
         const double f2 = ctl->f2;
         const double r2 = blob_total(v.parts_rr, v.tot, kTotRR, b, s0, s1, sh);
         const double res = r2 / f2;
-        if (res < v.eps2 || it >= v.maxit) {
+        bool ok;
+        if (cg_stop<kMass>(res, it, v, ok)) {
             if (decider) {
                 ctl->done = 1;
-                ctl->status = res < v.eps2 ? 0 : -1;
+                ctl->status = ok ? 0 : -1;
                 ctl->iter = it;
                 ctl->res = res;
             }
@@ -814,15 +865,25 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     const T *val = reinterpret_cast<const T *>(m.val);
     const double *__restrict__ z = v.z;
     double acc = 0.0;
+    if constexpr (kMass) {
+        const double *mv = reinterpret_cast<const double *>(m.val);
 #pragma unroll 4
-    for (int t = 0; t < w; t++) {
-        const int64_t tile = base + (int64_t)t * 64;
-        const int c = stream_load<kNt>(m.col + tile + lane);
-        const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
-        const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
-        acc += (double)v0 * z[3 * c + 0];
-        acc += (double)v1 * z[3 * c + 1];
-        acc += (double)v2 * z[3 * c + 2];
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            acc += stream_load<kNt>(mv + tile + lane) * z[3 * c + ci];
+        }
+    } else {
+#pragma unroll 4
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
+            const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
+            acc += (double)v0 * z[3 * c + 0];
+            acc += (double)v1 * z[3 * c + 1];
+            acc += (double)v2 * z[3 * c + 2];
+        }
     }
     const size_t k = 3 * (size_t)row + ci;
     const double pn = z[k] + beta * v.p[k];
@@ -855,7 +916,7 @@ __device__ __forceinline__ double blob_total_as192(const double *parts, const do
 // the column index and the gathered z block are loaded once instead of three times.  Row sums,
 // reductions and the partial of p.q are formed in k_cg_spmv's exact order, so both variants
 // give bit-identical results (the tuner may pick either).
-template <typename T, bool kNt>
+template <typename T, bool kNt, bool kMass>
 __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, int mode) {
     const int slice = xcd_block(v.nslices);
     if (slice >= v.nslices) return;
@@ -868,24 +929,26 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
     if (it == 0) {
         const double f2 = blob_total_as192(v.parts_ff, v.tot, kTotFF, b, s0, s1);
         const double d0 = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
+        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
         if (decider) {
             ctl->f2 = f2;
             ctl->delta[0] = d0;
-            if (f2 == 0.0) {
+            if (stop) {
                 ctl->done = 1;
                 ctl->status = 0;
                 ctl->iter = 0;
             }
         }
-        if (f2 == 0.0) return;
+        if (stop) return;
     } else {
         const double f2 = ctl->f2;
         const double r2 = blob_total_as192(v.parts_rr, v.tot, kTotRR, b, s0, s1);
         const double res = r2 / f2;
-        if (res < v.eps2 || it >= v.maxit) {
+        bool ok;
+        if (cg_stop<kMass>(res, it, v, ok)) {
             if (decider) {
                 ctl->done = 1;
-                ctl->status = res < v.eps2 ? 0 : -1;
+                ctl->status = ok ? 0 : -1;
                 ctl->iter = it;
                 ctl->res = res;
             }
@@ -905,24 +968,37 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
     const T *val = reinterpret_cast<const T *>(m.val);
     const double *__restrict__ z = v.z;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if constexpr (kMass) {
+        const double *mv = reinterpret_cast<const double *>(m.val);
+#pragma unroll 4
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const double a = stream_load<kNt>(mv + tile + lane);
+            a0 += a * z[3 * c + 0];
+            a1 += a * z[3 * c + 1];
+            a2 += a * z[3 * c + 2];
+        }
+    } else {
 #pragma unroll 2
-    for (int t = 0; t < w; t++) {
-        const int64_t tile = base + (int64_t)t * 64;
-        const int c = stream_load<kNt>(m.col + tile + lane);
-        const T *vt = val + tile * 9 + lane;
-        T v[9];
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const T *vt = val + tile * 9 + lane;
+            T v[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) v[k] = stream_load<kNt>(vt + 64 * k);
-        const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-        a0 += (double)v[0] * z0;
-        a0 += (double)v[1] * z1;
-        a0 += (double)v[2] * z2;
-        a1 += (double)v[3] * z0;
-        a1 += (double)v[4] * z1;
-        a1 += (double)v[5] * z2;
-        a2 += (double)v[6] * z0;
-        a2 += (double)v[7] * z1;
-        a2 += (double)v[8] * z2;
+            for (int k = 0; k < 9; k++) v[k] = stream_load<kNt>(vt + 64 * k);
+            const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
+            a0 += (double)v[0] * z0;
+            a0 += (double)v[1] * z1;
+            a0 += (double)v[2] * z2;
+            a1 += (double)v[3] * z0;
+            a1 += (double)v[4] * z1;
+            a1 += (double)v[5] * z2;
+            a2 += (double)v[6] * z0;
+            a2 += (double)v[7] * z1;
+            a2 += (double)v[8] * z2;
+        }
     }
     const size_t k = 3 * (size_t)row;
     const double acc[3] = {a0, a1, a2};
@@ -965,10 +1041,12 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     }
 }
 
-// euler_integrate (src/Blob.cpp:3941-3953): vel = v; pos += v * dt.  Skipped after a CG
-// failure, as the reference skips it.  Thread 0 advances the stream by one step's draws.
+// euler_integrate (src/Blob.cpp:3941-3967): massless, vel = v and pos += v dt; mass-ful
+// (massful != 0), vel += a dt and pos += vel dt.  Skipped after a CG failure, as the reference
+// skips it.  Thread 0 advances the stream by one step's draws.
 __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, double dt,
-                                                     uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance) {
+                                                     uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance,
+                                                     int massful) {
     const int n = blockIdx.x * kBlock + threadIdx.x;
     if (n == 0 && advance) {
         uint32_t s[6];
@@ -978,7 +1056,12 @@ __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, do
         for (int k = 0; k < 6; k++) rng_state[k] = s[k];
     }
     if (n >= v.N || v.ctl[v.slice_blob[n >> 6]].status != 0) return;
-    const double x0 = v.x[3 * n], x1 = v.x[3 * n + 1], x2 = v.x[3 * n + 2];
+    double x0 = v.x[3 * n], x1 = v.x[3 * n + 1], x2 = v.x[3 * n + 2];
+    if (massful) {
+        x0 = v.vel[3 * n] + x0 * dt;
+        x1 = v.vel[3 * n + 1] + x1 * dt;
+        x2 = v.vel[3 * n + 2] + x2 * dt;
+    }
     v.vel[3 * n] = x0;
     v.vel[3 * n + 1] = x1;
     v.vel[3 * n + 2] = x2;
@@ -1055,6 +1138,32 @@ __global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *pa
     block_partial(part, parts, sh);
 }
 
+// Kinetic energy of the mass-ful solvers (Blob::make_measurements, src/Blob.cpp:1385-1407):
+// per element u_e . (M_e u_e) with apply_element_mass_matrix (src/tetra_element_linear.cpp:
+// 534-553); one partial per block.  The caller halves the sum.
+__global__ __launch_bounds__(kBlock) void k_kinetic_energy(ElemArgs g, const double *rho, double *parts) {
+    __shared__ double sh[4];
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double part = 0.0;
+    if (e < g.E) {
+        const int4 nn = g.enodes[e];
+        const int ni[4] = {nn.x, nn.y, nn.z, nn.w};
+        double u[12], mu[12];
+        for (int i = 0; i < 3; i++)
+            for (int a = 0; a < 4; a++) u[4 * i + a] = g.vel[3 * (size_t)ni[a] + i];
+        const double sc = g.vol0[e] * rho[e];
+        for (int k = 0; k < 12; k += 4)
+            for (int i = 0; i < 4; i++) {
+                double t = 0.0;
+                for (int j = 0; j < 4; j++) t += (i == j ? .1 : .05) * u[j + k];
+                mu[i + k] = t * sc;
+            }
+        part = u[0] * mu[0] + u[1] * mu[1] + u[2] * mu[2] + u[3] * mu[3] + u[4] * mu[4] + u[5] * mu[5] +
+               u[6] * mu[6] + u[7] * mu[7] + u[8] * mu[8] + u[9] * mu[9] + u[10] * mu[10] + u[11] * mu[11];
+    }
+    block_partial(part, parts, sh);
+}
+
 #include "ffh_mpnn.h"
 
 // ----------------------------------------------------------------------------------------
@@ -1099,6 +1208,11 @@ struct ffh_blob {
     bool use_tot = false;  // some blob has more than kTotSlicesMin slices
     double *ext = nullptr;  // external forces (ffh_blob_set_external_forces)
     bool ext_on = false;
+    // solver (ffh_blob_set_solver): mass matrix in the SELL slots + its inverse diagonal, or the
+    // lumped inverse masses; densities for the kinetic energy; midpoint velocities as given
+    int solver = FFH_SOLVER_CG_NOMASS;
+    double *mval = nullptr, *mdinv = nullptr, *minv = nullptr, *rho = nullptr;
+    std::vector<double> vel_mid;  // [3*N_t] or empty (all zero)
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
     int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
     bool spmv_nt = false;  // non-temporal matrix loads in the SpMV
@@ -1229,7 +1343,7 @@ struct ffh_blob {
         v.q = q;
         v.z = z;
         v.vel = vel;
-        v.dinv = dinv;
+        v.dinv = solver == FFH_SOLVER_CG ? mdinv : dinv;
         v.parts_rz = parts;
         v.parts_rr = parts + 1 * pstride;
         v.parts_ff = parts + 2 * pstride;
@@ -1258,9 +1372,11 @@ struct ffh_blob {
         m.slice_off = slice_off;
         m.slice_width = slice_width;
         m.col = slot_col;
-        m.val = val;
+        m.val = solver == FFH_SOLVER_CG ? (const void *)mval : (const void *)val;
         return m;
     }
+    bool massful() const { return solver != FFH_SOLVER_CG_NOMASS; }
+    bool lumped() const { return solver == FFH_SOLVER_MASSLUMPED; }
     ElemArgs elem_args() const {
         ElemArgs g;
         g.pos = pos;
@@ -1281,15 +1397,27 @@ struct ffh_blob {
         g.jump = jump;
         g.elem_base = 0;
         g.ctl = ctl;
+        g.vel = vel;
         return g;
     }
     static int grid(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
     int run_element() {
         ElemArgs g = elem_args();
-        if (precision == FFH_PRECISION_FP64)
-            return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<double, double>), xcd_grid(grid(E, elem_block)), elem_block, 0, stream, e0, e1, 0, g); });
-        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_element<float, float>), xcd_grid(grid(E, elem_block)), elem_block, 0, stream, e0, e1, 0, g); });
+        const int gr = xcd_grid(grid(E, elem_block));
+        return launch(0, [&](hipEvent_t e0, hipEvent_t e1) {
+            if (precision == FFH_PRECISION_FP64) {
+                if (massful())
+                    hipExtLaunchKernelGGL((k_element<double, double, true>), gr, elem_block, 0, stream, e0, e1, 0, g);
+                else
+                    hipExtLaunchKernelGGL((k_element<double, double, false>), gr, elem_block, 0, stream, e0, e1, 0, g);
+            } else {
+                if (massful())
+                    hipExtLaunchKernelGGL((k_element<float, float, true>), gr, elem_block, 0, stream, e0, e1, 0, g);
+                else
+                    hipExtLaunchKernelGGL((k_element<float, float, false>), gr, elem_block, 0, stream, e0, e1, 0, g);
+            }
+        });
     }
     int run_assemble() {
         mp_edges = false;
@@ -1358,6 +1486,8 @@ struct ffh_blob {
         g.kT = kT;
         g.dt = dt;
         g.ext = (ext_on && !f_in) ? ext : nullptr;
+        g.vel = (massful() && calc_stokes == 1 && !f_in) ? vel : nullptr;
+        g.lumped = lumped() ? minv : nullptr;
         return g;
     }
     int run_rhs(const double *f_in = nullptr) {
@@ -1369,23 +1499,29 @@ struct ffh_blob {
             rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
         else
             rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
-        if (rc) return rc;
+        if (rc || lumped()) return rc;
         return totals((1 << kTotFF) | (1 << kTotRZ) | (v.parts_uu ? 1 << kTotUU : 0));
     }
     int launch_spmv(int it, int mode) {
         CgArgs v = cg_args();
         SpmvArgs m = spmv_args();
         int rc = launch(3, [&](hipEvent_t e0, hipEvent_t e1) {
-            const int sel = (spmv_waves == 1 ? 1 : 0) | (spmv_nt ? 2 : 0) | (precision == FFH_PRECISION_FP64 ? 4 : 0);
+            const int sel = (spmv_waves == 1 ? 1 : 0) | (spmv_nt ? 2 : 0) | (precision == FFH_PRECISION_FP64 ? 4 : 0) |
+                            (solver == FFH_SOLVER_CG ? 8 : 0);
             switch (sel) {
-            case 0: hipExtLaunchKernelGGL((k_cg_spmv<float, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            case 1: hipExtLaunchKernelGGL((k_cg_spmv1<float, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            case 2: hipExtLaunchKernelGGL((k_cg_spmv<float, true>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            case 3: hipExtLaunchKernelGGL((k_cg_spmv1<float, true>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            case 4: hipExtLaunchKernelGGL((k_cg_spmv<double, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            case 5: hipExtLaunchKernelGGL((k_cg_spmv1<double, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            case 6: hipExtLaunchKernelGGL((k_cg_spmv<double, true>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
-            default: hipExtLaunchKernelGGL((k_cg_spmv1<double, true>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 0: hipExtLaunchKernelGGL((k_cg_spmv<float, false, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 1: hipExtLaunchKernelGGL((k_cg_spmv1<float, false, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 2: hipExtLaunchKernelGGL((k_cg_spmv<float, true, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 3: hipExtLaunchKernelGGL((k_cg_spmv1<float, true, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 4: hipExtLaunchKernelGGL((k_cg_spmv<double, false, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 5: hipExtLaunchKernelGGL((k_cg_spmv1<double, false, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 6: hipExtLaunchKernelGGL((k_cg_spmv<double, true, false>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 7: hipExtLaunchKernelGGL((k_cg_spmv1<double, true, false>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            // the mass matrix is FP64 in both precision modes
+            case 8: case 12: hipExtLaunchKernelGGL((k_cg_spmv<double, false, true>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 9: case 13: hipExtLaunchKernelGGL((k_cg_spmv1<double, false, true>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            case 10: case 14: hipExtLaunchKernelGGL((k_cg_spmv<double, true, true>), P_slice, 192, 0, stream, e0, e1, 0, m, v, it, mode); break;
+            default: hipExtLaunchKernelGGL((k_cg_spmv1<double, true, true>), P_slice, 64, 0, stream, e0, e1, 0, m, v, it, mode); break;
             }
         });
         if (rc) return rc;
@@ -1412,7 +1548,9 @@ struct ffh_blob {
         return launch(11, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_blob_totals, nblobs, 1024, 0, stream, e0, e1, 0, t); });
     }
     // ---- learned preconditioner ----
-    bool learned_solve() const { return precond == FFH_PRECOND_LEARNED && mp_ready && plain_count >= 0; }
+    bool learned_solve() const {
+        return precond == FFH_PRECOND_LEARNED && mp_ready && plain_count >= 0 && !massful();
+    }
     int mp_alloc() {
         if (mp_alloced) return FFH_OK;
         const size_t n = (size_t)N;
@@ -1589,7 +1727,7 @@ struct ffh_blob {
         c.learned = learned_solve();
         c.F = c.learned ? std::max(1, fallback_iters > 0 ? fallback_iters : plain_count) : 0;
         c.snap0 = hv_count;
-        c.harvesting = hv_on && !c.learned;
+        c.harvesting = hv_on && !c.learned && !massful();
         c.it = 0;
         return c;
     }
@@ -1640,6 +1778,21 @@ struct ffh_blob {
         if ((rc = cg_begin(c))) return rc;
         return cg_run_rest(c, false, false);
     }
+    // MassLumpedSolver: the force kernel has already formed x = f / m.  Reads the control
+    // blocks back (inversion count) as a CG poll would.
+    int lumped_finish() {
+        bool all_done = false;
+        int rc;
+        if ((rc = cg_poll(all_done))) return rc;
+        last_iters = 0;
+        last_res = 0;
+        std::fill(blob_iters.begin(), blob_iters.end(), 0);
+        std::fill(blob_res.begin(), blob_res.end(), 0.0);
+        std::fill(blob_status.begin(), blob_status.end(), 0);
+        return FFH_OK;
+    }
+    // The blob's Solver::solve after run_rhs.
+    int run_solve() { return lumped() ? lumped_finish() : run_cg(); }
     int cg_finish(CgRun &c, bool all_done) {
         const bool learned = c.learned, harvesting = c.harvesting;
         const int snap0 = c.snap0;
@@ -1680,7 +1833,8 @@ struct ffh_blob {
         int adv = plan.draws_per_step > 0 ? 1 : 0;
         Mat3u j1 = plan.step_jump[0], j2 = plan.step_jump[1];
         return launch(5, [&](hipEvent_t e0, hipEvent_t e1) {
-            hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, pos, dt, rng_state, j1, j2, adv);
+            hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, pos, dt, rng_state, j1, j2, adv,
+                                  massful() ? 1 : 0);
         });
     }
     // ---- autotuning of launch shapes (results do not depend on them) ----
@@ -1788,11 +1942,11 @@ struct ffh_blob {
         HIP_OR_FAIL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int rc = reset_inversions();
         if (!rc) rc = run_element();
-        if (!rc) rc = run_assemble();
+        if (!rc && !massful()) rc = run_assemble();
         if (!rc) rc = run_rhs();
         CgRun c = cg_setup();
-        if (!rc) rc = cg_begin(c);
-        if (!rc) rc = cg_iterations(c, K);
+        if (!rc && !lumped()) rc = cg_begin(c);
+        if (!rc && !lumped()) rc = cg_iterations(c, K);
         hipError_t e = hipStreamEndCapture(stream, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
@@ -1819,12 +1973,12 @@ struct ffh_blob {
         int rc;
         if ((rc = reset_inversions())) return rc;
         if ((rc = run_element())) return rc;
-        if ((rc = run_assemble())) return rc;
+        if (!massful() && (rc = run_assemble())) return rc;  // mass-ful solvers use no viscosity matrix
         if ((rc = run_rhs())) return rc;
-        return run_cg();  // synchronises: the inversion count is then final too
+        return run_solve();  // synchronises: the inversion count is then final too
     }
     int step_graph() {
-        const int K = std::max(1, std::min(cg_chunk, maxit));
+        const int K = lumped() ? 0 : std::max(1, std::min(cg_chunk, maxit));
         hipGraphExec_t ge = nullptr;
         int rc;
         if (graph_for(K, ge) != FFH_OK) {  // capture not available: ordinary launches from now on
@@ -1833,6 +1987,7 @@ struct ffh_blob {
             return step_eager();
         }
         HIP_OR_FAIL(hipGraphLaunch(ge, stream));
+        if (lumped()) return lumped_finish();
         CgRun c = cg_setup();
         c.it = K;
         bool all_done = false;
@@ -2111,7 +2266,7 @@ int ffh_blob_update_internal_forces(ffh_blob *b, int *num_inversions) {
     int rc;
     if ((rc = b->reset_inversions())) return rc;
     if ((rc = b->run_element())) return rc;
-    if ((rc = b->run_assemble())) return rc;
+    if (!b->massful() && (rc = b->run_assemble())) return rc;
     return b->read_inversions(num_inversions);
 }
 
@@ -2119,7 +2274,7 @@ int ffh_blob_update_positions(ffh_blob *b) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     int rc;
     if ((rc = b->run_rhs())) return rc;
-    int cg = b->run_cg();
+    int cg = b->run_solve();
     if ((rc = b->run_integrate())) return rc;
     b->steps_done++;
     return cg;
@@ -2240,7 +2395,7 @@ int ffh_blob_solve(ffh_blob *b, double *xio) {
             for (int c = 0; c < 3; c++) fl[3 * i + c] = xio[3 * (size_t)P.orig_of[i] + c];
     HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     if (int rc0 = b->run_rhs(b->tmp_in)) return rc0;
-    int rc = b->run_cg();
+    int rc = b->run_solve();
     std::vector<double> xl(fl.size());
     HIP_OR_FAIL(hipMemcpyAsync(xl.data(), b->x, xl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
@@ -2252,6 +2407,7 @@ int ffh_blob_solve(ffh_blob *b, double *xio) {
 
 int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     if (!b || !in || !result) return fail(FFH_E_ARG, "null argument");
+    if (b->massful()) return fail(FFH_E_ARG, "apply_matrix: the viscosity matrix of the massless solver only");
     const ffh::Plan &P = b->plan;
     std::vector<double> il(3 * (size_t)b->N, 0.0), ol(3 * (size_t)b->N);
     for (int i = 0; i < b->N; i++)
@@ -2326,7 +2482,78 @@ static int get_node_vector(ffh_blob *b, const double *dev, double *out) {
 
 int ffh_blob_get_velocities(ffh_blob *b, double *vel) {
     if (!b || !vel) return fail(FFH_E_ARG, "null argument");
-    return get_node_vector(b, b->vel, vel);
+    int rc = get_node_vector(b, b->vel, vel);
+    if (rc || b->vel_mid.empty() || !b->massful()) return rc;
+    for (int m : b->plan.mid_index)
+        for (int c = 0; c < 3; c++) vel[3 * (size_t)m + c] = b->vel_mid[3 * (size_t)m + c];
+    return FFH_OK;
+}
+
+int ffh_blob_set_velocities(ffh_blob *b, const double *vel) {
+    if (!b || !vel) return fail(FFH_E_ARG, "null argument");
+    const ffh::Plan &P = b->plan;
+    std::vector<double> h(3 * (size_t)b->N, 0.0);
+    for (int i = 0; i < b->N; i++)
+        if (P.orig_of[i] >= 0)
+            for (int c = 0; c < 3; c++) h[3 * (size_t)i + c] = vel[3 * (size_t)P.orig_of[i] + c];
+    b->vel_mid.assign(3 * (size_t)P.num_nodes_total, 0.0);
+    bool any = false;
+    for (int m : P.mid_index)
+        for (int c = 0; c < 3; c++) any |= (b->vel_mid[3 * (size_t)m + c] = vel[3 * (size_t)m + c]) != 0.0;
+    if (!any) b->vel_mid.clear();
+    HIP_OR_FAIL(hipMemcpyAsync(b->vel, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
+}
+
+int ffh_blob_set_solver(ffh_blob *b, int solver) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (solver == 0) return fail(FFH_E_ARG, "the direct solver (FFEA_DIRECT_SOLVER) is not on this path");
+    if (solver != FFH_SOLVER_CG && solver != FFH_SOLVER_MASSLUMPED && solver != FFH_SOLVER_CG_NOMASS)
+        return fail(FFH_E_ARG, "unknown solver " + std::to_string(solver));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    int rc = FFH_OK;
+    if (solver != FFH_SOLVER_CG_NOMASS && !b->rho) rc = b->upload(b->rho, b->plan.matRho);
+    if (!rc && solver == FFH_SOLVER_CG && !b->mval) {
+        std::vector<double> val, d3, inv;
+        std::string msg = ffh::build_mass(b->plan, 1, val, d3, inv);
+        if (!msg.empty()) return fail(FFH_E_ARG, msg);
+        rc = b->upload(b->mval, val);
+        if (!rc) rc = b->upload(b->mdinv, d3);
+    }
+    if (!rc && solver == FFH_SOLVER_MASSLUMPED && !b->minv) {
+        std::vector<double> val, d3, inv;
+        std::string msg = ffh::build_mass(b->plan, 2, val, d3, inv);
+        if (!msg.empty()) return fail(FFH_E_ARG, msg);
+        rc = b->upload(b->minv, inv);
+    }
+    if (rc) return rc;
+    if (solver != b->solver) {
+        for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the step's kernels change
+        b->graphs.clear();
+        b->solver = solver;
+        b->plain_count = -1;
+        b->cg_chunk = 16;
+    }
+    return FFH_OK;
+}
+
+int ffh_blob_kinetic_energy(ffh_blob *b, double *energy) {
+    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
+    if (!b->massful()) {
+        *energy = 0.0;
+        return FFH_OK;
+    }
+    ElemArgs g = b->elem_args();
+    hipLaunchKernelGGL(k_kinetic_energy, b->P_elem, kBlock, 0, b->stream, g, (const double *)b->rho, b->parts_energy());
+    HIP_OR_FAIL(hipGetLastError());
+    std::vector<double> h(b->P_elem);
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b->parts_energy(), h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *energy = 0.5 * s;
+    return FFH_OK;
 }
 
 int ffh_blob_get_rhs(ffh_blob *b, double *f) {

@@ -212,6 +212,7 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     P.matB.resize(E);
     P.matG.resize(E);
     P.matE.resize(E);
+    P.matRho.resize(E);
     for (int e = 0; e < E; e++) {
         int n[4];
         for (int a = 0; a < 4; a++) n[a] = P.enodes[4 * (size_t)e + a] = P.lin_of[topo[10 * e + a]];
@@ -220,8 +221,7 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         P.matB[e] = m[2] / (kMesoPressure * kMesoTime) - (2.0 / 3.0) * m[1];
         P.matG[e] = m[3] / kMesoPressure;
         P.matE[e] = m[4] / kMesoPressure;
-        (void)kMesoVolume;
-        (void)kMesoMass;
+        P.matRho[e] = m[0] * kMesoVolume / kMesoMass;
         double J[3][3];
         jacobian(X, Y, Z, n, J);
         // mat3_invert (src/mat_vec_fns.cpp:148-177)
@@ -428,6 +428,69 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         P.step_jump[comp] = to32(powm(comp, P.draws_per_step));
         // A^(7 l): the lane offsets of a wavefront of consecutive elements
         for (int l = 0; l < 64; l++) P.jump[4 * 256 * 2 + 2 * l + comp] = to32(powm(comp, 7ull * (uint64_t)l));
+    }
+    return "";
+}
+
+std::string build_mass(const Plan &P, int solver, std::vector<double> &slot_val, std::vector<double> &dinv3,
+                       std::vector<double> &inv_mass) {
+    const int N = P.num_linear, E = P.num_elements;
+    if (solver == 2) {
+        // inv_M[ni] += .25 * rho * vol_0 over the elements in order, pinned rows 1, then 1 / inv_M
+        inv_mass.assign(N, 0.0);
+        for (int e = 0; e < E; e++)
+            for (int a = 0; a < 4; a++) inv_mass[P.enodes[4 * (size_t)e + a]] += .25 * P.matRho[e] * P.vol0[e];
+        for (int i = 0; i < N; i++) {
+            if (P.pinned[i]) inv_mass[i] = 1.0;
+            inv_mass[i] = 1.0 / inv_mass[i];
+        }
+        return "";
+    }
+    if (solver != 1) return "unknown mass-ful solver";
+    // Real slots of a row are its leading slots with strictly increasing columns; the padding
+    // slots after them repeat the row's own (diagonal) column.
+    std::vector<int> len(N, 0);
+    auto slot_at = [&](int r, int t) -> int64_t { return P.slice_off[r / kSlice] + (int64_t)t * kSlice + r % kSlice; };
+    for (int r = 0; r < N; r++) {
+        const int w = P.slice_width[r / kSlice];
+        int prev = -1, t = 0;
+        for (; t < w; t++) {
+            const int c = P.slot_col[slot_at(r, t)];
+            if (c <= prev) break;
+            prev = c;
+        }
+        len[r] = t;
+    }
+    auto find = [&](int r, int c) -> int64_t {
+        int lo = 0, hi = len[r];
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (P.slot_col[slot_at(r, mid)] < c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return (lo < len[r] && P.slot_col[slot_at(r, lo)] == c) ? slot_at(r, lo) : -1;
+    };
+    slot_val.assign(P.num_slots, 0.0);
+    // mass_LU[ni][nj] += (.1 on the diagonal, .05 off it) * rho * vol_0, element by element
+    for (int e = 0; e < E; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                const int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
+                if (P.pinned[r] || P.pinned[c]) continue;
+                const int64_t k = find(r, c);
+                if (k < 0) return "mass matrix entry outside the block pattern";
+                slot_val[k] += (a == b ? .1 : .05) * P.matRho[e] * P.vol0[e];
+            }
+    dinv3.assign(3 * (size_t)N, 0.0);
+    for (int r = 0; r < N; r++) {
+        const int64_t k = find(r, r);
+        if (k < 0) return "mass matrix row without a diagonal";
+        if (P.pinned[r]) slot_val[k] = 1.0;
+        if (slot_val[k] == 0.0) return "zero mass on a linear node (density 0)";
+        const double d = 1.0 / slot_val[k];
+        for (int c = 0; c < 3; c++) dinv3[3 * (size_t)r + c] = d;
     }
     return "";
 }

@@ -54,6 +54,7 @@ struct Plan {
     std::vector<double> vol0;      // [E]
     std::vector<double> last_det;  // [E]
     std::vector<double> matA, matB, matG, matE;  // [E]
+    std::vector<double> matRho;                  // [E] density in meso units
     // per linear node
     std::vector<double> pos;     // [3*N] SoA x|y|z of the current positions
     std::vector<double> zeta;    // [N] Stokes drag (0 when calc_stokes == 0)
@@ -99,6 +100,19 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
                        const double *material_si, const double *stokes_radius, int num_pinned,
                        const int *pinned, double stokes_visc, int calc_noise, int calc_stokes,
                        int num_blobs = 1, const int *blob_node_off = nullptr);
+
+// The mass-ful solvers' matrices on the linear rows; midpoint rows are identity rows with a zero
+// right-hand side in the reference (their forces are folded onto the vertices first), so they
+// drop out of the system as they do for the viscosity matrix.
+//  solver 1 (ConjugateGradientSolver::init, src/ConjugateGradientSolver.cpp:62-203): the
+//    consistent mass matrix, one scalar per SELL slot of the block pattern (it acts on the three
+//    components alike), each entry summed over the elements in ascending order; identity rows
+//    for pinned and padding rows; dinv3 [3N] its inverse diagonal per scalar row.
+//  solver 2 (MassLumpedSolver::init, src/MassLumpedSolver.cpp:40-79): inv_mass [N], the inverse
+//    of a quarter of the mass of every element at the vertex; 1 on pinned and padding rows.
+// Returns "" or an error message.
+std::string build_mass(const Plan &P, int solver, std::vector<double> &slot_val, std::vector<double> &dinv3,
+                       std::vector<double> &inv_mass);
 
 // Exact jump matrices of MRG32k3a component c (0: m1, 1: m2) raised to n.
 Mat3u mrg_pow(int comp, uint64_t n);

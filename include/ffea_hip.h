@@ -161,15 +161,40 @@ int ffh_blob_steps_done(ffh_blob *b, long long *steps);
 /*
  * Solver::solve (include/Solver.h:51; NoMassCGSolver::solve, src/NoMassCGSolver.cpp:165-197):
  * x is [3*N_t] host memory, the force on entry, the velocity on exit.  Uses the viscosity
- * matrix of the last update_internal_forces (or of creation).
+ * matrix of the last update_internal_forces (or of creation).  With a mass-ful solver
+ * (ffh_blob_set_solver) the mass matrix, and x is the acceleration on exit.
  */
 int ffh_blob_solve(ffh_blob *b, double *x);
 
 /*
  * Solver::apply_matrix (include/Solver.h:54): result = K * in over [3*N_t] host vectors
  * (K = assembled viscosity matrix with Stokes drag; midpoint and pinned rows identity).
+ * Massless solver only (FFH_E_ARG otherwise).
  */
 int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result);
+
+/*
+ * Mass-ful solvers: Blob::init's solver choice (src/Blob.cpp:358-373) with the codes of
+ * include/FFEA_return_codes.h:30-33.  FFH_SOLVER_CG is ConjugateGradientSolver on the
+ * consistent mass matrix (src/ConjugateGradientSolver.cpp:62-230), FFH_SOLVER_MASSLUMPED is
+ * MassLumpedSolver (src/MassLumpedSolver.cpp:40-90), FFH_SOLVER_CG_NOMASS (the default) the
+ * massless NoMassCGSolver.  With a mass-ful solver the element phase adds the viscous force
+ * K_e u_e of the node velocities (src/Blob.cpp:703-706), the force phase the Stokes drag
+ * -zeta v (:3854-3877), the solve gives accelerations and the integration is vel += a dt,
+ * pos += vel dt (:3955-3967); the viscosity matrix is not assembled.  The direct solver
+ * (FFEA_DIRECT_SOLVER) is not on this path: FFH_E_ARG.  Velocities start at zero.
+ */
+#define FFH_SOLVER_CG 1
+#define FFH_SOLVER_MASSLUMPED 2
+#define FFH_SOLVER_CG_NOMASS 3
+int ffh_blob_set_solver(ffh_blob *b, int solver);
+/* Velocities of all N_t nodes [3*N_t], meso units (a mass-ful restart).  Midpoint velocities
+ * are kept as given and reported back: the solve gives midpoints zero acceleration, and their
+ * drag is zero on this path, so they never change and never act. */
+int ffh_blob_set_velocities(ffh_blob *b, const double *vel);
+/* Kinetic energy of Blob::make_measurements (src/Blob.cpp:1385-1407, :1431), summed over the
+ * handle's blobs, meso energy units; 0 with the massless solver. */
+int ffh_blob_kinetic_energy(ffh_blob *b, double *energy);
 
 /* Node state over all N_t nodes, host memory [3*N_t], meso units.  get_positions re-averages
  * midpoints (linearise_element); get_velocities reports 0 on midpoints, like the reference. */

@@ -470,6 +470,12 @@ typedef struct {
     /* CG work vectors (vector3 per node in the reference) */
     double *r, *p, *z, *q, *f;
     int last_cg_iterations, last_cg_status, last_inversions;
+    /* solver choice (include/FFEA_return_codes.h:30-33): 1 ConjugateGradientSolver on the
+     * mass matrix, 2 MassLumpedSolver, 3 NoMassCGSolver (the default) */
+    int linear_solver;
+    int *mkey, *mcol;      /* mass matrix, scalar CSR over N_t rows */
+    double *mval, *mprec;  /* its values and Jacobi preconditioner */
+    double *inv_M;         /* lumped inverse masses [N_t] */
 } orc_blob;
 
 static void *xcalloc(size_t n, size_t s) {
@@ -539,6 +545,7 @@ orc_blob *orc_blob_create(int num_nodes, int num_elements, const double *pos, co
     b->calc_noise = calc_noise;
     b->calc_stokes = calc_stokes;
     b->stokes_visc = stokes_visc;
+    b->linear_solver = 3;
     b->pos = xcalloc(3 * (size_t)N, sizeof(double));
     b->vel = xcalloc(3 * (size_t)N, sizeof(double));
     b->force = xcalloc(3 * (size_t)N, sizeof(double));
@@ -691,7 +698,7 @@ void orc_blob_destroy(orc_blob *b) {
                     b->topo, b->rho, b->A, b->B, b->G, b->E, b->mass, b->J_inv_0, b->vol_0, b->vol,
                     b->last_det, b->dpsi, b->F_ij, b->visc, b->node_force, b->stress_mag, b->contrib_off,
                     b->contrib, b->key, b->col, b->src_off, (void *)b->src, b->val, b->diag_inv, b->diag_idx,
-                    b->r, b->p, b->z, b->q, b->f};
+                    b->r, b->p, b->z, b->q, b->f, b->mkey, b->mcol, b->mval, b->mprec, b->inv_M};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(b);
 }
@@ -728,7 +735,22 @@ int orc_blob_update_internal_forces(orc_blob *b, orc_stream *g) {
                                 stress[2][2] * stress[2][2] +
                                 2 * (stress[0][1] * stress[0][1] + stress[0][2] * stress[0][2] +
                                      stress[1][2] * stress[1][2]));
-        for (int k = 0; k < 12; k++) du[k] = 0;
+        if (b->linear_solver != 3) {
+            /* mass-ful solvers: viscous force K_e u_e (src/Blob.cpp:703-706) from
+             * get_element_velocity_vector (:454-469), x-major over the 4 vertices, and
+             * mat12_apply (src/mat_vec_fns.cpp:26-34) */
+            double v[12];
+            const double(*V)[12] = (const double(*)[12]) & b->visc[144 * n];
+            for (int i = 0; i < 3; i++)
+                for (int a = 0; a < 4; a++) v[4 * i + a] = b->vel[3 * t[a] + i];
+            for (int i = 0; i < 12; i++) {
+                double acc = 0;
+                for (int j = 0; j < 12; j++) acc += V[i][j] * v[j];
+                du[i] = acc;
+            }
+        } else {
+            for (int k = 0; k < 12; k++) du[k] = 0;
+        }
         /* apply_stress_tensor (:441-449) */
         double vol = b->vol[n];
         for (int i = 0; i < 3; i++)
@@ -825,7 +847,15 @@ static int blob_aggregate_forces(orc_blob *b, orc_stream *g) {
             b->force[3 * n + 1] += c[1];
             b->force[3 * n + 2] += c[2];
         }
-    if (b->calc_stokes == 1 && b->calc_noise == 1)
+    if (b->calc_stokes == 1 && b->linear_solver != 3) {
+        /* mass-ful branch (src/Blob.cpp:3854-3877): drag on the node velocity, then noise */
+        for (int i = 0; i < N; i++) {
+            for (int c = 0; c < 3; c++) b->force[3 * i + c] -= b->vel[3 * i + c] * b->stokes_drag[i];
+            if (b->calc_noise == 1)
+                for (int c = 0; c < 3; c++)
+                    b->force[3 * i + c] -= rand_half(g) * sqrt((24 * b->kT * b->stokes_drag[i]) / (b->dt));
+        }
+    } else if (b->calc_stokes == 1 && b->calc_noise == 1)
         for (int i = 0; i < N; i++)
             for (int c = 0; c < 3; c++)
                 b->force[3 * i + c] -= rand_half(g) * sqrt((24 * b->kT * b->stokes_drag[i]) / (b->dt));
@@ -848,16 +878,234 @@ static int blob_aggregate_forces(orc_blob *b, orc_stream *g) {
     return 0;
 }
 
-/* Blob::update_positions (src/Blob.cpp:737-764) with euler_integrate (:3937-3953). */
+/* ---- mass-ful solvers ------------------------------------------------------ */
+
+/* ConjugateGradientSolver::init (src/ConjugateGradientSolver.cpp:62-203): the consistent
+ * mass matrix of the linear vertices (0.1 rho vol_0 on the diagonal, 0.05 off it, summed in
+ * element order), 1 on the diagonal of pinned rows and of midpoints, the nonzeros of each
+ * dense row in ascending column order, and the inverse diagonal as preconditioner.  The
+ * dense N_t x N_t scan of the reference is restated row by row over a sorted column list. */
+static void mass_cg_init(orc_blob *b) {
+    int N = b->num_nodes, E = b->num_elements;
+    /* per-row (col, value) lists, built in the reference's accumulation order */
+    int *cnt = xcalloc(N, sizeof(int));
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < 10; i++) cnt[b->topo[10 * e + i]] += 10;
+    int *off = xcalloc((size_t)N + 1, sizeof(int));
+    for (int i = 0; i < N; i++) off[i + 1] = off[i] + cnt[i];
+    int *cols = xcalloc((size_t)off[N], sizeof(int));
+    double *vals = xcalloc((size_t)off[N], sizeof(double));
+    int *len = xcalloc(N, sizeof(int));
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < 10; i++)
+            for (int j = 0; j < 10; j++) {
+                int ni = b->topo[10 * e + i], nj = b->topo[10 * e + j], k;
+                if (!(i < 4 && j < 4) && i != j) continue;
+                for (k = 0; k < len[ni]; k++)
+                    if (cols[off[ni] + k] == nj) break;
+                if (k == len[ni]) {
+                    cols[off[ni] + k] = nj;
+                    vals[off[ni] + k] = 0;
+                    len[ni]++;
+                }
+                double *m = &vals[off[ni] + k];
+                if (i < 4 && j < 4) {
+                    if (b->is_pinned[ni] == 0 && b->is_pinned[nj] == 0)
+                        *m += (i == j ? .1 : .05) * b->rho[e] * b->vol_0[e];
+                    else if (i == j)
+                        *m = 1;
+                } else {
+                    *m = 1; /* i == j, a midpoint */
+                }
+            }
+    /* nonzeros in ascending column order */
+    int nnz = 0;
+    for (int i = 0; i < N; i++) {
+        int *c = &cols[off[i]];
+        double *v = &vals[off[i]];
+        for (int a = 1; a < len[i]; a++)
+            for (int q = a; q > 0 && c[q - 1] > c[q]; q--) {
+                int tc = c[q];
+                c[q] = c[q - 1];
+                c[q - 1] = tc;
+                double tv = v[q];
+                v[q] = v[q - 1];
+                v[q - 1] = tv;
+            }
+        for (int a = 0; a < len[i]; a++) nnz += v[a] != 0;
+    }
+    b->mkey = xcalloc((size_t)N + 1, sizeof(int));
+    b->mcol = xcalloc(nnz, sizeof(int));
+    b->mval = xcalloc(nnz, sizeof(double));
+    b->mprec = xcalloc(N, sizeof(double));
+    int k = 0;
+    for (int i = 0; i < N; i++) {
+        b->mkey[i] = k;
+        double diag = 0;
+        for (int a = 0; a < len[i]; a++) {
+            double v = vals[off[i] + a];
+            if (cols[off[i] + a] == i) diag = v;
+            if (v == 0) continue;
+            b->mcol[k] = cols[off[i] + a];
+            b->mval[k] = v;
+            k++;
+        }
+        b->mprec[i] = 1.0 / diag;
+    }
+    b->mkey[N] = k;
+    free(cnt);
+    free(off);
+    free(cols);
+    free(vals);
+    free(len);
+}
+
+/* MassLumpedSolver::init (src/MassLumpedSolver.cpp:40-79). */
+static void mass_lumped_init(orc_blob *b) {
+    int N = b->num_nodes;
+    b->inv_M = xcalloc(N, sizeof(double));
+    for (int e = 0; e < b->num_elements; e++)
+        for (int i = 0; i < 10; i++) {
+            int ni = b->topo[10 * e + i];
+            if (i < 4)
+                b->inv_M[ni] += .25 * b->rho[e] * b->vol_0[e];
+            else
+                b->inv_M[ni] = 1;
+        }
+    for (int i = 0; i < b->num_pinned; i++) b->inv_M[b->pinned[i]] = 1.0;
+    for (int i = 0; i < N; i++) b->inv_M[i] = 1.0 / b->inv_M[i];
+}
+
+/* Selects the Blob's linear solver (Blob::init, src/Blob.cpp:358-373): 1 ConjugateGradient
+ * (mass matrix), 2 MassLumped, 3 NoMassCG.  Returns 0, or -1 for another value. */
+int orc_blob_set_solver(orc_blob *b, int solver) {
+    if (solver < 1 || solver > 3) return -1;
+    b->linear_solver = solver;
+    if (solver == 1 && !b->mkey) mass_cg_init(b);
+    if (solver == 2 && !b->inv_M) mass_lumped_init(b);
+    return 0;
+}
+
+/* ConjugateGradientSolver::solve (src/ConjugateGradientSolver.cpp:205-230) with
+ * conjugate_gradient_residual_assume_x_zero (:241-265), parallel_sparse_matrix_apply
+ * (:268-287), parallel_vector_add_self / parallel_vector_add (:289-311),
+ * parallel_apply_preconditioner (:314-329) and residual2 (:332-349): every sum in row order. */
+static int mass_cg_solve(orc_blob *b, double *x) {
+    int N = b->num_nodes;
+    double *r = b->r, *d = b->p, *s = b->z, *q = b->q, *f = b->f, *P = b->mprec;
+    double delta_new = 0, delta_old, dTq, alpha;
+    for (int i = 0; i < N; i++) {
+        for (int c = 0; c < 3; c++) {
+            r[3 * i + c] = x[3 * i + c];
+            f[3 * i + c] = x[3 * i + c];
+            x[3 * i + c] = 0;
+            d[3 * i + c] = P[i] * r[3 * i + c];
+        }
+        delta_new += r[3 * i] * d[3 * i] + r[3 * i + 1] * d[3 * i + 1] + r[3 * i + 2] * d[3 * i + 2];
+    }
+    for (int it = 0; it < b->max_iterations_cg; it++) {
+        double r2 = 0, f2 = 0;
+        for (int i = 0; i < N; i++) {
+            r2 += r[3 * i] * r[3 * i] + r[3 * i + 1] * r[3 * i + 1] + r[3 * i + 2] * r[3 * i + 2];
+            f2 += f[3 * i] * f[3 * i] + f[3 * i + 1] * f[3 * i + 1] + f[3 * i + 2] * f[3 * i + 2];
+        }
+        double res = f2 == 0.0 ? 0.0 : r2 / f2;
+        if (res < b->epsilon2) {
+            b->last_cg_iterations = it;
+            b->last_cg_status = 0;
+            return 0;
+        }
+        dTq = 0;
+        for (int i = 0; i < N; i++) {
+            double qx = 0, qy = 0, qz = 0;
+            for (int j = b->mkey[i]; j < b->mkey[i + 1]; j++) {
+                int c = b->mcol[j];
+                qx += b->mval[j] * d[3 * c];
+                qy += b->mval[j] * d[3 * c + 1];
+                qz += b->mval[j] * d[3 * c + 2];
+            }
+            q[3 * i] = qx;
+            q[3 * i + 1] = qy;
+            q[3 * i + 2] = qz;
+            dTq += d[3 * i] * qx + d[3 * i + 1] * qy + d[3 * i + 2] * qz;
+        }
+        alpha = delta_new / dTq;
+        for (int i = 0; i < 3 * N; i++) x[i] += alpha * d[i];
+        for (int i = 0; i < 3 * N; i++) r[i] += -alpha * q[i];
+        delta_old = delta_new;
+        delta_new = 0;
+        for (int i = 0; i < N; i++) {
+            for (int c = 0; c < 3; c++) s[3 * i + c] = P[i] * r[3 * i + c];
+            delta_new += r[3 * i] * s[3 * i] + r[3 * i + 1] * s[3 * i + 1] + r[3 * i + 2] * s[3 * i + 2];
+        }
+        for (int i = 0; i < 3 * N; i++) d[i] = s[i] + (delta_new / delta_old) * d[i];
+    }
+    b->last_cg_iterations = b->max_iterations_cg;
+    b->last_cg_status = -1;
+    return -1;
+}
+
+/* MassLumpedSolver::solve (src/MassLumpedSolver.cpp:82-90). */
+static int mass_lumped_solve(orc_blob *b, double *x) {
+    memcpy(b->f, x, sizeof(double) * 3 * (size_t)b->num_nodes); /* kept for comparisons only */
+    for (int i = 0; i < b->num_nodes; i++)
+        for (int c = 0; c < 3; c++) x[3 * i + c] = x[3 * i + c] * b->inv_M[i];
+    b->last_cg_iterations = 0;
+    b->last_cg_status = 0;
+    return 0;
+}
+
+static int blob_solver_solve(orc_blob *b, double *x) {
+    if (b->linear_solver == 1) return mass_cg_solve(b, x);
+    if (b->linear_solver == 2) return mass_lumped_solve(b, x);
+    return blob_solve(b, x);
+}
+
+/* Overwrite all N_t node velocities (a mass-ful restart frame). */
+void orc_blob_set_velocities(orc_blob *b, const double *vel) {
+    memcpy(b->vel, vel, sizeof(double) * 3 * (size_t)b->num_nodes);
+}
+
+/* Kinetic energy of Blob::make_measurements (src/Blob.cpp:1385-1407, :1436): per element
+ * u_e . (M_e u_e) with apply_element_mass_matrix (src/tetra_element_linear.cpp:534-553),
+ * halved.  Zero for the massless solver. */
+double orc_blob_kinetic_energy(const orc_blob *b) {
+    static const double M[4][4] = {{.1, .05, .05, .05}, {.05, .1, .05, .05}, {.05, .05, .1, .05}, {.05, .05, .05, .1}};
+    if (b->linear_solver == 3) return 0.0;
+    double ke = 0.0;
+    for (int n = 0; n < b->num_elements; n++) {
+        const int *t = &b->topo[10 * n];
+        double u[12], mu[12] = {0};
+        for (int i = 0; i < 3; i++)
+            for (int a = 0; a < 4; a++) u[4 * i + a] = b->vel[3 * t[a] + i];
+        for (int k = 0; k < 12; k += 4)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) mu[i + k] += M[i][j] * u[j + k];
+        for (int i = 0; i < 12; i++) mu[i] = mu[i] * (b->vol_0[n] * b->rho[n]);
+        ke += u[0] * mu[0] + u[1] * mu[1] + u[2] * mu[2] + u[3] * mu[3] + u[4] * mu[4] + u[5] * mu[5] +
+              u[6] * mu[6] + u[7] * mu[7] + u[8] * mu[8] + u[9] * mu[9] + u[10] * mu[10] + u[11] * mu[11];
+    }
+    return ke * 0.5;
+}
+
+/* Blob::update_positions (src/Blob.cpp:737-764) with euler_integrate (:3937-3967). */
 int orc_blob_update_positions(orc_blob *b, orc_stream *g) {
     blob_aggregate_forces(b, g);
-    int st = blob_solve(b, b->force);
+    int st = blob_solver_solve(b, b->force);
     if (st != 0) return st; /* the reference skips integration on a solver error */
-    for (int i = 0; i < b->num_nodes; i++)
-        for (int c = 0; c < 3; c++) {
-            b->vel[3 * i + c] = b->force[3 * i + c];
-            b->pos[3 * i + c] += b->force[3 * i + c] * b->dt;
-        }
+    if (b->linear_solver == 3) {
+        for (int i = 0; i < b->num_nodes; i++)
+            for (int c = 0; c < 3; c++) {
+                b->vel[3 * i + c] = b->force[3 * i + c];
+                b->pos[3 * i + c] += b->force[3 * i + c] * b->dt;
+            }
+    } else {
+        for (int i = 0; i < b->num_nodes; i++)
+            for (int c = 0; c < 3; c++) {
+                b->vel[3 * i + c] += b->force[3 * i + c] * b->dt;
+                b->pos[3 * i + c] += b->vel[3 * i + c] * b->dt;
+            }
+    }
     for (int e = 0; e < b->num_elements; e++) elem_linearise(b->pos, &b->topo[10 * e]);
     return 0;
 }
@@ -888,8 +1136,9 @@ int orc_blob_forces_only(orc_blob *b, orc_stream *g, double *out_rhs) {
     return inv;
 }
 
-/* NoMassCGSolver::solve on a caller RHS (in/out) with the element matrices currently held. */
-int orc_blob_solve(orc_blob *b, double *x) { return blob_solve(b, x); }
+/* The blob's solver on a caller RHS (in/out); NoMassCGSolver with the element matrices
+ * currently held. */
+int orc_blob_solve(orc_blob *b, double *x) { return blob_solver_solve(b, x); }
 
 /* V * in with the matrix as last built (call after a solve or orc_blob_build). */
 void orc_blob_build(orc_blob *b) { blob_build_matrix(b); }

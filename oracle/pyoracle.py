@@ -70,6 +70,13 @@ class _Lib:
         getattr(L, p + "package_seed_from_rng_seed").argtypes = [C.c_int, _ulp]
         getattr(L, p + "rng_set_package_seed").argtypes = [_ulp]
         getattr(L, p + "rng_set_package_seed").restype = C.c_int
+        f = getattr(L, p + "blob_set_solver")
+        f.argtypes = [C.c_void_p, C.c_int]
+        f.restype = C.c_int
+        getattr(L, p + "blob_set_velocities").argtypes = [C.c_void_p, _dp]
+        f = getattr(L, p + "blob_kinetic_energy")
+        f.argtypes = [C.c_void_p]
+        f.restype = C.c_double
 
     def fn(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -119,6 +126,19 @@ class Blob:
 
     def set_positions(self, pos_meso):
         self.lib.fn("blob_set_positions")(self.h, np.ascontiguousarray(pos_meso, np.float64).ravel())
+
+    # solver codes of include/FFEA_return_codes.h:30-33
+    SOLVERS = {"CG": 1, "masslumped": 2, "CG_nomass": 3}
+
+    def set_solver(self, name: str):
+        if self.lib.fn("blob_set_solver")(self.h, self.SOLVERS[name]) != 0:
+            raise ValueError(name)
+
+    def set_velocities(self, vel_meso):
+        self.lib.fn("blob_set_velocities")(self.h, np.ascontiguousarray(vel_meso, np.float64).ravel())
+
+    def kinetic_energy(self) -> float:
+        return self.lib.fn("blob_kinetic_energy")(self.h)
 
     def get(self, what: str) -> np.ndarray:
         sizes = {"pos": 3 * self.N, "vel": 3 * self.N, "force": 3 * self.N, "rhs": 3 * self.N,

@@ -1,7 +1,8 @@
 /*
  * ref_harness.cpp -- drives the reference's OWN compiled hot-path objects
- * (tetra_element_linear, NoMassCGSolver, SparsityPattern, SparseMatrixFixedPattern,
- * SparseMatrixTypes, mat_vec_fns, RngStream, mesh_node) from /root/reference/src.
+ * (tetra_element_linear, NoMassCGSolver, ConjugateGradientSolver, MassLumpedSolver,
+ * SparsityPattern, SparseMatrixFixedPattern, SparseMatrixTypes, mat_vec_fns, RngStream,
+ * mesh_node) from /root/reference/src.
  *
  * TEST INFRASTRUCTURE ONLY (the checker of the checker).  Built by oracle/Makefile
  * into oracle/_ref/libffea_ref.so; nothing in the product links it.
@@ -30,6 +31,8 @@
 #define private public /* read NoMassCGSolver::V for matrix comparisons */
 #include "NoMassCGSolver.h"
 #undef private
+#include "ConjugateGradientSolver.h"
+#include "MassLumpedSolver.h"
 
 namespace {
 const double meso_volume = 4.913e-30, meso_mass = 1.994307387553024e-26;
@@ -47,6 +50,8 @@ struct ref_blob {
     int *pinned;
     double *rhs;
     int last_inversions;
+    int linear_solver;  // FFEA_ITERATIVE_SOLVER, FFEA_MASSLUMPED_SOLVER or FFEA_NOMASS_CG_SOLVER
+    Solver *msolver;    // the mass-ful solver when one is selected
 };
 
 extern "C" {
@@ -162,12 +167,56 @@ ref_blob *ref_blob_create(int num_nodes, int num_elements, const double *pos, co
         for (int i = 0; i < num_nodes; ++i)
             b->node[i].stokes_drag = 6.0 * ffea_const::pi * stokes_visc * b->node[i].stokes_radius;
     b->rhs = new double[3 * (size_t)num_nodes]();
+    b->linear_solver = FFEA_NOMASS_CG_SOLVER;
+    b->msolver = nullptr;
     return b;
+}
+
+/* Blob::init's solver choice (src/Blob.cpp:358-380) for the two mass-ful solvers. */
+int ref_blob_set_solver(ref_blob *b, int solver) {
+    if (solver != FFEA_ITERATIVE_SOLVER && solver != FFEA_MASSLUMPED_SOLVER && solver != FFEA_NOMASS_CG_SOLVER)
+        return -1;
+    delete b->msolver;
+    b->msolver = nullptr;
+    b->linear_solver = solver;
+    if (solver == FFEA_NOMASS_CG_SOLVER) return 0;
+    if (solver == FFEA_ITERATIVE_SOLVER)
+        b->msolver = new ConjugateGradientSolver();
+    else
+        b->msolver = new MassLumpedSolver();
+    std::set<int> bsite;
+    return b->msolver->init(b->N, b->E, b->node, b->elem, b->params, b->num_pinned, b->pinned, bsite) == FFEA_OK ? 0 : -1;
+}
+
+void ref_blob_set_velocities(ref_blob *b, const double *vel) {
+    for (int i = 0; i < b->N; i++) {
+        b->node[i].vel.x = vel[3 * i];
+        b->node[i].vel.y = vel[3 * i + 1];
+        b->node[i].vel.z = vel[3 * i + 2];
+    }
+}
+
+/* Kinetic energy of Blob::make_measurements (src/Blob.cpp:1385-1407, :1431). */
+double ref_blob_kinetic_energy(ref_blob *b) {
+    if (b->linear_solver == FFEA_NOMASS_CG_SOLVER) return 0.0;
+    scalar kenergy = 0.0;
+    vector12 vec;
+    for (int n = 0; n < b->E; n++) {
+        tetra_element_linear &e = b->elem[n];
+        e.get_element_velocity_vector(vec);
+        e.apply_element_mass_matrix(vec);
+        kenergy += e.n[0]->vel.x * vec[0] + e.n[1]->vel.x * vec[1] + e.n[2]->vel.x * vec[2] +
+                   e.n[3]->vel.x * vec[3] + e.n[0]->vel.y * vec[4] + e.n[1]->vel.y * vec[5] +
+                   e.n[2]->vel.y * vec[6] + e.n[3]->vel.y * vec[7] + e.n[0]->vel.z * vec[8] +
+                   e.n[1]->vel.z * vec[9] + e.n[2]->vel.z * vec[10] + e.n[3]->vel.z * vec[11];
+    }
+    return kenergy * 0.5;
 }
 
 void ref_blob_destroy(ref_blob *b) {
     if (!b) return;
     delete b->solver;
+    delete b->msolver;
     delete[] b->elem;
     delete[] b->node;
     delete[] b->force;
@@ -190,7 +239,7 @@ static void zero_forces(ref_blob *b) {
     for (int i = 0; i < b->N; i++) b->force[i].x = b->force[i].y = b->force[i].z = 0;
 }
 
-/* Blob::update_internal_forces, nomass (src/Blob.cpp:660-724); OpenMP over elements with one
+/* Blob::update_internal_forces (src/Blob.cpp:660-724); OpenMP over elements with one
  * stream per thread in the FFEA_PARALLEL_WITHIN_BLOB build, exactly one thread otherwise. */
 int ref_blob_update_internal_forces(ref_blob *b, void *rngp) {
     RngStream *rng = static_cast<RngStream *>(rngp);
@@ -215,7 +264,12 @@ int ref_blob_update_internal_forces(ref_blob *b, void *rngp) {
         b->elem[n].add_bulk_elastic_stress(stress);
         if (b->params->calc_noise == 1) b->elem[n].add_fluctuating_stress(b->params, rng, stress, tid);
         b->elem[n].internal_stress_mag = sqrt(mat3_double_contraction_symmetric(stress));
-        vec12_set_zero(du);
+        if (b->linear_solver != FFEA_NOMASS_CG_SOLVER) {
+            b->elem[n].get_element_velocity_vector(du);
+            mat12_apply(b->elem[n].viscosity_matrix, du);
+        } else {
+            vec12_set_zero(du);
+        }
         b->elem[n].apply_stress_tensor(stress, du);
         b->elem[n].add_element_force_vector(du);
     }
@@ -235,7 +289,30 @@ static void aggregate(ref_blob *b, RngStream *rng) {
             b->force[n][1] += b->node[n].force_contributions[m]->y;
             b->force[n][2] += b->node[n].force_contributions[m]->z;
         }
-    if (b->params->calc_stokes == 1 && b->params->calc_noise == 1)
+    if (b->params->calc_stokes == 1 && b->linear_solver != FFEA_NOMASS_CG_SOLVER) {
+        /* mass-ful branch (src/Blob.cpp:3854-3877) */
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+#pragma omp parallel
+#endif
+        {
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+            const int thread_id = omp_get_thread_num();
+#pragma omp for schedule(guided)
+#else
+            const int thread_id = 0;
+#endif
+            for (int i = 0; i < b->N; i++) {
+                b->force[i].x -= b->node[i].vel.x * b->node[i].stokes_drag;
+                b->force[i].y -= b->node[i].vel.y * b->node[i].stokes_drag;
+                b->force[i].z -= b->node[i].vel.z * b->node[i].stokes_drag;
+                if (b->params->calc_noise == 1) {
+                    b->force[i].x -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
+                    b->force[i].y -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
+                    b->force[i].z -= RAND(-.5, .5) * sqrt((24 * b->params->kT * b->node[i].stokes_drag) / (b->params->dt));
+                }
+            }
+        }
+    } else if (b->params->calc_stokes == 1 && b->params->calc_noise == 1)
 #ifdef FFEA_PARALLEL_WITHIN_BLOB
 #pragma omp parallel
 #endif
@@ -276,20 +353,38 @@ static void aggregate(ref_blob *b, RngStream *rng) {
     }
 }
 
-/* Blob::update_positions (src/Blob.cpp:737-764) + euler_integrate (:3937-3953). */
+static Solver *active_solver(ref_blob *b) {
+    return b->linear_solver == FFEA_NOMASS_CG_SOLVER ? static_cast<Solver *>(b->solver) : b->msolver;
+}
+
+/* Blob::update_positions (src/Blob.cpp:737-764) + euler_integrate (:3937-3967). */
 int ref_blob_update_positions(ref_blob *b, void *rngp) {
     aggregate(b, static_cast<RngStream *>(rngp));
-    if (b->solver->solve(b->force) == FFEA_ERROR) return -1;
+    if (active_solver(b)->solve(b->force) == FFEA_ERROR) return -1;
+    if (b->linear_solver == FFEA_NOMASS_CG_SOLVER) {
 #ifdef FFEA_PARALLEL_WITHIN_BLOB
 #pragma omp parallel for schedule(static)
 #endif
-    for (int i = 0; i < b->N; i++) {
-        b->node[i].vel[0] = b->force[i][0];
-        b->node[i].vel[1] = b->force[i][1];
-        b->node[i].vel[2] = b->force[i][2];
-        b->node[i].pos[0] += b->force[i][0] * b->params->dt;
-        b->node[i].pos[1] += b->force[i][1] * b->params->dt;
-        b->node[i].pos[2] += b->force[i][2] * b->params->dt;
+        for (int i = 0; i < b->N; i++) {
+            b->node[i].vel[0] = b->force[i][0];
+            b->node[i].vel[1] = b->force[i][1];
+            b->node[i].vel[2] = b->force[i][2];
+            b->node[i].pos[0] += b->force[i][0] * b->params->dt;
+            b->node[i].pos[1] += b->force[i][1] * b->params->dt;
+            b->node[i].pos[2] += b->force[i][2] * b->params->dt;
+        }
+    } else {
+#ifdef FFEA_PARALLEL_WITHIN_BLOB
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < b->N; i++) {
+            b->node[i].vel[0] += b->force[i][0] * b->params->dt;
+            b->node[i].vel[1] += b->force[i][1] * b->params->dt;
+            b->node[i].vel[2] += b->force[i][2] * b->params->dt;
+            b->node[i].pos[0] += b->node[i].vel[0] * b->params->dt;
+            b->node[i].pos[1] += b->node[i].vel[1] * b->params->dt;
+            b->node[i].pos[2] += b->node[i].vel[2] * b->params->dt;
+        }
     }
     for (int n = 0; n < b->E; n++) b->elem[n].linearise_element();
     return 0;
@@ -314,11 +409,11 @@ int ref_blob_forces_only(ref_blob *b, void *rng, double *out_rhs) {
     return inv;
 }
 
-/* NoMassCGSolver::solve on a caller RHS (in/out, [3*N_t]). */
+/* The blob's solver on a caller RHS (in/out, [3*N_t]). */
 int ref_blob_solve(ref_blob *b, double *x) {
     vector3 *v = new vector3[b->N];
     for (int i = 0; i < b->N; i++) v[i].assign(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
-    int st = b->solver->solve(v);
+    int st = active_solver(b)->solve(v);
     for (int i = 0; i < b->N; i++) {
         x[3 * i] = v[i].x;
         x[3 * i + 1] = v[i].y;
