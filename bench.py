@@ -155,16 +155,75 @@ def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
     return steps / dt, dt, kt, sizes, float(np.mean(iters)), dt_prof / steps
 
 
-def kernel_table(kt, sizes, bm):
+def algorithmic_bytes_mass(kernel, sizes, bm, solver):
+    """The same count for the mass-ful solvers: the element kernel also gathers the node
+    velocities (K_e u_e), the force kernel reads them for the drag (MassLumpedSolver: it writes
+    f and x = f / m only), the mass matrix holds one FP64 value per block, the integration reads
+    the velocity it updates.  No assembly."""
+    N, B = sizes["num_linear"], sizes["num_blocks"]
+    v = 24 * N
+    base = algorithmic_bytes(kernel, sizes, bm)
+    E = sizes["num_elements"]
+    return {
+        "element": base + v if base is not None else None,
+        "rhs": (12 * bm * E + 16 * E + 39 * N + 3 * v + 8 * N) if solver == "masslumped" else base + v,
+        "cg_spmv": 12 * B + 5 * v,
+        "cg_update": 8 * v,
+        "integrate": 5 * v,
+    }.get(kernel)
+
+
+def kernel_table(kt, sizes, bm, solver="CG_nomass"):
     out = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
         avg_s = ms / cnt / 1e3
-        byts = algorithmic_bytes(k, sizes, bm)
+        byts = algorithmic_bytes(k, sizes, bm) if solver == "CG_nomass" else algorithmic_bytes_mass(k, sizes, bm, solver)
         out[k] = {"launches": cnt, "avg_us": avg_s * 1e6, "total_ms": ms}
         if byts is not None:
             out[k].update(alg_bytes=int(byts), achieved_GBs=byts / avg_s / 1e9)
+    return out
+
+
+def massful(ms, prm, sc, xs, steps, warmup):
+    """The mass-ful solvers on the same workload (FP64): ConjugateGradientSolver on the
+    consistent mass matrix and MassLumpedSolver, steps/s and per-kernel times.  Time step 1 fs:
+    the explicit mass-ful update of these nanometre elements is unstable at 10 fs (in the
+    reference as in this build); the work per step does not depend on it."""
+    import dataclasses
+    from ffea_amd.blob import Batch, Blob, package_seed
+    p = dataclasses.replace(prm, dt=prm.dt / 10)
+    out = {"dt_s": 1e-15}
+    for solver in ("CG", "masslumped"):
+        if len(ms) == 1:
+            b = Blob(ms[0], p, sc, rng_state=package_seed(p.rng_seed), pos=xs[0])
+        else:
+            b = Batch(ms, p, sc, rng_state=package_seed(p.rng_seed))
+            b.set_positions(np.concatenate(xs) * sc)
+        b.set_solver(solver)
+        b.step(warmup)
+        b.synchronize()
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.step(1)
+            iters.append(b.last_cg()[0])
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        b.set_profiling(True)
+        b.step(steps)
+        b.synchronize()
+        kt = b.kernel_times()
+        b.set_profiling(False)
+        sizes = b.sizes()
+        ke = b.kinetic_energy()
+        b.close()
+        log(f"[{solver}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s, CG iterations mean "
+            f"{np.mean(iters):.2f}, kinetic energy {ke:.4g} (3/2 N kT = {1.5 * sizes['num_linear'] * p.kT:.4g})")
+        out[solver] = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+                       "cg_iters_mean": float(np.mean(iters)), "kinetic_energy_meso": ke,
+                       "kernels": kernel_table(kt, sizes, 8, solver)}
     return out
 
 
@@ -567,6 +626,8 @@ def main():
                     help="noise-off steps compared with the reference's fast build (0: skip; c1 asks for 1000)")
     ap.add_argument("--equipartition-steps", type=int, default=4000,
                     help="noise-on steps of the equipartition check, averaged over the second half (0: skip)")
+    ap.add_argument("--massful-steps", type=int, default=100,
+                    help="timed steps of each mass-ful solver on the same workload (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -605,6 +666,7 @@ def main():
         equi = {p: equipartition(ms, prm, sc, xs, p, a.equipartition_steps, a.equipartition_steps // 2)
                 for p in (("fp64",) if a.no_mixed else ("fp64", "mixed"))}
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
+    mass = massful(ms, prm, sc, xs, a.massful_steps, a.warmup) if a.massful_steps > 0 else None
     out = {
         "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
         "value": sps,
@@ -633,6 +695,7 @@ def main():
         "mixed_vs_fp64": mvf,
         "trajectory_vs_reference": traj,
         "equipartition": equi,
+        "massful": mass,
     }
     if calib:
         out["copy_calibration"] = calib

@@ -15,6 +15,8 @@ MESO_MASS = 1.994307387553024e-26
 MESO_VOLUME = 4.913e-30
 MESO_TIME = 3.7302915560886126e-13
 MESO_PRESSURE = 8.4305832688784826e8
+MESO_FORCE = 2.4364705882352941e-11
+MESO_VELOCITY = 4.5572845297447225e2
 
 KB = 1.380649e-23
 PI = 3.141592653589793

@@ -1,5 +1,7 @@
-"""World-level driver of the massless path: a .ffea script's dynamic CG_nomass blobs stepped
-together on one GPU (ffea_amd.blob.Batch), with FFEA's output and checkpoint files.
+"""World-level driver: a .ffea script's dynamic blobs stepped together on one GPU
+(ffea_amd.blob.Batch), with FFEA's output and checkpoint files.  Blobs use the massless
+CG_nomass solver, or one of the mass-ful solvers (CG, masslumped); all blobs of a script use the
+same one.
 
 Follows World::run's order per step (src/World.cpp:1811-2043):
   1. at check steps, the RngStream state is written to the checkpoint file (before the element
@@ -13,12 +15,16 @@ Steps run from step_initial to num_steps inclusive.
 
 File formats (written as the reference writes them):
   .ftj  header (src/World.cpp:532-548), frames of "Blob b, Conformation 0, step s" + DYNAMIC +
-        one "%e" x 10 line per node (positions in m, then 7 zeros on the massless path,
-        Blob::write_nodes_to_file src/Blob.cpp:1143-1180), "*", the conformation-change block, "*";
-  .fm   "Measurements:" then Time StrainEnergy Centroid.x/y/z RMSD in "%-14.6e" columns
-        (World::write_measurements_to_file, src/World.cpp:4342-4366);
+        one "%e" x 10 line per node (Blob::write_nodes_to_file src/Blob.cpp:1143-1180: positions in
+        m, then 7 zeros on the massless path; with a mass-ful solver velocities in m/s, the
+        potential (0 without electrostatics) and the node force, which World::run has set to zero
+        before the frame is written, src/World.cpp:1907), "*", the conformation-change block, "*";
+  .fm   "Measurements:" then Time [KineticEnergy when a blob has mass] StrainEnergy
+        Centroid.x/y/z RMSD in "%-14.6e" columns (World::write_measurements_to_file,
+        src/World.cpp:4342-4366);
   .fcp  "RNGStreams dedicated to the thermal stress: K" then K lines of six integers.
-Restart (restart = 1, checkpoint_in; src/World.cpp:645-800): positions from the last complete
+Restart (restart = 1, checkpoint_in; src/World.cpp:645-800): positions (and, with a mass-ful
+solver, velocities: Blob::read_nodes_from_file, src/Blob.cpp:1268-1306) from the last complete
 trajectory frame (which is removed and written again), the last measurement line removed,
 RngStream state from checkpoint_in, stepping resumes at that frame's step.
 
@@ -51,13 +57,19 @@ def traj_header(nodes_per_blob) -> str:
     return s + "\n*\n"
 
 
-def traj_frame(step: int, positions_m) -> str:
-    """One frame; positions_m: per blob [N_t, 3] in metres."""
+def traj_frame(step: int, positions_m, velocities=None) -> str:
+    """One frame; positions_m: per blob [N_t, 3] in metres; velocities (mass-ful solvers): per
+    blob [N_t, 3] in m/s, or None for the massless layout."""
     out = []
     for b, p in enumerate(positions_m):
         out.append("Blob %d, Conformation %d, step %d\nDYNAMIC\n" % (b, 0, step))
-        out.append("".join("%e %e %e %e %e %e %e %e %e %e\n" % (x, y, z, 0., 0., 0., 0., 0., 0., 0.)
-                           for x, y, z in np.asarray(p, np.float64)))
+        if velocities is None:
+            out.append("".join("%e %e %e %e %e %e %e %e %e %e\n" % (x, y, z, 0., 0., 0., 0., 0., 0., 0.)
+                               for x, y, z in np.asarray(p, np.float64)))
+        else:
+            pv = np.concatenate([np.asarray(p, np.float64), np.asarray(velocities[b], np.float64)], axis=1)
+            out.append("".join("%e %e %e %e %e %e %e %e %e %e\n" % (x, y, z, u, v, w, 0., 0., 0., 0.)
+                               for x, y, z, u, v, w in pv))
     out.append("*\nConformation Changes:\n")
     out.append("".join("Blob %d: Conformation %d -> Conformation %d\n" % (b, 0, 0) for b in range(len(positions_m))))
     out.append("*\n")
@@ -65,6 +77,7 @@ def traj_frame(step: int, positions_m) -> str:
 
 
 MEAS_COLUMNS = ("Time", "StrainEnergy", "Centroid.x", "Centroid.y", "Centroid.z", "RMSD")
+MEAS_COLUMNS_MASS = ("Time", "KineticEnergy") + MEAS_COLUMNS[1:]
 
 
 def meas_line(values) -> str:
@@ -134,7 +147,7 @@ def read_trajectory(path):
 # the world
 # ---------------------------------------------------------------------------------------------
 class World:
-    """The dynamic CG_nomass blobs of a .ffea script on one GPU (World::init / World::run)."""
+    """The dynamic blobs of a .ffea script on one GPU (World::init / World::run)."""
 
     def __init__(self, script: str, precision: str = "fp64", quiet: bool = True):
         self.script = os.path.abspath(script)
@@ -161,14 +174,20 @@ class World:
         if self.ckpt_in and os.path.abspath(self.ckpt_in) == os.path.abspath(self.ckpt_out):
             raise F.FormatError("checkpoint_in and checkpoint_out must differ")
         self.meshes, self.scales = [], []
+        solvers = set()
         for b in s["blobs"]:
             solver = b.get("solver", "CG_nomass")
-            if solver != "CG_nomass":
-                raise F.FormatError(f"blob solver {solver}: only CG_nomass is on this path")
+            if solver not in _lib.FFH_SOLVERS:
+                raise F.FormatError(f"blob solver {solver}: one of {sorted(_lib.FFH_SOLVERS)} on this path")
+            solvers.add(solver)
             if b.get("motion_state", "DYNAMIC") != "DYNAMIC":
                 raise F.FormatError("only DYNAMIC blobs are on this path")
             self.meshes.append(F.load_blob(b["nodes"], b["topology"], b["material"], b["stokes"], b.get("pin")))
             self.scales.append(U.blob_scale(float(b.get("scale", 1.0))))
+        if len(solvers) > 1:
+            raise F.FormatError("all blobs of a script must use the same solver on this path")
+        self.solver = solvers.pop() if solvers else "CG_nomass"
+        self.mass_in_system = self.solver != "CG_nomass"  # src/World.cpp:2545-2556
         self.nodes = [m.num_nodes for m in self.meshes]
         self.pos0 = [m.pos * sc for m, sc in zip(self.meshes, self.scales)]  # meso units
         if self.restart:
@@ -176,6 +195,7 @@ class World:
         else:
             st = package_seed(self.params.rng_seed)
         self.batch = Batch(self.meshes, self.params, self.scales, rng_state=st, precision=precision)
+        self.batch.set_solver(self.solver)
         self.quiet = quiet
         self.step_initial = 0
         if self.restart:
@@ -189,7 +209,7 @@ class World:
     def _meas_header(self) -> str:
         p = self.raw
         s = "FFEA Global Measurement File\n\nSimulation Details:\n"
-        s += "\tImplementation: ffea_amd (MI355X massless path)\n"
+        s += "\tImplementation: ffea_amd (MI355X)\n"
         s += f"\tScript Filename = {os.path.basename(self.script)}\n\tSimulation Type = Full\n\n"
         s += "Parameters:\n\tSystem parameters:\n"
         for k in ("restart", "dt", "num_steps", "check", "rng_seed", "kT", "max_iterations_cg", "epsilon",
@@ -198,7 +218,8 @@ class World:
                 s += f"\t{k} = {p[k]}\n"
         s += "\n\tCalculations enabled:\n"
         s += f"\tcalc_noise = {self.params.calc_noise}\n\tcalc_stokes = {self.params.calc_stokes}\n\n\n"
-        s += "Measurements:\n" + "".join("%-14s" % c for c in MEAS_COLUMNS) + "\n"
+        cols = MEAS_COLUMNS_MASS if self.mass_in_system else MEAS_COLUMNS
+        s += "Measurements:\n" + "".join("%-14s" % c for c in cols) + "\n"
         return s
 
     def _restart_files(self):
@@ -210,6 +231,8 @@ class World:
             raise F.FormatError(f"{self.traj_path}: frame has {len(blobs)} blobs, script has {len(self.meshes)}")
         pos = np.concatenate([b[:, :3] / U.MESO_LENGTH for b in blobs])  # read_nodes_from_file
         self.batch.set_positions(pos)
+        if self.mass_in_system:
+            self.batch.set_velocities(np.concatenate([b[:, 3:6] / U.MESO_VELOCITY for b in blobs]))
         with open(self.traj_path, "r+b") as f:
             f.truncate(start)
         with open(self.meas_path, "rb") as f:
@@ -235,7 +258,8 @@ class World:
             cog += c * n
             msd += r * r * n
         cog /= tot
-        return (step * self.dt_si, self.batch.strain_energy() * U.MESO_ENERGY, *(cog * U.MESO_LENGTH),
+        ke = (self.batch.kinetic_energy() * U.MESO_ENERGY,) if self.mass_in_system else ()
+        return (step * self.dt_si, *ke, self.batch.strain_energy() * U.MESO_ENERGY, *(cog * U.MESO_LENGTH),
                 np.sqrt(msd / tot) * U.MESO_LENGTH), pos
 
     def run(self):
@@ -249,8 +273,11 @@ class World:
                     f.write(checkpoint_text([b.rng_state()]))
                 b.update_internal_forces()  # an inversion raises: the run stops, as the reference's does
                 vals, pos = self.measurements(step)
+                vel = None
+                if self.mass_in_system:
+                    vel = [v * U.MESO_VELOCITY for v in b.split(b.velocities())]
                 with open(self.traj_path, "a") as f:
-                    f.write(traj_frame(step, [p * U.MESO_LENGTH for p in pos]))
+                    f.write(traj_frame(step, [p * U.MESO_LENGTH for p in pos], vel))
                 with open(self.meas_path, "a") as f:
                     f.write(meas_line(vals))
                 self._positions_phase()
@@ -279,7 +306,7 @@ class World:
 
 def main(argv=None):
     import argparse
-    ap = argparse.ArgumentParser(description="Run a .ffea script's CG_nomass blobs on one GPU")
+    ap = argparse.ArgumentParser(description="Run a .ffea script's dynamic blobs on one GPU")
     ap.add_argument("script")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
     a = ap.parse_args(argv)

@@ -1,5 +1,5 @@
-"""The reference's physics pins for the massless path, restated (its checkers are Python 2) and
-run on the GPU path with the reference's own fixture meshes and parameters.
+"""The reference's physics pins, restated (its checkers are Python 2) and run on the GPU path
+with the reference's own fixture meshes and parameters.
 
 - Equipartition (tests/physics/sphere_63_120_nomass/averages.py:51-55): the mean strain energy of
   measurements 40..end of a 1e5-step run (check = 1000) is within 3 % of (3*63 - 6)/2 kT.
@@ -8,9 +8,23 @@ run on the GPU path with the reference's own fixture meshes and parameters.
   6 kT / (6 pi eta sum(r)) dt check.  The reference analyses frames 5000..10000 of a 1e6-step run;
   here 2000 frames (2e5 steps) after a 500-frame equilibration, same seed and parameters.
 
-Both runs are deterministic (fixed seed, fixed reduction order), so the results do not vary
+- Mass-ful equipartition (tests/physics/sphere_63_120_mass/averages.py:51-74), through the
+  World driver and its measurement file: with solver = CG, measurements 40..end of a 1e5-step run
+  give the kinetic energy within 4 % of 3*63/2 kT and the strain energy within 3 % of
+  (3*63 - 6)/2 kT.
+
+- Inertial diffusion (tests/physics/sphere_diffusion/diffusion_mass.py:49-79): with solver = CG
+  the centroid's mean squared displacement between frames `check` steps apart against
+  3 kT / M (dt check)^2.  The reference analyses frames 5000..10000 of a 2e6-step run; its own
+  objects' restatement (the oracle) gives a relative error of 0.086 there (bar 0.10).  Here 1000
+  frames after 250 frames of equilibration (2.5e5 steps), where the oracle gives 0.102: the GPU
+  run must reproduce the oracle's mean squared displacement within 1e-6 relative, and the
+  theory within 0.15.
+
+The runs are deterministic (fixed seed, fixed reduction order), so the results do not vary
 from run to run.
 """
+import shutil
 import os
 
 import numpy as np
@@ -70,3 +84,68 @@ def test_stokes_diffusion_sphere():
     err = abs(r2mean - r2theory) / r2theory
     print(f"diffusion: <dr^2> = {r2mean:.4e} m^2, theory {r2theory:.4e} m^2, rel err {err:.4f}")
     assert err <= 0.05
+
+
+def fm_columns(path):
+    """readAndAverage's parse of a measurement file (tests/physics/sphere_63_120_mass/
+    averages.py:27-41): column name first, then the values; RESTART lines skipped."""
+    lines = open(path).read().split("\n")
+    k = lines.index("Measurements:")
+    names = lines[k + 1].split()
+    rows = [[float(v) for v in l.split()] for l in lines[k + 2:] if l.strip() and "RESTART" not in l]
+    return {n: [n] + [r[i] for r in rows] for i, n in enumerate(names)}
+
+
+def test_mass_equipartition_sphere_63_120(tmp_path):
+    from ffea_amd.world import World
+    shutil.copytree(os.path.join(GOLDEN, "physics", "sphere_63_120_structure"), tmp_path / "sphere_63_120_structure")
+    run = tmp_path / "sphere_63_120_mass"
+    shutil.copytree(os.path.join(GOLDEN, "physics", "sphere_63_120_mass"), run)
+    w = World(str(run / "sphere_63_120_mass.ffea"))
+    assert w.solver == "CG" and w.mass_in_system
+    w.run()
+    H = fm_columns(run / "sphere_63_120_mass_measurement.out")
+    nodes, kT = 63, 4.11e-21
+    theory = {"KineticEnergy": kT * (3 * nodes) / 2, "StrainEnergy": kT * (3 * nodes - 6) / 2}
+    tol = {"KineticEnergy": 0.04, "StrainEnergy": 0.03}
+    for k in theory:
+        err = abs(np.mean(H[k][40:-1]) / theory[k] - 1)
+        print(f"{k}: rel err {err:.4f} (tolerance {tol[k]})", flush=True)
+        assert err < tol[k], k
+
+
+def test_inertial_diffusion_sphere(oracle):
+    import pyoracle
+    from ffea_amd.blob import Blob, package_seed
+    script = os.path.join(GOLDEN, "physics", "sphere_diffusion", "sphere_mass_norestart.ffea")
+    m, prm, sc, raw = load(script)
+    check, equil, frames = int(float(raw["check"])), 250, 1000
+    b = Blob(m, prm, sc, rng_state=package_seed(prm.rng_seed), precision="fp64")
+    b.set_solver("CG")
+    b.step(equil * check)
+    cents = []
+    for _ in range(frames + 1):
+        cents.append(b.positions().mean(axis=0))  # FFEA_frame.calc_centroid: all nodes
+        b.step(check)
+    d = np.diff(np.array(cents), axis=0)
+    r2 = np.mean(np.sum(d * d, axis=1))
+    # the oracle over the same steps
+    oracle.set_package_seed(package_seed(prm.rng_seed))
+    s = oracle.stream()
+    bo = oracle.blob(m, prm, sc)
+    bo.set_solver("CG")
+    for _ in range(equil * check):
+        pyoracle.world_step(oracle, [bo], s)
+    co = []
+    for _ in range(frames + 1):
+        co.append(bo.get("pos").reshape(-1, 3).mean(axis=0))
+        for _ in range(check):
+            pyoracle.world_step(oracle, [bo], s)
+    do = np.diff(np.array(co), axis=0)
+    r2o = np.mean(np.sum(do * do, axis=1))
+    mass = (bo.get("material")[0::5] * bo.get("vol_0")).sum()
+    theory = 3 * (prm.kT / mass) * (prm.dt * check) ** 2
+    err = abs(r2 - theory) / theory
+    print(f"inertial diffusion: <dr^2> = {r2:.6e}, oracle {r2o:.6e}, theory {theory:.6e}, rel err {err:.4f}")
+    assert abs(r2 - r2o) <= 1e-6 * r2o
+    assert err <= 0.15

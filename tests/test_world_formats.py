@@ -52,3 +52,23 @@ def test_checkpoint_round_trip():
     st = W.read_checkpoint(path)
     assert st.shape == (8, 6) and st.dtype == np.uint64
     assert W.checkpoint_text(st) == open(path).read()
+
+
+def test_mass_frame_layout_and_round_trip(tmp_path):
+    """A mass-ful frame (Blob::write_nodes_to_file, src/Blob.cpp:1154-1161): positions,
+    velocities, potential and force per node; read back by the restart reader."""
+    from ffea_amd import world as W
+    rng = np.random.default_rng(1)
+    p = [rng.normal(size=(4, 3)) * 1e-9, rng.normal(size=(2, 3)) * 1e-9]
+    v = [rng.normal(size=(4, 3)), rng.normal(size=(2, 3))]
+    text = W.traj_header([4, 2]) + W.traj_frame(3, p, v)
+    row = text.split("DYNAMIC\n")[1].split("\n")[0].split()
+    assert len(row) == 10 and row[6:] == ["0.000000e+00"] * 4
+    assert float(row[3]) == float("%e" % v[0][0, 0])
+    path = tmp_path / "m.ftj"
+    path.write_text(text)
+    (step, blobs, _), = W.read_trajectory(str(path))
+    assert step == 3
+    for b, pp, vv in zip(blobs, p, v):
+        assert np.allclose(b[:, :3], pp, rtol=1e-6) and np.allclose(b[:, 3:6], vv, rtol=1e-6)
+    assert W.MEAS_COLUMNS_MASS[:3] == ("Time", "KineticEnergy", "StrainEnergy")
