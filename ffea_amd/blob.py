@@ -121,6 +121,43 @@ class Blob:
             raise ValueError("external forces need 3 values per node")
         _lib.check(self._L.ffh_blob_set_external_forces(self._h, f.ctypes.data_as(C.c_void_p)))
 
+    def set_springs(self, node0, node1, k, l) -> None:
+        """Springs between nodes (N_t ids of this handle), k and l in meso units; empty clears."""
+        n0, n1 = (np.ascontiguousarray(x, np.int32).ravel() for x in (node0, node1))
+        kk, ll = (np.ascontiguousarray(x, np.float64).ravel() for x in (k, l))
+        if not (n0.size == n1.size == kk.size == ll.size):
+            raise ValueError("springs need node0, node1, k and l of one length")
+        z = np.zeros(1, np.int32)
+        _lib.check(self._L.ffh_blob_set_springs(self._h, n0.size, n0 if n0.size else z, n1 if n1.size else z,
+                                                kk if kk.size else np.zeros(1), ll if ll.size else np.zeros(1)))
+
+    def spring_energy(self) -> float:
+        e = C.c_double(0)
+        _lib.check(self._L.ffh_blob_spring_energy(self._h, C.byref(e)))
+        return e.value
+
+    def set_constant_forces(self, nodes, forces) -> None:
+        """Constant forces [n, 3] (meso units) on nodes [n] (N_t ids), in the reference's order."""
+        nd = np.ascontiguousarray(nodes, np.int32).ravel()
+        fr = np.ascontiguousarray(forces, np.float64).ravel()
+        if fr.size != 3 * nd.size:
+            raise ValueError("one 3-vector per node")
+        _lib.check(self._L.ffh_blob_set_constant_forces(self._h, nd.size, nd if nd.size else np.zeros(1, np.int32),
+                                                        fr if fr.size else np.zeros(3)))
+
+    def set_surface_forces(self, faces, sets, set_forces) -> None:
+        """Linear surface forces: faces [F, 3] (vertex ids), sets: lists of face indices,
+        set_forces [S, 3] total force of each set (meso units); empty sets clear."""
+        fc = np.ascontiguousarray(faces, np.int32).ravel()
+        off = np.concatenate([[0], np.cumsum([len(x) for x in sets])]).astype(np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in sets]) if sets else np.zeros(1),
+                                    np.int32)
+        sf = np.ascontiguousarray(set_forces, np.float64).ravel()
+        if sf.size != 3 * len(sets):
+            raise ValueError("one force per set")
+        _lib.check(self._L.ffh_blob_set_surface_forces(self._h, fc.size // 3, fc if fc.size else np.zeros(3, np.int32),
+                                                       len(sets), off, flat, sf if sf.size else np.zeros(3)))
+
     def set_graphs(self, enable: bool) -> None:
         _lib.check(self._L.ffh_blob_set_graphs(self._h, int(bool(enable))))
 

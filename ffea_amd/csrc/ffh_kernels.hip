@@ -27,6 +27,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ffea_hip.h"
@@ -686,6 +687,91 @@ struct CgArgs {
     const double *tot;  // per-blob totals (large blobs) or nullptr: consumers add the partials
 };
 
+// Forces from outside the element loop (ffh::ForcePlan on the device); null lists: none.
+struct ForceArgs {
+    const int32_t *pre_off, *pre_item;  // springs then constant forces, before the element forces
+    const int2 *spring_nodes;
+    const double2 *spring_kl;
+    const double *cforce;
+    const int32_t *grp_off, *grp_ent_off, *grp_ent;  // surface-face groups, after them
+    const double *trac;                               // [3 * entries], this step's tractions
+    const int32_t *fold_off;                          // folded midpoint forces, after the noise
+    const double *fold;
+    const double4 *pos;
+};
+
+// Component i of spring s's force on its end `end` (World::apply_springs,
+// src/World.cpp:3812-3842): direction by arr3Normalise2 (src/mat_vec_fns_II.cpp:164-176, zero
+// for coincident ends), magnitude k (|sep| - l) with mag (:229-239), the far end negated.
+__device__ __forceinline__ double spring_component(const ForceArgs &fa, int s, int end, int i) {
+    const int2 nn = fa.spring_nodes[s];
+    const double4 p0 = fa.pos[nn.x], p1 = fa.pos[nn.y];
+    const double sx = p1.x - p0.x, sy = p1.y - p0.y, sz = p1.z - p0.z;
+    double norm = 0.0;
+    norm += sx * sx;
+    norm += sy * sy;
+    norm += sz * sz;
+    const double c = i == 0 ? sx : (i == 1 ? sy : sz);
+    const double len = sqrt(norm);
+    const double dir = norm == 0.0 ? 0.0 : c / len;
+    const double2 kl = fa.spring_kl[s];
+    const double fm = kl.x * (len - kl.y);
+    return end == 0 ? fm * dir : (-1.0 * fm) * dir;
+}
+
+// Face::get_area (src/Face.cpp:244-261).
+__device__ __forceinline__ double face_area(const double4 *pos, const int32_t *n) {
+    const double4 a0 = pos[n[0]], a1 = pos[n[1]], a2 = pos[n[2]];
+    const double ax = a1.x - a0.x, ay = a1.y - a0.y, az = a1.z - a0.z;
+    const double bx = a2.x - a0.x, by = a2.y - a0.y, bz = a2.z - a0.z;
+    const double tx = ay * bz - az * by, ty = az * bx - ax * bz, tz = ax * by - ay * bx;
+    return .5 * sqrt(tx * tx + ty * ty + tz * tz);
+}
+
+// Linear surface forces (Blob::apply_ctforces CTF 3, src/Blob.cpp:2137-2159): one workgroup
+// per set; each face's share of the set's force is its area over three times the set's total
+// area, summed in the set's face order by one thread.
+struct FaceArgs {
+    const int32_t *set_off, *ent_nodes;
+    const double *set_force;
+    const double4 *pos;
+    double *area, *trac;
+};
+__global__ __launch_bounds__(256) void k_face_tractions(FaceArgs a) {
+    __shared__ double total;
+    const int s = blockIdx.x;
+    const int e0 = a.set_off[s], e1 = a.set_off[s + 1];
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) a.area[e] = face_area(a.pos, a.ent_nodes + 3 * (size_t)e);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int e = e0; e < e1; e++) t += a.area[e];
+        total = t;
+    }
+    __syncthreads();
+    const double t3 = 3 * total;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) {
+        const double f = a.area[e] / t3;
+        for (int k = 0; k < 3; k++) a.trac[3 * (size_t)e + k] = f * a.set_force[3 * (size_t)s + k];
+    }
+}
+
+// Spring energies (World::make_measurements, src/World.cpp:4322-4328), one per spring.
+__global__ __launch_bounds__(256) void k_spring_energy(ForceArgs fa, int n, double *out) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int2 nn = fa.spring_nodes[s];
+    const double4 a = fa.pos[nn.x], b = fa.pos[nn.y];
+    const double cx = a.x - b.x, cy = a.y - b.y, cz = a.z - b.z;
+    double m = 0.0;
+    m += cx * cx;
+    m += cy * cy;
+    m += cz * cz;
+    m = sqrt(m);
+    const double2 kl = fa.spring_kl[s];
+    out[s] = 0.5 * kl.x * (m - kl.y) * (m - kl.y);
+}
+
 struct RhsArgs {
     const void *eforce;  // T SoA [12][E]
     int E;
@@ -704,6 +790,7 @@ struct RhsArgs {
     const double *ext;  // external forces per linear node [3N], midpoints folded; or nullptr
     const double *vel;     // mass-ful solvers with Stokes drag: node velocities [3N]; or nullptr
     const double *lumped;  // MassLumpedSolver: inverse masses [N] (the solve is x = f / m); or nullptr
+    ForceArgs fa;          // springs, constant and surface forces
 };
 
 template <typename T>
@@ -731,6 +818,15 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
     if (g.f_in) {
         f = g.f_in[3 * (size_t)n + i];
     } else if (live) {
+        // springs, then constant forces, onto the zeroed force (World::run: apply_springs, then
+        // update_internal_forces' apply_ctforces)
+        if (g.fa.pre_off) {
+            const int t1 = g.fa.pre_off[n + 1];
+            for (int t = g.fa.pre_off[n]; t < t1; t++) {
+                const int it = g.fa.pre_item[t];
+                f += it >= 0 ? spring_component(g.fa, it >> 1, it & 1, i) : g.fa.cforce[3 * (size_t)(~it) + i];
+            }
+        }
         const T *ef = reinterpret_cast<const T *>(g.eforce);
         const int cnt = g.cnt[n];
         const uint32_t *tl = g.ctb + g.toff[blk] + lane;
@@ -739,8 +835,17 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             const uint32_t code = tl[64 * k];
             f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
         }
-        // surface / external forces come after the element forces and before the Stokes noise
-        // (src/Blob.cpp:3842-3851, then :3895-3898)
+        // surface face forces, face by face, after the element forces (src/Blob.cpp:3842-3851)
+        if (g.fa.grp_off) {
+            const int q1 = g.fa.grp_off[n + 1];
+            for (int q = g.fa.grp_off[n]; q < q1; q++) {
+                double sf = 0.0;
+                const int t1 = g.fa.grp_ent_off[q + 1];
+                for (int t = g.fa.grp_ent_off[q]; t < t1; t++) sf += g.fa.trac[3 * (size_t)g.fa.grp_ent[t] + i];
+                f += sf;
+            }
+        }
+        // external forces come after the element forces and before the Stokes noise
         if (g.ext) f += g.ext[3 * (size_t)n + i];
         // mass-ful solvers: Stokes drag on the node's velocity, before its noise (:3854-3877)
         if (g.vel) f -= g.vel[3 * (size_t)n + i] * g.zeta[n];
@@ -750,6 +855,11 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             for (int k = 0; k < 6; k++) st.s[k] = rs[k][lane];
             for (int k = 0; k < i; k++) (void)rng_u01(st);
             f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
+        }
+        // constant forces on edge midpoints, folded element by element (linearise_force)
+        if (g.fa.fold_off) {
+            const int t1 = g.fa.fold_off[n + 1];
+            for (int t = g.fa.fold_off[n]; t < t1; t++) f += g.fa.fold[3 * (size_t)t + i];
         }
     }
     const size_t k = 3 * (size_t)n + i;
@@ -1213,6 +1323,15 @@ struct ffh_blob {
     int solver = FFH_SOLVER_CG_NOMASS;
     double *mval = nullptr, *mdinv = nullptr, *minv = nullptr, *rho = nullptr;
     std::vector<double> vel_mid;  // [3*N_t] or empty (all zero)
+    // springs, constant and surface forces (ffh_blob_set_springs / _constant_forces /
+    // _surface_forces): the caller's lists, and their per-node form on the device
+    ffh::ForceInput fin;
+    ffh::ForcePlan fplan;
+    std::vector<void *> force_allocs;
+    ForceArgs fdev = {};
+    FaceArgs face = {};
+    int nsprings = 0, nsets = 0;
+    double *spring_e = nullptr;
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
     int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
     bool spmv_nt = false;  // non-temporal matrix loads in the SpMV
@@ -1258,6 +1377,7 @@ struct ffh_blob {
 
     ~ffh_blob() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : force_allocs) (void)hipFree(p);
         for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto &pe : pending) {
             (void)hipEventDestroy(pe.second.first);
@@ -1488,9 +1608,17 @@ struct ffh_blob {
         g.ext = (ext_on && !f_in) ? ext : nullptr;
         g.vel = (massful() && calc_stokes == 1 && !f_in) ? vel : nullptr;
         g.lumped = lumped() ? minv : nullptr;
+        g.fa = fdev;
+        g.fa.pos = pos;
         return g;
     }
     int run_rhs(const double *f_in = nullptr) {
+        if (nsets > 0 && !f_in) {  // this step's surface tractions, from the current positions
+            FaceArgs fa = face;
+            fa.pos = pos;
+            int rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_face_tractions, nsets, 256, 0, stream, e0, e1, 0, fa); });
+            if (rc) return rc;
+        }
         RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1, f_in);
         CgArgs v = cg_args();
         if (learned_solve()) v.parts_uu = parts_uu();
@@ -1917,6 +2045,66 @@ struct ffh_blob {
         std::fill(blob_res.begin(), blob_res.end(), 0.0);
         std::fill(blob_status.begin(), blob_status.end(), 0);
         HIP_OR_FAIL(hipStreamSynchronize(stream));
+        return FFH_OK;
+    }
+    // Rebuilds the per-node force lists from `fin` and uploads them (replacing the previous ones).
+    int upload_forces() {
+        HIP_OR_FAIL(hipStreamSynchronize(stream));
+        ffh::ForcePlan F;
+        std::string msg = ffh::build_forces(plan, fin, F);
+        if (!msg.empty()) return fail(FFH_E_ARG, msg);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);  // the force kernels' arguments change
+        graphs.clear();
+        for (void *q : force_allocs) (void)hipFree(q);
+        force_allocs.clear();
+        fdev = ForceArgs{};
+        face = FaceArgs{};
+        spring_e = nullptr;
+        auto put = [&](auto *&dst, const auto &h) -> int {
+            using X = std::remove_reference_t<decltype(h[0])>;
+            void *q = nullptr;
+            if (hipMalloc(&q, std::max<size_t>(1, h.size()) * sizeof(X)) != hipSuccess) return fail(FFH_E_NOMEM, "hipMalloc (force lists)");
+            force_allocs.push_back(q);
+            if (!h.empty() && hipMemcpy(q, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(FFH_E_HIP, "force list upload");
+            dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(q);
+            return FFH_OK;
+        };
+        int rc = 0;
+        nsprings = (int)fin.spring_n0.size();
+        nsets = F.set_off.empty() ? 0 : (int)F.set_off.size() - 1;
+#define FFH_PUT(dst, h) \
+    if ((rc = put(dst, h))) return rc
+        if (!F.pre_off.empty()) {
+            FFH_PUT(fdev.pre_off, F.pre_off);
+            FFH_PUT(fdev.pre_item, F.pre_item);
+            FFH_PUT(fdev.spring_nodes, F.spring_nodes);
+            FFH_PUT(fdev.spring_kl, F.spring_kl);
+            FFH_PUT(fdev.cforce, F.cforce);
+        }
+        if (nsprings > 0) {
+            std::vector<double> z(nsprings, 0.0);
+            FFH_PUT(spring_e, z);
+        }
+        if (nsets > 0) {
+            FFH_PUT(fdev.grp_off, F.grp_off);
+            FFH_PUT(fdev.grp_ent_off, F.grp_ent_off);
+            FFH_PUT(fdev.grp_ent, F.grp_ent);
+            FFH_PUT(face.set_off, F.set_off);
+            FFH_PUT(face.ent_nodes, F.ent_nodes);
+            FFH_PUT(face.set_force, F.set_force);
+            std::vector<double> z((size_t)std::max(1, F.set_off.back()), 0.0);
+            FFH_PUT(face.area, z);
+            std::vector<double> z3(3 * z.size(), 0.0);
+            FFH_PUT(face.trac, z3);
+            fdev.trac = face.trac;
+        }
+        if (!F.fold_off.empty()) {
+            FFH_PUT(fdev.fold_off, F.fold_off);
+            FFH_PUT(fdev.fold, F.fold);
+        }
+#undef FFH_PUT
+        fplan = std::move(F);
         return FFH_OK;
     }
     int reset_inversions() {  // a kernel, not a memset: it is also recorded into step graphs
@@ -2504,6 +2692,89 @@ int ffh_blob_set_velocities(ffh_blob *b, const double *vel) {
     HIP_OR_FAIL(hipMemcpyAsync(b->vel, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return FFH_OK;
+}
+
+int ffh_blob_set_springs(ffh_blob *b, int num_springs, const int *node0, const int *node1, const double *k,
+                         const double *l) {
+    if (!b || num_springs < 0 || (num_springs > 0 && (!node0 || !node1 || !k || !l))) return fail(FFH_E_ARG, "null argument");
+    for (int i = 0; i < num_springs; i++)
+        if (!(k[i] >= 0) || !(l[i] >= 0) || !std::isfinite(k[i]) || !std::isfinite(l[i]))
+            return fail(FFH_E_ARG, "spring " + std::to_string(i) + ": k and l must be finite and >= 0");
+    ffh::ForceInput keep = b->fin;
+    b->fin.spring_n0.assign(node0, node0 + num_springs);
+    b->fin.spring_n1.assign(node1, node1 + num_springs);
+    b->fin.spring_k.assign(k, k + num_springs);
+    b->fin.spring_l.assign(l, l + num_springs);
+    int rc = b->upload_forces();
+    if (rc) {
+        b->fin = keep;
+        (void)b->upload_forces();
+    }
+    return rc;
+}
+
+int ffh_blob_spring_energy(ffh_blob *b, double *energy) {
+    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
+    *energy = 0.0;
+    if (b->nsprings == 0) return FFH_OK;
+    ForceArgs fa = b->fdev;
+    fa.pos = b->pos;
+    hipLaunchKernelGGL(k_spring_energy, (b->nsprings + 255) / 256, 256, 0, b->stream, fa, b->nsprings, b->spring_e);
+    HIP_OR_FAIL(hipGetLastError());
+    std::vector<double> e(b->nsprings);
+    HIP_OR_FAIL(hipMemcpyAsync(e.data(), b->spring_e, e.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    // springfieldenergy[blob0][blob1] in spring order, then summed over the pairs
+    const int nb = b->nblobs;
+    std::vector<double> pair((size_t)nb * nb, 0.0);
+    for (int s = 0; s < b->nsprings; s++)
+        pair[(size_t)b->fplan.spring_blob[2 * s] * nb + b->fplan.spring_blob[2 * s + 1]] += e[s];
+    double tot = 0.0;
+    for (double v : pair) tot += v;
+    *energy = tot;
+    return FFH_OK;
+}
+
+int ffh_blob_set_constant_forces(ffh_blob *b, int num_forces, const int *node, const double *force) {
+    if (!b || num_forces < 0 || (num_forces > 0 && (!node || !force))) return fail(FFH_E_ARG, "null argument");
+    for (int i = 0; i < 3 * num_forces; i++)
+        if (!std::isfinite(force[i])) return fail(FFH_E_ARG, "non-finite constant force");
+    ffh::ForceInput keep = b->fin;
+    b->fin.const_node.assign(node, node + num_forces);
+    b->fin.const_force.assign(force, force + 3 * (size_t)num_forces);
+    int rc = b->upload_forces();
+    if (rc) {
+        b->fin = keep;
+        (void)b->upload_forces();
+    }
+    return rc;
+}
+
+int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_nodes, int num_sets, const int *set_off,
+                                const int *set_faces, const double *set_force) {
+    if (!b || num_faces < 0 || num_sets < 0 || (num_sets > 0 && (!face_nodes || !set_off || !set_faces || !set_force)))
+        return fail(FFH_E_ARG, "null argument");
+    ffh::ForceInput keep = b->fin;
+    if (num_sets == 0) {
+        b->fin.faces.clear();
+        b->fin.set_off.clear();
+        b->fin.set_faces.clear();
+        b->fin.set_force.clear();
+    } else {
+        if (set_off[0] != 0 || set_off[num_sets] < 0) return fail(FFH_E_ARG, "inconsistent surface set offsets");
+        for (int i = 0; i < 3 * num_sets; i++)
+            if (!std::isfinite(set_force[i])) return fail(FFH_E_ARG, "non-finite surface force");
+        b->fin.faces.assign(face_nodes, face_nodes + 3 * (size_t)num_faces);
+        b->fin.set_off.assign(set_off, set_off + num_sets + 1);
+        b->fin.set_faces.assign(set_faces, set_faces + set_off[num_sets]);
+        b->fin.set_force.assign(set_force, set_force + 3 * (size_t)num_sets);
+    }
+    int rc = b->upload_forces();
+    if (rc) {
+        b->fin = keep;
+        (void)b->upload_forces();
+    }
+    return rc;
 }
 
 int ffh_blob_set_solver(ffh_blob *b, int solver) {

@@ -495,4 +495,119 @@ std::string build_mass(const Plan &P, int solver, std::vector<double> &slot_val,
     return "";
 }
 
+std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
+    const int N = P.num_linear, NT = P.num_nodes_total, E = P.num_elements;
+    F = ForcePlan();
+    const size_t ns = in.spring_n0.size(), nc = in.const_node.size(), nf = in.faces.size() / 3;
+    const size_t nset = in.set_off.empty() ? 0 : in.set_off.size() - 1;
+    if (in.spring_n1.size() != ns || in.spring_k.size() != ns || in.spring_l.size() != ns ||
+        in.const_force.size() != 3 * nc || in.faces.size() != 3 * nf || in.set_force.size() != 3 * nset)
+        return "inconsistent force list sizes";
+    auto vertex = [&](int32_t v, int &c) -> std::string {
+        if (v < 0 || v >= NT) return "node index " + std::to_string(v) + " out of range";
+        c = P.lin_of[v];
+        if (c < 0) return "node " + std::to_string(v) + " is an edge midpoint";
+        return "";
+    };
+    std::vector<std::vector<int32_t>> pre(N);
+    F.spring_nodes.resize(2 * ns);
+    F.spring_kl.resize(2 * ns);
+    F.spring_blob.resize(2 * ns);
+    for (size_t s = 0; s < ns; s++) {
+        int c0 = -1, c1 = -1;
+        std::string m = vertex(in.spring_n0[s], c0);
+        if (m.empty()) m = vertex(in.spring_n1[s], c1);
+        if (!m.empty()) return "spring " + std::to_string(s) + ": " + m;
+        if (c0 == c1) return "spring " + std::to_string(s) + " joins a node to itself";
+        F.spring_nodes[2 * s] = c0;
+        F.spring_nodes[2 * s + 1] = c1;
+        F.spring_kl[2 * s] = in.spring_k[s];
+        F.spring_kl[2 * s + 1] = in.spring_l[s];
+        F.spring_blob[2 * s] = P.slice_blob[c0 / kSlice];
+        F.spring_blob[2 * s + 1] = P.slice_blob[c1 / kSlice];
+        pre[c0].push_back((int32_t)(2 * s));
+        pre[c1].push_back((int32_t)(2 * s + 1));
+    }
+    // constant forces: on vertices into the pre lists, on midpoints accumulated for folding
+    std::vector<double> fm;
+    for (size_t i = 0; i < nc; i++) {
+        const int32_t v = in.const_node[i];
+        if (v < 0 || v >= NT) return "constant force " + std::to_string(i) + ": node out of range";
+        const int c = P.lin_of[v];
+        if (c >= 0) {
+            pre[c].push_back(~(int32_t)(F.cforce.size() / 3));
+            for (int k = 0; k < 3; k++) F.cforce.push_back(in.const_force[3 * i + k]);
+        } else {
+            if (fm.empty()) fm.assign(3 * (size_t)NT, 0.0);
+            for (int k = 0; k < 3; k++) fm[3 * (size_t)v + k] += in.const_force[3 * i + k];
+        }
+    }
+    if (ns || !F.cforce.empty()) {
+        F.pre_off.assign((size_t)N + 1, 0);
+        for (int i = 0; i < N; i++) F.pre_off[i + 1] = F.pre_off[i] + (int32_t)pre[i].size();
+        for (int i = 0; i < N; i++) F.pre_item.insert(F.pre_item.end(), pre[i].begin(), pre[i].end());
+    }
+    // surface sets
+    if (nset) {
+        F.set_off = in.set_off;
+        F.set_force = in.set_force;
+        if (F.set_off[0] != 0 || (size_t)F.set_off[nset] != in.set_faces.size()) return "inconsistent surface set offsets";
+        for (size_t s = 0; s < nset; s++)
+            if (F.set_off[s + 1] < F.set_off[s]) return "inconsistent surface set offsets";
+        const size_t nent = in.set_faces.size();
+        std::vector<std::vector<int32_t>> ent_of_face(nf);
+        F.ent_nodes.resize(3 * nent);
+        for (size_t e = 0; e < nent; e++) {
+            const int32_t f = in.set_faces[e];
+            if (f < 0 || (size_t)f >= nf) return "surface set face index out of range";
+            ent_of_face[f].push_back((int32_t)e);
+            for (int k = 0; k < 3; k++) {
+                int c = -1;
+                std::string m = vertex(in.faces[3 * (size_t)f + k], c);
+                if (!m.empty()) return "face " + std::to_string(f) + ": " + m;
+                F.ent_nodes[3 * e + k] = c;
+            }
+        }
+        std::vector<std::vector<int32_t>> grp(N);  // per node: face ids, ascending
+        for (size_t f = 0; f < nf; f++) {
+            if (ent_of_face[f].empty()) continue;
+            for (int k = 0; k < 3; k++) grp[P.lin_of[in.faces[3 * f + k]]].push_back((int32_t)f);
+        }
+        F.grp_off.assign((size_t)N + 1, 0);
+        F.grp_ent_off.assign(1, 0);
+        for (int i = 0; i < N; i++) {
+            for (int32_t f : grp[i]) {
+                F.grp_ent.insert(F.grp_ent.end(), ent_of_face[f].begin(), ent_of_face[f].end());
+                F.grp_ent_off.push_back((int32_t)F.grp_ent.size());
+            }
+            F.grp_off[i + 1] = (int32_t)(F.grp_ent_off.size() - 1);
+        }
+    }
+    // midpoint constant forces folded element by element
+    if (!fm.empty()) {
+        static const int sets[4][3] = {{0, 1, 2}, {0, 3, 4}, {1, 3, 5}, {2, 4, 5}};
+        std::vector<std::vector<double>> terms(N);
+        for (int e = 0; e < E; e++) {
+            const int32_t *m = &P.emid[6 * (size_t)e];
+            for (int a = 0; a < 4; a++) {
+                const double *x = &fm[3 * (size_t)m[sets[a][0]]], *y = &fm[3 * (size_t)m[sets[a][1]]],
+                             *z = &fm[3 * (size_t)m[sets[a][2]]];
+                bool any = false;
+                for (int k = 0; k < 3; k++) any |= x[k] != 0.0 || y[k] != 0.0 || z[k] != 0.0;
+                if (!any) continue;
+                auto &t = terms[P.enodes[4 * (size_t)e + a]];
+                for (int k = 0; k < 3; k++) t.push_back(0.5 * (x[k] + y[k] + z[k]));
+            }
+            for (int k = 0; k < 6; k++)
+                for (int j = 0; j < 3; j++) fm[3 * (size_t)m[k] + j] = 0.0;
+        }
+        F.fold_off.assign((size_t)N + 1, 0);
+        for (int i = 0; i < N; i++) {
+            F.fold_off[i + 1] = F.fold_off[i] + (int32_t)(terms[i].size() / 3);
+            F.fold.insert(F.fold.end(), terms[i].begin(), terms[i].end());
+        }
+    }
+    return "";
+}
+
 }  // namespace ffh

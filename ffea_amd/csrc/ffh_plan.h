@@ -114,6 +114,53 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
 std::string build_mass(const Plan &P, int solver, std::vector<double> &slot_val, std::vector<double> &dinv3,
                        std::vector<double> &inv_mass);
 
+// Forces from outside the element loop, as the caller gives them (node ids in the N_t
+// numbering of the handle: a batch's blobs concatenated; meso units).
+struct ForceInput {
+    // World::apply_springs (src/World.cpp:3808-3846)
+    std::vector<int32_t> spring_n0, spring_n1;
+    std::vector<double> spring_k, spring_l;
+    // Blob::apply_ctforces CTF 1 (src/Blob.cpp:2096-2103), in the stored order
+    std::vector<int32_t> const_node;
+    std::vector<double> const_force;  // [3C]
+    // Blob::apply_ctforces CTF 3 (src/Blob.cpp:2137-2159): surface faces (Face::n[0..2]) and
+    // sets of face indices with their total force
+    std::vector<int32_t> faces;       // [3F]
+    std::vector<int32_t> set_off;     // [S+1]
+    std::vector<int32_t> set_faces;
+    std::vector<double> set_force;    // [3S]
+};
+
+// The same forces as per-node lists in the order the reference adds them to a node's force:
+//  1. before the element forces: the node's springs (ascending spring index), then its
+//     constant forces (in stored order) -- World::run applies the springs, then
+//     update_internal_forces the constant forces, onto the zeroed force vector;
+//  2. after the element forces: one group per surface face holding the node (ascending face
+//     index), each the sum of the face's set tractions in set order (Face::force, gathered in
+//     aggregate_forces_and_solve, src/Blob.cpp:3842-3851);
+//  3. after the noise: constant forces on edge midpoints, folded onto the edge ends element by
+//     element as linearise_force does (src/Blob.cpp:1066-1085).
+struct ForcePlan {
+    std::vector<int32_t> pre_off;    // [N+1]
+    std::vector<int32_t> pre_item;   // >= 0: 2 * spring + end; < 0: ~(constant index)
+    std::vector<int32_t> spring_nodes;  // [2 * springs] compact ids
+    std::vector<double> spring_kl;      // [2 * springs]
+    std::vector<double> cforce;         // [3 * constants on vertices]
+    std::vector<int32_t> grp_off;       // [N+1]
+    std::vector<int32_t> grp_ent_off;   // [groups + 1]
+    std::vector<int32_t> grp_ent;       // set-entry ids
+    std::vector<int32_t> set_off;       // [S+1] entries of each set
+    std::vector<int32_t> ent_nodes;     // [3 * entries] compact ids of the entry's face
+    std::vector<double> set_force;      // [3S]
+    std::vector<int32_t> fold_off;      // [N+1]
+    std::vector<double> fold;           // [3 * terms]
+    std::vector<int32_t> spring_blob;   // [2 * springs] blobs of the two ends (energy grouping)
+};
+
+// Returns "" or an error message (ids out of range, a spring end or face node on an edge
+// midpoint, a spring joining a node to itself).
+std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F);
+
 // Exact jump matrices of MRG32k3a component c (0: m1, 1: m2) raised to n.
 Mat3u mrg_pow(int comp, uint64_t n);
 

@@ -174,6 +174,36 @@ int ffh_blob_solve(ffh_blob *b, double *x);
 int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result);
 
 /*
+ * Forces from outside the element loop, computed on the device every step from the current
+ * positions and added where the reference adds them.  Node ids are in the handle's N_t
+ * numbering (a batch: its blobs' nodes concatenated), forces in meso units.  Each call
+ * replaces that kind of force; a count of 0 clears it.
+ *
+ * ffh_blob_set_springs: World::apply_springs (src/World.cpp:3808-3846) -- spring i pulls its
+ *   ends node0[i], node1[i] (vertices, possibly in different blobs of a batch) with k (|d| - l)
+ *   along their separation d; k and l as World::load_springs converts them (:3509-3510).  Added
+ *   to the zeroed force vector before the constant and element forces.
+ * ffh_blob_spring_energy: World::make_measurements' spring energy (:4316-4329): sum of
+ *   k (|d| - l)^2 / 2, summed per blob pair in spring order, then over the pairs.
+ * ffh_blob_set_constant_forces: Blob::apply_ctforces' linear forces (src/Blob.cpp:2096-2103),
+ *   force[3i..3i+2] on node[i] in the order Blob::load_ctforces stores them (:3184-3241);
+ *   added after the springs, before the element forces.  On an edge midpoint the force reaches
+ *   the edge ends as linearise_force passes it on (:1066-1085).
+ * ffh_blob_set_surface_forces: Blob::apply_ctforces' linear surface forces (:2137-2159) --
+ *   face_nodes [3*num_faces] are the surface faces' vertices (Face::n[0..2]); set s is the
+ *   faces set_faces[set_off[s] .. set_off[s+1]) with total force set_force[3s..3s+2], shared
+ *   over its faces in proportion to their current areas, a third to each vertex; face forces
+ *   are added face by face after the element forces (:3842-3851).
+ * Rotational constant forces (CTF 2) and the interaction solvers are not on this path.
+ */
+int ffh_blob_set_springs(ffh_blob *b, int num_springs, const int *node0, const int *node1, const double *k,
+                         const double *l);
+int ffh_blob_spring_energy(ffh_blob *b, double *energy);
+int ffh_blob_set_constant_forces(ffh_blob *b, int num_forces, const int *node, const double *force);
+int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_nodes, int num_sets, const int *set_off,
+                                const int *set_faces, const double *set_force);
+
+/*
  * Mass-ful solvers: Blob::init's solver choice (src/Blob.cpp:358-373) with the codes of
  * include/FFEA_return_codes.h:30-33.  FFH_SOLVER_CG is ConjugateGradientSolver on the
  * consistent mass matrix (src/ConjugateGradientSolver.cpp:62-230), FFH_SOLVER_MASSLUMPED is

@@ -476,6 +476,16 @@ typedef struct {
     int *mkey, *mcol;      /* mass matrix, scalar CSR over N_t rows */
     double *mval, *mprec;  /* its values and Jacobi preconditioner */
     double *inv_M;         /* lumped inverse masses [N_t] */
+    /* constant forces (Blob::apply_ctforces, src/Blob.cpp:2094-2162): linear (CTF 1) and
+     * linear surface (CTF 3) */
+    int n_ctf;
+    int *ctf_node;
+    double *ctf_force;     /* [3 n_ctf] */
+    int n_faces, n_sets;
+    int *face_nodes;       /* [3 n_faces] */
+    int *set_off, *set_faces;
+    double *set_force;     /* [3 n_sets] */
+    double *face_force;    /* [9 n_faces]: the three face nodes' forces (Face::force[0..2]) */
 } orc_blob;
 
 static void *xcalloc(size_t n, size_t s) {
@@ -698,7 +708,9 @@ void orc_blob_destroy(orc_blob *b) {
                     b->topo, b->rho, b->A, b->B, b->G, b->E, b->mass, b->J_inv_0, b->vol_0, b->vol,
                     b->last_det, b->dpsi, b->F_ij, b->visc, b->node_force, b->stress_mag, b->contrib_off,
                     b->contrib, b->key, b->col, b->src_off, (void *)b->src, b->val, b->diag_inv, b->diag_idx,
-                    b->r, b->p, b->z, b->q, b->f, b->mkey, b->mcol, b->mval, b->mprec, b->inv_M};
+                    b->r, b->p, b->z, b->q, b->f, b->mkey, b->mcol, b->mval, b->mprec, b->inv_M,
+                    b->ctf_node, b->ctf_force, b->face_nodes, b->set_off, b->set_faces, b->set_force,
+                    b->face_force};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(b);
 }
@@ -712,11 +724,168 @@ void orc_blob_set_positions(orc_blob *b, const double *pos) {
 static void blob_zero_forces(orc_blob *b) {
     memset(b->node_force, 0, sizeof(double) * 30 * (size_t)b->num_elements);
     memset(b->force, 0, sizeof(double) * 3 * (size_t)b->num_nodes);
+    if (b->face_force) memset(b->face_force, 0, sizeof(double) * 9 * (size_t)b->n_faces);
+}
+
+/* ---- forces from outside the element loop ----------------------------------- */
+
+/* Linear constant forces, [3n] meso units, in the order Blob::load_ctforces stores them
+ * (src/Blob.cpp:3184-3241).  n = 0 clears. */
+void orc_blob_set_constant_forces(orc_blob *b, int n, const int *node, const double *force) {
+    free(b->ctf_node);
+    free(b->ctf_force);
+    b->n_ctf = n;
+    b->ctf_node = xcalloc(n > 0 ? n : 1, sizeof(int));
+    b->ctf_force = xcalloc(3 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    if (n > 0) {
+        memcpy(b->ctf_node, node, sizeof(int) * n);
+        memcpy(b->ctf_force, force, sizeof(double) * 3 * n);
+    }
+}
+
+/* Linear surface forces (src/Blob.cpp:3350-3424): the blob's surface faces [3 n_faces] (node
+ * ids of Face::n[0..2]) and n_sets sets of face indices with their total force [3 n_sets]. */
+void orc_blob_set_surface_forces(orc_blob *b, int n_faces, const int *faces, int n_sets, const int *set_off,
+                                 const int *set_faces, const double *set_force) {
+    free(b->face_nodes);
+    free(b->set_off);
+    free(b->set_faces);
+    free(b->set_force);
+    free(b->face_force);
+    b->n_faces = n_faces;
+    b->n_sets = n_sets;
+    b->face_nodes = xcalloc(3 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(int));
+    b->set_off = xcalloc((size_t)n_sets + 1, sizeof(int));
+    b->set_faces = xcalloc(n_sets > 0 && set_off[n_sets] > 0 ? set_off[n_sets] : 1, sizeof(int));
+    b->set_force = xcalloc(3 * (size_t)(n_sets > 0 ? n_sets : 1), sizeof(double));
+    b->face_force = xcalloc(9 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
+    if (n_faces > 0) memcpy(b->face_nodes, faces, sizeof(int) * 3 * n_faces);
+    if (n_sets > 0) {
+        memcpy(b->set_off, set_off, sizeof(int) * (n_sets + 1));
+        memcpy(b->set_faces, set_faces, sizeof(int) * set_off[n_sets]);
+        memcpy(b->set_force, set_force, sizeof(double) * 3 * n_sets);
+    }
+}
+
+/* Face::get_area (src/Face.cpp:244-261). */
+static double face_area(const double *pos, const int *n) {
+    double a[3], c[3], t[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = pos[3 * n[1] + k] - pos[3 * n[0] + k];
+        c[k] = pos[3 * n[2] + k] - pos[3 * n[0] + k];
+    }
+    t[0] = a[1] * c[2] - a[2] * c[1];
+    t[1] = a[2] * c[0] - a[0] * c[2];
+    t[2] = a[0] * c[1] - a[1] * c[0];
+    return .5 * sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+}
+
+/* Blob::apply_ctforces, CTF 1 and CTF 3 (src/Blob.cpp:2094-2162): the linear forces go onto
+ * the blob's force vector, the surface tractions onto the faces' node forces; rotational
+ * forces (CTF 2) are not restated. */
+static void blob_apply_ctforces(orc_blob *b) {
+    for (int i = 0; i < b->n_ctf; i++)
+        for (int k = 0; k < 3; k++) b->force[3 * b->ctf_node[i] + k] += b->ctf_force[3 * i + k];
+    for (int s = 0; s < b->n_sets; s++) {
+        int j0 = b->set_off[s], j1 = b->set_off[s + 1];
+        double *areas = xcalloc(j1 > j0 ? j1 - j0 : 1, sizeof(double));
+        double total = 0;
+        for (int j = j0; j < j1; j++) {
+            areas[j - j0] = face_area(b->pos, &b->face_nodes[3 * b->set_faces[j]]);
+            total += areas[j - j0];
+        }
+        for (int j = j0; j < j1; j++) {
+            double t[3], f = areas[j - j0] / (3 * total);
+            for (int k = 0; k < 3; k++) t[k] = f * b->set_force[3 * s + k];
+            double *ff = &b->face_force[9 * b->set_faces[j]];
+            for (int v = 0; v < 3; v++)
+                for (int k = 0; k < 3; k++) ff[3 * v + k] += t[k];
+        }
+        free(areas);
+    }
+}
+
+/* Springs of World::apply_springs (src/World.cpp:3808-3846) over the blobs of one World:
+ * spring i joins node node0[i] of blob blob0[i] to node node1[i] of blob blob1[i]; k and l in
+ * meso units (World::load_springs, src/World.cpp:3509-3510).  n = 0 clears. */
+static int g_nspr;
+static int *g_spr_b0, *g_spr_n0, *g_spr_b1, *g_spr_n1;
+static double *g_spr_k, *g_spr_l;
+void orc_world_set_springs(int n, const int *blob0, const int *node0, const int *blob1, const int *node1,
+                           const double *k, const double *l) {
+    free(g_spr_b0);
+    free(g_spr_n0);
+    free(g_spr_b1);
+    free(g_spr_n1);
+    free(g_spr_k);
+    free(g_spr_l);
+    g_nspr = n;
+    size_t m = n > 0 ? n : 1;
+    g_spr_b0 = xcalloc(m, sizeof(int));
+    g_spr_n0 = xcalloc(m, sizeof(int));
+    g_spr_b1 = xcalloc(m, sizeof(int));
+    g_spr_n1 = xcalloc(m, sizeof(int));
+    g_spr_k = xcalloc(m, sizeof(double));
+    g_spr_l = xcalloc(m, sizeof(double));
+    if (n > 0) {
+        memcpy(g_spr_b0, blob0, sizeof(int) * n);
+        memcpy(g_spr_n0, node0, sizeof(int) * n);
+        memcpy(g_spr_b1, blob1, sizeof(int) * n);
+        memcpy(g_spr_n1, node1, sizeof(int) * n);
+        memcpy(g_spr_k, k, sizeof(double) * n);
+        memcpy(g_spr_l, l, sizeof(double) * n);
+    }
+}
+
+/* mag (src/mat_vec_fns_II.cpp:229-239). */
+static double mag3(const double v[3]) {
+    double s = 0.0;
+    for (int i = 0; i < 3; i++) s += v[i] * v[i];
+    return sqrt(s);
+}
+
+static void world_apply_springs(orc_blob **blobs) {
+    for (int i = 0; i < g_nspr; i++) {
+        const double *p0 = &blobs[g_spr_b0[i]]->pos[3 * g_spr_n0[i]], *p1 = &blobs[g_spr_b1[i]]->pos[3 * g_spr_n1[i]];
+        double sep[3], sn[3], norm = 0.0;
+        for (int k = 0; k < 3; k++) sep[k] = p1[k] - p0[k];
+        /* arr3Normalise2 (src/mat_vec_fns_II.cpp:164-176): a zero separation gives a zero direction */
+        for (int k = 0; k < 3; k++) norm += sep[k] * sep[k];
+        if (norm == 0.0) {
+            sn[0] = sn[1] = sn[2] = 0.0;
+        } else {
+            norm = sqrt(norm);
+            for (int k = 0; k < 3; k++) sn[k] = sep[k] / norm;
+        }
+        double fm = g_spr_k[i] * (mag3(sep) - g_spr_l[i]);
+        double *f0 = &blobs[g_spr_b0[i]]->force[3 * g_spr_n0[i]], *f1 = &blobs[g_spr_b1[i]]->force[3 * g_spr_n1[i]];
+        for (int k = 0; k < 3; k++) f0[k] += fm * sn[k];
+        for (int k = 0; k < 3; k++) f1[k] += -1 * fm * sn[k];
+    }
+}
+
+/* Spring energy of World::make_measurements (src/World.cpp:4316-4329) and
+ * get_spring_field_energy(-1, -1) (:3848-3858): summed per (blob0, blob1) pair in spring
+ * order, then over the pairs. */
+double orc_world_spring_energy(orc_blob **blobs, int nblobs) {
+    double *e = xcalloc((size_t)nblobs * nblobs, sizeof(double));
+    for (int i = 0; i < g_nspr; i++) {
+        const double *a = &blobs[g_spr_b0[i]]->pos[3 * g_spr_n0[i]], *c = &blobs[g_spr_b1[i]]->pos[3 * g_spr_n1[i]];
+        double d[3];
+        for (int k = 0; k < 3; k++) d[k] = a[k] - c[k];
+        double m = mag3(d);
+        e[g_spr_b0[i] * nblobs + g_spr_b1[i]] += 0.5 * g_spr_k[i] * (m - g_spr_l[i]) * (m - g_spr_l[i]);
+    }
+    double tot = 0.0;
+    for (int i = 0; i < nblobs * nblobs; i++) tot += e[i];
+    free(e);
+    return tot;
 }
 
 /* Blob::update_internal_forces (src/Blob.cpp:638-735), single thread. Returns #inversions. */
 int orc_blob_update_internal_forces(orc_blob *b, orc_stream *g) {
     int inversions = 0;
+    blob_apply_ctforces(b); /* src/Blob.cpp:647 */
     for (int n = 0; n < b->num_elements; n++) {
         m3 J, stress, F;
         double du[12];
@@ -847,6 +1016,11 @@ static int blob_aggregate_forces(orc_blob *b, orc_stream *g) {
             b->force[3 * n + 1] += c[1];
             b->force[3 * n + 2] += c[2];
         }
+    /* surface face forces (src/Blob.cpp:3842-3851), face by face; the fourth node's slot only
+     * carries interaction forces, zero here */
+    for (int n = 0; n < b->n_faces; n++)
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) b->force[3 * b->face_nodes[3 * n + i] + k] += b->face_force[9 * n + 3 * i + k];
     if (b->calc_stokes == 1 && b->linear_solver != 3) {
         /* mass-ful branch (src/Blob.cpp:3854-3877): drag on the node velocity, then noise */
         for (int i = 0; i < N; i++) {
@@ -1117,6 +1291,7 @@ int orc_blob_update_positions(orc_blob *b, orc_stream *g) {
 int orc_world_step(orc_blob **blobs, int nblobs, orc_stream *g) {
     int inv = 0, err = 0;
     for (int i = 0; i < nblobs; i++) blob_zero_forces(blobs[i]);
+    world_apply_springs(blobs); /* src/World.cpp:1933 */
     for (int i = 0; i < nblobs; i++) inv += orc_blob_update_internal_forces(blobs[i], g);
     if (inv > 0) return inv;
     for (int i = 0; i < nblobs; i++)

@@ -140,6 +140,23 @@ class Blob:
     def kinetic_energy(self) -> float:
         return self.lib.fn("blob_kinetic_energy")(self.h)
 
+    def set_constant_forces(self, nodes, forces):
+        """Linear constant forces (oracle only): nodes [n], forces [n, 3] meso units."""
+        nodes = np.ascontiguousarray(nodes, np.int32).ravel()
+        forces = np.ascontiguousarray(forces, np.float64).ravel()
+        self.lib.lib.orc_blob_set_constant_forces(self.h, nodes.size, nodes if nodes.size else np.zeros(1, np.int32),
+                                                  forces if forces.size else np.zeros(3))
+
+    def set_surface_forces(self, faces, sets, set_forces):
+        """Linear surface forces (oracle only): faces [F, 3] node ids; sets: list of face-index
+        lists; set_forces [S, 3] meso units."""
+        faces = np.ascontiguousarray(faces, np.int32).ravel()
+        off = np.concatenate([[0], np.cumsum([len(x) for x in sets])]).astype(np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in sets]) if sets else np.zeros(1), np.int32)
+        sf = np.ascontiguousarray(set_forces, np.float64).ravel()
+        self.lib.lib.orc_blob_set_surface_forces(self.h, faces.size // 3, faces, len(sets), off, flat,
+                                                 sf if sf.size else np.zeros(3))
+
     def get(self, what: str) -> np.ndarray:
         sizes = {"pos": 3 * self.N, "vel": 3 * self.N, "force": 3 * self.N, "rhs": 3 * self.N,
                  "dpsi": 12 * self.E, "vol": self.E, "vol_0": self.E, "J_inv_0": 9 * self.E,
@@ -208,6 +225,26 @@ class Oracle(_Lib):
         L.orc_blob_strain_energy.restype = C.c_double
         L.orc_blob_last_cg_iterations.argtypes = [C.c_void_p]
         L.orc_blob_last_cg_iterations.restype = C.c_int
+        L.orc_blob_set_constant_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.orc_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
+        L.orc_world_set_springs.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
+        L.orc_world_spring_energy.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+        L.orc_world_spring_energy.restype = C.c_double
+
+    def set_springs(self, blob0, node0, blob1, node1, k, l):
+        """World springs over the blobs of the next world_step calls (meso units)."""
+        a = [np.ascontiguousarray(x, np.int32).ravel() for x in (blob0, node0, blob1, node1)]
+        kk, ll = (np.ascontiguousarray(x, np.float64).ravel() for x in (k, l))
+        n = a[0].size
+        if n == 0:
+            z = np.zeros(1, np.int32)
+            self.lib.orc_world_set_springs(0, z, z, z, z, np.zeros(1), np.zeros(1))
+        else:
+            self.lib.orc_world_set_springs(n, *a, kk, ll)
+
+    def spring_energy(self, blobs) -> float:
+        arr = (C.c_void_p * len(blobs))(*[b.h for b in blobs])
+        return self.lib.orc_world_spring_energy(arr, len(blobs))
 
     def stream(self) -> "OracleStream":
         return OracleStream(self)
