@@ -695,10 +695,39 @@ struct ForceArgs {
     const double *cforce;
     const int32_t *grp_off, *grp_ent_off, *grp_ent;  // surface-face groups, after them
     const double *trac;                               // [3 * entries], this step's tractions
-    const int32_t *fold_off;                          // folded midpoint forces, after the noise
-    const double *fold;
+    // midpoints carrying a force: their constant items and face groups, this step's totals,
+    // and per vertex the fold terms (three midpoint indices each, -1 for none)
+    const int32_t *mid_pre_off, *mid_grp_off;
+    double *mf;
+    const int32_t *fold_off, *fold_mid;
     const double4 *pos;
 };
+
+// The sum of a node's face groups for component i: each group is one face's force (its set
+// tractions summed in set order from zero), added face by face.
+__device__ __forceinline__ double face_groups(const ForceArgs &fa, int q0, int q1, int i, double f) {
+    for (int q = q0; q < q1; q++) {
+        double sf = 0.0;
+        const int t1 = fa.grp_ent_off[q + 1];
+        for (int t = fa.grp_ent_off[q]; t < t1; t++) sf += fa.trac[3 * (size_t)fa.grp_ent[t] + i];
+        f += sf;
+    }
+    return f;
+}
+
+// This step's force on every midpoint that carries one: constant forces, then face forces
+// (the midpoint's element forces are zero, its drag too), before linearise_force moves it.
+__global__ __launch_bounds__(256) void k_mid_forces(ForceArgs fa, int M) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= M) return;
+    for (int i = 0; i < 3; i++) {
+        double f = 0.0;
+        if (fa.mid_pre_off)
+            for (int t = fa.mid_pre_off[m]; t < fa.mid_pre_off[m + 1]; t++) f += fa.cforce[3 * (size_t)(~fa.pre_item[t]) + i];
+        if (fa.mid_grp_off) f = face_groups(fa, fa.mid_grp_off[m], fa.mid_grp_off[m + 1], i, f);
+        fa.mf[3 * (size_t)m + i] = f;
+    }
+}
 
 // Component i of spring s's force on its end `end` (World::apply_springs,
 // src/World.cpp:3812-3842): direction by arr3Normalise2 (src/mat_vec_fns_II.cpp:164-176, zero
@@ -719,9 +748,17 @@ __device__ __forceinline__ double spring_component(const ForceArgs &fa, int s, i
     return end == 0 ? fm * dir : (-1.0 * fm) * dir;
 }
 
-// Face::get_area (src/Face.cpp:244-261).
-__device__ __forceinline__ double face_area(const double4 *pos, const int32_t *n) {
-    const double4 a0 = pos[n[0]], a1 = pos[n[1]], a2 = pos[n[2]];
+// A face corner's position: a vertex, or an edge midpoint as linearise_element places it
+// (.5 (a + b), src/tetra_element_linear.cpp:567-591); a vertex is stored as (v, v), for which
+// the same expression is exact.
+__device__ __forceinline__ double4 corner(const double4 *pos, const int32_t *ab) {
+    const double4 a = pos[ab[0]], b = pos[ab[1]];
+    return make_double4(.5 * (a.x + b.x), .5 * (a.y + b.y), .5 * (a.z + b.z), 0.0);
+}
+
+// Face::get_area (src/Face.cpp:244-261); ends: the three corners' edge-end pairs.
+__device__ __forceinline__ double face_area(const double4 *pos, const int32_t *ends) {
+    const double4 a0 = corner(pos, ends), a1 = corner(pos, ends + 2), a2 = corner(pos, ends + 4);
     const double ax = a1.x - a0.x, ay = a1.y - a0.y, az = a1.z - a0.z;
     const double bx = a2.x - a0.x, by = a2.y - a0.y, bz = a2.z - a0.z;
     const double tx = ay * bz - az * by, ty = az * bx - ax * bz, tz = ax * by - ay * bx;
@@ -732,7 +769,7 @@ __device__ __forceinline__ double face_area(const double4 *pos, const int32_t *n
 // per set; each face's share of the set's force is its area over three times the set's total
 // area, summed in the set's face order by one thread.
 struct FaceArgs {
-    const int32_t *set_off, *ent_nodes;
+    const int32_t *set_off, *ent_ends;
     const double *set_force;
     const double4 *pos;
     double *area, *trac;
@@ -741,7 +778,7 @@ __global__ __launch_bounds__(256) void k_face_tractions(FaceArgs a) {
     __shared__ double total;
     const int s = blockIdx.x;
     const int e0 = a.set_off[s], e1 = a.set_off[s + 1];
-    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) a.area[e] = face_area(a.pos, a.ent_nodes + 3 * (size_t)e);
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) a.area[e] = face_area(a.pos, a.ent_ends + 6 * (size_t)e);
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0;
@@ -836,15 +873,7 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
         }
         // surface face forces, face by face, after the element forces (src/Blob.cpp:3842-3851)
-        if (g.fa.grp_off) {
-            const int q1 = g.fa.grp_off[n + 1];
-            for (int q = g.fa.grp_off[n]; q < q1; q++) {
-                double sf = 0.0;
-                const int t1 = g.fa.grp_ent_off[q + 1];
-                for (int t = g.fa.grp_ent_off[q]; t < t1; t++) sf += g.fa.trac[3 * (size_t)g.fa.grp_ent[t] + i];
-                f += sf;
-            }
-        }
+        if (g.fa.grp_off) f = face_groups(g.fa, g.fa.grp_off[n], g.fa.grp_off[n + 1], i, f);
         // external forces come after the element forces and before the Stokes noise
         if (g.ext) f += g.ext[3 * (size_t)n + i];
         // mass-ful solvers: Stokes drag on the node's velocity, before its noise (:3854-3877)
@@ -856,10 +885,17 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             for (int k = 0; k < i; k++) (void)rng_u01(st);
             f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
         }
-        // constant forces on edge midpoints, folded element by element (linearise_force)
+        // forces on edge midpoints, passed on element by element (linearise_force,
+        // src/Blob.cpp:1066-1085)
         if (g.fa.fold_off) {
             const int t1 = g.fa.fold_off[n + 1];
-            for (int t = g.fa.fold_off[n]; t < t1; t++) f += g.fa.fold[3 * (size_t)t + i];
+            for (int t = g.fa.fold_off[n]; t < t1; t++) {
+                const int *mm = g.fa.fold_mid + 3 * (size_t)t;
+                const double x = mm[0] >= 0 ? g.fa.mf[3 * (size_t)mm[0] + i] : 0.0;
+                const double y = mm[1] >= 0 ? g.fa.mf[3 * (size_t)mm[1] + i] : 0.0;
+                const double z = mm[2] >= 0 ? g.fa.mf[3 * (size_t)mm[2] + i] : 0.0;
+                f += 0.5 * (x + y + z);
+            }
         }
     }
     const size_t k = 3 * (size_t)n + i;
@@ -1330,7 +1366,7 @@ struct ffh_blob {
     std::vector<void *> force_allocs;
     ForceArgs fdev = {};
     FaceArgs face = {};
-    int nsprings = 0, nsets = 0;
+    int nsprings = 0, nsets = 0, nmid = 0;
     double *spring_e = nullptr;
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
     int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
@@ -1617,6 +1653,13 @@ struct ffh_blob {
             FaceArgs fa = face;
             fa.pos = pos;
             int rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_face_tractions, nsets, 256, 0, stream, e0, e1, 0, fa); });
+            if (rc) return rc;
+        }
+        if (nmid > 0 && !f_in) {  // this step's forces on the midpoints that carry one
+            ForceArgs fa = fdev;
+            fa.pos = pos;
+            const int M = nmid;
+            int rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_mid_forces, (M + 255) / 256, 256, 0, stream, e0, e1, 0, fa, M); });
             if (rc) return rc;
         }
         RhsArgs g = rhs_args(calc_noise == 1 && calc_stokes == 1, f_in);
@@ -2091,7 +2134,7 @@ struct ffh_blob {
             FFH_PUT(fdev.grp_ent_off, F.grp_ent_off);
             FFH_PUT(fdev.grp_ent, F.grp_ent);
             FFH_PUT(face.set_off, F.set_off);
-            FFH_PUT(face.ent_nodes, F.ent_nodes);
+            FFH_PUT(face.ent_ends, F.ent_ends);
             FFH_PUT(face.set_force, F.set_force);
             std::vector<double> z((size_t)std::max(1, F.set_off.back()), 0.0);
             FFH_PUT(face.area, z);
@@ -2099,9 +2142,14 @@ struct ffh_blob {
             FFH_PUT(face.trac, z3);
             fdev.trac = face.trac;
         }
-        if (!F.fold_off.empty()) {
+        nmid = (int)(F.mid_ends.size() / 2);
+        if (nmid > 0) {
+            if (!F.mid_pre_off.empty()) FFH_PUT(fdev.mid_pre_off, F.mid_pre_off);
+            if (!F.mid_grp_off.empty()) FFH_PUT(fdev.mid_grp_off, F.mid_grp_off);
+            std::vector<double> z(3 * (size_t)nmid, 0.0);
+            FFH_PUT(fdev.mf, z);
             FFH_PUT(fdev.fold_off, F.fold_off);
-            FFH_PUT(fdev.fold, F.fold);
+            FFH_PUT(fdev.fold_mid, F.fold_mid);
         }
 #undef FFH_PUT
         fplan = std::move(F);

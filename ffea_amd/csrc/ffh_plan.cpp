@@ -503,51 +503,58 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
     if (in.spring_n1.size() != ns || in.spring_k.size() != ns || in.spring_l.size() != ns ||
         in.const_force.size() != 3 * nc || in.faces.size() != 3 * nf || in.set_force.size() != 3 * nset)
         return "inconsistent force list sizes";
-    auto vertex = [&](int32_t v, int &c) -> std::string {
-        if (v < 0 || v >= NT) return "node index " + std::to_string(v) + " out of range";
-        c = P.lin_of[v];
-        if (c < 0) return "node " + std::to_string(v) + " is an edge midpoint";
-        return "";
+    // edge ends of every midpoint (N_t ids -> compact ids)
+    std::vector<int32_t> end_a(NT, -1), end_b(NT, -1);
+    for (size_t k = 0; k < P.mid_index.size(); k++) {
+        end_a[P.mid_index[k]] = P.lin_of[P.mid_a[k]];
+        end_b[P.mid_index[k]] = P.lin_of[P.mid_b[k]];
+    }
+    std::vector<int32_t> mid_of(NT, -1);  // midpoint N_t id -> carrying-midpoint index
+    auto midpoint = [&](int32_t v) -> int32_t {
+        if (mid_of[v] < 0) {
+            mid_of[v] = (int32_t)(F.mid_ends.size() / 2);
+            F.mid_ends.push_back(end_a[v]);
+            F.mid_ends.push_back(end_b[v]);
+        }
+        return mid_of[v];
     };
-    std::vector<std::vector<int32_t>> pre(N);
+    std::vector<std::vector<int32_t>> pre(N), mpre;
     F.spring_nodes.resize(2 * ns);
     F.spring_kl.resize(2 * ns);
     F.spring_blob.resize(2 * ns);
     for (size_t s = 0; s < ns; s++) {
-        int c0 = -1, c1 = -1;
-        std::string m = vertex(in.spring_n0[s], c0);
-        if (m.empty()) m = vertex(in.spring_n1[s], c1);
-        if (!m.empty()) return "spring " + std::to_string(s) + ": " + m;
-        if (c0 == c1) return "spring " + std::to_string(s) + " joins a node to itself";
-        F.spring_nodes[2 * s] = c0;
-        F.spring_nodes[2 * s + 1] = c1;
+        int32_t v[2] = {in.spring_n0[s], in.spring_n1[s]};
+        int c[2];
+        for (int e = 0; e < 2; e++) {
+            if (v[e] < 0 || v[e] >= NT) return "spring " + std::to_string(s) + ": node index out of range";
+            c[e] = P.lin_of[v[e]];
+            if (c[e] < 0) return "spring " + std::to_string(s) + ": node " + std::to_string(v[e]) + " is an edge midpoint";
+        }
+        if (c[0] == c[1]) return "spring " + std::to_string(s) + " joins a node to itself";
+        for (int e = 0; e < 2; e++) {
+            F.spring_nodes[2 * s + e] = c[e];
+            F.spring_blob[2 * s + e] = P.slice_blob[c[e] / kSlice];
+            pre[c[e]].push_back((int32_t)(2 * s + e));
+        }
         F.spring_kl[2 * s] = in.spring_k[s];
         F.spring_kl[2 * s + 1] = in.spring_l[s];
-        F.spring_blob[2 * s] = P.slice_blob[c0 / kSlice];
-        F.spring_blob[2 * s + 1] = P.slice_blob[c1 / kSlice];
-        pre[c0].push_back((int32_t)(2 * s));
-        pre[c1].push_back((int32_t)(2 * s + 1));
     }
-    // constant forces: on vertices into the pre lists, on midpoints accumulated for folding
-    std::vector<double> fm;
     for (size_t i = 0; i < nc; i++) {
         const int32_t v = in.const_node[i];
         if (v < 0 || v >= NT) return "constant force " + std::to_string(i) + ": node out of range";
-        const int c = P.lin_of[v];
-        if (c >= 0) {
-            pre[c].push_back(~(int32_t)(F.cforce.size() / 3));
-            for (int k = 0; k < 3; k++) F.cforce.push_back(in.const_force[3 * i + k]);
+        const int32_t item = ~(int32_t)i;
+        if (P.lin_of[v] < 0 && end_a[v] < 0) return "constant force " + std::to_string(i) + ": node is in no element";
+        if (P.lin_of[v] >= 0) {
+            pre[P.lin_of[v]].push_back(item);
         } else {
-            if (fm.empty()) fm.assign(3 * (size_t)NT, 0.0);
-            for (int k = 0; k < 3; k++) fm[3 * (size_t)v + k] += in.const_force[3 * i + k];
+            const int32_t m = midpoint(v);
+            if ((size_t)m >= mpre.size()) mpre.resize(m + 1);
+            mpre[m].push_back(item);
         }
     }
-    if (ns || !F.cforce.empty()) {
-        F.pre_off.assign((size_t)N + 1, 0);
-        for (int i = 0; i < N; i++) F.pre_off[i + 1] = F.pre_off[i] + (int32_t)pre[i].size();
-        for (int i = 0; i < N; i++) F.pre_item.insert(F.pre_item.end(), pre[i].begin(), pre[i].end());
-    }
-    // surface sets
+    F.cforce = in.const_force;
+    // surface sets: entries, and per face the entries that load it (set order)
+    std::vector<std::vector<int32_t>> ent_of_face(nf);
     if (nset) {
         F.set_off = in.set_off;
         F.set_force = in.set_force;
@@ -555,56 +562,93 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
         for (size_t s = 0; s < nset; s++)
             if (F.set_off[s + 1] < F.set_off[s]) return "inconsistent surface set offsets";
         const size_t nent = in.set_faces.size();
-        std::vector<std::vector<int32_t>> ent_of_face(nf);
-        F.ent_nodes.resize(3 * nent);
+        F.ent_ends.resize(6 * nent);
         for (size_t e = 0; e < nent; e++) {
             const int32_t f = in.set_faces[e];
             if (f < 0 || (size_t)f >= nf) return "surface set face index out of range";
             ent_of_face[f].push_back((int32_t)e);
             for (int k = 0; k < 3; k++) {
-                int c = -1;
-                std::string m = vertex(in.faces[3 * (size_t)f + k], c);
-                if (!m.empty()) return "face " + std::to_string(f) + ": " + m;
-                F.ent_nodes[3 * e + k] = c;
+                const int32_t v = in.faces[3 * (size_t)f + k];
+                if (v < 0 || v >= NT) return "face " + std::to_string(f) + ": node index out of range";
+                const int c = P.lin_of[v];
+                if (c < 0 && end_a[v] < 0) return "face " + std::to_string(f) + ": node " + std::to_string(v) + " is in no element";
+                F.ent_ends[6 * e + 2 * k] = c >= 0 ? c : end_a[v];
+                F.ent_ends[6 * e + 2 * k + 1] = c >= 0 ? c : end_b[v];
             }
         }
-        std::vector<std::vector<int32_t>> grp(N);  // per node: face ids, ascending
-        for (size_t f = 0; f < nf; f++) {
-            if (ent_of_face[f].empty()) continue;
-            for (int k = 0; k < 3; k++) grp[P.lin_of[in.faces[3 * f + k]]].push_back((int32_t)f);
+    }
+    // face groups: vertices first, then carrying midpoints
+    std::vector<std::vector<int32_t>> grp(N), mgrp;
+    for (size_t f = 0; f < nf; f++) {
+        if (ent_of_face[f].empty()) continue;
+        for (int k = 0; k < 3; k++) {
+            const int32_t v = in.faces[3 * f + k];
+            if (P.lin_of[v] >= 0) {
+                grp[P.lin_of[v]].push_back((int32_t)f);
+            } else {
+                const int32_t m = midpoint(v);
+                if ((size_t)m >= mgrp.size()) mgrp.resize(m + 1);
+                mgrp[m].push_back((int32_t)f);
+            }
         }
+    }
+    const int M = (int)(F.mid_ends.size() / 2);
+    mpre.resize(M);
+    mgrp.resize(M);
+    bool any_pre = ns > 0;
+    for (int i = 0; i < N && !any_pre; i++) any_pre = !pre[i].empty();
+    if (any_pre || M > 0) {
+        F.pre_off.assign((size_t)N + 1, 0);
+        for (int i = 0; i < N; i++) {
+            F.pre_item.insert(F.pre_item.end(), pre[i].begin(), pre[i].end());
+            F.pre_off[i + 1] = (int32_t)F.pre_item.size();
+        }
+        F.mid_pre_off.assign((size_t)M + 1, F.pre_off[N]);
+        for (int m = 0; m < M; m++) {
+            F.pre_item.insert(F.pre_item.end(), mpre[m].begin(), mpre[m].end());
+            F.mid_pre_off[m + 1] = (int32_t)F.pre_item.size();
+        }
+    }
+    if (nset) {
         F.grp_off.assign((size_t)N + 1, 0);
         F.grp_ent_off.assign(1, 0);
-        for (int i = 0; i < N; i++) {
-            for (int32_t f : grp[i]) {
+        auto add_groups = [&](const std::vector<int32_t> &faces) {
+            for (int32_t f : faces) {
                 F.grp_ent.insert(F.grp_ent.end(), ent_of_face[f].begin(), ent_of_face[f].end());
                 F.grp_ent_off.push_back((int32_t)F.grp_ent.size());
             }
-            F.grp_off[i + 1] = (int32_t)(F.grp_ent_off.size() - 1);
-        }
+            return (int32_t)(F.grp_ent_off.size() - 1);
+        };
+        for (int i = 0; i < N; i++) F.grp_off[i + 1] = add_groups(grp[i]);
+        F.mid_grp_off.assign((size_t)M + 1, F.grp_off[N]);
+        for (int m = 0; m < M; m++) F.mid_grp_off[m + 1] = add_groups(mgrp[m]);
     }
-    // midpoint constant forces folded element by element
-    if (!fm.empty()) {
+    // linearise_force: element by element, a vertex takes half the sum of its three adjacent
+    // midpoints; the element's midpoints are then zeroed (counted once, in their first element)
+    if (M > 0) {
         static const int sets[4][3] = {{0, 1, 2}, {0, 3, 4}, {1, 3, 5}, {2, 4, 5}};
-        std::vector<std::vector<double>> terms(N);
+        std::vector<uint8_t> used(NT, 0);
+        std::vector<std::vector<int32_t>> terms(N);
         for (int e = 0; e < E; e++) {
             const int32_t *m = &P.emid[6 * (size_t)e];
             for (int a = 0; a < 4; a++) {
-                const double *x = &fm[3 * (size_t)m[sets[a][0]]], *y = &fm[3 * (size_t)m[sets[a][1]]],
-                             *z = &fm[3 * (size_t)m[sets[a][2]]];
+                int32_t t[3];
                 bool any = false;
-                for (int k = 0; k < 3; k++) any |= x[k] != 0.0 || y[k] != 0.0 || z[k] != 0.0;
+                for (int k = 0; k < 3; k++) {
+                    const int32_t v = m[sets[a][k]];
+                    t[k] = used[v] ? -1 : mid_of[v];
+                    any |= t[k] >= 0;
+                }
                 if (!any) continue;
-                auto &t = terms[P.enodes[4 * (size_t)e + a]];
-                for (int k = 0; k < 3; k++) t.push_back(0.5 * (x[k] + y[k] + z[k]));
+                auto &tt = terms[P.enodes[4 * (size_t)e + a]];
+                tt.insert(tt.end(), t, t + 3);
             }
-            for (int k = 0; k < 6; k++)
-                for (int j = 0; j < 3; j++) fm[3 * (size_t)m[k] + j] = 0.0;
+            for (int k = 0; k < 6; k++) used[m[k]] = 1;
         }
         F.fold_off.assign((size_t)N + 1, 0);
         for (int i = 0; i < N; i++) {
-            F.fold_off[i + 1] = F.fold_off[i] + (int32_t)(terms[i].size() / 3);
-            F.fold.insert(F.fold.end(), terms[i].begin(), terms[i].end());
+            F.fold_mid.insert(F.fold_mid.end(), terms[i].begin(), terms[i].end());
+            F.fold_off[i + 1] = (int32_t)(F.fold_mid.size() / 3);
         }
     }
     return "";

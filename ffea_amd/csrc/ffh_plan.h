@@ -138,27 +138,36 @@ struct ForceInput {
 //  2. after the element forces: one group per surface face holding the node (ascending face
 //     index), each the sum of the face's set tractions in set order (Face::force, gathered in
 //     aggregate_forces_and_solve, src/Blob.cpp:3842-3851);
-//  3. after the noise: constant forces on edge midpoints, folded onto the edge ends element by
-//     element as linearise_force does (src/Blob.cpp:1066-1085).
+//  3. after the noise: the forces on edge midpoints (constant forces, then face groups, as in
+//     1 and 2), passed on to the edge ends element by element as linearise_force does
+//     (src/Blob.cpp:1066-1085): each (vertex, element) term is half the sum of the element's
+//     three midpoints next to the vertex, a midpoint counting only in the first element that
+//     holds it.
+// Second-order surfaces are split into triangles that use edge midpoints as corners; such a
+// corner's position is the average of its edge ends, as linearise_element leaves it.
 struct ForcePlan {
     std::vector<int32_t> pre_off;    // [N+1]
     std::vector<int32_t> pre_item;   // >= 0: 2 * spring + end; < 0: ~(constant index)
     std::vector<int32_t> spring_nodes;  // [2 * springs] compact ids
     std::vector<double> spring_kl;      // [2 * springs]
-    std::vector<double> cforce;         // [3 * constants on vertices]
-    std::vector<int32_t> grp_off;       // [N+1]
+    std::vector<double> cforce;         // [3 * constants]
+    std::vector<int32_t> grp_off;       // [N+1] face groups of the vertices
     std::vector<int32_t> grp_ent_off;   // [groups + 1]
     std::vector<int32_t> grp_ent;       // set-entry ids
     std::vector<int32_t> set_off;       // [S+1] entries of each set
-    std::vector<int32_t> ent_nodes;     // [3 * entries] compact ids of the entry's face
+    std::vector<int32_t> ent_ends;      // [6 * entries] per face corner its two edge ends (a vertex twice)
     std::vector<double> set_force;      // [3S]
-    std::vector<int32_t> fold_off;      // [N+1]
-    std::vector<double> fold;           // [3 * terms]
+    // midpoints that carry a force: edge ends, constant-force items, face groups
+    std::vector<int32_t> mid_ends;      // [2M] compact ids
+    std::vector<int32_t> mid_pre_off;   // [M+1] into pre_item (constants only)
+    std::vector<int32_t> mid_grp_off;   // [M+1] into the groups
+    std::vector<int32_t> fold_off;      // [N+1] fold terms of each vertex
+    std::vector<int32_t> fold_mid;      // [3 * terms] midpoint indices, -1 for none
     std::vector<int32_t> spring_blob;   // [2 * springs] blobs of the two ends (energy grouping)
 };
 
-// Returns "" or an error message (ids out of range, a spring end or face node on an edge
-// midpoint, a spring joining a node to itself).
+// Returns "" or an error message (ids out of range, a spring end on an edge midpoint, a spring
+// joining a node to itself).
 std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F);
 
 // Exact jump matrices of MRG32k3a component c (0: m1, 1: m2) raised to n.

@@ -130,6 +130,119 @@ def read_surface(path):
     return np.array([[int(x) for x in L[3 + i].split()[:4]] for i in range(n)], np.int32)
 
 
+def _script_lines(path):
+    """FFEA_input_reader::file_to_lines (src/FFEA_input_reader.cpp:36-113): <!-- --> comments
+    removed (also across lines), each line trimmed, empty lines dropped."""
+    text = re.sub(r"<!--.*?(-->|$)", "", open(path).read(), flags=re.S)
+    return [l.strip() for l in text.split("\n") if l.strip()]
+
+
+def read_springs(path):
+    """World::load_springs (src/World.cpp:3473-3560): per spring k and l converted to meso
+    units (k *= area / Energy, l /= length), blob, conformation and node of each end.
+    Returns a dict of arrays k, l, blob0, blob1, conf0, conf1, node0, node1."""
+    from .units import MESO_AREA, MESO_ENERGY, MESO_LENGTH
+    L = _lines(path)
+    if L[0] != "ffea springs file":
+        raise FormatError(f"{path}: this is not a 'ffea spring file' (read '{L[0]}')")
+    n = _count(L[1], "num_springs", path)
+    i = 2 + (L[2].strip() == "springs:")
+    rows = []
+    for j in range(n):
+        v = L[i + j].split()
+        if len(v) < 8:
+            raise FormatError(f"{path}: problem reading spring data (spring {j})")
+        rows.append(v[:8])
+    out = {k: np.array([int(r[c]) for r in rows], np.int32)
+           for c, k in ((2, "blob0"), (3, "blob1"), (4, "conf0"), (5, "conf1"), (6, "node0"), (7, "node1"))}
+    k = np.array([float(r[0]) for r in rows])
+    out["k"] = k * (MESO_AREA / MESO_ENERGY)
+    out["l"] = np.array([float(r[1]) for r in rows]) / MESO_LENGTH
+    if (out["k"] < 0).any() or (out["l"] < 0).any():
+        raise FormatError(f"{path}: spring constants and lengths must be >= 0")
+    same = (out["blob0"] == out["blob1"]) & (out["conf0"] == out["conf1"]) & (out["node0"] == out["node1"])
+    if same.any():
+        raise FormatError(f"{path}: spring {int(np.flatnonzero(same)[0])} is connected to the same node")
+    return out
+
+
+def _normalised(d):
+    """arr3Normalise (src/mat_vec_fns_II.cpp:149-160)."""
+    norm = 0.0
+    for x in d:
+        norm += x * x
+    norm = float(np.sqrt(norm))
+    return [x / norm for x in d]
+
+
+def read_ctforces(path, blob_index, num_nodes, conformation_index=0):
+    """Blob::load_ctforces (src/Blob.cpp:3084-3430) for one blob: the linear forces (CTF 1,
+    "all" expanded to every node) as (nodes [n], forces [n, 3] meso) and the linear surface
+    forces (CTF 3) as (list of face-index lists, forces [S, 3] meso).  Rotational forces
+    (CTF 2) for this blob raise FormatError: they are not on this path."""
+    from .units import MESO_FORCE
+    L = _script_lines(path)
+    if len(L) < 4:
+        raise FormatError(f"{path}: too short")
+    counts = {"num_ctforces": 0, "num_linear_forces": 0, "num_rot_forces": 0, "num_linear_surface_forces": 0}
+    r = 1
+    while r < 5 and r < len(L) and ":" not in L[r]:
+        w = L[r].split()
+        if w[0] in counts:
+            counts[w[0]] = int(w[1])
+        r += 1
+    nl, nr, ns = counts["num_linear_forces"], counts["num_rot_forces"], counts["num_linear_surface_forces"]
+    want = counts["num_ctforces"] + r + (nl > 0) + (nr > 0) + (ns > 0)
+    if counts["num_ctforces"] != nl + nr + ns or counts["num_ctforces"] == 0 or want != len(L):
+        raise FormatError(f"{path}: inconsistent ctforces header")
+    nodes, forces = [], []
+    if L[r].startswith("linear forces:"):
+        r += 1
+    elif nl > 0:
+        raise FormatError(f"{path}: expected 'linear forces:'")
+    for line in L[r:r + nl]:
+        w = line.split()
+        if len(w) != 7:
+            raise FormatError(f"{path}: invalid line '{line}'")
+        if int(w[4]) != blob_index or int(w[5]) != conformation_index:
+            continue
+        d = _normalised([float(w[1]), float(w[2]), float(w[3])])
+        F = float(w[0])
+        f = [F * d[0] / MESO_FORCE, F * d[1] / MESO_FORCE, F * d[2] / MESO_FORCE]
+        for nd in (range(num_nodes) if w[6] == "all" else [int(w[6])]):
+            nodes.append(nd)
+            forces.append(f)
+    r += nl
+    if r < len(L) and L[r].startswith("rotational forces:"):
+        r += 1
+    elif nr > 0:
+        raise FormatError(f"{path}: expected 'rotational forces:'")
+    for line in L[r:r + nr]:
+        w = line.split()
+        if len(w) == 11 and int(w[8]) == blob_index and int(w[9]) == conformation_index:
+            raise FormatError(f"{path}: rotational constant forces are not on this path")
+    r += nr
+    sets, set_forces = [], []
+    if r < len(L) and L[r].startswith("linear surface forces:"):
+        r += 1
+    elif ns > 0:
+        raise FormatError(f"{path}: expected 'linear surface forces:'")
+    for line in L[r:r + ns]:
+        w = line.split()
+        if len(w) < 7:
+            raise FormatError(f"{path}: invalid line '{line}'")
+        if int(w[4]) != blob_index or int(w[5]) != conformation_index:
+            continue
+        if w[6] == "all":
+            raise FormatError(f"{path}: 'all' surface faces are refused by the reference too")
+        d = _normalised([float(w[1]), float(w[2]), float(w[3])])
+        F = float(w[0])
+        set_forces.append([F * d[0] / MESO_FORCE, F * d[1] / MESO_FORCE, F * d[2] / MESO_FORCE])
+        sets.append([int(x) for x in w[6:]])
+    return ((np.array(nodes, np.int32), np.array(forces, np.float64).reshape(-1, 3)),
+            (sets, np.array(set_forces, np.float64).reshape(-1, 3)))
+
+
 def load_blob(node, top, mat, stokes, pin=None) -> Mesh:
     pos, _ = read_node(node)
     topo, _ = read_topology(top, pos.shape[0])
@@ -194,6 +307,15 @@ def write_pin(path, pins):
             f.write(f"{int(p)}\n")
 
 
+def write_surface(path, faces):
+    """faces [F, 4]: element, then the face's three nodes."""
+    faces = np.asarray(faces)
+    with open(path, "w") as f:
+        f.write(f"ffea surface file\nnum_surface_faces {faces.shape[0]}\nfaces:\n")
+        for row in faces:
+            f.write(" ".join(str(int(v)) for v in row) + "\n")
+
+
 def write_blob(prefix, mesh: Mesh):
     write_node(prefix + ".node", mesh.pos)
     write_topology(prefix + ".top", mesh.topo)
@@ -209,8 +331,10 @@ def read_script(path) -> dict:
     """Parses <param> key/values and the first conformation of each <blob>.
 
     Returns {"params": {key: str}, "blobs": [{"nodes":..., "topology":..., "material":...,
-    "stokes":..., "pin":..., "solver":..., "scale":...}]}; file paths resolved relative to
-    the script.
+    "stokes":..., "pin":..., "solver":..., "scale":...}], "springs": path or None,
+    "ctforces": path or None}; file paths resolved relative to the script.  The <springs> and
+    <ctforces> blocks of <interactions> name one file each (World::read_and_build_system,
+    src/World.cpp:2262-2297).
     """
     text = open(path).read()
     base = os.path.dirname(os.path.abspath(path))
@@ -233,7 +357,12 @@ def read_script(path) -> dict:
             if k in b:
                 b[k] = os.path.normpath(os.path.join(base, b[k]))
         blobs.append(b)
-    return {"params": params, "blobs": blobs}
+    files = {}
+    for block in ("springs", "ctforces"):
+        bm = re.search(rf"<{block}>(.*?)</{block}>", text, re.S)
+        tags = re.findall(r"<\s*([A-Za-z_0-9]+)\s*=\s*([^>]*)>", bm.group(1)) if bm else []
+        files[block] = os.path.normpath(os.path.join(base, tags[0][1].strip())) if tags else None
+    return {"params": params, "blobs": blobs, **files}
 
 
 def params_from_script(params: dict):

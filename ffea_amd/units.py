@@ -16,6 +16,7 @@ MESO_VOLUME = 4.913e-30
 MESO_TIME = 3.7302915560886126e-13
 MESO_PRESSURE = 8.4305832688784826e8
 MESO_FORCE = 2.4364705882352941e-11
+MESO_AREA = 2.89e-20
 MESO_VELOCITY = 4.5572845297447225e2
 
 KB = 1.380649e-23

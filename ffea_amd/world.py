@@ -80,6 +80,11 @@ MEAS_COLUMNS = ("Time", "StrainEnergy", "Centroid.x", "Centroid.y", "Centroid.z"
 MEAS_COLUMNS_MASS = ("Time", "KineticEnergy") + MEAS_COLUMNS[1:]
 
 
+def meas_columns(mass_in_system: bool, springs: bool):
+    """World::init's measurement header (src/World.cpp:565-580) for the columns of this path."""
+    return (MEAS_COLUMNS_MASS if mass_in_system else MEAS_COLUMNS) + (("SpringEnergy",) if springs else ())
+
+
 def meas_line(values) -> str:
     return "".join("%-14.6e" % v for v in values) + "\n"
 
@@ -196,6 +201,9 @@ class World:
             st = package_seed(self.params.rng_seed)
         self.batch = Batch(self.meshes, self.params, self.scales, rng_state=st, precision=precision)
         self.batch.set_solver(self.solver)
+        self.calc_springs = int(float(self.raw.get("calc_springs", 0))) != 0
+        self.calc_ctforces = int(float(self.raw.get("calc_ctforces", 0))) == 1
+        self._interactions(s)
         self.quiet = quiet
         self.step_initial = 0
         if self.restart:
@@ -205,6 +213,45 @@ class World:
                 f.write(traj_header(self.nodes))
             with open(self.meas_path, "w") as f:
                 f.write(self._meas_header())
+
+    def _interactions(self, s):
+        """Springs (World::load_springs) and constant forces (Blob::load_ctforces) of the
+        script onto the batch, node and face ids shifted to the batch's numbering."""
+        off = self.batch.node_off
+        if self.calc_springs:
+            if not s.get("springs"):
+                raise F.FormatError("calc_springs set but no <springs> file")
+            sp = F.read_springs(s["springs"])
+            nb = len(self.meshes)
+            if (sp["blob0"] >= nb).any() or (sp["blob1"] >= nb).any() or (sp["blob0"] < 0).any() or (sp["blob1"] < 0).any():
+                raise F.FormatError("spring blob index out of range")
+            if (sp["conf0"] != 0).any() or (sp["conf1"] != 0).any():
+                raise F.FormatError("springs on conformations other than 0 are not on this path")
+            for e in ("0", "1"):
+                bad = (sp["node" + e] < 0) | (sp["node" + e] >= np.array(self.nodes)[sp["blob" + e]])
+                if bad.any():
+                    raise F.FormatError("spring node index out of range")
+            self.batch.set_springs(off[sp["blob0"]] + sp["node0"], off[sp["blob1"]] + sp["node1"], sp["k"], sp["l"])
+        if self.calc_ctforces:
+            if not s.get("ctforces"):
+                raise F.FormatError("ctforces_fname not provided in <ctforces> block, while calc_ctforces set to 1")
+            nodes, forces, faces, sets, set_forces = [], [], [], [], []
+            face_off = 0
+            for b, (blob, m) in enumerate(zip(s["blobs"], self.meshes)):
+                (ln, lf), (bs, bf) = F.read_ctforces(s["ctforces"], b, m.num_nodes)
+                nodes.append(ln + off[b])
+                forces.append(lf)
+                if bs:
+                    if not blob.get("surface"):
+                        raise F.FormatError(f"blob {b}: surface forces need its surface file")
+                    fc = F.read_surface(blob["surface"])[:, 1:4]
+                    faces.append(fc + off[b])
+                    sets += [[f + face_off for f in x] for x in bs]
+                    set_forces.append(bf)
+                    face_off += fc.shape[0]
+            self.batch.set_constant_forces(np.concatenate(nodes), np.concatenate(forces))
+            if sets:
+                self.batch.set_surface_forces(np.concatenate(faces), sets, np.concatenate(set_forces))
 
     def _meas_header(self) -> str:
         p = self.raw
@@ -218,7 +265,7 @@ class World:
                 s += f"\t{k} = {p[k]}\n"
         s += "\n\tCalculations enabled:\n"
         s += f"\tcalc_noise = {self.params.calc_noise}\n\tcalc_stokes = {self.params.calc_stokes}\n\n\n"
-        cols = MEAS_COLUMNS_MASS if self.mass_in_system else MEAS_COLUMNS
+        cols = meas_columns(self.mass_in_system, self.calc_springs)
         s += "Measurements:\n" + "".join("%-14s" % c for c in cols) + "\n"
         return s
 
@@ -259,8 +306,9 @@ class World:
             msd += r * r * n
         cog /= tot
         ke = (self.batch.kinetic_energy() * U.MESO_ENERGY,) if self.mass_in_system else ()
+        se = (self.batch.spring_energy() * U.MESO_ENERGY,) if self.calc_springs else ()
         return (step * self.dt_si, *ke, self.batch.strain_energy() * U.MESO_ENERGY, *(cog * U.MESO_LENGTH),
-                np.sqrt(msd / tot) * U.MESO_LENGTH), pos
+                np.sqrt(msd / tot) * U.MESO_LENGTH, *se), pos
 
     def run(self):
         """World::run from step_initial to num_steps inclusive."""

@@ -6,7 +6,9 @@ springs, constant forces on a vertex and on an edge midpoint and two surface set
 face) is bit-exact -- force vector, velocities, positions, stream -- as is a two-blob batch whose
 spring joins the blobs; with the massless solver the force vector is bit-exact and positions
 within 1e-9 of the displacement (CG tree sums); spring energy equal to the oracle's; graph
-replay equal to ordinary launches; ends on edge midpoints are refused.
+replay equal to ordinary launches; spring ends on edge midpoints are refused.  Surface faces
+include second-order sub-triangles whose corners are edge midpoints: their forces reach the
+vertices through linearise_force, recomputed every step.
 """
 import numpy as np
 import pytest
@@ -32,9 +34,11 @@ def forces(m, x0, sc, node_off=0):
     springs = ([n0, n2], [n1, n0], [0.8, 0.3], [0.7 * d, 1.4 * np.linalg.norm(p[n2] - p[n0])])
     mid = int(m.topo[4, 4])
     consts = ([int(lin[5]), mid, int(lin[5])], np.array([[0.3, -0.2, 0.5], [-0.4, 0.1, 0.25], [0.05, 0.0, -0.1]]))
-    faces = np.array([m.topo[e, [0, 1, 2]] for e in range(0, 12)])
-    sets = [[0, 3, 5, 7], [2, 3, 11]]
-    sf = np.array([[1.0, 0.0, 0.5], [-0.2, 0.7, 0.0]])
+    # vertex triangles, then second-order sub-triangles with edge midpoints as corners
+    faces = np.array([m.topo[e, [0, 1, 2]] for e in range(0, 12)] + [m.topo[e, [0, 4, 5]] for e in range(12, 16)]
+                     + [m.topo[e, [4, 7, 5]] for e in range(16, 18)])
+    sets = [[0, 3, 5, 7], [2, 3, 11], [12, 13, 14, 15, 16, 17]]
+    sf = np.array([[1.0, 0.0, 0.5], [-0.2, 0.7, 0.0], [0.0, 0.4, -0.9]])
     return springs, consts, (faces, sets, sf)
 
 
@@ -154,7 +158,5 @@ def test_midpoint_ends_are_refused():
     mid = int(m.topo[0, 5])
     with pytest.raises(_lib.FFHError):
         b.set_springs([int(lin[0])], [mid], [1.0], [1.0])
-    with pytest.raises(_lib.FFHError):
-        b.set_surface_forces(np.array([[int(lin[0]), int(lin[1]), mid]]), [[0]], np.ones((1, 3)))
     with pytest.raises(_lib.FFHError):
         b.set_springs([int(lin[0])], [int(lin[0])], [1.0], [1.0])

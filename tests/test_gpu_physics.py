@@ -21,11 +21,18 @@ with the reference's own fixture meshes and parameters.
   run must reproduce the oracle's mean squared displacement within 1e-6 relative, and the
   theory within 0.15.
 
+- Young's modulus and flexural rigidity of a cylinder under surface forces
+  (tests/physics/E_cyl_160_fine/calcE.py:42-93, tests/physics/EI_cyl_160_fine/calcEI.py:43-97):
+  the reference's cyl_160_fine mesh, one end pinned, a linear surface force on the other end's
+  faces (ctforces CTF 3), run through the World driver with the reference's scripts; from the
+  end nodes' centroid displacement E = F L / (pi r^2 dL) within 0.5 % of 6e8 Pa and
+  EI = F L^3 / (3 dL) within 10 % of E pi r^4 / 4.
+
 The runs are deterministic (fixed seed, fixed reduction order), so the results do not vary
 from run to run.
 """
-import shutil
 import os
+import shutil
 
 import numpy as np
 import pytest
@@ -149,3 +156,48 @@ def test_inertial_diffusion_sphere(oracle):
     print(f"inertial diffusion: <dr^2> = {r2:.6e}, oracle {r2o:.6e}, theory {theory:.6e}, rel err {err:.4f}")
     assert abs(r2 - r2o) <= 1e-6 * r2o
     assert err <= 0.15
+
+
+# the end nodes both checkers average over (calcE.py:39, calcEI.py:40)
+CYL_ENDNODES = [60, 61, 63, 66, 67, 69, 72, 73, 75, 78, 79, 81, 84, 85, 87, 90, 91, 93, 96, 97, 99, 102, 103, 105,
+                108, 109, 111, 114, 116, 156, 160, 164, 168, 172, 176, 180, 184, 188] + list(range(4310, 4402))
+
+
+def cylinder_run(tmp_path, kind):
+    """Rebuilds the cyl_160_fine files from the fixture arrays and runs the reference's script
+    through the World driver; returns the end nodes' centroid displacement between the first
+    and the last frame (m)."""
+    from ffea_amd import formats as F
+    from ffea_amd.mesh import Mesh
+    from ffea_amd.world import World, read_trajectory
+    z = np.load(os.path.join(GOLDEN, "physics", "cyl_160_fine.npz"))
+    d = tmp_path / "cyl_160_fine"
+    d.mkdir()
+    m = Mesh(pos=z["pos"], topo=z["topo"], material=z["material"], stokes_radius=z["stokes"],
+             num_linear=int(np.unique(z["topo"][:, :4]).size), pinned=z["pinned"])
+    F.write_blob(str(d / "cyl_160_fine"), m)
+    F.write_surface(str(d / "cyl_160_fine.surf"), z["surface"])
+    run = tmp_path / f"{kind}_cyl_160_fine"
+    shutil.copytree(os.path.join(GOLDEN, "physics", f"cyl_160_fine_{kind}"), run)
+    w = World(str(run / f"cyl_160_fine-{kind}.ffea"))
+    assert w.calc_ctforces
+    w.run()
+    frames = read_trajectory(str(run / "cyl_160_fine_trajectory.ftj"))
+    cm = [f[1][0][CYL_ENDNODES, :3].mean(axis=0) for f in (frames[0], frames[-1])]
+    return float(np.sqrt(((cm[1] - cm[0]) ** 2).sum())), len(frames)
+
+
+def test_youngs_modulus_cylinder(tmp_path):
+    dL, nf = cylinder_run(tmp_path, "E")
+    F, L, r, E_t = 1e-11, 160e-9, 10e-9, 6e8
+    E_c = F * L / (np.pi * r ** 2 * dL)
+    print(f"Young's modulus: E_c = {E_c:.5e} Pa over {nf} frames, E_c/E_t = {E_c / E_t:.5f}", flush=True)
+    assert abs(E_c / E_t - 1) <= 0.005
+
+
+def test_flexural_rigidity_cylinder(tmp_path):
+    dL, nf = cylinder_run(tmp_path, "EI")
+    F, L, r, E_t = 1e-12, 160e-9, 10e-9, 6e8
+    EI_c, EI_t = F * L ** 3 / (3 * dL), E_t * np.pi * r ** 4 / 4.0
+    print(f"flexural rigidity: EI_c/EI_t = {EI_c / EI_t:.5f} over {nf} frames", flush=True)
+    assert abs(EI_c / EI_t - 1) <= 0.1

@@ -131,3 +131,18 @@ def test_surface_forces_spread_by_area(oracle, clean_springs):
     scale = np.abs(a).max()
     assert np.allclose(diff, want, rtol=0, atol=1e-12 * scale)
     assert np.allclose(diff.sum(axis=0), SF.sum(axis=0), rtol=0, atol=1e-11 * scale)
+
+
+def test_surface_forces_on_midpoint_corners_are_conserved(oracle, clean_springs):
+    """Second-order surface triangles have edge midpoints as corners; their tractions reach the
+    vertices through linearise_force, half to each edge end, so the total is still the set's."""
+    m, prm, sc, x0 = case()
+    faces = np.array([m.topo[e, [0, 4, 5]] for e in range(6)] + [m.topo[e, [4, 7, 5]] for e in range(6, 9)])
+    sets = [list(range(9))]
+    SF = np.array([[0.3, -1.1, 0.4]])
+    a, b = run_pair(oracle, m, prm, sc, x0, lambda bl: bl.set_surface_forces(faces, sets, SF))
+    diff = b - a
+    scale = np.abs(a).max()
+    assert np.allclose(diff.sum(axis=0), SF[0], rtol=0, atol=1e-11 * scale)
+    mids = np.unique(m.topo[:, 4:])
+    assert np.all(b[mids] == 0)
