@@ -186,7 +186,38 @@ def kernel_table(kt, sizes, bm, solver="CG_nomass"):
     return out
 
 
-def massful(ms, prm, sc, xs, steps, warmup):
+def massful_cpu(ms, p, sc, xs, max_seconds=20.0):
+    """MassLumpedSolver steps of the reference's own objects (oracle/_ref fast build, OpenMP
+    within the blob) on this host, a bounded sample; single blobs up to 2 M tets.  The
+    reference's ConjugateGradientSolver is not timed: its init builds a dense N_t x N_t matrix
+    (src/ConjugateGradientSolver.cpp:78-84)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if len(ms) != 1 or ms[0].num_elements > 2_000_000 or not os.path.exists(pyoracle.REF_FAST_SO):
+        return None
+    lib = pyoracle.Reference(pyoracle.REF_FAST_SO)
+    lib.set_package_seed(lib.package_seed(p.rng_seed))
+    s = lib.stream(per_thread=True)
+    t0 = time.time()
+    b = lib.blob(ms[0], p, sc, pos=xs[0])
+    b.set_solver("masslumped")
+    setup = time.time() - t0
+    assert pyoracle.world_step(lib, [b], s) == 0
+    n, t0 = 0, time.perf_counter()
+    while True:
+        assert pyoracle.world_step(lib, [b], s) == 0
+        n += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds or n >= 200:
+            break
+    b.close()
+    log(f"[cpu reference masslumped] {n} steps in {el:.2f}s")
+    return {"value": n / el, "unit": "steps/s", "cores": lib.num_threads(), "kind": "reference",
+            "sample": f"{n} noise-on MassLumpedSolver steps of the same mesh after 1 warm-up step (reference "
+                      f"objects -O3 -ffast-math, OpenMP within blob), setup {setup:.0f}s excluded"}
+
+
+def massful(ms, prm, sc, xs, steps, warmup, cpu_seconds=0.0):
     """The mass-ful solvers on the same workload (FP64): ConjugateGradientSolver on the
     consistent mass matrix and MassLumpedSolver, steps/s and per-kernel times.  Time step 1 fs:
     the explicit mass-ful update of these nanometre elements is unstable at 10 fs (in the
@@ -224,6 +255,8 @@ def massful(ms, prm, sc, xs, steps, warmup):
         out[solver] = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
                        "cg_iters_mean": float(np.mean(iters)), "kinetic_energy_meso": ke,
                        "kernels": kernel_table(kt, sizes, 8, solver)}
+    if cpu_seconds > 0:
+        out["masslumped"]["cpu_baseline"] = massful_cpu(ms, p, sc, xs, cpu_seconds)
     return out
 
 
@@ -666,7 +699,9 @@ def main():
         equi = {p: equipartition(ms, prm, sc, xs, p, a.equipartition_steps, a.equipartition_steps // 2)
                 for p in (("fp64",) if a.no_mixed else ("fp64", "mixed"))}
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
-    mass = massful(ms, prm, sc, xs, a.massful_steps, a.warmup) if a.massful_steps > 0 else None
+    mass = None
+    if a.massful_steps > 0:
+        mass = massful(ms, prm, sc, xs, a.massful_steps, a.warmup, 0.0 if a.no_cpu else min(a.cpu_seconds, 15.0))
     out = {
         "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
         "value": sps,
