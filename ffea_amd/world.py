@@ -28,6 +28,9 @@ solver, velocities: Blob::read_nodes_from_file, src/Blob.cpp:1268-1306) from the
 trajectory frame (which is removed and written again), the last measurement line removed,
 RngStream state from checkpoint_in, stepping resumes at that frame's step.
 
+Placement (World::init, src/World.cpp:2660-2705, :455-506): a blob's <centroid> / <rotation>,
+then move_into_box (default 1), after the rest state is taken from the positions as loaded.
+
 Deliberate differences: the checkpoint file is rewritten whole at each check (the reference
 rewinds without truncating); the .fm header lists this implementation's parameters; default
 output names are <script>.ftj / <script>.fm / <script>.fcp.
@@ -149,6 +152,112 @@ def read_trajectory(path):
 
 
 # ---------------------------------------------------------------------------------------------
+# placing the blobs (World::init, src/World.cpp:2660-2705 and :455-506)
+# ---------------------------------------------------------------------------------------------
+def _seq_sum(x):
+    """Sum in index order, as the reference's loops add (numpy's sum is pairwise)."""
+    return float(np.add.accumulate(np.asarray(x, np.float64))[-1]) if len(x) else 0.0
+
+
+def _centroid(p):
+    """Blob::get_centroid (src/Blob.cpp:971-983): node sums divided by the node count."""
+    n = p.shape[0]
+    return np.array([_seq_sum(p[:, k]) / n for k in range(3)])
+
+
+def _vec3(text):
+    return [float(v) for v in text.strip().strip("()").split(",")]
+
+
+def _rotation_matrix(vals):
+    """The float matrix Blob::rotate uses (src/Blob.cpp:804-828): three angles in degrees,
+    converted and combined in single precision (libm's cosf / sinf), or nine entries."""
+    f = np.float32
+    if len(vals) == 9:
+        return [f(v) for v in vals]
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.cosf.restype = libm.sinf.restype = ctypes.c_float
+    libm.cosf.argtypes = libm.sinf.argtypes = [ctypes.c_float]
+    deg = np.pi / 180.0
+    xa, ya, za = (f(float(f(v)) * deg) for v in vals)
+    c = lambda a: f(libm.cosf(float(a)))  # noqa: E731
+    s_ = lambda a: f(libm.sinf(float(a)))  # noqa: E731
+    cx, sx, cy, sy, cz, sz = c(xa), s_(xa), c(ya), s_(ya), c(za), s_(za)
+    return [cy * cz, sx * sy * cz - cx * sz, cx * sy * cz + sx * sz,
+            cy * sz, sx * sy * sz + cx * cz, cx * sy * sz - sx * cz,
+            f(-1) * sy, sx * cy, cx * cy]
+
+
+def linearised(pos, topo):
+    """Midpoints set to their edge averages, as Blob::init's linearise_elements leaves them
+    (src/tetra_element_linear.cpp:567-591)."""
+    p = np.array(pos, np.float64)
+    for k, (a, b) in enumerate(((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))):
+        p[topo[:, 4 + k]] = .5 * (p[topo[:, a]] + p[topo[:, b]])
+    return p
+
+
+def place_blobs(params: dict, blobs, positions):
+    """Positions (meso units, [N_t, 3] per blob) after World::init places the blobs:
+    <centroid> (Blob::position, src/Blob.cpp:886-918: the centroid times 1/N, moved to the
+    given point times the blob's scale), <rotation> (Blob::rotate about the centroid), then
+    move_into_box (default 1): every blob shifted so that the system centroid sits at the
+    centre of a box of es_N * ssint_cutoff per axis, es_N = 2 ceil(extent / ssint_cutoff)
+    unless given (and an interaction or electrostatics flag is on)."""
+    out = []
+    for b, p in zip(blobs, positions):
+        p = np.array(p, np.float64)
+        sc = U.blob_scale(float(b.get("scale", 1.0)))
+        key = "centroid" if "centroid" in b else ("centroid_pos" if "centroid_pos" in b else None)
+        if key:
+            target = [v * sc for v in _vec3(b[key])]
+            n = p.shape[0]
+            cen = [_seq_sum(p[:, k]) * (1.0 / n) for k in range(3)]
+            v = [target[k] - cen[k] for k in range(3)]
+            for k in range(3):
+                p[:, k] += v[k]
+        if "rotation" in b:
+            r = [float(x) for x in _rotation_matrix(_vec3(b["rotation"]))]
+            com = _centroid(p)
+            q = p - com
+            x, y, z = q[:, 0], q[:, 1], q[:, 2]
+            p = np.stack([x * r[0] + y * r[1] + z * r[2] + com[0], x * r[3] + y * r[4] + z * r[5] + com[1],
+                          x * r[6] + y * r[7] + z * r[8] + com[2]], axis=1)
+        out.append(p)
+    if int(float(params.get("move_into_box", 1))) != 1:
+        return out
+    # World::get_system_dimensions (src/World.cpp:3413-3466) with Blob::get_min_max
+    # (src/Blob.cpp:2383-2411), whose per-axis update is "max, else min"
+    lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+    for p in out:
+        bmin, bmax = np.full(3, np.inf), np.full(3, -np.inf)
+        for row in p:
+            for k in range(3):
+                if row[k] > bmax[k]:
+                    bmax[k] = row[k]
+                elif row[k] < bmin[k]:
+                    bmin[k] = row[k]
+        lo, hi = np.minimum(lo, bmin), np.maximum(hi, bmax)
+    dim = hi - lo
+    cutoff = float(params.get("ssint_cutoff", params.get("vdw_cutoff", 3e-9))) / U.MESO_LENGTH
+    flags = [int(float(params.get(k, d))) for k, d in (("calc_steric", 1), ("calc_es", 0), ("calc_preComp", 0))]
+    flags.append(int(float(params.get("calc_ssint", params.get("calc_vdw", 1)))))
+    es = [int(float(params.get(f"es_N_{a}", -1))) if any(flags) else 0 for a in "xyz"]
+    if min(es) < 1:
+        es = [2 * int(np.ceil(d / cutoff)) for d in dim]
+    box = np.array([cutoff * n for n in es])
+    # World::get_system_centroid (:3381-3411): blob centroids weighted by their node counts
+    tot = sum(p.shape[0] for p in out)
+    wc = np.zeros(3)
+    for p in out:
+        wc += _centroid(p) * p.shape[0]
+    wc /= tot
+    shift = box / 2.0 - wc
+    return [p + shift for p in out]
+
+
+# ---------------------------------------------------------------------------------------------
 # the world
 # ---------------------------------------------------------------------------------------------
 class World:
@@ -194,12 +303,16 @@ class World:
         self.solver = solvers.pop() if solvers else "CG_nomass"
         self.mass_in_system = self.solver != "CG_nomass"  # src/World.cpp:2545-2556
         self.nodes = [m.num_nodes for m in self.meshes]
-        self.pos0 = [m.pos * sc for m, sc in zip(self.meshes, self.scales)]  # meso units
+        # the rest state comes from the positions as loaded; World::init then places the blobs,
+        # and pos_0 (the RMSD reference) is taken after that (src/World.cpp:502-506)
+        self.pos0 = place_blobs(self.raw, s["blobs"],
+                                [linearised(m.pos * sc, m.topo) for m, sc in zip(self.meshes, self.scales)])
         if self.restart:
             st = read_checkpoint(self.ckpt_in)[0]
         else:
             st = package_seed(self.params.rng_seed)
         self.batch = Batch(self.meshes, self.params, self.scales, rng_state=st, precision=precision)
+        self.batch.set_positions(np.concatenate(self.pos0))
         self.batch.set_solver(self.solver)
         self.calc_springs = int(float(self.raw.get("calc_springs", 0))) != 0
         self.calc_ctforces = int(float(self.raw.get("calc_ctforces", 0))) == 1

@@ -104,3 +104,50 @@ def test_script_interaction_files():
     s = F.read_script(os.path.join(GOLDEN, "physics", "cyl_160_fine_E", "cyl_160_fine-E.ffea"))
     assert s["ctforces"].endswith("cyl_160_fine_E/cyl_160_fine_surf.ctforces") and s["springs"] is None
     assert s["params"]["calc_ctforces"] == "1"
+
+
+def _sphere():
+    from ffea_amd import formats as F, units as U
+    from conftest import SPHERE_FFEA
+    s = F.read_script(SPHERE_FFEA)
+    b = {k: v for k, v in s["blobs"][0].items() if k not in ("centroid", "centroid_pos")}
+    m = F.load_blob(b["nodes"], b["topology"], b["material"], b["stokes"], b.get("pin"))
+    return s, b, m, U.blob_scale(float(b["scale"]))
+
+
+def test_move_into_box_centres_the_system():
+    """World::init (src/World.cpp:455-489): the box is es_N * ssint_cutoff per axis with
+    es_N = 2 ceil(extent / cutoff) (3 nm default cutoff), and the system centroid ends at its
+    centre; move_into_box = 0 leaves the positions alone."""
+    from ffea_amd import units as U, world as W
+    s, b, m, sc = _sphere()
+    p = W.linearised(m.pos * sc, m.topo)
+    (q,) = W.place_blobs({}, [b], [p])
+    cut = 3e-9 / U.MESO_LENGTH
+    box = np.array([2 * np.ceil((p[:, k].max() - p[:, k].min()) / cut) * cut for k in range(3)])
+    assert np.allclose(q.mean(axis=0), box / 2, rtol=0, atol=1e-12 * box.max())
+    assert np.allclose(q - q[0], p - p[0], rtol=0, atol=1e-12 * box.max())  # a pure translation
+    (r,) = W.place_blobs({"move_into_box": "0"}, [b], [p])
+    assert np.array_equal(r, p)
+    # es_N given (and an interaction flag on, as calc_steric is by default): that box
+    (q2,) = W.place_blobs({"es_N_x": "10", "es_N_y": "10", "es_N_z": "10"}, [b], [p])
+    assert np.allclose(q2.mean(axis=0), np.full(3, 5 * cut), rtol=0, atol=1e-12 * cut)
+
+
+def test_blob_centroid_and_rotation():
+    """<centroid> puts the blob's centroid at the point times its scale; <rotation> (degrees
+    about x, y, z, applied about the centroid) of 180 about z maps (x, y) to (-x, -y)."""
+    from ffea_amd import world as W
+    s, b, m, sc = _sphere()
+    p = W.linearised(m.pos * sc, m.topo)
+    bb = dict(b, centroid="(1.0,2.0,-3.0)")
+    (q,) = W.place_blobs({"move_into_box": "0"}, [bb], [p])
+    assert np.allclose(q.mean(axis=0), np.array([1.0, 2.0, -3.0]) * sc, rtol=0, atol=1e-9 * sc)
+    br = dict(b, rotation="(0.0,0.0,180)")
+    (r,) = W.place_blobs({"move_into_box": "0"}, [br], [p])
+    c = p.mean(axis=0)
+    want = np.stack([2 * c[0] - p[:, 0], 2 * c[1] - p[:, 1], p[:, 2]], axis=1)
+    assert np.allclose(r, want, rtol=0, atol=1e-6 * np.abs(p).max())
+    d0 = np.linalg.norm(p[1:] - p[0], axis=1)
+    d1 = np.linalg.norm(r[1:] - r[0], axis=1)
+    assert np.allclose(d1, d0, rtol=1e-6)
