@@ -1,0 +1,199 @@
+// ffh_device.h -- device building blocks: MRG32k3a jumps and draws, fixed-order
+// reductions, per-blob totals, streamed loads, XCD-aware block order, the CG control block.
+// Part of ffh_kernels.hip: included inside its anonymous namespace, after the headers it
+// lists there; not a stand-alone header.
+#pragma once
+
+// ----------------------------------------------------------------------------------------
+// MRG32k3a on the device: exact integer arithmetic mod m = 2^32 - c.
+// ----------------------------------------------------------------------------------------
+constexpr uint32_t kC1 = 4294967296ull - ffh::kM1;  // 209
+constexpr uint32_t kC2 = 4294967296ull - ffh::kM2;  // 22853
+constexpr double kNorm = 1.0 / (4294967087.0 + 1.0);
+
+template <uint32_t M, uint32_t Cc>
+__device__ __forceinline__ uint32_t mred(uint64_t x) {
+    x = (x >> 32) * Cc + (x & 0xffffffffull);
+    x = (x >> 32) * Cc + (x & 0xffffffffull);
+    return (uint32_t)(x >= M ? x - M : x);
+}
+
+template <uint32_t M, uint32_t Cc>
+__device__ __forceinline__ void matvec(const Mat3u &A, uint32_t s[3]) {
+    uint32_t v[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        uint64_t acc = (uint64_t)mred<M, Cc>((uint64_t)A.m[i][0] * s[0]) +
+                       (uint64_t)mred<M, Cc>((uint64_t)A.m[i][1] * s[1]) +
+                       (uint64_t)mred<M, Cc>((uint64_t)A.m[i][2] * s[2]);
+        v[i] = mred<M, Cc>(acc);
+    }
+    s[0] = v[0];
+    s[1] = v[1];
+    s[2] = v[2];
+}
+
+struct Stream6 {
+    uint32_t s[6];
+};
+
+// State of the stream `off` draws after `base`, via the base-256 digit tables
+// tbl[(d*256 + v)*2 + comp] = A_comp^(v 256^d).
+__device__ __forceinline__ Stream6 rng_jump(const uint32_t *__restrict__ base, const Mat3u *__restrict__ tbl,
+                                            uint32_t off) {
+    Stream6 g;
+#pragma unroll
+    for (int k = 0; k < 6; k++) g.s[k] = base[k];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        uint32_t v = (off >> (8 * d)) & 255u;
+        if (v) {
+            matvec<ffh::kM1, kC1>(tbl[(d * 256 + v) * 2 + 0], &g.s[0]);
+            matvec<ffh::kM2, kC2>(tbl[(d * 256 + v) * 2 + 1], &g.s[3]);
+        }
+    }
+    return g;
+}
+
+__device__ __forceinline__ void rng_apply(Stream6 &g, const Mat3u &m1, const Mat3u &m2) {
+    matvec<ffh::kM1, kC1>(m1, &g.s[0]);
+    matvec<ffh::kM2, kC2>(m2, &g.s[3]);
+}
+
+// RngStream::U01 (src/RngStream.cpp:266-289) in exact integers; u rounds as the reference.
+__device__ __forceinline__ double rng_u01(Stream6 &g) {
+    uint32_t p1 = mred<ffh::kM1, kC1>((uint64_t)1403580u * g.s[1]) +
+                  0u;  // a12 * s1
+    uint32_t t1 = mred<ffh::kM1, kC1>((uint64_t)(ffh::kM1 - 810728u) * g.s[0]);
+    uint64_t q1 = (uint64_t)p1 + t1;
+    p1 = (uint32_t)(q1 >= ffh::kM1 ? q1 - ffh::kM1 : q1);
+    g.s[0] = g.s[1];
+    g.s[1] = g.s[2];
+    g.s[2] = p1;
+    uint32_t p2 = mred<ffh::kM2, kC2>((uint64_t)527612u * g.s[5]);
+    uint32_t t2 = mred<ffh::kM2, kC2>((uint64_t)(ffh::kM2 - 1370589u) * g.s[3]);
+    uint64_t q2 = (uint64_t)p2 + t2;
+    p2 = (uint32_t)(q2 >= ffh::kM2 ? q2 - ffh::kM2 : q2);
+    g.s[3] = g.s[4];
+    g.s[4] = g.s[5];
+    g.s[5] = p2;
+    double d = (double)p1 - (double)p2;
+    return (p1 > p2) ? d * kNorm : (d + 4294967087.0) * kNorm;
+}
+
+// RAND(-.5,.5) (include/tetra_element_linear.h:118)
+__device__ __forceinline__ double rand_half(Stream6 &g) { return (-.5) + ((.5) - (-.5)) * rng_u01(g); }
+
+// ----------------------------------------------------------------------------------------
+// Reductions.  A producing kernel writes one partial sum per block (block tree, fixed order);
+// every block of the consuming kernel re-adds those partials in the same fixed order, so all
+// blocks see bitwise the same scalar and runs are reproducible -- no atomics, no fences.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Block sum of v; the total is returned to every thread.  sh: >= blockDim.x/64 doubles.
+__device__ __forceinline__ double block_sum_all(double v, double *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; i++) t += sh[i];
+    return t;
+}
+
+// Writes this block's sum of v to parts[blockIdx.x].
+__device__ __forceinline__ void block_partial(double v, double *__restrict__ parts, double *sh) {
+    double t = block_sum_all(v, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = t;
+}
+
+// Writes this block's sum of v to parts[idx].
+__device__ __forceinline__ void block_partial_at(double v, double *__restrict__ parts, int idx, double *sh) {
+    double t = block_sum_all(v, sh);
+    if (threadIdx.x == 0) parts[idx] = t;
+}
+
+// Fixed-order total of the partials [i0, i1): one blob's slices.
+__device__ __forceinline__ double parts_range(const double *__restrict__ parts, int i0, int i1, double *sh) {
+    double s = 0.0;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) s += parts[i];
+    return block_sum_all(s, sh);
+}
+
+// Per-blob totals.  A blob of many slices would make every consumer block re-add thousands of
+// partials (O(slices^2) work in all); above kTotSlicesMin slices per blob a small kernel adds
+// each blob's partials once (same fixed order) into tot[blob][slot] and consumers read that.
+constexpr int kTotFF = 0, kTotRZ = 1, kTotRR = 2, kTotPQ = 3, kTotUU = 4, kTotZRP = 5, kTotDEN = 6;
+constexpr int kTotSlots = 8;
+constexpr int kTotSlicesMin = 3072;
+
+__device__ __forceinline__ double blob_total(const double *__restrict__ parts, const double *__restrict__ tot, int slot,
+                                             int b, int s0, int s1, double *sh) {
+    if (tot) return tot[b * kTotSlots + slot];
+    return parts_range(parts, s0, s1, sh);
+}
+
+struct TotArgs {
+    const double *parts[kTotSlots];
+    const int32_t *blob_slice_off;
+    double *tot;
+    int mask;
+};
+
+// One block of 1024 threads per blob; each thread adds every 1024th partial, four loads in
+// flight, then a block tree -- a fixed order.
+__global__ __launch_bounds__(1024) void k_blob_totals(TotArgs t) {
+    __shared__ double sh[16];
+    const int b = blockIdx.x;
+    const int s0 = t.blob_slice_off[b], s1 = t.blob_slice_off[b + 1];
+    for (int k = 0; k < kTotSlots; k++)
+        if ((t.mask >> k) & 1) {
+            const double *p = t.parts[k];
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int i = s0 + (int)threadIdx.x;
+            for (; i + 3 * 1024 < s1; i += 4 * 1024) {
+                a0 += p[i];
+                a1 += p[i + 1024];
+                a2 += p[i + 2 * 1024];
+                a3 += p[i + 3 * 1024];
+            }
+            for (; i < s1; i += 1024) a0 += p[i];
+            const double v = block_sum_all((a0 + a1) + (a2 + a3), sh);
+            if (threadIdx.x == 0) t.tot[b * kTotSlots + k] = v;
+        }
+}
+
+// Streamed-once loads of the SpMV (matrix values, column indices).  Non-temporal (kNt) they do
+// not push the gathered vector out of the XCD's L2 -- a gain when the matrix streams from HBM,
+// a loss when it fits the Infinity Cache and would be re-read from there; the tuner decides.
+template <bool kNt, typename X>
+__device__ __forceinline__ X stream_load(const X *p) {
+    if constexpr (kNt)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The gathering
+// kernels remap the dispatch index so that XCD x works on the contiguous logical blocks
+// [x per, (x+1) per): neighbouring rows, elements and records then share one L2.  The grid is
+// launched with 8 per blocks; logical blocks >= n do no work.
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int xcd_grid(int n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+__device__ __forceinline__ int xcd_block(int n) {
+    const int per = (n + kXcds - 1) / kXcds;
+    return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+}
+
+// CG control block (device memory), written by one thread of the deciding kernel.
+struct CgCtl {
+    double res, f2;
+    double delta[2];  // delta_k = r_k . D^-1 r_k, slot k & 1
+    int iter, done, status, inversions;
+};

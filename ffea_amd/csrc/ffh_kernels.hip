@@ -1,23 +1,26 @@
-// ffh_kernels.hip -- gfx950 kernels of the massless FFEA time step and the C-ABI
-// (include/ffea_hip.h) that drives them.
-//
-// Kernels (one blob, everything resident in HBM, one HIP stream):
-//   k_element     one element per lane: Jacobian, shape gradients, volume + inversion,
-//                 Kelvin-Voigt viscosity blocks, elastic + fluctuating stress, nodal forces
-//                 (src/tetra_element_linear.cpp:193-490, src/Blob.cpp:675-724)
-//   k_assemble    one SELL slot per lane: sums the element blocks of its (row, col) in
-//                 ascending element order (the reference's source-list order,
-//                 src/SparseMatrixFixedPattern.cpp:85-93, src/SparseMatrixTypes.cpp:52-58),
-//                 adds Stokes drag on the diagonal, writes the Jacobi inverse diagonal
-//   k_rhs         one node per lane: force gather in element order, Stokes noise, pinned
-//                 rows, CG initialisation (src/Blob.cpp:3826-3923, src/NoMassCGSolver.cpp:205-230)
-//   k_cg_spmv     p = beta p + D^-1 r and q = K p (3x3-block SELL SpMV), p.q, alpha
-//   k_cg_update   x += alpha p, r -= alpha q, ||r||^2/||f||^2 check, beta
-//   k_integrate   v = x, pos += v dt, RngStream advanced by one step's draws
+// ffh_kernels.hip -- gfx950 kernels of FFEA's time step and the C-ABI (include/ffea_hip.h)
+// that drives them.  One translation unit; the device code sits in headers by subject:
+//   ffh_device.h   MRG32k3a jumps and draws, fixed-order reductions, per-blob totals
+//   ffh_element.h  k_element (Jacobian, shape gradients, volume + inversion, viscous and
+//                  elastic + fluctuating stress, nodal forces; src/tetra_element_linear.cpp:193-490,
+//                  src/Blob.cpp:638-735) and the assembly of the viscosity matrix (k_assemble,
+//                  k_assemble2: each SELL block the sum of its element blocks in ascending element
+//                  order, src/SparseMatrixFixedPattern.cpp:85-93)
+//   ffh_forces.h   springs, constant and surface forces (World::apply_springs,
+//                  Blob::apply_ctforces)
+//   ffh_solve.h    k_rhs (force gather in element order, drag and noise, pins, CG start or the
+//                  lumped solve; src/Blob.cpp:3826-3923), the CG kernels (k_cg_spmv / k_cg_spmv1,
+//                  k_cg_update, k_blob_totals; src/NoMassCGSolver.cpp:165-292 and
+//                  src/ConjugateGradientSolver.cpp:205-349), k_integrate (src/Blob.cpp:3937-3967),
+//                  energies
+//   ffh_mpnn.h     the learned preconditioner
+// This file holds the per-handle state (ffh_blob: buffers, launches, step graphs, tuning) and
+// the C-ABI.
 //
 // The FP64 path rounds exactly as the reference's strict build (no FMA contraction: the
 // file is compiled with -ffp-contract=off); only the order of the CG dot-product sums
 // differs (tree instead of sequential), deterministically.
+
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -56,1259 +59,10 @@ int fail(int code, const std::string &msg) {
 constexpr int kBlock = 256;
 constexpr int kLaneTable = 4 * 256 * 2;  // jump[kLaneTable + 2 l + comp] = A_comp^(7 l), l < 64
 
-// ----------------------------------------------------------------------------------------
-// MRG32k3a on the device: exact integer arithmetic mod m = 2^32 - c.
-// ----------------------------------------------------------------------------------------
-constexpr uint32_t kC1 = 4294967296ull - ffh::kM1;  // 209
-constexpr uint32_t kC2 = 4294967296ull - ffh::kM2;  // 22853
-constexpr double kNorm = 1.0 / (4294967087.0 + 1.0);
-
-template <uint32_t M, uint32_t Cc>
-__device__ __forceinline__ uint32_t mred(uint64_t x) {
-    x = (x >> 32) * Cc + (x & 0xffffffffull);
-    x = (x >> 32) * Cc + (x & 0xffffffffull);
-    return (uint32_t)(x >= M ? x - M : x);
-}
-
-template <uint32_t M, uint32_t Cc>
-__device__ __forceinline__ void matvec(const Mat3u &A, uint32_t s[3]) {
-    uint32_t v[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        uint64_t acc = (uint64_t)mred<M, Cc>((uint64_t)A.m[i][0] * s[0]) +
-                       (uint64_t)mred<M, Cc>((uint64_t)A.m[i][1] * s[1]) +
-                       (uint64_t)mred<M, Cc>((uint64_t)A.m[i][2] * s[2]);
-        v[i] = mred<M, Cc>(acc);
-    }
-    s[0] = v[0];
-    s[1] = v[1];
-    s[2] = v[2];
-}
-
-struct Stream6 {
-    uint32_t s[6];
-};
-
-// State of the stream `off` draws after `base`, via the base-256 digit tables
-// tbl[(d*256 + v)*2 + comp] = A_comp^(v 256^d).
-__device__ __forceinline__ Stream6 rng_jump(const uint32_t *__restrict__ base, const Mat3u *__restrict__ tbl,
-                                            uint32_t off) {
-    Stream6 g;
-#pragma unroll
-    for (int k = 0; k < 6; k++) g.s[k] = base[k];
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-        uint32_t v = (off >> (8 * d)) & 255u;
-        if (v) {
-            matvec<ffh::kM1, kC1>(tbl[(d * 256 + v) * 2 + 0], &g.s[0]);
-            matvec<ffh::kM2, kC2>(tbl[(d * 256 + v) * 2 + 1], &g.s[3]);
-        }
-    }
-    return g;
-}
-
-__device__ __forceinline__ void rng_apply(Stream6 &g, const Mat3u &m1, const Mat3u &m2) {
-    matvec<ffh::kM1, kC1>(m1, &g.s[0]);
-    matvec<ffh::kM2, kC2>(m2, &g.s[3]);
-}
-
-// RngStream::U01 (src/RngStream.cpp:266-289) in exact integers; u rounds as the reference.
-__device__ __forceinline__ double rng_u01(Stream6 &g) {
-    uint32_t p1 = mred<ffh::kM1, kC1>((uint64_t)1403580u * g.s[1]) +
-                  0u;  // a12 * s1
-    uint32_t t1 = mred<ffh::kM1, kC1>((uint64_t)(ffh::kM1 - 810728u) * g.s[0]);
-    uint64_t q1 = (uint64_t)p1 + t1;
-    p1 = (uint32_t)(q1 >= ffh::kM1 ? q1 - ffh::kM1 : q1);
-    g.s[0] = g.s[1];
-    g.s[1] = g.s[2];
-    g.s[2] = p1;
-    uint32_t p2 = mred<ffh::kM2, kC2>((uint64_t)527612u * g.s[5]);
-    uint32_t t2 = mred<ffh::kM2, kC2>((uint64_t)(ffh::kM2 - 1370589u) * g.s[3]);
-    uint64_t q2 = (uint64_t)p2 + t2;
-    p2 = (uint32_t)(q2 >= ffh::kM2 ? q2 - ffh::kM2 : q2);
-    g.s[3] = g.s[4];
-    g.s[4] = g.s[5];
-    g.s[5] = p2;
-    double d = (double)p1 - (double)p2;
-    return (p1 > p2) ? d * kNorm : (d + 4294967087.0) * kNorm;
-}
-
-// RAND(-.5,.5) (include/tetra_element_linear.h:118)
-__device__ __forceinline__ double rand_half(Stream6 &g) { return (-.5) + ((.5) - (-.5)) * rng_u01(g); }
-
-// ----------------------------------------------------------------------------------------
-// Reductions.  A producing kernel writes one partial sum per block (block tree, fixed order);
-// every block of the consuming kernel re-adds those partials in the same fixed order, so all
-// blocks see bitwise the same scalar and runs are reproducible -- no atomics, no fences.
-// ----------------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// Block sum of v; the total is returned to every thread.  sh: >= blockDim.x/64 doubles.
-__device__ __forceinline__ double block_sum_all(double v, double *sh) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    for (int i = 0; i < nw; i++) t += sh[i];
-    return t;
-}
-
-// Writes this block's sum of v to parts[blockIdx.x].
-__device__ __forceinline__ void block_partial(double v, double *__restrict__ parts, double *sh) {
-    double t = block_sum_all(v, sh);
-    if (threadIdx.x == 0) parts[blockIdx.x] = t;
-}
-
-// Writes this block's sum of v to parts[idx].
-__device__ __forceinline__ void block_partial_at(double v, double *__restrict__ parts, int idx, double *sh) {
-    double t = block_sum_all(v, sh);
-    if (threadIdx.x == 0) parts[idx] = t;
-}
-
-// Fixed-order total of the partials [i0, i1): one blob's slices.
-__device__ __forceinline__ double parts_range(const double *__restrict__ parts, int i0, int i1, double *sh) {
-    double s = 0.0;
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) s += parts[i];
-    return block_sum_all(s, sh);
-}
-
-// Per-blob totals.  A blob of many slices would make every consumer block re-add thousands of
-// partials (O(slices^2) work in all); above kTotSlicesMin slices per blob a small kernel adds
-// each blob's partials once (same fixed order) into tot[blob][slot] and consumers read that.
-constexpr int kTotFF = 0, kTotRZ = 1, kTotRR = 2, kTotPQ = 3, kTotUU = 4, kTotZRP = 5, kTotDEN = 6;
-constexpr int kTotSlots = 8;
-constexpr int kTotSlicesMin = 3072;
-
-__device__ __forceinline__ double blob_total(const double *__restrict__ parts, const double *__restrict__ tot, int slot,
-                                             int b, int s0, int s1, double *sh) {
-    if (tot) return tot[b * kTotSlots + slot];
-    return parts_range(parts, s0, s1, sh);
-}
-
-struct TotArgs {
-    const double *parts[kTotSlots];
-    const int32_t *blob_slice_off;
-    double *tot;
-    int mask;
-};
-
-// One block of 1024 threads per blob; each thread adds every 1024th partial, four loads in
-// flight, then a block tree -- a fixed order.
-__global__ __launch_bounds__(1024) void k_blob_totals(TotArgs t) {
-    __shared__ double sh[16];
-    const int b = blockIdx.x;
-    const int s0 = t.blob_slice_off[b], s1 = t.blob_slice_off[b + 1];
-    for (int k = 0; k < kTotSlots; k++)
-        if ((t.mask >> k) & 1) {
-            const double *p = t.parts[k];
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int i = s0 + (int)threadIdx.x;
-            for (; i + 3 * 1024 < s1; i += 4 * 1024) {
-                a0 += p[i];
-                a1 += p[i + 1024];
-                a2 += p[i + 2 * 1024];
-                a3 += p[i + 3 * 1024];
-            }
-            for (; i < s1; i += 1024) a0 += p[i];
-            const double v = block_sum_all((a0 + a1) + (a2 + a3), sh);
-            if (threadIdx.x == 0) t.tot[b * kTotSlots + k] = v;
-        }
-}
-
-// Streamed-once loads of the SpMV (matrix values, column indices).  Non-temporal (kNt) they do
-// not push the gathered vector out of the XCD's L2 -- a gain when the matrix streams from HBM,
-// a loss when it fits the Infinity Cache and would be re-read from there; the tuner decides.
-template <bool kNt, typename X>
-__device__ __forceinline__ X stream_load(const X *p) {
-    if constexpr (kNt)
-        return __builtin_nontemporal_load(p);
-    else
-        return *p;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The gathering
-// kernels remap the dispatch index so that XCD x works on the contiguous logical blocks
-// [x per, (x+1) per): neighbouring rows, elements and records then share one L2.  The grid is
-// launched with 8 per blocks; logical blocks >= n do no work.
-constexpr int kXcds = 8;
-__host__ __device__ __forceinline__ int xcd_grid(int n) { return kXcds * ((n + kXcds - 1) / kXcds); }
-__device__ __forceinline__ int xcd_block(int n) {
-    const int per = (n + kXcds - 1) / kXcds;
-    return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
-}
-
-// CG control block (device memory), written by one thread of the deciding kernel.
-struct CgCtl {
-    double res, f2;
-    double delta[2];  // delta_k = r_k . D^-1 r_k, slot k & 1
-    int iter, done, status, inversions;
-};
-
-// ----------------------------------------------------------------------------------------
-// Element kernel
-// ----------------------------------------------------------------------------------------
-struct ElemArgs {
-    const double4 *pos;  // (x, y, z, 0) per linear node: one 32-byte load per node
-    const int4 *enodes;
-    const double *jinv0;  // SoA [9][E]
-    const double *vol0, *A, *B, *G, *Em;
-    double *last_det;
-    void *eforce;  // T SoA [12][E]: node_force[a][i] at (3a+i)*E + e
-    int E;
-    int calc_noise;
-    double kT, dt;
-    const uint32_t *rng_state;
-    const Mat3u *jump;
-    uint32_t elem_base;
-    CgCtl *ctl;
-    const double *vel;  // [3N] node velocities (mass-ful solvers: the viscous force K_e u_e)
-};
-
-// Entry (R, C) of the element viscosity matrix (create_viscosity_matrix,
-// src/tetra_element_linear.cpp:299-328, with the macros of include/tetra_element_linear.h:55-116
-// and the diffusion term :599-638): the upper triangle as built, the lower one its mirror.
-template <typename R>
-__device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, int Cc) {
-    if (Rr > Cc) {
-        int t = Rr;
-        Rr = Cc;
-        Cc = t;
-    }
-    const int i = Rr >> 2, a = Rr & 3, j = Cc >> 2, b = Cc & 3;
-    R v;
-    if (i == j) {
-        R del2 = dpsi[a] * dpsi[b] + dpsi[4 + a] * dpsi[4 + b] + dpsi[8 + a] * dpsi[8 + b];
-        v = dpsi[Rr] * dpsi[Cc] * (A + B);
-        v += del2 * A;
-    } else if (a == b) {
-        v = (B + A) * (dpsi[Rr] * dpsi[Cc]);
-    } else {
-        v = B * (dpsi[Rr] * dpsi[Cc]) + A * (dpsi[4 * i + b] * dpsi[4 * j + a]);
-    }
-    return v * vol;
-}
-
-// calculate_jacobian (src/tetra_element_linear.cpp:193-205), from FP64 positions, and
-// calc_shape_function_derivatives_and_volume (:223-293): J, det and the shape gradients
-// dpsi[a + 4 dim] (x-major).  Shared by the element kernel and the assembly, which recomputes
-// them per contribution: the same code rounds the same way, so the matrix is unchanged.
-template <typename R>
-__device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos, const int4 n, R J[3][3],
-                                                 R dpsi[12], R &det) {
-    const double4 p0 = pos[n.x], p1 = pos[n.y], p2 = pos[n.z], p3 = pos[n.w];
-    const double4 pr[3] = {p1, p2, p3};
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        J[r][0] = (R)(pr[r].x - p0.x);
-        J[r][1] = (R)(pr[r].y - p0.y);
-        J[r][2] = (R)(pr[r].z - p0.z);
-    }
-    dpsi[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
-    dpsi[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
-    dpsi[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
-    dpsi[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
-    dpsi[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
-    dpsi[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
-    dpsi[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
-    dpsi[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
-    dpsi[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
-    det = J[0][0] * dpsi[1] + J[1][0] * dpsi[2] + J[2][0] * dpsi[3];
-    const R dinv = (R)1.0 / det;
-    dpsi[1] *= dinv;
-    dpsi[2] *= dinv;
-    dpsi[3] *= dinv;
-    dpsi[5] *= dinv;
-    dpsi[6] *= dinv;
-    dpsi[7] *= dinv;
-    dpsi[9] *= dinv;
-    dpsi[10] *= dinv;
-    dpsi[11] *= dinv;
-    dpsi[0] = -(dpsi[1] + dpsi[2] + dpsi[3]);
-    dpsi[4] = -(dpsi[5] + dpsi[6] + dpsi[7]);
-    dpsi[8] = -(dpsi[9] + dpsi[10] + dpsi[11]);
-}
-
-template <typename R, typename T, bool kMass>
-__global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
-    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned); waves stay 64-aligned
-    const int e = xcd_block((g.E + bd - 1) / bd) * bd + threadIdx.x;
-    if (e >= g.E) return;
-    R J[3][3], dpsi[12], det;
-    element_geometry<R>(g.pos, g.enodes[e], J, dpsi, det);
-    const double last = g.last_det[e];
-    if (last * (double)det < 0) atomicAdd(&g.ctl->inversions, 1);
-    g.last_det[e] = (double)det;
-    const R vol = (R)(1.0 / 6.0) * fabs(det);
-
-    const R A = (R)g.A[e], B = (R)g.B[e], G = (R)g.G[e], Em = (R)g.Em[e], vol0 = (R)g.vol0[e];
-
-    // add_shear_elastic_stress (:378-396): F_ij = sum_k J[k][i] Jinv0[j][k]; S = F F^T
-    R F[3][3], S[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            R f = (R)0;
-#pragma unroll
-            for (int k = 0; k < 3; k++) f += J[k][i] * (R)g.jinv0[(size_t)(3 * j + k) * g.E + e];
-            F[i][j] = f;
-        }
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            R s = (R)0;
-#pragma unroll
-            for (int k = 0; k < 3; k++) s += F[i][k] * F[j][k];
-            S[i][j] = s;
-        }
-    const R sc = (G * vol0 / vol);
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) S[i][j] *= sc;
-    S[0][0] -= G;
-    S[1][1] -= G;
-    S[2][2] -= G;
-    // add_bulk_elastic_stress (:401-408)
-    {
-        const R c_2 = Em - G * (R)2.0 / (R)3.0;
-        const R c = G * ((R)1.0 - (vol0 / vol)) + (R)0.5 * c_2 * ((vol / vol0) - (vol0 / vol));
-        S[0][0] += c;
-        S[1][1] += c;
-        S[2][2] += c;
-    }
-    // add_fluctuating_stress (:417-436): 7 draws at this element's stream offset
-    if (g.calc_noise) {
-        // the wave's first element's state (uniform), then A^(7 lane) for this lane
-        const int lane = threadIdx.x & 63;
-        const uint32_t e0 = __builtin_amdgcn_readfirstlane((uint32_t)(e - lane));
-        Stream6 st = rng_jump(g.rng_state, g.jump, g.elem_base + 7u * e0);
-        rng_apply(st, g.jump[kLaneTable + 2 * lane], g.jump[kLaneTable + 2 * lane + 1]);
-        const R c = sqrt(((R)24 * (R)g.kT) / (vol * (R)g.dt));
-        const R bf = sqrt(B) * (R)rand_half(st);
-        S[0][0] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
-        S[1][1] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
-        S[2][2] += c * (sqrt((R)2 * A) * (R)rand_half(st) + bf);
-        S[0][1] += c * sqrt(A) * (R)rand_half(st);
-        S[0][2] += c * sqrt(A) * (R)rand_half(st);
-        S[1][2] += c * sqrt(A) * (R)rand_half(st);
-        S[1][0] = S[0][1];
-        S[2][0] = S[0][2];
-        S[2][1] = S[1][2];
-    }
-    // The mass-ful solvers start du from the viscous force K_e u_e (src/Blob.cpp:703-706):
-    // get_element_velocity_vector (:454-469, x-major over the vertices) and mat12_apply
-    // (src/mat_vec_fns.cpp:26-34, each row summed in column order from zero); the massless
-    // solver starts it from zero.
-    R kv[12];
-    if constexpr (kMass) {
-        const int4 nn = g.enodes[e];
-        const int ni[4] = {nn.x, nn.y, nn.z, nn.w};
-        R u[12];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int a = 0; a < 4; a++) u[4 * i + a] = (R)g.vel[3 * (size_t)ni[a] + i];
-#pragma unroll
-        for (int rr = 0; rr < 12; rr++) {
-            R acc = (R)0;
-#pragma unroll
-            for (int cc = 0; cc < 12; cc++) acc += visc_entry<R>(dpsi, A, B, vol, rr, cc) * u[cc];
-            kv[rr] = acc;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 12; k++) kv[k] = (R)0;
-    }
-    // apply_stress_tensor (:441-449) + add_element_force_vector (:474-490)
-    T fo[12];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            R du = kv[4 * i + a] + vol * (dpsi[a] * S[i][0] + dpsi[4 + a] * S[i][1] + dpsi[8 + a] * S[i][2]);
-            fo[3 * a + i] = (T)((R)0 - du);
-        }
-    T *ef = reinterpret_cast<T *>(g.eforce);
-#pragma unroll
-    for (int k = 0; k < 12; k++) ef[(size_t)k * g.E + e] = fo[k];
-}
-
-// ----------------------------------------------------------------------------------------
-// Assembly over the upper triangle of the block pattern
-// ----------------------------------------------------------------------------------------
-struct AsmArgs {
-    const double4 *pos;  // positions: the element geometry is recomputed per contribution
-    const uint32_t *flat, *eoff;  // contributions entry-major, [eoff[u], eoff[u+1]) for entry u
-    const int4 *enodes;
-    const double *A, *B;
-    const int64_t *toff;
-    const uint16_t *twidth, *cnt;
-    const uint32_t *ctb;
-    const int32_t *slot, *mirror, *node;
-    const uint8_t *flags;
-    const double *zeta;
-    void *val;  // T tiles
-    double *dinv;
-    int64_t nupper;
-};
-
-// SparseMatrixFixedPattern::build (src/SparseMatrixFixedPattern.cpp:85-93): each entry is the
-// sum of its sources in registration order -- ascending element, Stokes drag last
-// (src/NoMassCGSolver.cpp:93-139, src/SparseMatrixTypes.cpp:52-58).  K is symmetric entry by
-// entry (each element block pair is an exact mirror), so the lower block is the transposed
-// upper one.  Diagonal blocks also give calc_inverse_diagonal (:325-333).
-//
-// Block (a,b) of an element, entry (i,j) = V[a+4i][b+4j] with g_a = grad psi_a, g_b = grad psi_b
-// (dpsi[4i+a] = g_a[i], recomputed from the positions); these are the expressions of visc_entry with
-// every product written in an order that rounds identically:
-//   i == j : ((g_a[i] g_b[i]) (A+B) + (g_a.g_b) A) vol
-//   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
-template <typename T, int kUnroll>
-__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
-    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned)
-    const int64_t u = (int64_t)xcd_block((int)((g.nupper + bd - 1) / bd)) * bd + threadIdx.x;
-    if (u >= g.nupper) return;
-    const uint8_t fl = g.flags[u];
-    if (fl & 4) return;  // tile padding
-    const bool diag = (fl & 1) != 0;
-    T acc[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-    const int64_t grp = u >> 6;
-    const int lane = (int)(u & 63);
-    const int cnt = g.cnt[u];
-    const uint32_t *tl = g.ctb + g.toff[grp] + lane;
-#pragma unroll kUnroll
-    for (int k = 0; k < cnt; k++) {
-        const uint32_t code = tl[64 * k];
-        const int a = (code >> 2) & 3, b = code & 3;
-        const int e = (int)(code >> 4);
-        T J[3][3], dpsi[12], det;
-        element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
-        const T vol = (T)(1.0 / 6.0) * fabs(det);
-        const T A = (T)g.A[e], B = (T)g.B[e];
-        T ga[3], gb[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {  // selects, not a dynamic register index
-            const T *d = dpsi + 4 * i;
-            ga[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : d[3];
-            gb[i] = b == 0 ? d[0] : b == 1 ? d[1] : b == 2 ? d[2] : d[3];
-        }
-        const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                T v;
-                if (i == j) {
-                    v = ga[i] * gb[i] * (A + B);
-                    v += del2 * A;
-                } else if (diag) {
-                    v = (B + A) * (ga[i] * gb[j]);
-                } else {
-                    v = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
-                }
-                acc[3 * i + j] += v * vol;
-            }
-    }
-    const int node = g.node[u];
-    if (diag) {
-        if (fl & 2) {
-#pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-            acc[0] = acc[4] = acc[8] = (T)1.0;
-        } else {
-            const T z = (T)g.zeta[node];
-            acc[0] += z;
-            acc[4] += z;
-            acc[8] += z;
-        }
-        g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[0];
-        g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[4];
-        g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[8];
-    }
-    T *v = reinterpret_cast<T *>(g.val);
-    {
-        const int64_t slot = g.slot[u];
-        const int64_t tile = slot & ~(int64_t)63;
-        const int lane = (int)(slot & 63);
-#pragma unroll
-        for (int k = 0; k < 9; k++) v[tile * 9 + k * 64 + lane] = acc[k];
-    }
-    const int32_t mslot = g.mirror[u];
-    if (mslot >= 0) {
-        const int64_t tile = (int64_t)mslot & ~(int64_t)63;
-        const int lane = mslot & 63;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) v[tile * 9 + (3 * i + j) * 64 + lane] = acc[3 * j + i];
-    }
-}
-
-// One contribution (element block (a,b) of code) of an upper block, entry by entry, premultiplied
-// by vol as k_assemble adds it.  a == b exactly for the diagonal blocks.
-template <typename T>
-__device__ __forceinline__ void contribution(const AsmArgs &g, uint32_t code, T w[9]) {
-    const int a = (code >> 2) & 3, b = code & 3;
-    const int e = (int)(code >> 4);
-    T J[3][3], dpsi[12], det;
-    element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
-    const T vol = (T)(1.0 / 6.0) * fabs(det);
-    const T A = (T)g.A[e], B = (T)g.B[e];
-    T ga[3], gb[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const T *d = dpsi + 4 * i;
-        ga[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : d[3];
-        gb[i] = b == 0 ? d[0] : b == 1 ? d[1] : b == 2 ? d[2] : d[3];
-    }
-    const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            T v;
-            if (i == j) {
-                v = ga[i] * gb[i] * (A + B);
-                v += del2 * A;
-            } else if (a == b) {
-                v = (B + A) * (ga[i] * gb[j]);
-            } else {
-                v = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
-            }
-            w[3 * i + j] = v * vol;
-        }
-}
-
-// Two-phase assembly: one workgroup per tile of 64 upper blocks.  Phase 1: all 256 threads
-// evaluate the tile's contributions (entry-major, each block's in ascending element order) in
-// chunks of 256, nine values each into LDS -- independent gathers in flight instead of one
-// dependent chain per lane.  Phase 2: the lane of each block adds its contributions of the
-// chunk in order.  The values and the order of additions are k_assemble's: bit-identical.
-template <typename T>
-__global__ __launch_bounds__(256) void k_assemble2(AsmArgs g) {
-    __shared__ T lv[9][256];
-    const int ntiles = (int)(g.nupper >> 6);
-    const int tile = xcd_block(ntiles);
-    if (tile >= ntiles) return;
-    const int t = threadIdx.x;
-    const int64_t u0 = (int64_t)tile * 64;
-    const uint32_t c0 = g.eoff[u0], c1 = g.eoff[u0 + 64];
-    uint32_t s = 0, en = 0;
-    if (t < 64) {
-        s = g.eoff[u0 + t];
-        en = g.eoff[u0 + t + 1];
-    }
-    T acc[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-    for (uint32_t ch = c0; ch < c1; ch += 256) {
-        const uint32_t f = ch + t;
-        if (f < c1) {
-            T w[9];
-            contribution<T>(g, g.flat[f], w);
-#pragma unroll
-            for (int k = 0; k < 9; k++) lv[k][t] = w[k];
-        }
-        __syncthreads();
-        if (t < 64) {
-            const uint32_t lo = s > ch ? s : ch, hi = en < ch + 256 ? en : ch + 256;
-            for (uint32_t i = lo; i < hi; i++)
-#pragma unroll
-                for (int k = 0; k < 9; k++) acc[k] += lv[k][i - ch];
-        }
-        __syncthreads();
-    }
-    if (t >= 64) return;
-    const int64_t u = u0 + t;
-    const uint8_t fl = g.flags[u];
-    if (fl & 4) return;  // tile padding
-    const bool diag = (fl & 1) != 0;
-    const int node = g.node[u];
-    if (diag) {
-        if (fl & 2) {
-#pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-            acc[0] = acc[4] = acc[8] = (T)1.0;
-        } else {
-            const T z = (T)g.zeta[node];
-            acc[0] += z;
-            acc[4] += z;
-            acc[8] += z;
-        }
-        g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[0];
-        g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[4];
-        g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[8];
-    }
-    T *v = reinterpret_cast<T *>(g.val);
-    {
-        const int64_t slot = g.slot[u];
-        const int64_t tl = slot & ~(int64_t)63;
-        const int ln = (int)(slot & 63);
-#pragma unroll
-        for (int k = 0; k < 9; k++) v[tl * 9 + k * 64 + ln] = acc[k];
-    }
-    const int32_t mslot = g.mirror[u];
-    if (mslot >= 0) {
-        const int64_t tl = (int64_t)mslot & ~(int64_t)63;
-        const int ln = mslot & 63;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) v[tl * 9 + (3 * i + j) * 64 + ln] = acc[3 * j + i];
-    }
-}
-
-// ----------------------------------------------------------------------------------------
-// RHS: force gather, Stokes noise, pinned rows, CG initialisation.  64 nodes x 3 components
-// per block (wave = component).
-// ----------------------------------------------------------------------------------------
-struct CgArgs {
-    int N, nslices;
-    double *f, *r, *x, *p, *q, *z, *vel;
-    const double *dinv;
-    double *parts_rz, *parts_rr, *parts_ff, *parts_pq;  // one partial per slice
-    const int32_t *slice_blob, *blob_slice_off;
-    double eps2;
-    int maxit;
-    CgCtl *ctl;  // one per blob
-    // learned preconditioner (nullptr otherwise): partials of |D^-1 r|^2, r before the update,
-    // partials of z.r_prev for the flexible beta
-    double *parts_uu, *r_prev, *parts_zrp;
-    const double *tot;  // per-blob totals (large blobs) or nullptr: consumers add the partials
-};
-
-// Forces from outside the element loop (ffh::ForcePlan on the device); null lists: none.
-struct ForceArgs {
-    const int32_t *pre_off, *pre_item;  // springs then constant forces, before the element forces
-    const int2 *spring_nodes;
-    const double2 *spring_kl;
-    const double *cforce;
-    const int32_t *grp_off, *grp_ent_off, *grp_ent;  // surface-face groups, after them
-    const double *trac;                               // [3 * entries], this step's tractions
-    // midpoints carrying a force: their constant items and face groups, this step's totals,
-    // and per vertex the fold terms (three midpoint indices each, -1 for none)
-    const int32_t *mid_pre_off, *mid_grp_off;
-    double *mf;
-    const int32_t *fold_off, *fold_mid;
-    const double4 *pos;
-};
-
-// The sum of a node's face groups for component i: each group is one face's force (its set
-// tractions summed in set order from zero), added face by face.
-__device__ __forceinline__ double face_groups(const ForceArgs &fa, int q0, int q1, int i, double f) {
-    for (int q = q0; q < q1; q++) {
-        double sf = 0.0;
-        const int t1 = fa.grp_ent_off[q + 1];
-        for (int t = fa.grp_ent_off[q]; t < t1; t++) sf += fa.trac[3 * (size_t)fa.grp_ent[t] + i];
-        f += sf;
-    }
-    return f;
-}
-
-// This step's force on every midpoint that carries one: constant forces, then face forces
-// (the midpoint's element forces are zero, its drag too), before linearise_force moves it.
-__global__ __launch_bounds__(256) void k_mid_forces(ForceArgs fa, int M) {
-    const int m = blockIdx.x * 256 + threadIdx.x;
-    if (m >= M) return;
-    for (int i = 0; i < 3; i++) {
-        double f = 0.0;
-        if (fa.mid_pre_off)
-            for (int t = fa.mid_pre_off[m]; t < fa.mid_pre_off[m + 1]; t++) f += fa.cforce[3 * (size_t)(~fa.pre_item[t]) + i];
-        if (fa.mid_grp_off) f = face_groups(fa, fa.mid_grp_off[m], fa.mid_grp_off[m + 1], i, f);
-        fa.mf[3 * (size_t)m + i] = f;
-    }
-}
-
-// Component i of spring s's force on its end `end` (World::apply_springs,
-// src/World.cpp:3812-3842): direction by arr3Normalise2 (src/mat_vec_fns_II.cpp:164-176, zero
-// for coincident ends), magnitude k (|sep| - l) with mag (:229-239), the far end negated.
-__device__ __forceinline__ double spring_component(const ForceArgs &fa, int s, int end, int i) {
-    const int2 nn = fa.spring_nodes[s];
-    const double4 p0 = fa.pos[nn.x], p1 = fa.pos[nn.y];
-    const double sx = p1.x - p0.x, sy = p1.y - p0.y, sz = p1.z - p0.z;
-    double norm = 0.0;
-    norm += sx * sx;
-    norm += sy * sy;
-    norm += sz * sz;
-    const double c = i == 0 ? sx : (i == 1 ? sy : sz);
-    const double len = sqrt(norm);
-    const double dir = norm == 0.0 ? 0.0 : c / len;
-    const double2 kl = fa.spring_kl[s];
-    const double fm = kl.x * (len - kl.y);
-    return end == 0 ? fm * dir : (-1.0 * fm) * dir;
-}
-
-// A face corner's position: a vertex, or an edge midpoint as linearise_element places it
-// (.5 (a + b), src/tetra_element_linear.cpp:567-591); a vertex is stored as (v, v), for which
-// the same expression is exact.
-__device__ __forceinline__ double4 corner(const double4 *pos, const int32_t *ab) {
-    const double4 a = pos[ab[0]], b = pos[ab[1]];
-    return make_double4(.5 * (a.x + b.x), .5 * (a.y + b.y), .5 * (a.z + b.z), 0.0);
-}
-
-// Face::get_area (src/Face.cpp:244-261); ends: the three corners' edge-end pairs.
-__device__ __forceinline__ double face_area(const double4 *pos, const int32_t *ends) {
-    const double4 a0 = corner(pos, ends), a1 = corner(pos, ends + 2), a2 = corner(pos, ends + 4);
-    const double ax = a1.x - a0.x, ay = a1.y - a0.y, az = a1.z - a0.z;
-    const double bx = a2.x - a0.x, by = a2.y - a0.y, bz = a2.z - a0.z;
-    const double tx = ay * bz - az * by, ty = az * bx - ax * bz, tz = ax * by - ay * bx;
-    return .5 * sqrt(tx * tx + ty * ty + tz * tz);
-}
-
-// Linear surface forces (Blob::apply_ctforces CTF 3, src/Blob.cpp:2137-2159): one workgroup
-// per set; each face's share of the set's force is its area over three times the set's total
-// area, summed in the set's face order by one thread.
-struct FaceArgs {
-    const int32_t *set_off, *ent_ends;
-    const double *set_force;
-    const double4 *pos;
-    double *area, *trac;
-};
-__global__ __launch_bounds__(256) void k_face_tractions(FaceArgs a) {
-    __shared__ double total;
-    const int s = blockIdx.x;
-    const int e0 = a.set_off[s], e1 = a.set_off[s + 1];
-    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) a.area[e] = face_area(a.pos, a.ent_ends + 6 * (size_t)e);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0;
-        for (int e = e0; e < e1; e++) t += a.area[e];
-        total = t;
-    }
-    __syncthreads();
-    const double t3 = 3 * total;
-    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) {
-        const double f = a.area[e] / t3;
-        for (int k = 0; k < 3; k++) a.trac[3 * (size_t)e + k] = f * a.set_force[3 * (size_t)s + k];
-    }
-}
-
-// Spring energies (World::make_measurements, src/World.cpp:4322-4328), one per spring.
-__global__ __launch_bounds__(256) void k_spring_energy(ForceArgs fa, int n, double *out) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= n) return;
-    const int2 nn = fa.spring_nodes[s];
-    const double4 a = fa.pos[nn.x], b = fa.pos[nn.y];
-    const double cx = a.x - b.x, cy = a.y - b.y, cz = a.z - b.z;
-    double m = 0.0;
-    m += cx * cx;
-    m += cy * cy;
-    m += cz * cz;
-    m = sqrt(m);
-    const double2 kl = fa.spring_kl[s];
-    out[s] = 0.5 * kl.x * (m - kl.y) * (m - kl.y);
-}
-
-struct RhsArgs {
-    const void *eforce;  // T SoA [12][E]
-    int E;
-    const double *f_in;  // solve() RHS given directly ([3N]); nullptr in the time step
-    const int64_t *toff;
-    const uint16_t *cnt;
-    const uint32_t *ctb;
-    const double *zeta;
-    const uint8_t *pinned;
-    const uint32_t *rng_off;
-    const uint32_t *rng_state;
-    const Mat3u *jump;
-    uint32_t node_base;
-    int noise;
-    double kT, dt;
-    const double *ext;  // external forces per linear node [3N], midpoints folded; or nullptr
-    const double *vel;     // mass-ful solvers with Stokes drag: node velocities [3N]; or nullptr
-    const double *lumped;  // MassLumpedSolver: inverse masses [N] (the solve is x = f / m); or nullptr
-    ForceArgs fa;          // springs, constant and surface forces
-};
-
-template <typename T>
-__global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
-    __shared__ double sh[4];
-    __shared__ uint32_t rs[6][64];
-    const int i = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int blk = xcd_block(v.nslices);
-    if (blk >= v.nslices) return;
-    const int n = blk * 64 + lane;
-    const bool live = !g.pinned[n] || g.f_in;
-    // Stokes noise (src/Blob.cpp:3895-3898): draws 3*node + {x,y,z} of the node phase.  Wave 0
-    // jumps to the node's first draw; waves 1, 2 step on from the shared state.  Pinned rows
-    // are zeroed afterwards in the reference, so their draws are skipped (not evaluated).
-    const bool noise = g.noise && !g.f_in;
-    if (noise) {
-        if (i == 0 && live) {
-            Stream6 st = rng_jump(g.rng_state, g.jump, g.node_base + g.rng_off[n]);
-#pragma unroll
-            for (int k = 0; k < 6; k++) rs[k][lane] = st.s[k];
-        }
-        __syncthreads();
-    }
-    double f = 0.0;
-    if (g.f_in) {
-        f = g.f_in[3 * (size_t)n + i];
-    } else if (live) {
-        // springs, then constant forces, onto the zeroed force (World::run: apply_springs, then
-        // update_internal_forces' apply_ctforces)
-        if (g.fa.pre_off) {
-            const int t1 = g.fa.pre_off[n + 1];
-            for (int t = g.fa.pre_off[n]; t < t1; t++) {
-                const int it = g.fa.pre_item[t];
-                f += it >= 0 ? spring_component(g.fa, it >> 1, it & 1, i) : g.fa.cforce[3 * (size_t)(~it) + i];
-            }
-        }
-        const T *ef = reinterpret_cast<const T *>(g.eforce);
-        const int cnt = g.cnt[n];
-        const uint32_t *tl = g.ctb + g.toff[blk] + lane;
-#pragma unroll 4
-        for (int k = 0; k < cnt; k++) {
-            const uint32_t code = tl[64 * k];
-            f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
-        }
-        // surface face forces, face by face, after the element forces (src/Blob.cpp:3842-3851)
-        if (g.fa.grp_off) f = face_groups(g.fa, g.fa.grp_off[n], g.fa.grp_off[n + 1], i, f);
-        // external forces come after the element forces and before the Stokes noise
-        if (g.ext) f += g.ext[3 * (size_t)n + i];
-        // mass-ful solvers: Stokes drag on the node's velocity, before its noise (:3854-3877)
-        if (g.vel) f -= g.vel[3 * (size_t)n + i] * g.zeta[n];
-        if (noise) {
-            Stream6 st;
-#pragma unroll
-            for (int k = 0; k < 6; k++) st.s[k] = rs[k][lane];
-            for (int k = 0; k < i; k++) (void)rng_u01(st);
-            f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
-        }
-        // forces on edge midpoints, passed on element by element (linearise_force,
-        // src/Blob.cpp:1066-1085)
-        if (g.fa.fold_off) {
-            const int t1 = g.fa.fold_off[n + 1];
-            for (int t = g.fa.fold_off[n]; t < t1; t++) {
-                const int *mm = g.fa.fold_mid + 3 * (size_t)t;
-                const double x = mm[0] >= 0 ? g.fa.mf[3 * (size_t)mm[0] + i] : 0.0;
-                const double y = mm[1] >= 0 ? g.fa.mf[3 * (size_t)mm[1] + i] : 0.0;
-                const double z = mm[2] >= 0 ? g.fa.mf[3 * (size_t)mm[2] + i] : 0.0;
-                f += 0.5 * (x + y + z);
-            }
-        }
-    }
-    const size_t k = 3 * (size_t)n + i;
-    if (g.lumped) {  // MassLumpedSolver::solve (src/MassLumpedSolver.cpp:82-90)
-        v.f[k] = f;
-        v.x[k] = f * g.lumped[n];
-    } else {
-        const double z = v.dinv[k] * f;
-        v.f[k] = f;
-        v.r[k] = f;
-        v.x[k] = 0.0;
-        v.p[k] = 0.0;
-        v.q[k] = 0.0;
-        v.z[k] = z;
-        block_partial_at(f * z, v.parts_rz, blk, sh);
-        block_partial_at(f * f, v.parts_ff, blk, sh);
-        if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
-    }
-    const int b = v.slice_blob[blk];
-    if (blk == v.blob_slice_off[b] && threadIdx.x == 0) {
-        CgCtl *c = &v.ctl[b];
-        c->done = 0;
-        c->status = 0;
-        c->iter = 0;
-        c->res = 0.0;
-    }
-}
-
-// ----------------------------------------------------------------------------------------
-// CG iteration `it` (the reference's loop index, src/NoMassCGSolver.cpp:175-194)
-// ----------------------------------------------------------------------------------------
-struct SpmvArgs {
-    const int64_t *slice_off;
-    const int32_t *slice_width;
-    const int32_t *col;
-    const void *val;
-};
-
-// Convergence test of update(it-1) (||r||^2/||f||^2 < eps^2, src/NoMassCGSolver.cpp:182-191,
-// :233-248); beta = delta_it / delta_(it-1); p = z + beta p (vec3_scale_and_add,
-// src/mat_vec_fns.cpp:265-275); q = K p, formed as K z + beta q_old so that only z is gathered
-// (K is applied block by block, each scalar row summed in ascending column order like
-// SparseMatrixFixedPattern::apply, src/SparseMatrixFixedPattern.cpp:148-191); p.q partials.
-// One block = one 64-row slice; wave w computes component w of every row.  Every scalar is
-// per blob: the blob's slices are a contiguous range of partials.
-// mode 0: Jacobi-PCG (Fletcher-Reeves beta); 1: flexible CG with the learned preconditioner
-// (beta = z.(r - r_prev) / delta_prev, Polak-Ribiere form); 2: restart (beta = 0: p = z).
-//
-// kMass: ConjugateGradientSolver on the mass matrix (src/ConjugateGradientSolver.cpp:205-230),
-// whose matrix holds one double per block (the same for the three components) and whose loop
-// tests ||r||^2/||f||^2 < eps^2 before each product -- first on r = f, where the ratio is 1 --
-// and fails after max_iterations_cg updates without testing the last one.
-template <bool kMass>
-__device__ __forceinline__ bool cg_stop_first(double f2, double eps2) {
-    return f2 == 0.0 || (kMass && 1.0 < eps2);
-}
-template <bool kMass>
-__device__ __forceinline__ bool cg_stop(double res, int it, const CgArgs &v, bool &ok) {
-    ok = kMass ? (it < v.maxit && res < v.eps2) : res < v.eps2;
-    return ok || it >= v.maxit;
-}
-
-template <typename T, bool kNt, bool kMass>
-__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
-    __shared__ double sh[4];
-    const int slice = xcd_block(v.nslices);
-    if (slice >= v.nslices) return;
-    const int b = v.slice_blob[slice];
-    CgCtl *ctl = &v.ctl[b];
-    if (ctl->done) return;
-    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
-    const bool decider = slice == s0 && threadIdx.x == 0;
-    double beta = 0.0;
-    if (it == 0) {
-        const double f2 = blob_total(v.parts_ff, v.tot, kTotFF, b, s0, s1, sh);
-        const double d0 = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
-        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
-        if (decider) {
-            ctl->f2 = f2;
-            ctl->delta[0] = d0;
-            if (stop) {  // zero force: zero solution
-                ctl->done = 1;
-                ctl->status = 0;
-                ctl->iter = 0;
-            }
-        }
-        if (stop) return;
-    } else {
-        const double f2 = ctl->f2;
-        const double r2 = blob_total(v.parts_rr, v.tot, kTotRR, b, s0, s1, sh);
-        const double res = r2 / f2;
-        bool ok;
-        if (cg_stop<kMass>(res, it, v, ok)) {
-            if (decider) {
-                ctl->done = 1;
-                ctl->status = ok ? 0 : -1;
-                ctl->iter = it;
-                ctl->res = res;
-            }
-            return;
-        }
-        const double dnew = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
-        if (mode == 0)
-            beta = dnew / ctl->delta[(it - 1) & 1];
-        else if (mode == 1)
-            beta = (dnew - blob_total(v.parts_zrp, v.tot, kTotZRP, b, s0, s1, sh)) / ctl->delta[(it - 1) & 1];
-        if (decider) ctl->delta[it & 1] = dnew;
-    }
-    const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = slice * 64 + lane;
-    const int w = m.slice_width[slice];
-    const int64_t base = m.slice_off[slice];
-    const T *val = reinterpret_cast<const T *>(m.val);
-    const double *__restrict__ z = v.z;
-    double acc = 0.0;
-    if constexpr (kMass) {
-        const double *mv = reinterpret_cast<const double *>(m.val);
-#pragma unroll 4
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            acc += stream_load<kNt>(mv + tile + lane) * z[3 * c + ci];
-        }
-    } else {
-#pragma unroll 4
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
-            const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
-            acc += (double)v0 * z[3 * c + 0];
-            acc += (double)v1 * z[3 * c + 1];
-            acc += (double)v2 * z[3 * c + 2];
-        }
-    }
-    const size_t k = 3 * (size_t)row + ci;
-    const double pn = z[k] + beta * v.p[k];
-    const double qn = acc + beta * v.q[k];
-    v.p[k] = pn;
-    v.q[k] = qn;
-    block_partial_at(pn * qn, v.parts_pq, slice, sh);
-}
-
-// Sum of parts[i0, i1) in exactly the order parts_range gives with a 192-thread block, computed
-// by one 64-thread wave (virtual thread t + 64 w for w = 0, 1, 2).
-__device__ __forceinline__ double parts_range_as192(const double *__restrict__ parts, int i0, int i1) {
-    const int t = threadIdx.x & 63;
-    double tot = 0.0;
-#pragma unroll
-    for (int w = 0; w < 3; w++) {
-        double s = 0.0;
-        for (int i = i0 + t + 64 * w; i < i1; i += 192) s += parts[i];
-        tot += __shfl(wave_sum(s), 0, 64);
-    }
-    return tot;
-}
-
-__device__ __forceinline__ double blob_total_as192(const double *parts, const double *tot, int slot, int b, int s0, int s1) {
-    if (tot) return tot[b * kTotSlots + slot];
-    return parts_range_as192(parts, s0, s1);
-}
-
-// k_cg_spmv with one 64-lane wave per slice: each lane computes the 3 components of its row, so
-// the column index and the gathered z block are loaded once instead of three times.  Row sums,
-// reductions and the partial of p.q are formed in k_cg_spmv's exact order, so both variants
-// give bit-identical results (the tuner may pick either).
-template <typename T, bool kNt, bool kMass>
-__global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, int mode) {
-    const int slice = xcd_block(v.nslices);
-    if (slice >= v.nslices) return;
-    const int b = v.slice_blob[slice];
-    CgCtl *ctl = &v.ctl[b];
-    if (ctl->done) return;
-    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
-    const bool decider = slice == s0 && threadIdx.x == 0;
-    double beta = 0.0;
-    if (it == 0) {
-        const double f2 = blob_total_as192(v.parts_ff, v.tot, kTotFF, b, s0, s1);
-        const double d0 = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
-        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
-        if (decider) {
-            ctl->f2 = f2;
-            ctl->delta[0] = d0;
-            if (stop) {
-                ctl->done = 1;
-                ctl->status = 0;
-                ctl->iter = 0;
-            }
-        }
-        if (stop) return;
-    } else {
-        const double f2 = ctl->f2;
-        const double r2 = blob_total_as192(v.parts_rr, v.tot, kTotRR, b, s0, s1);
-        const double res = r2 / f2;
-        bool ok;
-        if (cg_stop<kMass>(res, it, v, ok)) {
-            if (decider) {
-                ctl->done = 1;
-                ctl->status = ok ? 0 : -1;
-                ctl->iter = it;
-                ctl->res = res;
-            }
-            return;
-        }
-        const double dnew = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
-        if (mode == 0)
-            beta = dnew / ctl->delta[(it - 1) & 1];
-        else if (mode == 1)
-            beta = (dnew - blob_total_as192(v.parts_zrp, v.tot, kTotZRP, b, s0, s1)) / ctl->delta[(it - 1) & 1];
-        if (decider) ctl->delta[it & 1] = dnew;
-    }
-    const int lane = threadIdx.x & 63;
-    const int row = slice * 64 + lane;
-    const int w = m.slice_width[slice];
-    const int64_t base = m.slice_off[slice];
-    const T *val = reinterpret_cast<const T *>(m.val);
-    const double *__restrict__ z = v.z;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    if constexpr (kMass) {
-        const double *mv = reinterpret_cast<const double *>(m.val);
-#pragma unroll 4
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            const double a = stream_load<kNt>(mv + tile + lane);
-            a0 += a * z[3 * c + 0];
-            a1 += a * z[3 * c + 1];
-            a2 += a * z[3 * c + 2];
-        }
-    } else {
-#pragma unroll 2
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9 + lane;
-            T v[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) v[k] = stream_load<kNt>(vt + 64 * k);
-            const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-            a0 += (double)v[0] * z0;
-            a0 += (double)v[1] * z1;
-            a0 += (double)v[2] * z2;
-            a1 += (double)v[3] * z0;
-            a1 += (double)v[4] * z1;
-            a1 += (double)v[5] * z2;
-            a2 += (double)v[6] * z0;
-            a2 += (double)v[7] * z1;
-            a2 += (double)v[8] * z2;
-        }
-    }
-    const size_t k = 3 * (size_t)row;
-    const double acc[3] = {a0, a1, a2};
-    double part = 0.0;
-#pragma unroll
-    for (int ci = 0; ci < 3; ci++) {
-        const double pn = z[k + ci] + beta * v.p[k + ci];
-        const double qn = acc[ci] + beta * v.q[k + ci];
-        v.p[k + ci] = pn;
-        v.q[k + ci] = qn;
-        part += __shfl(wave_sum(pn * qn), 0, 64);  // ((0 + W0) + W1) + W2, as the 3-wave block
-    }
-    if (lane == 0) v.parts_pq[slice] = part;
-}
-
-// alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
-// z = D^-1 r (parallel_apply_preconditioner, src/NoMassCGSolver.cpp:278-292); partials of r.r
-// and r.z.  One block = the 3 x 64 scalars of one slice.
-__global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
-    __shared__ double sh[4];
-    const int slice = xcd_block(v.nslices);
-    if (slice >= v.nslices) return;
-    const int b = v.slice_blob[slice];
-    CgCtl *ctl = &v.ctl[b];
-    if (ctl->done) return;
-    const double alpha =
-        ctl->delta[it & 1] / blob_total(v.parts_pq, v.tot, kTotPQ, b, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
-    const size_t k = (size_t)slice * 192 + threadIdx.x;
-    v.x[k] += alpha * v.p[k];
-    const double r0 = v.r[k];
-    const double r = r0 + (-alpha) * v.q[k];
-    v.r[k] = r;
-    const double z = v.dinv[k] * r;
-    v.z[k] = z;
-    block_partial_at(r * r, v.parts_rr, slice, sh);
-    block_partial_at(r * z, v.parts_rz, slice, sh);
-    if (v.parts_uu) {  // learned preconditioner follows: z holds u = D^-1 r
-        v.r_prev[k] = r0;
-        block_partial_at(z * z, v.parts_uu, slice, sh);
-    }
-}
-
-// euler_integrate (src/Blob.cpp:3941-3967): massless, vel = v and pos += v dt; mass-ful
-// (massful != 0), vel += a dt and pos += vel dt.  Skipped after a CG failure, as the reference
-// skips it.  Thread 0 advances the stream by one step's draws.
-__global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, double dt,
-                                                     uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance,
-                                                     int massful) {
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    if (n == 0 && advance) {
-        uint32_t s[6];
-        for (int k = 0; k < 6; k++) s[k] = rng_state[k];
-        matvec<ffh::kM1, kC1>(j1, &s[0]);
-        matvec<ffh::kM2, kC2>(j2, &s[3]);
-        for (int k = 0; k < 6; k++) rng_state[k] = s[k];
-    }
-    if (n >= v.N || v.ctl[v.slice_blob[n >> 6]].status != 0) return;
-    double x0 = v.x[3 * n], x1 = v.x[3 * n + 1], x2 = v.x[3 * n + 2];
-    if (massful) {
-        x0 = v.vel[3 * n] + x0 * dt;
-        x1 = v.vel[3 * n + 1] + x1 * dt;
-        x2 = v.vel[3 * n + 2] + x2 * dt;
-    }
-    v.vel[3 * n] = x0;
-    v.vel[3 * n + 1] = x1;
-    v.vel[3 * n + 2] = x2;
-    double4 p = pos[n];
-    p.x += x0 * dt;
-    p.y += x1 * dt;
-    p.z += x2 * dt;
-    pos[n] = p;
-}
-
-__global__ void k_reset_inversions(CgCtl *ctl) {
-    if (threadIdx.x == 0) ctl->inversions = 0;
-}
-
-// result = K in for Solver::apply_matrix (linear rows; midpoint rows handled on the host).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_apply(SpmvArgs m, int N, const double *__restrict__ in,
-                                                  double *__restrict__ out) {
-    const int row = blockIdx.x * kBlock + threadIdx.x;
-    if (row >= N) return;
-    const int s = row >> 6, lane = row & 63;
-    const int w = m.slice_width[s];
-    const int64_t base = m.slice_off[s];
-    const T *val = reinterpret_cast<const T *>(m.val);
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < w; t++) {
-        const int64_t tile = base + (int64_t)t * 64;
-        const int c = m.col[tile + lane];
-        const T *vt = val + tile * 9 + lane;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) acc[i] += (double)vt[(3 * i + j) * 64] * in[3 * c + j];
-    }
-    out[3 * row] = acc[0];
-    out[3 * row + 1] = acc[1];
-    out[3 * row + 2] = acc[2];
-}
-
-// Strain energy terms (src/Blob.cpp:1424-1432) recomputed from current positions; one partial
-// per block, summed on the host.
-__global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *parts) {
-    __shared__ double sh[4];
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    double part = 0.0;
-    if (e < g.E) {
-        const int4 n = g.enodes[e];
-        const double4 p0 = g.pos[n.x], pr[3] = {g.pos[n.y], g.pos[n.z], g.pos[n.w]};
-        double J[3][3];
-        for (int r = 0; r < 3; r++) {
-            J[r][0] = pr[r].x - p0.x;
-            J[r][1] = pr[r].y - p0.y;
-            J[r][2] = pr[r].z - p0.z;
-        }
-        double c1 = J[2][2] * J[1][1] - J[2][1] * J[1][2];
-        double c2 = J[2][1] * J[0][2] - J[2][2] * J[0][1];
-        double c3 = J[1][2] * J[0][1] - J[1][1] * J[0][2];
-        double det = J[0][0] * c1 + J[1][0] * c2 + J[2][0] * c3;
-        double vol = (1.0 / 6.0) * fabs(det);
-        double F[3][3];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                double f = 0;
-                for (int k = 0; k < 3; k++) f += J[k][i] * g.jinv0[(size_t)(3 * j + k) * g.E + e];
-                F[i][j] = f;
-            }
-        double dc = F[0][0] * F[0][0] + F[1][1] * F[1][1] + F[2][2] * F[2][2] + F[0][1] * F[0][1] +
-                    F[1][0] * F[1][0] + F[0][2] * F[0][2] + F[2][0] * F[2][0] + F[1][2] * F[1][2] +
-                    F[2][1] * F[2][1];
-        double G = g.G[e], C = g.Em[e] - (2.0 / 3.0) * G, vol0 = g.vol0[e];
-        double t = vol / vol0;
-        part = vol0 * (G * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * G) * log(t));
-    }
-    block_partial(part, parts, sh);
-}
-
-// Kinetic energy of the mass-ful solvers (Blob::make_measurements, src/Blob.cpp:1385-1407):
-// per element u_e . (M_e u_e) with apply_element_mass_matrix (src/tetra_element_linear.cpp:
-// 534-553); one partial per block.  The caller halves the sum.
-__global__ __launch_bounds__(kBlock) void k_kinetic_energy(ElemArgs g, const double *rho, double *parts) {
-    __shared__ double sh[4];
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    double part = 0.0;
-    if (e < g.E) {
-        const int4 nn = g.enodes[e];
-        const int ni[4] = {nn.x, nn.y, nn.z, nn.w};
-        double u[12], mu[12];
-        for (int i = 0; i < 3; i++)
-            for (int a = 0; a < 4; a++) u[4 * i + a] = g.vel[3 * (size_t)ni[a] + i];
-        const double sc = g.vol0[e] * rho[e];
-        for (int k = 0; k < 12; k += 4)
-            for (int i = 0; i < 4; i++) {
-                double t = 0.0;
-                for (int j = 0; j < 4; j++) t += (i == j ? .1 : .05) * u[j + k];
-                mu[i + k] = t * sc;
-            }
-        part = u[0] * mu[0] + u[1] * mu[1] + u[2] * mu[2] + u[3] * mu[3] + u[4] * mu[4] + u[5] * mu[5] +
-               u[6] * mu[6] + u[7] * mu[7] + u[8] * mu[8] + u[9] * mu[9] + u[10] * mu[10] + u[11] * mu[11];
-    }
-    block_partial(part, parts, sh);
-}
+#include "ffh_device.h"
+#include "ffh_element.h"
+#include "ffh_forces.h"
+#include "ffh_solve.h"
 
 #include "ffh_mpnn.h"
 

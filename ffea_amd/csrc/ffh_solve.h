@@ -1,0 +1,526 @@
+// ffh_solve.h -- the force phase and the solvers: force gather + CG start, the CG
+// SpMV and update kernels, integration, matrix apply, strain and kinetic energy.
+// Part of ffh_kernels.hip: included inside its anonymous namespace, after the headers it
+// lists there; not a stand-alone header.
+#pragma once
+
+// ----------------------------------------------------------------------------------------
+// RHS: force gather, Stokes noise, pinned rows, CG initialisation.  64 nodes x 3 components
+// per block (wave = component).
+// ----------------------------------------------------------------------------------------
+struct CgArgs {
+    int N, nslices;
+    double *f, *r, *x, *p, *q, *z, *vel;
+    const double *dinv;
+    double *parts_rz, *parts_rr, *parts_ff, *parts_pq;  // one partial per slice
+    const int32_t *slice_blob, *blob_slice_off;
+    double eps2;
+    int maxit;
+    CgCtl *ctl;  // one per blob
+    // learned preconditioner (nullptr otherwise): partials of |D^-1 r|^2, r before the update,
+    // partials of z.r_prev for the flexible beta
+    double *parts_uu, *r_prev, *parts_zrp;
+    const double *tot;  // per-blob totals (large blobs) or nullptr: consumers add the partials
+};
+
+
+struct RhsArgs {
+    const void *eforce;  // T SoA [12][E]
+    int E;
+    const double *f_in;  // solve() RHS given directly ([3N]); nullptr in the time step
+    const int64_t *toff;
+    const uint16_t *cnt;
+    const uint32_t *ctb;
+    const double *zeta;
+    const uint8_t *pinned;
+    const uint32_t *rng_off;
+    const uint32_t *rng_state;
+    const Mat3u *jump;
+    uint32_t node_base;
+    int noise;
+    double kT, dt;
+    const double *ext;  // external forces per linear node [3N], midpoints folded; or nullptr
+    const double *vel;     // mass-ful solvers with Stokes drag: node velocities [3N]; or nullptr
+    const double *lumped;  // MassLumpedSolver: inverse masses [N] (the solve is x = f / m); or nullptr
+    ForceArgs fa;          // springs, constant and surface forces
+};
+
+template <typename T>
+__global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
+    __shared__ double sh[4];
+    __shared__ uint32_t rs[6][64];
+    const int i = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int blk = xcd_block(v.nslices);
+    if (blk >= v.nslices) return;
+    const int n = blk * 64 + lane;
+    const bool live = !g.pinned[n] || g.f_in;
+    // Stokes noise (src/Blob.cpp:3895-3898): draws 3*node + {x,y,z} of the node phase.  Wave 0
+    // jumps to the node's first draw; waves 1, 2 step on from the shared state.  Pinned rows
+    // are zeroed afterwards in the reference, so their draws are skipped (not evaluated).
+    const bool noise = g.noise && !g.f_in;
+    if (noise) {
+        if (i == 0 && live) {
+            Stream6 st = rng_jump(g.rng_state, g.jump, g.node_base + g.rng_off[n]);
+#pragma unroll
+            for (int k = 0; k < 6; k++) rs[k][lane] = st.s[k];
+        }
+        __syncthreads();
+    }
+    double f = 0.0;
+    if (g.f_in) {
+        f = g.f_in[3 * (size_t)n + i];
+    } else if (live) {
+        // springs, then constant forces, onto the zeroed force (World::run: apply_springs, then
+        // update_internal_forces' apply_ctforces)
+        if (g.fa.pre_off) {
+            const int t1 = g.fa.pre_off[n + 1];
+            for (int t = g.fa.pre_off[n]; t < t1; t++) {
+                const int it = g.fa.pre_item[t];
+                f += it >= 0 ? spring_component(g.fa, it >> 1, it & 1, i) : g.fa.cforce[3 * (size_t)(~it) + i];
+            }
+        }
+        const T *ef = reinterpret_cast<const T *>(g.eforce);
+        const int cnt = g.cnt[n];
+        const uint32_t *tl = g.ctb + g.toff[blk] + lane;
+#pragma unroll 4
+        for (int k = 0; k < cnt; k++) {
+            const uint32_t code = tl[64 * k];
+            f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
+        }
+        // surface face forces, face by face, after the element forces (src/Blob.cpp:3842-3851)
+        if (g.fa.grp_off) f = face_groups(g.fa, g.fa.grp_off[n], g.fa.grp_off[n + 1], i, f);
+        // external forces come after the element forces and before the Stokes noise
+        if (g.ext) f += g.ext[3 * (size_t)n + i];
+        // mass-ful solvers: Stokes drag on the node's velocity, before its noise (:3854-3877)
+        if (g.vel) f -= g.vel[3 * (size_t)n + i] * g.zeta[n];
+        if (noise) {
+            Stream6 st;
+#pragma unroll
+            for (int k = 0; k < 6; k++) st.s[k] = rs[k][lane];
+            for (int k = 0; k < i; k++) (void)rng_u01(st);
+            f -= rand_half(st) * sqrt((24 * g.kT * g.zeta[n]) / (g.dt));
+        }
+        // forces on edge midpoints, passed on element by element (linearise_force,
+        // src/Blob.cpp:1066-1085)
+        if (g.fa.fold_off) {
+            const int t1 = g.fa.fold_off[n + 1];
+            for (int t = g.fa.fold_off[n]; t < t1; t++) {
+                const int *mm = g.fa.fold_mid + 3 * (size_t)t;
+                const double x = mm[0] >= 0 ? g.fa.mf[3 * (size_t)mm[0] + i] : 0.0;
+                const double y = mm[1] >= 0 ? g.fa.mf[3 * (size_t)mm[1] + i] : 0.0;
+                const double z = mm[2] >= 0 ? g.fa.mf[3 * (size_t)mm[2] + i] : 0.0;
+                f += 0.5 * (x + y + z);
+            }
+        }
+    }
+    const size_t k = 3 * (size_t)n + i;
+    if (g.lumped) {  // MassLumpedSolver::solve (src/MassLumpedSolver.cpp:82-90)
+        v.f[k] = f;
+        v.x[k] = f * g.lumped[n];
+    } else {
+        const double z = v.dinv[k] * f;
+        v.f[k] = f;
+        v.r[k] = f;
+        v.x[k] = 0.0;
+        v.p[k] = 0.0;
+        v.q[k] = 0.0;
+        v.z[k] = z;
+        block_partial_at(f * z, v.parts_rz, blk, sh);
+        block_partial_at(f * f, v.parts_ff, blk, sh);
+        if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
+    }
+    const int b = v.slice_blob[blk];
+    if (blk == v.blob_slice_off[b] && threadIdx.x == 0) {
+        CgCtl *c = &v.ctl[b];
+        c->done = 0;
+        c->status = 0;
+        c->iter = 0;
+        c->res = 0.0;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// CG iteration `it` (the reference's loop index, src/NoMassCGSolver.cpp:175-194)
+// ----------------------------------------------------------------------------------------
+struct SpmvArgs {
+    const int64_t *slice_off;
+    const int32_t *slice_width;
+    const int32_t *col;
+    const void *val;
+};
+
+// Convergence test of update(it-1) (||r||^2/||f||^2 < eps^2, src/NoMassCGSolver.cpp:182-191,
+// :233-248); beta = delta_it / delta_(it-1); p = z + beta p (vec3_scale_and_add,
+// src/mat_vec_fns.cpp:265-275); q = K p, formed as K z + beta q_old so that only z is gathered
+// (K is applied block by block, each scalar row summed in ascending column order like
+// SparseMatrixFixedPattern::apply, src/SparseMatrixFixedPattern.cpp:148-191); p.q partials.
+// One block = one 64-row slice; wave w computes component w of every row.  Every scalar is
+// per blob: the blob's slices are a contiguous range of partials.
+// mode 0: Jacobi-PCG (Fletcher-Reeves beta); 1: flexible CG with the learned preconditioner
+// (beta = z.(r - r_prev) / delta_prev, Polak-Ribiere form); 2: restart (beta = 0: p = z).
+//
+// kMass: ConjugateGradientSolver on the mass matrix (src/ConjugateGradientSolver.cpp:205-230),
+// whose matrix holds one double per block (the same for the three components) and whose loop
+// tests ||r||^2/||f||^2 < eps^2 before each product -- first on r = f, where the ratio is 1 --
+// and fails after max_iterations_cg updates without testing the last one.
+template <bool kMass>
+__device__ __forceinline__ bool cg_stop_first(double f2, double eps2) {
+    return f2 == 0.0 || (kMass && 1.0 < eps2);
+}
+template <bool kMass>
+__device__ __forceinline__ bool cg_stop(double res, int it, const CgArgs &v, bool &ok) {
+    ok = kMass ? (it < v.maxit && res < v.eps2) : res < v.eps2;
+    return ok || it >= v.maxit;
+}
+
+template <typename T, bool kNt, bool kMass>
+__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
+    __shared__ double sh[4];
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
+    const bool decider = slice == s0 && threadIdx.x == 0;
+    double beta = 0.0;
+    if (it == 0) {
+        const double f2 = blob_total(v.parts_ff, v.tot, kTotFF, b, s0, s1, sh);
+        const double d0 = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
+        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
+        if (decider) {
+            ctl->f2 = f2;
+            ctl->delta[0] = d0;
+            if (stop) {  // zero force: zero solution
+                ctl->done = 1;
+                ctl->status = 0;
+                ctl->iter = 0;
+            }
+        }
+        if (stop) return;
+    } else {
+        const double f2 = ctl->f2;
+        const double r2 = blob_total(v.parts_rr, v.tot, kTotRR, b, s0, s1, sh);
+        const double res = r2 / f2;
+        bool ok;
+        if (cg_stop<kMass>(res, it, v, ok)) {
+            if (decider) {
+                ctl->done = 1;
+                ctl->status = ok ? 0 : -1;
+                ctl->iter = it;
+                ctl->res = res;
+            }
+            return;
+        }
+        const double dnew = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
+        if (mode == 0)
+            beta = dnew / ctl->delta[(it - 1) & 1];
+        else if (mode == 1)
+            beta = (dnew - blob_total(v.parts_zrp, v.tot, kTotZRP, b, s0, s1, sh)) / ctl->delta[(it - 1) & 1];
+        if (decider) ctl->delta[it & 1] = dnew;
+    }
+    const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = slice * 64 + lane;
+    const int w = m.slice_width[slice];
+    const int64_t base = m.slice_off[slice];
+    const T *val = reinterpret_cast<const T *>(m.val);
+    const double *__restrict__ z = v.z;
+    double acc = 0.0;
+    if constexpr (kMass) {
+        const double *mv = reinterpret_cast<const double *>(m.val);
+#pragma unroll 4
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            acc += stream_load<kNt>(mv + tile + lane) * z[3 * c + ci];
+        }
+    } else {
+#pragma unroll 4
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
+            const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
+            acc += (double)v0 * z[3 * c + 0];
+            acc += (double)v1 * z[3 * c + 1];
+            acc += (double)v2 * z[3 * c + 2];
+        }
+    }
+    const size_t k = 3 * (size_t)row + ci;
+    const double pn = z[k] + beta * v.p[k];
+    const double qn = acc + beta * v.q[k];
+    v.p[k] = pn;
+    v.q[k] = qn;
+    block_partial_at(pn * qn, v.parts_pq, slice, sh);
+}
+
+// Sum of parts[i0, i1) in exactly the order parts_range gives with a 192-thread block, computed
+// by one 64-thread wave (virtual thread t + 64 w for w = 0, 1, 2).
+__device__ __forceinline__ double parts_range_as192(const double *__restrict__ parts, int i0, int i1) {
+    const int t = threadIdx.x & 63;
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        double s = 0.0;
+        for (int i = i0 + t + 64 * w; i < i1; i += 192) s += parts[i];
+        tot += __shfl(wave_sum(s), 0, 64);
+    }
+    return tot;
+}
+
+__device__ __forceinline__ double blob_total_as192(const double *parts, const double *tot, int slot, int b, int s0, int s1) {
+    if (tot) return tot[b * kTotSlots + slot];
+    return parts_range_as192(parts, s0, s1);
+}
+
+// k_cg_spmv with one 64-lane wave per slice: each lane computes the 3 components of its row, so
+// the column index and the gathered z block are loaded once instead of three times.  Row sums,
+// reductions and the partial of p.q are formed in k_cg_spmv's exact order, so both variants
+// give bit-identical results (the tuner may pick either).
+template <typename T, bool kNt, bool kMass>
+__global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, int mode) {
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
+    const bool decider = slice == s0 && threadIdx.x == 0;
+    double beta = 0.0;
+    if (it == 0) {
+        const double f2 = blob_total_as192(v.parts_ff, v.tot, kTotFF, b, s0, s1);
+        const double d0 = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
+        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
+        if (decider) {
+            ctl->f2 = f2;
+            ctl->delta[0] = d0;
+            if (stop) {
+                ctl->done = 1;
+                ctl->status = 0;
+                ctl->iter = 0;
+            }
+        }
+        if (stop) return;
+    } else {
+        const double f2 = ctl->f2;
+        const double r2 = blob_total_as192(v.parts_rr, v.tot, kTotRR, b, s0, s1);
+        const double res = r2 / f2;
+        bool ok;
+        if (cg_stop<kMass>(res, it, v, ok)) {
+            if (decider) {
+                ctl->done = 1;
+                ctl->status = ok ? 0 : -1;
+                ctl->iter = it;
+                ctl->res = res;
+            }
+            return;
+        }
+        const double dnew = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
+        if (mode == 0)
+            beta = dnew / ctl->delta[(it - 1) & 1];
+        else if (mode == 1)
+            beta = (dnew - blob_total_as192(v.parts_zrp, v.tot, kTotZRP, b, s0, s1)) / ctl->delta[(it - 1) & 1];
+        if (decider) ctl->delta[it & 1] = dnew;
+    }
+    const int lane = threadIdx.x & 63;
+    const int row = slice * 64 + lane;
+    const int w = m.slice_width[slice];
+    const int64_t base = m.slice_off[slice];
+    const T *val = reinterpret_cast<const T *>(m.val);
+    const double *__restrict__ z = v.z;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if constexpr (kMass) {
+        const double *mv = reinterpret_cast<const double *>(m.val);
+#pragma unroll 4
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const double a = stream_load<kNt>(mv + tile + lane);
+            a0 += a * z[3 * c + 0];
+            a1 += a * z[3 * c + 1];
+            a2 += a * z[3 * c + 2];
+        }
+    } else {
+#pragma unroll 2
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const T *vt = val + tile * 9 + lane;
+            T v[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) v[k] = stream_load<kNt>(vt + 64 * k);
+            const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
+            a0 += (double)v[0] * z0;
+            a0 += (double)v[1] * z1;
+            a0 += (double)v[2] * z2;
+            a1 += (double)v[3] * z0;
+            a1 += (double)v[4] * z1;
+            a1 += (double)v[5] * z2;
+            a2 += (double)v[6] * z0;
+            a2 += (double)v[7] * z1;
+            a2 += (double)v[8] * z2;
+        }
+    }
+    const size_t k = 3 * (size_t)row;
+    const double acc[3] = {a0, a1, a2};
+    double part = 0.0;
+#pragma unroll
+    for (int ci = 0; ci < 3; ci++) {
+        const double pn = z[k + ci] + beta * v.p[k + ci];
+        const double qn = acc[ci] + beta * v.q[k + ci];
+        v.p[k + ci] = pn;
+        v.q[k + ci] = qn;
+        part += __shfl(wave_sum(pn * qn), 0, 64);  // ((0 + W0) + W1) + W2, as the 3-wave block
+    }
+    if (lane == 0) v.parts_pq[slice] = part;
+}
+
+// alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
+// z = D^-1 r (parallel_apply_preconditioner, src/NoMassCGSolver.cpp:278-292); partials of r.r
+// and r.z.  One block = the 3 x 64 scalars of one slice.
+__global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
+    __shared__ double sh[4];
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const double alpha =
+        ctl->delta[it & 1] / blob_total(v.parts_pq, v.tot, kTotPQ, b, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
+    const size_t k = (size_t)slice * 192 + threadIdx.x;
+    v.x[k] += alpha * v.p[k];
+    const double r0 = v.r[k];
+    const double r = r0 + (-alpha) * v.q[k];
+    v.r[k] = r;
+    const double z = v.dinv[k] * r;
+    v.z[k] = z;
+    block_partial_at(r * r, v.parts_rr, slice, sh);
+    block_partial_at(r * z, v.parts_rz, slice, sh);
+    if (v.parts_uu) {  // learned preconditioner follows: z holds u = D^-1 r
+        v.r_prev[k] = r0;
+        block_partial_at(z * z, v.parts_uu, slice, sh);
+    }
+}
+
+// euler_integrate (src/Blob.cpp:3941-3967): massless, vel = v and pos += v dt; mass-ful
+// (massful != 0), vel += a dt and pos += vel dt.  Skipped after a CG failure, as the reference
+// skips it.  Thread 0 advances the stream by one step's draws.
+__global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, double dt,
+                                                     uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance,
+                                                     int massful) {
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n == 0 && advance) {
+        uint32_t s[6];
+        for (int k = 0; k < 6; k++) s[k] = rng_state[k];
+        matvec<ffh::kM1, kC1>(j1, &s[0]);
+        matvec<ffh::kM2, kC2>(j2, &s[3]);
+        for (int k = 0; k < 6; k++) rng_state[k] = s[k];
+    }
+    if (n >= v.N || v.ctl[v.slice_blob[n >> 6]].status != 0) return;
+    double x0 = v.x[3 * n], x1 = v.x[3 * n + 1], x2 = v.x[3 * n + 2];
+    if (massful) {
+        x0 = v.vel[3 * n] + x0 * dt;
+        x1 = v.vel[3 * n + 1] + x1 * dt;
+        x2 = v.vel[3 * n + 2] + x2 * dt;
+    }
+    v.vel[3 * n] = x0;
+    v.vel[3 * n + 1] = x1;
+    v.vel[3 * n + 2] = x2;
+    double4 p = pos[n];
+    p.x += x0 * dt;
+    p.y += x1 * dt;
+    p.z += x2 * dt;
+    pos[n] = p;
+}
+
+__global__ void k_reset_inversions(CgCtl *ctl) {
+    if (threadIdx.x == 0) ctl->inversions = 0;
+}
+
+// result = K in for Solver::apply_matrix (linear rows; midpoint rows handled on the host).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply(SpmvArgs m, int N, const double *__restrict__ in,
+                                                  double *__restrict__ out) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= N) return;
+    const int s = row >> 6, lane = row & 63;
+    const int w = m.slice_width[s];
+    const int64_t base = m.slice_off[s];
+    const T *val = reinterpret_cast<const T *>(m.val);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < w; t++) {
+        const int64_t tile = base + (int64_t)t * 64;
+        const int c = m.col[tile + lane];
+        const T *vt = val + tile * 9 + lane;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc[i] += (double)vt[(3 * i + j) * 64] * in[3 * c + j];
+    }
+    out[3 * row] = acc[0];
+    out[3 * row + 1] = acc[1];
+    out[3 * row + 2] = acc[2];
+}
+
+// Strain energy terms (src/Blob.cpp:1424-1432) recomputed from current positions; one partial
+// per block, summed on the host.
+__global__ __launch_bounds__(kBlock) void k_strain_energy(ElemArgs g, double *parts) {
+    __shared__ double sh[4];
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double part = 0.0;
+    if (e < g.E) {
+        const int4 n = g.enodes[e];
+        const double4 p0 = g.pos[n.x], pr[3] = {g.pos[n.y], g.pos[n.z], g.pos[n.w]};
+        double J[3][3];
+        for (int r = 0; r < 3; r++) {
+            J[r][0] = pr[r].x - p0.x;
+            J[r][1] = pr[r].y - p0.y;
+            J[r][2] = pr[r].z - p0.z;
+        }
+        double c1 = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+        double c2 = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+        double c3 = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+        double det = J[0][0] * c1 + J[1][0] * c2 + J[2][0] * c3;
+        double vol = (1.0 / 6.0) * fabs(det);
+        double F[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double f = 0;
+                for (int k = 0; k < 3; k++) f += J[k][i] * g.jinv0[(size_t)(3 * j + k) * g.E + e];
+                F[i][j] = f;
+            }
+        double dc = F[0][0] * F[0][0] + F[1][1] * F[1][1] + F[2][2] * F[2][2] + F[0][1] * F[0][1] +
+                    F[1][0] * F[1][0] + F[0][2] * F[0][2] + F[2][0] * F[2][0] + F[1][2] * F[1][2] +
+                    F[2][1] * F[2][1];
+        double G = g.G[e], C = g.Em[e] - (2.0 / 3.0) * G, vol0 = g.vol0[e];
+        double t = vol / vol0;
+        part = vol0 * (G * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * G) * log(t));
+    }
+    block_partial(part, parts, sh);
+}
+
+// Kinetic energy of the mass-ful solvers (Blob::make_measurements, src/Blob.cpp:1385-1407):
+// per element u_e . (M_e u_e) with apply_element_mass_matrix (src/tetra_element_linear.cpp:
+// 534-553); one partial per block.  The caller halves the sum.
+__global__ __launch_bounds__(kBlock) void k_kinetic_energy(ElemArgs g, const double *rho, double *parts) {
+    __shared__ double sh[4];
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    double part = 0.0;
+    if (e < g.E) {
+        const int4 nn = g.enodes[e];
+        const int ni[4] = {nn.x, nn.y, nn.z, nn.w};
+        double u[12], mu[12];
+        for (int i = 0; i < 3; i++)
+            for (int a = 0; a < 4; a++) u[4 * i + a] = g.vel[3 * (size_t)ni[a] + i];
+        const double sc = g.vol0[e] * rho[e];
+        for (int k = 0; k < 12; k += 4)
+            for (int i = 0; i < 4; i++) {
+                double t = 0.0;
+                for (int j = 0; j < 4; j++) t += (i == j ? .1 : .05) * u[j + k];
+                mu[i + k] = t * sc;
+            }
+        part = u[0] * mu[0] + u[1] * mu[1] + u[2] * mu[2] + u[3] * mu[3] + u[4] * mu[4] + u[5] * mu[5] +
+               u[6] * mu[6] + u[7] * mu[7] + u[8] * mu[8] + u[9] * mu[9] + u[10] * mu[10] + u[11] * mu[11];
+    }
+    block_partial(part, parts, sh);
+}
