@@ -145,6 +145,22 @@ class Blob:
         _lib.check(self._L.ffh_blob_set_constant_forces(self._h, nd.size, nd if nd.size else np.zeros(1, np.int32),
                                                         fr if fr.size else np.zeros(3)))
 
+    def set_rotational_forces(self, nodes, F, axis, torque) -> None:
+        """Rotational constant forces on nodes [n] (N_t ids): F [n] (meso force), axis [n, 6]
+        (point in meso length, unit direction), torque [n] (1: 't', 0: 'f'); empty clears."""
+        nd = np.ascontiguousarray(nodes, np.int32).ravel()
+        fr = np.ascontiguousarray(F, np.float64).ravel()
+        ax = np.ascontiguousarray(axis, np.float64).ravel()
+        tq = np.ascontiguousarray(torque, np.int32).ravel()
+        if fr.size != nd.size or ax.size != 6 * nd.size or tq.size != nd.size:
+            raise ValueError("one F, axis and type per node")
+        if nd.size == 0:
+            nd, fr, ax, tq = np.zeros(1, np.int32), np.zeros(1), np.zeros(6), np.zeros(1, np.int32)
+            n = 0
+        else:
+            n = nd.size
+        _lib.check(self._L.ffh_blob_set_rotational_forces(self._h, n, nd, fr, ax, tq))
+
     def set_surface_forces(self, faces, sets, set_forces) -> None:
         """Linear surface forces: faces [F, 3] (vertex ids), sets: lists of face indices,
         set_forces [S, 3] total force of each set (meso units); empty sets clear."""

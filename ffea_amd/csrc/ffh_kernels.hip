@@ -878,6 +878,8 @@ struct ffh_blob {
             FFH_PUT(fdev.spring_nodes, F.spring_nodes);
             FFH_PUT(fdev.spring_kl, F.spring_kl);
             FFH_PUT(fdev.cforce, F.cforce);
+            fdev.nconst = F.nconst;
+            if (!F.rot.empty()) FFH_PUT(fdev.rot, F.rot);
         }
         if (nsprings > 0) {
             std::vector<double> z(nsprings, 0.0);
@@ -900,6 +902,7 @@ struct ffh_blob {
         if (nmid > 0) {
             if (!F.mid_pre_off.empty()) FFH_PUT(fdev.mid_pre_off, F.mid_pre_off);
             if (!F.mid_grp_off.empty()) FFH_PUT(fdev.mid_grp_off, F.mid_grp_off);
+            FFH_PUT(fdev.mid_ends, F.mid_ends);
             std::vector<double> z(3 * (size_t)nmid, 0.0);
             FFH_PUT(fdev.mf, z);
             FFH_PUT(fdev.fold_off, F.fold_off);
@@ -1544,6 +1547,27 @@ int ffh_blob_set_constant_forces(ffh_blob *b, int num_forces, const int *node, c
     ffh::ForceInput keep = b->fin;
     b->fin.const_node.assign(node, node + num_forces);
     b->fin.const_force.assign(force, force + 3 * (size_t)num_forces);
+    int rc = b->upload_forces();
+    if (rc) {
+        b->fin = keep;
+        (void)b->upload_forces();
+    }
+    return rc;
+}
+
+int ffh_blob_set_rotational_forces(ffh_blob *b, int num_forces, const int *node, const double *F, const double *axis,
+                                   const int *torque) {
+    if (!b || num_forces < 0 || (num_forces > 0 && (!node || !F || !axis || !torque))) return fail(FFH_E_ARG, "null argument");
+    for (int i = 0; i < num_forces; i++) {
+        bool ok = std::isfinite(F[i]);
+        for (int k = 0; k < 6; k++) ok = ok && std::isfinite(axis[6 * i + k]);
+        if (!ok) return fail(FFH_E_ARG, "non-finite rotational force or axis");
+    }
+    ffh::ForceInput keep = b->fin;
+    b->fin.rot_node.assign(node, node + num_forces);
+    b->fin.rot_F.assign(F, F + num_forces);
+    b->fin.rot_axis.assign(axis, axis + 6 * (size_t)num_forces);
+    b->fin.rot_torque.assign(torque, torque + num_forces);
     int rc = b->upload_forces();
     if (rc) {
         b->fin = keep;

@@ -499,9 +499,11 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
     const int N = P.num_linear, NT = P.num_nodes_total, E = P.num_elements;
     F = ForcePlan();
     const size_t ns = in.spring_n0.size(), nc = in.const_node.size(), nf = in.faces.size() / 3;
+    const size_t nr = in.rot_node.size();
     const size_t nset = in.set_off.empty() ? 0 : in.set_off.size() - 1;
     if (in.spring_n1.size() != ns || in.spring_k.size() != ns || in.spring_l.size() != ns ||
-        in.const_force.size() != 3 * nc || in.faces.size() != 3 * nf || in.set_force.size() != 3 * nset)
+        in.const_force.size() != 3 * nc || in.faces.size() != 3 * nf || in.set_force.size() != 3 * nset ||
+        in.rot_F.size() != nr || in.rot_axis.size() != 6 * nr || in.rot_torque.size() != nr)
         return "inconsistent force list sizes";
     // edge ends of every midpoint (N_t ids -> compact ids)
     std::vector<int32_t> end_a(NT, -1), end_b(NT, -1);
@@ -539,11 +541,15 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
         F.spring_kl[2 * s] = in.spring_k[s];
         F.spring_kl[2 * s + 1] = in.spring_l[s];
     }
-    for (size_t i = 0; i < nc; i++) {
-        const int32_t v = in.const_node[i];
-        if (v < 0 || v >= NT) return "constant force " + std::to_string(i) + ": node out of range";
+    // linear constant forces, then rotational ones: a node's items in that order
+    for (size_t i = 0; i < nc + nr; i++) {
+        const bool rot = i >= nc;
+        const int32_t v = rot ? in.rot_node[i - nc] : in.const_node[i];
+        const std::string what = rot ? "rotational force " + std::to_string(i - nc) : "constant force " + std::to_string(i);
+        if (v < 0 || v >= NT) return what + ": node out of range";
+        if (rot && in.rot_torque[i - nc] != 0 && in.rot_torque[i - nc] != 1) return what + ": type must be 0 or 1";
         const int32_t item = ~(int32_t)i;
-        if (P.lin_of[v] < 0 && end_a[v] < 0) return "constant force " + std::to_string(i) + ": node is in no element";
+        if (P.lin_of[v] < 0 && end_a[v] < 0) return what + ": node is in no element";
         if (P.lin_of[v] >= 0) {
             pre[P.lin_of[v]].push_back(item);
         } else {
@@ -553,6 +559,13 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
         }
     }
     F.cforce = in.const_force;
+    F.nconst = (int32_t)nc;
+    F.rot.resize(8 * nr);
+    for (size_t r = 0; r < nr; r++) {
+        for (int k = 0; k < 6; k++) F.rot[8 * r + k] = in.rot_axis[6 * r + k];
+        F.rot[8 * r + 6] = in.rot_F[r];
+        F.rot[8 * r + 7] = in.rot_torque[r];
+    }
     // surface sets: entries, and per face the entries that load it (set order)
     std::vector<std::vector<int32_t>> ent_of_face(nf);
     if (nset) {

@@ -123,6 +123,12 @@ struct ForceInput {
     // Blob::apply_ctforces CTF 1 (src/Blob.cpp:2096-2103), in the stored order
     std::vector<int32_t> const_node;
     std::vector<double> const_force;  // [3C]
+    // CTF 2 (src/Blob.cpp:2104-2136), point + direction axes: per force its node, F, the
+    // axis (point, unit direction) and 1 for a torque ('t'), 0 for a circular force ('f')
+    std::vector<int32_t> rot_node;
+    std::vector<double> rot_F;
+    std::vector<double> rot_axis;     // [6R]
+    std::vector<int32_t> rot_torque;
     // Blob::apply_ctforces CTF 3 (src/Blob.cpp:2137-2159): surface faces (Face::n[0..2]) and
     // sets of face indices with their total force
     std::vector<int32_t> faces;       // [3F]
@@ -133,8 +139,9 @@ struct ForceInput {
 
 // The same forces as per-node lists in the order the reference adds them to a node's force:
 //  1. before the element forces: the node's springs (ascending spring index), then its
-//     constant forces (in stored order) -- World::run applies the springs, then
-//     update_internal_forces the constant forces, onto the zeroed force vector;
+//     linear constant forces, then its rotational ones (each in stored order) -- World::run
+//     applies the springs, then update_internal_forces the constant forces, onto the zeroed
+//     force vector;
 //  2. after the element forces: one group per surface face holding the node (ascending face
 //     index), each the sum of the face's set tractions in set order (Face::force, gathered in
 //     aggregate_forces_and_solve, src/Blob.cpp:3842-3851);
@@ -147,10 +154,13 @@ struct ForceInput {
 // corner's position is the average of its edge ends, as linearise_element leaves it.
 struct ForcePlan {
     std::vector<int32_t> pre_off;    // [N+1]
-    std::vector<int32_t> pre_item;   // >= 0: 2 * spring + end; < 0: ~(constant index)
+    std::vector<int32_t> pre_item;   // >= 0: 2 * spring + end; < 0: ~c, c < nconst a linear
+                                     // constant force, else rotational force c - nconst
     std::vector<int32_t> spring_nodes;  // [2 * springs] compact ids
     std::vector<double> spring_kl;      // [2 * springs]
     std::vector<double> cforce;         // [3 * constants]
+    int32_t nconst = 0;
+    std::vector<double> rot;            // [8 * rotational]: axis point, direction, F, torque
     std::vector<int32_t> grp_off;       // [N+1] face groups of the vertices
     std::vector<int32_t> grp_ent_off;   // [groups + 1]
     std::vector<int32_t> grp_ent;       // set-entry ids
@@ -159,7 +169,7 @@ struct ForcePlan {
     std::vector<double> set_force;      // [3S]
     // midpoints that carry a force: edge ends, constant-force items, face groups
     std::vector<int32_t> mid_ends;      // [2M] compact ids
-    std::vector<int32_t> mid_pre_off;   // [M+1] into pre_item (constants only)
+    std::vector<int32_t> mid_pre_off;   // [M+1] into pre_item (constant forces only)
     std::vector<int32_t> mid_grp_off;   // [M+1] into the groups
     std::vector<int32_t> fold_off;      // [N+1] fold terms of each vertex
     std::vector<int32_t> fold_mid;      // [3 * terms] midpoint indices, -1 for none

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
             const int t1 = g.fa.pre_off[n + 1];
             for (int t = g.fa.pre_off[n]; t < t1; t++) {
                 const int it = g.fa.pre_item[t];
-                f += it >= 0 ? spring_component(g.fa, it >> 1, it & 1, i) : g.fa.cforce[3 * (size_t)(~it) + i];
+                f += it >= 0 ? spring_component(g.fa, it >> 1, it & 1, i) : const_component(g.fa, ~it, g.fa.pos + n, i);
             }
         }
         const T *ef = reinterpret_cast<const T *>(g.eforce);

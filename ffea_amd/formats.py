@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import os
 import re
+from typing import NamedTuple
 
 import numpy as np
 
@@ -175,12 +176,20 @@ def _normalised(d):
     return [x / norm for x in d]
 
 
-def read_ctforces(path, blob_index, num_nodes, conformation_index=0):
+class CtForces(NamedTuple):
+    """One blob's constant forces from a .ctforces file (meso units)."""
+    linear: tuple        # (nodes [n], forces [n, 3])
+    rotational: tuple    # (nodes [n], F [n], axis [n, 6]: point, unit direction, torque [n] 0/1)
+    surface: tuple       # (list of face-index lists, forces [S, 3])
+
+
+def read_ctforces(path, blob_index, num_nodes, conformation_index=0) -> CtForces:
     """Blob::load_ctforces (src/Blob.cpp:3084-3430) for one blob: the linear forces (CTF 1,
-    "all" expanded to every node) as (nodes [n], forces [n, 3] meso) and the linear surface
-    forces (CTF 3) as (list of face-index lists, forces [S, 3] meso).  Rotational forces
-    (CTF 2) for this blob raise FormatError: they are not on this path."""
-    from .units import MESO_FORCE
+    "all" expanded to every node), the rotational forces (CTF 2: F / force unit, the axis point
+    scaled by the inverse length unit, the direction normalised, type 'pf' or 'pt'; "all"
+    expanded) and the linear surface forces (CTF 3).  A rotational axis given by two nodes
+    ('n' types) raises FormatError: the reference leaves its axis undefined (src/Blob.cpp:2114-2124)."""
+    from .units import MESO_FORCE, MESO_LENGTH
     L = _script_lines(path)
     if len(L) < 4:
         raise FormatError(f"{path}: too short")
@@ -217,10 +226,29 @@ def read_ctforces(path, blob_index, num_nodes, conformation_index=0):
         r += 1
     elif nr > 0:
         raise FormatError(f"{path}: expected 'rotational forces:'")
+    mdfm1, mdlm1 = 1.0 / MESO_FORCE, 1.0 / MESO_LENGTH
+    rnodes, rF, raxis, rtorque = [], [], [], []
     for line in L[r:r + nr]:
         w = line.split()
-        if len(w) == 11 and int(w[8]) == blob_index and int(w[9]) == conformation_index:
-            raise FormatError(f"{path}: rotational constant forces are not on this path")
+        if len(w) != 11:
+            raise FormatError(f"{path}: invalid line '{line}'")
+        if int(w[8]) != blob_index or int(w[9]) != conformation_index:
+            continue
+        typ = w[1]
+        if typ[:1] == "n":
+            raise FormatError(f"{path}: rotational forces about an axis between two nodes are undefined in the reference")
+        if typ[:1] != "p":
+            raise FormatError(f"{path}: invalid rotational force: {typ}")
+        if typ[1:2] not in ("f", "t"):
+            raise FormatError(f"{path}: invalid rotational force type: {typ[1:]}")
+        F = float(w[0]) * mdfm1
+        pt = [mdlm1 * float(x) for x in w[2:5]]
+        d = _normalised([float(x) for x in w[5:8]])
+        for nd in (range(num_nodes) if w[10] == "all" else [int(w[10])]):
+            rnodes.append(nd)
+            rF.append(F)
+            raxis.append(pt + d)
+            rtorque.append(1 if typ[1] == "t" else 0)
     r += nr
     sets, set_forces = [], []
     if r < len(L) and L[r].startswith("linear surface forces:"):
@@ -239,8 +267,10 @@ def read_ctforces(path, blob_index, num_nodes, conformation_index=0):
         F = float(w[0])
         set_forces.append([F * d[0] / MESO_FORCE, F * d[1] / MESO_FORCE, F * d[2] / MESO_FORCE])
         sets.append([int(x) for x in w[6:]])
-    return ((np.array(nodes, np.int32), np.array(forces, np.float64).reshape(-1, 3)),
-            (sets, np.array(set_forces, np.float64).reshape(-1, 3)))
+    return CtForces((np.array(nodes, np.int32), np.array(forces, np.float64).reshape(-1, 3)),
+                    (np.array(rnodes, np.int32), np.array(rF, np.float64), np.array(raxis, np.float64).reshape(-1, 6),
+                     np.array(rtorque, np.int32)),
+                    (sets, np.array(set_forces, np.float64).reshape(-1, 3)))
 
 
 def load_blob(node, top, mat, stokes, pin=None) -> Mesh:

@@ -349,11 +349,16 @@ class World:
             if not s.get("ctforces"):
                 raise F.FormatError("ctforces_fname not provided in <ctforces> block, while calc_ctforces set to 1")
             nodes, forces, faces, sets, set_forces = [], [], [], [], []
+            rot = [[], [], [], []]
             face_off = 0
             for b, (blob, m) in enumerate(zip(s["blobs"], self.meshes)):
-                (ln, lf), (bs, bf) = F.read_ctforces(s["ctforces"], b, m.num_nodes)
+                ct = F.read_ctforces(s["ctforces"], b, m.num_nodes)
+                (ln, lf), (bs, bf) = ct.linear, ct.surface
                 nodes.append(ln + off[b])
                 forces.append(lf)
+                rot[0].append(ct.rotational[0] + off[b])
+                for k in (1, 2, 3):
+                    rot[k].append(ct.rotational[k])
                 if bs:
                     if not blob.get("surface"):
                         raise F.FormatError(f"blob {b}: surface forces need its surface file")
@@ -363,6 +368,9 @@ class World:
                     set_forces.append(bf)
                     face_off += fc.shape[0]
             self.batch.set_constant_forces(np.concatenate(nodes), np.concatenate(forces))
+            rot = [np.concatenate(x) for x in rot]
+            if rot[0].size:
+                self.batch.set_rotational_forces(*rot)
             if sets:
                 self.batch.set_surface_forces(np.concatenate(faces), sets, np.concatenate(set_forces))
 

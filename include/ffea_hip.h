@@ -189,17 +189,25 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result);
  *   force[3i..3i+2] on node[i] in the order Blob::load_ctforces stores them (:3184-3241);
  *   added after the springs, before the element forces.  On an edge midpoint the force reaches
  *   the edge ends as linearise_force passes it on (:1066-1085).
+ * ffh_blob_set_rotational_forces: Blob::apply_ctforces' rotational forces (CTF 2, :2104-2136)
+ *   with a point + direction axis ('p' types, as Blob::load_ctforces stores them, :3246-3346):
+ *   axis[6i..6i+5] the axis point (meso length) and unit direction, F[i] in meso force,
+ *   torque[i] 1 for 't', 0 for 'f'.  Every step, with r from the node to its foot on the axis,
+ *   the node gets F (r x d) ('f') or F/|r| (r x d) ('t'), after the linear constant forces;
+ *   on an edge midpoint evaluated at the midpoint and passed on like a linear force.
  * ffh_blob_set_surface_forces: Blob::apply_ctforces' linear surface forces (:2137-2159) --
  *   face_nodes [3*num_faces] are the surface faces' vertices (Face::n[0..2]); set s is the
  *   faces set_faces[set_off[s] .. set_off[s+1]) with total force set_force[3s..3s+2], shared
  *   over its faces in proportion to their current areas, a third to each vertex; face forces
  *   are added face by face after the element forces (:3842-3851).
- * Rotational constant forces (CTF 2) and the interaction solvers are not on this path.
+ * The interaction solvers (VdW / steric / LJ, PreComp) are not on this path.
  */
 int ffh_blob_set_springs(ffh_blob *b, int num_springs, const int *node0, const int *node1, const double *k,
                          const double *l);
 int ffh_blob_spring_energy(ffh_blob *b, double *energy);
 int ffh_blob_set_constant_forces(ffh_blob *b, int num_forces, const int *node, const double *force);
+int ffh_blob_set_rotational_forces(ffh_blob *b, int num_forces, const int *node, const double *F, const double *axis,
+                                   const int *torque);
 int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_nodes, int num_sets, const int *set_off,
                                 const int *set_faces, const double *set_force);
 

@@ -48,6 +48,7 @@
  * (oracle/_ref) for elements, assembled matrix, CG solutions and trajectories;
  * closed forms (unit tetrahedron, rest state stress-free, rigid motions).
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -481,6 +482,11 @@ typedef struct {
     int n_ctf;
     int *ctf_node;
     double *ctf_force;     /* [3 n_ctf] */
+    int n_rctf;            /* rotational (CTF 2), point + direction axes only */
+    int *rctf_node;
+    double *rctf_F;        /* [n_rctf] meso force */
+    double *rctf_axis;     /* [6 n_rctf] axis point (meso length), unit direction */
+    int *rctf_torque;      /* [n_rctf] 0: 'f' (circular force), 1: 't' (torque) */
     int n_faces, n_sets;
     int *face_nodes;       /* [3 n_faces] */
     int *set_off, *set_faces;
@@ -709,7 +715,7 @@ void orc_blob_destroy(orc_blob *b) {
                     b->last_det, b->dpsi, b->F_ij, b->visc, b->node_force, b->stress_mag, b->contrib_off,
                     b->contrib, b->key, b->col, b->src_off, (void *)b->src, b->val, b->diag_inv, b->diag_idx,
                     b->r, b->p, b->z, b->q, b->f, b->mkey, b->mcol, b->mval, b->mprec, b->inv_M,
-                    b->ctf_node, b->ctf_force, b->face_nodes, b->set_off, b->set_faces, b->set_force,
+                    b->ctf_node, b->ctf_force, b->rctf_node, b->rctf_F, b->rctf_axis, b->rctf_torque, b->face_nodes, b->set_off, b->set_faces, b->set_force,
                     b->face_force};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(b);
@@ -741,6 +747,76 @@ void orc_blob_set_constant_forces(orc_blob *b, int n, const int *node, const dou
         memcpy(b->ctf_node, node, sizeof(int) * n);
         memcpy(b->ctf_force, force, sizeof(double) * 3 * n);
     }
+}
+
+/* Rotational constant forces (CTF 2) as Blob::load_ctforces stores them
+ * (src/Blob.cpp:3246-3346), point + direction type only: axis [6n] = the axis point in meso
+ * length units and the normalised direction; torque[i] 1 for 't', 0 for 'f'.  n = 0 clears. */
+void orc_blob_set_rotational_forces(orc_blob *b, int n, const int *node, const double *F, const double *axis,
+                                    const int *torque) {
+    free(b->rctf_node);
+    free(b->rctf_F);
+    free(b->rctf_axis);
+    free(b->rctf_torque);
+    b->n_rctf = n;
+    b->rctf_node = xcalloc(n > 0 ? n : 1, sizeof(int));
+    b->rctf_F = xcalloc(n > 0 ? n : 1, sizeof(double));
+    b->rctf_axis = xcalloc(6 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    b->rctf_torque = xcalloc(n > 0 ? n : 1, sizeof(int));
+    if (n > 0) {
+        memcpy(b->rctf_node, node, sizeof(int) * n);
+        memcpy(b->rctf_F, F, sizeof(double) * n);
+        memcpy(b->rctf_axis, axis, sizeof(double) * 6 * n);
+        memcpy(b->rctf_torque, torque, sizeof(int) * n);
+    }
+}
+
+/* arr3arr3VectorProduct (src/mat_vec_fns_II.cpp:127-134). */
+static void cross_ref(const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - v[1] * u[2];
+    w[1] = -u[0] * v[2] + v[0] * u[2];
+    w[2] = u[0] * v[1] - v[0] * u[1];
+}
+
+/* mag (src/mat_vec_fns_II.cpp:229-239). */
+static double mag_ref(const double *v) {
+    double s = 0.0;
+    for (int i = 0; i < 3; i++) s += v[i] * v[i];
+    return sqrt(s);
+}
+
+/* The force of one rotational constant force on its node at p (src/Blob.cpp:2104-2133): r from
+ * p to its foot on the axis (intersectingPointToLine, src/mat_vec_fns_II.cpp:632-680, with
+ * detByRows :261-269; a point within 3 eps of the axis is its own foot), f = r x d, scaled by F
+ * ('f') or by F / |r| ('t'). */
+static void rot_force(const double *p, const double *axis, double F, int torque, double *f) {
+    const double *p1 = axis, *d = axis + 3;
+    double p0p1[3], v2[3], cp[3], v1v2[3], r[3];
+    for (int k = 0; k < 3; k++) p0p1[k] = p[k] - p1[k];
+    cross_ref(p0p1, d, v2);
+    if (mag_ref(v2) < 3.0 * DBL_EPSILON) {
+        for (int k = 0; k < 3; k++) r[k] = p[k];
+    } else {
+        cross_ref(d, v2, cp);
+        cross_ref(d, cp, v1v2);
+        double c = 0;
+        c = p0p1[0] * (d[1] * v1v2[2] - d[2] * v1v2[1]);
+        c += p0p1[1] * (d[2] * v1v2[0] - d[0] * v1v2[2]);
+        c += p0p1[2] * (d[0] * v1v2[1] - d[1] * v1v2[0]);
+        c /= (v1v2[0] * v1v2[0] + v1v2[1] * v1v2[1] + v1v2[2] * v1v2[2]);
+        for (int k = 0; k < 3; k++) cp[k] = c * cp[k];
+        for (int k = 0; k < 3; k++) r[k] = p[k] + cp[k];
+    }
+    for (int k = 0; k < 3; k++) r[k] = r[k] - p[k];
+    cross_ref(r, d, f);
+    const double s = torque ? F / mag_ref(r) : F;
+    for (int k = 0; k < 3; k++) f[k] = s * f[k];
+}
+
+/* rot_force for the tests (the checker of the checker compares it with the reference's own
+ * helpers). */
+void orc_rot_force(const double *p, const double *axis, double F, int torque, double *f) {
+    rot_force(p, axis, F, torque, f);
 }
 
 /* Linear surface forces (src/Blob.cpp:3350-3424): the blob's surface faces [3 n_faces] (node
@@ -786,6 +862,12 @@ static double face_area(const double *pos, const int *n) {
 static void blob_apply_ctforces(orc_blob *b) {
     for (int i = 0; i < b->n_ctf; i++)
         for (int k = 0; k < 3; k++) b->force[3 * b->ctf_node[i] + k] += b->ctf_force[3 * i + k];
+    for (int i = 0; i < b->n_rctf; i++) {
+        double f[3];
+        const int n = b->rctf_node[i];
+        rot_force(&b->pos[3 * n], &b->rctf_axis[6 * i], b->rctf_F[i], b->rctf_torque[i], f);
+        for (int k = 0; k < 3; k++) b->force[3 * n + k] += f[k];
+    }
     for (int s = 0; s < b->n_sets; s++) {
         int j0 = b->set_off[s], j1 = b->set_off[s + 1];
         double *areas = xcalloc(j1 > j0 ? j1 - j0 : 1, sizeof(double));

@@ -147,6 +147,18 @@ class Blob:
         self.lib.lib.orc_blob_set_constant_forces(self.h, nodes.size, nodes if nodes.size else np.zeros(1, np.int32),
                                                   forces if forces.size else np.zeros(3))
 
+    def set_rotational_forces(self, nodes, F, axis, torque):
+        """Rotational constant forces (oracle only): nodes [n], F [n] meso force, axis [n, 6]
+        (point in meso length, unit direction), torque [n] (1: 't', 0: 'f')."""
+        nodes = np.ascontiguousarray(nodes, np.int32).ravel()
+        n = nodes.size
+        F = np.ascontiguousarray(F, np.float64).ravel()
+        axis = np.ascontiguousarray(axis, np.float64).ravel()
+        torque = np.ascontiguousarray(torque, np.int32).ravel()
+        if n == 0:
+            nodes, F, axis, torque = np.zeros(1, np.int32), np.zeros(1), np.zeros(6), np.zeros(1, np.int32)
+        self.lib.lib.orc_blob_set_rotational_forces(self.h, n, nodes, F, axis, torque)
+
     def set_surface_forces(self, faces, sets, set_forces):
         """Linear surface forces (oracle only): faces [F, 3] node ids; sets: list of face-index
         lists; set_forces [S, 3] meso units."""
@@ -227,6 +239,8 @@ class Oracle(_Lib):
         L.orc_blob_last_cg_iterations.restype = C.c_int
         L.orc_blob_set_constant_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
         L.orc_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
+        L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
+        L.orc_rot_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         L.orc_world_set_springs.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
         L.orc_world_spring_energy.argtypes = [C.POINTER(C.c_void_p), C.c_int]
         L.orc_world_spring_energy.restype = C.c_double
@@ -248,6 +262,13 @@ class Oracle(_Lib):
 
     def stream(self) -> "OracleStream":
         return OracleStream(self)
+
+    def rot_force(self, p, axis, F, torque) -> np.ndarray:
+        """One rotational constant force on a node at p (meso units)."""
+        out = np.zeros(3)
+        self.lib.orc_rot_force(np.ascontiguousarray(p, np.float64), np.ascontiguousarray(axis, np.float64),
+                               float(F), int(torque), out)
+        return out
 
     def strain_energy(self, blob: Blob) -> float:
         return self.lib.orc_blob_strain_energy(blob.h)
@@ -318,6 +339,14 @@ class Reference(_Lib):
         L.ref_rng_get_state.argtypes = [C.c_void_p, _ulp]
         L.ref_rng_set_seed.argtypes = [C.c_void_p, _ulp]
         L.ref_rng_advance_state.argtypes = [C.c_void_p, C.c_long, C.c_long]
+        L.ref_rot_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
+
+    def rot_force(self, p, axis, F, torque) -> np.ndarray:
+        """The same through the reference's own geometry helpers."""
+        out = np.zeros(3)
+        self.lib.ref_rot_force(np.ascontiguousarray(p, np.float64), np.ascontiguousarray(axis, np.float64),
+                               float(F), int(torque), out)
+        return out
 
     def stream(self, per_thread: bool = False) -> "RefStream":
         """One RngStream, or (per_thread) one per OpenMP thread as World::init makes them."""

@@ -1,8 +1,8 @@
 /*
  * ref_harness.cpp -- drives the reference's OWN compiled hot-path objects
  * (tetra_element_linear, NoMassCGSolver, ConjugateGradientSolver, MassLumpedSolver,
- * SparsityPattern, SparseMatrixFixedPattern, SparseMatrixTypes, mat_vec_fns, RngStream,
- * mesh_node) from /root/reference/src.
+ * SparsityPattern, SparseMatrixFixedPattern, SparseMatrixTypes, mat_vec_fns, mat_vec_fns_II,
+ * RngStream, mesh_node) from /root/reference/src.
  *
  * TEST INFRASTRUCTURE ONLY (the checker of the checker).  Built by oracle/Makefile
  * into oracle/_ref/libffea_ref.so; nothing in the product links it.
@@ -26,6 +26,7 @@
 
 #include "RngStream.h"
 #include "mat_vec_fns.h"
+#include "mat_vec_fns_II.h"
 #include "mesh_node.h"
 #include "tetra_element_linear.h"
 #define private public /* read NoMassCGSolver::V for matrix comparisons */
@@ -67,6 +68,27 @@ int ref_num_threads(void) {
     return 1;
 #endif
 }
+/* One rotational constant force on a node at p, as Blob::apply_ctforces CTF 2 computes it
+ * (src/Blob.cpp:2104-2136, type 'p'): the reference's own intersectingPointToLine,
+ * arr3arr3Substract, arr3arr3VectorProduct, mag and arr3Resize (src/mat_vec_fns_II.cpp). */
+void ref_rot_force(const double *p, const double *axis, double F, int torque, double *out) {
+    vector3 pos;
+    pos.assign(p[0], p[1], p[2]);
+    arr3 r, f;
+    double a[6];
+    for (int k = 0; k < 6; k++) a[k] = axis[k];
+    intersectingPointToLine<scalar, arr3>(pos, arr3_view<scalar, arr3>(a, 3), arr3_view<scalar, arr3>(a + 3, 3), r);
+    arr3arr3Substract<scalar, arr3>(r, pos.data, r);
+    arr3arr3VectorProduct<scalar, arr3>(r, arr3_view<scalar, arr3>(a + 3, 3), f);
+    if (torque) {
+        scalar rsize = mag<scalar, arr3>(r);
+        arr3Resize<scalar, arr3>(F / rsize, f);
+    } else {
+        arr3Resize<scalar, arr3>(F, f);
+    }
+    for (int k = 0; k < 3; k++) out[k] = f[k];
+}
+
 void ref_rng_delete(void *g) { delete[] static_cast<RngStream *>(g); }
 int ref_rng_set_package_seed(const unsigned long seed[6]) { return RngStream::SetPackageSeed(seed) ? 1 : 0; }
 double ref_rng_randu01(void *g) { return static_cast<RngStream *>(g)->RandU01(); }

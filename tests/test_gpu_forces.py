@@ -1,9 +1,10 @@
 """GPU parity of the forces from outside the element loop (ffh_blob_set_springs /
-_constant_forces / _surface_forces) against the oracle (pinned in tests/test_oracle_forces.py).
+_constant_forces / _rotational_forces / _surface_forces) against the oracle (pinned in tests/test_oracle_forces.py).
 
 Bars: with MassLumpedSolver, whose solve has no reduction, a noisy 10-step trajectory with
-springs, constant forces on a vertex and on an edge midpoint and two surface sets (sharing a
-face) is bit-exact -- force vector, velocities, positions, stream -- as is a two-blob batch whose
+springs, constant forces on a vertex and on an edge midpoint, rotational forces (circular and
+torque, on vertices and a midpoint, one axis through its node) and surface sets (two sharing
+a face) is bit-exact -- force vector, velocities, positions, stream -- as is a two-blob batch whose
 spring joins the blobs; with the massless solver the force vector is bit-exact and positions
 within 1e-9 of the displacement (CG tree sums); spring energy equal to the oracle's; graph
 replay equal to ordinary launches; spring ends on edge midpoints are refused.  Surface faces
@@ -39,18 +40,29 @@ def forces(m, x0, sc, node_off=0):
                      + [m.topo[e, [4, 7, 5]] for e in range(16, 18)])
     sets = [[0, 3, 5, 7], [2, 3, 11], [12, 13, 14, 15, 16, 17]]
     sf = np.array([[1.0, 0.0, 0.5], [-0.2, 0.7, 0.0], [0.0, 0.4, -0.9]])
-    return springs, consts, (faces, sets, sf)
+    # rotational forces: circular and torque, on vertices (one also carrying a linear force)
+    # and on an edge midpoint; one axis passes through a vertex (its own foot)
+    c = p[lin].mean(axis=0)
+    d = np.array([0.3, -0.4, 0.866])
+    d = d / np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+    rmid = int(m.topo[9, 8])
+    rot = ([int(lin[5]), int(lin[11]), rmid, int(lin[2])], [0.6, 0.25, 0.4, 0.3],
+           np.array([list(c) + list(d), list(c) + list(d), list(c) + [1.0, 0.0, 0.0], list(p[lin[2]]) + list(d)]),
+           [0, 1, 1, 0])
+    return springs, (consts, rot), (faces, sets, sf)
 
 
 def apply_oracle(oracle, bo, springs, consts, surf):
     oracle.set_springs([0] * len(springs[0]), springs[0], [0] * len(springs[0]), springs[1], springs[2], springs[3])
-    bo.set_constant_forces(consts[0], consts[1])
+    bo.set_constant_forces(*consts[0])
+    bo.set_rotational_forces(*consts[1])
     bo.set_surface_forces(*surf)
 
 
 def apply_gpu(b, springs, consts, surf):
     b.set_springs(*springs)
-    b.set_constant_forces(*consts)
+    b.set_constant_forces(*consts[0])
+    b.set_rotational_forces(*consts[1])
     b.set_surface_forces(*surf)
 
 
@@ -143,6 +155,7 @@ def test_graph_replay_with_forces():
     for x in (a, b):
         x.set_springs([], [], [], [])
         x.set_constant_forces([], np.zeros((0, 3)))
+        x.set_rotational_forces([], [], np.zeros((0, 6)), [])
         x.set_surface_forces(np.zeros((0, 3)), [], np.zeros((0, 3)))
     a.step(2)
     b.step(2)
@@ -160,3 +173,8 @@ def test_midpoint_ends_are_refused():
         b.set_springs([int(lin[0])], [mid], [1.0], [1.0])
     with pytest.raises(_lib.FFHError):
         b.set_springs([int(lin[0])], [int(lin[0])], [1.0], [1.0])
+    ax = np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 1.0]])
+    with pytest.raises(_lib.FFHError):  # neither 'f' (0) nor 't' (1)
+        b.set_rotational_forces([int(lin[0])], [1.0], ax, [2])
+    with pytest.raises(_lib.FFHError):
+        b.set_rotational_forces([m.num_nodes], [1.0], ax, [0])

@@ -146,3 +146,66 @@ def test_surface_forces_on_midpoint_corners_are_conserved(oracle, clean_springs)
     assert np.allclose(diff.sum(axis=0), SF[0], rtol=0, atol=1e-11 * scale)
     mids = np.unique(m.topo[:, 4:])
     assert np.all(b[mids] == 0)
+
+
+# ---- rotational constant forces (CTF 2, src/Blob.cpp:2104-2136) -----------------------------
+
+def rot_cases(seed=3, n=200):
+    rng = np.random.default_rng(seed)
+    for k in range(n):
+        p = rng.normal(size=3) * 10 ** rng.uniform(-2, 2)
+        d = rng.normal(size=3)
+        d /= np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+        a = np.concatenate([rng.normal(size=3), d])
+        yield p, a, rng.uniform(0.1, 3.0), k & 1
+
+
+def test_rot_force_matches_reference_helpers(oracle, reference):
+    """The restatement against the reference's own intersectingPointToLine, cross product,
+    mag and resize (src/mat_vec_fns_II.cpp), bit for bit, including a point on the axis."""
+    for p, a, F, t in rot_cases():
+        assert np.array_equal(oracle.rot_force(p, a, F, t), reference.rot_force(p, a, F, t))
+    a = np.array([0.5, -1.0, 2.0, 0.0, 0.6, 0.8])
+    p = a[:3] + 2.5 * a[3:]  # on the axis: its own foot
+    for t in (0, 1):
+        assert np.array_equal(oracle.rot_force(p, a, 1.2, t), reference.rot_force(p, a, 1.2, t), equal_nan=True)
+
+
+def test_rot_force_geometry(oracle):
+    """r runs from the node to its foot on the axis, perpendicular to the axis, as long as the
+    node's distance to it; f = F (r x d): tangential, |f| = F |r| for 'f' and F for 't'."""
+    for p, a, F, t in rot_cases(seed=11, n=100):
+        x0, d = a[:3], a[3:]
+        r = x0 + np.dot(p - x0, d) * d - p
+        dist = np.linalg.norm(np.cross(p - x0, d))
+        f = oracle.rot_force(p, a, F, t)
+        want = (F / dist if t else F) * np.cross(r, d)
+        assert np.allclose(f, want, rtol=1e-9, atol=1e-12 * np.linalg.norm(want))
+        assert np.linalg.norm(f) == pytest.approx(F if t else F * dist, rel=1e-9)
+        assert abs(np.dot(f, d)) <= 1e-9 * np.linalg.norm(f)
+    p, a = np.array([1.0, 2.0, 3.0]), np.array([1.0, 2.0, 0.0, 0.0, 0.0, 1.0])
+    assert np.array_equal(oracle.rot_force(p, a, 2.0, 0), np.zeros(3))
+
+
+def test_rotational_force_on_vertex_and_midpoint(oracle, clean_springs):
+    """Added to the node's force after the linear constant forces; on an edge midpoint it is
+    evaluated at the midpoint and passed on half to each edge end."""
+    m, prm, sc, x0 = case()
+    lin = np.unique(m.topo[:, :4])
+    v = int(lin[7])
+    e = 6
+    mid, ea, eb = int(m.topo[e, 5]), int(m.topo[e, 0]), int(m.topo[e, 2])  # midpoint of edge (0, 2)
+    ax = np.array([[0.1, 0.2, 0.0, 0.0, 0.6, 0.8], [0.0, 0.0, 0.3, 1.0, 0.0, 0.0]])
+    F, T = [0.7, 0.4], [0, 1]
+    a, b = run_pair(oracle, m, prm, sc, x0, lambda bl: bl.set_rotational_forces([v, mid], F, ax, T))
+    diff = b - a
+    p = x0 * sc
+    pm = 0.5 * (p[ea] + p[eb])
+    fv = oracle.rot_force(p[v], ax[0], F[0], T[0])
+    fm = oracle.rot_force(pm, ax[1], F[1], T[1])
+    scale = np.abs(a).max()
+    assert np.allclose(diff[v], fv, rtol=0, atol=1e-12 * scale)
+    assert np.allclose(diff[ea], 0.5 * fm, rtol=0, atol=1e-12 * scale)
+    assert np.allclose(diff[eb], 0.5 * fm, rtol=0, atol=1e-12 * scale)
+    others = np.setdiff1d(np.arange(m.num_nodes), [v, ea, eb])
+    assert np.array_equal(b[others], a[others])

@@ -5,6 +5,7 @@ from the values the readers parse, and the restart reader keeps only complete fr
 import os
 
 import numpy as np
+import pytest
 
 from conftest import GOLDEN
 
@@ -91,12 +92,57 @@ def test_ctforces_file():
     (tests/physics/E_cyl_160_fine): one linear surface set of 220 faces, 1e-11 N along x."""
     from ffea_amd import formats as F, units as U
     path = os.path.join(GOLDEN, "physics", "cyl_160_fine_E", "cyl_160_fine_surf.ctforces")
-    (nodes, forces), (sets, sf) = F.read_ctforces(path, 0, 49819)
+    ct = F.read_ctforces(path, 0, 49819)
+    (nodes, forces), (sets, sf) = ct.linear, ct.surface
+    assert ct.rotational[0].size == 0
     assert nodes.size == 0 and len(sets) == 1 and sets[0][:3] == [8580, 8581, 8582] and len(sets[0]) == 220
     assert sf[0, 0] == 1e-11 * 1.0 / U.MESO_FORCE and sf[0, 1] == 0 and sf[0, 2] == 0
     # a blob the file does not name gets nothing
-    (nodes, _), (sets, _) = F.read_ctforces(path, 1, 10)
+    ct = F.read_ctforces(path, 1, 10)
+    (nodes, _), (sets, _) = ct.linear, ct.surface
     assert nodes.size == 0 and sets == []
+
+
+def test_ctforces_rotational(tmp_path):
+    """The example .ctforces file of the reference's user manual (docs/2-UserManual/
+    7-FFEAFiles.md:257-270), with all three kinds: per blob the lines that name it, "all"
+    expanded; the rotational axis point scaled by the inverse length unit, F by the inverse
+    force unit (src/Blob.cpp:3296-3312).  The manual prints num_ctforces 5 for its six forces,
+    which the reader's count check refuses (src/Blob.cpp:3152-3170); the fixture says 6."""
+    from ffea_amd import formats as F, units as U
+    path = os.path.join(GOLDEN, "formats", "manual_example.ctforces")
+    ct = F.read_ctforces(path, 0, 12)
+    nodes, Fr, axis, torque = ct.rotational
+    assert list(nodes) == list(range(12)) and (torque == 1).all()
+    assert (Fr == 1e-10 * (1.0 / U.MESO_FORCE)).all()
+    want = [(1.0 / U.MESO_LENGTH) * x for x in (-1.9e-9, -1.4313e-9, 8.911e-10)] + [0.0, 1.0, 0.0]
+    assert (axis == np.array(want)).all()
+    assert ct.linear[0].size == 0 and ct.surface[0] == [[280, 281, 282, 283, 284, 285]]
+    ct = F.read_ctforces(path, 1, 7)
+    assert list(ct.linear[0]) == list(range(7)) and ct.rotational[0].size == 0 and ct.surface[0] == []
+    assert (ct.linear[1] == np.array([0.0, 0.0, 1e-9 / U.MESO_FORCE])).all()
+    ct = F.read_ctforces(path, 2, 200)
+    assert list(ct.linear[0]) == [104, 134, 152]
+    bad = tmp_path / "as_printed.ctforces"
+    bad.write_text(open(path).read().replace("num_ctforces 6", "num_ctforces 5"))
+    with pytest.raises(F.FormatError):
+        F.read_ctforces(str(bad), 0, 12)
+
+
+def test_ctforces_rotational_types(tmp_path):
+    """'pf' is a circular force (torque 0), an unnormalised direction is normalised, and an
+    axis through two nodes ('n' types), which the reference leaves undefined, is refused."""
+    from ffea_amd import formats as F
+    head = "ffea ctforces file\nnum_ctforces 1\nnum_rot_forces 1\nrotational forces:\n"
+    p = tmp_path / "a.ctforces"
+    p.write_text(head + "2e-11 pf 0 0 0 0 0 2 0 0 5\n")
+    nodes, Fr, axis, torque = F.read_ctforces(str(p), 0, 10).rotational
+    assert list(nodes) == [5] and list(torque) == [0] and list(axis[0, 3:]) == [0.0, 0.0, 1.0]
+    for bad in ("2e-11 nf 0 0 0 0 0 2 0 0 5", "2e-11 px 0 0 0 0 0 1 0 0 5", "2e-11 qt 0 0 0 0 0 1 0 0 5",
+                "2e-11 pf 0 0 0 0 0 1 0 0"):
+        p.write_text(head + bad + "\n")
+        with pytest.raises(F.FormatError):
+            F.read_ctforces(str(p), 0, 10)
 
 
 def test_script_interaction_files():
