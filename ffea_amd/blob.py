@@ -161,6 +161,18 @@ class Blob:
             n = nd.size
         _lib.check(self._L.ffh_blob_set_rotational_forces(self._h, n, nd, fr, ax, tq))
 
+    def set_box(self, box=None, periodic_mask: int = 0b111111) -> None:
+        """World::run's periodic box (meso units): each step first moves a blob whose centroid
+        has left the box through a periodic wall back in.  periodic_mask bits 0..5:
+        wall_x_1, wall_x_2, wall_y_1, wall_y_2, wall_z_1, wall_z_2.  box None removes it."""
+        if box is None:
+            _lib.check(self._L.ffh_blob_set_box(self._h, np.zeros(3), 0))
+            return
+        bx = np.ascontiguousarray(box, np.float64).ravel()
+        if bx.size != 3:
+            raise ValueError("box is three lengths")
+        _lib.check(self._L.ffh_blob_set_box(self._h, bx, int(periodic_mask)))
+
     def set_surface_forces(self, faces, sets, set_forces) -> None:
         """Linear surface forces: faces [F, 3] (vertex ids), sets: lists of face indices,
         set_forces [S, 3] total force of each set (meso units); empty sets clear."""

@@ -13,6 +13,7 @@
 //                  k_cg_update, k_blob_totals; src/NoMassCGSolver.cpp:165-292 and
 //                  src/ConjugateGradientSolver.cpp:205-349), k_integrate (src/Blob.cpp:3937-3967),
 //                  energies
+//   ffh_box.h      World::run's periodic box (a blob whose centroid leaves it moves back in)
 //   ffh_mpnn.h     the learned preconditioner
 // This file holds the per-handle state (ffh_blob: buffers, launches, step graphs, tuning) and
 // the C-ABI.
@@ -63,6 +64,7 @@ constexpr int kLaneTable = 4 * 256 * 2;  // jump[kLaneTable + 2 l + comp] = A_co
 #include "ffh_element.h"
 #include "ffh_forces.h"
 #include "ffh_solve.h"
+#include "ffh_box.h"
 
 #include "ffh_mpnn.h"
 
@@ -122,6 +124,11 @@ struct ffh_blob {
     FaceArgs face = {};
     int nsprings = 0, nsets = 0, nmid = 0;
     double *spring_e = nullptr;
+    // periodic box (ffh_blob_set_box); node_off_h: the blobs' N_t offsets
+    std::vector<int> node_off_h;
+    int box_mask = 0, wrap_chunks = 0;
+    WrapArgs wrap = {};
+    std::vector<void *> box_allocs;
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
     int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
     bool spmv_nt = false;  // non-temporal matrix loads in the SpMV
@@ -168,6 +175,7 @@ struct ffh_blob {
     ~ffh_blob() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : force_allocs) (void)hipFree(p);
+        for (void *p : box_allocs) (void)hipFree(p);
         for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto &pe : pending) {
             (void)hipEventDestroy(pe.second.first);
@@ -912,6 +920,17 @@ struct ffh_blob {
         fplan = std::move(F);
         return FFH_OK;
     }
+    // World::run's periodic wrap at the start of a step (src/World.cpp:1850-1901)
+    int run_wrap() {
+        if (!box_mask) return FFH_OK;
+        WrapArgs w = wrap;
+        w.pos = pos;
+        const int nc = wrap_chunks, nb = nblobs;
+        int rc = launch(5, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_partials, nc, 256, 0, stream, e0, e1, 0, w); });
+        if (!rc) rc = launch(5, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_decide, (nb + 63) / 64, 64, 0, stream, e0, e1, 0, w, nb); });
+        if (!rc) rc = launch(5, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_move, grid(N, 256), 256, 0, stream, e0, e1, 0, w); });
+        return rc;
+    }
     int reset_inversions() {  // a kernel, not a memset: it is also recorded into step graphs
         hipLaunchKernelGGL(k_reset_inversions, 1, 64, 0, stream, ctl);
         HIP_OR_FAIL(hipGetLastError());
@@ -933,7 +952,8 @@ struct ffh_blob {
             }
         hipGraph_t g = nullptr;
         HIP_OR_FAIL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = reset_inversions();
+        int rc = run_wrap();
+        if (!rc) rc = reset_inversions();
         if (!rc) rc = run_element();
         if (!rc && !massful()) rc = run_assemble();
         if (!rc) rc = run_rhs();
@@ -964,6 +984,7 @@ struct ffh_blob {
     // One step through a graph replay; returns like the ordinary step (inversion, CG status).
     int step_eager() {
         int rc;
+        if ((rc = run_wrap())) return rc;
         if ((rc = reset_inversions())) return rc;
         if ((rc = run_element())) return rc;
         if (!massful() && (rc = run_assemble())) return rc;  // mass-ful solvers use no viscosity matrix
@@ -1054,6 +1075,10 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     b->P_elem = ffh_blob::grid(b->E);
     b->pstride = (size_t)std::max(P.num_slices, b->P_elem);
     b->nblobs = P.num_blobs;
+    if (blob_node_off)
+        b->node_off_h.assign(blob_node_off, blob_node_off + num_blobs + 1);
+    else
+        b->node_off_h = {0, num_nodes};
     b->blob_iters.assign(b->nblobs, 0);
     b->blob_res.assign(b->nblobs, 0.0);
     b->blob_status.assign(b->nblobs, 0);
@@ -1257,6 +1282,7 @@ void ffh_blob_destroy(ffh_blob *b) { delete b; }
 int ffh_blob_update_internal_forces(ffh_blob *b, int *num_inversions) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     int rc;
+    if ((rc = b->run_wrap())) return rc;
     if ((rc = b->reset_inversions())) return rc;
     if ((rc = b->run_element())) return rc;
     if (!b->massful() && (rc = b->run_assemble())) return rc;
@@ -1574,6 +1600,68 @@ int ffh_blob_set_rotational_forces(ffh_blob *b, int num_forces, const int *node,
         (void)b->upload_forces();
     }
     return rc;
+}
+
+int ffh_blob_set_box(ffh_blob *b, const double *box, int periodic_mask) {
+    if (!b || (periodic_mask && !box)) return fail(FFH_E_ARG, "null argument");
+    if (periodic_mask < 0 || periodic_mask > 63) return fail(FFH_E_ARG, "periodic wall mask must be within 0..63");
+    if (periodic_mask)
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(box[k])) return fail(FFH_E_ARG, "non-finite box size");
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the step gains or loses the wrap
+    b->graphs.clear();
+    for (void *q : b->box_allocs) (void)hipFree(q);
+    b->box_allocs.clear();
+    b->box_mask = 0;
+    b->wrap = WrapArgs{};
+    if (!periodic_mask) return FFH_OK;
+    const ffh::Plan &P = b->plan;
+    const int NT = P.num_nodes_total;
+    std::vector<int2> ends(NT, make_int2(-1, -1));
+    for (int v = 0; v < NT; v++)
+        if (P.lin_of[v] >= 0) ends[v] = make_int2(P.lin_of[v], P.lin_of[v]);
+    for (size_t k = 0; k < P.mid_index.size(); k++)
+        ends[P.mid_index[k]] = make_int2(P.lin_of[P.mid_a[k]], P.lin_of[P.mid_b[k]]);
+    for (int v = 0; v < NT; v++)
+        if (ends[v].x < 0 || ends[v].y < 0) return fail(FFH_E_ARG, "node " + std::to_string(v) + " is in no element");
+    std::vector<int32_t> item_off(b->node_off_h.begin(), b->node_off_h.end()), lo, chunk_off(1, 0);
+    for (int k = 0; k < b->nblobs; k++) {
+        for (int t = item_off[k]; t < item_off[k + 1]; t += kWrapChunk) lo.push_back(t);
+        chunk_off.push_back((int32_t)lo.size());
+    }
+    lo.push_back(NT);
+    const int nc = (int)lo.size() - 1;
+    auto put = [&](const void *h, size_t bytes, void **dst) -> int {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(bytes, 8)) != hipSuccess) return fail(FFH_E_NOMEM, "hipMalloc (box)");
+        b->box_allocs.push_back(q);
+        if (h && hipMemcpy(q, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(FFH_E_HIP, "box upload");
+        *dst = q;
+        return FFH_OK;
+    };
+    WrapArgs w = {};
+    int rc = put(ends.data(), ends.size() * sizeof(int2), (void **)&w.ends);
+    if (!rc) rc = put(item_off.data(), item_off.size() * sizeof(int32_t), (void **)&w.blob_item_off);
+    if (!rc) rc = put(lo.data(), lo.size() * sizeof(int32_t), (void **)&w.chunk_lo);
+    if (!rc) rc = put(chunk_off.data(), chunk_off.size() * sizeof(int32_t), (void **)&w.blob_chunk_off);
+    if (!rc) rc = put(nullptr, 6 * (size_t)nc * sizeof(double), (void **)&w.parts);
+    if (!rc) rc = put(nullptr, (size_t)b->nblobs * sizeof(double4), (void **)&w.shift);
+    if (rc) {
+        for (void *q : b->box_allocs) (void)hipFree(q);
+        b->box_allocs.clear();
+        return rc;
+    }
+    w.slice_blob = b->slice_blob;
+    w.box_x = box[0];
+    w.box_y = box[1];
+    w.box_z = box[2];
+    w.mask = periodic_mask;
+    w.N = b->N;
+    b->wrap = w;
+    b->wrap_chunks = nc;
+    b->box_mask = periodic_mask;
+    return FFH_OK;
 }
 
 int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_nodes, int num_sets, const int *set_off,

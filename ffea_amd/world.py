@@ -198,13 +198,15 @@ def linearised(pos, topo):
     return p
 
 
-def place_blobs(params: dict, blobs, positions):
+def place_blobs(params: dict, blobs, positions, return_box: bool = False):
     """Positions (meso units, [N_t, 3] per blob) after World::init places the blobs:
     <centroid> (Blob::position, src/Blob.cpp:886-918: the centroid times 1/N, moved to the
     given point times the blob's scale), <rotation> (Blob::rotate about the centroid), then
     move_into_box (default 1): every blob shifted so that the system centroid sits at the
     centre of a box of es_N * ssint_cutoff per axis, es_N = 2 ceil(extent / ssint_cutoff)
-    unless given (and an interaction or electrostatics flag is on)."""
+    unless given (and an interaction or electrostatics flag is on).  The box (box_dim,
+    src/World.cpp:455-471) is set up whether or not the blobs move into it; return_box adds
+    it to the result."""
     out = []
     for b, p in zip(blobs, positions):
         p = np.array(p, np.float64)
@@ -225,8 +227,6 @@ def place_blobs(params: dict, blobs, positions):
             p = np.stack([x * r[0] + y * r[1] + z * r[2] + com[0], x * r[3] + y * r[4] + z * r[5] + com[1],
                           x * r[6] + y * r[7] + z * r[8] + com[2]], axis=1)
         out.append(p)
-    if int(float(params.get("move_into_box", 1))) != 1:
-        return out
     # World::get_system_dimensions (src/World.cpp:3413-3466) with Blob::get_min_max
     # (src/Blob.cpp:2383-2411), whose per-axis update is "max, else min"
     lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
@@ -254,7 +254,24 @@ def place_blobs(params: dict, blobs, positions):
         wc += _centroid(p) * p.shape[0]
     wc /= tot
     shift = box / 2.0 - wc
-    return [p + shift for p in out]
+    if int(float(params.get("move_into_box", 1))) == 1:
+        out = [p + shift for p in out]
+    return (out, box) if return_box else out
+
+
+# SimulationParams wall keys (src/SimulationParams.cpp:420-480; PBC by default), in the bit
+# order of ffh_blob_set_box's periodic mask
+WALL_KEYS = ("wall_x_1", "wall_x_2", "wall_y_1", "wall_y_2", "wall_z_1", "wall_z_2")
+
+
+def periodic_mask(params: dict) -> int:
+    """Which walls are periodic (PBC, the default); an unknown value keeps the default, as the
+    reference's parser does."""
+    m = 0
+    for k, key in enumerate(WALL_KEYS):
+        if params.get(key, "PBC") not in ("HARD", "STOP"):
+            m |= 1 << k
+    return m
 
 
 # ---------------------------------------------------------------------------------------------
@@ -305,8 +322,9 @@ class World:
         self.nodes = [m.num_nodes for m in self.meshes]
         # the rest state comes from the positions as loaded; World::init then places the blobs,
         # and pos_0 (the RMSD reference) is taken after that (src/World.cpp:502-506)
-        self.pos0 = place_blobs(self.raw, s["blobs"],
-                                [linearised(m.pos * sc, m.topo) for m, sc in zip(self.meshes, self.scales)])
+        self.pos0, self.box = place_blobs(self.raw, s["blobs"],
+                                          [linearised(m.pos * sc, m.topo) for m, sc in zip(self.meshes, self.scales)],
+                                          return_box=True)
         if self.restart:
             st = read_checkpoint(self.ckpt_in)[0]
         else:
@@ -314,6 +332,15 @@ class World:
         self.batch = Batch(self.meshes, self.params, self.scales, rng_state=st, precision=precision)
         self.batch.set_positions(np.concatenate(self.pos0))
         self.batch.set_solver(self.solver)
+        # walls: periodic ones wrap a blob whose centroid leaves the box at the start of every
+        # step (src/World.cpp:1850-1901).  A hard wall zeroes outward velocities of surface nodes
+        # (Blob::enforce_box_boundaries, src/Blob.cpp:2272-2330), which the massless solve then
+        # replaces; with a mass-ful solver that clamp is not on this path.
+        if self.mass_in_system and any(self.raw.get(k) == "HARD" for k in WALL_KEYS):
+            raise F.FormatError("hard walls with a mass-ful solver are not on this path")
+        self.walls = periodic_mask(self.raw)
+        if self.walls:
+            self.batch.set_box(self.box, self.walls)
         self.calc_springs = int(float(self.raw.get("calc_springs", 0))) != 0
         self.calc_ctforces = int(float(self.raw.get("calc_ctforces", 0))) == 1
         self._interactions(s)

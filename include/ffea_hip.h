@@ -212,6 +212,18 @@ int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_node
                                 const int *set_faces, const double *set_force);
 
 /*
+ * World::run's periodic box (src/World.cpp:1850-1901; box_dim from World::init, :455-471).
+ * With periodic_mask != 0, every step (ffh_blob_step) and every ffh_blob_update_internal_forces
+ * begins as the reference's step does: for each blob, the centroid of its N_t nodes
+ * (Blob::get_centroid, src/Blob.cpp:971-983); per axis, below 0 with that low wall periodic,
+ * else above box[axis] with that high wall periodic, the blob moves by +/- box[axis]
+ * (Blob::move, :935-949).  periodic_mask bits 0..5: wall_x_1, wall_x_2, wall_y_1, wall_y_2,
+ * wall_z_1, wall_z_2 (SimulationParams: PBC is the default for all six); 0 removes the box.
+ * Hard walls (Blob::enforce_box_boundaries) are not on this path.
+ */
+int ffh_blob_set_box(ffh_blob *b, const double *box, int periodic_mask);
+
+/*
  * Mass-ful solvers: Blob::init's solver choice (src/Blob.cpp:358-373) with the codes of
  * include/FFEA_return_codes.h:30-33.  FFH_SOLVER_CG is ConjugateGradientSolver on the
  * consistent mass matrix (src/ConjugateGradientSolver.cpp:62-230), FFH_SOLVER_MASSLUMPED is
@@ -267,7 +279,7 @@ int ffh_blob_synchronize(ffh_blob *b);
  * bracketed by events; ffh_blob_kernel_times returns, for each kernel id, the summed
  * milliseconds and launch counts since the last reset.  Kernel ids:
  *   0 element   1 assemble   2 rhs+CG init   3 CG spmv (p update, q = K p, p.q)
- *   4 CG update (x, r, residual)   5 integrate
+ *   4 CG update (x, r, residual)   5 integrate (and the periodic box wrap)
  *   learned preconditioner: 6 couplings A_ij   7 embedding   8 message (A h)
  *   9 dense layer (MFMA)   10 output layer
  *   11 per-blob totals of the CG reductions (blobs of more than 3072 64-row slices)

@@ -36,7 +36,7 @@
  *   sparse pattern / matrix    src/SparsityPattern.cpp:57-143,
  *                              src/SparseMatrixFixedPattern.cpp:48-68, :85-93,
  *                              :148-191, :325-333; src/SparseMatrixTypes.cpp:52-58
- *   step order                 src/World.cpp:1811-2043
+ *   step order                 src/World.cpp:1811-2043 (periodic box :1850-1901)
  *
  * Layout is the reference's: all N_t nodes (linear + quadratic midpoints),
  * 10-node elements, a scalar CSR over 3*N_t rows whose every entry keeps the
@@ -1370,9 +1370,56 @@ int orc_blob_update_positions(orc_blob *b, orc_stream *g) {
  * zero forces, element phase for every blob, then position phase for every blob.
  * Returns the total inversion count (the reference aborts the run if > 0), or -1 if a
  * solve failed. */
+/* World::run's periodic box (src/World.cpp:1850-1901): box_dim and which walls are periodic,
+ * bits 0..5 = wall_x_1, wall_x_2, wall_y_1, wall_y_2, wall_z_1, wall_z_2.  mask 0: no walls
+ * act (the default here, for the tests that drive blobs without a World box). */
+static double g_box[3];
+static int g_box_mask;
+void orc_world_set_box(const double *box, int mask) {
+    for (int k = 0; k < 3; k++) g_box[k] = box[k];
+    g_box_mask = mask;
+}
+
+/* Blob::get_centroid (src/Blob.cpp:971-983) over all N_t nodes. */
+void orc_blob_centroid(const orc_blob *b, double *com) {
+    com[0] = com[1] = com[2] = 0;
+    for (int n = 0; n < b->num_nodes; n++)
+        for (int k = 0; k < 3; k++) com[k] += b->pos[3 * n + k];
+    for (int k = 0; k < 3; k++) com[k] /= b->num_nodes;
+}
+
+/* A blob whose centroid has left the box through a periodic wall moves by one box length
+ * back in (Blob::move, src/Blob.cpp:935-949: every node); per axis the low wall is checked
+ * first, the high one only if the centroid is not below zero. */
+static void blob_wrap(orc_blob *b) {
+    if (!g_box_mask) return;
+    double com[3], d[3] = {0, 0, 0};
+    int move = 0;
+    orc_blob_centroid(b, com);
+    for (int k = 0; k < 3; k++) {
+        if (com[k] < 0) {
+            if (g_box_mask & (1 << (2 * k))) {
+                d[k] += g_box[k];
+                move = 1;
+            }
+        } else if (com[k] > g_box[k]) {
+            if (g_box_mask & (2 << (2 * k))) {
+                d[k] -= g_box[k];
+                move = 1;
+            }
+        }
+    }
+    if (move)
+        for (int n = 0; n < b->num_nodes; n++)
+            for (int k = 0; k < 3; k++) b->pos[3 * n + k] += d[k];
+}
+
 int orc_world_step(orc_blob **blobs, int nblobs, orc_stream *g) {
     int inv = 0, err = 0;
-    for (int i = 0; i < nblobs; i++) blob_zero_forces(blobs[i]);
+    for (int i = 0; i < nblobs; i++) {
+        blob_zero_forces(blobs[i]);
+        blob_wrap(blobs[i]); /* src/World.cpp:1843-1904 */
+    }
     world_apply_springs(blobs); /* src/World.cpp:1933 */
     for (int i = 0; i < nblobs; i++) inv += orc_blob_update_internal_forces(blobs[i], g);
     if (inv > 0) return inv;

@@ -241,6 +241,8 @@ class Oracle(_Lib):
         L.orc_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
         L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
         L.orc_rot_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_world_set_box.argtypes = [_dp, C.c_int]
+        L.orc_blob_centroid.argtypes = [C.c_void_p, _dp]
         L.orc_world_set_springs.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
         L.orc_world_spring_energy.argtypes = [C.POINTER(C.c_void_p), C.c_int]
         L.orc_world_spring_energy.restype = C.c_double
@@ -262,6 +264,16 @@ class Oracle(_Lib):
 
     def stream(self) -> "OracleStream":
         return OracleStream(self)
+
+    def set_box(self, box, mask: int):
+        """World::run's periodic box for the next world_step calls: box [3] meso, mask bits
+        0..5 = periodic wall_x_1, x_2, y_1, y_2, z_1, z_2 (0: no walls)."""
+        self.lib.orc_world_set_box(np.ascontiguousarray(box, np.float64), int(mask))
+
+    def centroid(self, blob: Blob) -> np.ndarray:
+        out = np.zeros(3)
+        self.lib.orc_blob_centroid(blob.h, out)
+        return out
 
     def rot_force(self, p, axis, F, torque) -> np.ndarray:
         """One rotational constant force on a node at p (meso units)."""
