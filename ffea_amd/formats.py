@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import os
 import re
+import time
 from typing import NamedTuple
 
 import numpy as np
@@ -129,6 +130,51 @@ def read_surface(path):
     if L[2] != "faces:":
         raise FormatError(f"{path}: could not find 'faces:' line")
     return np.array([[int(x) for x in L[3 + i].split()[:4]] for i in range(n)], np.int32)
+
+
+def read_ssint_types(path, num_faces):
+    """Blob::load_ssint (src/Blob.cpp:3494-3578): one interaction type per surface face, -1
+    for an inactive face ("ffea vdw file" / "ffea ssint file")."""
+    L = _lines(path)
+    if L[0] not in ("walrus vdw file", "ffea vdw file", "ffea ssint file"):
+        raise FormatError(f"{path}: this is not a 'ffea ssint file' (read '{L[0]}')")
+    n = _count(L[1], "num_faces", path)
+    if n != num_faces:
+        raise FormatError(f"{path}: {n} faces, the surface file has {num_faces}")
+    if L[2] not in ("vdw params:", "ssint params:"):
+        raise FormatError(f"{path}: could not find 'ssint params:' line")
+    if len(L) < 3 + n:
+        raise FormatError(f"{path}: expected {n} face types")
+    return np.array([int(L[3 + i].split()[0]) for i in range(n)], np.int32)
+
+
+def read_ssint_params(path, ssint_type="lennard-jones", ssint_cutoff=None):
+    """SSINT_matrix::init_ssint (src/LJ_matrix.cpp:122-199): the ntypes x ntypes table of
+    (Emin, Rmin[, k0]) converted to meso units -- Emin * area^2 / Energy, Rmin / length,
+    k0 * area / Energy; with "gensoft" Rmin is the cutoff.  Returns (Emin, Rmin, k0) as
+    [ntypes, ntypes] arrays, entry [i, j] for face types i and j."""
+    from .units import MESO_AREA, MESO_ENERGY, MESO_LENGTH
+    with open(path) as f:
+        lines = f.read().split("\n")
+    if lines[0] not in ("ffea vdw forcefield params file", "ffea ssint forcefield params file"):
+        raise FormatError(f"{path}: this is not a 'ffea ssint forcefield params file' (read '{lines[0]}')")
+    nt = int(lines[1].split(" ")[1])
+    E, R, K = (np.zeros((nt, nt)) for _ in range(3))
+    for i in range(nt):
+        sets = lines[2 + i].split(")")
+        for j in range(nt):
+            vals = sets[j].strip().replace("(", "", 1).split(",")
+            if len(vals) > 3 or len(vals) < 2:
+                raise FormatError(f"{path}: parameter set ({i}, {j}) needs 2 or 3 values")
+            v = [float(x) for x in vals] + [0.0] * (3 - len(vals))
+            if v[0] <= 0 or v[1] <= 0:
+                raise FormatError(f"{path}: Emin and Rmin must be greater than 0")
+            if ssint_type == "gensoft":
+                v[1] = ssint_cutoff * MESO_LENGTH
+            E[i, j] = v[0] * MESO_AREA * MESO_AREA / MESO_ENERGY
+            R[i, j] = v[1] / MESO_LENGTH
+            K[i, j] = v[2] * MESO_AREA / MESO_ENERGY
+    return E, R, K
 
 
 def _script_lines(path):
@@ -383,7 +429,7 @@ def read_script(path) -> dict:
             if k in b:
                 continue  # first conformation only
             b[k] = v
-        for k in ("nodes", "topology", "surface", "material", "stokes", "pin", "vdw"):
+        for k in ("nodes", "topology", "surface", "material", "stokes", "pin", "vdw", "ssint"):
             if k in b:
                 b[k] = os.path.normpath(os.path.join(base, b[k]))
         blobs.append(b)
@@ -405,7 +451,9 @@ def params_from_script(params: dict):
 
     eps = g("epsilon", 0.1)  # default epsilon2 = 0.01 (src/SimulationParams.cpp:36)
     stokes = int(float(params.get("calc_stokes", params.get("do_stokes", 1))))
+    # rng_seed "time": the clock's seconds (src/SimulationParams.cpp:328-333)
+    seed = int(time.time()) if params.get("rng_seed") == "time" else int(g("rng_seed", 0))
     return SimParams.from_si(dt=g("dt", 1e-14), kT=g("kT", 4.11e-21), epsilon=eps,
                              max_iterations_cg=int(g("max_iterations_cg", 1000)),
                              calc_noise=int(g("calc_noise", 1)), calc_stokes=stokes,
-                             stokes_visc=g("stokes_visc", 1e-3), rng_seed=int(g("rng_seed", 0)))
+                             stokes_visc=g("stokes_visc", 1e-3), rng_seed=seed)

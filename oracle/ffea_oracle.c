@@ -492,6 +492,10 @@ typedef struct {
     int *set_off, *set_faces;
     double *set_force;     /* [3 n_sets] */
     double *face_force;    /* [9 n_faces]: the three face nodes' forces (Face::force[0..2]) */
+    /* surface-surface interactions: per surface face its type (-1: inactive) and the
+     * geometry Face::calc_area_normal_centroid last left (centroid, normal, area) */
+    int *ssint_type;       /* [n_faces] or NULL */
+    double *face_geom;     /* [7 n_faces] */
 } orc_blob;
 
 static void *xcalloc(size_t n, size_t s) {
@@ -716,7 +720,7 @@ void orc_blob_destroy(orc_blob *b) {
                     b->contrib, b->key, b->col, b->src_off, (void *)b->src, b->val, b->diag_inv, b->diag_idx,
                     b->r, b->p, b->z, b->q, b->f, b->mkey, b->mcol, b->mval, b->mprec, b->inv_M,
                     b->ctf_node, b->ctf_force, b->rctf_node, b->rctf_F, b->rctf_axis, b->rctf_torque, b->face_nodes, b->set_off, b->set_faces, b->set_force,
-                    b->face_force};
+                    b->face_force, b->ssint_type, b->face_geom};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(b);
 }
@@ -835,6 +839,10 @@ void orc_blob_set_surface_forces(orc_blob *b, int n_faces, const int *faces, int
     b->set_faces = xcalloc(n_sets > 0 && set_off[n_sets] > 0 ? set_off[n_sets] : 1, sizeof(int));
     b->set_force = xcalloc(3 * (size_t)(n_sets > 0 ? n_sets : 1), sizeof(double));
     b->face_force = xcalloc(9 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
+    free(b->ssint_type);
+    free(b->face_geom);
+    b->ssint_type = NULL;
+    b->face_geom = xcalloc(7 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
     if (n_faces > 0) memcpy(b->face_nodes, faces, sizeof(int) * 3 * n_faces);
     if (n_sets > 0) {
         memcpy(b->set_off, set_off, sizeof(int) * (n_sets + 1));
@@ -1414,13 +1422,239 @@ static void blob_wrap(orc_blob *b) {
             for (int k = 0; k < 3; k++) b->pos[3 * n + k] += d[k];
 }
 
+/* ---- surface-surface interactions: the Lennard-Jones VdW_solver ---------------------- */
+
+/* VdW_solver's three-point triangle rule (src/VdW_solver.cpp:62-73): weight, then the
+ * barycentric coordinates, as the decimal constants the reference writes. */
+static const double g_gw = 0.333333333333333;
+static const double g_geta[3][3] = {{0.666666666666667, 0.166666666666667, 0.166666666666667},
+                                    {0.166666666666667, 0.666666666666667, 0.166666666666667},
+                                    {0.166666666666667, 0.166666666666667, 0.666666666666667}};
+/* adjacent_cell_lookup_table (src/VdW_solver.cpp:31-59): dx, dy, dz each -1..1, z fastest */
+static const int g_adj[27][3] = {{-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, -1}, {-1, 0, 0}, {-1, 0, 1},
+                                 {-1, 1, -1},  {-1, 1, 0},  {-1, 1, 1},  {0, -1, -1}, {0, -1, 0}, {0, -1, 1},
+                                 {0, 0, -1},   {0, 0, 0},   {0, 0, 1},   {0, 1, -1},  {0, 1, 0},  {0, 1, 1},
+                                 {1, -1, -1},  {1, -1, 0},  {1, -1, 1},  {1, 0, -1},  {1, 0, 0},  {1, 0, 1},
+                                 {1, 1, -1},   {1, 1, 0},   {1, 1, 1}};
+
+/* SSINT_matrix in meso units (LJI(i, j) = i * ntypes + j), the lookup grid and the solver's
+ * switches (World::init, src/World.cpp:896-970; VdW_solver::init). ntypes 0: off. */
+static int g_ss_ntypes, g_ss_inc_self, g_ss_es_update, g_ss_es_count, g_ss_N[3];
+static double *g_ss_emin, *g_ss_rmin, g_ss_h;
+static double *g_ss_energy; /* fieldenergy [nblobs * nblobs] of the last solve */
+static int g_ss_nblobs;
+/* the lookup grid: per pool entry (blob, face) its cell and the next entry down its stack */
+static int g_ss_npool, *g_ss_pool_blob, *g_ss_pool_face, *g_ss_pool_cell, *g_ss_next, *g_ss_root;
+
+void orc_world_set_ssint(int ntypes, const double *emin, const double *rmin, const int *N, double h, int inc_self,
+                         int es_update) {
+    free(g_ss_emin);
+    free(g_ss_rmin);
+    free(g_ss_root);
+    g_ss_emin = g_ss_rmin = NULL;
+    g_ss_root = NULL;
+    g_ss_ntypes = ntypes;
+    if (ntypes <= 0) return;
+    g_ss_emin = xcalloc((size_t)ntypes * ntypes, sizeof(double));
+    g_ss_rmin = xcalloc((size_t)ntypes * ntypes, sizeof(double));
+    memcpy(g_ss_emin, emin, sizeof(double) * ntypes * ntypes);
+    memcpy(g_ss_rmin, rmin, sizeof(double) * ntypes * ntypes);
+    for (int k = 0; k < 3; k++) g_ss_N[k] = N[k];
+    g_ss_h = h;
+    g_ss_inc_self = inc_self;
+    g_ss_es_update = es_update;
+    g_ss_es_count = es_update; /* World::run starts with an update (src/World.cpp:1805) */
+    g_ss_root = xcalloc((size_t)N[0] * N[1] * N[2], sizeof(int));
+}
+
+/* Blob::load_ssint (src/Blob.cpp:3494-3578): the type of every surface face (-1 inactive);
+ * call after orc_blob_set_surface_forces has given the faces. */
+void orc_blob_set_ssint_types(orc_blob *b, const int *types) {
+    free(b->ssint_type);
+    b->ssint_type = xcalloc(b->n_faces > 0 ? b->n_faces : 1, sizeof(int));
+    if (b->n_faces > 0) memcpy(b->ssint_type, types, sizeof(int) * b->n_faces);
+}
+
+/* Face::calc_area_normal_centroid (src/Face.cpp:197-219). */
+static void face_geometry(const double *pos, const int *fn, double *geo) {
+    const double *p0 = &pos[3 * fn[0]], *p1 = &pos[3 * fn[1]], *p2 = &pos[3 * fn[2]];
+    double a[3], c[3], nrm[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = p1[k] - p0[k];
+        c[k] = p2[k] - p0[k];
+    }
+    nrm[0] = a[1] * c[2] - a[2] * c[1];
+    nrm[1] = a[2] * c[0] - a[0] * c[2];
+    nrm[2] = a[0] * c[1] - a[1] * c[0];
+    const double m = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    geo[6] = .5 * m;
+    for (int k = 0; k < 3; k++) geo[3 + k] = nrm[k] / m;
+    for (int k = 0; k < 3; k++) geo[k] = (1.0 / 3.0) * (p0[k] + p1[k] + p2[k]);
+}
+
+/* LinkedListCube::pbc (src/LinkedListCube.tpp:282-299): one period at most. */
+static void cell_pbc(int *x) {
+    for (int k = 0; k < 3; k++) {
+        if (x[k] < 0)
+            x[k] += g_ss_N[k];
+        else if (x[k] >= g_ss_N[k])
+            x[k] -= g_ss_N[k];
+    }
+}
+
+/* The es_update refresh (src/World.cpp:1821-1837, 1907-1925): every face's geometry, then
+ * NearestNeighbourLinkedListCube::build_nearest_neighbour_lookup
+ * (src/NearestNeighbourLinkedListCube.cpp:26-77): the pool is the active faces blob by blob in
+ * face order (src/World.cpp:940-962); entry i goes on top of its cell's stack.  Returns -1 for a
+ * centroid outside the grid after one period. */
+static int ssint_refresh(orc_blob **blobs, int nblobs) {
+    int n = 0;
+    for (int b = 0; b < nblobs; b++)
+        for (int f = 0; f < blobs[b]->n_faces; f++) {
+            face_geometry(blobs[b]->pos, &blobs[b]->face_nodes[3 * f], &blobs[b]->face_geom[7 * f]);
+            if (blobs[b]->ssint_type && blobs[b]->ssint_type[f] != -1) n++;
+        }
+    free(g_ss_pool_blob);
+    free(g_ss_pool_face);
+    free(g_ss_pool_cell);
+    free(g_ss_next);
+    g_ss_pool_blob = xcalloc(n ? n : 1, sizeof(int));
+    g_ss_pool_face = xcalloc(n ? n : 1, sizeof(int));
+    g_ss_pool_cell = xcalloc(n ? n : 1, sizeof(int));
+    g_ss_next = xcalloc(n ? n : 1, sizeof(int));
+    g_ss_npool = n;
+    const int ncell = g_ss_N[0] * g_ss_N[1] * g_ss_N[2];
+    for (int c = 0; c < ncell; c++) g_ss_root[c] = -1;
+    int i = 0;
+    for (int b = 0; b < nblobs; b++)
+        for (int f = 0; f < blobs[b]->n_faces; f++) {
+            if (!blobs[b]->ssint_type || blobs[b]->ssint_type[f] == -1) continue;
+            const double *geo = &blobs[b]->face_geom[7 * f];
+            int x[3];
+            for (int k = 0; k < 3; k++) x[k] = (int)floor(geo[k] / g_ss_h);
+            cell_pbc(x);
+            for (int k = 0; k < 3; k++)
+                if (x[k] < 0 || x[k] >= g_ss_N[k]) return -1;
+            const int c = x[0] * g_ss_N[1] * g_ss_N[2] + x[1] * g_ss_N[2] + x[2];
+            g_ss_pool_blob[i] = b;
+            g_ss_pool_face[i] = f;
+            g_ss_pool_cell[i] = c;
+            g_ss_next[i] = g_ss_root[c];
+            g_ss_root[c] = i;
+            i++;
+        }
+    return 0;
+}
+
+/* VdW_solver::calc_lj_factors (src/VdW_solver.cpp:713-731). */
+static void lj_factors(double mag_r, int k, int l, double Emin, double Rmin_6, double *force_mag, double *e) {
+    const double ri = 1. / mag_r;
+    const double ri2 = ri * ri, ri4 = ri2 * ri2, ri6 = ri4 * ri2, ri7 = ri6 * ri;
+    const double f6 = Rmin_6 * ri6;
+    *force_mag = 12 * ri7 * Rmin_6 * Emin * (f6 - 1);
+    *e = g_gw * g_gw * Emin * f6 * (f6 - 2);
+    (void)k;
+    (void)l;
+}
+
+/* VdW_solver::do_lj_interaction (src/VdW_solver.cpp:236-324) with calc_lj_force_pair_matrix
+ * (:663-697): Gauss points of both faces, the force pair matrix (its lower triangle a copy of
+ * the upper, as the reference fills it), the energy scaled by both areas, and the forces on
+ * the three corners of each face added in corner order, face 1 then face 2. */
+static void lj_pair(orc_blob *b1, int f1, orc_blob *b2, int f2, double *energy_out) {
+    const int t1 = b1->ssint_type[f1], t2 = b2->ssint_type[f2];
+    const double Emin = g_ss_emin[t1 * g_ss_ntypes + t2], Rmin = g_ss_rmin[t1 * g_ss_ntypes + t2];
+    const int *n1 = &b1->face_nodes[3 * f1], *n2 = &b2->face_nodes[3 * f2];
+    double p[3][3], q[3][3], fpm[3][3][3];
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) {
+            p[i][k] = g_geta[i][0] * b1->pos[3 * n1[0] + k] + g_geta[i][1] * b1->pos[3 * n1[1] + k] +
+                      g_geta[i][2] * b1->pos[3 * n1[2] + k];
+            q[i][k] = g_geta[i][0] * b2->pos[3 * n2[0] + k] + g_geta[i][1] * b2->pos[3 * n2[1] + k] +
+                      g_geta[i][2] * b2->pos[3 * n2[2] + k];
+        }
+    const double Rmin_2 = Rmin * Rmin, Rmin_4 = Rmin_2 * Rmin_2, Rmin_6 = Rmin_4 * Rmin_2;
+    double energy = 0.0, fm, e;
+    for (int k = 0; k < 3; k++) {
+        for (int l = k; l < 3; l++) {
+            const double dx = p[k][0] - q[l][0], dy = p[k][1] - q[l][1], dz = p[k][2] - q[l][2];
+            const double mag_r = sqrt(dx * dx + dy * dy + dz * dz);
+            lj_factors(mag_r, k, l, Emin, Rmin_6, &fm, &e);
+            energy += l == k ? e : 2 * e;
+            for (int c = 0; c < 3; c++) fpm[k][l][c] = fm * ((p[k][c] - q[l][c]) / mag_r);
+            if (l != k)
+                for (int c = 0; c < 3; c++) fpm[l][k][c] = fpm[k][l][c];
+        }
+    }
+    const double ApAq = b1->face_geom[7 * f1 + 6] * b2->face_geom[7 * f2 + 6];
+    *energy_out = energy * ApAq;
+    for (int j = 0; j < 3; j++) {
+        double F1[3] = {0, 0, 0}, F2[3] = {0, 0, 0};
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) {
+                const double c = g_gw * g_gw * g_geta[l][j];
+                const double d = g_gw * g_gw * g_geta[k][j];
+                for (int x = 0; x < 3; x++) F1[x] += c * fpm[k][l][x];
+                for (int x = 0; x < 3; x++) F2[x] -= d * fpm[l][k][x];
+            }
+        for (int x = 0; x < 3; x++) b1->face_force[9 * f1 + 3 * j + x] += F1[x] * ApAq;
+        for (int x = 0; x < 3; x++) b2->face_force[9 * f2 + 3 * j + x] += F2[x] * ApAq;
+    }
+}
+
+/* VdW_solver::solve (src/VdW_solver.cpp:158-210) with consider_interaction (:332-384), Lennard-
+ * Jones type: each pool entry i against the stacks of its cell and the 26 around it (top
+ * down), pairs with i < j, of different blobs unless inc_self_ssint, whose normals do not face
+ * the same way (dot product not above zero). */
+static void ssint_solve(orc_blob **blobs, int nblobs) {
+    free(g_ss_energy);
+    g_ss_energy = xcalloc((size_t)nblobs * nblobs, sizeof(double));
+    g_ss_nblobs = nblobs;
+    for (int i = 0; i < g_ss_npool; i++) {
+        const int bi = g_ss_pool_blob[i], fi = g_ss_pool_face[i], ci = g_ss_pool_cell[i];
+        const int cx = ci / (g_ss_N[1] * g_ss_N[2]), cy = (ci / g_ss_N[2]) % g_ss_N[1], cz = ci % g_ss_N[2];
+        const double *ni = &blobs[bi]->face_geom[7 * fi + 3];
+        for (int c = 0; c < 27; c++) {
+            int x[3] = {cx + g_adj[c][0], cy + g_adj[c][1], cz + g_adj[c][2]};
+            cell_pbc(x);
+            for (int j = g_ss_root[x[0] * g_ss_N[1] * g_ss_N[2] + x[1] * g_ss_N[2] + x[2]]; j >= 0; j = g_ss_next[j]) {
+                if (!(i < j)) continue;
+                const int bj = g_ss_pool_blob[j], fj = g_ss_pool_face[j];
+                if (!g_ss_inc_self && bi == bj) continue;
+                const double *nj = &blobs[bj]->face_geom[7 * fj + 3];
+                if (ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2] > 0.0) continue;
+                double e;
+                lj_pair(blobs[bi], fi, blobs[bj], fj, &e);
+                g_ss_energy[bi * nblobs + bj] += e;
+            }
+        }
+    }
+}
+
+/* VdW_solver::get_field_energy(-1, -1) (src/VdW_solver.cpp:635-661): every blob pair's sum. */
+double orc_world_ssint_energy(void) {
+    double e = 0.0;
+    for (int i = 0; i < g_ss_nblobs; i++)
+        for (int j = 0; j < g_ss_nblobs; j++) e += g_ss_energy[i * g_ss_nblobs + j];
+    return e;
+}
+
 int orc_world_step(orc_blob **blobs, int nblobs, orc_stream *g) {
     int inv = 0, err = 0;
     for (int i = 0; i < nblobs; i++) {
         blob_zero_forces(blobs[i]);
         blob_wrap(blobs[i]); /* src/World.cpp:1843-1904 */
     }
+    if (g_ss_ntypes > 0) { /* the es_update refresh, at the top of the step (src/World.cpp:1821-1925) */
+        if (g_ss_es_count == g_ss_es_update) {
+            g_ss_es_count = 1;
+            if (ssint_refresh(blobs, nblobs) != 0) return -2;
+        } else {
+            g_ss_es_count++;
+        }
+    }
     world_apply_springs(blobs); /* src/World.cpp:1933 */
+    if (g_ss_ntypes > 0) ssint_solve(blobs, nblobs); /* src/World.cpp:2002-2004 */
     for (int i = 0; i < nblobs; i++) inv += orc_blob_update_internal_forces(blobs[i], g);
     if (inv > 0) return inv;
     for (int i = 0; i < nblobs; i++)

@@ -159,6 +159,10 @@ class Blob:
             nodes, F, axis, torque = np.zeros(1, np.int32), np.zeros(1), np.zeros(6), np.zeros(1, np.int32)
         self.lib.lib.orc_blob_set_rotational_forces(self.h, n, nodes, F, axis, torque)
 
+    def set_ssint_types(self, types):
+        """Surface-surface interaction type of every face given to set_surface_forces (-1 off)."""
+        self.lib.lib.orc_blob_set_ssint_types(self.h, np.ascontiguousarray(types, np.int32))
+
     def set_surface_forces(self, faces, sets, set_forces):
         """Linear surface forces (oracle only): faces [F, 3] node ids; sets: list of face-index
         lists; set_forces [S, 3] meso units."""
@@ -242,6 +246,9 @@ class Oracle(_Lib):
         L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
         L.orc_rot_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         L.orc_world_set_box.argtypes = [_dp, C.c_int]
+        L.orc_world_set_ssint.argtypes = [C.c_int, _dp, _dp, _ip, C.c_double, C.c_int, C.c_int]
+        L.orc_blob_set_ssint_types.argtypes = [C.c_void_p, _ip]
+        L.orc_world_ssint_energy.restype = C.c_double
         L.orc_blob_centroid.argtypes = [C.c_void_p, _dp]
         L.orc_world_set_springs.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
         L.orc_world_spring_energy.argtypes = [C.POINTER(C.c_void_p), C.c_int]
@@ -264,6 +271,20 @@ class Oracle(_Lib):
 
     def stream(self) -> "OracleStream":
         return OracleStream(self)
+
+    def set_ssint(self, emin=None, rmin=None, grid=(1, 1, 1), h=1.0, inc_self=1, es_update=1):
+        """Lennard-Jones surface-surface interactions for the next world_step calls: emin, rmin
+        [ntypes, ntypes] meso, the lookup grid's cell counts and cell size (meso), inc_self_ssint
+        and es_update.  emin None switches them off."""
+        if emin is None:
+            self.lib.orc_world_set_ssint(0, np.zeros(1), np.zeros(1), np.zeros(3, np.int32), 1.0, 0, 1)
+            return
+        e = np.ascontiguousarray(emin, np.float64)
+        self.lib.orc_world_set_ssint(e.shape[0], e.ravel(), np.ascontiguousarray(rmin, np.float64).ravel(),
+                                     np.ascontiguousarray(grid, np.int32), float(h), int(inc_self), int(es_update))
+
+    def ssint_energy(self) -> float:
+        return self.lib.orc_world_ssint_energy()
 
     def set_box(self, box, mask: int):
         """World::run's periodic box for the next world_step calls: box [3] meso, mask bits

@@ -1,0 +1,127 @@
+"""Lennard-Jones surface-surface interactions (VdW_solver with ssint_type lennard-jones,
+src/VdW_solver.cpp:158-384, 663-731) in the oracle.  VdW_solver, Face and the lookup grid need
+Blob and World, which do not compile here (Eigen), so they are pinned by:
+
+- the reference's own physics test tests/physics/cubes_lj (checkE.py): two unit cubes whose top
+  faces (200 active triangles each) face each other 1 nm apart, Emin = 2.537e17 J/m^4,
+  Rmin = 1 nm; the double surface integral of E(1/r^12 - 2/r^6) gives -1.7870e-19 J (CUBA), and
+  the measured SurfSurfEnergy must be within 1 %;
+- Newton's third law: every pair adds equal and opposite totals to its two faces, so the
+  interaction forces sum to zero over all nodes;
+- the lookup grid only decides which pairs are considered: with every face in range of every
+  other (a grid of one cell per axis, whose 27 neighbours are all that cell) the energy is that
+  of all pairs, each counted 27 times;
+- inc_self_ssint 0 leaves a blob alone; faces whose normals point the same way do not interact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from conftest import GOLDEN
+
+CUBES = os.path.join(GOLDEN, "physics", "cubes_lj", "cube_veryFine.ffea")
+
+
+def cubes_world(oracle):
+    from ffea_amd import formats as F, units as U
+    from ffea_amd.world import linearised, place_blobs
+    s = F.read_script(CUBES)
+    prm = F.params_from_script(s["params"])
+    meshes, scales = [], []
+    for b in s["blobs"]:
+        meshes.append(F.load_blob(b["nodes"], b["topology"], b["material"], b["stokes"], b.get("pin")))
+        scales.append(U.blob_scale(float(b["scale"])))
+    pos, box = place_blobs(s["params"], s["blobs"], [linearised(m.pos * sc, m.topo) for m, sc in zip(meshes, scales)],
+                           return_box=True)
+    blobs = []
+    for b, m, sc, p in zip(s["blobs"], meshes, scales, pos):
+        o = oracle.blob(m, prm, sc)
+        o.set_positions(p.ravel())
+        faces = F.read_surface(b["surface"])[:, 1:4]
+        o.set_surface_forces(faces, [], np.zeros((0, 3)))
+        o.set_ssint_types(F.read_ssint_types(b["vdw"], faces.shape[0]))
+        blobs.append(o)
+    lj = os.path.join(os.path.dirname(CUBES), s["params"]["vdw_forcefield_params"])
+    E, R, _ = F.read_ssint_params(lj, "lennard-jones")
+    cutoff = 3e-9 / U.MESO_LENGTH
+    return blobs, E, R, cutoff, s
+
+
+@pytest.fixture
+def ssint_off(oracle):
+    yield
+    oracle.set_ssint()
+
+
+def test_cubes_lj_energy(oracle, ssint_off):
+    from ffea_amd import units as U
+    blobs, E, R, h, s = cubes_world(oracle)
+    oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=0, es_update=1)
+    oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+    assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
+    e = oracle.ssint_energy() * U.MESO_ENERGY
+    assert abs(e - -1.7870e-19) / 1.7870e-19 < 1e-2, e
+
+
+def interaction_rhs(oracle, blobs, E, R, grid, h, inc_self):
+    """The force vectors handed to the solvers with and without the interaction (noise off)."""
+    out = []
+    pos = [b.get("pos").copy() for b in blobs]
+    for on in (False, True):
+        for b, p in zip(blobs, pos):
+            b.set_positions(p)
+        if on:
+            oracle.set_ssint(E, R, grid, h, inc_self=inc_self, es_update=1)
+        else:
+            oracle.set_ssint()
+        oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+        assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
+        out.append([b.get("rhs").reshape(-1, 3) for b in blobs])
+    for b, p in zip(blobs, pos):
+        b.set_positions(p)
+    return out
+
+
+def test_momentum_conserved(oracle, ssint_off):
+    blobs, E, R, h, _ = cubes_world(oracle)
+    for b in blobs:
+        b.set_solver("masslumped")  # the force vector is the solver's input, pins zeroed
+    (a0, a1), (b0, b1) = interaction_rhs(oracle, blobs, E, R, (10, 10, 10), h, 0)
+    d = [b0 - a0, b1 - a1]
+    tot = d[0].sum(axis=0) + d[1].sum(axis=0)
+    scale = np.abs(d[0]).sum()
+    assert scale > 0
+    assert np.abs(tot).max() < 1e-9 * scale
+    # the cubes attract: blob 0's interaction force points to blob 1 (+y), and back
+    assert d[0].sum(axis=0)[1] > 0 and d[1].sum(axis=0)[1] < 0
+
+
+def test_grid_only_selects_pairs(oracle, ssint_off):
+    """One cell per axis: all 27 neighbour offsets wrap onto the same cell, so every pair is met
+    27 times; the energy is 27 times the 10-cell grid's, whose cells of 3 nm already hold both
+    top faces (1 nm apart) as neighbours."""
+    blobs, E, R, h, _ = cubes_world(oracle)
+    oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=0, es_update=1)
+    oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+    pos = [b.get("pos").copy() for b in blobs]
+    assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
+    e10 = oracle.ssint_energy()
+    for b, p in zip(blobs, pos):
+        b.set_positions(p)
+    oracle.set_ssint(E, R, (1, 1, 1), 1e3, inc_self=0, es_update=1)
+    oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+    assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
+    assert oracle.ssint_energy() == pytest.approx(27 * e10, rel=1e-12)
+
+
+def test_self_interaction_switch_and_normals(oracle, ssint_off):
+    """With inc_self_ssint 0 a lone cube feels nothing; with 1 its active top face (coplanar
+    triangles, normals all the same way) still feels nothing."""
+    blobs, E, R, h, _ = cubes_world(oracle)
+    one = blobs[:1]
+    for inc in (0, 1):
+        (a,), (b,) = interaction_rhs(oracle, one, E, R, (10, 10, 10), h, inc)
+        assert np.array_equal(a, b)
+        assert oracle.ssint_energy() == 0.0
