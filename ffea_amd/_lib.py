@@ -41,7 +41,7 @@ EXPORTS = (
     "ffh_blob_set_tuning", "ffh_blob_set_external_forces", "ffh_blob_set_solver",
     "ffh_blob_set_velocities", "ffh_blob_kinetic_energy", "ffh_blob_set_springs",
     "ffh_blob_spring_energy", "ffh_blob_set_constant_forces", "ffh_blob_set_surface_forces",
-    "ffh_blob_set_rotational_forces", "ffh_blob_set_box",
+    "ffh_blob_set_rotational_forces", "ffh_blob_set_box", "ffh_blob_set_ssint", "ffh_blob_ssint_energy",
 )
 FFH_PRECOND_JACOBI = 0
 FFH_PRECOND_LEARNED = 1
@@ -136,6 +136,9 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_set_constant_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     L.ffh_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
     L.ffh_blob_set_box.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.ffh_blob_set_ssint.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _dp, _dp, _ip, C.c_double, C.c_int,
+                                     C.c_int]
+    L.ffh_blob_ssint_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
     for n in EXPORTS:
         if n not in ("ffh_last_error", "ffh_version", "ffh_blob_create", "ffh_blob_destroy",

@@ -14,6 +14,8 @@
 //                  src/ConjugateGradientSolver.cpp:205-349), k_integrate (src/Blob.cpp:3937-3967),
 //                  energies
 //   ffh_box.h      World::run's periodic box (a blob whose centroid leaves it moves back in)
+//   ffh_ssint.h    Lennard-Jones surface-surface interactions (VdW_solver): face geometry and
+//                  lookup grid, and each face's force in the reference's summation order
 //   ffh_mpnn.h     the learned preconditioner
 // This file holds the per-handle state (ffh_blob: buffers, launches, step graphs, tuning) and
 // the C-ABI.
@@ -24,6 +26,7 @@
 
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdlib>
@@ -65,6 +68,7 @@ constexpr int kLaneTable = 4 * 256 * 2;  // jump[kLaneTable + 2 l + comp] = A_co
 #include "ffh_forces.h"
 #include "ffh_solve.h"
 #include "ffh_box.h"
+#include "ffh_ssint.h"
 
 #include "ffh_mpnn.h"
 
@@ -124,6 +128,50 @@ struct ffh_blob {
     FaceArgs face = {};
     int nsprings = 0, nsets = 0, nmid = 0;
     double *spring_e = nullptr;
+    double *trac_rows = nullptr;  // the face groups' value rows (fdev.trac, writable)
+    // surface-surface interactions (ffh_blob_set_ssint): the kernels' arguments, the radix
+    // sort's scratch, a host-visible error flag and the energy's per-blob-pair sums
+    bool ss_on = false;
+    SsArgs ss = {};
+    std::vector<void *> ss_allocs;
+    void *ss_temp = nullptr;
+    size_t ss_temp_bytes = 0;
+    int ss_bits = 1;
+    int *ss_err_host = nullptr;
+    double *ss_fe = nullptr, *ss_total = nullptr;
+    void free_ssint() {
+        for (void *q : ss_allocs) (void)hipFree(q);
+        ss_allocs.clear();
+        if (ss_err_host) (void)hipHostFree(ss_err_host);
+        ss_err_host = nullptr;
+        ss_on = false;
+        ss = SsArgs{};
+        ss_temp = nullptr;
+        ss_temp_bytes = 0;
+    }
+    // the interaction forces of this step's positions, before the element phase as in
+    // World::run (src/World.cpp:1821-1925, 2002-2004); the face groups of the force kernel read them
+    int run_ssint() {
+        if (!ss_on) return FFH_OK;
+        SsArgs a = ss;
+        a.pos = pos;
+        a.out = trac_rows + 3 * (size_t)fplan.ssint_row0;
+        const int P = a.P, nc = a.ncell;
+        hipLaunchKernelGGL(k_ss_tick, 1, 64, 0, stream, a.ctl);
+        HIP_OR_FAIL(hipGetLastError());
+        int rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_geom, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
+        if (rc) return rc;
+        size_t tb = ss_temp_bytes;
+        HIP_OR_FAIL(hipcub::DeviceRadixSort::SortPairs(ss_temp, tb, a.cell, a.sorted_cell, a.order, a.sorted, P, 0, ss_bits, stream));
+        rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_clear, grid(nc, 256), 256, 0, stream, e0, e1, 0, a); });
+        if (!rc) rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_bounds, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
+        if (!rc) rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces, grid(P, 128), 128, 0, stream, e0, e1, 0, a); });
+        return rc;
+    }
+    int check_ssint() {
+        if (ss_on && *ss_err_host) return fail(FFH_E_ARG, "a face centroid lies outside the lookup grid (LinkedListCube)");
+        return FFH_OK;
+    }
     // periodic box (ffh_blob_set_box); node_off_h: the blobs' N_t offsets
     std::vector<int> node_off_h;
     int box_mask = 0, wrap_chunks = 0;
@@ -176,6 +224,7 @@ struct ffh_blob {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : force_allocs) (void)hipFree(p);
         for (void *p : box_allocs) (void)hipFree(p);
+        free_ssint();
         for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto &pe : pending) {
             (void)hipEventDestroy(pe.second.first);
@@ -865,6 +914,7 @@ struct ffh_blob {
         fdev = ForceArgs{};
         face = FaceArgs{};
         spring_e = nullptr;
+        trac_rows = nullptr;
         auto put = [&](auto *&dst, const auto &h) -> int {
             using X = std::remove_reference_t<decltype(h[0])>;
             void *q = nullptr;
@@ -893,18 +943,21 @@ struct ffh_blob {
             std::vector<double> z(nsprings, 0.0);
             FFH_PUT(spring_e, z);
         }
-        if (nsets > 0) {
+        if (!F.grp_off.empty()) {  // face groups: set tractions, or interaction corner forces
             FFH_PUT(fdev.grp_off, F.grp_off);
             FFH_PUT(fdev.grp_ent_off, F.grp_ent_off);
             FFH_PUT(fdev.grp_ent, F.grp_ent);
+            std::vector<double> z3(3 * ((size_t)F.ssint_row0 + fin.ssint_faces.size()) + 3, 0.0);
+            FFH_PUT(trac_rows, z3);
+            fdev.trac = trac_rows;
+        }
+        if (nsets > 0) {
             FFH_PUT(face.set_off, F.set_off);
             FFH_PUT(face.ent_ends, F.ent_ends);
             FFH_PUT(face.set_force, F.set_force);
             std::vector<double> z((size_t)std::max(1, F.set_off.back()), 0.0);
             FFH_PUT(face.area, z);
-            std::vector<double> z3(3 * z.size(), 0.0);
-            FFH_PUT(face.trac, z3);
-            fdev.trac = face.trac;
+            face.trac = trac_rows;
         }
         nmid = (int)(F.mid_ends.size() / 2);
         if (nmid > 0) {
@@ -953,6 +1006,7 @@ struct ffh_blob {
         hipGraph_t g = nullptr;
         HIP_OR_FAIL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int rc = run_wrap();
+        if (!rc) rc = run_ssint();
         if (!rc) rc = reset_inversions();
         if (!rc) rc = run_element();
         if (!rc && !massful()) rc = run_assemble();
@@ -985,6 +1039,7 @@ struct ffh_blob {
     int step_eager() {
         int rc;
         if ((rc = run_wrap())) return rc;
+        if ((rc = run_ssint())) return rc;
         if ((rc = reset_inversions())) return rc;
         if ((rc = run_element())) return rc;
         if (!massful() && (rc = run_assemble())) return rc;  // mass-ful solvers use no viscosity matrix
@@ -1283,10 +1338,12 @@ int ffh_blob_update_internal_forces(ffh_blob *b, int *num_inversions) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     int rc;
     if ((rc = b->run_wrap())) return rc;
+    if ((rc = b->run_ssint())) return rc;
     if ((rc = b->reset_inversions())) return rc;
     if ((rc = b->run_element())) return rc;
     if (!b->massful() && (rc = b->run_assemble())) return rc;
-    return b->read_inversions(num_inversions);
+    if ((rc = b->read_inversions(num_inversions))) return rc;
+    return b->check_ssint();
 }
 
 int ffh_blob_update_positions(ffh_blob *b) {
@@ -1305,6 +1362,7 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
         int rc, cg;
         cg = b->graph_step_ok() ? b->step_graph() : b->step_eager();  // both synchronise
         if (cg != FFH_OK && cg != FFH_E_CG) return cg;
+        if ((rc = b->check_ssint())) return rc;
         if (b->ctl_host->inversions > 0) {
             // the reference aborts before the position phase (src/World.cpp:2025-2028)
             int ninv = 0;
@@ -1600,6 +1658,142 @@ int ffh_blob_set_rotational_forces(ffh_blob *b, int num_forces, const int *node,
         (void)b->upload_forces();
     }
     return rc;
+}
+
+int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const int *face_type, int num_types,
+                       const double *emin, const double *rmin, const int *grid, double cell_size, int inc_self,
+                       int es_update) {
+    if (!b || num_faces < 0 || (num_faces > 0 && (!face_nodes || !face_type))) return fail(FFH_E_ARG, "null argument");
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the step gains or loses the interaction
+    b->graphs.clear();
+    b->free_ssint();
+    ffh::ForceInput keep = b->fin;
+    std::vector<int32_t> pool_nodes, pool_type;
+    for (int f = 0; f < num_faces; f++) {
+        if (face_type[f] == -1) continue;
+        if (face_type[f] < 0 || face_type[f] >= num_types)
+            return fail(FFH_E_ARG, "face " + std::to_string(f) + ": interaction type out of range");
+        for (int k = 0; k < 3; k++) pool_nodes.push_back(face_nodes[3 * f + k]);
+        pool_type.push_back(face_type[f]);
+    }
+    const int P = (int)pool_type.size();
+    b->fin.ssint_faces.clear();
+    if (P == 0) {  // nothing active: no interaction
+        int rc = b->upload_forces();
+        if (rc) {
+            b->fin = keep;
+            (void)b->upload_forces();
+        }
+        return rc;
+    }
+    if (!emin || !rmin || !grid) return fail(FFH_E_ARG, "null argument");
+    if (grid[0] < 1 || grid[1] < 1 || grid[2] < 1 || !(cell_size > 0) || es_update < 1 ||
+        (long long)grid[0] * grid[1] * grid[2] > (1 << 28))
+        return fail(FFH_E_ARG, "invalid lookup grid or es_update");
+    for (int t = 0; t < num_types * num_types; t++)
+        if (!(emin[t] > 0) || !(rmin[t] > 0)) return fail(FFH_E_ARG, "Emin and Rmin must be greater than 0");
+    b->fin.ssint_faces = pool_nodes;
+    int rc = b->upload_forces();
+    if (rc) {
+        b->fin = keep;
+        (void)b->upload_forces();
+        return rc;
+    }
+    const ffh::Plan &Pl = b->plan;
+    std::vector<int32_t> ends(6 * (size_t)P), blob(P);
+    for (int p = 0; p < P; p++) {
+        for (int k = 0; k < 3; k++) {
+            const int v = pool_nodes[3 * p + k];
+            int a = Pl.lin_of[v], c = a;
+            if (a < 0)
+                for (size_t m = 0; m < Pl.mid_index.size(); m++)
+                    if (Pl.mid_index[m] == v) {
+                        a = Pl.lin_of[Pl.mid_a[m]];
+                        c = Pl.lin_of[Pl.mid_b[m]];
+                        break;
+                    }
+            ends[6 * (size_t)p + 2 * k] = a;
+            ends[6 * (size_t)p + 2 * k + 1] = c;
+        }
+        blob[p] = Pl.slice_blob[ends[6 * (size_t)p] / ffh::kSlice];
+    }
+    std::vector<int32_t> order(P);
+    for (int p = 0; p < P; p++) order[p] = p;
+    const int ncell = grid[0] * grid[1] * grid[2];
+    int bits = 1;
+    while ((1LL << bits) < ncell) bits++;
+    SsArgs a = {};
+    auto put = [&](const void *h, size_t bytes, void **dst) -> int {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(bytes, 8)) != hipSuccess) return fail(FFH_E_NOMEM, "hipMalloc (interactions)");
+        b->ss_allocs.push_back(q);
+        if (h && hipMemcpy(q, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(FFH_E_HIP, "interaction upload");
+        *dst = q;
+        return FFH_OK;
+    };
+    SsCtl c0 = {es_update, es_update, 0, 0};  // World::run starts with a refresh (src/World.cpp:1805)
+    rc = put(ends.data(), ends.size() * 4, (void **)&a.ends);
+    if (!rc) rc = put(blob.data(), blob.size() * 4, (void **)&a.blob);
+    if (!rc) rc = put(pool_type.data(), pool_type.size() * 4, (void **)&a.type);
+    if (!rc) rc = put(emin, sizeof(double) * num_types * num_types, (void **)&a.emin);
+    if (!rc) rc = put(rmin, sizeof(double) * num_types * num_types, (void **)&a.rmin);
+    if (!rc) rc = put(nullptr, 8 * sizeof(double) * (size_t)P, (void **)&a.geom);
+    if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&a.cell);
+    if (!rc) rc = put(order.data(), 4 * (size_t)P, (void **)&a.order);
+    if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&a.sorted);
+    if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&a.sorted_cell);
+    if (!rc) rc = put(nullptr, 4 * (size_t)ncell, (void **)&a.cstart);
+    if (!rc) rc = put(nullptr, 4 * (size_t)ncell, (void **)&a.cend);
+    if (!rc) rc = put(&c0, sizeof(c0), (void **)&a.ctl);
+    if (!rc) rc = put(nullptr, sizeof(double) * (size_t)b->nblobs * b->nblobs, (void **)&b->ss_fe);
+    if (!rc) rc = put(nullptr, sizeof(double), (void **)&b->ss_total);
+    size_t tb = 0;
+    if (!rc && hipcub::DeviceRadixSort::SortPairs(nullptr, tb, a.cell, a.sorted_cell, a.order, a.sorted, P, 0, bits,
+                                                  b->stream) != hipSuccess)
+        rc = fail(FFH_E_HIP, "radix sort set-up");
+    if (!rc) rc = put(nullptr, tb, &b->ss_temp);
+    if (!rc && hipHostMalloc((void **)&b->ss_err_host, sizeof(int), hipHostMallocMapped) != hipSuccess)
+        rc = fail(FFH_E_NOMEM, "hipHostMalloc (interaction flag)");
+    int *err_dev = nullptr;
+    if (!rc && hipHostGetDevicePointer((void **)&err_dev, b->ss_err_host, 0) != hipSuccess)
+        rc = fail(FFH_E_HIP, "hipHostGetDevicePointer");
+    if (rc) {
+        b->free_ssint();
+        b->fin = keep;
+        (void)b->upload_forces();
+        return rc;
+    }
+    *b->ss_err_host = 0;
+    a.err = err_dev;
+    a.ntypes = num_types;
+    a.nx = grid[0];
+    a.ny = grid[1];
+    a.nz = grid[2];
+    a.h = cell_size;
+    a.inc_self = inc_self ? 1 : 0;
+    a.P = P;
+    a.ncell = ncell;
+    b->ss = a;
+    b->ss_temp_bytes = tb;
+    b->ss_bits = bits;
+    b->ss_on = true;
+    return FFH_OK;
+}
+
+int ffh_blob_ssint_energy(ffh_blob *b, double *energy) {
+    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
+    if (!b->ss_on) {
+        *energy = 0.0;
+        return FFH_OK;
+    }
+    SsArgs a = b->ss;
+    a.pos = b->pos;
+    hipLaunchKernelGGL(k_ss_energy, 1, 64, 0, b->stream, a, b->nblobs, b->ss_fe, b->ss_total);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipMemcpyAsync(energy, b->ss_total, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return FFH_OK;
 }
 
 int ffh_blob_set_box(ffh_blob *b, const double *box, int periodic_mask) {

@@ -501,6 +501,9 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
     const size_t ns = in.spring_n0.size(), nc = in.const_node.size(), nf = in.faces.size() / 3;
     const size_t nr = in.rot_node.size();
     const size_t nset = in.set_off.empty() ? 0 : in.set_off.size() - 1;
+    const size_t nss = in.ssint_faces.size() / 3;
+    if (nss && nset)
+        return "surface sets (CTF 3) together with surface-surface interactions are not on this path";
     if (in.spring_n1.size() != ns || in.spring_k.size() != ns || in.spring_l.size() != ns ||
         in.const_force.size() != 3 * nc || in.faces.size() != 3 * nf || in.set_force.size() != 3 * nset ||
         in.rot_F.size() != nr || in.rot_axis.size() != 6 * nr || in.rot_torque.size() != nr)
@@ -590,21 +593,31 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
             }
         }
     }
-    // face groups: vertices first, then carrying midpoints
+    // face groups: vertices first, then carrying midpoints; a group key is a set face, or
+    // 3 p + corner for interaction face p
     std::vector<std::vector<int32_t>> grp(N), mgrp;
+    auto add_key = [&](int32_t v, int32_t key) {
+        if (P.lin_of[v] >= 0) {
+            grp[P.lin_of[v]].push_back(key);
+        } else {
+            const int32_t m = midpoint(v);
+            if ((size_t)m >= mgrp.size()) mgrp.resize(m + 1);
+            mgrp[m].push_back(key);
+        }
+    };
     for (size_t f = 0; f < nf; f++) {
         if (ent_of_face[f].empty()) continue;
-        for (int k = 0; k < 3; k++) {
-            const int32_t v = in.faces[3 * f + k];
-            if (P.lin_of[v] >= 0) {
-                grp[P.lin_of[v]].push_back((int32_t)f);
-            } else {
-                const int32_t m = midpoint(v);
-                if ((size_t)m >= mgrp.size()) mgrp.resize(m + 1);
-                mgrp[m].push_back((int32_t)f);
-            }
-        }
+        for (int k = 0; k < 3; k++) add_key(in.faces[3 * f + k], (int32_t)f);
     }
+    F.ssint_row0 = (int32_t)in.set_faces.size();
+    for (size_t p = 0; p < nss; p++)
+        for (int k = 0; k < 3; k++) {
+            const int32_t v = in.ssint_faces[3 * p + k];
+            if (v < 0 || v >= NT) return "interaction face " + std::to_string(p) + ": node index out of range";
+            if (P.lin_of[v] < 0 && end_a[v] < 0)
+                return "interaction face " + std::to_string(p) + ": node " + std::to_string(v) + " is in no element";
+            add_key(v, (int32_t)(3 * p + k));
+        }
     const int M = (int)(F.mid_ends.size() / 2);
     mpre.resize(M);
     mgrp.resize(M);
@@ -622,12 +635,15 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
             F.mid_pre_off[m + 1] = (int32_t)F.pre_item.size();
         }
     }
-    if (nset) {
+    if (nset || nss) {
         F.grp_off.assign((size_t)N + 1, 0);
         F.grp_ent_off.assign(1, 0);
-        auto add_groups = [&](const std::vector<int32_t> &faces) {
-            for (int32_t f : faces) {
-                F.grp_ent.insert(F.grp_ent.end(), ent_of_face[f].begin(), ent_of_face[f].end());
+        auto add_groups = [&](const std::vector<int32_t> &keys) {
+            for (int32_t f : keys) {
+                if (nset)
+                    F.grp_ent.insert(F.grp_ent.end(), ent_of_face[f].begin(), ent_of_face[f].end());
+                else
+                    F.grp_ent.push_back(F.ssint_row0 + f);
                 F.grp_ent_off.push_back((int32_t)F.grp_ent.size());
             }
             return (int32_t)(F.grp_ent_off.size() - 1);

@@ -135,6 +135,9 @@ struct ForceInput {
     std::vector<int32_t> set_off;     // [S+1]
     std::vector<int32_t> set_faces;
     std::vector<double> set_force;    // [3S]
+    // surface-surface interactions (VdW_solver): the active faces' corners in pool order; each
+    // corner's Face::force is one value row the interaction kernel writes every step
+    std::vector<int32_t> ssint_faces; // [3P]
 };
 
 // The same forces as per-node lists in the order the reference adds them to a node's force:
@@ -143,8 +146,9 @@ struct ForceInput {
 //     applies the springs, then update_internal_forces the constant forces, onto the zeroed
 //     force vector;
 //  2. after the element forces: one group per surface face holding the node (ascending face
-//     index), each the sum of the face's set tractions in set order (Face::force, gathered in
-//     aggregate_forces_and_solve, src/Blob.cpp:3842-3851);
+//     index), each the sum of the face's set tractions in set order, or the face corner's
+//     surface-surface interaction force (Face::force, gathered in aggregate_forces_and_solve,
+//     src/Blob.cpp:3842-3851);
 //  3. after the noise: the forces on edge midpoints (constant forces, then face groups, as in
 //     1 and 2), passed on to the edge ends element by element as linearise_force does
 //     (src/Blob.cpp:1066-1085): each (vertex, element) term is half the sum of the element's
@@ -163,7 +167,8 @@ struct ForcePlan {
     std::vector<double> rot;            // [8 * rotational]: axis point, direction, F, torque
     std::vector<int32_t> grp_off;       // [N+1] face groups of the vertices
     std::vector<int32_t> grp_ent_off;   // [groups + 1]
-    std::vector<int32_t> grp_ent;       // set-entry ids
+    std::vector<int32_t> grp_ent;       // value rows: set entries, then 3 p + corner of interaction face p
+    int32_t ssint_row0 = 0;             // first interaction row (after the set entries)
     std::vector<int32_t> set_off;       // [S+1] entries of each set
     std::vector<int32_t> ent_ends;      // [6 * entries] per face corner its two edge ends (a vertex twice)
     std::vector<double> set_force;      // [3S]
@@ -177,7 +182,7 @@ struct ForcePlan {
 };
 
 // Returns "" or an error message (ids out of range, a spring end on an edge midpoint, a spring
-// joining a node to itself).
+// joining a node to itself, surface sets together with surface-surface interactions).
 std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F);
 
 // Exact jump matrices of MRG32k3a component c (0: m1, 1: m2) raised to n.

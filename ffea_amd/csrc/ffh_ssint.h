@@ -1,0 +1,244 @@
+// ffh_ssint.h -- surface-surface Lennard-Jones interactions (VdW_solver with ssint_type
+// "lennard-jones", src/VdW_solver.cpp:158-384, 663-731): face geometry and the lookup grid at
+// es_update steps, and every step each active face's Face::force summed in the reference's
+// order.
+// Part of ffh_kernels.hip: included inside its anonymous namespace, after the headers it
+// lists there; not a stand-alone header.
+#pragma once
+
+// VdW_solver's three-point triangle rule (src/VdW_solver.cpp:62-73): the weight and the
+// barycentric coordinates as the decimal constants the reference writes.
+constexpr double kSsW = 0.333333333333333;
+__constant__ double kSsEta[3][3] = {{0.666666666666667, 0.166666666666667, 0.166666666666667},
+                                    {0.166666666666667, 0.666666666666667, 0.166666666666667},
+                                    {0.166666666666667, 0.166666666666667, 0.666666666666667}};
+
+struct SsCtl {
+    int es_count, es_update, refresh, pad;
+};
+
+// The pool is the active faces in the reference's order (blobs in order, faces in surface
+// order, src/World.cpp:940-962); pool index = LinkedListNode index.
+struct SsArgs {
+    const double4 *pos;
+    const int32_t *ends;   // [6P] each corner's edge ends (compact ids; a vertex twice)
+    const int32_t *blob;   // [P]
+    const int32_t *type;   // [P]
+    const double *emin, *rmin;  // [ntypes^2] meso, entry type1 * ntypes + type2
+    int ntypes;
+    double *geom;          // [8P] centroid, unit normal, area (Face::calc_area_normal_centroid)
+    int32_t *cell;         // [P] lookup-grid cell of each face at the last refresh
+    int32_t *order;        // [P] 0..P-1, the sort's values
+    int32_t *sorted, *sorted_cell;  // [P] pool ids ordered by cell, ascending id within a cell
+    int32_t *cstart, *cend;         // [ncell] each cell's range in `sorted`
+    int nx, ny, nz;
+    double h;
+    int inc_self, P, ncell;
+    SsCtl *ctl;
+    int *err;              // host-visible: a face centroid fell outside the grid
+    double *out;           // [9P] Face::force of the three corners of every pool face
+};
+
+// LinkedListCube::pbc (src/LinkedListCube.tpp:282-299): one period at most.
+__device__ __forceinline__ int ss_wrap(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
+// adjacent_cell_lookup_table entry c (src/VdW_solver.cpp:31-59): z fastest, each -1..1.
+__device__ __forceinline__ int ss_neighbour(const SsArgs &a, int cell, int c) {
+    const int cx = cell / (a.ny * a.nz), cy = (cell / a.nz) % a.ny, cz = cell % a.nz;
+    const int x = ss_wrap(cx + c / 9 - 1, a.nx), y = ss_wrap(cy + (c / 3) % 3 - 1, a.ny), z = ss_wrap(cz + c % 3 - 1, a.nz);
+    return x * a.ny * a.nz + y * a.nz + z;
+}
+
+// es_update bookkeeping of World::run (src/World.cpp:1805, 1821-1837): a refresh on the first
+// step and every es_update steps after it.
+__global__ void k_ss_tick(SsCtl *c) {
+    if (threadIdx.x != 0) return;
+    const int r = c->es_count == c->es_update;
+    c->refresh = r;
+    c->es_count = r ? 1 : c->es_count + 1;
+}
+
+// Face::calc_area_normal_centroid (src/Face.cpp:197-219) and the face's cell in
+// NearestNeighbourLinkedListCube::build_nearest_neighbour_lookup
+// (src/NearestNeighbourLinkedListCube.cpp:26-77), at refresh steps.
+__global__ __launch_bounds__(256) void k_ss_geom(SsArgs a) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= a.P || !a.ctl->refresh) return;
+    const double4 p0 = corner(a.pos, a.ends + 6 * (size_t)f), p1 = corner(a.pos, a.ends + 6 * (size_t)f + 2),
+                  p2 = corner(a.pos, a.ends + 6 * (size_t)f + 4);
+    const double ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
+    const double bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
+    const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+    const double m = sqrt(nx * nx + ny * ny + nz * nz);
+    double *g = a.geom + 8 * (size_t)f;
+    g[6] = .5 * m;
+    g[3] = nx / m;
+    g[4] = ny / m;
+    g[5] = nz / m;
+    g[0] = (1.0 / 3.0) * (p0.x + p1.x + p2.x);
+    g[1] = (1.0 / 3.0) * (p0.y + p1.y + p2.y);
+    g[2] = (1.0 / 3.0) * (p0.z + p1.z + p2.z);
+    int c[3];
+    const int n[3] = {a.nx, a.ny, a.nz};
+    bool ok = true;
+    for (int k = 0; k < 3; k++) {
+        const double q = floor(g[k] / a.h);
+        ok = ok && q > -4.0e8 && q < 4.0e8;
+        c[k] = ok ? ss_wrap((int)q, n[k]) : 0;
+        ok = ok && c[k] >= 0 && c[k] < n[k];
+    }
+    if (!ok) {
+        *a.err = 1;
+        c[0] = c[1] = c[2] = 0;
+    }
+    a.cell[f] = c[0] * a.ny * a.nz + c[1] * a.nz + c[2];
+}
+
+__global__ __launch_bounds__(256) void k_ss_clear(SsArgs a) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= a.ncell || !a.ctl->refresh) return;
+    a.cstart[c] = 0;
+    a.cend[c] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_ss_bounds(SsArgs a) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= a.P || !a.ctl->refresh) return;
+    const int c = a.sorted_cell[s];
+    if (s == 0 || a.sorted_cell[s - 1] != c) a.cstart[c] = s;
+    if (s == a.P - 1 || a.sorted_cell[s + 1] != c) a.cend[c] = s + 1;
+}
+
+// consider_interaction for the Lennard-Jones type (src/VdW_solver.cpp:332-352): other blob
+// unless inc_self_ssint, and normals whose dot product is not above zero.
+__device__ __forceinline__ bool ss_consider(const SsArgs &a, int i, int j) {
+    if (!a.inc_self && a.blob[i] == a.blob[j]) return false;
+    const double *ni = a.geom + 8 * (size_t)i + 3, *nj = a.geom + 8 * (size_t)j + 3;
+    return !(ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2] > 0.0);
+}
+
+// do_lj_interaction (src/VdW_solver.cpp:236-324) for faces f1 < f2 with
+// calc_lj_force_pair_matrix and calc_lj_factors (:663-731): side 0 adds face 1's corner forces
+// to acc, side 1 face 2's; the pair's energy (scaled by both areas) goes to *energy if given.
+__device__ void ss_pair(const SsArgs &a, int f1, int f2, int side, double *acc, double *energy) {
+    const int t = a.type[f1] * a.ntypes + a.type[f2];
+    const double Emin = a.emin[t], Rmin = a.rmin[t];
+    double p[3][3], q[3][3];
+    {
+        const double4 a0 = corner(a.pos, a.ends + 6 * (size_t)f1), a1 = corner(a.pos, a.ends + 6 * (size_t)f1 + 2),
+                      a2 = corner(a.pos, a.ends + 6 * (size_t)f1 + 4);
+        const double4 b0 = corner(a.pos, a.ends + 6 * (size_t)f2), b1 = corner(a.pos, a.ends + 6 * (size_t)f2 + 2),
+                      b2 = corner(a.pos, a.ends + 6 * (size_t)f2 + 4);
+        for (int i = 0; i < 3; i++) {
+            const double e0 = kSsEta[i][0], e1 = kSsEta[i][1], e2 = kSsEta[i][2];
+            p[i][0] = e0 * a0.x + e1 * a1.x + e2 * a2.x;
+            p[i][1] = e0 * a0.y + e1 * a1.y + e2 * a2.y;
+            p[i][2] = e0 * a0.z + e1 * a1.z + e2 * a2.z;
+            q[i][0] = e0 * b0.x + e1 * b1.x + e2 * b2.x;
+            q[i][1] = e0 * b0.y + e1 * b1.y + e2 * b2.y;
+            q[i][2] = e0 * b0.z + e1 * b1.z + e2 * b2.z;
+        }
+    }
+    const double Rmin_2 = Rmin * Rmin, Rmin_4 = Rmin_2 * Rmin_2, Rmin_6 = Rmin_4 * Rmin_2;
+    double fpm[3][3][3];
+    double en = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = k; l < 3; l++) {
+            const double dx = p[k][0] - q[l][0], dy = p[k][1] - q[l][1], dz = p[k][2] - q[l][2];
+            const double mag_r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ri = 1. / mag_r;
+            const double ri2 = ri * ri, ri4 = ri2 * ri2, ri6 = ri4 * ri2, ri7 = ri6 * ri;
+            const double f6 = Rmin_6 * ri6;
+            const double fm = 12 * ri7 * Rmin_6 * Emin * (f6 - 1);
+            const double e = kSsW * kSsW * Emin * f6 * (f6 - 2);
+            en += l == k ? e : 2 * e;
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                fpm[k][l][x] = fm * ((p[k][x] - q[l][x]) / mag_r);
+                fpm[l][k][x] = fpm[k][l][x];
+            }
+        }
+    const double ApAq = a.geom[8 * (size_t)f1 + 6] * a.geom[8 * (size_t)f2 + 6];
+    if (energy) *energy = en * ApAq;
+    if (!acc) return;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        double F[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                if (side == 0) {
+                    const double c = kSsW * kSsW * kSsEta[l][j];
+                    for (int x = 0; x < 3; x++) F[x] += c * fpm[k][l][x];
+                } else {
+                    const double d = kSsW * kSsW * kSsEta[k][j];
+                    for (int x = 0; x < 3; x++) F[x] -= d * fpm[l][k][x];
+                }
+            }
+        for (int x = 0; x < 3; x++) acc[3 * j + x] += F[x] * ApAq;
+    }
+}
+
+// Every step: face f's Face::force as VdW_solver::solve leaves it.  The solve visits pool
+// entries i in order, each against the stacks of its 27 neighbour cells (top down: descending
+// pool index) with i < j; face f therefore first receives, as face 2, the pairs (i, f) for
+// i < f in ascending i (a cell reached through several offsets counts that often), then, as
+// face 1, its own pairs (f, j) in its visiting order.
+__global__ __launch_bounds__(128) void k_ss_forces(SsArgs a) {
+    const int f = blockIdx.x * 128 + threadIdx.x;
+    if (f >= a.P) return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int cf = a.cell[f];
+    int cur[27], lim[27];
+    for (int c = 0; c < 27; c++) {
+        const int cc = ss_neighbour(a, cf, c);
+        cur[c] = a.cstart[cc];
+        lim[c] = a.cend[cc];
+    }
+    for (;;) {  // ascending merge of the 27 ranges (each ascending), while below f
+        int best = a.P, bc = -1;
+        for (int c = 0; c < 27; c++)
+            if (cur[c] < lim[c]) {
+                const int v = a.sorted[cur[c]];
+                if (v < best) {
+                    best = v;
+                    bc = c;
+                }
+            }
+        if (bc < 0 || best >= f) break;
+        cur[bc]++;
+        if (ss_consider(a, best, f)) ss_pair(a, best, f, 1, acc, nullptr);
+    }
+    for (int c = 0; c < 27; c++) {
+        const int cc = ss_neighbour(a, cf, c);
+        for (int s = a.cend[cc] - 1; s >= a.cstart[cc]; s--) {
+            const int j = a.sorted[s];
+            if (j > f && ss_consider(a, f, j)) ss_pair(a, f, j, 0, acc, nullptr);
+        }
+    }
+    for (int k = 0; k < 9; k++) a.out[9 * (size_t)f + k] = acc[k];
+}
+
+// VdW_solver::get_field_energy(-1, -1) (src/VdW_solver.cpp:635-661) of the last solve: the
+// per-blob-pair sums in the solve's visiting order, then over the pairs; one thread.
+__global__ void k_ss_energy(SsArgs a, int nblobs, double *fe, double *total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < nblobs * nblobs; k++) fe[k] = 0.0;
+    for (int i = 0; i < a.P; i++)
+        for (int c = 0; c < 27; c++) {
+            const int cc = ss_neighbour(a, a.cell[i], c);
+            for (int s = a.cend[cc] - 1; s >= a.cstart[cc]; s--) {
+                const int j = a.sorted[s];
+                if (!(i < j) || !ss_consider(a, i, j)) continue;
+                double e;
+                ss_pair(a, i, j, 0, nullptr, &e);
+                fe[a.blob[i] * nblobs + a.blob[j]] += e;
+            }
+        }
+    double t = 0.0;
+    for (int k = 0; k < nblobs * nblobs; k++) t += fe[k];
+    *total = t;
+}

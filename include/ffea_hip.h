@@ -212,6 +212,31 @@ int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_node
                                 const int *set_faces, const double *set_force);
 
 /*
+ * Surface-surface interactions with ssint_type "lennard-jones" (VdW_solver::solve and
+ * do_lj_interaction, src/VdW_solver.cpp:158-384, 663-731; SSINT_matrix, src/LJ_matrix.cpp:122-199;
+ * the lookup grid of NearestNeighbourLinkedListCube, src/NearestNeighbourLinkedListCube.cpp:26-77).
+ * face_nodes [3*num_faces]: the surface faces' corners (N_t ids of the handle, blobs in order,
+ * each blob's faces in .surf order); face_type [num_faces]: the .vdw / .ssint type, -1 for an
+ * inactive face; emin, rmin [num_types^2] in meso units (Emin * area^2 / Energy, Rmin / length),
+ * entry type1 * num_types + type2; grid [3] = es_N_x/y/z and cell_size = ssint_cutoff (meso) of
+ * the lookup grid; inc_self_ssint; es_update (face geometry and grid refreshed on the first step
+ * and every es_update steps, as World::run does).  Every step then starts with the solve: each
+ * pair of active faces in neighbouring cells (i < j in pool order, other blob unless inc_self,
+ * normals not facing the same way) adds its three-point-quadrature forces to both faces, which
+ * reach the nodes with the other face forces after the element forces (src/Blob.cpp:3842-3851).
+ * All faces inactive (or num_faces 0) removes the interaction.  Linear surface forces (CTF 3)
+ * together with it are refused; the steric, ljsteric and gensoft types are not on this path.
+ * A face centroid outside the grid after one period fails the step (FFH_E_ARG), as
+ * LinkedListCube::add_node_to_stack does.
+ * ffh_blob_ssint_energy: VdW_solver::get_field_energy(-1, -1) of the last solve (meso energy),
+ * summed in the solve's order.
+ */
+int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const int *face_type, int num_types,
+                       const double *emin, const double *rmin, const int *grid, double cell_size, int inc_self,
+                       int es_update);
+int ffh_blob_ssint_energy(ffh_blob *b, double *energy);
+
+/*
  * World::run's periodic box (src/World.cpp:1850-1901; box_dim from World::init, :455-471).
  * With periodic_mask != 0, every step (ffh_blob_step) and every ffh_blob_update_internal_forces
  * begins as the reference's step does: for each blob, the centroid of its N_t nodes
