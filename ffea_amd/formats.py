@@ -392,6 +392,14 @@ def write_surface(path, faces):
             f.write(" ".join(str(int(v)) for v in row) + "\n")
 
 
+def write_ssint_types(path, types):
+    """A .vdw file (the format Blob::load_ssint reads): one type per surface face, -1 off."""
+    types = np.asarray(types)
+    with open(path, "w") as f:
+        f.write(f"ffea vdw file\nnum_faces {types.size}\nvdw params:\n")
+        f.write("".join(f"{int(t)}\n" for t in types))
+
+
 def write_blob(prefix, mesh: Mesh):
     write_node(prefix + ".node", mesh.pos)
     write_topology(prefix + ".top", mesh.topo)

@@ -83,13 +83,23 @@ MEAS_COLUMNS = ("Time", "StrainEnergy", "Centroid.x", "Centroid.y", "Centroid.z"
 MEAS_COLUMNS_MASS = ("Time", "KineticEnergy") + MEAS_COLUMNS[1:]
 
 
-def meas_columns(mass_in_system: bool, springs: bool):
-    """World::init's measurement header (src/World.cpp:565-580) for the columns of this path."""
-    return (MEAS_COLUMNS_MASS if mass_in_system else MEAS_COLUMNS) + (("SpringEnergy",) if springs else ())
+def meas_columns(mass_in_system: bool, springs: bool, ssint: bool = False):
+    """World::init's measurement header (src/World.cpp:565-583) for the columns of this path."""
+    return ((MEAS_COLUMNS_MASS if mass_in_system else MEAS_COLUMNS) + (("SpringEnergy",) if springs else ())
+            + (("SurfSurfEnergy",) if ssint else ()))
 
 
-def meas_line(values) -> str:
-    return "".join("%-14.6e" % v for v in values) + "\n"
+def meas_header_line(cols) -> str:
+    return "".join(("%-15s" if c == "SurfSurfEnergy" else "%-14s") % c for c in cols) + "\n"
+
+
+def meas_line(values, ssint: bool = False) -> str:
+    """World::write_measurements_to_file (src/World.cpp:4342-4365): %-14.6e, the surface-surface
+    energy (the last column when present) %-15.6e."""
+    fm = ["%-14.6e"] * len(values)
+    if ssint:
+        fm[-1] = "%-15.6e"
+    return "".join(f % v for f, v in zip(fm, values)) + "\n"
 
 
 def checkpoint_text(states) -> str:
@@ -206,7 +216,7 @@ def place_blobs(params: dict, blobs, positions, return_box: bool = False):
     centre of a box of es_N * ssint_cutoff per axis, es_N = 2 ceil(extent / ssint_cutoff)
     unless given (and an interaction or electrostatics flag is on).  The box (box_dim,
     src/World.cpp:455-471) is set up whether or not the blobs move into it; return_box adds
-    it to the result."""
+    it and es_N (the lookup grid's cells per axis) to the result."""
     out = []
     for b, p in zip(blobs, positions):
         p = np.array(p, np.float64)
@@ -256,7 +266,7 @@ def place_blobs(params: dict, blobs, positions, return_box: bool = False):
     shift = box / 2.0 - wc
     if int(float(params.get("move_into_box", 1))) == 1:
         out = [p + shift for p in out]
-    return (out, box) if return_box else out
+    return (out, box, es) if return_box else out
 
 
 # SimulationParams wall keys (src/SimulationParams.cpp:420-480; PBC by default), in the bit
@@ -322,7 +332,7 @@ class World:
         self.nodes = [m.num_nodes for m in self.meshes]
         # the rest state comes from the positions as loaded; World::init then places the blobs,
         # and pos_0 (the RMSD reference) is taken after that (src/World.cpp:502-506)
-        self.pos0, self.box = place_blobs(self.raw, s["blobs"],
+        self.pos0, self.box, self.es_N = place_blobs(self.raw, s["blobs"],
                                           [linearised(m.pos * sc, m.topo) for m, sc in zip(self.meshes, self.scales)],
                                           return_box=True)
         if self.restart:
@@ -344,6 +354,7 @@ class World:
         self.calc_springs = int(float(self.raw.get("calc_springs", 0))) != 0
         self.calc_ctforces = int(float(self.raw.get("calc_ctforces", 0))) == 1
         self._interactions(s)
+        self._surface_interactions(s)
         self.quiet = quiet
         self.step_initial = 0
         if self.restart:
@@ -401,6 +412,78 @@ class World:
             if sets:
                 self.batch.set_surface_forces(np.concatenate(faces), sets, np.concatenate(set_forces))
 
+    def _surface_interactions(self, s):
+        """Surface-surface interactions as World::init sets them up (src/World.cpp:273-280,
+        896-970; SimulationParams checks, src/SimulationParams.cpp:626-660): the faces of the
+        blobs' .vdw / .ssint files that are active (type != -1), Lennard-Jones on the device.
+        The SurfSurfEnergy column is written whenever calc_ssint or calc_steric is on (the
+        defaults: ljsteric with steric on).  Active faces with the steric, ljsteric or gensoft
+        types, periodic blob corrections (force_pbc) and sticky walls are not on this path."""
+        p = self.raw
+        calc_ssint = int(float(p.get("calc_ssint", p.get("calc_vdw", 1))))
+        calc_steric = int(float(p.get("calc_steric", 1)))
+        kind = p.get("ssint_type", p.get("vdw_type", "ljsteric"))
+        self.ssint_column = calc_ssint == 1 or calc_steric == 1
+        if int(float(p.get("sticky_wall_xz", 0))) == 1:
+            raise F.FormatError("sticky_wall_xz is not on this path")
+        if not self.ssint_column:
+            return
+        if calc_ssint == 1:
+            if kind == "steric":
+                if calc_steric == 0:
+                    raise F.FormatError("ssint_type steric needs calc_steric = 1")
+                calc_ssint = 0
+            elif kind not in ("lennard-jones", "ljsteric", "gensoft"):
+                raise F.FormatError("ssint_type must be lennard-jones, ljsteric or gensoft")
+            elif kind != "lennard-jones" and calc_steric == 0:
+                raise F.FormatError(f"ssint_type {kind} needs calc_steric = 1")
+        fname = None
+        for k in ("vdw_in_fname", "ssint_in_fname", "vdw_forcefield_params", "ssint_forcefield_params", "ljparams",
+                  "lj_params"):
+            if k in p:
+                fname = os.path.normpath(os.path.join(os.path.dirname(self.script), p[k]))
+        if calc_ssint == 1 and fname is None:
+            raise F.FormatError("surface-surface forcefield params file name required (ssint_in_fname)")
+        E = R = None
+        ntypes = 1
+        if calc_ssint == 1:
+            cutoff = float(p.get("ssint_cutoff", p.get("vdw_cutoff", 3e-9)))
+            E, R, _ = F.read_ssint_params(fname, kind, cutoff / U.MESO_LENGTH)
+            ntypes = E.shape[0]
+        off = self.batch.node_off
+        faces, types = [], []
+        for b, blob in enumerate(s["blobs"]):
+            path = blob.get("vdw", blob.get("ssint"))
+            if not path or not blob.get("surface"):
+                raise F.FormatError(f"blob {b}: surface-surface interactions need its surface and vdw files")
+            fc = F.read_surface(blob["surface"])[:, 1:4]
+            ty = F.read_ssint_types(path, fc.shape[0])
+            if calc_ssint == 1:
+                if (ty > ntypes - 1).any():
+                    raise F.FormatError(f"blob {b}: a face type is higher than the forcefield file allows")
+            else:
+                ty = np.where(ty > ntypes - 1, 0, ty)  # steric only: every type is 0 (src/Blob.cpp:3541-3552)
+            faces.append(fc + off[b])
+            types.append(ty)
+        types = np.concatenate(types)
+        if not (types != -1).any():
+            return
+        if calc_ssint != 1:
+            # steric volume exclusion alone (the reference's default with calc_ssint 0): its
+            # forces and energy are zero unless surface elements overlap, and it is not built
+            print("warning: steric surface interactions are not computed on this path (zero unless surface "
+                  "elements overlap); SurfSurfEnergy is written as 0", file=sys.stderr)
+            return
+        if kind != "lennard-jones":
+            raise F.FormatError(f"surface-surface interactions of type {kind} are not on this path "
+                                "(lennard-jones is)")
+        if int(float(p.get("force_pbc", 0))) == 1:
+            raise F.FormatError("force_pbc is not on this path")
+        cutoff = float(p.get("ssint_cutoff", p.get("vdw_cutoff", 3e-9))) / U.MESO_LENGTH
+        inc_self = int(float(p.get("inc_self_ssint", p.get("inc_self_vdw", 1))))
+        es_update = int(float(p.get("es_update", 10)))
+        self.batch.set_ssint(np.concatenate(faces), types, E, R, self.es_N, cutoff, inc_self, es_update)
+
     def _meas_header(self) -> str:
         p = self.raw
         s = "FFEA Global Measurement File\n\nSimulation Details:\n"
@@ -413,8 +496,8 @@ class World:
                 s += f"\t{k} = {p[k]}\n"
         s += "\n\tCalculations enabled:\n"
         s += f"\tcalc_noise = {self.params.calc_noise}\n\tcalc_stokes = {self.params.calc_stokes}\n\n\n"
-        cols = meas_columns(self.mass_in_system, self.calc_springs)
-        s += "Measurements:\n" + "".join("%-14s" % c for c in cols) + "\n"
+        cols = meas_columns(self.mass_in_system, self.calc_springs, self.ssint_column)
+        s += "Measurements:\n" + meas_header_line(cols)
         return s
 
     def _restart_files(self):
@@ -455,8 +538,9 @@ class World:
         cog /= tot
         ke = (self.batch.kinetic_energy() * U.MESO_ENERGY,) if self.mass_in_system else ()
         se = (self.batch.spring_energy() * U.MESO_ENERGY,) if self.calc_springs else ()
+        ss = (self.batch.ssint_energy() * U.MESO_ENERGY,) if self.ssint_column else ()
         return (step * self.dt_si, *ke, self.batch.strain_energy() * U.MESO_ENERGY, *(cog * U.MESO_LENGTH),
-                np.sqrt(msd / tot) * U.MESO_LENGTH, *se), pos
+                np.sqrt(msd / tot) * U.MESO_LENGTH, *se, *ss), pos
 
     def run(self):
         """World::run from step_initial to num_steps inclusive."""
@@ -475,7 +559,7 @@ class World:
                 with open(self.traj_path, "a") as f:
                     f.write(traj_frame(step, [p * U.MESO_LENGTH for p in pos], vel))
                 with open(self.meas_path, "a") as f:
-                    f.write(meas_line(vals))
+                    f.write(meas_line(vals, self.ssint_column))
                 self._positions_phase()
                 step += 1
                 if not self.quiet:

@@ -177,6 +177,7 @@ def cylinder_run(tmp_path, kind):
              num_linear=int(np.unique(z["topo"][:, :4]).size), pinned=z["pinned"])
     F.write_blob(str(d / "cyl_160_fine"), m)
     F.write_surface(str(d / "cyl_160_fine.surf"), z["surface"])
+    F.write_ssint_types(str(d / "cyl_160_fine.vdw"), -np.ones(z["surface"].shape[0], np.int32))  # all off, as there
     run = tmp_path / f"{kind}_cyl_160_fine"
     shutil.copytree(os.path.join(GOLDEN, "physics", f"cyl_160_fine_{kind}"), run)
     w = World(str(run / f"cyl_160_fine-{kind}.ffea"))
@@ -201,3 +202,18 @@ def test_flexural_rigidity_cylinder(tmp_path):
     EI_c, EI_t = F * L ** 3 / (3 * dL), E_t * np.pi * r ** 4 / 4.0
     print(f"flexural rigidity: EI_c/EI_t = {EI_c / EI_t:.5f} over {nf} frames", flush=True)
     assert abs(EI_c / EI_t - 1) <= 0.1
+
+
+def test_cubes_lennard_jones_energy(tmp_path):
+    """The reference's tests/physics/cubes_lj through the World driver: two unit cubes whose top
+    faces face each other 1 nm apart; checkE.py wants the step-0 SurfSurfEnergy within 1 % of
+    the double surface integral -1.7870e-19 J (computed with CUBA)."""
+    from ffea_amd.world import World
+    run = tmp_path / "cubes_lj"
+    shutil.copytree(os.path.join(GOLDEN, "physics", "cubes_lj"), run)
+    w = World(str(run / "cube_veryFine.ffea"))
+    w.run()
+    H = fm_columns(run / "veryFinefine1face1face.fm")
+    e = H["SurfSurfEnergy"][-1]
+    print(f"cubes_lj: SurfSurfEnergy {e:.6e} J, rel err {abs(e / -1.7870e-19 - 1):.4f}", flush=True)
+    assert abs((e - -1.7870e-19) / -1.7870e-19) < 1e-2

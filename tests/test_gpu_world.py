@@ -41,7 +41,7 @@ def test_restart_reproduces_continuous_run(tmp_path):
     assert a[:2] == b[:2], "checkpoint files should be exactly the same"
     m1 = measurements(run / "sphere_63_120_nomass_I_measurement.out")
     m2 = measurements(run / "sphere_63_120_nomass_II_measurement.out")
-    assert m1.shape == m2.shape == (11, 6)
+    assert m1.shape == m2.shape == (11, 7)  # SurfSurfEnergy too: calc_steric is on by default
     with np.errstate(divide="ignore", invalid="ignore"):
         d = np.where(m1 != 0, np.abs((m2 - m1) / m1), 0.0)
     assert d.max() <= 5e-4, d.max()
@@ -84,7 +84,7 @@ def test_mass_restart_reproduces_continuous_run(tmp_path):
     assert a[:2] == b[:2]
     m1 = measurements(run / "sphere_63_120_nomass_I_measurement.out")
     m2 = measurements(run / "sphere_63_120_nomass_II_measurement.out")
-    assert m1.shape == m2.shape == (11, 7)
+    assert m1.shape == m2.shape == (11, 8)  # kinetic energy and SurfSurfEnergy too
     assert m1[1:, 1].min() > 0  # kinetic energy
     with np.errstate(divide="ignore", invalid="ignore"):
         d = np.where(m1 != 0, np.abs((m2 - m1) / m1), 0.0)

@@ -33,8 +33,7 @@ def cubes_world(oracle):
     for b in s["blobs"]:
         meshes.append(F.load_blob(b["nodes"], b["topology"], b["material"], b["stokes"], b.get("pin")))
         scales.append(U.blob_scale(float(b["scale"])))
-    pos, box = place_blobs(s["params"], s["blobs"], [linearised(m.pos * sc, m.topo) for m, sc in zip(meshes, scales)],
-                           return_box=True)
+    pos = place_blobs(s["params"], s["blobs"], [linearised(m.pos * sc, m.topo) for m, sc in zip(meshes, scales)])
     blobs = []
     for b, m, sc, p in zip(s["blobs"], meshes, scales, pos):
         o = oracle.blob(m, prm, sc)

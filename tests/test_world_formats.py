@@ -39,12 +39,14 @@ def test_measurement_lines_match_reference_columns():
     from ffea_amd import world as W
     lines = open(os.path.join(FMT, "tet_ascii.1.fm")).read().split("\n")
     k = lines.index("Measurements:")
-    assert lines[k + 1].startswith("".join("%-14s" % c for c in W.MEAS_COLUMNS))
+    cols = W.meas_columns(False, False, ssint=True)
+    assert lines[k + 1] == W.meas_header_line(cols).rstrip("\n")
     for line in lines[k + 2:]:
         if not line.strip():
             continue
-        vals = [float(v) for v in line.split()[:6]]
-        assert W.meas_line(vals).rstrip("\n") == line[:84]
+        vals = [float(v) for v in line.split()]
+        assert W.meas_line(vals[:6]).rstrip("\n") == line[:84]
+        assert W.meas_line(vals, ssint=True).rstrip("\n") == line
 
 
 def test_checkpoint_round_trip():
