@@ -260,6 +260,147 @@ def massful(ms, prm, sc, xs, steps, warmup, cpu_seconds=0.0):
     return out
 
 
+def boundary_faces(t4, p):
+    """Outward-oriented boundary triangles of a linear tetrahedral mesh (faces of one element)."""
+    f = np.concatenate([t4[:, [1, 2, 3]], t4[:, [0, 3, 2]], t4[:, [0, 1, 3]], t4[:, [0, 2, 1]]])
+    opp = np.concatenate([t4[:, 0], t4[:, 1], t4[:, 2], t4[:, 3]])
+    _, idx, cnt = np.unique(np.sort(f, axis=1), axis=0, return_index=True, return_counts=True)
+    keep = np.sort(idx[cnt == 1])
+    f, opp = f[keep].copy(), opp[keep]
+    n = np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]])
+    flip = (n * (p[opp] - p[f[:, 0]])).sum(axis=1) > 0
+    f[flip, 1], f[flip, 2] = f[flip, 2].copy(), f[flip, 1].copy()
+    return f
+
+
+def lj_pairs(ps, fs, cut, es):
+    """Face pairs the Lennard-Jones solve visits (other blob, neighbour cells with one period,
+    normals not facing the same way), counted with the lookup grid of the first step."""
+    cen = np.concatenate([p[f].mean(axis=1) for p, f in zip(ps, fs)])
+    nrm = np.concatenate([np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]]) for p, f in zip(ps, fs)])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    blob = np.concatenate([np.full(f.shape[0], k) for k, f in enumerate(fs)])
+    es = np.array(es)
+    cell = np.floor(cen / cut).astype(np.int64) % es
+    cid = (cell[:, 0] * es[1] + cell[:, 1]) * es[2] + cell[:, 2]
+    members = {}
+    for i, c in enumerate(cid):
+        members.setdefault(c, []).append(i)
+    pairs = 0
+    for i in range(cen.shape[0]):
+        for d in np.indices((3, 3, 3)).reshape(3, -1).T - 1:
+            c = (cell[i] + d) % es
+            js = np.array(members.get((c[0] * es[1] + c[1]) * es[2] + c[2], []), np.int64)
+            if js.size:
+                js = js[(js > i) & (blob[js] != blob[i])]
+                pairs += int(((nrm[js] @ nrm[i]) <= 0).sum())
+    return pairs
+
+
+def lj_case(n=(48, 8, 48), gap=1.0):
+    """Two Kuhn box blocks, one above the other, their facing 48 x 48 nm sides 1 nm apart, every
+    boundary face active with the cubes_lj forcefield (Emin 2.537e17 J/m^4, Rmin 1 nm), centred
+    in the lookup grid of 3 nm cells World::init would build."""
+    from ffea_amd import mesh as M, units as U
+    sc = U.blob_scale(1e-9)
+    ms = [M.box_mesh(*n), M.box_mesh(*n)]
+    ps = [ms[0].pos * sc, ms[1].pos * sc + np.array([0.0, (n[1] + gap) * sc, 0.0])]
+    cut = 3e-9 / U.MESO_LENGTH
+    allp = np.concatenate(ps)
+    es = [2 * int(np.ceil(d / cut)) for d in allp.max(0) - allp.min(0)]
+    shift = np.array(es) * cut / 2 - allp.mean(0)
+    ps = [p + shift for p in ps]
+    fs = [boundary_faces(m.topo[:, :4], p) for m, p in zip(ms, ps)]
+    E = np.array([[2.537e17 * U.MESO_AREA ** 2 / U.MESO_ENERGY]])
+    R = np.array([[1e-9 / U.MESO_LENGTH]])
+    return ms, sc, ps, fs, E, R, cut, es
+
+
+def ssint_leg(steps, warmup, cpu_seconds=0.0):
+    """Lennard-Jones surface-surface interactions (FP64): steps/s of the two-block workload with
+    and without the interaction, the interaction kernels' times, pair evaluations per second,
+    and the oracle's C restatement timed on a bounded sample of the same steps (port)."""
+    from ffea_amd import units as U
+    from ffea_amd.blob import Batch, package_seed
+    ms, sc, ps, fs, E, R, cut, es = lj_case()
+    prm = U.SimParams.from_si(dt=1e-14, epsilon=1e-6, calc_noise=1, calc_stokes=1, rng_seed=44)
+    nf = [f.shape[0] for f in fs]
+    faces = np.concatenate([fs[0], fs[1] + ms[0].pos.shape[0]])
+    pairs = lj_pairs(ps, fs, cut, es)
+    out = {"workload": f"two {ms[0].num_elements}-element Kuhn blocks 1 nm apart, {sum(nf)} active faces, "
+                       f"grid {es} of {cut * U.MESO_LENGTH * 1e9:.0f} nm cells, es_update 1, noise on, dt 1e-14 s",
+           "faces": sum(nf), "pairs_per_step": pairs}
+    for on in (False, True):
+        g = Batch(ms, prm, [sc, sc], rng_state=package_seed(44))
+        g.set_positions(np.concatenate(ps))
+        if on:
+            g.set_ssint(faces, np.zeros(faces.shape[0], np.int32), E, R, es, cut, 0, 1)
+        g.step(warmup)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        key = "with" if on else "without"
+        out[key] = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps}
+        if on:
+            g.set_profiling(True)
+            g.step(steps)
+            g.synchronize()
+            kt = g.kernel_times()
+            g.set_profiling(False)
+            out["energy_J"] = g.ssint_energy() * U.MESO_ENERGY
+            kf, kg = kt["ssint_forces"], kt["ssint_grid"]
+            avg = kf[0] / kf[1] / 1e3
+            out["kernels"] = {"ssint_forces": {"launches": kf[1], "avg_us": 1e6 * avg},
+                              "ssint_grid": {"launches": kg[1], "avg_us": 1e3 * kg[0] / max(kg[1], 1)}}
+            # each pair is evaluated by both of its faces
+            out["pair_evaluations_per_s"] = 2 * pairs / avg
+        g.close()
+        log(f"[ssint {key}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s")
+    out["interaction_ms_per_step"] = out["with"]["ms_per_step"] - out["without"]["ms_per_step"]
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = ssint_cpu(ms, prm, sc, ps, fs, E, R, cut, es, cpu_seconds)
+    return out
+
+
+def ssint_cpu(ms, prm, sc, ps, fs, E, R, cut, es, max_seconds):
+    """The oracle's world step with and without the interaction on this host (one thread), a
+    bounded sample; the difference is the interaction's time per step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle()
+    per = {}
+    for on in (False, True):
+        o.set_package_seed(o.package_seed(44))
+        s = o.stream()
+        bo = []
+        for m, p, f in zip(ms, ps, fs):
+            b = o.blob(m, prm, sc)
+            b.set_positions(p.ravel())
+            b.set_surface_forces(f, [], np.zeros((0, 3)))
+            b.set_ssint_types(np.zeros(f.shape[0], np.int32))
+            bo.append(b)
+        if on:
+            o.set_ssint(E, R, es, cut, inc_self=0, es_update=1)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            assert pyoracle.world_step(o, bo, s) == 0
+            n += 1
+            el = time.perf_counter() - t0
+            if el > max_seconds / 2 or n >= 20:
+                break
+        o.set_ssint()
+        per[on] = el / n
+        for b in bo:
+            b.close()
+    log(f"[cpu oracle ssint] {per[True]:.3f} s/step with, {per[False]:.3f} without")
+    return {"value": 1.0 / per[True], "unit": "steps/s", "cores": 1, "kind": "port",
+            "interaction_s_per_step": per[True] - per[False],
+            "sample": "oracle (FP64 C restatement) world steps of the same two blocks with and without the "
+                      "interaction, after set-up"}
+
+
 def cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     """Runs _cpu_baseline in a thread with a 2 GiB stack: the reference's SpMV keeps two
     3*N_t-double arrays on the stack (src/SparseMatrixFixedPattern.cpp:153-154), 44 MB at c1."""
@@ -661,6 +802,8 @@ def main():
                     help="noise-on steps of the equipartition check, averaged over the second half (0: skip)")
     ap.add_argument("--massful-steps", type=int, default=100,
                     help="timed steps of each mass-ful solver on the same workload (0: skip)")
+    ap.add_argument("--ssint-steps", type=int, default=50,
+                    help="timed steps of the Lennard-Jones surface-interaction workload (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -702,6 +845,9 @@ def main():
     mass = None
     if a.massful_steps > 0:
         mass = massful(ms, prm, sc, xs, a.massful_steps, a.warmup, 0.0 if a.no_cpu else min(a.cpu_seconds, 15.0))
+    ssint = None
+    if a.ssint_steps > 0:
+        ssint = ssint_leg(a.ssint_steps, a.warmup, 0.0 if a.no_cpu else min(a.cpu_seconds, 20.0))
     out = {
         "metric": f"steps/s at CG eps=1e-6 (FP64 path), {a.config}",
         "value": sps,
@@ -731,6 +877,7 @@ def main():
         "trajectory_vs_reference": traj,
         "equipartition": equi,
         "massful": mass,
+        "ssint": ssint,
     }
     if calib:
         out["copy_calibration"] = calib

@@ -21,9 +21,10 @@ FFH_E_CG = -4
 FFH_E_NOMEM = -5
 FFH_PRECISION_FP64 = 0
 FFH_PRECISION_MIXED = 1
-FFH_NUM_KERNELS = 12
+FFH_NUM_KERNELS = 15
 KERNEL_NAMES = ("element", "assemble", "rhs", "cg_spmv", "cg_update", "integrate", "mpnn_edges",
-                "mpnn_embed", "mpnn_message", "mpnn_dense", "mpnn_out", "cg_totals")
+                "mpnn_embed", "mpnn_message", "mpnn_dense", "mpnn_out", "cg_totals", "ssint_forces", "ssint_grid",
+                "box_wrap")
 
 # Every symbol include/ffea_hip.h declares.
 EXPORTS = (

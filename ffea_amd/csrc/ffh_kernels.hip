@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -159,13 +160,13 @@ struct ffh_blob {
         const int P = a.P, nc = a.ncell;
         hipLaunchKernelGGL(k_ss_tick, 1, 64, 0, stream, a.ctl);
         HIP_OR_FAIL(hipGetLastError());
-        int rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_geom, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
+        int rc = launch(13, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_geom, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
         if (rc) return rc;
         size_t tb = ss_temp_bytes;
         HIP_OR_FAIL(hipcub::DeviceRadixSort::SortPairs(ss_temp, tb, a.cell, a.sorted_cell, a.order, a.sorted, P, 0, ss_bits, stream));
-        rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_clear, grid(nc, 256), 256, 0, stream, e0, e1, 0, a); });
-        if (!rc) rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_bounds, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
-        if (!rc) rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces, grid(P, 128), 128, 0, stream, e0, e1, 0, a); });
+        rc = launch(13, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_clear, grid(nc, 256), 256, 0, stream, e0, e1, 0, a); });
+        if (!rc) rc = launch(13, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_bounds, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
+        if (!rc) rc = launch(12, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces, P, 64, 0, stream, e0, e1, 0, a); });
         return rc;
     }
     int check_ssint() {
@@ -979,9 +980,9 @@ struct ffh_blob {
         WrapArgs w = wrap;
         w.pos = pos;
         const int nc = wrap_chunks, nb = nblobs;
-        int rc = launch(5, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_partials, nc, 256, 0, stream, e0, e1, 0, w); });
-        if (!rc) rc = launch(5, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_decide, (nb + 63) / 64, 64, 0, stream, e0, e1, 0, w, nb); });
-        if (!rc) rc = launch(5, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_move, grid(N, 256), 256, 0, stream, e0, e1, 0, w); });
+        int rc = launch(14, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_partials, nc, 256, 0, stream, e0, e1, 0, w); });
+        if (!rc) rc = launch(14, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_decide, (nb + 63) / 64, 64, 0, stream, e0, e1, 0, w, nb); });
+        if (!rc) rc = launch(14, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_wrap_move, grid(N, 256), 256, 0, stream, e0, e1, 0, w); });
         return rc;
     }
     int reset_inversions() {  // a kernel, not a memset: it is also recorded into step graphs

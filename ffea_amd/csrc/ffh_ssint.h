@@ -182,15 +182,10 @@ __device__ void ss_pair(const SsArgs &a, int f1, int f2, int side, double *acc, 
     }
 }
 
-// Every step: face f's Face::force as VdW_solver::solve leaves it.  The solve visits pool
-// entries i in order, each against the stacks of its 27 neighbour cells (top down: descending
-// pool index) with i < j; face f therefore first receives, as face 2, the pairs (i, f) for
-// i < f in ascending i (a cell reached through several offsets counts that often), then, as
-// face 1, its own pairs (f, j) in its visiting order.
-__global__ __launch_bounds__(128) void k_ss_forces(SsArgs a) {
-    const int f = blockIdx.x * 128 + threadIdx.x;
-    if (f >= a.P) return;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+// Face f's pairs as face 2, (i, f) for i < f in ascending i (a cell reached through several
+// offsets counts that often), summed into acc by one thread: a merge of the 27 neighbour
+// ranges.  The fallback of k_ss_forces for faces with more candidates than its LDS list holds.
+__device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
     const int cf = a.cell[f];
     int cur[27], lim[27];
     for (int c = 0; c < 27; c++) {
@@ -198,7 +193,7 @@ __global__ __launch_bounds__(128) void k_ss_forces(SsArgs a) {
         cur[c] = a.cstart[cc];
         lim[c] = a.cend[cc];
     }
-    for (;;) {  // ascending merge of the 27 ranges (each ascending), while below f
+    for (;;) {
         int best = a.P, bc = -1;
         for (int c = 0; c < 27; c++)
             if (cur[c] < lim[c]) {
@@ -212,14 +207,130 @@ __global__ __launch_bounds__(128) void k_ss_forces(SsArgs a) {
         cur[bc]++;
         if (ss_consider(a, best, f)) ss_pair(a, best, f, 1, acc, nullptr);
     }
-    for (int c = 0; c < 27; c++) {
-        const int cc = ss_neighbour(a, cf, c);
-        for (int s = a.cend[cc] - 1; s >= a.cstart[cc]; s--) {
-            const int j = a.sorted[s];
-            if (j > f && ss_consider(a, f, j)) ss_pair(a, f, j, 0, acc, nullptr);
+}
+
+constexpr int kSsCap = 1024;  // candidates below f a wave sorts in LDS
+
+// Every step: face f's Face::force as VdW_solver::solve leaves it, one wavefront per face.
+// The solve visits pool entries i in order, each against the stacks of its 27 neighbour cells
+// (top down: descending pool index) with i < j; face f therefore first receives, as face 2,
+// the pairs (i, f) for i < f in ascending i, then, as face 1, its own pairs (f, j) in its
+// visiting order.  The lanes evaluate 64 pairs at a time; the contributions then go through
+// LDS and are added one by one in that order by lanes 0..8 (one force component each), so
+// every sum has the reference's order.
+__global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
+    __shared__ int rs[27], re[27], pre[28];
+    __shared__ int cand[kSsCap];
+    __shared__ int ncand;
+    __shared__ double con[64][9];
+    __shared__ int use[64];
+    const int f = blockIdx.x, lane = threadIdx.x;
+    if (f >= a.P) return;
+    if (lane < 27) {
+        const int cc = ss_neighbour(a, a.cell[f], lane);
+        rs[lane] = a.cstart[cc];
+        re[lane] = a.cend[cc];
+    }
+    if (lane == 0) ncand = 0;
+    __syncthreads();
+    if (lane == 0) {
+        int t = 0;
+        for (int c = 0; c < 27; c++) {
+            pre[c] = t;
+            t += re[c] - rs[c];
+        }
+        pre[27] = t;
+    }
+    __syncthreads();
+    const int n = pre[27];
+    double acc = 0.0;  // lane k < 9: component k of the face force (corner k / 3, axis k % 3)
+    // the candidate at visiting position t: cell c, k-th from the top of its stack
+    auto visit = [&](int t) {
+        int c = 0;
+        while (pre[c + 1] <= t) c++;
+        return a.sorted[re[c] - 1 - (t - pre[c])];
+    };
+    // (a) as face 2: the candidates below f, sorted ascending
+    for (int t = lane; t < n; t += 64) {
+        const int i = visit(t);
+        if (i < f) {
+            const int slot = atomicAdd(&ncand, 1);
+            if (slot < kSsCap) cand[slot] = i;
         }
     }
-    for (int k = 0; k < 9; k++) a.out[9 * (size_t)f + k] = acc[k];
+    __syncthreads();
+    const int na = ncand;
+    if (na > kSsCap) {  // too many to sort here: one lane merges
+        double acc9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (lane == 0) {
+            ss_face2_serial(a, f, acc9);
+            for (int k = 0; k < 9; k++) con[0][k] = acc9[k];
+        }
+        __syncthreads();
+        if (lane < 9) acc = con[0][lane];
+        __syncthreads();
+    } else {
+        int m = 1;
+        while (m < na) m <<= 1;
+        for (int t = na + lane; t < m; t += 64) cand[t] = INT_MAX;
+        __syncthreads();
+        for (int k = 2; k <= m; k <<= 1)  // bitonic sort, ascending
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = lane; t < m; t += 64) {
+                    const int u = t ^ j;
+                    if (u > t) {
+                        const int x = cand[t], y = cand[u];
+                        const bool up = (t & k) == 0;
+                        if ((x > y) == up) {
+                            cand[t] = y;
+                            cand[u] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int base = 0; base < na; base += 64) {
+            const int t = base + lane;
+            int ok = 0;
+            double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (t < na && ss_consider(a, cand[t], f)) {
+                ss_pair(a, cand[t], f, 1, c9, nullptr);
+                ok = 1;
+            }
+            use[lane] = ok;
+            for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
+            __syncthreads();
+            if (lane < 9) {
+                const int e = min(64, na - base);
+                for (int q = 0; q < e; q++)
+                    if (use[q]) acc += con[q][lane];
+            }
+            __syncthreads();
+        }
+    }
+    // (b) as face 1: its own pairs in visiting order
+    for (int base = 0; base < n; base += 64) {
+        const int t = base + lane;
+        int ok = 0;
+        double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (t < n) {
+            const int j = visit(t);
+            if (j > f && ss_consider(a, f, j)) {
+                ss_pair(a, f, j, 0, c9, nullptr);
+                ok = 1;
+            }
+        }
+        use[lane] = ok;
+        for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
+        __syncthreads();
+        if (lane < 9) {
+            const int e = min(64, n - base);
+            for (int q = 0; q < e; q++)
+                if (use[q]) acc += con[q][lane];
+        }
+        __syncthreads();
+    }
+    if (lane < 9) a.out[9 * (size_t)f + lane] = acc;
 }
 
 // VdW_solver::get_field_energy(-1, -1) (src/VdW_solver.cpp:635-661) of the last solve: the

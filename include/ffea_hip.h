@@ -304,12 +304,14 @@ int ffh_blob_synchronize(ffh_blob *b);
  * bracketed by events; ffh_blob_kernel_times returns, for each kernel id, the summed
  * milliseconds and launch counts since the last reset.  Kernel ids:
  *   0 element   1 assemble   2 rhs+CG init   3 CG spmv (p update, q = K p, p.q)
- *   4 CG update (x, r, residual)   5 integrate (and the periodic box wrap)
+ *   4 CG update (x, r, residual)   5 integrate
  *   learned preconditioner: 6 couplings A_ij   7 embedding   8 message (A h)
  *   9 dense layer (MFMA)   10 output layer
  *   11 per-blob totals of the CG reductions (blobs of more than 3072 64-row slices)
+ *   12 surface-surface interaction forces   13 their face geometry and lookup grid
+ *   14 periodic box wrap
  */
-#define FFH_NUM_KERNELS 12
+#define FFH_NUM_KERNELS 15
 int ffh_blob_set_profiling(ffh_blob *b, int enable);
 int ffh_blob_kernel_times(ffh_blob *b, double *ms, long long *counts);
 
