@@ -140,9 +140,19 @@ struct ffh_blob {
     int ss_bits = 1;
     int *ss_err_host = nullptr;
     double *ss_fe = nullptr, *ss_total = nullptr;
+    // the energy's pair list: per face count and offset, the pairs' energies and blob pairs
+    int *ss_cnt = nullptr, *ss_off = nullptr, *ss_pb = nullptr;
+    double *ss_pe = nullptr;
+    size_t ss_pair_cap = 0;
+    void *ss_scan_temp = nullptr;
+    size_t ss_scan_bytes = 0;
     void free_ssint() {
         for (void *q : ss_allocs) (void)hipFree(q);
         ss_allocs.clear();
+        for (void *q : {(void *)ss_pb, (void *)ss_pe}) (void)hipFree(q);
+        ss_pb = nullptr;
+        ss_pe = nullptr;
+        ss_pair_cap = 0;
         if (ss_err_host) (void)hipHostFree(ss_err_host);
         ss_err_host = nullptr;
         ss_on = false;
@@ -1749,6 +1759,13 @@ int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const 
     if (!rc) rc = put(&c0, sizeof(c0), (void **)&a.ctl);
     if (!rc) rc = put(nullptr, sizeof(double) * (size_t)b->nblobs * b->nblobs, (void **)&b->ss_fe);
     if (!rc) rc = put(nullptr, sizeof(double), (void **)&b->ss_total);
+    if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&b->ss_cnt);
+    if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&b->ss_off);
+    size_t sb = 0;
+    if (!rc && hipcub::DeviceScan::ExclusiveSum(nullptr, sb, b->ss_cnt, b->ss_off, P, b->stream) != hipSuccess)
+        rc = fail(FFH_E_HIP, "scan set-up");
+    if (!rc) rc = put(nullptr, sb, &b->ss_scan_temp);
+    b->ss_scan_bytes = sb;
     size_t tb = 0;
     if (!rc && hipcub::DeviceRadixSort::SortPairs(nullptr, tb, a.cell, a.sorted_cell, a.order, a.sorted, P, 0, bits,
                                                   b->stream) != hipSuccess)
@@ -1790,7 +1807,32 @@ int ffh_blob_ssint_energy(ffh_blob *b, double *energy) {
     }
     SsArgs a = b->ss;
     a.pos = b->pos;
-    hipLaunchKernelGGL(k_ss_energy, 1, 64, 0, b->stream, a, b->nblobs, b->ss_fe, b->ss_total);
+    const int P = a.P;
+    hipLaunchKernelGGL(k_ss_pair_count, P, 64, 0, b->stream, a, b->ss_cnt);
+    HIP_OR_FAIL(hipGetLastError());
+    size_t sb = b->ss_scan_bytes;
+    HIP_OR_FAIL(hipcub::DeviceScan::ExclusiveSum(b->ss_scan_temp, sb, b->ss_cnt, b->ss_off, P, b->stream));
+    int last[2];
+    HIP_OR_FAIL(hipMemcpyAsync(&last[0], b->ss_off + P - 1, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(&last[1], b->ss_cnt + P - 1, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    const size_t npairs = (size_t)last[0] + last[1];
+    if (npairs > b->ss_pair_cap) {
+        (void)hipFree(b->ss_pe);
+        (void)hipFree(b->ss_pb);
+        b->ss_pe = nullptr;
+        b->ss_pb = nullptr;
+        b->ss_pair_cap = 0;
+        const size_t cap = npairs + npairs / 4 + 1024;
+        if (hipMalloc((void **)&b->ss_pe, cap * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&b->ss_pb, cap * sizeof(int)) != hipSuccess)
+            return fail(FFH_E_NOMEM, "hipMalloc (interaction pair energies)");
+        b->ss_pair_cap = cap;
+    }
+    hipLaunchKernelGGL(k_ss_pair_energy, P, 64, 0, b->stream, a, b->nblobs, (const int *)b->ss_off, b->ss_pe, b->ss_pb);
+    HIP_OR_FAIL(hipGetLastError());
+    hipLaunchKernelGGL(k_ss_energy_sum, 1, 64, 0, b->stream, (int)npairs, (const double *)b->ss_pe,
+                       (const int *)b->ss_pb, b->nblobs, b->ss_fe, b->ss_total);
     HIP_OR_FAIL(hipGetLastError());
     HIP_OR_FAIL(hipMemcpyAsync(energy, b->ss_total, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));

@@ -333,23 +333,120 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     if (lane < 9) a.out[9 * (size_t)f + lane] = acc;
 }
 
-// VdW_solver::get_field_energy(-1, -1) (src/VdW_solver.cpp:635-661) of the last solve: the
-// per-blob-pair sums in the solve's visiting order, then over the pairs; one thread.
-__global__ void k_ss_energy(SsArgs a, int nblobs, double *fe, double *total) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int k = 0; k < nblobs * nblobs; k++) fe[k] = 0.0;
-    for (int i = 0; i < a.P; i++)
+// The energy of the last solve (VdW_solver::get_field_energy(-1, -1),
+// src/VdW_solver.cpp:635-661): fieldenergy[b1][b2] is one running sum per blob pair over the
+// pairs in the solve's visiting order (i ascending, then its 27 cells, stacks top down), then
+// the pairs' sums are added row by row.  k_ss_pair_count counts each face's own pairs (one
+// wavefront per face), k_ss_pair_energy writes their energies in visiting order at the face's
+// offset, and k_ss_energy_sum adds them up in that order on one thread.
+__device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const int *re, int t) {
+    int c = 0;
+    while (pre[c + 1] <= t) c++;
+    return a.sorted[re[c] - 1 - (t - pre[c])];
+}
+
+__device__ __forceinline__ void ss_ranges(const SsArgs &a, int f, int lane, int *rs, int *re, int *pre) {
+    if (lane < 27) {
+        const int cc = ss_neighbour(a, a.cell[f], lane);
+        rs[lane] = a.cstart[cc];
+        re[lane] = a.cend[cc];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int t = 0;
         for (int c = 0; c < 27; c++) {
-            const int cc = ss_neighbour(a, a.cell[i], c);
-            for (int s = a.cend[cc] - 1; s >= a.cstart[cc]; s--) {
-                const int j = a.sorted[s];
-                if (!(i < j) || !ss_consider(a, i, j)) continue;
-                double e;
-                ss_pair(a, i, j, 0, nullptr, &e);
-                fe[a.blob[i] * nblobs + a.blob[j]] += e;
+            pre[c] = t;
+            t += re[c] - rs[c];
+        }
+        pre[27] = t;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_ss_pair_count(SsArgs a, int *cnt) {
+    __shared__ int rs[27], re[27], pre[28];
+    const int f = blockIdx.x, lane = threadIdx.x;
+    if (f >= a.P) return;
+    ss_ranges(a, f, lane, rs, re, pre);
+    const int n = pre[27];
+    int k = 0;
+    for (int t = lane; t < n; t += 64) {
+        const int j = ss_visit(a, pre, re, t);
+        k += j > f && ss_consider(a, f, j);
+    }
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o);
+    if (lane == 0) cnt[f] = k;
+}
+
+__global__ __launch_bounds__(64) void k_ss_pair_energy(SsArgs a, int nblobs, const int *off, double *pe, int *pb) {
+    __shared__ int rs[27], re[27], pre[28];
+    const int f = blockIdx.x, lane = threadIdx.x;
+    if (f >= a.P) return;
+    ss_ranges(a, f, lane, rs, re, pre);
+    const int n = pre[27];
+    int w = off[f];
+    for (int base = 0; base < n; base += 64) {
+        const int t = base + lane;
+        int j = -1;
+        bool ok = false;
+        if (t < n) {
+            j = ss_visit(a, pre, re, t);
+            ok = j > f && ss_consider(a, f, j);
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
+            const int r = __popcll(m & ((1ull << lane) - 1ull));
+            double e;
+            ss_pair(a, f, j, 0, nullptr, &e);
+            pe[w + r] = e;
+            pb[w + r] = a.blob[f] * nblobs + a.blob[j];
+        }
+        w += __popcll(m);
+    }
+}
+
+// One wavefront: the lanes stage 64 pairs at a time in LDS, lane 0 adds them in order.  A
+// blob pair's running sum stays in a register while consecutive pairs share it and is written
+// back when the blob pair changes -- the same additions as fieldenergy[b1][b2] += e.
+constexpr int kSsFeLds = 4096;  // blob pairs kept in LDS (64 blobs); more stay in global memory
+__global__ __launch_bounds__(64) void k_ss_energy_sum(int npairs, const double *pe, const int *pb, int nblobs,
+                                                       double *fe, double *total) {
+    __shared__ double fl[kSsFeLds];
+    __shared__ double ce[64];
+    __shared__ int cb[64];
+    const int lane = threadIdx.x, nf = nblobs * nblobs;
+    const bool in_lds = nf <= kSsFeLds;
+    double *f = in_lds ? fl : fe;
+    for (int k = lane; k < nf; k += 64) f[k] = 0.0;
+    __syncthreads();
+    int key = -1;
+    double run = 0.0;
+    for (int base = 0; base < npairs; base += 64) {
+        if (base + lane < npairs) {
+            ce[lane] = pe[base + lane];
+            cb[lane] = pb[base + lane];
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const int e = min(64, npairs - base);
+            for (int q = 0; q < e; q++) {
+                if (cb[q] != key) {
+                    if (key >= 0) f[key] = run;
+                    key = cb[q];
+                    run = f[key];
+                }
+                run += ce[q];
             }
         }
-    double t = 0.0;
-    for (int k = 0; k < nblobs * nblobs; k++) t += fe[k];
-    *total = t;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        if (key >= 0) f[key] = run;
+        double t = 0.0;
+        for (int k = 0; k < nf; k++) t += f[k];
+        *total = t;
+    }
+    __syncthreads();
+    if (in_lds)
+        for (int k = lane; k < nf; k += 64) fe[k] = fl[k];
 }
