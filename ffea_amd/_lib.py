@@ -48,6 +48,8 @@ FFH_PRECOND_JACOBI = 0
 FFH_PRECOND_LEARNED = 1
 # solver codes of include/FFEA_return_codes.h:30-33, by .ffea "solver" value
 FFH_SOLVERS = {"CG": 1, "masslumped": 2, "CG_nomass": 3}
+# surface-surface interaction types on the device, by .ffea ssint_type value
+FFH_SSINT_KINDS = {"lennard-jones": 1, "ljsteric": 2, "gensoft": 3}
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -137,8 +139,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_set_constant_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     L.ffh_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
     L.ffh_blob_set_box.argtypes = [C.c_void_p, _dp, C.c_int]
-    L.ffh_blob_set_ssint.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _dp, _dp, _ip, C.c_double, C.c_int,
-                                     C.c_int]
+    L.ffh_blob_set_ssint.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int, _dp, _dp, _dp, _ip,
+                                     C.c_double, C.c_int, C.c_int]
     L.ffh_blob_ssint_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
     for n in EXPORTS:

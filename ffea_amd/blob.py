@@ -162,26 +162,34 @@ class Blob:
         _lib.check(self._L.ffh_blob_set_rotational_forces(self._h, n, nd, fr, ax, tq))
 
     def set_ssint(self, faces=None, types=None, emin=None, rmin=None, grid=(1, 1, 1), cell_size=1.0,
-                  inc_self: int = 1, es_update: int = 1) -> None:
-        """Lennard-Jones surface-surface interactions: faces [F, 3] (N_t ids), types [F] (-1
-        inactive), emin / rmin [ntypes, ntypes] meso, the lookup grid (es_N per axis, cell size
-        = ssint_cutoff in meso), inc_self_ssint and es_update.  faces None removes them."""
+                  inc_self: int = 1, es_update: int = 1, kind: str = "lennard-jones", opposite=None,
+                  k0=None) -> None:
+        """Surface-surface interactions: faces [F, 3] (N_t ids), types [F] (-1 inactive),
+        emin / rmin (/ k0) [ntypes, ntypes] meso, the lookup grid (es_N per axis, cell size =
+        ssint_cutoff in meso), inc_self_ssint, es_update, the type ("lennard-jones", "ljsteric",
+        "gensoft") and each face's opposite element vertex (the latter two types).  faces None
+        removes them."""
         if faces is None:
-            _lib.check(self._L.ffh_blob_set_ssint(self._h, 0, np.zeros(3, np.int32), np.zeros(1, np.int32), 0,
-                                                  np.zeros(1), np.zeros(1), np.ones(3, np.int32), 1.0, 0, 1))
+            z = np.zeros(1)
+            _lib.check(self._L.ffh_blob_set_ssint(self._h, 1, 0, np.zeros(3, np.int32), np.zeros(1, np.int32),
+                                                  np.zeros(1, np.int32), 0, z, z, z, np.ones(3, np.int32), 1.0, 0, 1))
             return
         fc = np.ascontiguousarray(faces, np.int32).ravel()
         ty = np.ascontiguousarray(types, np.int32).ravel()
         if fc.size != 3 * ty.size:
             raise ValueError("one type per face")
+        op = np.full(ty.size, -1, np.int32) if opposite is None else np.ascontiguousarray(opposite, np.int32).ravel()
         e = np.ascontiguousarray(emin, np.float64)
         r = np.ascontiguousarray(rmin, np.float64)
-        if e.ndim != 2 or e.shape[0] != e.shape[1] or r.shape != e.shape:
-            raise ValueError("emin and rmin are square tables of one size")
-        _lib.check(self._L.ffh_blob_set_ssint(self._h, ty.size, fc if fc.size else np.zeros(3, np.int32),
+        k = np.zeros_like(e) if k0 is None else np.ascontiguousarray(k0, np.float64)
+        if e.ndim != 2 or e.shape[0] != e.shape[1] or r.shape != e.shape or k.shape != e.shape:
+            raise ValueError("emin, rmin and k0 are square tables of one size")
+        _lib.check(self._L.ffh_blob_set_ssint(self._h, _lib.FFH_SSINT_KINDS[kind], ty.size,
+                                              fc if fc.size else np.zeros(3, np.int32),
+                                              op if op.size else np.zeros(1, np.int32),
                                               ty if ty.size else np.zeros(1, np.int32), e.shape[0], e.ravel(),
-                                              r.ravel(), np.ascontiguousarray(grid, np.int32), float(cell_size),
-                                              int(inc_self), int(es_update)))
+                                              r.ravel(), k.ravel(), np.ascontiguousarray(grid, np.int32),
+                                              float(cell_size), int(inc_self), int(es_update)))
 
     def ssint_energy(self) -> float:
         """The surface-surface interaction energy of the last step's solve (meso units)."""

@@ -1671,21 +1671,25 @@ int ffh_blob_set_rotational_forces(ffh_blob *b, int num_forces, const int *node,
     return rc;
 }
 
-int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const int *face_type, int num_types,
-                       const double *emin, const double *rmin, const int *grid, double cell_size, int inc_self,
-                       int es_update) {
+int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nodes, const int *face_opposite,
+                       const int *face_type, int num_types, const double *emin, const double *rmin, const double *k0,
+                       const int *grid, double cell_size, int inc_self, int es_update) {
     if (!b || num_faces < 0 || (num_faces > 0 && (!face_nodes || !face_type))) return fail(FFH_E_ARG, "null argument");
+    if (kind < FFH_SSINT_LENNARD_JONES || kind > FFH_SSINT_GENSOFT) return fail(FFH_E_ARG, "unknown interaction type");
+    if (kind != FFH_SSINT_LENNARD_JONES && num_faces > 0 && !face_opposite) return fail(FFH_E_ARG, "null argument");
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the step gains or loses the interaction
     b->graphs.clear();
     b->free_ssint();
     ffh::ForceInput keep = b->fin;
-    std::vector<int32_t> pool_nodes, pool_type;
+    std::vector<int32_t> pool_nodes, pool_type, pool_nodes4;
     for (int f = 0; f < num_faces; f++) {
         if (face_type[f] == -1) continue;
         if (face_type[f] < 0 || face_type[f] >= num_types)
             return fail(FFH_E_ARG, "face " + std::to_string(f) + ": interaction type out of range");
         for (int k = 0; k < 3; k++) pool_nodes.push_back(face_nodes[3 * f + k]);
+        for (int k = 0; k < 3; k++) pool_nodes4.push_back(face_nodes[3 * f + k]);
+        pool_nodes4.push_back(face_opposite ? face_opposite[f] : -1);
         pool_type.push_back(face_type[f]);
     }
     const int P = (int)pool_type.size();
@@ -1698,7 +1702,7 @@ int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const 
         }
         return rc;
     }
-    if (!emin || !rmin || !grid) return fail(FFH_E_ARG, "null argument");
+    if (!emin || !rmin || !grid || !k0) return fail(FFH_E_ARG, "null argument");
     if (grid[0] < 1 || grid[1] < 1 || grid[2] < 1 || !(cell_size > 0) || es_update < 1 ||
         (long long)grid[0] * grid[1] * grid[2] > (1 << 28))
         return fail(FFH_E_ARG, "invalid lookup grid or es_update");
@@ -1749,6 +1753,8 @@ int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const 
     if (!rc) rc = put(pool_type.data(), pool_type.size() * 4, (void **)&a.type);
     if (!rc) rc = put(emin, sizeof(double) * num_types * num_types, (void **)&a.emin);
     if (!rc) rc = put(rmin, sizeof(double) * num_types * num_types, (void **)&a.rmin);
+    if (!rc) rc = put(k0, sizeof(double) * num_types * num_types, (void **)&a.k0);
+    if (!rc) rc = put(pool_nodes4.data(), pool_nodes4.size() * 4, (void **)&a.nodes4);
     if (!rc) rc = put(nullptr, 8 * sizeof(double) * (size_t)P, (void **)&a.geom);
     if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&a.cell);
     if (!rc) rc = put(order.data(), 4 * (size_t)P, (void **)&a.order);
@@ -1785,6 +1791,7 @@ int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const 
     *b->ss_err_host = 0;
     a.err = err_dev;
     a.ntypes = num_types;
+    a.kind = kind;
     a.nx = grid[0];
     a.ny = grid[1];
     a.nz = grid[2];

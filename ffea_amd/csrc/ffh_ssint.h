@@ -1,7 +1,8 @@
-// ffh_ssint.h -- surface-surface Lennard-Jones interactions (VdW_solver with ssint_type
-// "lennard-jones", src/VdW_solver.cpp:158-384, 663-731): face geometry and the lookup grid at
-// es_update steps, and every step each active face's Face::force summed in the reference's
-// order.
+// ffh_ssint.h -- surface-surface interactions (VdW_solver, src/VdW_solver.cpp:158-384,
+// 663-831): the lennard-jones type, ljsteric's Lennard-Jones with its soft core below Rmin, and
+// gensoft; face geometry and the lookup grid at es_update steps, and every step each active
+// face's Face::force summed in the reference's order.  The steric volume exclusion that
+// ljsteric and gensoft apply instead where face elements overlap is not built.
 // Part of ffh_kernels.hip: included inside its anonymous namespace, after the headers it
 // lists there; not a stand-alone header.
 #pragma once
@@ -24,8 +25,10 @@ struct SsArgs {
     const int32_t *ends;   // [6P] each corner's edge ends (compact ids; a vertex twice)
     const int32_t *blob;   // [P]
     const int32_t *type;   // [P]
-    const double *emin, *rmin;  // [ntypes^2] meso, entry type1 * ntypes + type2
+    const double *emin, *rmin, *k0;  // [ntypes^2] meso, entry type1 * ntypes + type2
     int ntypes;
+    int kind;              // 1 lennard-jones, 2 ljsteric (its soft core below Rmin), 3 gensoft
+    const int32_t *nodes4; // [4P] corners and opposite element vertex (N_t ids), for kinds 2 and 3
     double *geom;          // [8P] centroid, unit normal, area (Face::calc_area_normal_centroid)
     int32_t *cell;         // [P] lookup-grid cell of each face at the last refresh
     int32_t *order;        // [P] 0..P-1, the sort's values
@@ -109,12 +112,33 @@ __global__ __launch_bounds__(256) void k_ss_bounds(SsArgs a) {
     if (s == a.P - 1 || a.sorted_cell[s + 1] != c) a.cend[c] = s + 1;
 }
 
-// consider_interaction for the Lennard-Jones type (src/VdW_solver.cpp:332-352): other blob
-// unless inc_self_ssint, and normals whose dot product is not above zero.
+// consider_interaction (src/VdW_solver.cpp:332-384): other blob unless inc_self_ssint, normals
+// whose dot product is not above zero; for the types other than lennard-jones also faces in
+// front of each other (sep from centroid i to centroid j, arr3arr3DotProduct: sep.n_i < 0 with
+// sep.n_j > 0 rejects) and, within one blob, no shared node among corners and opposite vertex.
 __device__ __forceinline__ bool ss_consider(const SsArgs &a, int i, int j) {
-    if (!a.inc_self && a.blob[i] == a.blob[j]) return false;
-    const double *ni = a.geom + 8 * (size_t)i + 3, *nj = a.geom + 8 * (size_t)j + 3;
-    return !(ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2] > 0.0);
+    const bool same = a.blob[i] == a.blob[j];
+    if (!a.inc_self && same) return false;
+    const double *gi = a.geom + 8 * (size_t)i, *gj = a.geom + 8 * (size_t)j;
+    if (gi[3] * gj[3] + gi[4] * gj[4] + gi[5] * gj[5] > 0.0) return false;
+    if (a.kind == 1) return true;
+    const double sx = gj[0] - gi[0], sy = gj[1] - gi[1], sz = gj[2] - gi[2];
+    double d1 = 0.0, d2 = 0.0;
+    d1 += sx * gi[3];
+    d1 += sy * gi[4];
+    d1 += sz * gi[5];
+    d2 += sx * gj[3];
+    d2 += sy * gj[4];
+    d2 += sz * gj[5];
+    if (d1 < 0 && d2 > 0) return false;
+    if (a.inc_self && same) {
+        const int32_t *ni = a.nodes4 + 4 * (size_t)i, *nj = a.nodes4 + 4 * (size_t)j;
+        if (ni[3] == nj[3]) return false;
+        for (int p = 0; p < 4; p++)
+            for (int q = 0; q < 4; q++)
+                if (nj[q] == ni[p]) return false;
+    }
+    return true;
 }
 
 // do_lj_interaction (src/VdW_solver.cpp:236-324) for faces f1 < f2 with
@@ -122,7 +146,7 @@ __device__ __forceinline__ bool ss_consider(const SsArgs &a, int i, int j) {
 // to acc, side 1 face 2's; the pair's energy (scaled by both areas) goes to *energy if given.
 __device__ void ss_pair(const SsArgs &a, int f1, int f2, int side, double *acc, double *energy) {
     const int t = a.type[f1] * a.ntypes + a.type[f2];
-    const double Emin = a.emin[t], Rmin = a.rmin[t];
+    const double Emin = a.emin[t], Rmin = a.rmin[t], k0 = a.k0[t], Rmini = 1.0 / Rmin;
     double p[3][3], q[3][3];
     {
         const double4 a0 = corner(a.pos, a.ends + 6 * (size_t)f1), a1 = corner(a.pos, a.ends + 6 * (size_t)f1 + 2),
@@ -148,11 +172,25 @@ __device__ void ss_pair(const SsArgs &a, int f1, int f2, int side, double *acc, 
         for (int l = k; l < 3; l++) {
             const double dx = p[k][0] - q[l][0], dy = p[k][1] - q[l][1], dz = p[k][2] - q[l][2];
             const double mag_r = sqrt(dx * dx + dy * dy + dz * dz);
-            const double ri = 1. / mag_r;
-            const double ri2 = ri * ri, ri4 = ri2 * ri2, ri6 = ri4 * ri2, ri7 = ri6 * ri;
-            const double f6 = Rmin_6 * ri6;
-            const double fm = 12 * ri7 * Rmin_6 * Emin * (f6 - 1);
-            const double e = kSsW * kSsW * Emin * f6 * (f6 - 2);
+            double fm, e;
+            if (a.kind == 3) {  // calc_gensoft_factors (src/VdW_solver.cpp:754-771)
+                const double r2 = mag_r * mag_r, R2 = Rmin * Rmin;
+                const double g2 = r2 / R2, g3 = g2 * mag_r / Rmin;
+                const double k0rm = k0 * Rmin, k0rm2 = k0rm * Rmin, eps = Emin / Rmin;
+                fm = 2 * (k0rm - 6 * eps) * g3 + 3 * (4 * eps - k0rm) * g2 + k0 * mag_r;
+                const double emag = (0.5 * k0rm2 - 3 * Emin) * g2 * g2 + (4 * Emin - k0rm2) * g3 + 0.5 * k0 * r2 - Emin;
+                e = kSsW * kSsW * emag;
+            } else if (a.kind == 2 && mag_r < Rmin) {  // calc_ljinterpolated_factors (:774-784)
+                const double x = mag_r * Rmini, x2 = x * x;
+                e = kSsW * kSsW * Emin * x2 * (2 * x - 3);
+                fm = 6 * Emin * Rmini * x * (1 - x);
+            } else {  // calc_lj_factors (:713-731)
+                const double ri = 1. / mag_r;
+                const double ri2 = ri * ri, ri4 = ri2 * ri2, ri6 = ri4 * ri2, ri7 = ri6 * ri;
+                const double f6 = Rmin_6 * ri6;
+                fm = 12 * ri7 * Rmin_6 * Emin * (f6 - 1);
+                e = kSsW * kSsW * Emin * f6 * (f6 - 2);
+            }
             en += l == k ? e : 2 * e;
 #pragma unroll
             for (int x = 0; x < 3; x++) {

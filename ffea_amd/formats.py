@@ -148,6 +148,28 @@ def read_ssint_types(path, num_faces):
     return np.array([int(L[3 + i].split()[0]) for i in range(n)], np.int32)
 
 
+# tetra_element_linear::get_opposite_node (src/tetra_element_linear.cpp:645-663): the element
+# vertex opposite a second-order surface sub-triangle, keyed by the sum of the squares of the
+# triangle's local node numbers (e.g. [0, 4, 5] -> 41 -> vertex 3).
+_OPPOSITE = {114: 0, 134: 0, 154: 0, 194: 0, 110: 1, 142: 1, 126: 1, 61: 1,
+             116: 2, 81: 2, 109: 2, 52: 2, 66: 3, 41: 3, 78: 3, 90: 3}
+
+
+def surface_opposite(surf, topo):
+    """Face::n[3] of every .surf face (element, three nodes): the element's vertex opposite the
+    face (Blob::load_surface, src/Blob.cpp:2700-2719); FormatError where the face is not one of
+    the split second-order sub-triangles."""
+    surf = np.asarray(surf)
+    out = np.empty(surf.shape[0], np.int32)
+    for i, (e, *nodes) in enumerate(surf[:, :4]):
+        local = [int(np.where(topo[e] == n)[0][0]) if (topo[e] == n).any() else -1 for n in nodes]
+        op = _OPPOSITE.get(sum(k * k for k in local)) if min(local) >= 0 else None
+        if op is None:
+            raise FormatError(f"surface face {i}: could not find the opposite node")
+        out[i] = topo[e][op]
+    return out
+
+
 def read_ssint_params(path, ssint_type="lennard-jones", ssint_cutoff=None):
     """SSINT_matrix::init_ssint (src/LJ_matrix.cpp:122-199): the ntypes x ntypes table of
     (Emin, Rmin[, k0]) converted to meso units -- Emin * area^2 / Energy, Rmin / length,

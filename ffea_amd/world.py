@@ -417,8 +417,10 @@ class World:
         896-970; SimulationParams checks, src/SimulationParams.cpp:626-660): the faces of the
         blobs' .vdw / .ssint files that are active (type != -1), Lennard-Jones on the device.
         The SurfSurfEnergy column is written whenever calc_ssint or calc_steric is on (the
-        defaults: ljsteric with steric on).  Active faces with the steric, ljsteric or gensoft
-        types, periodic blob corrections (force_pbc) and sticky walls are not on this path."""
+        defaults: ljsteric with steric on).  lennard-jones, ljsteric and gensoft run on the
+        device; the steric volume exclusion (alone, or where ljsteric / gensoft face elements
+        overlap) is not computed.  Periodic blob corrections (force_pbc) and sticky walls are
+        not on this path."""
         p = self.raw
         calc_ssint = int(float(p.get("calc_ssint", p.get("calc_vdw", 1))))
         calc_steric = int(float(p.get("calc_steric", 1)))
@@ -444,19 +446,22 @@ class World:
                 fname = os.path.normpath(os.path.join(os.path.dirname(self.script), p[k]))
         if calc_ssint == 1 and fname is None:
             raise F.FormatError("surface-surface forcefield params file name required (ssint_in_fname)")
-        E = R = None
+        E = R = K = None
         ntypes = 1
         if calc_ssint == 1:
             cutoff = float(p.get("ssint_cutoff", p.get("vdw_cutoff", 3e-9)))
-            E, R, _ = F.read_ssint_params(fname, kind, cutoff / U.MESO_LENGTH)
+            E, R, K = F.read_ssint_params(fname, kind, cutoff / U.MESO_LENGTH)
             ntypes = E.shape[0]
         off = self.batch.node_off
-        faces, types = [], []
+        faces, types, opposite = [], [], []
         for b, blob in enumerate(s["blobs"]):
             path = blob.get("vdw", blob.get("ssint"))
             if not path or not blob.get("surface"):
                 raise F.FormatError(f"blob {b}: surface-surface interactions need its surface and vdw files")
-            fc = F.read_surface(blob["surface"])[:, 1:4]
+            surf = F.read_surface(blob["surface"])
+            fc = surf[:, 1:4]
+            if kind != "lennard-jones":
+                opposite.append(F.surface_opposite(surf, self.meshes[b].topo) + off[b])
             ty = F.read_ssint_types(path, fc.shape[0])
             if calc_ssint == 1:
                 if (ty > ntypes - 1).any():
@@ -475,14 +480,15 @@ class World:
                   "elements overlap); SurfSurfEnergy is written as 0", file=sys.stderr)
             return
         if kind != "lennard-jones":
-            raise F.FormatError(f"surface-surface interactions of type {kind} are not on this path "
-                                "(lennard-jones is)")
+            print(f"warning: {kind}: the steric volume exclusion it applies where face elements overlap is not "
+                  "computed on this path; elsewhere the interaction is the reference's", file=sys.stderr)
         if int(float(p.get("force_pbc", 0))) == 1:
             raise F.FormatError("force_pbc is not on this path")
         cutoff = float(p.get("ssint_cutoff", p.get("vdw_cutoff", 3e-9))) / U.MESO_LENGTH
         inc_self = int(float(p.get("inc_self_ssint", p.get("inc_self_vdw", 1))))
         es_update = int(float(p.get("es_update", 10)))
-        self.batch.set_ssint(np.concatenate(faces), types, E, R, self.es_N, cutoff, inc_self, es_update)
+        self.batch.set_ssint(np.concatenate(faces), types, E, R, self.es_N, cutoff, inc_self, es_update, kind=kind,
+                             opposite=np.concatenate(opposite) if opposite else None, k0=K)
 
     def _meas_header(self) -> str:
         p = self.raw

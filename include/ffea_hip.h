@@ -212,28 +212,39 @@ int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_node
                                 const int *set_faces, const double *set_force);
 
 /*
- * Surface-surface interactions with ssint_type "lennard-jones" (VdW_solver::solve and
- * do_lj_interaction, src/VdW_solver.cpp:158-384, 663-731; SSINT_matrix, src/LJ_matrix.cpp:122-199;
- * the lookup grid of NearestNeighbourLinkedListCube, src/NearestNeighbourLinkedListCube.cpp:26-77).
+ * Surface-surface interactions (VdW_solver::solve, do_lj_interaction, do_gensoft_interaction,
+ * src/VdW_solver.cpp:158-384, 536-831; SSINT_matrix, src/LJ_matrix.cpp:122-199; the lookup grid
+ * of NearestNeighbourLinkedListCube, src/NearestNeighbourLinkedListCube.cpp:26-77).
+ * kind: FFH_SSINT_LENNARD_JONES, FFH_SSINT_LJSTERIC (Lennard-Jones with the soft core below
+ * Rmin, calc_ljinterpolated_force_pair_matrix) or FFH_SSINT_GENSOFT (rmin = the cutoff, k0).
  * face_nodes [3*num_faces]: the surface faces' corners (N_t ids of the handle, blobs in order,
- * each blob's faces in .surf order); face_type [num_faces]: the .vdw / .ssint type, -1 for an
- * inactive face; emin, rmin [num_types^2] in meso units (Emin * area^2 / Energy, Rmin / length),
- * entry type1 * num_types + type2; grid [3] = es_N_x/y/z and cell_size = ssint_cutoff (meso) of
- * the lookup grid; inc_self_ssint; es_update (face geometry and grid refreshed on the first step
- * and every es_update steps, as World::run does).  Every step then starts with the solve: each
- * pair of active faces in neighbouring cells (i < j in pool order, other blob unless inc_self,
- * normals not facing the same way) adds its three-point-quadrature forces to both faces, which
- * reach the nodes with the other face forces after the element forces (src/Blob.cpp:3842-3851).
- * All faces inactive (or num_faces 0) removes the interaction.  Linear surface forces (CTF 3)
- * together with it are refused; the steric, ljsteric and gensoft types are not on this path.
- * A face centroid outside the grid after one period fails the step (FFH_E_ARG), as
+ * each blob's faces in .surf order); face_opposite [num_faces]: Face::n[3], the element vertex
+ * opposite each face (read by the ljsteric / gensoft pair tests; may be null for
+ * lennard-jones); face_type [num_faces]: the .vdw / .ssint type, -1 for an inactive face;
+ * emin, rmin, k0 [num_types^2] in meso units (Emin * area^2 / Energy, Rmin / length,
+ * k0 * area / Energy), entry type1 * num_types + type2; grid [3] = es_N_x/y/z and cell_size =
+ * ssint_cutoff (meso) of the lookup grid; inc_self_ssint; es_update (face geometry and grid
+ * refreshed on the first step and every es_update steps, as World::run does).  Every step
+ * then starts with the solve: each pair of active faces in neighbouring cells (i < j in pool
+ * order, other blob unless inc_self, normals not facing the same way; for ljsteric and gensoft
+ * also faces in front of each other and no shared node within a blob) adds its
+ * three-point-quadrature forces to both faces, which reach the nodes with the other face
+ * forces after the element forces (src/Blob.cpp:3842-3851).  The steric volume exclusion that
+ * ljsteric and gensoft apply instead where the two face elements overlap
+ * (Face::checkTetraIntersection) is not on this path.  All faces inactive (or num_faces 0)
+ * removes the interaction.  Linear surface forces (CTF 3) together with it are refused.  A
+ * face centroid outside the grid after one period fails the step (FFH_E_ARG), as
  * LinkedListCube::add_node_to_stack does.
  * ffh_blob_ssint_energy: VdW_solver::get_field_energy(-1, -1) of the last solve (meso energy),
- * summed in the solve's order.
+ * summed in the solve's order; call it between ffh_blob_update_internal_forces and
+ * ffh_blob_update_positions, as World::make_measurements does.
  */
-int ffh_blob_set_ssint(ffh_blob *b, int num_faces, const int *face_nodes, const int *face_type, int num_types,
-                       const double *emin, const double *rmin, const int *grid, double cell_size, int inc_self,
-                       int es_update);
+#define FFH_SSINT_LENNARD_JONES 1
+#define FFH_SSINT_LJSTERIC 2
+#define FFH_SSINT_GENSOFT 3
+int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nodes, const int *face_opposite,
+                       const int *face_type, int num_types, const double *emin, const double *rmin, const double *k0,
+                       const int *grid, double cell_size, int inc_self, int es_update);
 int ffh_blob_ssint_energy(ffh_blob *b, double *energy);
 
 /*

@@ -495,6 +495,7 @@ typedef struct {
     /* surface-surface interactions: per surface face its type (-1: inactive) and the
      * geometry Face::calc_area_normal_centroid last left (centroid, normal, area) */
     int *ssint_type;       /* [n_faces] or NULL */
+    int *ssint_opp;        /* [n_faces] the face element's vertex opposite the face (Face::n[3]) */
     double *face_geom;     /* [7 n_faces] */
 } orc_blob;
 
@@ -720,7 +721,7 @@ void orc_blob_destroy(orc_blob *b) {
                     b->contrib, b->key, b->col, b->src_off, (void *)b->src, b->val, b->diag_inv, b->diag_idx,
                     b->r, b->p, b->z, b->q, b->f, b->mkey, b->mcol, b->mval, b->mprec, b->inv_M,
                     b->ctf_node, b->ctf_force, b->rctf_node, b->rctf_F, b->rctf_axis, b->rctf_torque, b->face_nodes, b->set_off, b->set_faces, b->set_force,
-                    b->face_force, b->ssint_type, b->face_geom};
+                    b->face_force, b->ssint_type, b->ssint_opp, b->face_geom};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(b);
 }
@@ -840,8 +841,10 @@ void orc_blob_set_surface_forces(orc_blob *b, int n_faces, const int *faces, int
     b->set_force = xcalloc(3 * (size_t)(n_sets > 0 ? n_sets : 1), sizeof(double));
     b->face_force = xcalloc(9 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
     free(b->ssint_type);
+    free(b->ssint_opp);
     free(b->face_geom);
     b->ssint_type = NULL;
+    b->ssint_opp = NULL;
     b->face_geom = xcalloc(7 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
     if (n_faces > 0) memcpy(b->face_nodes, faces, sizeof(int) * 3 * n_faces);
     if (n_sets > 0) {
@@ -1438,27 +1441,32 @@ static const int g_adj[27][3] = {{-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0,
                                  {1, 1, -1},   {1, 1, 0},   {1, 1, 1}};
 
 /* SSINT_matrix in meso units (LJI(i, j) = i * ntypes + j), the lookup grid and the solver's
- * switches (World::init, src/World.cpp:896-970; VdW_solver::init). ntypes 0: off. */
-static int g_ss_ntypes, g_ss_inc_self, g_ss_es_update, g_ss_es_count, g_ss_N[3];
-static double *g_ss_emin, *g_ss_rmin, g_ss_h;
+ * switches (World::init, src/World.cpp:896-970; VdW_solver::init). ntypes 0: off.  kind: 1
+ * lennard-jones, 2 ljsteric (its soft-core branch below Rmin), 3 gensoft. */
+static int g_ss_ntypes, g_ss_inc_self, g_ss_es_update, g_ss_es_count, g_ss_N[3], g_ss_kind = 1;
+static double *g_ss_emin, *g_ss_rmin, *g_ss_k0, g_ss_h;
 static double *g_ss_energy; /* fieldenergy [nblobs * nblobs] of the last solve */
 static int g_ss_nblobs;
 /* the lookup grid: per pool entry (blob, face) its cell and the next entry down its stack */
 static int g_ss_npool, *g_ss_pool_blob, *g_ss_pool_face, *g_ss_pool_cell, *g_ss_next, *g_ss_root;
 
-void orc_world_set_ssint(int ntypes, const double *emin, const double *rmin, const int *N, double h, int inc_self,
-                         int es_update) {
+void orc_world_set_ssint(int kind, int ntypes, const double *emin, const double *rmin, const double *k0, const int *N,
+                         double h, int inc_self, int es_update) {
     free(g_ss_emin);
     free(g_ss_rmin);
+    free(g_ss_k0);
     free(g_ss_root);
-    g_ss_emin = g_ss_rmin = NULL;
+    g_ss_emin = g_ss_rmin = g_ss_k0 = NULL;
     g_ss_root = NULL;
     g_ss_ntypes = ntypes;
+    g_ss_kind = kind;
     if (ntypes <= 0) return;
     g_ss_emin = xcalloc((size_t)ntypes * ntypes, sizeof(double));
     g_ss_rmin = xcalloc((size_t)ntypes * ntypes, sizeof(double));
+    g_ss_k0 = xcalloc((size_t)ntypes * ntypes, sizeof(double));
     memcpy(g_ss_emin, emin, sizeof(double) * ntypes * ntypes);
     memcpy(g_ss_rmin, rmin, sizeof(double) * ntypes * ntypes);
+    memcpy(g_ss_k0, k0, sizeof(double) * ntypes * ntypes);
     for (int k = 0; k < 3; k++) g_ss_N[k] = N[k];
     g_ss_h = h;
     g_ss_inc_self = inc_self;
@@ -1469,10 +1477,15 @@ void orc_world_set_ssint(int ntypes, const double *emin, const double *rmin, con
 
 /* Blob::load_ssint (src/Blob.cpp:3494-3578): the type of every surface face (-1 inactive);
  * call after orc_blob_set_surface_forces has given the faces. */
-void orc_blob_set_ssint_types(orc_blob *b, const int *types) {
+void orc_blob_set_ssint_types(orc_blob *b, const int *types, const int *opposite) {
     free(b->ssint_type);
+    free(b->ssint_opp);
     b->ssint_type = xcalloc(b->n_faces > 0 ? b->n_faces : 1, sizeof(int));
-    if (b->n_faces > 0) memcpy(b->ssint_type, types, sizeof(int) * b->n_faces);
+    b->ssint_opp = xcalloc(b->n_faces > 0 ? b->n_faces : 1, sizeof(int));
+    if (b->n_faces > 0) {
+        memcpy(b->ssint_type, types, sizeof(int) * b->n_faces);
+        memcpy(b->ssint_opp, opposite, sizeof(int) * b->n_faces);
+    }
 }
 
 /* Face::calc_area_normal_centroid (src/Face.cpp:197-219). */
@@ -1546,6 +1559,39 @@ static int ssint_refresh(orc_blob **blobs, int nblobs) {
     return 0;
 }
 
+/* VdW_solver::calc_ljinterpolated_factors (src/VdW_solver.cpp:774-784): the soft core below
+ * Rmin that ljsteric uses. */
+static void lj_soft_factors(double mag_r, double Emin, double Rmini, double *force_mag, double *e) {
+    const double f = mag_r * Rmini, f2 = f * f;
+    *e = g_gw * g_gw * Emin * f2 * (2 * f - 3);
+    *force_mag = 6 * Emin * Rmini * f * (1 - f);
+}
+
+/* VdW_solver::calc_gensoft_factors (src/VdW_solver.cpp:754-771): Rmin is the cutoff. */
+static void gensoft_factors(double mag_r, double Emin, double Rmin, double k0, double *force_mag, double *e) {
+    const double r2 = mag_r * mag_r;
+    const double R2 = Rmin * Rmin;
+    const double g2 = r2 / R2, g3 = g2 * mag_r / Rmin;
+    const double k0rm = k0 * Rmin, k0rm2 = k0rm * Rmin, eps = Emin / Rmin;
+    *force_mag = 2 * (k0rm - 6 * eps) * g3 + 3 * (4 * eps - k0rm) * g2 + k0 * mag_r;
+    const double emag = (0.5 * k0rm2 - 3 * Emin) * g2 * g2 + (4 * Emin - k0rm2) * g3 + 0.5 * k0 * r2 - Emin;
+    *e = g_gw * g_gw * emag;
+}
+
+static void lj_factors(double mag_r, int k, int l, double Emin, double Rmin_6, double *force_mag, double *e);
+
+/* The force magnitude and the energy of one Gauss-point pair at distance r, for the tests:
+ * kind 1 lennard-jones, 2 ljsteric, 3 gensoft (its Rmin is the cutoff). */
+void orc_ssint_factors(int kind, double r, double Emin, double Rmin, double k0, double *out) {
+    const double R2 = Rmin * Rmin, R4 = R2 * R2, R6 = R4 * R2;
+    if (kind == 3)
+        gensoft_factors(r, Emin, Rmin, k0, &out[0], &out[1]);
+    else if (kind == 2 && r < Rmin)
+        lj_soft_factors(r, Emin, 1.0 / Rmin, &out[0], &out[1]);
+    else
+        lj_factors(r, 0, 0, Emin, R6, &out[0], &out[1]);
+}
+
 /* VdW_solver::calc_lj_factors (src/VdW_solver.cpp:713-731). */
 static void lj_factors(double mag_r, int k, int l, double Emin, double Rmin_6, double *force_mag, double *e) {
     const double ri = 1. / mag_r;
@@ -1564,6 +1610,7 @@ static void lj_factors(double mag_r, int k, int l, double Emin, double Rmin_6, d
 static void lj_pair(orc_blob *b1, int f1, orc_blob *b2, int f2, double *energy_out) {
     const int t1 = b1->ssint_type[f1], t2 = b2->ssint_type[f2];
     const double Emin = g_ss_emin[t1 * g_ss_ntypes + t2], Rmin = g_ss_rmin[t1 * g_ss_ntypes + t2];
+    const double k0 = g_ss_k0[t1 * g_ss_ntypes + t2], Rmini = 1.0 / Rmin;
     const int *n1 = &b1->face_nodes[3 * f1], *n2 = &b2->face_nodes[3 * f2];
     double p[3][3], q[3][3], fpm[3][3][3];
     for (int i = 0; i < 3; i++)
@@ -1579,7 +1626,12 @@ static void lj_pair(orc_blob *b1, int f1, orc_blob *b2, int f2, double *energy_o
         for (int l = k; l < 3; l++) {
             const double dx = p[k][0] - q[l][0], dy = p[k][1] - q[l][1], dz = p[k][2] - q[l][2];
             const double mag_r = sqrt(dx * dx + dy * dy + dz * dz);
-            lj_factors(mag_r, k, l, Emin, Rmin_6, &fm, &e);
+            if (g_ss_kind == 3)
+                gensoft_factors(mag_r, Emin, Rmin, k0, &fm, &e);
+            else if (g_ss_kind == 2 && mag_r < Rmin)
+                lj_soft_factors(mag_r, Emin, Rmini, &fm, &e);
+            else
+                lj_factors(mag_r, k, l, Emin, Rmin_6, &fm, &e);
             energy += l == k ? e : 2 * e;
             for (int c = 0; c < 3; c++) fpm[k][l][c] = fm * ((p[k][c] - q[l][c]) / mag_r);
             if (l != k)
@@ -1600,6 +1652,28 @@ static void lj_pair(orc_blob *b1, int f1, orc_blob *b2, int f2, double *energy_o
         for (int x = 0; x < 3; x++) b1->face_force[9 * f1 + 3 * j + x] += F1[x] * ApAq;
         for (int x = 0; x < 3; x++) b2->face_force[9 * f2 + 3 * j + x] += F2[x] * ApAq;
     }
+}
+
+/* consider_interaction's further tests for the types other than lennard-jones
+ * (src/VdW_solver.cpp:354-381): the faces must be in front of each other (sep from face i's
+ * centroid to face j's, arr3arr3DotProduct: sep.n_i < 0 with sep.n_j > 0 rejects), and two
+ * faces of one blob may share no node (the three corners and the opposite vertex). */
+static int ssint_steric_checks(const orc_blob *b1, int f1, const orc_blob *b2, int f2, int same_blob) {
+    const double *c1 = &b1->face_geom[7 * f1], *c2 = &b2->face_geom[7 * f2];
+    double sep[3], d1 = 0.0, d2 = 0.0;
+    for (int k = 0; k < 3; k++) sep[k] = c2[k] - c1[k];
+    for (int k = 0; k < 3; k++) d1 += sep[k] * c1[3 + k];
+    for (int k = 0; k < 3; k++) d2 += sep[k] * c2[3 + k];
+    if (d1 < 0 && d2 > 0) return 0;
+    if (g_ss_inc_self && same_blob) {
+        if (b1->ssint_opp[f1] == b2->ssint_opp[f2]) return 0;
+        int n1[4] = {b1->face_nodes[3 * f1], b1->face_nodes[3 * f1 + 1], b1->face_nodes[3 * f1 + 2], b1->ssint_opp[f1]};
+        int n2[4] = {b2->face_nodes[3 * f2], b2->face_nodes[3 * f2 + 1], b2->face_nodes[3 * f2 + 2], b2->ssint_opp[f2]};
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                if (n2[j] == n1[i]) return 0;
+    }
+    return 1;
 }
 
 /* VdW_solver::solve (src/VdW_solver.cpp:158-210) with consider_interaction (:332-384), Lennard-
@@ -1623,6 +1697,7 @@ static void ssint_solve(orc_blob **blobs, int nblobs) {
                 if (!g_ss_inc_self && bi == bj) continue;
                 const double *nj = &blobs[bj]->face_geom[7 * fj + 3];
                 if (ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2] > 0.0) continue;
+                if (g_ss_kind != 1 && !ssint_steric_checks(blobs[bi], fi, blobs[bj], fj, bi == bj)) continue;
                 double e;
                 lj_pair(blobs[bi], fi, blobs[bj], fj, &e);
                 g_ss_energy[bi * nblobs + bj] += e;

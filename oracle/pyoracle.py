@@ -159,9 +159,12 @@ class Blob:
             nodes, F, axis, torque = np.zeros(1, np.int32), np.zeros(1), np.zeros(6), np.zeros(1, np.int32)
         self.lib.lib.orc_blob_set_rotational_forces(self.h, n, nodes, F, axis, torque)
 
-    def set_ssint_types(self, types):
-        """Surface-surface interaction type of every face given to set_surface_forces (-1 off)."""
-        self.lib.lib.orc_blob_set_ssint_types(self.h, np.ascontiguousarray(types, np.int32))
+    def set_ssint_types(self, types, opposite=None):
+        """Surface-surface interaction type of every face given to set_surface_forces (-1 off)
+        and each face's opposite element vertex (only the non-Lennard-Jones types read it)."""
+        t = np.ascontiguousarray(types, np.int32)
+        o = np.full_like(t, -1) if opposite is None else np.ascontiguousarray(opposite, np.int32)
+        self.lib.lib.orc_blob_set_ssint_types(self.h, t, o)
 
     def set_surface_forces(self, faces, sets, set_forces):
         """Linear surface forces (oracle only): faces [F, 3] node ids; sets: list of face-index
@@ -246,9 +249,10 @@ class Oracle(_Lib):
         L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
         L.orc_rot_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         L.orc_world_set_box.argtypes = [_dp, C.c_int]
-        L.orc_world_set_ssint.argtypes = [C.c_int, _dp, _dp, _ip, C.c_double, C.c_int, C.c_int]
-        L.orc_blob_set_ssint_types.argtypes = [C.c_void_p, _ip]
+        L.orc_world_set_ssint.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _ip, C.c_double, C.c_int, C.c_int]
+        L.orc_blob_set_ssint_types.argtypes = [C.c_void_p, _ip, _ip]
         L.orc_world_ssint_energy.restype = C.c_double
+        L.orc_ssint_factors.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_blob_centroid.argtypes = [C.c_void_p, _dp]
         L.orc_world_set_springs.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
         L.orc_world_spring_energy.argtypes = [C.POINTER(C.c_void_p), C.c_int]
@@ -272,16 +276,27 @@ class Oracle(_Lib):
     def stream(self) -> "OracleStream":
         return OracleStream(self)
 
-    def set_ssint(self, emin=None, rmin=None, grid=(1, 1, 1), h=1.0, inc_self=1, es_update=1):
-        """Lennard-Jones surface-surface interactions for the next world_step calls: emin, rmin
-        [ntypes, ntypes] meso, the lookup grid's cell counts and cell size (meso), inc_self_ssint
-        and es_update.  emin None switches them off."""
+    SSINT_KINDS = {"lennard-jones": 1, "ljsteric": 2, "gensoft": 3}
+
+    def set_ssint(self, emin=None, rmin=None, grid=(1, 1, 1), h=1.0, inc_self=1, es_update=1, kind="lennard-jones",
+                  k0=None):
+        """Surface-surface interactions for the next world_step calls: emin, rmin (and k0 for
+        gensoft) [ntypes, ntypes] meso, the lookup grid's cell counts and cell size (meso),
+        inc_self_ssint, es_update and the type.  emin None switches them off."""
         if emin is None:
-            self.lib.orc_world_set_ssint(0, np.zeros(1), np.zeros(1), np.zeros(3, np.int32), 1.0, 0, 1)
+            self.lib.orc_world_set_ssint(1, 0, np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(3, np.int32), 1.0, 0, 1)
             return
         e = np.ascontiguousarray(emin, np.float64)
-        self.lib.orc_world_set_ssint(e.shape[0], e.ravel(), np.ascontiguousarray(rmin, np.float64).ravel(),
+        k = np.zeros_like(e) if k0 is None else np.ascontiguousarray(k0, np.float64)
+        self.lib.orc_world_set_ssint(self.SSINT_KINDS[kind], e.shape[0], e.ravel(),
+                                     np.ascontiguousarray(rmin, np.float64).ravel(), k.ravel(),
                                      np.ascontiguousarray(grid, np.int32), float(h), int(inc_self), int(es_update))
+
+    def ssint_factors(self, kind, r, Emin, Rmin, k0=0.0):
+        """(force magnitude, energy) of one Gauss-point pair at distance r (meso)."""
+        out = np.zeros(2)
+        self.lib.orc_ssint_factors(self.SSINT_KINDS[kind], float(r), float(Emin), float(Rmin), float(k0), out)
+        return out
 
     def ssint_energy(self) -> float:
         return self.lib.orc_world_ssint_energy()

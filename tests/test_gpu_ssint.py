@@ -6,7 +6,9 @@ midpoints as corners) with MassLumpedSolver (no reductions in the solve), dt = 1
 on: force vectors, positions and interaction energies bit-exact step by step, for the cubes_lj
 set-up (top faces active, other blob only), every face active with self-interaction, lookup
 grids of one and two cells per axis (neighbour offsets wrapping onto the same cell: pairs met
-several times), and es_update 3 (geometry and grid held between refreshes).  The massless
+several times), es_update 3 (geometry and grid held between refreshes), ljsteric with the
+cubes 0.6 nm apart (its soft core below Rmin) and with every face of both cubes active and
+self-interaction on (its in-front and shared-node tests), and gensoft (Rmin the cutoff, k0).  The massless
 solver: force vector bit-exact, positions within 1e-9.  A centroid outside the grid fails the
 step, as in the oracle.  Graph replay equals ordinary launches.
 """
@@ -22,7 +24,7 @@ pytestmark = pytest.mark.gpu
 CUBES = os.path.join(GOLDEN, "physics", "cubes_lj", "cube_veryFine.ffea")
 
 
-def cubes(oracle, solver="masslumped", seed=11, all_faces=False):
+def cubes(oracle, solver="masslumped", seed=11, all_faces=False, shift=0.0):
     from ffea_amd import formats as F, units as U
     from ffea_amd.blob import Batch
     from ffea_amd.world import linearised, place_blobs
@@ -35,32 +37,37 @@ def cubes(oracle, solver="masslumped", seed=11, all_faces=False):
     pos = place_blobs(s["params"], s["blobs"], [linearised(m.pos * sc, m.topo) for m, sc in zip(meshes, scales)])
     # midpoints as averages of the placed vertices, as the device evaluates them (the reference
     # keeps the shifted loaded midpoints until the first step's end)
+    pos[1][:, 1] += shift * 1e-9 / U.MESO_LENGTH
     pos = [linearised(p, m.topo) for p, m in zip(pos, meshes)]
     sd = oracle.package_seed(seed)
     oracle.set_package_seed(sd)
     st = oracle.stream()
-    bo, faces, types = [], [], []
+    bo, faces, types, opps = [], [], [], []
     for b, m, sc, p in zip(s["blobs"], meshes, scales, pos):
         o = oracle.blob(m, prm, sc)
         o.set_solver(solver)
         o.set_positions(p.ravel())
-        fc = F.read_surface(b["surface"])[:, 1:4]
+        surf = F.read_surface(b["surface"])
+        fc = surf[:, 1:4]
+        op = F.surface_opposite(surf, m.topo)
         ty = F.read_ssint_types(b["vdw"], fc.shape[0])
         if all_faces:
             ty = np.ones_like(ty)
         o.set_surface_forces(fc, [], np.zeros((0, 3)))
-        o.set_ssint_types(ty)
+        o.set_ssint_types(ty, op)
         bo.append(o)
         faces.append(fc)
         types.append(ty)
+        opps.append(op)
     g = Batch(meshes, prm, scales, rng_state=sd)
     g.set_positions(np.concatenate(pos))
     g.set_solver(solver)
     off = g.node_off
     gf = np.concatenate([fc + off[k] for k, fc in enumerate(faces)])
+    gop = np.concatenate([op + off[k] for k, op in enumerate(opps)])
     lj = os.path.join(os.path.dirname(CUBES), s["params"]["vdw_forcefield_params"])
     E, R, _ = F.read_ssint_params(lj, "lennard-jones")
-    return g, bo, st, gf, np.concatenate(types), E, R, 3e-9 / U.MESO_LENGTH
+    return g, bo, st, gf, np.concatenate(types), E, R, 3e-9 / U.MESO_LENGTH, gop
 
 
 @pytest.fixture
@@ -99,12 +106,19 @@ def run_both(oracle, g, bo, st, nsteps, exact=True):
     assert np.array_equal(g.rng_state(), st.state())
 
 
-@pytest.mark.parametrize("case", ["cubes_lj", "all_self", "grid1", "grid2", "es3"])
+@pytest.mark.parametrize("case", ["cubes_lj", "all_self", "grid1", "grid2", "es3", "ljsteric_close",
+                                  "ljsteric_self", "gensoft"])
 def test_trajectory(oracle, ssint_off, case):
-    g, bo, st, faces, types, E, R, h = cubes(oracle, all_faces=case == "all_self")
+    g, bo, st, faces, types, E, R, h, opp = cubes(oracle, all_faces=case in ("all_self", "ljsteric_self"),
+                                                  shift=-0.4 if case == "ljsteric_close" else 0.0)
     grid, cell, inc, es = (10, 10, 10), h, 0, 1
-    if case == "all_self":
+    kind, K = "lennard-jones", np.zeros_like(E)
+    if case in ("all_self", "ljsteric_self"):
         inc = 1
+    if case.startswith("ljsteric"):
+        kind = "ljsteric"
+    if case == "gensoft":  # Rmin is the cutoff; a spring constant as well
+        kind, R, K = "gensoft", np.full_like(R, h), np.full_like(E, 0.02)
     elif case == "grid1":
         grid, cell = (1, 1, 1), 1e3
     elif case == "grid2":
@@ -112,13 +126,13 @@ def test_trajectory(oracle, ssint_off, case):
     elif case == "es3":
         es = 3
     print(case, int((types >= 0).sum()), "active faces", flush=True)
-    oracle.set_ssint(E, R, grid, cell, inc_self=inc, es_update=es)
-    g.set_ssint(faces, types, E, R, grid, cell, inc, es)
+    oracle.set_ssint(E, R, grid, cell, inc_self=inc, es_update=es, kind=kind, k0=K)
+    g.set_ssint(faces, types, E, R, grid, cell, inc, es, kind=kind, opposite=opp, k0=K)
     run_both(oracle, g, bo, st, 4 if case != "es3" else 5)
 
 
 def test_massless(oracle, ssint_off):
-    g, bo, st, faces, types, E, R, h = cubes(oracle, solver="CG_nomass")
+    g, bo, st, faces, types, E, R, h, _ = cubes(oracle, solver="CG_nomass")
     oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=0, es_update=1)
     g.set_ssint(faces, types, E, R, (10, 10, 10), h, 0, 1)
     run_both(oracle, g, bo, st, 3, exact=False)
@@ -127,7 +141,7 @@ def test_massless(oracle, ssint_off):
 def test_grid_overflow_fails(oracle, ssint_off):
     import pyoracle
     from ffea_amd import _lib
-    g, bo, st, faces, types, E, R, h = cubes(oracle)
+    g, bo, st, faces, types, E, R, h, _ = cubes(oracle)
     oracle.set_ssint(E, R, (10, 10, 10), 1.0, inc_self=0, es_update=1)
     assert pyoracle.world_step(oracle, bo, st) == -2
     g.set_ssint(faces, types, E, R, (10, 10, 10), 1.0, 0, 1)
@@ -136,8 +150,8 @@ def test_grid_overflow_fails(oracle, ssint_off):
 
 
 def test_graph_replay(oracle):
-    g, _, _, faces, types, E, R, h = cubes(oracle, seed=3)
-    a, _, _, _, _, _, _, _ = cubes(oracle, seed=3)
+    g, _, _, faces, types, E, R, h, _ = cubes(oracle, seed=3)
+    a = cubes(oracle, seed=3)[0]
     for x in (a, g):
         x.set_ssint(faces, types, E, R, (10, 10, 10), h, 0, 2)
     a.set_graphs(False)

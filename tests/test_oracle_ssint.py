@@ -124,3 +124,77 @@ def test_self_interaction_switch_and_normals(oracle, ssint_off):
         (a,), (b,) = interaction_rhs(oracle, one, E, R, (10, 10, 10), h, inc)
         assert np.array_equal(a, b)
         assert oracle.ssint_energy() == 0.0
+
+
+# ---- the other types' pair potentials and checks (ljsteric's soft core, gensoft) ------------
+
+W2 = 0.333333333333333 * 0.333333333333333
+
+
+@pytest.mark.parametrize("kind", ["lennard-jones", "ljsteric", "gensoft"])
+def test_pair_potentials(oracle, kind):
+    """Closed forms of one Gauss-point pair's energy and force magnitude
+    (src/VdW_solver.cpp:713-784): Lennard-Jones has its minimum -Emin (times both weights) at
+    Rmin with zero force; ljsteric's soft core below Rmin joins it there with value and force,
+    and is -Emin x^2 (3 - 2x) in x = r / Rmin; gensoft (Rmin = the cutoff) rises from -Emin at
+    r = 0 to 0 at the cutoff, force zero at both.  The force magnitude is minus the energy's
+    derivative for the first two and plus it for gensoft, as the reference writes them."""
+    E, R, k0 = 0.7, 1.3, 0.4
+    f, e = oracle.ssint_factors(kind, R, E, R, k0)
+    assert f == pytest.approx(0.0, abs=1e-12)
+    assert e == pytest.approx(0.0 if kind == "gensoft" else -W2 * E, abs=1e-12)
+    for r in (0.3 * R, 0.8 * R, 1.2 * R, 1.7 * R):
+        f, e = oracle.ssint_factors(kind, r, E, R, k0)
+        h = 1e-6 * r
+        de = (oracle.ssint_factors(kind, r + h, E, R, k0)[1] - oracle.ssint_factors(kind, r - h, E, R, k0)[1]) / (2 * h)
+        sign = 1 if kind == "gensoft" else -1
+        assert f == pytest.approx(sign * de / W2, rel=1e-6, abs=1e-9)
+        if kind == "ljsteric" and r < R:
+            x = r / R
+            assert e == pytest.approx(-W2 * E * x * x * (3 - 2 * x), rel=1e-13)
+            assert e < oracle.ssint_factors("lennard-jones", r, E, R)[1]
+    if kind == "gensoft":
+        assert oracle.ssint_factors(kind, 0.0, E, R, k0)[1] == pytest.approx(-W2 * E, rel=1e-13)
+
+
+def test_ljsteric_is_lennard_jones_beyond_rmin(oracle, ssint_off):
+    """With every Gauss-point distance above Rmin (the cubes 1.1 nm apart) ljsteric takes the
+    Lennard-Jones branch everywhere and its extra tests reject nothing on facing planes: the
+    same energy to the last bit; closer (0.6 nm) its soft core is below Lennard-Jones."""
+    from ffea_amd import units as U
+    out = {}
+    for gap, kinds in ((0.1, ("lennard-jones", "ljsteric")), (-0.4, ("lennard-jones", "ljsteric"))):
+        for kind in kinds:
+            blobs, E, R, h, _ = cubes_world(oracle)
+            p = blobs[1].get("pos").reshape(-1, 3)
+            p[:, 1] += gap * 1e-9 / U.MESO_LENGTH
+            blobs[1].set_positions(p.ravel())
+            oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=0, es_update=1, kind=kind)
+            oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+            assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
+            out[gap, kind] = oracle.ssint_energy()
+    assert out[0.1, "ljsteric"] == out[0.1, "lennard-jones"]
+    assert out[-0.4, "ljsteric"] < out[-0.4, "lennard-jones"]
+
+
+def test_steric_checks_exclude_neighbours(oracle, ssint_off):
+    """With self-interaction on and every face of one cube active, ljsteric's shared-node test
+    drops adjacent faces (which Lennard-Jones keeps, at tiny distances): a finite energy far
+    below the Lennard-Jones one."""
+    from ffea_amd import formats as F
+    blobs, E, R, h, s = cubes_world(oracle)
+    b = s["blobs"][0]
+    m = F.load_blob(b["nodes"], b["topology"], b["material"], b["stokes"])
+    surf = F.read_surface(b["surface"])
+    opp = F.surface_opposite(surf, m.topo)
+    one = blobs[:1]
+    one[0].set_ssint_types(np.ones(surf.shape[0], np.int32), opp)
+    res = {}
+    for kind in ("lennard-jones", "ljsteric"):
+        pos = one[0].get("pos").copy()
+        oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=1, es_update=1, kind=kind)
+        oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+        assert pyoracle.world_step(oracle, one, oracle.stream()) == 0
+        res[kind] = oracle.ssint_energy()
+        one[0].set_positions(pos)
+    assert np.isfinite(res["ljsteric"]) and abs(res["ljsteric"]) < abs(res["lennard-jones"])
