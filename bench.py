@@ -221,7 +221,7 @@ def massful(ms, prm, sc, xs, steps, warmup, cpu_seconds=0.0):
     """The mass-ful solvers on the same workload (FP64): ConjugateGradientSolver on the
     consistent mass matrix and MassLumpedSolver, steps/s and per-kernel times.  Time step 1 fs:
     the explicit mass-ful update of these nanometre elements is unstable at 10 fs (in the
-    reference as in this build); the work per step does not depend on it."""
+    reference as here); the work per step does not depend on it."""
     import dataclasses
     from ffea_amd.blob import Batch, Blob, package_seed
     p = dataclasses.replace(prm, dt=prm.dt / 10)
@@ -808,7 +808,7 @@ def main():
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
               "(independent blobs shard by blob with no communication; one large blob by node slabs with a "
               "boundary exchange and two scalar all-reduces per CG iteration)",
               file=sys.stderr)
