@@ -89,3 +89,30 @@ def test_mass_restart_reproduces_continuous_run(tmp_path):
     with np.errstate(divide="ignore", invalid="ignore"):
         d = np.where(m1 != 0, np.abs((m2 - m1) / m1), 0.0)
     assert d.max() <= 5e-4, d.max()
+
+
+def test_world_wraps_blob_into_periodic_box(tmp_path):
+    """World::run's periodic box through the World driver: the sphere placed at centroid
+    (-2, 0.3, -1.5) (times its 5 nm scale) without move_into_box; the box is 2 ceil(extent /
+    3 nm) cells of 3 nm per axis (World::init, src/World.cpp:455-471).  At step 0 the axes whose
+    centroid lies below zero move up by one box length (src/World.cpp:1850-1901), so the first
+    frame's centroid is inside the box; the other axis is untouched."""
+    from ffea_amd import units as U
+    from ffea_amd.world import World, read_trajectory
+    shutil.copytree(os.path.join(SRC, "sphere_63_120_structure"), tmp_path / "sphere_63_120_structure")
+    run = tmp_path / "box"
+    shutil.copytree(os.path.join(SRC, "sphere_63_120_nomass"), run)
+    p = run / "sphere_63_120_nomass.ffea"
+    t = p.read_text().replace("<num_steps = 1e5>", "<num_steps = 3>").replace("<check = 1e3>", "<check = 1>")
+    t = t.replace("<centroid = (0.0,0.0,0.0)>", "<centroid = (-2.0,0.3,-1.5)>")
+    t = t.replace("</param>", "\t<move_into_box = 0>\n</param>")
+    p.write_text(t)
+    w = World(str(p))
+    assert w.walls == 0b111111
+    box = w.box * U.MESO_LENGTH
+    w.run()
+    frames = read_trajectory(str(run / "sphere_63_120_nomass_trajectory.out"))
+    c0 = frames[0][1][0][:, :3].mean(axis=0)
+    placed = np.array([-2.0, 0.3, -1.5]) * 5e-9
+    assert np.allclose(c0, placed + np.array([box[0], 0.0, box[2]]), rtol=0, atol=1e-3 * box.max())
+    assert (c0 > 0).all() and (c0 < box).all()

@@ -199,3 +199,26 @@ def test_blob_centroid_and_rotation():
     d0 = np.linalg.norm(p[1:] - p[0], axis=1)
     d1 = np.linalg.norm(r[1:] - r[0], axis=1)
     assert np.allclose(d1, d0, rtol=1e-6)
+
+
+def test_ssint_files():
+    """Blob::load_ssint and SSINT_matrix::init_ssint on the cubes_lj fixture (the reference's
+    tests/physics/fine_cube_structure): 1256 faces, the 200 of the top face type 0; 7 types,
+    entry (0, 0) = (2.537e17 J/m^4, 1 nm) in meso units, the rest (1e-15, 1e-10); no k0 given.
+    Face::n[3] of every sub-triangle is a vertex of its element (get_opposite_node)."""
+    from ffea_amd import formats as F, units as U
+    d = os.path.join(GOLDEN, "physics", "cubes_lj")
+    t = F.read_ssint_types(os.path.join(d, "veryFine_topface0.vdw"), 1256)
+    assert (t == 0).sum() == 200 and (t == -1).sum() == 1056
+    E, R, K = F.read_ssint_params(os.path.join(d, "veryFine.lj"))
+    assert E.shape == (7, 7) and E[0, 0] == 2.537e17 * U.MESO_AREA * U.MESO_AREA / U.MESO_ENERGY
+    assert R[0, 0] == 1e-9 / U.MESO_LENGTH and R[1, 2] == 1e-10 / U.MESO_LENGTH and (K == 0).all()
+    Eg, Rg, _ = F.read_ssint_params(os.path.join(d, "veryFine.lj"), "gensoft", 3e-9 / U.MESO_LENGTH)
+    assert np.allclose(Rg, 3e-9 / U.MESO_LENGTH, rtol=1e-15) and (Eg == E).all()
+    m = F.load_blob(*(os.path.join(d, "veryFine." + x) for x in ("node", "top", "mat", "stokes")))
+    surf = F.read_surface(os.path.join(d, "veryFine.surf"))
+    opp = F.surface_opposite(surf, m.topo)
+    for (e, *nodes), o in zip(surf, opp):
+        assert o in m.topo[e, :4] and o not in nodes
+    with pytest.raises(F.FormatError):
+        F.read_ssint_types(os.path.join(d, "veryFine_topface0.vdw"), 1000)
