@@ -94,13 +94,8 @@ def test_stokes_diffusion_sphere():
 
 
 def fm_columns(path):
-    """readAndAverage's parse of a measurement file (tests/physics/sphere_63_120_mass/
-    averages.py:27-41): column name first, then the values; RESTART lines skipped."""
-    lines = open(path).read().split("\n")
-    k = lines.index("Measurements:")
-    names = lines[k + 1].split()
-    rows = [[float(v) for v in l.split()] for l in lines[k + 2:] if l.strip() and "RESTART" not in l]
-    return {n: [n] + [r[i] for r in rows] for i, n in enumerate(names)}
+    from ffea_amd.analysis import read_measurements
+    return read_measurements(path)
 
 
 def test_mass_equipartition_sphere_63_120(tmp_path):
@@ -111,14 +106,13 @@ def test_mass_equipartition_sphere_63_120(tmp_path):
     w = World(str(run / "sphere_63_120_mass.ffea"))
     assert w.solver == "CG" and w.mass_in_system
     w.run()
+    from ffea_amd.analysis import equipartition
     H = fm_columns(run / "sphere_63_120_mass_measurement.out")
-    nodes, kT = 63, 4.11e-21
-    theory = {"KineticEnergy": kT * (3 * nodes) / 2, "StrainEnergy": kT * (3 * nodes - 6) / 2}
+    errs = equipartition(H, nodes=63, kT=4.11e-21, start=39)  # the checker's H rows start with the name
     tol = {"KineticEnergy": 0.04, "StrainEnergy": 0.03}
-    for k in theory:
-        err = abs(np.mean(H[k][40:-1]) / theory[k] - 1)
-        print(f"{k}: rel err {err:.4f} (tolerance {tol[k]})", flush=True)
-        assert err < tol[k], k
+    for k in tol:
+        print(f"{k}: rel err {errs[k]:.4f} (tolerance {tol[k]})", flush=True)
+        assert errs[k] < tol[k], k
 
 
 def test_inertial_diffusion_sphere(oracle):
@@ -183,23 +177,25 @@ def cylinder_run(tmp_path, kind):
     w = World(str(run / f"cyl_160_fine-{kind}.ffea"))
     assert w.calc_ctforces
     w.run()
-    frames = read_trajectory(str(run / "cyl_160_fine_trajectory.ftj"))
-    cm = [f[1][0][CYL_ENDNODES, :3].mean(axis=0) for f in (frames[0], frames[-1])]
-    return float(np.sqrt(((cm[1] - cm[0]) ** 2).sum())), len(frames)
+    from ffea_amd.analysis import end_displacement
+    path = str(run / "cyl_160_fine_trajectory.ftj")
+    return end_displacement(path, CYL_ENDNODES), len(read_trajectory(path))
 
 
 def test_youngs_modulus_cylinder(tmp_path):
+    from ffea_amd.analysis import youngs_modulus
     dL, nf = cylinder_run(tmp_path, "E")
     F, L, r, E_t = 1e-11, 160e-9, 10e-9, 6e8
-    E_c = F * L / (np.pi * r ** 2 * dL)
+    E_c = youngs_modulus(dL, F, L, r)
     print(f"Young's modulus: E_c = {E_c:.5e} Pa over {nf} frames, E_c/E_t = {E_c / E_t:.5f}", flush=True)
     assert abs(E_c / E_t - 1) <= 0.005
 
 
 def test_flexural_rigidity_cylinder(tmp_path):
+    from ffea_amd.analysis import flexural_rigidity
     dL, nf = cylinder_run(tmp_path, "EI")
     F, L, r, E_t = 1e-12, 160e-9, 10e-9, 6e8
-    EI_c, EI_t = F * L ** 3 / (3 * dL), E_t * np.pi * r ** 4 / 4.0
+    EI_c, EI_t = flexural_rigidity(dL, F, L), E_t * np.pi * r ** 4 / 4.0
     print(f"flexural rigidity: EI_c/EI_t = {EI_c / EI_t:.5f} over {nf} frames", flush=True)
     assert abs(EI_c / EI_t - 1) <= 0.1
 
@@ -213,7 +209,7 @@ def test_cubes_lennard_jones_energy(tmp_path):
     shutil.copytree(os.path.join(GOLDEN, "physics", "cubes_lj"), run)
     w = World(str(run / "cube_veryFine.ffea"))
     w.run()
-    H = fm_columns(run / "veryFinefine1face1face.fm")
-    e = H["SurfSurfEnergy"][-1]
+    from ffea_amd.analysis import surface_energy
+    e = surface_energy(fm_columns(run / "veryFinefine1face1face.fm"))
     print(f"cubes_lj: SurfSurfEnergy {e:.6e} J, rel err {abs(e / -1.7870e-19 - 1):.4f}", flush=True)
     assert abs((e - -1.7870e-19) / -1.7870e-19) < 1e-2

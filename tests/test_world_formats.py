@@ -222,3 +222,23 @@ def test_ssint_files():
         assert o in m.topo[e, :4] and o not in nodes
     with pytest.raises(F.FormatError):
         F.read_ssint_types(os.path.join(d, "veryFine_topface0.vdw"), 1000)
+
+
+def test_analysis_on_reference_files():
+    """The checker functions (ffea_amd.analysis) on the reference's own ffeatools fixtures:
+    the measurement file's columns and the trajectory's frame centroids; and the closed forms
+    of the checkers' formulas."""
+    from ffea_amd import analysis as A
+    H = A.read_measurements(os.path.join(FMT, "tet_ascii.1.fm"))
+    assert list(H)[:2] == ["Time", "StrainEnergy"] and H["Time"][0] == 1e-14
+    assert A.surface_energy(H) == H["SurfSurfEnergy"][-1]
+    c = A.frame_centroids(os.path.join(FMT, "tet_ascii.1.ftj"))
+    assert c.shape[1] == 3 and np.isfinite(c).all()
+    # a straight walk of equal steps: <dr^2> is the step length squared
+    cs = np.arange(5)[:, None] * np.array([[3e-9, 4e-9, 0.0]])
+    r2, th, _ = A.stokes_diffusion(cs, 4.11e-21, 1e-3, 2.0, 1e-9, 1e-14, 100)
+    assert r2 == pytest.approx(25e-18) and th == pytest.approx(6 * 4.11e-21 / (6 * np.pi * 1e-3 * 2e-9) * 1e-12)
+    assert A.youngs_modulus(1e-9, 1e-11, 160e-9, 10e-9) == pytest.approx(1e-11 * 160e-9 / (np.pi * 1e-16 * 1e-9))
+    assert A.flexural_rigidity(2e-9, 1e-12, 160e-9) == pytest.approx(1e-12 * (160e-9) ** 3 / 6e-9)
+    e = A.equipartition({"StrainEnergy": np.full(50, 4.11e-21 * 183 / 2)}, 63, 4.11e-21)
+    assert e["StrainEnergy"] == pytest.approx(0.0, abs=1e-15)
