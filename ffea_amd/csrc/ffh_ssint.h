@@ -249,6 +249,22 @@ __device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
 
 constexpr int kSsCap = 1024;  // candidates below f a wave sorts in LDS
 
+// acc += column k of the first e staged contributions, in order.  A pair that does not
+// interact stages +0.0, and adding +0.0 leaves every sum unchanged: a face force starts at +0.0
+// and never becomes -0.0 (in the reference either), so the loop needs no test and its loads
+// pipeline.
+__device__ __forceinline__ void ss_ordered_add(const double (*con)[9], int k, int e, double &acc) {
+    int q = 0;
+    for (; q + 8 <= e; q += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = con[q + u][k];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u];
+    }
+    for (; q < e; q++) acc += con[q][k];
+}
+
 // Every step: face f's Face::force as VdW_solver::solve leaves it, one wavefront per face.
 // The solve visits pool entries i in order, each against the stacks of its 27 neighbour cells
 // (top down: descending pool index) with i < j; face f therefore first receives, as face 2,
@@ -261,7 +277,6 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     __shared__ int cand[kSsCap];
     __shared__ int ncand;
     __shared__ double con[64][9];
-    __shared__ int use[64];
     const int f = blockIdx.x, lane = threadIdx.x;
     if (f >= a.P) return;
     if (lane < 27) {
@@ -329,43 +344,25 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
             }
         for (int base = 0; base < na; base += 64) {
             const int t = base + lane;
-            int ok = 0;
             double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            if (t < na && ss_consider(a, cand[t], f)) {
-                ss_pair(a, cand[t], f, 1, c9, nullptr);
-                ok = 1;
-            }
-            use[lane] = ok;
+            if (t < na && ss_consider(a, cand[t], f)) ss_pair(a, cand[t], f, 1, c9, nullptr);
             for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
             __syncthreads();
-            if (lane < 9) {
-                const int e = min(64, na - base);
-                for (int q = 0; q < e; q++)
-                    if (use[q]) acc += con[q][lane];
-            }
+            if (lane < 9) ss_ordered_add(con, lane, min(64, na - base), acc);
             __syncthreads();
         }
     }
     // (b) as face 1: its own pairs in visiting order
     for (int base = 0; base < n; base += 64) {
         const int t = base + lane;
-        int ok = 0;
         double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (t < n) {
             const int j = visit(t);
-            if (j > f && ss_consider(a, f, j)) {
-                ss_pair(a, f, j, 0, c9, nullptr);
-                ok = 1;
-            }
+            if (j > f && ss_consider(a, f, j)) ss_pair(a, f, j, 0, c9, nullptr);
         }
-        use[lane] = ok;
         for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
         __syncthreads();
-        if (lane < 9) {
-            const int e = min(64, n - base);
-            for (int q = 0; q < e; q++)
-                if (use[q]) acc += con[q][lane];
-        }
+        if (lane < 9) ss_ordered_add(con, lane, min(64, n - base), acc);
         __syncthreads();
     }
     if (lane < 9) a.out[9 * (size_t)f + lane] = acc;
