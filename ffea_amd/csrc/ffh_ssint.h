@@ -440,15 +440,16 @@ __global__ __launch_bounds__(64) void k_ss_pair_energy(SsArgs a, int nblobs, con
     }
 }
 
-// One wavefront: the lanes stage 64 pairs at a time in LDS, lane 0 adds them in order.  A
+// One wavefront: the lanes stage 512 pairs at a time in LDS, lane 0 adds them in order.  A
 // blob pair's running sum stays in a register while consecutive pairs share it and is written
 // back when the blob pair changes -- the same additions as fieldenergy[b1][b2] += e.
 constexpr int kSsFeLds = 4096;  // blob pairs kept in LDS (64 blobs); more stay in global memory
+constexpr int kSsStage = 512;   // pairs staged in LDS per round
 __global__ __launch_bounds__(64) void k_ss_energy_sum(int npairs, const double *pe, const int *pb, int nblobs,
                                                        double *fe, double *total) {
     __shared__ double fl[kSsFeLds];
-    __shared__ double ce[64];
-    __shared__ int cb[64];
+    __shared__ double ce[kSsStage];
+    __shared__ int cb[kSsStage];
     const int lane = threadIdx.x, nf = nblobs * nblobs;
     const bool in_lds = nf <= kSsFeLds;
     double *f = in_lds ? fl : fe;
@@ -456,15 +457,34 @@ __global__ __launch_bounds__(64) void k_ss_energy_sum(int npairs, const double *
     __syncthreads();
     int key = -1;
     double run = 0.0;
-    for (int base = 0; base < npairs; base += 64) {
-        if (base + lane < npairs) {
-            ce[lane] = pe[base + lane];
-            cb[lane] = pb[base + lane];
+    for (int base = 0; base < npairs; base += kSsStage) {
+        for (int t = lane; t < kSsStage && base + t < npairs; t += 64) {
+            ce[t] = pe[base + t];
+            cb[t] = pb[base + t];
         }
         __syncthreads();
         if (lane == 0) {
-            const int e = min(64, npairs - base);
-            for (int q = 0; q < e; q++) {
+            const int e = min(kSsStage, npairs - base);
+            int q = 0;
+            for (; q + 16 <= e; q += 16) {  // sixteen staged pairs at a time into registers
+                double v[16];
+                int kb[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    v[u] = ce[q + u];
+                    kb[u] = cb[q + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    if (kb[u] != key) {
+                        if (key >= 0) f[key] = run;
+                        key = kb[u];
+                        run = f[key];
+                    }
+                    run += v[u];
+                }
+            }
+            for (; q < e; q++) {
                 if (cb[q] != key) {
                     if (key >= 0) f[key] = run;
                     key = cb[q];
