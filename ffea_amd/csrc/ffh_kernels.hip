@@ -1716,18 +1716,18 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
         return rc;
     }
     const ffh::Plan &Pl = b->plan;
+    std::vector<int32_t> mid_of(Pl.num_nodes_total, -1);  // N_t id -> midpoint record
+    for (size_t m = 0; m < Pl.mid_index.size(); m++) mid_of[Pl.mid_index[m]] = (int32_t)m;
     std::vector<int32_t> ends(6 * (size_t)P), blob(P);
     for (int p = 0; p < P; p++) {
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 3; k++) {  // build_forces has checked every corner is in an element
             const int v = pool_nodes[3 * p + k];
             int a = Pl.lin_of[v], c = a;
-            if (a < 0)
-                for (size_t m = 0; m < Pl.mid_index.size(); m++)
-                    if (Pl.mid_index[m] == v) {
-                        a = Pl.lin_of[Pl.mid_a[m]];
-                        c = Pl.lin_of[Pl.mid_b[m]];
-                        break;
-                    }
+            if (a < 0) {
+                const int m = mid_of[v];
+                a = Pl.lin_of[Pl.mid_a[m]];
+                c = Pl.lin_of[Pl.mid_b[m]];
+            }
             ends[6 * (size_t)p + 2 * k] = a;
             ends[6 * (size_t)p + 2 * k + 1] = c;
         }
