@@ -415,8 +415,7 @@ class World:
     def _surface_interactions(self, s):
         """Surface-surface interactions as World::init sets them up (src/World.cpp:273-280,
         896-970; SimulationParams checks, src/SimulationParams.cpp:626-660): the faces of the
-        blobs' .vdw / .ssint files that are active (type != -1), Lennard-Jones on the device.
-        The SurfSurfEnergy column is written whenever calc_ssint or calc_steric is on (the
+        blobs' .vdw / .ssint files that are active (type != -1).  The SurfSurfEnergy column is written whenever calc_ssint or calc_steric is on (the
         defaults: ljsteric with steric on).  lennard-jones, ljsteric and gensoft run on the
         device; the steric volume exclusion (alone, or where ljsteric / gensoft face elements
         overlap) is not computed.  Periodic blob corrections (force_pbc) and sticky walls are
@@ -460,7 +459,7 @@ class World:
                 raise F.FormatError(f"blob {b}: surface-surface interactions need its surface and vdw files")
             surf = F.read_surface(blob["surface"])
             fc = surf[:, 1:4]
-            if kind != "lennard-jones":
+            if calc_ssint == 1 and kind != "lennard-jones":  # Face::n[3], read by their pair tests
                 opposite.append(F.surface_opposite(surf, self.meshes[b].topo) + off[b])
             ty = F.read_ssint_types(path, fc.shape[0])
             if calc_ssint == 1:
