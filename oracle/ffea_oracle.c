@@ -37,6 +37,14 @@
  *                              src/SparseMatrixFixedPattern.cpp:48-68, :85-93,
  *                              :148-191, :325-333; src/SparseMatrixTypes.cpp:52-58
  *   step order                 src/World.cpp:1811-2043 (periodic box :1850-1901)
+ *   mass-ful solvers           src/ConjugateGradientSolver.cpp:62-230,
+ *                              src/MassLumpedSolver.cpp:40-90, src/Blob.cpp:1385-1407
+ *                              (kinetic energy)
+ *   forces outside elements    src/Blob.cpp:2094-2162 (CTF 1/2/3), :3350-3424
+ *                              (surface forces); src/World.cpp:3808-3846 (springs)
+ *   surface interactions       src/VdW_solver.cpp:31-73, :158-384, :635-661,
+ *                              :713-784; src/LinkedListCube.tpp:282-299;
+ *                              src/Face.cpp:197-261; src/Blob.cpp:3494-3578
  *
  * Layout is the reference's: all N_t nodes (linear + quadratic midpoints),
  * 10-node elements, a scalar CSR over 3*N_t rows whose every entry keeps the
@@ -45,8 +53,12 @@
  * Parity pins (tests/): RngStream known-answer sum 39.697547445251
  * (tests/consistency/rngstream/testRngStream.cpp); sequential-vs-jump stream
  * equality; bitwise equality against the reference's own compiled objects
- * (oracle/_ref) for elements, assembled matrix, CG solutions and trajectories;
- * closed forms (unit tetrahedron, rest state stress-free, rigid motions).
+ * (oracle/_ref) for elements, assembled matrix, CG solutions (massless and
+ * mass-ful), trajectories, and the rotational force (intersectingPointToLine);
+ * closed forms (unit tetrahedron, rest state stress-free, rigid motions, the
+ * momentum and energy sums of the interaction forces, LJ minimum at Rmin,
+ * soft-core continuity at Rmin); the manual's ctforces example; the reference's
+ * cubes_lj golden energy (-1.7870e-19 J within 1 %).
  */
 #include <float.h>
 #include <math.h>
