@@ -269,13 +269,16 @@ __device__ __forceinline__ void ss_ordered_add(const double (*con)[9], int k, in
 // The solve visits pool entries i in order, each against the stacks of its 27 neighbour cells
 // (top down: descending pool index) with i < j; face f therefore first receives, as face 2,
 // the pairs (i, f) for i < f in ascending i, then, as face 1, its own pairs (f, j) in its
-// visiting order.  The lanes evaluate 64 pairs at a time; the contributions then go through
-// LDS and are added one by one in that order by lanes 0..8 (one force component each), so
-// every sum has the reference's order.
+// visiting order.  The pairs that pass consider_interaction join a queue in that order (a
+// ballot per 64 candidates); the lanes evaluate 64 queued pairs at a time, and the
+// contributions then go through LDS and are added one by one in queue order by lanes 0..8 (one
+// force component each), so every sum has the reference's order.  Pairs that do not pass add
+// nothing in the reference and are left out here.
 __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     __shared__ int rs[27], re[27], pre[28];
     __shared__ int cand[kSsCap];
     __shared__ int ncand;
+    __shared__ int queue[128];  // pending pairs: the other face j for (f, j), ~i for (i, f)
     __shared__ double con[64][9];
     const int f = blockIdx.x, lane = threadIdx.x;
     if (f >= a.P) return;
@@ -297,6 +300,7 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     __syncthreads();
     const int n = pre[27];
     double acc = 0.0;  // lane k < 9: component k of the face force (corner k / 3, axis k % 3)
+    int nq = 0;        // queued pairs (the same in every lane)
     // the candidate at visiting position t: cell c, k-th from the top of its stack
     auto visit = [&](int t) {
         int c = 0;
@@ -342,28 +346,50 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
                 }
                 __syncthreads();
             }
-        for (int base = 0; base < na; base += 64) {
-            const int t = base + lane;
+    }
+    // the queue: part (a) chunks of 64 sorted candidates, then part (b) chunks of 64 visiting
+    // positions, then one more pass that evaluates what is left
+    const int chunks_a = na > kSsCap ? 0 : (na + 63) / 64, chunks = chunks_a + (n + 63) / 64;
+    for (int ch = 0; ch <= chunks; ch++) {
+        int i1 = -1, i2 = -1;  // the pair (i1 < i2) this lane queues, if it passes
+        if (ch < chunks_a) {
+            const int t = ch * 64 + lane;
+            if (t < na) {
+                i1 = cand[t];
+                i2 = f;
+            }
+        } else if (ch < chunks) {
+            const int t = (ch - chunks_a) * 64 + lane;
+            if (t < n) {
+                const int j = visit(t);
+                if (j > f) {
+                    i1 = f;
+                    i2 = j;
+                }
+            }
+        }
+        const bool pass = i1 >= 0 && ss_consider(a, i1, i2);
+        const unsigned long long m = __ballot(pass);
+        if (pass) queue[nq + __popcll(m & ((1ull << lane) - 1))] = i2 == f ? ~i1 : i2;
+        nq += __popcll(m);
+        __syncthreads();
+        const int e = ch == chunks ? nq : (nq >= 64 ? 64 : 0);
+        if (e > 0) {  // evaluate the first e queued pairs, one per lane; add them in queue order
             double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            if (t < na && ss_consider(a, cand[t], f)) ss_pair(a, cand[t], f, 1, c9, nullptr);
+            if (lane < e) {
+                const int o = queue[lane];
+                const bool second = o < 0;
+                ss_pair(a, second ? ~o : f, second ? f : o, second ? 1 : 0, c9, nullptr);
+            }
             for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
             __syncthreads();
-            if (lane < 9) ss_ordered_add(con, lane, min(64, na - base), acc);
+            if (lane < 9) ss_ordered_add(con, lane, e, acc);
+            nq -= e;
+            const int carry = lane < nq ? queue[64 + lane] : 0;
+            __syncthreads();
+            if (lane < nq) queue[lane] = carry;
             __syncthreads();
         }
-    }
-    // (b) as face 1: its own pairs in visiting order
-    for (int base = 0; base < n; base += 64) {
-        const int t = base + lane;
-        double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (t < n) {
-            const int j = visit(t);
-            if (j > f && ss_consider(a, f, j)) ss_pair(a, f, j, 0, c9, nullptr);
-        }
-        for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
-        __syncthreads();
-        if (lane < 9) ss_ordered_add(con, lane, min(64, n - base), acc);
-        __syncthreads();
     }
     if (lane < 9) a.out[9 * (size_t)f + lane] = acc;
 }
