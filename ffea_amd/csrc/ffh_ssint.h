@@ -249,6 +249,22 @@ __device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
 
 constexpr int kSsCap = 1024;  // candidates below f a wave sorts in LDS
 
+// The candidate at visiting position t < pre[27] of a face whose 27 neighbour cells end at
+// re[c] in the sorted pool and start at visiting position pre[c]: the first cell c with
+// pre[c + 1] > t (a binary search; empty cells have pre[c] == pre[c + 1]), k-th from the top of
+// its stack.
+__device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const int *re, int t) {
+    int lo = 0, hi = 26;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid + 1] > t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return a.sorted[re[lo] - 1 - (t - pre[lo])];
+}
+
 // acc += column k of the first e staged contributions, in order.  A pair that does not
 // interact stages +0.0, and adding +0.0 leaves every sum unchanged: a face force starts at +0.0
 // and never becomes -0.0 (in the reference either), so the loop needs no test and its loads
@@ -301,15 +317,9 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     const int n = pre[27];
     double acc = 0.0;  // lane k < 9: component k of the face force (corner k / 3, axis k % 3)
     int nq = 0;        // queued pairs (the same in every lane)
-    // the candidate at visiting position t: cell c, k-th from the top of its stack
-    auto visit = [&](int t) {
-        int c = 0;
-        while (pre[c + 1] <= t) c++;
-        return a.sorted[re[c] - 1 - (t - pre[c])];
-    };
     // (a) as face 2: the candidates below f, sorted ascending
     for (int t = lane; t < n; t += 64) {
-        const int i = visit(t);
+        const int i = ss_visit(a, pre, re, t);
         if (i < f) {
             const int slot = atomicAdd(&ncand, 1);
             if (slot < kSsCap) cand[slot] = i;
@@ -361,7 +371,7 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
         } else if (ch < chunks) {
             const int t = (ch - chunks_a) * 64 + lane;
             if (t < n) {
-                const int j = visit(t);
+                const int j = ss_visit(a, pre, re, t);
                 if (j > f) {
                     i1 = f;
                     i2 = j;
@@ -400,12 +410,6 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
 // the pairs' sums are added row by row.  k_ss_pair_count counts each face's own pairs (one
 // wavefront per face), k_ss_pair_energy writes their energies in visiting order at the face's
 // offset, and k_ss_energy_sum adds them up in that order on one thread.
-__device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const int *re, int t) {
-    int c = 0;
-    while (pre[c + 1] <= t) c++;
-    return a.sorted[re[c] - 1 - (t - pre[c])];
-}
-
 __device__ __forceinline__ void ss_ranges(const SsArgs &a, int f, int lane, int *rs, int *re, int *pre) {
     if (lane < 27) {
         const int cc = ss_neighbour(a, a.cell[f], lane);
