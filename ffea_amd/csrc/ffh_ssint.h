@@ -247,7 +247,7 @@ __device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
     }
 }
 
-constexpr int kSsCap = 1024;  // candidates below f a wave sorts in LDS
+constexpr int kSsCap = 1024;  // candidates below f a wave sorts (in registers, 16 per lane)
 
 // The candidate at visiting position t < pre[27] of a face whose 27 neighbour cells end at
 // re[c] in the sorted pool and start at visiting position pre[c]: the first cell c with
@@ -263,6 +263,42 @@ __device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const i
             lo = mid + 1;
     }
     return a.sorted[re[lo] - 1 - (t - pre[lo])];
+}
+
+// Sorts cand[0, na) ascending, 64 * R >= na: a bitonic network over R * 64 entries held R per
+// lane (entry r * 64 + lane in v[r], the tail padded with INT_MAX); exchanges across lanes are
+// shuffles, those across a lane's own entries plain swaps.
+template <int R>
+__device__ __forceinline__ void ss_sort_cand(int *cand, int na, int lane) {
+    int v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = r * 64 + lane < na ? cand[r * 64 + lane] : INT_MAX;
+#pragma unroll
+    for (int k = 2; k <= 64 * R; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int u = r ^ (j >> 6);
+                    if (u > r) {
+                        const bool up = ((r * 64 + lane) & k) == 0;
+                        const int x = v[r], y = v[u];
+                        v[r] = up ? min(x, y) : max(x, y);
+                        v[u] = up ? max(x, y) : min(x, y);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int y = __shfl_xor(v[r], j);
+                    const bool up = ((r * 64 + lane) & k) == 0, low = (lane & j) == 0;
+                    v[r] = low == up ? min(v[r], y) : max(v[r], y);
+                }
+            }
+        }
+#pragma unroll
+    for (int r = 0; r < R; r++) cand[r * 64 + lane] = v[r];
 }
 
 // acc += column k of the first e staged contributions, in order.  A pair that does not
@@ -336,26 +372,17 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
         __syncthreads();
         if (lane < 9) acc = con[0][lane];
         __syncthreads();
-    } else {
-        int m = 1;
+    } else if (na > 1) {
+        int m = 64;
         while (m < na) m <<= 1;
-        for (int t = na + lane; t < m; t += 64) cand[t] = INT_MAX;
+        switch (m) {
+            case 64: ss_sort_cand<1>(cand, na, lane); break;
+            case 128: ss_sort_cand<2>(cand, na, lane); break;
+            case 256: ss_sort_cand<4>(cand, na, lane); break;
+            case 512: ss_sort_cand<8>(cand, na, lane); break;
+            default: ss_sort_cand<16>(cand, na, lane); break;
+        }
         __syncthreads();
-        for (int k = 2; k <= m; k <<= 1)  // bitonic sort, ascending
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int t = lane; t < m; t += 64) {
-                    const int u = t ^ j;
-                    if (u > t) {
-                        const int x = cand[t], y = cand[u];
-                        const bool up = (t & k) == 0;
-                        if ((x > y) == up) {
-                            cand[t] = y;
-                            cand[u] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
     }
     // the queue: part (a) chunks of 64 sorted candidates, then part (b) chunks of 64 visiting
     // positions, then one more pass that evaluates what is left
