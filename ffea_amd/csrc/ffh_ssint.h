@@ -249,11 +249,10 @@ __device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
 
 constexpr int kSsCap = 1024;  // candidates below f a wave sorts (in registers, 16 per lane)
 
-// The candidate at visiting position t < pre[27] of a face whose 27 neighbour cells end at
-// re[c] in the sorted pool and start at visiting position pre[c]: the first cell c with
-// pre[c + 1] > t (a binary search; empty cells have pre[c] == pre[c + 1]), k-th from the top of
-// its stack.
-__device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const int *re, int t) {
+// The cell of position t < pre[27] when the 27 neighbour cells' runs are laid end to end, run
+// c starting at pre[c]: the first c with pre[c + 1] > t (a binary search; an empty run has
+// pre[c] == pre[c + 1]).
+__device__ __forceinline__ int ss_cell_at(const int *pre, int t) {
     int lo = 0, hi = 26;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -262,7 +261,14 @@ __device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const i
         else
             lo = mid + 1;
     }
-    return a.sorted[re[lo] - 1 - (t - pre[lo])];
+    return lo;
+}
+
+// The candidate at visiting position t: the runs are the tops of the stacks (cells ending at
+// re[c] in the sorted pool), visited top down.
+__device__ __forceinline__ int ss_visit(const SsArgs &a, const int *pre, const int *re, int t) {
+    const int c = ss_cell_at(pre, t);
+    return a.sorted[re[c] - 1 - (t - pre[c])];
 }
 
 // Sorts cand[0, na) ascending, 64 * R >= na: a bitonic network over R * 64 entries held R per
@@ -327,42 +333,50 @@ __device__ __forceinline__ void ss_ordered_add(const double (*con)[9], int k, in
 // force component each), so every sum has the reference's order.  Pairs that do not pass add
 // nothing in the reference and are left out here.
 __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
-    __shared__ int rs[27], re[27], pre[28];
+    __shared__ int rs[27], re[27], pa[28], pb[28];
     __shared__ int cand[kSsCap];
-    __shared__ int ncand;
     __shared__ int queue[128];  // pending pairs: the other face j for (f, j), ~i for (i, f)
     __shared__ double con[64][9];
     const int f = blockIdx.x, lane = threadIdx.x;
     if (f >= a.P) return;
     if (lane < 27) {
+        // the cell's stack sorted[s0, s1) ascends by pool index: the entries below f, then f
+        // itself (in its own cell), then the entries above f
         const int cc = ss_neighbour(a, a.cell[f], lane);
-        rs[lane] = a.cstart[cc];
-        re[lane] = a.cend[cc];
+        const int s0 = a.cstart[cc], s1 = a.cend[cc];
+        int lo = s0, hi = s1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.sorted[mid] < f)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        rs[lane] = s0;
+        re[lane] = s1;
+        pa[lane + 1] = lo - s0;  // counts here, starts after the prefix sums
+        pb[lane + 1] = s1 - (lo < s1 && a.sorted[lo] == f ? lo + 1 : lo);
     }
-    if (lane == 0) ncand = 0;
     __syncthreads();
     if (lane == 0) {
-        int t = 0;
+        pa[0] = pb[0] = 0;
         for (int c = 0; c < 27; c++) {
-            pre[c] = t;
-            t += re[c] - rs[c];
+            pa[c + 1] += pa[c];
+            pb[c + 1] += pb[c];
         }
-        pre[27] = t;
     }
     __syncthreads();
-    const int n = pre[27];
+    const int na = pa[27], nb = pb[27];
     double acc = 0.0;  // lane k < 9: component k of the face force (corner k / 3, axis k % 3)
     int nq = 0;        // queued pairs (the same in every lane)
-    // (a) as face 2: the candidates below f, sorted ascending
-    for (int t = lane; t < n; t += 64) {
-        const int i = ss_visit(a, pre, re, t);
-        if (i < f) {
-            const int slot = atomicAdd(&ncand, 1);
-            if (slot < kSsCap) cand[slot] = i;
+    // (a) as face 2: the candidates below f (the bottoms of the stacks), sorted ascending
+    if (na <= kSsCap) {
+        for (int t = lane; t < na; t += 64) {
+            const int c = ss_cell_at(pa, t);
+            cand[t] = a.sorted[rs[c] + (t - pa[c])];
         }
+        __syncthreads();
     }
-    __syncthreads();
-    const int na = ncand;
     if (na > kSsCap) {  // too many to sort here: one lane merges
         double acc9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (lane == 0) {
@@ -386,7 +400,7 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     }
     // the queue: part (a) chunks of 64 sorted candidates, then part (b) chunks of 64 visiting
     // positions, then one more pass that evaluates what is left
-    const int chunks_a = na > kSsCap ? 0 : (na + 63) / 64, chunks = chunks_a + (n + 63) / 64;
+    const int chunks_a = na > kSsCap ? 0 : (na + 63) / 64, chunks = chunks_a + (nb + 63) / 64;
     for (int ch = 0; ch <= chunks; ch++) {
         int i1 = -1, i2 = -1;  // the pair (i1 < i2) this lane queues, if it passes
         if (ch < chunks_a) {
@@ -397,12 +411,9 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
             }
         } else if (ch < chunks) {
             const int t = (ch - chunks_a) * 64 + lane;
-            if (t < n) {
-                const int j = ss_visit(a, pre, re, t);
-                if (j > f) {
-                    i1 = f;
-                    i2 = j;
-                }
+            if (t < nb) {  // the tops of the stacks, in visiting order
+                i1 = f;
+                i2 = ss_visit(a, pb, re, t);
             }
         }
         const bool pass = i1 >= 0 && ss_consider(a, i1, i2);
