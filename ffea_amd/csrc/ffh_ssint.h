@@ -508,7 +508,7 @@ __global__ __launch_bounds__(64) void k_ss_pair_energy(SsArgs a, int nblobs, con
     }
 }
 
-// One wavefront: the lanes stage 512 pairs at a time in LDS, lane 0 adds them in order.  A
+// One wavefront: the lanes stage 512 pairs at a time in LDS and all add them in order.  A
 // blob pair's running sum stays in a register while consecutive pairs share it and is written
 // back when the blob pair changes -- the same additions as fieldenergy[b1][b2] += e.
 constexpr int kSsFeLds = 4096;  // blob pairs kept in LDS (64 blobs); more stay in global memory
@@ -525,46 +525,63 @@ __global__ __launch_bounds__(64) void k_ss_energy_sum(int npairs, const double *
     __syncthreads();
     int key = -1;
     double run = 0.0;
+    // each round's pairs are loaded into registers during the previous round's sum
+    constexpr int kPer = kSsStage / 64;
+    double pv[kPer];
+    int pk[kPer];
+    auto fetch = [&](int base) {
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int t = base + u * 64 + lane;
+            pv[u] = t < npairs ? pe[t] : 0.0;
+            pk[u] = t < npairs ? pb[t] : 0;
+        }
+    };
+    fetch(0);
     for (int base = 0; base < npairs; base += kSsStage) {
-        for (int t = lane; t < kSsStage && base + t < npairs; t += 64) {
-            ce[t] = pe[base + t];
-            cb[t] = pb[base + t];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            ce[u * 64 + lane] = pv[u];
+            cb[u * 64 + lane] = pk[u];
         }
         __syncthreads();
-        if (lane == 0) {
-            const int e = min(kSsStage, npairs - base);
-            int q = 0;
-            for (; q + 16 <= e; q += 16) {  // sixteen staged pairs at a time into registers
-                double v[16];
-                int kb[16];
+        if (base + kSsStage < npairs) fetch(base + kSsStage);
+        // every lane adds the same staged pairs in the same order; the blob-pair key is read as
+        // a wave-uniform value, so the switches are scalar branches
+        const int e = min(kSsStage, npairs - base);
+        int q = 0;
+        for (; q + 16 <= e; q += 16) {  // sixteen staged pairs at a time into registers
+            double v[16];
+            int kb[16];
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    v[u] = ce[q + u];
-                    kb[u] = cb[q + u];
-                }
-#pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    if (kb[u] != key) {
-                        if (key >= 0) f[key] = run;
-                        key = kb[u];
-                        run = f[key];
-                    }
-                    run += v[u];
-                }
+            for (int u = 0; u < 16; u++) {
+                v[u] = ce[q + u];
+                kb[u] = __builtin_amdgcn_readfirstlane(cb[q + u]);
             }
-            for (; q < e; q++) {
-                if (cb[q] != key) {
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                if (kb[u] != key) {
                     if (key >= 0) f[key] = run;
-                    key = cb[q];
+                    key = kb[u];
                     run = f[key];
                 }
-                run += ce[q];
+                run += v[u];
             }
+        }
+        for (; q < e; q++) {
+            const int kq = __builtin_amdgcn_readfirstlane(cb[q]);
+            if (kq != key) {
+                if (key >= 0) f[key] = run;
+                key = kq;
+                run = f[key];
+            }
+            run += ce[q];
         }
         __syncthreads();
     }
+    if (key >= 0) f[key] = run;  // the same value from every lane
+    __syncthreads();
     if (lane == 0) {
-        if (key >= 0) f[key] = run;
         double t = 0.0;
         for (int k = 0; k < nf; k++) t += f[k];
         *total = t;
