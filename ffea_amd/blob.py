@@ -19,17 +19,19 @@ PRECISIONS = {"fp64": _lib.FFH_PRECISION_FP64, "mixed": _lib.FFH_PRECISION_MIXED
 
 def package_seed(rng_seed: int) -> np.ndarray:
     """World::init RNG.2 (src/World.cpp:299-305): six RngStream seeds drawn from the host
-    libc's srand/rand; the int sum rand()+rand() wraps before the unsigned conversion.
+    libc's srand/rand, (rand() + rand()) % twoMax[i/3].
+
+    The int sum can exceed INT_MAX, which is undefined behaviour in C++. The reference's
+    optimised build widens both operands to the 64-bit type of twoMax before adding, so the
+    sum does not wrap: its own checkpoint tests/physics/cyl_160_fine/cyl_160_fine.fcp
+    (rng_seed 753, no draws made) holds exactly the streams of the non-wrapping sum.
     With one thread the first RngStream starts from exactly these seeds."""
     libc = C.CDLL(None)
     libc.srand(C.c_uint(rng_seed & 0xffffffff))
     two_max = (4294967087, 4294944443)
     out = np.zeros(6, np.uint64)
     for i in range(6):
-        s = (libc.rand() + libc.rand()) & 0xffffffff
-        if s >= 1 << 31:
-            s -= 1 << 32
-        out[i] = (s % (1 << 64)) % two_max[i // 3]
+        out[i] = (int(libc.rand()) + int(libc.rand())) % two_max[i // 3]
     return out
 
 

@@ -280,15 +280,16 @@ void orc_rng_advance_state(orc_stream *g, long e, long c) {
 }
 
 /* World::init RNG.2 (src/World.cpp:299-305): six seeds from glibc srand/rand.  The int
- * sum rand()+rand() wraps (two's complement), then converts to unsigned long. */
+ * sum rand()+rand() may exceed INT_MAX (undefined in C++); the reference's optimised build
+ * adds in the 64-bit type of twoMax, so the sum does not wrap.  The reference's checkpoint
+ * tests/physics/cyl_160_fine/cyl_160_fine.fcp (rng_seed 753) holds the non-wrapping seeds. */
 void orc_package_seed_from_rng_seed(int rng_seed, unsigned long sixseed[6]) {
     const unsigned long two_max[2] = {4294967087UL, 4294944443UL};
     srand((unsigned)rng_seed);
     for (int i = 0; i < 6; i++) {
-        unsigned r1 = (unsigned)rand();
-        unsigned r2 = (unsigned)rand();
-        int s = (int)(r1 + r2);
-        sixseed[i] = ((unsigned long)(long)s) % two_max[i / 3];
+        unsigned long r1 = (unsigned long)rand();
+        unsigned long r2 = (unsigned long)rand();
+        sixseed[i] = (r1 + r2) % two_max[i / 3];
     }
 }
 

@@ -96,13 +96,16 @@ void ref_rng_get_state(void *g, unsigned long out[6]) { static_cast<RngStream *>
 int ref_rng_set_seed(void *g, const unsigned long seed[6]) { return static_cast<RngStream *>(g)->SetSeed(seed) ? 1 : 0; }
 void ref_rng_advance_state(void *g, long e, long c) { static_cast<RngStream *>(g)->AdvanceState(e, c); }
 
-/* World::init RNG.2 (src/World.cpp:299-305), verbatim semantics with the host libc rand(). */
+/* World::init RNG.2 (src/World.cpp:299-305) with the host libc rand().  The reference's int
+ * sum rand()+rand() is undefined when it exceeds INT_MAX; its optimised build adds in 64 bits
+ * (the type of twoMax), which is what its checkpoint fixture cyl_160_fine.fcp holds. */
 void ref_package_seed_from_rng_seed(int rng_seed, unsigned long sixseed[6]) {
     unsigned long twoMax[2] = {4294967087UL, 4294944443UL};
     srand(rng_seed);
     for (int i = 0; i < 6; i++) {
-        int s = (int)((unsigned)rand() + (unsigned)rand());
-        sixseed[i] = ((unsigned long)s) % twoMax[i / 3];
+        unsigned long r1 = (unsigned long)rand();
+        unsigned long r2 = (unsigned long)rand();
+        sixseed[i] = (r1 + r2) % twoMax[i / 3];
     }
 }
 

@@ -81,6 +81,50 @@ def test_package_seed_matches_reference_fixture(oracle):
     assert np.array_equal(package_seed(44), gold["seed"])
 
 
+def _fcp_streams(path):
+    with open(path) as fh:
+        head = fh.readline()
+        n = int(head.split(":")[1])
+        return np.array([[int(t) for t in fh.readline().split()] for _ in range(n)], np.uint64)
+
+
+@pytest.mark.parametrize("who", ["product", "oracle", "reference"])
+def test_package_seed_reproduces_reference_checkpoint(oracle, who):
+    """The reference's own checkpoint tests/physics/cyl_160_fine/cyl_160_fine.fcp was written by
+    an FFEA run of cyl_160_fine-E.ffea (rng_seed = 753, calc_noise = 0, calc_stokes = 0: no draw
+    is ever made), with 8 threads, so its 8 lines are the 8 initial RngStreams of World::init
+    (src/World.cpp:299-313): package seed from srand(753)/rand(), then each stream 2^127 steps
+    after the previous one. All 48 integers must match."""
+    from ffea_amd.blob import package_seed
+    want = _fcp_streams(os.path.join(GOLDEN, "formats", "cyl_160_fine.fcp"))
+    assert want.shape == (8, 6)
+    if who == "product":
+        seed = package_seed(753)
+    elif who == "oracle":
+        seed = oracle.package_seed(753)
+    else:
+        import pyoracle
+        if not pyoracle.have_reference():
+            pytest.skip("oracle/_ref (the reference's compiled objects) is not built here")
+        seed = pyoracle.Reference().package_seed(753)
+    assert oracle.set_package_seed(seed)
+    got = np.array([oracle.stream().state() for _ in range(8)], np.uint64)
+    assert np.array_equal(got, want), (got[0], want[0])
+
+
+def test_package_seed_does_not_wrap():
+    """Seed 44 (every config and the reference's physics fixtures) has rand()+rand() above
+    INT_MAX in words 0 and 3: the seed must hold the 64-bit sum, not the wrapped int."""
+    import ctypes as C
+    from ffea_amd.blob import package_seed
+    libc = C.CDLL(None)
+    libc.srand(44)
+    sums = [int(libc.rand()) + int(libc.rand()) for _ in range(6)]
+    assert sums[0] > 2**31 - 1 and sums[3] > 2**31 - 1
+    two_max = (4294967087, 4294944443)
+    assert [int(v) for v in package_seed(44)] == [s % two_max[i // 3] for i, s in enumerate(sums)]
+
+
 def test_draws_and_jumps_match_reference_fixture(oracle):
     gold = np.load(os.path.join(GOLDEN, "rng_seed44.npz"))
     oracle.set_package_seed(gold["seed"])
