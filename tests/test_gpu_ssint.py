@@ -91,7 +91,11 @@ def run_both(oracle, g, bo, st, nsteps, exact=True):
         want = np.concatenate([o.get("rhs") for o in bo])
         if exact or k == 0:
             assert np.array_equal(rhs, want), k
-        assert e == oracle.ssint_energy(), k
+            assert e == oracle.ssint_energy(), k
+        else:
+            # the massless solver's positions after the first step agree to rounding only (CG
+            # dot products summed in tree order), so the energy of those positions does too
+            assert e == pytest.approx(oracle.ssint_energy(), rel=1e-9, abs=0), k
         g.step(1)  # and an ordinary step
         assert pyoracle.world_step(oracle, bo, st) == 0
         if exact:
