@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -91,6 +92,7 @@ struct ffh_blob {
     int maxit = 1000, calc_noise = 0, calc_stokes = 0;
     int N = 0, E = 0;
     size_t tsize = 8;
+    int device = 0;  // the HIP device current at creation
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     // device arrays
@@ -1150,8 +1152,8 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     b->blob_status.assign(b->nblobs, 0);
     for (int k = 0; k < b->nblobs; k++)
         if (P.blob_slice_off[k + 1] - P.blob_slice_off[k] > kTotSlicesMin) b->use_tot = true;
-    if (hipStreamCreate(&b->stream) != hipSuccess) {
-        fail(FFH_E_HIP, "hipStreamCreate failed");
+    if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) {
+        fail(FFH_E_HIP, "hipGetDevice / hipStreamCreate failed");
         return nullptr;
     }
     const int N = b->N, E = b->E;
@@ -1806,12 +1808,12 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
     return FFH_OK;
 }
 
-int ffh_blob_ssint_energy(ffh_blob *b, double *energy) {
-    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
-    if (!b->ss_on) {
-        *energy = 0.0;
-        return FFH_OK;
-    }
+}  // extern "C"
+
+// VdW_solver::get_field_energy over the last solve's pairs, with the pair energies and their
+// blob-pair keys in pe / pb (room for cap pairs).  grow: the handle's own buffers, enlarged
+// with hipMalloc when the pairs do not fit; otherwise more than cap pairs is an error.
+static int ssint_energy_impl(ffh_blob *b, double *energy, double *pe, int *pb, size_t cap, bool grow) {
     SsArgs a = b->ss;
     a.pos = b->pos;
     const int P = a.P;
@@ -1824,26 +1826,89 @@ int ffh_blob_ssint_energy(ffh_blob *b, double *energy) {
     HIP_OR_FAIL(hipMemcpyAsync(&last[1], b->ss_cnt + P - 1, sizeof(int), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     const size_t npairs = (size_t)last[0] + last[1];
-    if (npairs > b->ss_pair_cap) {
+    if (npairs > cap) {
+        if (!grow)
+            return fail(FFH_E_ARG, "the interaction energy has " + std::to_string(npairs) +
+                                       " face pairs, more than the no-allocation scratch holds (" +
+                                       std::to_string(cap) + ")");
         (void)hipFree(b->ss_pe);
         (void)hipFree(b->ss_pb);
         b->ss_pe = nullptr;
         b->ss_pb = nullptr;
         b->ss_pair_cap = 0;
-        const size_t cap = npairs + npairs / 4 + 1024;
-        if (hipMalloc((void **)&b->ss_pe, cap * sizeof(double)) != hipSuccess ||
-            hipMalloc((void **)&b->ss_pb, cap * sizeof(int)) != hipSuccess)
+        const size_t ncap = npairs + npairs / 4 + 1024;
+        if (hipMalloc((void **)&b->ss_pe, ncap * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&b->ss_pb, ncap * sizeof(int)) != hipSuccess)
             return fail(FFH_E_NOMEM, "hipMalloc (interaction pair energies)");
-        b->ss_pair_cap = cap;
+        b->ss_pair_cap = ncap;
+        pe = b->ss_pe;
+        pb = b->ss_pb;
     }
-    hipLaunchKernelGGL(k_ss_pair_energy, P, 64, 0, b->stream, a, b->nblobs, (const int *)b->ss_off, b->ss_pe, b->ss_pb);
+    hipLaunchKernelGGL(k_ss_pair_energy, P, 64, 0, b->stream, a, b->nblobs, (const int *)b->ss_off, pe, pb);
     HIP_OR_FAIL(hipGetLastError());
-    hipLaunchKernelGGL(k_ss_energy_sum, 1, 64, 0, b->stream, (int)npairs, (const double *)b->ss_pe,
-                       (const int *)b->ss_pb, b->nblobs, b->ss_fe, b->ss_total);
+    hipLaunchKernelGGL(k_ss_energy_sum, 1, 64, 0, b->stream, (int)npairs, (const double *)pe, (const int *)pb,
+                       b->nblobs, b->ss_fe, b->ss_total);
     HIP_OR_FAIL(hipGetLastError());
     HIP_OR_FAIL(hipMemcpyAsync(energy, b->ss_total, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return FFH_OK;
+}
+
+// Scratch of ffh_blob_ssint_energy_noalloc: the pair energies (FP64), then their blob-pair
+// keys (int32), of at most kNoallocPairs face pairs (12 MiB).  Each GPU holds its own copy
+// from the module's load to the end of the process; in host code the name is only a symbol,
+// so its device address is looked up once per device (noalloc_scratch) and used instead.
+constexpr size_t kNoallocPairs = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocPairs * (sizeof(double) + sizeof(int))];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up on the first call for that device with
+// that device current (the caller's current device is restored).
+int noalloc_scratch(int device, unsigned char **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(FFH_E_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        HIP_OR_FAIL(hipGetDevice(&cur));
+        if (cur != device) HIP_OR_FAIL(hipSetDevice(device));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) HIP_OR_FAIL(hipSetDevice(cur));
+        if (e != hipSuccess) return fail(FFH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return FFH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ffh_blob_ssint_energy(ffh_blob *b, double *energy) {
+    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
+    if (!b->ss_on) {
+        *energy = 0.0;
+        return FFH_OK;
+    }
+    return ssint_energy_impl(b, energy, b->ss_pe, b->ss_pb, b->ss_pair_cap, true);
+}
+
+int ffh_blob_ssint_energy_noalloc(ffh_blob *b, double *energy) {
+    if (!b || !energy) return fail(FFH_E_ARG, "null argument");
+    if (!b->ss_on) {
+        *energy = 0.0;
+        return FFH_OK;
+    }
+    unsigned char *ptr = nullptr;
+    int rc = noalloc_scratch(b->device, &ptr);
+    if (rc) return rc;
+    double *pe = reinterpret_cast<double *>(ptr);
+    int *pb = reinterpret_cast<int *>(ptr + kNoallocPairs * sizeof(double));
+    return ssint_energy_impl(b, energy, pe, pb, kNoallocPairs, false);
 }
 
 int ffh_blob_set_box(ffh_blob *b, const double *box, int periodic_mask) {

@@ -248,6 +248,19 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
 int ffh_blob_ssint_energy(ffh_blob *b, double *energy);
 
 /*
+ * ffh_blob_ssint_energy without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  ffh_blob_ssint_energy enlarges the handle's pair-energy
+ * buffers with hipMalloc when a solve has more face pairs than before; this variant instead
+ * keeps the pair energies and keys in one library-owned device array per GPU (12 MiB, held
+ * until the process ends), calls no hipMalloc / hipFree, and returns FFH_E_ARG when the last
+ * solve has more than 1,048,576 face pairs.  The result equals ffh_blob_ssint_energy's bit for
+ * bit.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+ * threads, or two handles on different streams) race and must be serialised by the caller.
+ */
+int ffh_blob_ssint_energy_noalloc(ffh_blob *b, double *energy);
+
+/*
  * World::run's periodic box (src/World.cpp:1850-1901; box_dim from World::init, :455-471).
  * With periodic_mask != 0, every step (ffh_blob_step) and every ffh_blob_update_internal_forces
  * begins as the reference's step does: for each blob, the centroid of its N_t nodes
