@@ -657,11 +657,12 @@ def parity_vs_reference(cfg, ms, prm, sc, xs, steps):
 def kernel_symbol(kernel, precision, tuning):
     """The kernel's name in rocprofv3 records for the launch shapes the tuner picked."""
     t = "double" if precision == "fp64" else "float"
-    u = tuning.get("assembly_unroll", 2)
+    u = tuning.get("assembly_tiles", 2)
+    waves = tuning.get("assembly_block", 512) // 64
     nt = "true" if tuning.get("spmv_nontemporal") else "false"
     spmv = "k_cg_spmv1" if tuning.get("spmv_waves_per_slice") == 1 else "k_cg_spmv"
     return {"element": f"k_element<{t}, {t}>",
-            "assemble": f"k_assemble2<{t}>" if u == 0 else f"k_assemble<{t}, {u}>",
+            "assemble": f"k_assemble_slice<{t}, {waves}, {u}, true>",
             "rhs": f"k_rhs<{t}>", "cg_spmv": f"{spmv}<{t}, {nt}>", "cg_update": "k_cg_update",
             "integrate": "k_integrate"}.get(kernel)
 

@@ -99,7 +99,7 @@ class Blob:
     def step(self, nsteps: int = 1) -> None:
         _lib.check(self._L.ffh_blob_step(self._h, int(nsteps)))
 
-    TUNING_KEYS = ("element_block", "assembly_block", "assembly_unroll", "spmv_waves_per_slice",
+    TUNING_KEYS = ("element_block", "assembly_block", "assembly_tiles", "spmv_waves_per_slice",
                    "spmv_nontemporal")
 
     def tuning(self) -> dict:
@@ -107,10 +107,10 @@ class Blob:
         _lib.check(self._L.ffh_blob_tuning(self._h, *[C.byref(x) for x in v]))
         return dict(zip(self.TUNING_KEYS, (x.value for x in v)))
 
-    def set_tuning(self, element_block: int, assembly_block: int, assembly_unroll: int,
+    def set_tuning(self, element_block: int, assembly_block: int, assembly_tiles: int,
                    spmv_waves_per_slice: int, spmv_nontemporal: int = 0) -> None:
         _lib.check(self._L.ffh_blob_set_tuning(self._h, int(element_block), int(assembly_block),
-                                               int(assembly_unroll), int(spmv_waves_per_slice),
+                                               int(assembly_tiles), int(spmv_waves_per_slice),
                                                int(spmv_nontemporal)))
 
     def set_external_forces(self, force_meso=None) -> None:

@@ -32,7 +32,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(obj_dir, exist_ok=True)
     plan_o = os.path.join(obj_dir, "ffh_plan.o")
     kern_o = os.path.join(obj_dir, "ffh_kernels.o")
-    common = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+    common = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function", "-pthread"]
     cmds = [
         ["g++", *common, "-c", os.path.join(CSRC, "ffh_plan.cpp"), "-o", plan_o],
         [hipcc, f"--offload-arch={ARCH}", *common, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

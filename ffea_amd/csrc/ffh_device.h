@@ -129,7 +129,7 @@ __device__ __forceinline__ double parts_range(const double *__restrict__ parts, 
 // Per-blob totals.  A blob of many slices would make every consumer block re-add thousands of
 // partials (O(slices^2) work in all); above kTotSlicesMin slices per blob a small kernel adds
 // each blob's partials once (same fixed order) into tot[blob][slot] and consumers read that.
-constexpr int kTotFF = 0, kTotRZ = 1, kTotRR = 2, kTotPQ = 3, kTotUU = 4, kTotZRP = 5, kTotDEN = 6;
+constexpr int kTotFF = 0, kTotRZ = 1, kTotRR = 2, kTotPQ = 3, kTotUU = 4, kTotZRP = 5, kTotDEN = 6, kTotTR = 7;
 constexpr int kTotSlots = 8;
 constexpr int kTotSlicesMin = 3072;
 
@@ -196,4 +196,6 @@ struct CgCtl {
     double res, f2;
     double delta[2];  // delta_k = r_k . D^-1 r_k, slot k & 1
     int iter, done, status, inversions;
+    int check;  // mixed path: this iteration's SpMV also formed the true residual f - K x
+    int pad_;
 };

@@ -195,228 +195,180 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
 }
 
 // ----------------------------------------------------------------------------------------
-// Assembly over the upper triangle of the block pattern
+// Assembly of the block pattern, one workgroup per 64-row slice
 // ----------------------------------------------------------------------------------------
 struct AsmArgs {
-    const double4 *pos;  // positions: the element geometry is recomputed per contribution
-    const uint32_t *flat, *eoff;  // contributions entry-major, [eoff[u], eoff[u+1]) for entry u
+    const double4 *pos;  // positions: the element geometry is recomputed from them
     const int4 *enodes;
     const double *A, *B;
-    const int64_t *toff;
-    const uint16_t *twidth, *cnt;
-    const uint32_t *ctb;
-    const int32_t *slot, *mirror, *node;
-    const uint8_t *flags;
+    const int64_t *eoff;      // [slices + 1] into elem
+    const int32_t *elem;      // per slice, the distinct elements its slots sum over, ascending
+    const int32_t *width;     // [slices] tiles per slice
+    const int64_t *slice_off; // [slices + 1] first slot; slot = slice_off[s] + 64 t + lane
+    const int64_t *toff;      // [tiles] first code of each 64-slot tile
+    const uint16_t *code;     // slice-local element << 4 | a << 2 | b, tile-major [k][64]
+    const uint8_t *cnt;       // [slots] contributions
+    const uint8_t *flags;     // [slots] ffh::kSlotDiag / kSlotIdentity / kSlotPad
     const double *zeta;
     void *val;  // T tiles
     double *dinv;
-    int64_t nupper;
+    int nslices;
 };
 
 // SparseMatrixFixedPattern::build (src/SparseMatrixFixedPattern.cpp:85-93): each entry is the
 // sum of its sources in registration order -- ascending element, Stokes drag last
-// (src/NoMassCGSolver.cpp:93-139, src/SparseMatrixTypes.cpp:52-58).  K is symmetric entry by
-// entry (each element block pair is an exact mirror), so the lower block is the transposed
-// upper one.  Diagonal blocks also give calc_inverse_diagonal (:325-333).
+// (src/NoMassCGSolver.cpp:93-139, src/SparseMatrixTypes.cpp:52-58); diagonal blocks also give
+// calc_inverse_diagonal (:325-333).
 //
-// Block (a,b) of an element, entry (i,j) = V[a+4i][b+4j] with g_a = grad psi_a, g_b = grad psi_b
-// (dpsi[4i+a] = g_a[i], recomputed from the positions); these are the expressions of visc_entry with
-// every product written in an order that rounds identically:
+// Element block (a, b) (row vertex a, column vertex b), entry (i, j) = V[a+4i][b+4j] with
+// g_a = grad psi_a, g_b = grad psi_b (dpsi[4i+a] = g_a[i]); these are visc_entry's expressions
+// with every product written in an order that rounds identically:
 //   i == j : ((g_a[i] g_b[i]) (A+B) + (g_a.g_b) A) vol
 //   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
-template <typename T, int kUnroll>
-__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
-    const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned)
-    const int64_t u = (int64_t)xcd_block((int)((g.nupper + bd - 1) / bd)) * bd + threadIdx.x;
-    if (u >= g.nupper) return;
-    const uint8_t fl = g.flags[u];
-    if (fl & 4) return;  // tile padding
-    const bool diag = (fl & 1) != 0;
-    T acc[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-    const int64_t grp = u >> 6;
-    const int lane = (int)(u & 63);
-    const int cnt = g.cnt[u];
-    const uint32_t *tl = g.ctb + g.toff[grp] + lane;
-#pragma unroll kUnroll
-    for (int k = 0; k < cnt; k++) {
-        const uint32_t code = tl[64 * k];
-        const int a = (code >> 2) & 3, b = code & 3;
-        const int e = (int)(code >> 4);
-        T J[3][3], dpsi[12], det;
-        element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
-        const T vol = (T)(1.0 / 6.0) * fabs(det);
-        const T A = (T)g.A[e], B = (T)g.B[e];
-        T ga[3], gb[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {  // selects, not a dynamic register index
-            const T *d = dpsi + 4 * i;
-            ga[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : d[3];
-            gb[i] = b == 0 ? d[0] : b == 1 ? d[1] : b == 2 ? d[2] : d[3];
-        }
-        const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                T v;
-                if (i == j) {
-                    v = ga[i] * gb[i] * (A + B);
-                    v += del2 * A;
-                } else if (diag) {
-                    v = (B + A) * (ga[i] * gb[j]);
-                } else {
-                    v = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
-                }
-                acc[3 * i + j] += v * vol;
-            }
-    }
-    const int node = g.node[u];
-    if (diag) {
-        if (fl & 2) {
-#pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-            acc[0] = acc[4] = acc[8] = (T)1.0;
-        } else {
-            const T z = (T)g.zeta[node];
-            acc[0] += z;
-            acc[4] += z;
-            acc[8] += z;
-        }
-        g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[0];
-        g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[4];
-        g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[8];
-    }
-    T *v = reinterpret_cast<T *>(g.val);
-    {
-        const int64_t slot = g.slot[u];
-        const int64_t tile = slot & ~(int64_t)63;
-        const int lane = (int)(slot & 63);
-#pragma unroll
-        for (int k = 0; k < 9; k++) v[tile * 9 + k * 64 + lane] = acc[k];
-    }
-    const int32_t mslot = g.mirror[u];
-    if (mslot >= 0) {
-        const int64_t tile = (int64_t)mslot & ~(int64_t)63;
-        const int lane = mslot & 63;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) v[tile * 9 + (3 * i + j) * 64 + lane] = acc[3 * j + i];
-    }
-}
+// Block (b, a) of the same element at entry (j, i) multiplies the same pairs of factors (IEEE
+// products commute), so every lower block equals the transposed upper one bit for bit and each
+// slot is computed where it is stored.
+//
+// One workgroup per slice.  The slice's distinct elements are evaluated once each, kThreads at
+// a time, into LDS (shape gradients, volume, A, B: element_geometry, the element kernel's own
+// code, so they round as there); then the thread of every slot adds its contributions of that
+// chunk in order.  Wave w owns the slice's tiles w, w + kWaves, ... (kTpt of them per pass).
+// kLdsCodes: the slice's contribution codes (at most kAsmCodeLds) are first copied to LDS with
+// 16-byte loads, and each thread finds where its list leaves the chunk by binary search, so the
+// summation loop has no data-dependent exit and its LDS reads pipeline.
+constexpr int kAsmCodeLds = 9216;
 
-// One contribution (element block (a,b) of code) of an upper block, entry by entry, premultiplied
-// by vol as k_assemble adds it.  a == b exactly for the diagonal blocks.
-template <typename T>
-__device__ __forceinline__ void contribution(const AsmArgs &g, uint32_t code, T w[9]) {
+template <typename T, int kThreads>
+__device__ __forceinline__ void asm_add(const T (&lds)[15][kThreads], uint32_t code, int le, T acc[9]) {
     const int a = (code >> 2) & 3, b = code & 3;
-    const int e = (int)(code >> 4);
-    T J[3][3], dpsi[12], det;
-    element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
-    const T vol = (T)(1.0 / 6.0) * fabs(det);
-    const T A = (T)g.A[e], B = (T)g.B[e];
     T ga[3], gb[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const T *d = dpsi + 4 * i;
-        ga[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : d[3];
-        gb[i] = b == 0 ? d[0] : b == 1 ? d[1] : b == 2 ? d[2] : d[3];
+        ga[i] = lds[a + 4 * i][le];
+        gb[i] = lds[b + 4 * i][le];
     }
+    const T vol = lds[12][le], A = lds[13][le], B = lds[14][le];
     const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
+        for (int jj = 0; jj < 3; jj++) {
             T v;
-            if (i == j) {
+            if (i == jj) {
                 v = ga[i] * gb[i] * (A + B);
                 v += del2 * A;
             } else if (a == b) {
-                v = (B + A) * (ga[i] * gb[j]);
+                v = (B + A) * (ga[i] * gb[jj]);
             } else {
-                v = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
+                v = B * (ga[i] * gb[jj]) + A * (gb[i] * ga[jj]);
             }
-            w[3 * i + j] = v * vol;
+            acc[3 * i + jj] += v * vol;
         }
 }
 
-// Two-phase assembly: one workgroup per tile of 64 upper blocks.  Phase 1: all 256 threads
-// evaluate the tile's contributions (entry-major, each block's in ascending element order) in
-// chunks of 256, nine values each into LDS -- independent gathers in flight instead of one
-// dependent chain per lane.  Phase 2: the lane of each block adds its contributions of the
-// chunk in order.  The values and the order of additions are k_assemble's: bit-identical.
-template <typename T>
-__global__ __launch_bounds__(256) void k_assemble2(AsmArgs g) {
-    __shared__ T lv[9][256];
-    const int ntiles = (int)(g.nupper >> 6);
-    const int tile = xcd_block(ntiles);
-    if (tile >= ntiles) return;
-    const int t = threadIdx.x;
-    const int64_t u0 = (int64_t)tile * 64;
-    const uint32_t c0 = g.eoff[u0], c1 = g.eoff[u0 + 64];
-    uint32_t s = 0, en = 0;
-    if (t < 64) {
-        s = g.eoff[u0 + t];
-        en = g.eoff[u0 + t + 1];
+template <typename T, int kWaves, int kTpt, bool kLdsCodes>
+__global__ __launch_bounds__(64 * kWaves) void k_assemble_slice(AsmArgs g) {
+    constexpr int kThreads = 64 * kWaves;
+    __shared__ T lds[15][kThreads];
+    __shared__ __align__(16) uint16_t lcode[kLdsCodes ? kAsmCodeLds : 8];
+    const int s = xcd_block(g.nslices);
+    if (s >= g.nslices) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = g.width[s];
+    const int64_t e0 = g.eoff[s];
+    const int ne = (int)(g.eoff[s + 1] - e0);
+    const int64_t slot0 = g.slice_off[s];
+    const int64_t cb = g.toff[slot0 >> 6];  // the slice's codes: [cb, toff[tile0 + w]), 128-byte aligned
+    if constexpr (kLdsCodes) {
+        const int n8 = (int)((g.toff[(slot0 >> 6) + w] - cb) >> 3);
+        const uint4 *src = reinterpret_cast<const uint4 *>(g.code + cb);
+        uint4 *dst = reinterpret_cast<uint4 *>(lcode);
+        for (int i = threadIdx.x; i < n8; i += kThreads) dst[i] = src[i];  // visible after the first barrier
     }
-    T acc[9];
+    for (int tg = 0; tg < w; tg += kWaves * kTpt) {  // one pass unless the slice is very wide
+        T acc[kTpt][9];
+        int k[kTpt], cnt[kTpt];
+        int64_t co[kTpt];  // this lane's first code: in LDS relative to cb, else in global memory
 #pragma unroll
-    for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-    for (uint32_t ch = c0; ch < c1; ch += 256) {
-        const uint32_t f = ch + t;
-        if (f < c1) {
-            T w[9];
-            contribution<T>(g, g.flat[f], w);
+        for (int j = 0; j < kTpt; j++) {
+            const int t = tg + wave + kWaves * j;
+            const int64_t slot = slot0 + (int64_t)t * 64 + lane;
+            cnt[j] = t < w ? (int)g.cnt[slot] : 0;
+            co[j] = (t < w ? g.toff[slot >> 6] - (kLdsCodes ? cb : 0) : 0) + lane;
+            k[j] = 0;
 #pragma unroll
-            for (int k = 0; k < 9; k++) lv[k][t] = w[k];
+            for (int q = 0; q < 9; q++) acc[j][q] = (T)0.0;
         }
-        __syncthreads();
-        if (t < 64) {
-            const uint32_t lo = s > ch ? s : ch, hi = en < ch + 256 ? en : ch + 256;
-            for (uint32_t i = lo; i < hi; i++)
+        for (int c0 = 0; c0 < ne; c0 += kThreads) {
+            if (c0 + (int)threadIdx.x < ne) {
+                const int e = g.elem[e0 + c0 + threadIdx.x];
+                T J[3][3], dpsi[12], det;
+                element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
 #pragma unroll
-                for (int k = 0; k < 9; k++) acc[k] += lv[k][i - ch];
+                for (int q = 0; q < 12; q++) lds[q][threadIdx.x] = dpsi[q];
+                lds[12][threadIdx.x] = (T)(1.0 / 6.0) * fabs(det);
+                lds[13][threadIdx.x] = (T)g.A[e];
+                lds[14][threadIdx.x] = (T)g.B[e];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kTpt; j++) {
+                if constexpr (kLdsCodes) {
+                    // codes of this lane's list are ascending: the first one at or past the chunk end
+                    const uint32_t lim = (uint32_t)(c0 + kThreads) << 4;
+                    int lo = k[j], hi = cnt[j];
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((uint32_t)lcode[co[j] + 64 * mid] < lim)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+#pragma unroll 2
+                    for (int kk = k[j]; kk < lo; kk++) {
+                        const uint32_t code = lcode[co[j] + 64 * kk];
+                        asm_add<T, kThreads>(lds, code, (int)(code >> 4) - c0, acc[j]);
+                    }
+                    k[j] = lo;
+                } else {
+                    while (k[j] < cnt[j]) {
+                        const uint32_t code = g.code[co[j] + 64 * k[j]];
+                        const int le = (int)(code >> 4) - c0;
+                        if (le >= kThreads) break;  // from a later chunk
+                        asm_add<T, kThreads>(lds, code, le, acc[j]);
+                        k[j]++;
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    if (t >= 64) return;
-    const int64_t u = u0 + t;
-    const uint8_t fl = g.flags[u];
-    if (fl & 4) return;  // tile padding
-    const bool diag = (fl & 1) != 0;
-    const int node = g.node[u];
-    if (diag) {
-        if (fl & 2) {
+        T *val = reinterpret_cast<T *>(g.val);
 #pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] = (T)0.0;
-            acc[0] = acc[4] = acc[8] = (T)1.0;
-        } else {
-            const T z = (T)g.zeta[node];
-            acc[0] += z;
-            acc[4] += z;
-            acc[8] += z;
+        for (int j = 0; j < kTpt; j++) {
+            const int t = tg + wave + kWaves * j;
+            if (t >= w) continue;
+            const int64_t slot = slot0 + (int64_t)t * 64 + lane;
+            const uint8_t fl = g.flags[slot];
+            if (fl & ffh::kSlotPad) continue;  // zero for good
+            if (fl & ffh::kSlotDiag) {
+                const int node = s * 64 + lane;
+                if (fl & ffh::kSlotIdentity) {
+#pragma unroll
+                    for (int q = 0; q < 9; q++) acc[j][q] = (T)0.0;
+                    acc[j][0] = acc[j][4] = acc[j][8] = (T)1.0;
+                } else {
+                    const T z = (T)g.zeta[node];
+                    acc[j][0] += z;
+                    acc[j][4] += z;
+                    acc[j][8] += z;
+                }
+                g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[j][0];
+                g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[j][4];
+                g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[j][8];
+            }
+            const int64_t tile = slot & ~(int64_t)63;
+#pragma unroll
+            for (int q = 0; q < 9; q++) val[tile * 9 + q * 64 + lane] = acc[j][q];
         }
-        g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[0];
-        g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[4];
-        g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[8];
-    }
-    T *v = reinterpret_cast<T *>(g.val);
-    {
-        const int64_t slot = g.slot[u];
-        const int64_t tl = slot & ~(int64_t)63;
-        const int ln = (int)(slot & 63);
-#pragma unroll
-        for (int k = 0; k < 9; k++) v[tl * 9 + k * 64 + ln] = acc[k];
-    }
-    const int32_t mslot = g.mirror[u];
-    if (mslot >= 0) {
-        const int64_t tl = (int64_t)mslot & ~(int64_t)63;
-        const int ln = mslot & 63;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) v[tl * 9 + (3 * i + j) * 64 + ln] = acc[3 * j + i];
     }
 }

@@ -101,13 +101,19 @@ struct ffh_blob {
     double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
            *last_det = nullptr;
     void *eforce = nullptr, *val = nullptr;
-    uint32_t *uctb = nullptr, *nctb = nullptr, *rng_off = nullptr, *uflat = nullptr, *ueoff = nullptr;
-    int64_t *ut_off = nullptr, *nt_off = nullptr;
-    uint16_t *ut_width = nullptr, *ut_cnt = nullptr, *nt_cnt = nullptr;
-    int32_t *slot_col = nullptr, *slice_width = nullptr, *up_slot = nullptr, *up_mirror = nullptr,
-            *up_node = nullptr;
+    uint32_t *nctb = nullptr, *rng_off = nullptr;
+    int64_t *nt_off = nullptr;
+    uint16_t *nt_cnt = nullptr;
+    int32_t *slot_col = nullptr, *slice_width = nullptr;
     int64_t *slice_off = nullptr;
-    uint8_t *up_flags = nullptr, *pinned = nullptr;
+    uint8_t *pinned = nullptr;
+    // slice assembly (k_assemble_slice): element lists, tiled contribution codes, slot tables
+    int64_t *sa_eoff = nullptr, *sa_toff = nullptr;
+    int32_t *sa_elem = nullptr;
+    uint16_t *sa_code = nullptr;
+    uint8_t *sa_cnt = nullptr, *slot_flags = nullptr;
+    int max_width = 0;      // widest slice, in tiles
+    bool codes_lds = true;  // every slice's assembly codes fit the kernel's LDS area
     double *zeta = nullptr, *dinv = nullptr;
     double *f = nullptr, *r = nullptr, *x = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *vel = nullptr;
     double *parts = nullptr, *tmp_in = nullptr, *tmp_out = nullptr;
@@ -191,7 +197,7 @@ struct ffh_blob {
     WrapArgs wrap = {};
     std::vector<void *> box_allocs;
     // launch shapes picked by autotune(): element / assembly block size, SpMV variant
-    int elem_block = kBlock, asm_block = kBlock, spmv_waves = 3, asm_unroll = 2;
+    int elem_block = kBlock, asm_block = 512, spmv_waves = 3, asm_tiles = 2;
     bool spmv_nt = false;  // non-temporal matrix loads in the SpMV
     double *tot = nullptr;
     CgCtl *ctl = nullptr;       // [nblobs]; ctl[0].inversions counts the whole batch
@@ -232,6 +238,9 @@ struct ffh_blob {
     std::vector<double> blob_res;
     std::vector<int> blob_status;
     int cg_chunk = 16;
+    // mixed path: the true residual f - K x is formed every rr_period CG iterations (and
+    // whenever the recurrence residual passes the test); FFH_RR_PERIOD overrides the default
+    int rr_period = 8;
 
     ~ffh_blob() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -337,6 +346,8 @@ struct ffh_blob {
         v.r_prev = nullptr;
         v.parts_zrp = parts_zrp();
         v.tot = use_tot ? tot : nullptr;
+        v.parts_tr = parts_tr();
+        v.rr_period = rr_period;
         return v;
     }
     double *parts_energy() const { return parts + 4 * pstride; }
@@ -347,6 +358,9 @@ struct ffh_blob {
     double *parts_zrp() const { return parts + 6 * pstride; }
     double *parts_den() const { return parts + 7 * pstride; }
     double *parts_loss() const { return parts + 8 * pstride; }
+    double *parts_tr() const { return parts + 9 * pstride; }
+    // the mixed path's CG: FP32 work vectors, FP64 iterate, true residual and stopping test
+    bool mixed_cg() const { return precision == FFH_PRECISION_MIXED && !massful(); }
     SpmvArgs spmv_args() const {
         SpmvArgs m;
         m.slice_off = slice_off;
@@ -406,47 +420,45 @@ struct ffh_blob {
         a.enodes = enodes;
         a.A = A;
         a.B = B;
-        a.flat = uflat;
-        a.eoff = ueoff;
-        a.toff = ut_off;
-        a.twidth = ut_width;
-        a.cnt = ut_cnt;
-        a.ctb = uctb;
-        a.slot = up_slot;
-        a.mirror = up_mirror;
-        a.node = up_node;
-        a.flags = up_flags;
+        a.eoff = sa_eoff;
+        a.elem = sa_elem;
+        a.width = slice_width;
+        a.slice_off = slice_off;
+        a.toff = sa_toff;
+        a.code = sa_code;
+        a.cnt = sa_cnt;
+        a.flags = slot_flags;
         a.zeta = zeta;
         a.val = val;
         a.dinv = dinv;
-        a.nupper = plan.num_upper;
-        const int gr = xcd_grid(grid(plan.num_upper, asm_block));
-        if (asm_unroll == 0) {  // two-phase: one workgroup per 64-entry tile
-            const int gt = xcd_grid((int)(plan.num_upper / 64));
-            return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
-                if (precision == FFH_PRECISION_FP64)
-                    hipExtLaunchKernelGGL((k_assemble2<double>), gt, 256, 0, stream, e0, e1, 0, a);
-                else
-                    hipExtLaunchKernelGGL((k_assemble2<float>), gt, 256, 0, stream, e0, e1, 0, a);
-            });
-        }
+        a.nslices = plan.num_slices;
         return launch(1, [&](hipEvent_t e0, hipEvent_t e1) {
-            if (precision == FFH_PRECISION_FP64) {
-                if (asm_unroll == 1)
-                    hipExtLaunchKernelGGL((k_assemble<double, 1>), gr, asm_block, 0, stream, e0, e1, 0, a);
-                else if (asm_unroll == 2)
-                    hipExtLaunchKernelGGL((k_assemble<double, 2>), gr, asm_block, 0, stream, e0, e1, 0, a);
-                else
-                    hipExtLaunchKernelGGL((k_assemble<double, 4>), gr, asm_block, 0, stream, e0, e1, 0, a);
-            } else {
-                if (asm_unroll == 1)
-                    hipExtLaunchKernelGGL((k_assemble<float, 1>), gr, asm_block, 0, stream, e0, e1, 0, a);
-                else if (asm_unroll == 2)
-                    hipExtLaunchKernelGGL((k_assemble<float, 2>), gr, asm_block, 0, stream, e0, e1, 0, a);
-                else
-                    hipExtLaunchKernelGGL((k_assemble<float, 4>), gr, asm_block, 0, stream, e0, e1, 0, a);
-            }
+            if (precision == FFH_PRECISION_FP64)
+                launch_assemble<double>(a, e0, e1);
+            else
+                launch_assemble<float>(a, e0, e1);
         });
+    }
+    // asm_block threads (4 or 8 waves) per slice, asm_tiles tiles per thread and pass; the
+    // slices' codes staged in LDS when the largest fits
+    template <typename T>
+    void launch_assemble(const AsmArgs &a, hipEvent_t e0, hipEvent_t e1) {
+        const int gr = P_slice;
+        const int sel = (asm_block == 512 ? 0 : 3) + (asm_tiles == 1 ? 0 : asm_tiles == 2 ? 1 : 2) + (codes_lds ? 0 : 6);
+        switch (sel) {
+        case 0: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 1, true>), gr, 512, 0, stream, e0, e1, 0, a); break;
+        case 1: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 2, true>), gr, 512, 0, stream, e0, e1, 0, a); break;
+        case 2: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 4, true>), gr, 512, 0, stream, e0, e1, 0, a); break;
+        case 3: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 1, true>), gr, 256, 0, stream, e0, e1, 0, a); break;
+        case 4: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 2, true>), gr, 256, 0, stream, e0, e1, 0, a); break;
+        case 5: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 4, true>), gr, 256, 0, stream, e0, e1, 0, a); break;
+        case 6: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 1, false>), gr, 512, 0, stream, e0, e1, 0, a); break;
+        case 7: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 2, false>), gr, 512, 0, stream, e0, e1, 0, a); break;
+        case 8: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 4, false>), gr, 512, 0, stream, e0, e1, 0, a); break;
+        case 9: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 1, false>), gr, 256, 0, stream, e0, e1, 0, a); break;
+        case 10: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 2, false>), gr, 256, 0, stream, e0, e1, 0, a); break;
+        default: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 4, false>), gr, 256, 0, stream, e0, e1, 0, a); break;
+        }
     }
     RhsArgs rhs_args(bool noise, const double *f_in) const {
         RhsArgs g;
@@ -491,9 +503,11 @@ struct ffh_blob {
         if (learned_solve()) v.parts_uu = parts_uu();
         int rc;
         if (precision == FFH_PRECISION_FP64)
-            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<double, double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+        else if (mixed_cg())
+            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float, float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
         else
-            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
+            rc = launch(2, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL((k_rhs<float, double>), P_slice, 192, 0, stream, e0, e1, 0, g, v); });
         if (rc || lumped()) return rc;
         return totals((1 << kTotFF) | (1 << kTotRZ) | (v.parts_uu ? 1 << kTotUU : 0));
     }
@@ -520,7 +534,7 @@ struct ffh_blob {
             }
         });
         if (rc) return rc;
-        return totals(1 << kTotPQ);
+        return totals((1 << kTotPQ) | (mixed_cg() ? 1 << kTotTR : 0));
     }
     int launch_update(int it, bool learned_next) {
         CgArgs v = cg_args();
@@ -528,14 +542,19 @@ struct ffh_blob {
             v.parts_uu = parts_uu();
             v.r_prev = r_prev;
         }
-        int rc = launch(4, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_cg_update, P_slice, 192, 0, stream, e0, e1, 0, v, it); });
+        int rc = launch(4, [&](hipEvent_t e0, hipEvent_t e1) {
+            if (mixed_cg())
+                hipExtLaunchKernelGGL((k_cg_update<float, true>), P_slice, 192, 0, stream, e0, e1, 0, v, it);
+            else
+                hipExtLaunchKernelGGL((k_cg_update<double, false>), P_slice, 192, 0, stream, e0, e1, 0, v, it);
+        });
         if (rc) return rc;
         return totals((1 << kTotRR) | (1 << kTotRZ) | (learned_next ? 1 << kTotUU : 0));
     }
     int totals(int mask) {
         if (!use_tot) return FFH_OK;
         TotArgs t;
-        double *pp[kTotSlots] = {parts_ff(), parts, parts_rr(), parts_pq(), parts_uu(), parts_zrp(), parts_den(), parts};
+        double *pp[kTotSlots] = {parts_ff(), parts, parts_rr(), parts_pq(), parts_uu(), parts_zrp(), parts_den(), parts_tr()};
         for (int k = 0; k < kTotSlots; k++) t.parts[k] = pp[k];
         t.blob_slice_off = blob_slice_off;
         t.tot = tot;
@@ -619,14 +638,17 @@ struct ffh_blob {
         return FFH_OK;
     }
     // Forward pass to z (CG).  Input: z = u = D^-1 r and the partials of |u|^2.
-    int mpnn_apply(const double *rvec, const double *rprev, bool skip_done) {
+    // Forward pass to z.  V: the CG vectors' type (float on the mixed path); the host-facing
+    // ffh_mpnn_apply works in double.
+    template <typename V>
+    int mpnn_apply(const V *rvec, const V *rprev, bool skip_done) {
         int rc;
         if ((rc = mp_edges_now())) return rc;
         MpnnArgs a = mp_args(skip_done);
         const int S = plan.num_slices;
         const double *uu = parts_uu();
         rc = launch(7, [&](hipEvent_t e0, hipEvent_t e1) {
-            hipExtLaunchKernelGGL((k_mpnn_embed<double>), P_slice, 256, 0, stream, e0, e1, 0, a, (const double *)z, uu,
+            hipExtLaunchKernelGGL((k_mpnn_embed<V>), P_slice, 256, 0, stream, e0, e1, 0, a, (const V *)z, uu,
                                   (float *)nullptr, mp_H[0], (const float *)nullptr, (float *)nullptr, (double *)nullptr);
         });
         if (rc) return rc;
@@ -641,7 +663,7 @@ struct ffh_blob {
         (void)S;
         double *zrp = rprev ? parts_zrp() : (double *)nullptr;
         rc = launch(10, [&](hipEvent_t e0, hipEvent_t e1) {
-            hipExtLaunchKernelGGL(k_mpnn_out_cg1, P_slice, 64, 0, stream, e0, e1, 0, a, (const bf16_t *)h, uu, z, rvec,
+            hipExtLaunchKernelGGL((k_mpnn_out_cg1<V>), P_slice, 64, 0, stream, e0, e1, 0, a, (const bf16_t *)h, uu, (V *)z, rvec,
                                   rprev, parts, zrp);
         });
         if (rc) return rc;
@@ -699,7 +721,11 @@ struct ffh_blob {
     int harvest_snap(int k) {
         if (!hv_on || k >= hv_cap) return FFH_OK;
         const size_t n3 = 3 * (size_t)N;
-        hipLaunchKernelGGL(k_mpnn_harvest, (int)((n3 + 255) / 256), 256, 0, stream, (int)n3, (const double *)r,
+        if (mixed_cg())
+            hipLaunchKernelGGL((k_mpnn_harvest<float>), (int)((n3 + 255) / 256), 256, 0, stream, (int)n3, (const float *)r,
+                               (const double *)x, hv_r + (size_t)k * n3, hv_x + (size_t)k * n3);
+        else
+            hipLaunchKernelGGL((k_mpnn_harvest<double>), (int)((n3 + 255) / 256), 256, 0, stream, (int)n3, (const double *)r,
                            (const double *)x, hv_r + (size_t)k * n3, hv_x + (size_t)k * n3);
         HIP_OR_FAIL(hipGetLastError());
         return FFH_OK;
@@ -713,10 +739,18 @@ struct ffh_blob {
     // fallback count, by default the iteration count of the last plain solve); iteration F + 1
     // restarts Jacobi-PCG from the current iterate (beta = 0), which then runs to the same
     // stopping test.  Snapshots for training are taken from plain solves only.
+    // On the mixed path the stopping decision of iteration `it` is made in update(it) (the true
+    // residual formed by spmv(it)), so there an iteration is spmv(it) followed at once by
+    // update(it), step graphs included; `updated` records that update(it) is already issued.
     struct CgRun {
-        bool learned, harvesting;
+        bool learned, harvesting, updated;
         int F, snap0, it;
     };
+    int mpnn_apply_cg(bool with_prev, bool skip_done) {
+        if (mixed_cg())
+            return mpnn_apply<float>((const float *)r, with_prev ? (const float *)r_prev : nullptr, skip_done);
+        return mpnn_apply<double>(r, with_prev ? r_prev : nullptr, skip_done);
+    }
     CgRun cg_setup() {
         CgRun c;
         c.learned = learned_solve();
@@ -724,24 +758,34 @@ struct ffh_blob {
         c.snap0 = hv_count;
         c.harvesting = hv_on && !c.learned && !massful();
         c.it = 0;
+        c.updated = false;
         return c;
     }
     int cg_begin(CgRun &c) {
         int rc;
-        if (c.learned && (rc = mpnn_apply(r, nullptr, false))) return rc;
+        if (c.learned && (rc = mpnn_apply_cg(false, false))) return rc;
         if ((rc = launch_spmv(0, 0))) return rc;
         if (c.harvesting && (rc = harvest_snap(c.snap0))) return rc;
+        if (mixed_cg()) {
+            if ((rc = launch_update(0, c.learned && 0 < c.F))) return rc;
+            c.updated = true;
+        }
         return FFH_OK;
     }
     int cg_iterations(CgRun &c, int count) {
         int rc;
         for (int k = 0; k < count && c.it < maxit; k++) {
             const bool lz = c.learned && c.it < c.F;
-            if ((rc = launch_update(c.it, lz))) return rc;
+            if (!c.updated && (rc = launch_update(c.it, lz))) return rc;
+            c.updated = false;
             c.it++;
             if (c.harvesting && (rc = harvest_snap(c.snap0 + c.it))) return rc;
-            if (lz && (rc = mpnn_apply(r, r_prev, true))) return rc;
+            if (lz && (rc = mpnn_apply_cg(true, true))) return rc;
             if ((rc = launch_spmv(c.it, lz ? 1 : (c.learned && c.it == c.F + 1 ? 2 : 0)))) return rc;
+            if (mixed_cg()) {
+                if ((rc = launch_update(c.it, c.learned && c.it < c.F))) return rc;
+                c.updated = true;
+            }
         }
         return FFH_OK;
     }
@@ -871,19 +915,18 @@ struct ffh_blob {
         }
         elem_block = pick;
         best = 1e30f;
-        int pick_u = asm_unroll;
+        int pick_u = asm_tiles;
         pick = asm_block;
-        const int unrolls[4] = {0, 1, 2, 4};  // 0: the two-phase kernel (block size unused)
-        for (int bs : sizes)
-            for (int un : unrolls) {
-                if (un == 0 && bs != 256) continue;
+        for (int bs : {256, 512})
+            for (int un : {1, 2, 4}) {
+                if (un > 1 && (bs / 64) * (un / 2) >= max_width) continue;  // a smaller shape covers the slice
                 asm_block = bs;
-                asm_unroll = un;
+                asm_tiles = un;
                 if ((rc = time_ms([&] { return run_assemble(); }, 5, ms))) return rc;
                 if (ms < best) best = ms, pick = bs, pick_u = un;
             }
         asm_block = pick;
-        asm_unroll = pick_u;
+        asm_tiles = pick_u;
         std::vector<double> ones(3 * (size_t)N, 1.0);
         HIP_OR_FAIL(hipMemcpyAsync(tmp_in, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         best = 1e30f;
@@ -1072,6 +1115,7 @@ struct ffh_blob {
         if (lumped()) return lumped_finish();
         CgRun c = cg_setup();
         c.it = K;
+        c.updated = mixed_cg();
         bool all_done = false;
         if ((rc = cg_poll(all_done))) return rc;
         return cg_run_rest(c, true, all_done);
@@ -1187,19 +1231,20 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
         // padding slots stay zero for good; real slots are rewritten by every assembly
         if (!rc && hipMemset(vv, 0, 9 * (size_t)P.num_slots * b->tsize) != hipSuccess) rc = 1;
     }
-    rc |= b->upload(b->ut_off, P.utile.off);
-    rc |= b->upload(b->ut_width, P.utile.width);
-    rc |= b->upload(b->ut_cnt, P.utile.cnt);
-    rc |= b->upload(b->uctb, P.utile.code);
-    rc |= b->upload(b->uflat, P.uctb);
-    rc |= b->upload(b->ueoff, P.uctb_off);
+    rc |= b->upload(b->sa_eoff, P.sa_eoff);
+    rc |= b->upload(b->sa_elem, P.sa_elem);
+    rc |= b->upload(b->sa_toff, P.sa_toff);
+    rc |= b->upload(b->sa_code, P.sa_code);
+    rc |= b->upload(b->sa_cnt, P.sa_cnt);
+    rc |= b->upload(b->slot_flags, P.slot_flags);
+    for (int s = 0; s < P.num_slices; s++) {
+        b->max_width = std::max(b->max_width, P.slice_width[s]);
+        const int64_t t0 = P.slice_off[s] / ffh::kSlice;
+        if (P.sa_toff[t0 + P.slice_width[s]] - P.sa_toff[t0] > kAsmCodeLds) b->codes_lds = false;
+    }
     rc |= b->upload(b->nt_off, P.ntile.off);
     rc |= b->upload(b->nt_cnt, P.ntile.cnt);
     rc |= b->upload(b->nctb, P.ntile.code);
-    rc |= b->upload(b->up_slot, P.up_slot);
-    rc |= b->upload(b->up_mirror, P.up_mirror);
-    rc |= b->upload(b->up_node, P.up_node);
-    rc |= b->upload(b->up_flags, P.up_flags);
     rc |= b->upload(b->rng_off, P.rng_node_off);
     rc |= b->upload(b->slot_col, P.slot_col);
     rc |= b->upload(b->slice_width, P.slice_width);
@@ -1216,7 +1261,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->alloc(b->vel, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_in, 3 * (size_t)N);
     rc |= b->alloc(b->tmp_out, 3 * (size_t)N);
-    rc |= b->alloc(b->parts, 9 * b->pstride);
+    rc |= b->alloc(b->parts, 10 * b->pstride);
     rc |= b->upload(b->slice_blob, P.slice_blob);
     rc |= b->upload(b->blob_slice_off, P.blob_slice_off);
     rc |= b->alloc(b->ctl, (size_t)b->nblobs);
@@ -1254,6 +1299,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
             return nullptr;
         }
     }
+    if (const char *rp = std::getenv("FFH_RR_PERIOD")) b->rr_period = std::max(0, std::atoi(rp));
     const char *tune = std::getenv("FFH_AUTOTUNE");
     if (!(tune && tune[0] == '0') && b->autotune() != FFH_OK) {
         if (g_err.empty()) fail(FFH_E_HIP, "autotuning failed");
@@ -1388,28 +1434,29 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
     return FFH_OK;
 }
 
-int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_unroll,
+int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_tiles,
                     int *spmv_waves_per_slice, int *spmv_nontemporal) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     if (element_block) *element_block = b->elem_block;
     if (assembly_block) *assembly_block = b->asm_block;
-    if (assembly_unroll) *assembly_unroll = b->asm_unroll;
+    if (assembly_tiles) *assembly_tiles = b->asm_tiles;
     if (spmv_waves_per_slice) *spmv_waves_per_slice = b->spmv_waves;
     if (spmv_nontemporal) *spmv_nontemporal = b->spmv_nt ? 1 : 0;
     return FFH_OK;
 }
 
-int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_unroll,
+int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_tiles,
                         int spmv_waves_per_slice, int spmv_nontemporal) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
-    if (!ok(element_block) || !ok(assembly_block) || (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3) ||
-        (assembly_unroll != 0 && assembly_unroll != 1 && assembly_unroll != 2 && assembly_unroll != 4))
-        return fail(FFH_E_ARG, "block sizes must be 64, 128 or 256, the assembly unroll depth 0 (two-phase), 1, 2 "
-                               "or 4 and SpMV waves per slice 1 or 3");
+    if (!ok(element_block) || (assembly_block != 256 && assembly_block != 512) ||
+        (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3) ||
+        (assembly_tiles != 1 && assembly_tiles != 2 && assembly_tiles != 4))
+        return fail(FFH_E_ARG, "the element block size must be 64, 128 or 256, the assembly workgroup 256 or 512 "
+                               "threads with 1, 2 or 4 tiles per thread, and SpMV waves per slice 1 or 3");
     b->elem_block = element_block;
     b->asm_block = assembly_block;
-    b->asm_unroll = assembly_unroll;
+    b->asm_tiles = assembly_tiles;
     b->spmv_waves = spmv_waves_per_slice;
     b->spmv_nt = spmv_nontemporal != 0;
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
@@ -2187,12 +2234,11 @@ int ffh_mpnn_apply(ffh_blob *b, const double *rin, double *zout) {
             for (int c = 0; c < 3; c++) rl[3 * i + c] = rin[3 * (size_t)P.orig_of[i] + c];
     HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, rl.data(), rl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     // u = D^-1 r into z, partials of |u|^2 -- the update kernel's work, here on a given r
-    CgArgs v = b->cg_args();
-    v.r = b->tmp_in;
-    hipLaunchKernelGGL((k_mpnn_u<CgArgs>), P.num_slices, 192, 0, b->stream, v, b->parts_uu());
+    hipLaunchKernelGGL(k_mpnn_u, P.num_slices, 192, 0, b->stream, P.num_slices, (const double *)b->dinv,
+                       (const double *)b->tmp_in, b->z, b->parts_uu());
     HIP_OR_FAIL(hipGetLastError());
     if (int rc = b->totals(1 << kTotUU)) return rc;
-    if (int rc = b->mpnn_apply(b->tmp_in, nullptr, false)) return rc;
+    if (int rc = b->mpnn_apply<double>(b->tmp_in, nullptr, false)) return rc;
     HIP_OR_FAIL(hipMemcpyAsync(zl.data(), b->z, zl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     std::memset(zout, 0, sizeof(double) * 3 * (size_t)P.num_nodes_total);

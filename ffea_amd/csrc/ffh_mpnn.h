@@ -295,9 +295,10 @@ __global__ __launch_bounds__(256) void k_mpnn_layer(MpnnArgs a, int layer, const
 
 // Output layer inside CG, one wave per slice (one lane per row): z = u + s_b (Wo h3 + bo), 0 on
 // pinned rows, with the partials of z.r and z.r_prev.
+template <typename V>
 __global__ __launch_bounds__(64) void k_mpnn_out_cg1(MpnnArgs a, const bf16_t *__restrict__ H3, const double *parts_uu,
-                                                     double *__restrict__ z, const double *__restrict__ r,
-                                                     const double *__restrict__ r_prev, double *parts_rz,
+                                                     V *__restrict__ z, const V *__restrict__ r,
+                                                     const V *__restrict__ r_prev, double *parts_rz,
                                                      double *parts_zrp) {
     __shared__ double sh[4];
     const int slice = xcd_block(a.nslices);
@@ -322,37 +323,13 @@ __global__ __launch_bounds__(64) void k_mpnn_out_cg1(MpnnArgs a, const bf16_t *_
     double rz = 0.0, zp = 0.0;
     for (int c = 0; c < kNOut; c++) {
         const size_t k = 3 * (size_t)row + c;
-        const double zz = zero ? 0.0 : z[k] + s * (double)o[c];
-        z[k] = zz;
-        rz += zz * r[k];
-        if (r_prev) zp += zz * r_prev[k];
+        const double zz = (double)(V)(zero ? 0.0 : (double)z[k] + s * (double)o[c]);
+        z[k] = (V)zz;
+        rz += zz * (double)r[k];
+        if (r_prev) zp += zz * (double)r_prev[k];
     }
     block_partial_at(rz, parts_rz, slice, sh);
     if (r_prev) block_partial_at(zp, parts_zrp, slice, sh);
-}
-
-// Output layer inside CG: z = u + s_b (Wo h3 + bo) (0 on pinned rows), with the partials of
-// z.r and z.r_prev for the flexible CG's beta.  One block = one slice, thread = (row, comp).
-__global__ __launch_bounds__(192) void k_mpnn_out_cg(MpnnArgs a, const bf16_t *__restrict__ H3, const double *parts_uu,
-                                                     double *__restrict__ z, const double *__restrict__ r,
-                                                     const double *__restrict__ r_prev, double *parts_rz,
-                                                     double *parts_zrp) {
-    __shared__ double sh[4];
-    const int slice = xcd_block(a.nslices);
-    if (slice >= a.nslices || blob_done(a, slice)) return;
-    const int b = a.slice_blob[slice];
-    const double s = blob_scale(a, parts_uu, b, sh);
-    const int rloc = threadIdx.x / 3, ci = threadIdx.x - 3 * rloc;
-    const int row = slice * 64 + rloc;
-    const size_t k = 3 * (size_t)row + ci;
-    const float *Wo = a.wf + kOffWo + ci * kHid;
-    float o = a.wf[kOffBo + ci];
-    const bf16_t *h = H3 + (size_t)row * kHid;
-    for (int j = 0; j < kHid; j++) o += Wo[j] * bf2f(h[j]);
-    const double zz = a.pinned[row] || s == 0.0 ? 0.0 : z[k] + s * (double)o;
-    z[k] = zz;
-    block_partial_at(zz * r[k], parts_rz, slice, sh);
-    if (r_prev) block_partial_at(zz * r_prev[k], parts_zrp, slice, sh);
 }
 
 // Training: u = D^-1 r for a harvested float residual, and the partials of |u|^2.
@@ -561,19 +538,20 @@ __global__ __launch_bounds__(256) void k_mpnn_pack(const float *__restrict__ w, 
 }
 
 // u = D^-1 r into z with the partials of |u|^2, for a residual given from the host.
-template <typename CA>
-__global__ __launch_bounds__(192) void k_mpnn_u(CA v, double *parts_uu) {
+__global__ __launch_bounds__(192) void k_mpnn_u(int nslices, const double *__restrict__ dinv, const double *__restrict__ r,
+                                                double *__restrict__ z, double *parts_uu) {
     __shared__ double sh[4];
     const int slice = blockIdx.x;
-    if (slice >= v.nslices) return;
+    if (slice >= nslices) return;
     const size_t k = (size_t)slice * 192 + threadIdx.x;
-    const double u = v.dinv[k] * v.r[k];
-    v.z[k] = u;
+    const double u = dinv[k] * r[k];
+    z[k] = u;
     block_partial_at(u * u, parts_uu, slice, sh);
 }
 
 // Harvest: snapshot k of a plain solve -- r_k and x_k as float.
-__global__ __launch_bounds__(256) void k_mpnn_harvest(int n3, const double *__restrict__ r, const double *__restrict__ x,
+template <typename V>
+__global__ __launch_bounds__(256) void k_mpnn_harvest(int n3, const V *__restrict__ r, const double *__restrict__ x,
                                                       float *__restrict__ rs, float *__restrict__ xs) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n3) return;

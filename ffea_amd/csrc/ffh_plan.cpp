@@ -9,8 +9,11 @@
 #include "ffh_plan.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace ffh {
 
@@ -81,6 +84,41 @@ void jacobian(const double *x, const double *y, const double *z, const int *n, d
     const double *p[3] = {x, y, z};
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) J[r][c] = p[c][n[r + 1]] - p[c][n[0]];
+}
+
+// Runs f(i, scratch) for i in [0, n) on up to FFH_HOST_THREADS (else OMP_NUM_THREADS, else 16)
+// threads; each thread owns one scratch vector.  Iterations must write disjoint data.
+template <typename F>
+void parallel_for(int64_t n, F &&f) {
+    int nt = 16;
+    for (const char *v : {"FFH_HOST_THREADS", "OMP_NUM_THREADS"}) {
+        const char *x = std::getenv(v);
+        if (x && std::atoi(x) > 0) {
+            nt = std::atoi(x);
+            break;
+        }
+    }
+    nt = std::max(1, std::min<int>({nt, (int)std::max(1u, std::thread::hardware_concurrency()), 64}));
+    if (n < 2 * (int64_t)nt || nt == 1) {
+        std::vector<int32_t> scratch;
+        for (int64_t i = 0; i < n; i++) f(i, scratch);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    const int64_t chunk = std::max<int64_t>(1, n / ((int64_t)nt * 16));
+    auto work = [&] {
+        std::vector<int32_t> scratch;
+        for (;;) {
+            const int64_t i0 = next.fetch_add(chunk);
+            if (i0 >= n) break;
+            const int64_t i1 = std::min(n, i0 + chunk);
+            for (int64_t i = i0; i < i1; i++) f(i, scratch);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
 }
 
 }  // namespace
@@ -278,57 +316,35 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     }
 
     // --- block pattern (linear, unpinned pairs + every diagonal), rows sorted by column ---
-    std::vector<uint64_t> keys;
-    keys.reserve(16 * (size_t)E + N);
-    for (int e = 0; e < E; e++)
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
-                if (P.pinned[r] || P.pinned[c]) continue;
-                keys.push_back(((uint64_t)r << 32) | (uint32_t)c);
-            }
-    for (int i = 0; i < N; i++) keys.push_back(((uint64_t)i << 32) | (uint32_t)i);
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    const int64_t NB = (int64_t)keys.size();
-    P.num_blocks = NB;
+    // Row r's columns: r itself and, for an unpinned row, the unpinned vertices of the elements
+    // that contain r (the pairs NoMassCGSolver::init registers, src/NoMassCGSolver.cpp:93-139).
     std::vector<int64_t> row_off((size_t)N + 1, 0);
-    std::vector<int32_t> bcol(NB);
-    for (int64_t k = 0; k < NB; k++) {
-        row_off[(keys[k] >> 32) + 1]++;
-        bcol[k] = (int32_t)(keys[k] & 0xffffffffu);
-    }
-    for (int i = 0; i < N; i++) row_off[i + 1] += row_off[i];
-    keys.clear();
-    keys.shrink_to_fit();
-    auto find_block = [&](int r, int c) -> int64_t {
-        auto b = bcol.begin() + row_off[r], en = bcol.begin() + row_off[r + 1];
-        auto it = std::lower_bound(b, en, c);
-        return (it != en && *it == c) ? (int64_t)(it - bcol.begin()) : -1;
-    };
-    // contributions per block, ascending element (the source-list order of the reference's
-    // pattern, src/NoMassCGSolver.cpp:93-139): code e << 4 | a << 2 | b
-    std::vector<uint32_t> bctb_off((size_t)NB + 1, 0);
-    for (int e = 0; e < E; e++)
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
-                if (P.pinned[r] || P.pinned[c] || r > c) continue;
-                bctb_off[find_block(r, c) + 1]++;
-            }
-    for (int64_t k = 0; k < NB; k++) bctb_off[k + 1] += bctb_off[k];
-    std::vector<uint32_t> bctb(bctb_off[NB]);
-    {
-        std::vector<uint32_t> cur(bctb_off.begin(), bctb_off.end() - 1);
-        for (int e = 0; e < E; e++)
-            for (int a = 0; a < 4; a++)
+    auto row_columns = [&](int r, std::vector<int32_t> &cols) {
+        cols.clear();
+        cols.push_back(r);
+        if (!P.pinned[r])
+            for (uint32_t m = P.nctb_off[r]; m < P.nctb_off[r + 1]; m++) {
+                const int e = (int)(P.nctb[m] >> 2);
                 for (int b = 0; b < 4; b++) {
-                    int r = P.enodes[4 * (size_t)e + a], c = P.enodes[4 * (size_t)e + b];
-                    if (P.pinned[r] || P.pinned[c] || r > c) continue;
-                    int64_t k = find_block(r, c);
-                    bctb[cur[k]++] = ((uint32_t)e << 4) | ((uint32_t)a << 2) | (uint32_t)b;
+                    const int c = P.enodes[4 * (size_t)e + b];
+                    if (!P.pinned[c]) cols.push_back(c);
                 }
-    }
+            }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    };
+    parallel_for(N, [&](int64_t r, std::vector<int32_t> &cols) {
+        row_columns((int)r, cols);
+        row_off[r + 1] = (int64_t)cols.size();
+    });
+    for (int i = 0; i < N; i++) row_off[i + 1] += row_off[i];
+    const int64_t NB = row_off[N];
+    P.num_blocks = NB;
+    std::vector<int32_t> bcol(NB);
+    parallel_for(N, [&](int64_t r, std::vector<int32_t> &cols) {
+        row_columns((int)r, cols);
+        std::copy(cols.begin(), cols.end(), bcol.begin() + row_off[r]);
+    });
 
     // --- SELL-64 layout ---
     P.num_slices = (N + kSlice - 1) / kSlice;
@@ -348,63 +364,108 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     for (int b = 0; b < num_blobs; b++)
         for (int sl = P.blob_slice_off[b]; sl < P.blob_slice_off[b + 1]; sl++) P.slice_blob[sl] = b;
     if (P.num_slots >= (int64_t)1 << 31) return "matrix pattern too large";
-    std::vector<int64_t> slot_of_block(NB, -1);
     P.slot_col.assign(P.num_slots, 0);
-    for (int s = 0; s < P.num_slices; s++)
+    P.slot_flags.assign(P.num_slots, kSlotPad);
+    parallel_for(P.num_slices, [&](int64_t s, std::vector<int32_t> &) {
         for (int t = 0; t < P.slice_width[s]; t++)
             for (int l = 0; l < kSlice; l++) {
-                int64_t slot = P.slice_off[s] + (int64_t)t * kSlice + l;
-                int r = s * kSlice + l;
+                const int64_t slot = P.slice_off[s] + (int64_t)t * kSlice + l;
+                const int r = (int)s * kSlice + l;
                 if (r < N && t < row_off[r + 1] - row_off[r]) {
-                    int64_t k = row_off[r] + t;
-                    P.slot_col[slot] = bcol[k];
-                    slot_of_block[k] = slot;
+                    const int c = bcol[row_off[r] + t];
+                    P.slot_col[slot] = c;
+                    P.slot_flags[slot] = c == r ? (uint8_t)(kSlotDiag | (P.pinned[r] ? kSlotIdentity : 0)) : 0;
                 } else {
                     P.slot_col[slot] = r < N ? r : N - 1;  // zero block on a valid column
                 }
             }
-    // upper-triangle assembly list, slice by slice: the slice's diagonal blocks (long lists, one
-    // tile), then its off-diagonal blocks padded with empty entries to whole 64-wide tiles, so a
-    // tile's lists have similar lengths and consecutive entries share their elements
-    P.up_slot.clear();
-    P.up_mirror.clear();
-    P.up_node.clear();
-    P.up_flags.clear();
-    P.uctb_off.assign(1, 0);
-    P.uctb.clear();
-    auto pad_to_tile = [&]() {
-        while (P.up_slot.size() % kSlice) {
-            P.up_slot.push_back(0);
-            P.up_mirror.push_back(-1);
-            P.up_node.push_back(0);
-            P.up_flags.push_back(4);  // empty entry
-            P.uctb_off.push_back((uint32_t)P.uctb.size());
-        }
+    });
+
+    // --- slice assembly lists (SparseMatrixFixedPattern::build's source lists,
+    // src/SparseMatrixFixedPattern.cpp:85-93): per slice the distinct elements its rows touch,
+    // ascending; per slot its contributions (element, local row vertex a, local column vertex b)
+    // in ascending element order, as codes local-element << 4 | a << 2 | b, tiled [k][64] ---
+    struct SliceLists {
+        std::vector<int32_t> elem;
+        std::vector<uint16_t> width;  // per tile of the slice: max contributions of its slots
+        std::vector<uint16_t> code;   // the slice's tiles, each [width][64]
+        std::vector<uint8_t> cnt;     // [w * 64]
+        std::string err;
     };
-    for (int s = 0; s < P.num_slices; s++) {
-        for (int pass = 0; pass < 2; pass++) {
-            for (int t = 0; t < P.slice_width[s]; t++)
-                for (int l = 0; l < kSlice; l++) {
-                    int r = s * kSlice + l;
-                    if (r >= N || t >= row_off[r + 1] - row_off[r]) continue;
-                    int64_t k = row_off[r] + t;
-                    int c = bcol[k];
-                    if (c < r || (pass == 0) != (c == r)) continue;
-                    P.up_slot.push_back((int32_t)slot_of_block[k]);
-                    P.up_mirror.push_back(c == r ? -1 : (int32_t)slot_of_block[find_block(c, r)]);
-                    P.up_node.push_back(r);
-                    P.up_flags.push_back((uint8_t)((c == r ? 1 : 0) | (P.pinned[r] ? 2 : 0)));
-                    P.uctb.insert(P.uctb.end(), bctb.begin() + bctb_off[k], bctb.begin() + bctb_off[k + 1]);
-                    P.uctb_off.push_back((uint32_t)P.uctb.size());
+    std::vector<SliceLists> sl(P.num_slices);
+    parallel_for(P.num_slices, [&](int64_t s, std::vector<int32_t> &pos_k) {
+        SliceLists &o = sl[s];
+        const int w = P.slice_width[s];
+        // each row's contributions, element by element (ascending, the node lists' order), then
+        // column vertex b: for a fixed slot they arrive in ascending element order
+        auto each = [&](auto &&fn) {
+            for (int l = 0; l < kSlice; l++) {
+                const int r = (int)s * kSlice + l;
+                if (r >= N || P.pinned[r]) continue;
+                const int32_t *cb = bcol.data() + row_off[r], *ce = bcol.data() + row_off[r + 1];
+                for (uint32_t m = P.nctb_off[r]; m < P.nctb_off[r + 1]; m++) {
+                    const uint32_t e = P.nctb[m] >> 2, a = P.nctb[m] & 3u;
+                    for (uint32_t b = 0; b < 4; b++) {
+                        const int c = P.enodes[4 * (size_t)e + b];
+                        if (P.pinned[c]) continue;
+                        fn((int)(std::lower_bound(cb, ce, c) - cb) * kSlice + l, e, a << 2 | b);
+                    }
                 }
-            pad_to_tile();
+            }
+        };
+        o.cnt.assign((size_t)w * kSlice, 0);
+        bool over = false;
+        each([&](int slot, uint32_t e, uint32_t) {
+            if (o.cnt[slot] == 255) over = true;
+            else o.cnt[slot]++;
+            o.elem.push_back((int32_t)e);
+        });
+        if (over) {
+            o.err = "a matrix block has more than 254 element contributions";
+            return;
         }
+        std::sort(o.elem.begin(), o.elem.end());
+        o.elem.erase(std::unique(o.elem.begin(), o.elem.end()), o.elem.end());
+        o.elem.shrink_to_fit();
+        if (o.elem.size() > 4096) {
+            o.err = "a 64-row slice touches more than 4096 elements";
+            return;
+        }
+        o.width.assign(w, 0);
+        std::vector<size_t> base(w + 1, 0);
+        for (int t = 0; t < w; t++) {
+            for (int l = 0; l < kSlice; l++) o.width[t] = std::max<uint16_t>(o.width[t], o.cnt[(size_t)t * kSlice + l]);
+            base[t + 1] = base[t] + (size_t)o.width[t] * kSlice;
+        }
+        o.code.assign(base[w], 0);
+        pos_k.assign((size_t)w * kSlice, 0);
+        each([&](int slot, uint32_t e, uint32_t ab) {
+            const uint32_t le = (uint32_t)(std::lower_bound(o.elem.begin(), o.elem.end(), (int32_t)e) - o.elem.begin());
+            const int t = slot / kSlice, l = slot % kSlice;
+            o.code[base[t] + (size_t)pos_k[slot]++ * kSlice + l] = (uint16_t)(le << 4 | ab);
+        });
+    });
+    P.sa_eoff.assign((size_t)P.num_slices + 1, 0);
+    P.sa_toff.assign((size_t)(P.num_slots / kSlice) + 1, 0);
+    for (int s = 0; s < P.num_slices; s++) {
+        if (!sl[s].err.empty()) return sl[s].err;
+        P.sa_eoff[s + 1] = P.sa_eoff[s] + (int64_t)sl[s].elem.size();
+        const int64_t tile0 = P.slice_off[s] / kSlice;
+        for (int t = 0; t < P.slice_width[s]; t++)
+            P.sa_toff[tile0 + t + 1] = P.sa_toff[tile0 + t] + (int64_t)sl[s].width[t] * kSlice;
     }
-    P.num_upper = (int64_t)P.up_slot.size();
+    P.sa_elem.resize(P.sa_eoff[P.num_slices]);
+    P.sa_code.resize(P.sa_toff.back());
+    P.sa_cnt.resize(P.num_slots);
+    parallel_for(P.num_slices, [&](int64_t s, std::vector<int32_t> &) {
+        std::copy(sl[s].elem.begin(), sl[s].elem.end(), P.sa_elem.begin() + P.sa_eoff[s]);
+        std::copy(sl[s].code.begin(), sl[s].code.end(), P.sa_code.begin() + P.sa_toff[P.slice_off[s] / kSlice]);
+        std::copy(sl[s].cnt.begin(), sl[s].cnt.end(), P.sa_cnt.begin() + P.slice_off[s]);
+        SliceLists().elem.swap(sl[s].elem);
+        std::vector<uint16_t>().swap(sl[s].code);
+    });
     {
-        std::string m = make_tiles(P.uctb_off, P.uctb, P.utile);
-        if (!m.empty()) return m;
-        m = make_tiles(P.nctb_off, P.nctb, P.ntile);
+        std::string m = make_tiles(P.nctb_off, P.nctb, P.ntile);
         if (!m.empty()) return m;
     }
 

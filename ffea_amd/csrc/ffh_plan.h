@@ -11,14 +11,9 @@
 namespace ffh {
 
 constexpr int kSlice = 64;  // rows per SELL slice = one wavefront
+constexpr uint8_t kSlotDiag = 1, kSlotIdentity = 2, kSlotPad = 4;
 constexpr uint32_t kM1 = 4294967087u;
 constexpr uint32_t kM2 = 4294944443u;
-
-// Index of the 3x3 block (a,b), a<=b, of an element's 12x12 viscosity matrix.
-inline int pair_index(int a, int b) {
-    static const int tbl[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
-    return tbl[a][b];
-}
 
 struct Mat3u {
     uint32_t m[3][3];
@@ -73,16 +68,17 @@ struct Plan {
     std::vector<int64_t> slice_off;  // [num_slices+1] first slot of each slice
     std::vector<int32_t> slice_width;
     std::vector<int32_t> slot_col;    // [num_slots]
-    // assembly over the upper triangle (row <= col): each entry fills its SELL slot and,
-    // off the diagonal, the mirrored slot (col,row) with the transposed block
-    int64_t num_upper = 0;
-    std::vector<int32_t> up_slot;     // [num_upper] SELL slot of (row, col)
-    std::vector<int32_t> up_mirror;   // [num_upper] SELL slot of (col, row), -1 on the diagonal
-    std::vector<int32_t> up_node;     // [num_upper] row node (diagonal: its Stokes drag and D^-1)
-    std::vector<uint8_t> up_flags;    // bit0 diagonal, bit1 identity (pinned row), bit2 empty (padding)
-    std::vector<uint32_t> uctb_off;   // [num_upper+1]
-    std::vector<uint32_t> uctb;       // e << 4 | a << 2 | b (local row / col node), ascending e
-    Tiles utile;                      // the same lists, tiled per 64 upper entries
+    // slot_flags: kSlotDiag (the row's diagonal block: Stokes drag and D^-1), kSlotIdentity (a
+    // pinned row's diagonal: identity), kSlotPad (padding: zero for good)
+    std::vector<uint8_t> slot_flags;  // [num_slots]
+    // slice assembly: per slice its distinct elements (ascending), per slot its contributions
+    // (slice-local element << 4 | local row vertex a << 2 | local column vertex b, ascending
+    // element), tiled [k][64] per 64-slot tile; tile g's codes start at sa_toff[g]
+    std::vector<int64_t> sa_eoff;   // [num_slices + 1]
+    std::vector<int32_t> sa_elem;   // global element ids
+    std::vector<int64_t> sa_toff;   // [num_slots / 64 + 1]
+    std::vector<uint16_t> sa_code;
+    std::vector<uint8_t> sa_cnt;    // [num_slots]
     // RNG jump tables: tbl[((d*256 + v)*2 + comp)] = A_comp^(v * 256^d), d=0..3
     std::vector<Mat3u> jump;
     Mat3u step_jump[2];        // A^(draws per step)

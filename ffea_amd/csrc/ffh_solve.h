@@ -8,9 +8,13 @@
 // RHS: force gather, Stokes noise, pinned rows, CG initialisation.  64 nodes x 3 components
 // per block (wave = component).
 // ----------------------------------------------------------------------------------------
+// The CG work vectors r, p, q, z (and r_prev) hold V values: double on the FP64 path and for
+// the mass-ful solvers; float on the mixed path, whose iterate x, right-hand side f, dot
+// products and true residual stay FP64.
 struct CgArgs {
     int N, nslices;
-    double *f, *r, *x, *p, *q, *z, *vel;
+    double *f, *x, *vel;
+    void *r, *p, *q, *z;
     const double *dinv;
     double *parts_rz, *parts_rr, *parts_ff, *parts_pq;  // one partial per slice
     const int32_t *slice_blob, *blob_slice_off;
@@ -19,8 +23,21 @@ struct CgArgs {
     CgCtl *ctl;  // one per blob
     // learned preconditioner (nullptr otherwise): partials of |D^-1 r|^2, r before the update,
     // partials of z.r_prev for the flexible beta
-    double *parts_uu, *r_prev, *parts_zrp;
+    double *parts_uu;
+    void *r_prev;
+    double *parts_zrp;
     const double *tot;  // per-blob totals (large blobs) or nullptr: consumers add the partials
+    // mixed path: partials of |f - K x|^2 of a true-residual pass, and its period (0: only when
+    // the recurrence residual passes the stopping test)
+    double *parts_tr;
+    int rr_period;
+};
+
+// Vector type and true-residual scheme of the CG kernels for element/matrix type T.
+template <typename T, bool kMass>
+struct CgVec {
+    static constexpr bool kMixed = !kMass && std::is_same<T, float>::value;
+    using V = typename std::conditional<kMixed, float, double>::type;
 };
 
 
@@ -45,7 +62,7 @@ struct RhsArgs {
     ForceArgs fa;          // springs, constant and surface forces
 };
 
-template <typename T>
+template <typename T, typename V>
 __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
     __shared__ double sh[4];
     __shared__ uint32_t rs[6][64];
@@ -118,14 +135,15 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         v.f[k] = f;
         v.x[k] = f * g.lumped[n];
     } else {
-        const double z = v.dinv[k] * f;
+        // stored values (rounded to V on the mixed path) are what the partials sum
+        const double r = (double)(V)f, z = (double)(V)(v.dinv[k] * f);
         v.f[k] = f;
-        v.r[k] = f;
+        static_cast<V *>(v.r)[k] = (V)r;
         v.x[k] = 0.0;
-        v.p[k] = 0.0;
-        v.q[k] = 0.0;
-        v.z[k] = z;
-        block_partial_at(f * z, v.parts_rz, blk, sh);
+        static_cast<V *>(v.p)[k] = (V)0.0;
+        static_cast<V *>(v.q)[k] = (V)0.0;
+        static_cast<V *>(v.z)[k] = (V)z;
+        block_partial_at(r * z, v.parts_rz, blk, sh);
         block_partial_at(f * f, v.parts_ff, blk, sh);
         if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
     }
@@ -173,35 +191,49 @@ __device__ __forceinline__ bool cg_stop(double res, int it, const CgArgs &v, boo
     return ok || it >= v.maxit;
 }
 
-template <typename T, bool kNt, bool kMass>
-__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
-    __shared__ double sh[4];
-    const int slice = xcd_block(v.nslices);
-    if (slice >= v.nslices) return;
-    const int b = v.slice_blob[slice];
-    CgCtl *ctl = &v.ctl[b];
-    if (ctl->done) return;
-    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
-    const bool decider = slice == s0 && threadIdx.x == 0;
-    double beta = 0.0;
+// Top of CG iteration `it`, shared by both SpMV variants (tot(parts, slot) = the blob's total
+// of a per-slice partial, in the variant's fixed order).  Returns false when the blob has
+// stopped or stops here.  Sets beta and, on the mixed path (kMixed), whether this pass also
+// forms the true residual f - K x: every rr_period iterations, and whenever the recurrence
+// residual passes the stopping test -- the update kernel then stops only if the true residual
+// passes it too (the final epsilon check of the FP64 path is on the recurrence residual, as
+// NoMassCGSolver::solve's is).
+template <bool kMass, bool kMixed, typename Tot>
+__device__ __forceinline__ bool cg_head(const CgArgs &v, CgCtl *ctl, int it, int mode, bool decider, Tot &&tot,
+                                        double &beta, bool &check) {
+    beta = 0.0;
+    check = false;
     if (it == 0) {
-        const double f2 = blob_total(v.parts_ff, v.tot, kTotFF, b, s0, s1, sh);
-        const double d0 = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
+        const double f2 = tot(v.parts_ff, kTotFF);
+        const double d0 = tot(v.parts_rz, kTotRZ);
         const bool stop = cg_stop_first<kMass>(f2, v.eps2);
         if (decider) {
             ctl->f2 = f2;
             ctl->delta[0] = d0;
+            ctl->check = 0;
             if (stop) {  // zero force: zero solution
                 ctl->done = 1;
                 ctl->status = 0;
                 ctl->iter = 0;
             }
         }
-        if (stop) return;
+        return !stop;
+    }
+    const double f2 = ctl->f2;
+    const double r2 = tot(v.parts_rr, kTotRR);
+    const double res = r2 / f2;
+    if constexpr (kMixed) {
+        check = res < v.eps2 || (v.rr_period > 0 && it % v.rr_period == 0);
+        if (!(res < v.eps2) && it >= v.maxit) {
+            if (decider) {
+                ctl->done = 1;
+                ctl->status = -1;
+                ctl->iter = it;
+                ctl->res = res;
+            }
+            return false;
+        }
     } else {
-        const double f2 = ctl->f2;
-        const double r2 = blob_total(v.parts_rr, v.tot, kTotRR, b, s0, s1, sh);
-        const double res = r2 / f2;
         bool ok;
         if (cg_stop<kMass>(res, it, v, ok)) {
             if (decider) {
@@ -210,22 +242,44 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
                 ctl->iter = it;
                 ctl->res = res;
             }
-            return;
+            return false;
         }
-        const double dnew = blob_total(v.parts_rz, v.tot, kTotRZ, b, s0, s1, sh);
-        if (mode == 0)
-            beta = dnew / ctl->delta[(it - 1) & 1];
-        else if (mode == 1)
-            beta = (dnew - blob_total(v.parts_zrp, v.tot, kTotZRP, b, s0, s1, sh)) / ctl->delta[(it - 1) & 1];
-        if (decider) ctl->delta[it & 1] = dnew;
     }
+    const double dnew = tot(v.parts_rz, kTotRZ);
+    if (mode == 0)
+        beta = dnew / ctl->delta[(it - 1) & 1];
+    else if (mode == 1)
+        beta = (dnew - tot(v.parts_zrp, kTotZRP)) / ctl->delta[(it - 1) & 1];
+    if (decider) {
+        ctl->delta[it & 1] = dnew;
+        ctl->check = check ? 1 : 0;
+    }
+    return true;
+}
+
+template <typename T, bool kNt, bool kMass>
+__global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
+    using V = typename CgVec<T, kMass>::V;
+    constexpr bool kMixed = CgVec<T, kMass>::kMixed;
+    __shared__ double sh[4];
+    const int slice = xcd_block(v.nslices);
+    if (slice >= v.nslices) return;
+    const int b = v.slice_blob[slice];
+    CgCtl *ctl = &v.ctl[b];
+    if (ctl->done) return;
+    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
+    double beta;
+    bool check;
+    auto tot = [&](const double *parts, int slot) { return blob_total(parts, v.tot, slot, b, s0, s1, sh); };
+    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, beta, check)) return;
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
     const int64_t base = m.slice_off[slice];
     const T *val = reinterpret_cast<const T *>(m.val);
-    const double *__restrict__ z = v.z;
-    double acc = 0.0;
+    const V *__restrict__ z = static_cast<const V *>(v.z);
+    const double *__restrict__ x = v.x;
+    double acc = 0.0, accx = 0.0;
     if constexpr (kMass) {
         const double *mv = reinterpret_cast<const double *>(m.val);
 #pragma unroll 4
@@ -234,6 +288,20 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             const int c = stream_load<kNt>(m.col + tile + lane);
             acc += stream_load<kNt>(mv + tile + lane) * z[3 * c + ci];
         }
+    } else if (kMixed && check) {  // K z and K x from one pass over the matrix
+#pragma unroll 4
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
+            const double v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
+            acc += v0 * (double)z[3 * c + 0];
+            acc += v1 * (double)z[3 * c + 1];
+            acc += v2 * (double)z[3 * c + 2];
+            accx += v0 * x[3 * c + 0];
+            accx += v1 * x[3 * c + 1];
+            accx += v2 * x[3 * c + 2];
+        }
     } else {
 #pragma unroll 4
         for (int t = 0; t < w; t++) {
@@ -241,17 +309,23 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             const int c = stream_load<kNt>(m.col + tile + lane);
             const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
             const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
-            acc += (double)v0 * z[3 * c + 0];
-            acc += (double)v1 * z[3 * c + 1];
-            acc += (double)v2 * z[3 * c + 2];
+            acc += (double)v0 * (double)z[3 * c + 0];
+            acc += (double)v1 * (double)z[3 * c + 1];
+            acc += (double)v2 * (double)z[3 * c + 2];
         }
     }
     const size_t k = 3 * (size_t)row + ci;
-    const double pn = z[k] + beta * v.p[k];
-    const double qn = acc + beta * v.q[k];
-    v.p[k] = pn;
-    v.q[k] = qn;
+    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
+    const double pn = (double)(V)((double)z[k] + beta * (double)p[k]);
+    const double qn = (double)(V)(acc + beta * (double)q[k]);
+    p[k] = (V)pn;
+    q[k] = (V)qn;
     block_partial_at(pn * qn, v.parts_pq, slice, sh);
+    if (kMixed && check) {
+        const double rt = v.f[k] - accx;
+        static_cast<V *>(v.r)[k] = (V)rt;
+        block_partial_at(rt * rt, v.parts_tr, slice, sh);
+    }
 }
 
 // Sum of parts[i0, i1) in exactly the order parts_range gives with a 192-thread block, computed
@@ -275,60 +349,30 @@ __device__ __forceinline__ double blob_total_as192(const double *parts, const do
 
 // k_cg_spmv with one 64-lane wave per slice: each lane computes the 3 components of its row, so
 // the column index and the gathered z block are loaded once instead of three times.  Row sums,
-// reductions and the partial of p.q are formed in k_cg_spmv's exact order, so both variants
-// give bit-identical results (the tuner may pick either).
+// reductions and the partials are formed in k_cg_spmv's exact order, so both variants give
+// bit-identical results (the tuner may pick either).
 template <typename T, bool kNt, bool kMass>
 __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, int mode) {
+    using V = typename CgVec<T, kMass>::V;
+    constexpr bool kMixed = CgVec<T, kMass>::kMixed;
     const int slice = xcd_block(v.nslices);
     if (slice >= v.nslices) return;
     const int b = v.slice_blob[slice];
     CgCtl *ctl = &v.ctl[b];
     if (ctl->done) return;
     const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
-    const bool decider = slice == s0 && threadIdx.x == 0;
-    double beta = 0.0;
-    if (it == 0) {
-        const double f2 = blob_total_as192(v.parts_ff, v.tot, kTotFF, b, s0, s1);
-        const double d0 = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
-        const bool stop = cg_stop_first<kMass>(f2, v.eps2);
-        if (decider) {
-            ctl->f2 = f2;
-            ctl->delta[0] = d0;
-            if (stop) {
-                ctl->done = 1;
-                ctl->status = 0;
-                ctl->iter = 0;
-            }
-        }
-        if (stop) return;
-    } else {
-        const double f2 = ctl->f2;
-        const double r2 = blob_total_as192(v.parts_rr, v.tot, kTotRR, b, s0, s1);
-        const double res = r2 / f2;
-        bool ok;
-        if (cg_stop<kMass>(res, it, v, ok)) {
-            if (decider) {
-                ctl->done = 1;
-                ctl->status = ok ? 0 : -1;
-                ctl->iter = it;
-                ctl->res = res;
-            }
-            return;
-        }
-        const double dnew = blob_total_as192(v.parts_rz, v.tot, kTotRZ, b, s0, s1);
-        if (mode == 0)
-            beta = dnew / ctl->delta[(it - 1) & 1];
-        else if (mode == 1)
-            beta = (dnew - blob_total_as192(v.parts_zrp, v.tot, kTotZRP, b, s0, s1)) / ctl->delta[(it - 1) & 1];
-        if (decider) ctl->delta[it & 1] = dnew;
-    }
+    double beta;
+    bool check;
+    auto tot = [&](const double *parts, int slot) { return blob_total_as192(parts, v.tot, slot, b, s0, s1); };
+    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, beta, check)) return;
     const int lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
     const int64_t base = m.slice_off[slice];
     const T *val = reinterpret_cast<const T *>(m.val);
-    const double *__restrict__ z = v.z;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const V *__restrict__ z = static_cast<const V *>(v.z);
+    const double *__restrict__ x = v.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, x0 = 0.0, x1 = 0.0, x2 = 0.0;
     if constexpr (kMass) {
         const double *mv = reinterpret_cast<const double *>(m.val);
 #pragma unroll 4
@@ -340,44 +384,87 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             a1 += a * z[3 * c + 1];
             a2 += a * z[3 * c + 2];
         }
+    } else if (kMixed && check) {  // K z and K x from one pass over the matrix
+#pragma unroll 2
+        for (int t = 0; t < w; t++) {
+            const int64_t tile = base + (int64_t)t * 64;
+            const int c = stream_load<kNt>(m.col + tile + lane);
+            const T *vt = val + tile * 9 + lane;
+            double vv[9];
+#pragma unroll
+            for (int kk = 0; kk < 9; kk++) vv[kk] = (double)stream_load<kNt>(vt + 64 * kk);
+            const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
+            const double y0 = x[3 * c + 0], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
+            a0 += vv[0] * z0;
+            a0 += vv[1] * z1;
+            a0 += vv[2] * z2;
+            a1 += vv[3] * z0;
+            a1 += vv[4] * z1;
+            a1 += vv[5] * z2;
+            a2 += vv[6] * z0;
+            a2 += vv[7] * z1;
+            a2 += vv[8] * z2;
+            x0 += vv[0] * y0;
+            x0 += vv[1] * y1;
+            x0 += vv[2] * y2;
+            x1 += vv[3] * y0;
+            x1 += vv[4] * y1;
+            x1 += vv[5] * y2;
+            x2 += vv[6] * y0;
+            x2 += vv[7] * y1;
+            x2 += vv[8] * y2;
+        }
     } else {
 #pragma unroll 2
         for (int t = 0; t < w; t++) {
             const int64_t tile = base + (int64_t)t * 64;
             const int c = stream_load<kNt>(m.col + tile + lane);
             const T *vt = val + tile * 9 + lane;
-            T v[9];
+            T vv[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) v[k] = stream_load<kNt>(vt + 64 * k);
+            for (int kk = 0; kk < 9; kk++) vv[kk] = stream_load<kNt>(vt + 64 * kk);
             const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-            a0 += (double)v[0] * z0;
-            a0 += (double)v[1] * z1;
-            a0 += (double)v[2] * z2;
-            a1 += (double)v[3] * z0;
-            a1 += (double)v[4] * z1;
-            a1 += (double)v[5] * z2;
-            a2 += (double)v[6] * z0;
-            a2 += (double)v[7] * z1;
-            a2 += (double)v[8] * z2;
+            a0 += (double)vv[0] * z0;
+            a0 += (double)vv[1] * z1;
+            a0 += (double)vv[2] * z2;
+            a1 += (double)vv[3] * z0;
+            a1 += (double)vv[4] * z1;
+            a1 += (double)vv[5] * z2;
+            a2 += (double)vv[6] * z0;
+            a2 += (double)vv[7] * z1;
+            a2 += (double)vv[8] * z2;
         }
     }
     const size_t k = 3 * (size_t)row;
-    const double acc[3] = {a0, a1, a2};
-    double part = 0.0;
+    const double acc[3] = {a0, a1, a2}, accx[3] = {x0, x1, x2};
+    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
+    double part = 0.0, tpart = 0.0;
 #pragma unroll
     for (int ci = 0; ci < 3; ci++) {
-        const double pn = z[k + ci] + beta * v.p[k + ci];
-        const double qn = acc[ci] + beta * v.q[k + ci];
-        v.p[k + ci] = pn;
-        v.q[k + ci] = qn;
+        const double pn = (double)(V)((double)z[k + ci] + beta * (double)p[k + ci]);
+        const double qn = (double)(V)(acc[ci] + beta * (double)q[k + ci]);
+        p[k + ci] = (V)pn;
+        q[k + ci] = (V)qn;
         part += __shfl(wave_sum(pn * qn), 0, 64);  // ((0 + W0) + W1) + W2, as the 3-wave block
+        if (kMixed && check) {
+            const double rt = v.f[k + ci] - accx[ci];
+            static_cast<V *>(v.r)[k + ci] = (V)rt;
+            tpart += __shfl(wave_sum(rt * rt), 0, 64);
+        }
     }
-    if (lane == 0) v.parts_pq[slice] = part;
+    if (lane == 0) {
+        v.parts_pq[slice] = part;
+        if (kMixed && check) v.parts_tr[slice] = tpart;
+    }
 }
 
 // alpha = delta / p.q; x += alpha p; r -= alpha q (vec3_add_to_scaled, src/mat_vec_fns.cpp:250-260);
 // z = D^-1 r (parallel_apply_preconditioner, src/NoMassCGSolver.cpp:278-292); partials of r.r
-// and r.z.  One block = the 3 x 64 scalars of one slice.
+// and r.z.  One block = the 3 x 64 scalars of one slice.  On the mixed path (kMixed) a pass whose
+// SpMV formed the true residual first decides on it: the blob stops if |f - K x|^2 / |f|^2 <
+// eps^2 (or fails at max_iterations_cg); otherwise r, which now holds f - K x, is updated from
+// there (residual replacement).
+template <typename V, bool kMixed>
 __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     __shared__ double sh[4];
     const int slice = xcd_block(v.nslices);
@@ -385,20 +472,36 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     const int b = v.slice_blob[slice];
     CgCtl *ctl = &v.ctl[b];
     if (ctl->done) return;
-    const double alpha =
-        ctl->delta[it & 1] / blob_total(v.parts_pq, v.tot, kTotPQ, b, v.blob_slice_off[b], v.blob_slice_off[b + 1], sh);
+    const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
+    if constexpr (kMixed) {
+        if (ctl->check) {
+            const double res = blob_total(v.parts_tr, v.tot, kTotTR, b, s0, s1, sh) / ctl->f2;
+            const bool ok = res < v.eps2;
+            if (ok || it >= v.maxit) {
+                if (slice == s0 && threadIdx.x == 0) {
+                    ctl->done = 1;
+                    ctl->status = ok ? 0 : -1;
+                    ctl->iter = it;
+                    ctl->res = res;
+                }
+                return;
+            }
+        }
+    }
+    const double alpha = ctl->delta[it & 1] / blob_total(v.parts_pq, v.tot, kTotPQ, b, s0, s1, sh);
     const size_t k = (size_t)slice * 192 + threadIdx.x;
-    v.x[k] += alpha * v.p[k];
-    const double r0 = v.r[k];
-    const double r = r0 + (-alpha) * v.q[k];
-    v.r[k] = r;
-    const double z = v.dinv[k] * r;
-    v.z[k] = z;
-    block_partial_at(r * r, v.parts_rr, slice, sh);
-    block_partial_at(r * z, v.parts_rz, slice, sh);
+    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q), *r = static_cast<V *>(v.r), *z = static_cast<V *>(v.z);
+    v.x[k] += alpha * (double)p[k];
+    const double r0 = (double)r[k];
+    const double rn = (double)(V)(r0 + (-alpha) * (double)q[k]);
+    r[k] = (V)rn;
+    const double zn = (double)(V)(v.dinv[k] * rn);
+    z[k] = (V)zn;
+    block_partial_at(rn * rn, v.parts_rr, slice, sh);
+    block_partial_at(rn * zn, v.parts_rz, slice, sh);
     if (v.parts_uu) {  // learned preconditioner follows: z holds u = D^-1 r
-        v.r_prev[k] = r0;
-        block_partial_at(z * z, v.parts_uu, slice, sh);
+        static_cast<V *>(v.r_prev)[k] = (V)r0;
+        block_partial_at(zn * zn, v.parts_uu, slice, sh);
     }
 }
 

@@ -34,6 +34,8 @@ class Mesh:
     stokes_radius: np.ndarray  # [N_t] float64, input units
     num_linear: int           # linear nodes are 0..num_linear-1
     pinned: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    # [N_t - num_linear, 2] edge ends of each midpoint node, when known (box meshes)
+    mid_ends: np.ndarray | None = None
 
     @property
     def num_nodes(self) -> int:
@@ -46,6 +48,10 @@ class Mesh:
     def linearise(self, pos: np.ndarray | None = None) -> np.ndarray:
         """Midpoints re-averaged from their edge ends (linearise_element)."""
         p = (self.pos if pos is None else pos).copy()
+        if self.mid_ends is not None:  # one write per midpoint (the same value every element writes)
+            e = self.mid_ends
+            p[self.num_linear:] = 0.5 * (p[e[:, 0]] + p[e[:, 1]])
+            return p
         for m, (a, b) in enumerate(EDGES):
             p[self.topo[:, 4 + m]] = 0.5 * (p[self.topo[:, a]] + p[self.topo[:, b]])
         return p
@@ -112,18 +118,30 @@ def box_mesh(nx: int, ny: int, nz: int, h: float = 1.0, material=DEFAULT_MATERIA
         else:
             raise RuntimeError("could not jitter mesh without inverting elements")
 
-    # unique edges -> midpoint ids appended after the linear nodes
-    e_all = np.stack([np.sort(tet4[:, list(ab)], axis=1) for ab in EDGES], axis=1)  # [E,6,2]
-    key = e_all[..., 0].astype(np.int64) * n_lin + e_all[..., 1]
-    uniq, inv = np.unique(key.ravel(), return_inverse=True)
-    mids = n_lin + inv.reshape(-1, 6)
-    a, b = uniq // n_lin, uniq % n_lin
+    # Midpoint ids appended after the linear nodes, one per edge in ascending (low, high)
+    # vertex order.  Every edge of a Kuhn box runs from its lower vertex a to a + o with o one
+    # of 7 offsets, ascending: +x, +y, +x+y, +z, +x+z, +y+z, +x+y+z, present wherever the cells
+    # it borders exist; so the sorted edge list is counted per vertex instead of sorted.
+    offs = np.array([1, sx, 1 + sx, sy, 1 + sy, sx + sy, 1 + sx + sy], np.int64)
+    gi, gj, gk = (np.arange(n_lin) % sx, (np.arange(n_lin) // sx) % (ny + 1), np.arange(n_lin) // sy)
+    ex, ey, ez = gi < nx, gj < ny, gk < nz
+    has = np.stack([ex, ey, ex & ey, ez, ex & ez, ey & ez, ex & ey & ez], axis=1)  # [n_lin, 7]
+    rank = np.cumsum(has, axis=1) - has
+    start = np.concatenate([[0], np.cumsum(has.sum(axis=1))[:-1]])
+    mid_of = n_lin + start[:, None] + rank  # valid where has
+    mids = np.empty((tet4.shape[0], 6), np.int64)
+    for m, (ea, eb) in enumerate(EDGES):
+        lo, hi = np.minimum(tet4[:, ea], tet4[:, eb]), np.maximum(tet4[:, ea], tet4[:, eb])
+        mids[:, m] = mid_of[lo, np.searchsorted(offs, hi - lo)]
+    a = np.repeat(np.arange(n_lin), has.sum(axis=1))
+    b = a + offs[np.nonzero(has)[1]]
     pos = np.concatenate([lin, 0.5 * (lin[a] + lin[b])], axis=0)
     topo = np.concatenate([tet4, mids], axis=1).astype(np.int32)
     mat = np.tile(np.asarray(material, float), (topo.shape[0], 1))
     rad = np.zeros(pos.shape[0])
     rad[:n_lin] = stokes_radius
-    return Mesh(pos=pos, topo=topo, material=mat, stokes_radius=rad, num_linear=n_lin)
+    return Mesh(pos=pos, topo=topo, material=mat, stokes_radius=rad, num_linear=n_lin,
+                mid_ends=np.stack([a, b], axis=1).astype(np.int32))
 
 
 def perturb(mesh: Mesh, sigma: float, seed: int) -> np.ndarray:

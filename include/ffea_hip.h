@@ -123,16 +123,16 @@ int ffh_blob_update_positions(ffh_blob *b);
 int ffh_blob_step(ffh_blob *b, int nsteps);
 
 /* Launch shapes, tuned at creation by timing this blob's own kernels (set FFH_AUTOTUNE=0 in
- * the environment to keep the defaults 256 / 256 / 2 / 3 / 0): threads per workgroup of the element
- * and assembly kernels (64, 128 or 256), the assembly variant (0: two-phase, a workgroup
- * evaluates a tile's contributions in parallel and each block's lane adds them in order; 1, 2,
- * 4: one lane per block with that loop unroll depth), waves per 64-row SpMV slice (3: one per
- * component; 1: one wave computes all three) and non-temporal matrix loads in the SpMV (1: the
- * streamed matrix leaves the caches to the gathered vector; wins when it streams from HBM, loses
- * when it fits the Infinity Cache).  Results are bit-identical for every choice. */
-int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_unroll,
+ * the environment to keep the defaults 256 / 512 / 2 / 3 / 0): threads per workgroup of the
+ * element kernel (64, 128 or 256); threads per workgroup of the assembly (256 or 512: one
+ * workgroup per 64-row slice) and the 64-slot tiles each assembly thread sums per pass (1, 2 or
+ * 4); waves per 64-row SpMV slice (3: one per component; 1: one wave computes all three) and
+ * non-temporal matrix loads in the SpMV (1: the streamed matrix leaves the caches to the
+ * gathered vector; wins when it streams from HBM, loses when it fits the Infinity Cache).
+ * Results are bit-identical for every choice. */
+int ffh_blob_tuning(ffh_blob *b, int *element_block, int *assembly_block, int *assembly_tiles,
                     int *spmv_waves_per_slice, int *spmv_nontemporal);
-int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_unroll,
+int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int assembly_tiles,
                         int spmv_waves_per_slice, int spmv_nontemporal);
 
 /*

@@ -79,12 +79,13 @@ def test_gradient_matches_oracle(oracle):
         assert np.linalg.norm(pg[k] - po[k]) <= 3e-2 * np.linalg.norm(po[k]) + 1e-12, k
 
 
-def test_training_reduces_loss_and_learned_solve_converges(oracle):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_training_reduces_loss_and_learned_solve_converges(oracle, precision):
     """Harvest (residual, correction) pairs from plain solves, train, and solve with the
     learned flexible CG: same stopping test, same solution within 1e-4."""
     from ffea_amd.blob import Blob
     m, prm, sc, x0 = box_case((6, 5, 4), noise=1, sigma=0.05, seed=11, jitter=0.1)
-    b = Blob(m, prm, sc, rng_state=np.arange(1, 7, dtype=np.uint64), pos=x0)
+    b = Blob(m, prm, sc, rng_state=np.arange(1, 7, dtype=np.uint64), pos=x0, precision=precision)
     b.mpnn_init(seed=2)
     b.mpnn_harvest(200)
     b.step(12)

@@ -146,12 +146,13 @@ def test_inversion_is_reported():
         b.step(1)
 
 
-def test_cg_nonconvergence_is_reported():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cg_nonconvergence_is_reported(precision):
     from ffea_amd import _lib
     m, prm, sc, x0 = box_case((3, 3, 3), noise=1)
     prm.max_iterations_cg = 1
     prm.epsilon2 = 1e-30
-    b = gpu_blob(m, prm, sc, np.array([1, 2, 3, 4, 5, 6], np.uint64), x0)
+    b = gpu_blob(m, prm, sc, np.array([1, 2, 3, 4, 5, 6], np.uint64), x0, precision)
     before = b.positions()
     with pytest.raises(_lib.CGError):
         b.step(1)
@@ -235,8 +236,8 @@ def test_tuned_launch_shapes_give_identical_results(precision, batch):
         def make():
             return gpu_blob(m, prm, sc, seed, x0, precision)
     ref = None
-    for shape in ((256, 256, 2, 3, 0), (64, 128, 1, 1, 1), (128, 64, 4, 3, 1), (256, 64, 4, 1, 0),
-                  (128, 256, 0, 1, 1)):
+    for shape in ((256, 512, 2, 3, 0), (64, 256, 1, 1, 1), (128, 512, 4, 3, 1), (256, 256, 4, 1, 0),
+                  (128, 512, 1, 1, 1), (64, 256, 2, 3, 0)):
         b = make()
         b.set_tuning(*shape)
         assert b.tuning() == dict(zip(b.TUNING_KEYS, shape))
@@ -293,3 +294,51 @@ def test_midpoint_external_force_goes_to_edge_ends():
         assert np.allclose(d[n], 0.5 * F[mid], rtol=0, atol=1e-12 * scale)
     others = np.setdiff1d(np.arange(m.num_nodes), ends)
     assert np.abs(d[others]).max() <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("period", [0, 3, 8])
+def test_mixed_cg_stops_on_the_true_residual(period, monkeypatch):
+    """The mixed path's CG (FP32 matrix and work vectors; FP64 iterate, dot products and true
+    residual) stops only when |f - K x|^2 / |f|^2 < eps^2 for the FP64 residual of its own FP32
+    matrix, formed every `period` iterations (0: only when the FP32 recurrence passes) and at
+    every candidate stop.  K x is checked independently through apply_matrix (the same FP32
+    blocks, rows summed in FP64).  Its solution agrees with the FP64 path's within 2e-5: both
+    meet eps = 1e-6 on matrices that differ by FP32 rounding, with kappa(D^-1 K) of a few."""
+    monkeypatch.setenv("FFH_RR_PERIOD", str(period))
+    m, prm, sc, x0 = box_case((6, 5, 4), noise=1, jitter=0.1, seed=5)
+    seed = np.arange(1, 7, dtype=np.uint64)
+    f = np.random.default_rng(7).normal(size=(m.num_nodes, 3))
+    f[m.num_linear:] = 0.0
+    f = f.ravel()
+    out = {}
+    for prec in ("fp64", "mixed"):
+        b = gpu_blob(m, prm, sc, seed, x0, prec)
+        assert b.update_internal_forces() == 0
+        v = b.solve(f)
+        it, res = b.last_cg()
+        Kv = b.apply_matrix(v)
+        true = np.sum((f - Kv) ** 2) / np.sum(f ** 2)
+        print(prec, "period", period, "iterations", it, "reported", res, "true", true, flush=True)
+        assert true < prm.epsilon2
+        if prec == "mixed":  # the stopping test was on the true residual itself
+            assert res == pytest.approx(true, rel=1e-9)
+        out[prec] = v
+    assert rel(out["mixed"], out["fp64"]) < 2e-5
+
+
+def test_mixed_cg_trajectory_against_fp64():
+    """Noisy steps on the mixed path against the FP64 path from the same state: the force
+    vectors agree to FP32 element rounding, velocities within 1e-4, and every solve converges."""
+    m, prm, sc, x0 = box_case((8, 8, 8), noise=1)
+    seed = np.arange(2, 8, dtype=np.uint64)
+    a = gpu_blob(m, prm, sc, seed, x0, "fp64")
+    b = gpu_blob(m, prm, sc, seed, x0, "mixed")
+    xs = x0.ravel() * sc
+    for k in range(10):
+        a.step(1)
+        b.step(1)
+        assert rel(b.rhs(), a.rhs()) < 1e-5, k
+        assert rel(b.velocities(), a.velocities()) < 1e-4, k
+        pa = a.positions().ravel()
+        assert np.linalg.norm(b.positions().ravel() - pa) <= 1e-4 * np.linalg.norm(pa - xs), k
+    assert np.array_equal(a.rng_state(), b.rng_state())
