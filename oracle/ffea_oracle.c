@@ -1762,6 +1762,18 @@ int orc_blob_forces_only(orc_blob *b, orc_stream *g, double *out_rhs) {
     return inv;
 }
 
+/* orc_blob_forces_only with the element phase drawing from g_elem and the force gather's
+ * Stokes noise from g_node: blob k of a World whose blobs share one stream draws its element
+ * noise after the earlier blobs' element draws and its node noise after every blob's element
+ * draws and the earlier blobs' node draws (World::run, src/World.cpp:2020-2041). */
+int orc_blob_forces_split(orc_blob *b, orc_stream *g_elem, orc_stream *g_node, double *out_rhs) {
+    blob_zero_forces(b);
+    int inv = orc_blob_update_internal_forces(b, g_elem);
+    blob_aggregate_forces(b, g_node);
+    memcpy(out_rhs, b->force, sizeof(double) * 3 * (size_t)b->num_nodes);
+    return inv;
+}
+
 /* The blob's solver on a caller RHS (in/out); NoMassCGSolver with the element matrices
  * currently held. */
 int orc_blob_solve(orc_blob *b, double *x) { return blob_solver_solve(b, x); }

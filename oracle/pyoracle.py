@@ -191,6 +191,16 @@ class Blob:
         inv = self.lib.fn("blob_forces_only")(self.h, stream.ptr, out)
         return inv, out
 
+    def forces_split(self, stream_elements, stream_nodes) -> tuple[int, np.ndarray]:
+        """forces_only with the element noise drawn from one stream and the node noise from
+        another (a blob of a multi-blob World: its two draw ranges are not adjacent)."""
+        out = np.zeros(3 * self.N, np.float64)
+        f = self.lib.lib.orc_blob_forces_split
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp]
+        f.restype = C.c_int
+        inv = f(self.h, stream_elements.ptr, stream_nodes.ptr, out)
+        return inv, out
+
     def solve(self, rhs) -> tuple[int, np.ndarray]:
         x = np.ascontiguousarray(rhs, np.float64).copy().ravel()
         st = self.lib.fn("blob_solve")(self.h, x)
