@@ -94,29 +94,49 @@ def make_case(cfg):
     return [m], prm, U.blob_scale(1e-9), [x0], desc
 
 
-def algorithmic_bytes(kernel, sizes, bm):
-    """Bytes each launch must move, each operand counted once (bm = bytes per matrix/element
-    value: 8 on the FP64 path, 4 on the mixed path; CG vectors are FP64 on both).
+RR_PERIOD = int(os.environ.get("FFH_RR_PERIOD", "16"))  # the library's true-residual period
 
-    N linear nodes, E elements, B 3x3 blocks, U = (B + N) / 2 upper-triangle blocks,
-    C = 10 E element-block contributions to them (4 diagonal + 6 off-diagonal per element)."""
+
+def algorithmic_bytes(kernel, sizes, bm, bv=8):
+    """Bytes each launch must move, each operand counted once (bm = bytes per matrix/element
+    value: 8 on the FP64 path, 4 on the mixed path; bv = bytes per CG work-vector entry r, p, q,
+    z: 8 on the FP64 path, 4 on the mixed path, whose iterate x, right-hand side f and D^-1 stay
+    FP64).
+
+    N linear nodes, E elements, B 3x3 blocks, S SELL slots (blocks + padding)."""
     N, E, B = sizes["num_linear"], sizes["num_elements"], sizes["num_blocks"]
-    U = (B + N) // 2
-    C = 10 * E
+    S = sizes.get("num_slots", B)
     v = 24 * N  # one FP64 3-vector over the linear nodes
+    w = 3 * bv * N  # one work vector
     return {
         # enodes 16, J0^-1 72, vol0 8, A B G E 32, last_det r+w 16; writes the 12 nodal forces
         "element": E * (16 + 72 + 8 + 32 + 16 + 12 * bm) + v,
-        # enodes + A, B and positions (the geometry is recomputed), contribution codes,
-        # per-entry tables; writes every block once + D^-1
-        "assemble": 32 * E + v + 4 * C + 15 * U + 9 * bm * B + 32 * N,
-        # element forces + codes + per-node data; writes f r x p q z
-        "rhs": 12 * bm * E + 16 * E + 39 * N + 6 * v,
+        # enodes + A, B once (32E), element lists (4E, each element once), contribution codes
+        # (16 per element, 2 B each), slot counts and flags (2S), positions (32N), drag (8N);
+        # writes every block once and D^-1 (24N)
+        "assemble": 68 * E + 2 * S + 64 * N + 9 * bm * B,
+        # element forces + codes + per-node data; writes f, x (FP64) and r, p, q, z
+        "rhs": 12 * bm * E + 16 * E + 39 * N + 2 * v + 4 * w,
         # matrix + column indices, z gathered once, p q read and written
-        "cg_spmv": 9 * bm * B + 4 * B + 5 * v,
-        "cg_update": 8 * v,
+        "cg_spmv": 9 * bm * B + 4 * B + 5 * w,
+        # x read and written, D^-1; p, q, z and r read and written
+        "cg_update": 3 * v + 5 * w,
         "integrate": 4 * v,
     }.get(kernel)
+
+
+def true_residual_bytes(sizes):
+    """Extra bytes of a mixed-path SpMV that also forms f - K x: x gathered and f read (FP64),
+    r written (FP32)."""
+    N = sizes["num_linear"]
+    return 24 * N + 24 * N + 12 * N
+
+
+def checks_per_solve(iters, period=RR_PERIOD):
+    """True-residual passes of a mixed-path solve of `iters` iterations: every `period`
+    iterations and at the final stop."""
+    n = int(round(iters))
+    return 1 + ((n - 1) // period if period > 0 else 0)
 
 
 def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
@@ -173,13 +193,18 @@ def algorithmic_bytes_mass(kernel, sizes, bm, solver):
     }.get(kernel)
 
 
-def kernel_table(kt, sizes, bm, solver="CG_nomass"):
+def kernel_table(kt, sizes, bm, solver="CG_nomass", bv=8, solves=0, iters=0.0):
+    """Per-kernel launches, average duration and achieved algorithmic GB/s.  On the mixed path
+    (bv = 4) the SpMV's bytes include its true-residual passes, checks_per_solve of them per
+    solve, spread over its launches."""
     out = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
         avg_s = ms / cnt / 1e3
-        byts = algorithmic_bytes(k, sizes, bm) if solver == "CG_nomass" else algorithmic_bytes_mass(k, sizes, bm, solver)
+        byts = algorithmic_bytes(k, sizes, bm, bv) if solver == "CG_nomass" else algorithmic_bytes_mass(k, sizes, bm, solver)
+        if k == "cg_spmv" and bv == 4 and solver == "CG_nomass" and solves:
+            byts += true_residual_bytes(sizes) * checks_per_solve(iters) * solves / cnt
         out[k] = {"launches": cnt, "avg_us": avg_s * 1e6, "total_ms": ms}
         if byts is not None:
             out[k].update(alg_bytes=int(byts), achieved_GBs=byts / avg_s / 1e9)
@@ -458,6 +483,8 @@ def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     setup = time.time() - t0
     log(f"[cpu {kind}] setup {setup:.1f}s")
     assert pyoracle.world_step(lib, blobs, s) == 0  # warm-up step
+    if kind == "reference":
+        lib.phase_times(reset=True)
     n, t0 = 0, time.perf_counter()
     while True:
         assert pyoracle.world_step(lib, blobs, s) == 0
@@ -465,6 +492,14 @@ def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
         el = time.perf_counter() - t0
         if el > max_seconds or n >= 50:
             break
+    phases = None
+    if kind == "reference":
+        ph = lib.phase_times(reset=True) / n
+        build = sum(lib.time_build(b) for b in blobs)
+        phases = {"element_loop_ms": 1e3 * ph[0], "force_gather_noise_ms": 1e3 * ph[1],
+                  "matrix_build_ms": 1e3 * build, "cg_ms": 1e3 * (ph[2] - build),
+                  "integration_ms": 1e3 * ph[3]}
+        log(f"[cpu {kind}] per step: {phases}")
     for b in blobs:
         b.close()
     del s
@@ -474,9 +509,69 @@ def _cpu_baseline(cfg, ms, prm, sc, xs, max_seconds=20.0):
     if note:
         part = note + f", rate scaled by {esub}/{etot}"
     return {"value": n / el * esub / etot, "unit": "steps/s", "cores": cores, "kind": kind,
+            "host": host_description(), "phases_per_step": phases,
             "sample": f"{n} noise-on steps of the same {cfg} workload{part} after 1 warm-up step "
                       f"({'reference objects -O3 -ffast-math, OpenMP within blob' if kind == 'reference' else 'FP64 restatement, 1 thread'}); "
                       f"host {platform.processor() or platform.machine()}, setup {setup:.0f}s excluded"}
+
+
+def host_description() -> dict:
+    """CPU model and logical CPU count of this host (the CPU baseline's machine)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": usable, "logical_cpus": os.cpu_count() or 1}
+
+
+def hbm_spmv_leg(steps=10, warmup=3):
+    """The FP32 block SpMV where its matrix streams from HBM: the c4 workload (192x96x48 box,
+    5.3 M tets; its FP32 matrix and columns, 540 MB, exceed the 256 MB Infinity Cache) on the
+    mixed path, per-launch SpMV times from the in-packet events; true residual only at each
+    solve's stop (FFH_RR_PERIOD=0), whose extra bytes are counted."""
+    from ffea_amd.blob import Blob, package_seed
+    ms, prm, sc, xs, desc = make_case("c4")
+    old = os.environ.get("FFH_RR_PERIOD")
+    os.environ["FFH_RR_PERIOD"] = "0"
+    try:
+        b = Blob(ms[0], prm, sc, rng_state=package_seed(prm.rng_seed), precision="mixed", pos=xs[0])
+    finally:
+        if old is None:
+            os.environ.pop("FFH_RR_PERIOD")
+        else:
+            os.environ["FFH_RR_PERIOD"] = old
+    b.step(warmup)
+    b.set_profiling(True)
+    its = []
+    for _ in range(steps):
+        b.step(1)
+        its.append(b.last_cg()[0])
+    b.synchronize()
+    kt = b.kernel_times()
+    b.set_profiling(False)
+    sizes = b.sizes()
+    tuning = b.tuning()
+    b.close()
+    ms_tot, cnt = kt["cg_spmv"]
+    avg_s = ms_tot / cnt / 1e3
+    alg = algorithmic_bytes("cg_spmv", sizes, 4, 4) + true_residual_bytes(sizes) * steps / cnt
+    out = {"workload": f"c4: {desc}", "precision": "mixed (FP32 matrix and work vectors)",
+           "launches": cnt, "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": int(alg),
+           "bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": alg / avg_s / 1e9 / HBM_PEAK_GBS, "cg_iters_mean": float(np.mean(its)),
+           "traffic": measured_traffic("c4", "cg_spmv", "mixed", tuning), "launch_shapes": tuning}
+    log(f"[hbm spmv c4 mixed] {out['avg_launch_us']:.1f} us per launch, {out['achieved']:.0f} GB/s = "
+        f"{out['frac']:.3f} of peak")
+    return out
 
 
 def equipartition(ms, prm, sc, xs, precision, steps=4000, start=2000, every=100):
@@ -655,15 +750,18 @@ def parity_vs_reference(cfg, ms, prm, sc, xs, steps):
 
 
 def kernel_symbol(kernel, precision, tuning):
-    """The kernel's name in rocprofv3 records for the launch shapes the tuner picked."""
+    """The kernel's name in rocprofv3 records (template arguments, no namespace or parameters)
+    for the launch shapes the tuner picked; massless solver."""
     t = "double" if precision == "fp64" else "float"
     u = tuning.get("assembly_tiles", 2)
     waves = tuning.get("assembly_block", 512) // 64
     nt = "true" if tuning.get("spmv_nontemporal") else "false"
     spmv = "k_cg_spmv1" if tuning.get("spmv_waves_per_slice") == 1 else "k_cg_spmv"
-    return {"element": f"k_element<{t}, {t}>",
+    mixed = precision != "fp64"
+    return {"element": f"k_element<{t}, {t}, false>",
             "assemble": f"k_assemble_slice<{t}, {waves}, {u}, true>",
-            "rhs": f"k_rhs<{t}>", "cg_spmv": f"{spmv}<{t}, {nt}>", "cg_update": "k_cg_update",
+            "rhs": f"k_rhs<{t}, {t}>", "cg_spmv": f"{spmv}<{t}, {nt}, false>",
+            "cg_update": "k_cg_update<float, true>" if mixed else "k_cg_update<double, false>",
             "integrate": "k_integrate"}.get(kernel)
 
 
@@ -805,6 +903,8 @@ def main():
                     help="timed steps of each mass-ful solver on the same workload (0: skip)")
     ap.add_argument("--ssint-steps", type=int, default=50,
                     help="timed steps of the Lennard-Jones surface-interaction workload (0: skip)")
+    ap.add_argument("--hbm-steps", type=int, default=10,
+                    help="c1: also time the FP32 SpMV of the c4 workload, whose matrix streams from HBM (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -824,7 +924,7 @@ def main():
     if not a.no_mixed:
         sps_m, wall_m, kt_m, sizes_m, iters_m, prof_m = run_gpu(ms, prm, sc, xs, "mixed", a.steps, a.warmup)
         mixed = {"steps_per_s": sps_m, "ns_per_day": sps_m * 1e-5 * 86400, "ms_per_step": 1e3 * wall_m / a.steps,
-                 "cg_iters_mean": iters_m, "kernels": kernel_table(kt_m, sizes_m, 4)}
+                 "cg_iters_mean": iters_m, "kernels": kernel_table(kt_m, sizes_m, 4, bv=4, solves=a.steps, iters=iters_m)}
     dom = max(ktab, key=lambda k: ktab[k]["total_ms"])
     d = ktab[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
@@ -843,6 +943,7 @@ def main():
         equi = {p: equipartition(ms, prm, sc, xs, p, a.equipartition_steps, a.equipartition_steps // 2)
                 for p in (("fp64",) if a.no_mixed else ("fp64", "mixed"))}
     calib = copy_calibration(a.copy_calibration) if a.copy_calibration else None
+    hbm = hbm_spmv_leg(a.hbm_steps, a.warmup) if (a.config == "c1" and a.hbm_steps > 0) else None
     mass = None
     if a.massful_steps > 0:
         mass = massful(ms, prm, sc, xs, a.massful_steps, a.warmup, 0.0 if a.no_cpu else min(a.cpu_seconds, 15.0))
@@ -872,6 +973,7 @@ def main():
         "kernels": ktab,
         "mixed": mixed,
         "roofline": roof,
+        "spmv_hbm": hbm,
         "cpu_baseline": cpu,
         "parity_vs_reference": parity,
         "mixed_vs_fp64": mvf,

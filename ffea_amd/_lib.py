@@ -43,7 +43,7 @@ EXPORTS = (
     "ffh_blob_set_velocities", "ffh_blob_kinetic_energy", "ffh_blob_set_springs",
     "ffh_blob_spring_energy", "ffh_blob_set_constant_forces", "ffh_blob_set_surface_forces",
     "ffh_blob_set_rotational_forces", "ffh_blob_set_box", "ffh_blob_set_ssint", "ffh_blob_ssint_energy",
-    "ffh_blob_ssint_energy_noalloc",
+    "ffh_blob_ssint_energy_noalloc", "ffh_blob_create_meso", "ffh_blob_update_matrix",
 )
 FFH_PRECOND_JACOBI = 0
 FFH_PRECOND_LEARNED = 1
@@ -89,6 +89,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_create.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, _ip, C.c_double,
                                   C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                   _u64p, C.c_int]
+    L.ffh_blob_create_meso.restype = C.c_void_p
+    L.ffh_blob_create_meso.argtypes = L.ffh_blob_create.argtypes
     L.ffh_batch_create.restype = C.c_void_p
     L.ffh_batch_create.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _dp, _dp, _ip, _ip, C.c_double,
                                    C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -144,10 +146,11 @@ def load(path: str | None = None) -> C.CDLL:
                                      C.c_double, C.c_int, C.c_int]
     L.ffh_blob_ssint_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_ssint_energy_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.ffh_blob_update_matrix.argtypes = [C.c_void_p]
     L.ffh_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
     for n in EXPORTS:
         if n not in ("ffh_last_error", "ffh_version", "ffh_blob_create", "ffh_blob_destroy",
-                     "ffh_blob_stream", "ffh_batch_create"):
+                     "ffh_blob_stream", "ffh_batch_create", "ffh_blob_create_meso"):
             getattr(L, n).restype = C.c_int
     if path is None:
         _lib = L

@@ -117,6 +117,16 @@ struct ffh_blob {
     double *zeta = nullptr, *dinv = nullptr;
     double *f = nullptr, *r = nullptr, *x = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *vel = nullptr;
     double *parts = nullptr, *tmp_in = nullptr, *tmp_out = nullptr;
+    // pinned host staging of the Solver route's vectors (ffh_blob_solve / apply_matrix), [3N]
+    double *stage_in = nullptr, *stage_out = nullptr;
+    int staging() {
+        if (stage_in) return FFH_OK;
+        const size_t bytes = 3 * (size_t)N * sizeof(double);
+        if (hipHostMalloc((void **)&stage_in, bytes, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&stage_out, bytes, hipHostMallocDefault) != hipSuccess)
+            return fail(FFH_E_NOMEM, "hipHostMalloc (solver staging)");
+        return FFH_OK;
+    }
     size_t pstride = 0;  // partials per reduction: max(slices, element blocks)
     int32_t *slice_blob = nullptr, *blob_slice_off = nullptr;
     int nblobs = 1;
@@ -240,7 +250,7 @@ struct ffh_blob {
     int cg_chunk = 16;
     // mixed path: the true residual f - K x is formed every rr_period CG iterations (and
     // whenever the recurrence residual passes the test); FFH_RR_PERIOD overrides the default
-    int rr_period = 8;
+    int rr_period = 16;
 
     ~ffh_blob() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -255,6 +265,8 @@ struct ffh_blob {
         for (auto ev : event_pool) (void)hipEventDestroy(ev);
         for (void *p : allocs) (void)hipFree(p);
         if (ctl_host) (void)hipHostFree(ctl_host);
+        if (stage_in) (void)hipHostFree(stage_in);
+        if (stage_out) (void)hipHostFree(stage_out);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1145,7 +1157,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
                              const int *pinned, double dt, double kT, double epsilon2,
                              int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
                              const unsigned long long *rng_state, int precision, int num_blobs,
-                             const int *blob_node_off) {
+                             const int *blob_node_off, bool material_meso = false) {
     if (!pos || !topo || !material_si || !stokes_radius || (num_pinned > 0 && !pinned) || !rng_state) {
         fail(FFH_E_ARG, "null argument");
         return nullptr;
@@ -1166,7 +1178,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     std::unique_ptr<ffh_blob> b(new ffh_blob());
     std::string msg = ffh::build_plan(b->plan, num_nodes, num_elements, pos, topo, material_si, stokes_radius,
                                       num_pinned, pinned, stokes_visc, calc_noise, calc_stokes, num_blobs,
-                                      blob_node_off);
+                                      blob_node_off, material_meso);
     if (!msg.empty()) {
         fail(FFH_E_ARG, msg);
         return nullptr;
@@ -1319,6 +1331,17 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
     return create_impl(num_nodes, num_elements, pos, topo, material_si, stokes_radius, num_pinned, pinned, dt,
                        kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, stokes_visc, rng_state, precision,
                        1, nullptr);
+}
+
+ffh_blob *ffh_blob_create_meso(int num_nodes, int num_elements, const double *pos, const int *topo,
+                               const double *material_meso, const double *stokes_radius, int num_pinned,
+                               const int *pinned, double dt, double kT, double epsilon2,
+                               int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                               const unsigned long long *rng_state, int precision) {
+    g_err.clear();
+    return create_impl(num_nodes, num_elements, pos, topo, material_meso, stokes_radius, num_pinned, pinned, dt,
+                       kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, stokes_visc, rng_state, precision,
+                       1, nullptr, true);
 }
 
 ffh_blob *ffh_batch_create(int num_blobs, const int *num_nodes, const int *num_elements, const double *pos,
@@ -1526,19 +1549,22 @@ int ffh_blob_solve(ffh_blob *b, double *xio) {
         if (xio[3 * m] != 0.0 || xio[3 * m + 1] != 0.0 || xio[3 * m + 2] != 0.0)
             return fail(FFH_E_ARG, "midpoint rows of the force vector must be zero");
     }
-    std::vector<double> fl(3 * (size_t)b->N, 0.0);
-    for (int i = 0; i < b->N; i++)
-        if (P.orig_of[i] >= 0)
-            for (int c = 0; c < 3; c++) fl[3 * i + c] = xio[3 * (size_t)P.orig_of[i] + c];
-    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (int rc0 = b->staging()) return rc0;
+    // linear rows compacted straight into pinned memory (padding rows zero), one DMA each way
+    double *fl = b->stage_in, *xl = b->stage_out;
+    for (int i = 0; i < b->N; i++) {
+        const int o = P.orig_of[i];
+        for (int c = 0; c < 3; c++) fl[3 * (size_t)i + c] = o >= 0 ? xio[3 * (size_t)o + c] : 0.0;
+    }
+    const size_t bytes = 3 * (size_t)b->N * sizeof(double);
+    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, fl, bytes, hipMemcpyHostToDevice, b->stream));
     if (int rc0 = b->run_rhs(b->tmp_in)) return rc0;
     int rc = b->run_solve();
-    std::vector<double> xl(fl.size());
-    HIP_OR_FAIL(hipMemcpyAsync(xl.data(), b->x, xl.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(xl, b->x, bytes, hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     for (int i = 0; i < b->N; i++)
         if (P.orig_of[i] >= 0)
-            for (int c = 0; c < 3; c++) xio[3 * (size_t)P.orig_of[i] + c] = xl[3 * i + c];
+            for (int c = 0; c < 3; c++) xio[3 * (size_t)P.orig_of[i] + c] = xl[3 * (size_t)i + c];
     return rc;
 }
 
@@ -1546,11 +1572,14 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     if (!b || !in || !result) return fail(FFH_E_ARG, "null argument");
     if (b->massful()) return fail(FFH_E_ARG, "apply_matrix: the viscosity matrix of the massless solver only");
     const ffh::Plan &P = b->plan;
-    std::vector<double> il(3 * (size_t)b->N, 0.0), ol(3 * (size_t)b->N);
-    for (int i = 0; i < b->N; i++)
-        if (P.orig_of[i] >= 0)
-            for (int c = 0; c < 3; c++) il[3 * i + c] = in[3 * (size_t)P.orig_of[i] + c];
-    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (int rc0 = b->staging()) return rc0;
+    double *il = b->stage_in, *ol = b->stage_out;
+    for (int i = 0; i < b->N; i++) {
+        const int o = P.orig_of[i];
+        for (int c = 0; c < 3; c++) il[3 * (size_t)i + c] = o >= 0 ? in[3 * (size_t)o + c] : 0.0;
+    }
+    const size_t bytes = 3 * (size_t)b->N * sizeof(double);
+    HIP_OR_FAIL(hipMemcpyAsync(b->tmp_in, il, bytes, hipMemcpyHostToDevice, b->stream));
     SpmvArgs m = b->spmv_args();
     double *ti = b->tmp_in, *to = b->tmp_out;
     int N = b->N;
@@ -1559,14 +1588,21 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     else
         hipLaunchKernelGGL((k_apply<float>), ffh_blob::grid(N), kBlock, 0, b->stream, m, N, (const double *)ti, to);
     HIP_OR_FAIL(hipGetLastError());
-    HIP_OR_FAIL(hipMemcpyAsync(ol.data(), b->tmp_out, ol.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(ol, b->tmp_out, bytes, hipMemcpyDeviceToHost, b->stream));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
-    // midpoint rows are identity (one + zero drag); linear rows from the device
-    for (int i = 0; i < P.num_nodes_total; i++)
-        for (int c = 0; c < 3; c++) result[3 * (size_t)i + c] = in[3 * (size_t)i + c];
+    // midpoint rows (and nodes of no element) are identity; linear rows from the device
+    std::memcpy(result, in, 3 * (size_t)P.num_nodes_total * sizeof(double));
     for (int i = 0; i < b->N; i++)
         if (P.orig_of[i] >= 0)
-            for (int c = 0; c < 3; c++) result[3 * (size_t)P.orig_of[i] + c] = ol[3 * i + c];
+            for (int c = 0; c < 3; c++) result[3 * (size_t)P.orig_of[i] + c] = ol[3 * (size_t)i + c];
+    return FFH_OK;
+}
+
+int ffh_blob_update_matrix(ffh_blob *b) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    if (b->massful()) return fail(FFH_E_ARG, "update_matrix: the viscosity matrix of the massless solver only");
+    if (int rc = b->run_assemble()) return rc;
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return FFH_OK;
 }
 

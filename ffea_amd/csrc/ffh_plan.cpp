@@ -148,7 +148,7 @@ std::string make_tiles(const std::vector<uint32_t> &off, const std::vector<uint3
 std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
                        const double *material_si, const double *stokes_radius, int num_pinned,
                        const int *pinned, double stokes_visc, int calc_noise, int calc_stokes,
-                       int num_blobs, const int *blob_node_off) {
+                       int num_blobs, const int *blob_node_off, bool material_meso) {
     if (num_nodes <= 0 || num_elements <= 0) return "mesh must have nodes and elements";
     if ((uint64_t)num_elements >= (1u << 27)) return "more than 2^27 elements per blob is not supported";
     const int NT = num_nodes, E = num_elements;
@@ -254,12 +254,21 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     for (int e = 0; e < E; e++) {
         int n[4];
         for (int a = 0; a < 4; a++) n[a] = P.enodes[4 * (size_t)e + a] = P.lin_of[topo[10 * e + a]];
-        const double *m = &material_si[6 * (size_t)e];
-        P.matA[e] = m[1] / (kMesoPressure * kMesoTime);
-        P.matB[e] = m[2] / (kMesoPressure * kMesoTime) - (2.0 / 3.0) * m[1];
-        P.matG[e] = m[3] / kMesoPressure;
-        P.matE[e] = m[4] / kMesoPressure;
-        P.matRho[e] = m[0] * kMesoVolume / kMesoMass;
+        if (material_meso) {  // rho, A, B, G, E as tetra_element_linear holds them
+            const double *m = &material_si[5 * (size_t)e];
+            P.matRho[e] = m[0];
+            P.matA[e] = m[1];
+            P.matB[e] = m[2];
+            P.matG[e] = m[3];
+            P.matE[e] = m[4];
+        } else {
+            const double *m = &material_si[6 * (size_t)e];
+            P.matA[e] = m[1] / (kMesoPressure * kMesoTime);
+            P.matB[e] = m[2] / (kMesoPressure * kMesoTime) - (2.0 / 3.0) * m[1];
+            P.matG[e] = m[3] / kMesoPressure;
+            P.matE[e] = m[4] / kMesoPressure;
+            P.matRho[e] = m[0] * kMesoVolume / kMesoMass;
+        }
         double J[3][3];
         jacobian(X, Y, Z, n, J);
         // mat3_invert (src/mat_vec_fns.cpp:148-177)

@@ -92,10 +92,12 @@ std::string make_tiles(const std::vector<uint32_t> &off, const std::vector<uint3
 // Builds the plan; returns "" on success or an error message.
 // blob_node_off: [num_blobs + 1] offsets of each blob's nodes in the concatenated node arrays
 // (nullptr: one blob); elements must only reference nodes of their own blob.
+// material_meso: material_si holds [5*E] rho, A, B, G, E already in meso units (as
+// tetra_element_linear holds them) instead of [6*E] .mat rows in SI.
 std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *pos, const int *topo,
                        const double *material_si, const double *stokes_radius, int num_pinned,
                        const int *pinned, double stokes_visc, int calc_noise, int calc_stokes,
-                       int num_blobs = 1, const int *blob_node_off = nullptr);
+                       int num_blobs = 1, const int *blob_node_off = nullptr, bool material_meso = false);
 
 // The mass-ful solvers' matrices on the linear rows; midpoint rows are identity rows with a zero
 // right-hand side in the reference (their forces are folded onto the vertices first), so they

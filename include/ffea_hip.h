@@ -65,6 +65,19 @@ ffh_blob *ffh_blob_create(int num_nodes, int num_elements, const double *pos, co
                           const unsigned long long *rng_state, int precision);
 
 /*
+ * ffh_blob_create with the materials as tetra_element_linear holds them after
+ * Blob::load_material_params (src/Blob.cpp:2861-2866): material_meso [5*E] = rho, A (shear
+ * viscosity), B (second viscosity coefficient), G (shear modulus), E (bulk modulus), meso units.
+ * For callers that hold converted elements, such as a Solver::init (include/Solver.h:45), so
+ * that no conversion back to SI and forth rounds them again.  Everything else as above.
+ */
+ffh_blob *ffh_blob_create_meso(int num_nodes, int num_elements, const double *pos, const int *topo,
+                               const double *material_meso, const double *stokes_radius, int num_pinned,
+                               const int *pinned, double dt, double kT, double epsilon2,
+                               int max_iterations_cg, int calc_noise, int calc_stokes, double stokes_visc,
+                               const unsigned long long *rng_state, int precision);
+
+/*
  * Create a batch: several blobs of one World on the current HIP device, stepped together (one
  * kernel launch per phase for all of them) and fed by one RngStream, in World::run's order
  * (src/World.cpp:2020-2041): the element phases of blobs 0..B-1 (7 draws per element), then
@@ -172,6 +185,16 @@ int ffh_blob_solve(ffh_blob *b, double *x);
  * Massless solver only (FFH_E_ARG otherwise).
  */
 int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result);
+
+/*
+ * The build step of NoMassCGSolver::solve (SparseMatrixFixedPattern::build,
+ * src/SparseMatrixFixedPattern.cpp:85-93, with calc_inverse_diagonal): the viscosity matrix and
+ * its inverse diagonal assembled at the handle's current positions, without the element
+ * loop's forces or draws.  ffh_blob_solve and ffh_blob_apply_matrix then use it.  Massless
+ * solver only.  ffh_blob_solve / ffh_blob_apply_matrix stage their vectors through pinned host
+ * buffers of 3 N doubles (N linear nodes), allocated on first use.
+ */
+int ffh_blob_update_matrix(ffh_blob *b);
 
 /*
  * Forces from outside the element loop, computed on the device every step from the current

@@ -14,6 +14,7 @@
  * reference's own code.  The C API mirrors oracle/ffea_oracle.c (ref_* for orc_*), so
  * the tests run one scenario through both and compare bit for bit.
  */
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -110,6 +111,13 @@ void ref_package_seed_from_rng_seed(int rng_seed, unsigned long sixseed[6]) {
 }
 
 /* ---- Blob --------------------------------------------------------------- */
+struct ref_blob;
+/* The blob's reference objects, for drivers that hand them to a Solver (mesh_node[N],
+ * tetra_element_linear[E], SimulationParams). */
+void *ref_blob_node_array(ref_blob *b);
+void *ref_blob_element_array(ref_blob *b);
+void *ref_blob_params_ptr(ref_blob *b);
+void ref_blob_set_max_iterations_cg(ref_blob *b, int n);
 ref_blob *ref_blob_create(int num_nodes, int num_elements, const double *pos, const int *topo,
                           const double *mat_si, const double *stokes_radius, int num_pinned,
                           const int *pinned, double dt, double kT, double epsilon2, int max_iterations_cg,
@@ -382,10 +390,37 @@ static Solver *active_solver(ref_blob *b) {
     return b->linear_solver == FFEA_NOMASS_CG_SOLVER ? static_cast<Solver *>(b->solver) : b->msolver;
 }
 
+/* Wall time per phase of the steps run so far (seconds): element loop, force gather + Stokes
+ * noise + pins, solve (NoMassCGSolver::solve: build, inverse diagonal, CG), integration +
+ * linearise_element -- the split of World::run's timers (src/World.cpp:1807-1809, 2089-2104). */
+static double g_phase[4] = {0, 0, 0, 0};
+static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+void ref_phase_times(double out[4], int reset) {
+    for (int k = 0; k < 4; k++) {
+        out[k] = g_phase[k];
+        if (reset) g_phase[k] = 0.0;
+    }
+}
+
+/* Seconds of one SparseMatrixFixedPattern::build of the blob's viscosity matrix (the assembly
+ * part of the solve phase). */
+double ref_blob_time_build(ref_blob *b) {
+    const double t0 = now_s();
+    b->solver->V->build();
+    return now_s() - t0;
+}
+
 /* Blob::update_positions (src/Blob.cpp:737-764) + euler_integrate (:3937-3967). */
 int ref_blob_update_positions(ref_blob *b, void *rngp) {
+    double t0 = now_s();
     aggregate(b, static_cast<RngStream *>(rngp));
-    if (active_solver(b)->solve(b->force) == FFEA_ERROR) return -1;
+    double t1 = now_s();
+    g_phase[1] += t1 - t0;
+    const int st = active_solver(b)->solve(b->force);
+    t0 = now_s();
+    g_phase[2] += t0 - t1;
+    if (st == FFEA_ERROR) return -1;
     if (b->linear_solver == FFEA_NOMASS_CG_SOLVER) {
 #ifdef FFEA_PARALLEL_WITHIN_BLOB
 #pragma omp parallel for schedule(static)
@@ -412,14 +447,22 @@ int ref_blob_update_positions(ref_blob *b, void *rngp) {
         }
     }
     for (int n = 0; n < b->E; n++) b->elem[n].linearise_element();
+    g_phase[3] += now_s() - t0;
     return 0;
 }
+
+void *ref_blob_node_array(ref_blob *b) { return b->node; }
+void *ref_blob_element_array(ref_blob *b) { return b->elem; }
+void *ref_blob_params_ptr(ref_blob *b) { return b->params; }
+void ref_blob_set_max_iterations_cg(ref_blob *b, int n) { b->params->max_iterations_cg = n; }
 
 /* One World::run iteration (src/World.cpp:1844-2043). */
 int ref_world_step(ref_blob **blobs, int nblobs, void *rng) {
     int inv = 0, err = 0;
     for (int i = 0; i < nblobs; i++) zero_forces(blobs[i]);
+    const double t0 = now_s();
     for (int i = 0; i < nblobs; i++) inv += ref_blob_update_internal_forces(blobs[i], rng);
+    g_phase[0] += now_s() - t0;
     if (inv > 0) return inv;
     for (int i = 0; i < nblobs; i++)
         if (ref_blob_update_positions(blobs[i], rng) != 0) err = 1;
