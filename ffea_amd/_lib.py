@@ -44,6 +44,7 @@ EXPORTS = (
     "ffh_blob_spring_energy", "ffh_blob_set_constant_forces", "ffh_blob_set_surface_forces",
     "ffh_blob_set_rotational_forces", "ffh_blob_set_box", "ffh_blob_set_ssint", "ffh_blob_ssint_energy",
     "ffh_blob_ssint_energy_noalloc", "ffh_blob_create_meso", "ffh_blob_update_matrix",
+    "ffh_blob_set_stress_output", "ffh_blob_get_stress_magnitudes",
 )
 FFH_PRECOND_JACOBI = 0
 FFH_PRECOND_LEARNED = 1
@@ -147,6 +148,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_ssint_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_ssint_energy_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_update_matrix.argtypes = [C.c_void_p]
+    L.ffh_blob_set_stress_output.argtypes = [C.c_void_p, C.c_int]
+    L.ffh_blob_get_stress_magnitudes.argtypes = [C.c_void_p, _dp]
     L.ffh_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ip, _ip, _dp]
     for n in EXPORTS:
         if n not in ("ffh_last_error", "ffh_version", "ffh_blob_create", "ffh_blob_destroy",

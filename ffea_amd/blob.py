@@ -193,6 +193,16 @@ class Blob:
                                               r.ravel(), k.ravel(), np.ascontiguousarray(grid, np.int32),
                                               float(cell_size), int(inc_self), int(es_update)))
 
+    def set_stress_output(self, enable: bool = True) -> None:
+        """Store each element's internal_stress_mag in every element phase (Blob's stress file)."""
+        _lib.check(self._L.ffh_blob_set_stress_output(self._h, int(bool(enable))))
+
+    def stress_magnitudes(self) -> np.ndarray:
+        """internal_stress_mag of the last element phase, one per element (meso pressure)."""
+        out = np.zeros(self.num_elements, np.float64)
+        _lib.check(self._L.ffh_blob_get_stress_magnitudes(self._h, out))
+        return out
+
     def ssint_energy(self) -> float:
         """The surface-surface interaction energy of the last step's solve (meso units)."""
         v = C.c_double()

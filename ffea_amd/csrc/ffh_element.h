@@ -21,6 +21,7 @@ struct ElemArgs {
     uint32_t elem_base;
     CgCtl *ctl;
     const double *vel;  // [3N] node velocities (mass-ful solvers: the viscous force K_e u_e)
+    double *stress_mag;  // [E] internal_stress_mag per element, or nullptr (not computed)
 };
 
 // Entry (R, C) of the element viscosity matrix (create_viscosity_matrix,
@@ -156,6 +157,11 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
         S[2][0] = S[0][2];
         S[2][1] = S[1][2];
     }
+    // internal_stress_mag (src/Blob.cpp:700): sqrt of mat3_double_contraction_symmetric
+    // (src/mat_vec_fns.cpp:131-134) of the stress with its noise
+    if (g.stress_mag)
+        g.stress_mag[e] = (double)sqrt(S[0][0] * S[0][0] + S[1][1] * S[1][1] + S[2][2] * S[2][2] +
+                                       (R)2 * (S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2]));
     // The mass-ful solvers start du from the viscous force K_e u_e (src/Blob.cpp:703-706):
     // get_element_velocity_vector (:454-469, x-major over the vertices) and mat12_apply
     // (src/mat_vec_fns.cpp:26-34, each row summed in column order from zero); the massless

@@ -132,6 +132,8 @@ struct ffh_blob {
     int nblobs = 1;
     bool use_tot = false;  // some blob has more than kTotSlicesMin slices
     double *ext = nullptr;  // external forces (ffh_blob_set_external_forces)
+    double *stress_mag = nullptr;  // [E] internal_stress_mag of the last element phase, when enabled
+    double *stress_buf = nullptr;
     bool ext_on = false;
     // solver (ffh_blob_set_solver): mass matrix in the SELL slots + its inverse diagonal, or the
     // lumped inverse masses; densities for the kinetic energy; midpoint velocities as given
@@ -404,6 +406,7 @@ struct ffh_blob {
         g.elem_base = 0;
         g.ctl = ctl;
         g.vel = vel;
+        g.stress_mag = stress_mag;
         return g;
     }
     static int grid(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
@@ -1595,6 +1598,31 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result) {
     for (int i = 0; i < b->N; i++)
         if (P.orig_of[i] >= 0)
             for (int c = 0; c < 3; c++) result[3 * (size_t)P.orig_of[i] + c] = ol[3 * (size_t)i + c];
+    return FFH_OK;
+}
+
+int ffh_blob_set_stress_output(ffh_blob *b, int enable) {
+    if (!b) return fail(FFH_E_ARG, "null blob");
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the element kernel's arguments change
+    b->graphs.clear();
+    if (!enable) {
+        b->stress_mag = nullptr;  // the buffer stays with the handle's allocations
+        return FFH_OK;
+    }
+    if (!b->stress_buf) {
+        int rc = b->alloc(b->stress_buf, (size_t)b->E);
+        if (rc) return rc;
+        HIP_OR_FAIL(hipMemset(b->stress_buf, 0, (size_t)b->E * sizeof(double)));
+    }
+    b->stress_mag = b->stress_buf;
+    return FFH_OK;
+}
+
+int ffh_blob_get_stress_magnitudes(ffh_blob *b, double *out) {
+    if (!b || !out) return fail(FFH_E_ARG, "null argument");
+    if (!b->stress_mag) return fail(FFH_E_ARG, "stress output is not enabled (ffh_blob_set_stress_output)");
+    HIP_OR_FAIL(hipMemcpyAsync(out, b->stress_mag, (size_t)b->E * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return FFH_OK;
 }
 

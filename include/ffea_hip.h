@@ -197,6 +197,16 @@ int ffh_blob_apply_matrix(ffh_blob *b, const double *in, double *result);
 int ffh_blob_update_matrix(ffh_blob *b);
 
 /*
+ * Element stress magnitudes for Blob::make_stress_measurements (src/Blob.cpp:1540-1550): with
+ * the output enabled, every element phase also stores internal_stress_mag per element
+ * (src/Blob.cpp:700: sqrt(mat3_double_contraction_symmetric(stress)) of the elastic + bulk +
+ * fluctuating stress, src/mat_vec_fns.cpp:131-134), E values in the handle's element order;
+ * ffh_blob_get_stress_magnitudes copies the last ones out (meso pressure units).
+ */
+int ffh_blob_set_stress_output(ffh_blob *b, int enable);
+int ffh_blob_get_stress_magnitudes(ffh_blob *b, double *out);
+
+/*
  * Forces from outside the element loop, computed on the device every step from the current
  * positions and added where the reference adds them.  Node ids are in the handle's N_t
  * numbering (a batch: its blobs' nodes concatenated), forces in meso units.  Each call

@@ -342,3 +342,29 @@ def test_mixed_cg_trajectory_against_fp64():
         pa = a.positions().ravel()
         assert np.linalg.norm(b.positions().ravel() - pa) <= 1e-4 * np.linalg.norm(pa - xs), k
     assert np.array_equal(a.rng_state(), b.rng_state())
+
+
+@pytest.mark.parametrize("noise", [0, 1])
+def test_stress_magnitudes_against_oracle(oracle, noise):
+    """internal_stress_mag per element (src/Blob.cpp:700) from the element kernel: FP64 bit-exact
+    against the oracle (itself bit-exact against the reference's objects), with and without the
+    fluctuating stress; the mixed path within FP32 rounding."""
+    m, prm, sc, x0 = box_case((4, 3, 3), noise=noise, sigma=0.05, seed=8, jitter=0.1)
+    seed = oracle.package_seed(44)
+    oracle.set_package_seed(seed)
+    s = oracle.stream()
+    bo = oracle.blob(m, prm, sc, pos=x0)
+    assert bo.forces_only(s)[0] == 0
+    want = bo.get("stress_mag")
+    assert want.min() > 0
+    for prec in ("fp64", "mixed"):
+        b = gpu_blob(m, prm, sc, seed, x0, prec)
+        b.set_stress_output(True)
+        assert b.update_internal_forces() == 0
+        got = b.stress_magnitudes()
+        if prec == "fp64":
+            assert np.array_equal(got, want)
+        else:
+            assert rel(got, want) < 1e-5
+        b.set_stress_output(False)
+        b.step(1)  # steps without the output keep working (graphs rebuilt without it)
