@@ -1808,6 +1808,8 @@ void orc_blob_get(const orc_blob *b, const char *what, double *out) {
     else if (!strcmp(what, "stress_mag")) memcpy(out, b->stress_mag, sizeof(double) * E);
     else if (!strcmp(what, "stokes_drag")) memcpy(out, b->stokes_drag, sizeof(double) * N);
     else if (!strcmp(what, "last_det")) memcpy(out, b->last_det, sizeof(double) * E);
+    else if (!strcmp(what, "face_force") && b->face_force)
+        memcpy(out, b->face_force, sizeof(double) * 9 * (size_t)b->n_faces);
     else if (!strcmp(what, "material")) {
         for (size_t e = 0; e < E; e++) {
             out[5 * e] = b->rho[e];

@@ -171,6 +171,7 @@ class Blob:
         """Linear surface forces (oracle only): faces [F, 3] node ids; sets: list of face-index
         lists; set_forces [S, 3] meso units."""
         faces = np.ascontiguousarray(faces, np.int32).ravel()
+        self.n_faces = faces.size // 3
         off = np.concatenate([[0], np.cumsum([len(x) for x in sets])]).astype(np.int32)
         flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in sets]) if sets else np.zeros(1), np.int32)
         sf = np.ascontiguousarray(set_forces, np.float64).ravel()
@@ -182,7 +183,7 @@ class Blob:
                  "dpsi": 12 * self.E, "vol": self.E, "vol_0": self.E, "J_inv_0": 9 * self.E,
                  "F_ij": 9 * self.E, "visc": 144 * self.E, "node_force": 30 * self.E,
                  "stress_mag": self.E, "stokes_drag": self.N, "material": 5 * self.E,
-                 "last_det": self.E}
+                 "last_det": self.E, "face_force": 9 * getattr(self, "n_faces", 0)}
         out = np.zeros(sizes[what], np.float64)
         self.lib.fn("blob_get")(self.h, what.encode(), out)
         return out

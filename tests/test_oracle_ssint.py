@@ -198,3 +198,61 @@ def test_steric_checks_exclude_neighbours(oracle, ssint_off):
         res[kind] = oracle.ssint_energy()
         one[0].set_positions(pos)
     assert np.isfinite(res["ljsteric"]) and abs(res["ljsteric"]) < abs(res["lennard-jones"])
+
+
+@pytest.mark.parametrize("kind", ["lennard-jones", "ljsteric", "gensoft"])
+def test_face_pair_force_spread_moments(oracle, ssint_off, kind):
+    """One facing pair of faces, one active face per cube: the forces the pair leaves on each
+    face's three corners (src/VdW_solver.cpp:288-320) against the Gauss-point forces
+    F_kl = f(|p_k - q_l|) (p_k - q_l) / |p_k - q_l| (k <= l, F_lk = F_kl: :663-697), with
+    p, q the 3-point rule's points on the two faces (eta = 2/3, 1/6, 1/6) and W = 1/3:
+      sum_j f1_j = +Ap Aq W^2 sum_kl F_kl,   sum_j x1_j (x) f1_j = +Ap Aq W^2 sum_kl p_l (x) F_kl,
+      sum_j f2_j = -Ap Aq W^2 sum_kl F_kl,   sum_j x2_j (x) f2_j = -Ap Aq W^2 sum_kl q_k (x) F_kl.
+    Linear interpolation carries the Gauss points' moments to the corners, so a corner or
+    Gauss index out of place, the faces' roles swapped or a wrong area moves these moments."""
+    from ffea_amd import formats as F
+    blobs, E, R, h, s = cubes_world(oracle)
+    tops = [F.read_ssint_types(b["vdw"], F.read_surface(b["surface"]).shape[0]) for b in s["blobs"]]
+    faces = [F.read_surface(b["surface"])[:, 1:4] for b in s["blobs"]]
+    pos = [b.get("pos").reshape(-1, 3).copy() for b in blobs]
+    cen = [p[fc].mean(axis=1) for p, fc in zip(pos, faces)]
+    a = np.flatnonzero(tops[0] >= 0)[17]
+    cand = np.flatnonzero(tops[1] >= 0)
+    b = cand[np.argmin(np.linalg.norm(cen[1][cand] - cen[0][a], axis=1))]
+    for o, t, keep in ((blobs[0], tops[0], a), (blobs[1], tops[1], b)):
+        one = np.full_like(t, -1)
+        one[keep] = t[keep]
+        o.set_ssint_types(one)
+    k0 = None
+    if kind == "gensoft":  # Rmin is the cutoff
+        R, k0 = np.full_like(R, h), np.full_like(E, 0.02)
+    oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=0, es_update=1, kind=kind, k0=k0)
+    oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
+    assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
+    f1 = blobs[0].get("face_force").reshape(-1, 3, 3)[a]
+    f2 = blobs[1].get("face_force").reshape(-1, 3, 3)[b]
+    x1, x2 = pos[0][faces[0][a]], pos[1][faces[1][b]]
+    eta = np.full((3, 3), 1 / 6) + np.eye(3) / 2
+    p, q = eta @ x1, eta @ x2
+    t0, t1 = int(tops[0][a]), int(tops[1][b])
+    Fkl = np.zeros((3, 3, 3))
+    for k in range(3):
+        for l in range(k, 3):
+            d = p[k] - q[l]
+            r = np.linalg.norm(d)
+            fm = oracle.ssint_factors(kind, r, E[t0, t1], R[t0, t1], 0.0 if k0 is None else k0[t0, t1])[0]
+            Fkl[k, l] = Fkl[l, k] = fm * d / r
+    area = [0.5 * np.linalg.norm(np.cross(x[1] - x[0], x[2] - x[0])) for x in (x1, x2)]
+    c = area[0] * area[1] / 9
+    tot = c * Fkl.sum(axis=(0, 1))
+    m1 = c * np.einsum("li,klj->ij", p, Fkl)
+    m2 = -c * np.einsum("ki,klj->ij", q, Fkl)
+    scale = np.abs(tot).max()
+    assert scale > 0
+    assert np.abs(f1.sum(axis=0) - tot).max() <= 1e-12 * scale
+    assert np.abs(f2.sum(axis=0) + tot).max() <= 1e-12 * scale
+    mscale = scale * np.abs(x1).max()
+    assert np.abs(np.einsum("ji,jk->ik", x1, f1) - m1).max() <= 1e-11 * mscale
+    assert np.abs(np.einsum("ji,jk->ik", x2, f2) - m2).max() <= 1e-11 * mscale
+    # the moments do not follow from the totals: the corners differ from the centroid's share
+    assert np.abs(f1 - f1.mean(axis=0)).max() > 1e-6 * scale
