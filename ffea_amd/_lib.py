@@ -42,7 +42,7 @@ EXPORTS = (
     "ffh_blob_set_tuning", "ffh_blob_set_external_forces", "ffh_blob_set_solver",
     "ffh_blob_set_velocities", "ffh_blob_kinetic_energy", "ffh_blob_set_springs",
     "ffh_blob_spring_energy", "ffh_blob_set_constant_forces", "ffh_blob_set_surface_forces",
-    "ffh_blob_set_rotational_forces", "ffh_blob_set_box", "ffh_blob_set_ssint", "ffh_blob_ssint_energy",
+    "ffh_blob_set_rotational_forces", "ffh_blob_set_box", "ffh_blob_set_ssint", "ffh_blob_set_steric", "ffh_blob_ssint_energy",
     "ffh_blob_ssint_energy_noalloc", "ffh_blob_create_meso", "ffh_blob_update_matrix",
     "ffh_blob_set_stress_output", "ffh_blob_get_stress_magnitudes",
 )
@@ -51,7 +51,7 @@ FFH_PRECOND_LEARNED = 1
 # solver codes of include/FFEA_return_codes.h:30-33, by .ffea "solver" value
 FFH_SOLVERS = {"CG": 1, "masslumped": 2, "CG_nomass": 3}
 # surface-surface interaction types on the device, by .ffea ssint_type value
-FFH_SSINT_KINDS = {"lennard-jones": 1, "ljsteric": 2, "gensoft": 3}
+FFH_SSINT_KINDS = {"lennard-jones": 1, "ljsteric": 2, "gensoft": 3, "steric": 4}
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -145,6 +145,7 @@ def load(path: str | None = None) -> C.CDLL:
     L.ffh_blob_set_box.argtypes = [C.c_void_p, _dp, C.c_int]
     L.ffh_blob_set_ssint.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int, _dp, _dp, _dp, _ip,
                                      C.c_double, C.c_int, C.c_int]
+    L.ffh_blob_set_steric.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.ffh_blob_ssint_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_ssint_energy_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ffh_blob_update_matrix.argtypes = [C.c_void_p]

@@ -169,8 +169,8 @@ class Blob:
         """Surface-surface interactions: faces [F, 3] (N_t ids), types [F] (-1 inactive),
         emin / rmin (/ k0) [ntypes, ntypes] meso, the lookup grid (es_N per axis, cell size =
         ssint_cutoff in meso), inc_self_ssint, es_update, the type ("lennard-jones", "ljsteric",
-        "gensoft") and each face's opposite element vertex (the latter two types).  faces None
-        removes them."""
+        "gensoft", "steric") and each face's opposite element vertex (all types but
+        lennard-jones; steric needs no emin / rmin).  faces None removes them."""
         if faces is None:
             z = np.zeros(1)
             _lib.check(self._L.ffh_blob_set_ssint(self._h, 1, 0, np.zeros(3, np.int32), np.zeros(1, np.int32),
@@ -181,6 +181,8 @@ class Blob:
         if fc.size != 3 * ty.size:
             raise ValueError("one type per face")
         op = np.full(ty.size, -1, np.int32) if opposite is None else np.ascontiguousarray(opposite, np.int32).ravel()
+        if kind == "steric" and emin is None:
+            emin = rmin = np.zeros((1, 1))
         e = np.ascontiguousarray(emin, np.float64)
         r = np.ascontiguousarray(rmin, np.float64)
         k = np.zeros_like(e) if k0 is None else np.ascontiguousarray(k0, np.float64)
@@ -192,6 +194,11 @@ class Blob:
                                               ty if ty.size else np.zeros(1, np.int32), e.shape[0], e.ravel(),
                                               r.ravel(), k.ravel(), np.ascontiguousarray(grid, np.int32),
                                               float(cell_size), int(inc_self), int(es_update)))
+
+    def set_steric(self, steric_factor: float = 1.0, steric_dr: float = 5e-3) -> None:
+        """steric_factor and steric_dr (meso, as SimulationParams holds them) of the steric
+        forces of the ljsteric, gensoft and steric interactions."""
+        _lib.check(self._L.ffh_blob_set_steric(self._h, float(steric_factor), float(steric_dr)))
 
     def set_stress_output(self, enable: bool = True) -> None:
         """Store each element's internal_stress_mag in every element phase (Blob's stress file)."""

@@ -14,8 +14,11 @@
 //                  src/ConjugateGradientSolver.cpp:205-349), k_integrate (src/Blob.cpp:3937-3967),
 //                  energies
 //   ffh_box.h      World::run's periodic box (a blob whose centroid leaves it moves back in)
-//   ffh_ssint.h    Lennard-Jones surface-surface interactions (VdW_solver): face geometry and
-//                  lookup grid, and each face's force in the reference's summation order
+//   ffh_steric.h   the steric volume exclusion between two face elements (overlap test,
+//                  intersection volume, its gradient; src/VolumeIntersection.cpp,
+//                  src/CheckTetrahedraOverlap.cpp)
+//   ffh_ssint.h    surface-surface interactions (VdW_solver): face geometry and lookup grid,
+//                  and each face's force in the reference's summation order
 //   ffh_mpnn.h     the learned preconditioner
 // This file holds the per-handle state (ffh_blob: buffers, launches, step graphs, tuning) and
 // the C-ABI.
@@ -70,6 +73,7 @@ constexpr int kLaneTable = 4 * 256 * 2;  // jump[kLaneTable + 2 l + comp] = A_co
 #include "ffh_forces.h"
 #include "ffh_solve.h"
 #include "ffh_box.h"
+#include "ffh_steric.h"
 #include "ffh_ssint.h"
 
 #include "ffh_mpnn.h"
@@ -154,6 +158,7 @@ struct ffh_blob {
     // sort's scratch, a host-visible error flag and the energy's per-blob-pair sums
     bool ss_on = false;
     SsArgs ss = {};
+    double steric_factor = 1.0, steric_dr = 5e-3;  // SimulationParams defaults (ffh_blob_set_steric)
     std::vector<void *> ss_allocs;
     void *ss_temp = nullptr;
     size_t ss_temp_bytes = 0;
@@ -196,10 +201,17 @@ struct ffh_blob {
         HIP_OR_FAIL(hipcub::DeviceRadixSort::SortPairs(ss_temp, tb, a.cell, a.sorted_cell, a.order, a.sorted, P, 0, ss_bits, stream));
         rc = launch(13, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_clear, grid(nc, 256), 256, 0, stream, e0, e1, 0, a); });
         if (!rc) rc = launch(13, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_bounds, grid(P, 256), 256, 0, stream, e0, e1, 0, a); });
-        if (!rc) rc = launch(12, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces, P, 64, 0, stream, e0, e1, 0, a); });
+        if (a.kind == FFH_SSINT_LENNARD_JONES) {
+            if (!rc) rc = launch(12, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces<kSsLJ>, P, 64, 0, stream, e0, e1, 0, a); });
+        } else {  // the pair potentials with the overlap screen, then the faces it deferred
+            if (!rc) rc = launch(12, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces<kSsScreen>, P, 64, 0, stream, e0, e1, 0, a); });
+            if (!rc) rc = launch(12, [&](hipEvent_t e0, hipEvent_t e1) { hipExtLaunchKernelGGL(k_ss_forces<kSsFull>, P, 64, 0, stream, e0, e1, 0, a); });
+        }
         return rc;
     }
     int check_ssint() {
+        if (ss_on && *ss_err_host == 2)
+            return fail(FFH_E_ARG, "a face element too degenerate for the steric volume (a zero normal; the reference throws)");
         if (ss_on && *ss_err_host) return fail(FFH_E_ARG, "a face centroid lies outside the lookup grid (LinkedListCube)");
         return FFH_OK;
     }
@@ -1788,7 +1800,7 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
                        const int *face_type, int num_types, const double *emin, const double *rmin, const double *k0,
                        const int *grid, double cell_size, int inc_self, int es_update) {
     if (!b || num_faces < 0 || (num_faces > 0 && (!face_nodes || !face_type))) return fail(FFH_E_ARG, "null argument");
-    if (kind < FFH_SSINT_LENNARD_JONES || kind > FFH_SSINT_GENSOFT) return fail(FFH_E_ARG, "unknown interaction type");
+    if (kind < FFH_SSINT_LENNARD_JONES || kind > FFH_SSINT_STERIC) return fail(FFH_E_ARG, "unknown interaction type");
     if (kind != FFH_SSINT_LENNARD_JONES && num_faces > 0 && !face_opposite) return fail(FFH_E_ARG, "null argument");
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the step gains or loses the interaction
@@ -1800,7 +1812,9 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
         if (face_type[f] == -1) continue;
         if (face_type[f] < 0 || face_type[f] >= num_types)
             return fail(FFH_E_ARG, "face " + std::to_string(f) + ": interaction type out of range");
+        // Face::n[0..3]: the opposite vertex takes steric forces (kinds 2-4) and joins the gather
         for (int k = 0; k < 3; k++) pool_nodes.push_back(face_nodes[3 * f + k]);
+        pool_nodes.push_back(kind != FFH_SSINT_LENNARD_JONES ? face_opposite[f] : -1);
         for (int k = 0; k < 3; k++) pool_nodes4.push_back(face_nodes[3 * f + k]);
         pool_nodes4.push_back(face_opposite ? face_opposite[f] : -1);
         pool_type.push_back(face_type[f]);
@@ -1815,11 +1829,15 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
         }
         return rc;
     }
-    if (!emin || !rmin || !grid || !k0) return fail(FFH_E_ARG, "null argument");
+    if (!grid || (kind != FFH_SSINT_STERIC && (!emin || !rmin || !k0))) return fail(FFH_E_ARG, "null argument");
     if (grid[0] < 1 || grid[1] < 1 || grid[2] < 1 || !(cell_size > 0) || es_update < 1 ||
         (long long)grid[0] * grid[1] * grid[2] > (1 << 28))
         return fail(FFH_E_ARG, "invalid lookup grid or es_update");
-    for (int t = 0; t < num_types * num_types; t++)
+    if (num_types < 1) return fail(FFH_E_ARG, "at least one interaction type");
+    // steric alone reads no pair parameters (the tables may be null)
+    const std::vector<double> none((size_t)num_types * num_types, 0.0);
+    if (kind == FFH_SSINT_STERIC) emin = rmin = k0 = none.data();
+    for (int t = 0; t < num_types * num_types && kind != FFH_SSINT_STERIC; t++)
         if (!(emin[t] > 0) || !(rmin[t] > 0)) return fail(FFH_E_ARG, "Emin and Rmin must be greater than 0");
     b->fin.ssint_faces = pool_nodes;
     int rc = b->upload_forces();
@@ -1831,10 +1849,20 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
     const ffh::Plan &Pl = b->plan;
     std::vector<int32_t> mid_of(Pl.num_nodes_total, -1);  // N_t id -> midpoint record
     for (size_t m = 0; m < Pl.mid_index.size(); m++) mid_of[Pl.mid_index[m]] = (int32_t)m;
-    std::vector<int32_t> ends(6 * (size_t)P), blob(P);
+    std::vector<int32_t> ends(6 * (size_t)P), blob(P), opp(P, 0);
     for (int p = 0; p < P; p++) {
+        if (kind != FFH_SSINT_LENNARD_JONES) {
+            const int v = pool_nodes[4 * p + 3];
+            if (v < 0 || v >= Pl.num_nodes_total || Pl.lin_of[v] < 0) {
+                b->fin = keep;
+                (void)b->upload_forces();
+                return fail(FFH_E_ARG, "interaction face " + std::to_string(p) + ": the opposite node " +
+                                           std::to_string(v) + " is out of range or an edge midpoint");
+            }
+            opp[p] = Pl.lin_of[v];
+        }
         for (int k = 0; k < 3; k++) {  // build_forces has checked every corner is in an element
-            const int v = pool_nodes[3 * p + k];
+            const int v = pool_nodes[4 * p + k];
             int a = Pl.lin_of[v], c = a;
             if (a < 0) {
                 const int m = mid_of[v];
@@ -1868,6 +1896,8 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
     if (!rc) rc = put(rmin, sizeof(double) * num_types * num_types, (void **)&a.rmin);
     if (!rc) rc = put(k0, sizeof(double) * num_types * num_types, (void **)&a.k0);
     if (!rc) rc = put(pool_nodes4.data(), pool_nodes4.size() * 4, (void **)&a.nodes4);
+    if (!rc) rc = put(opp.data(), opp.size() * 4, (void **)&a.opp);
+    if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&a.deferred);
     if (!rc) rc = put(nullptr, 8 * sizeof(double) * (size_t)P, (void **)&a.geom);
     if (!rc) rc = put(nullptr, 4 * (size_t)P, (void **)&a.cell);
     if (!rc) rc = put(order.data(), 4 * (size_t)P, (void **)&a.order);
@@ -1905,6 +1935,8 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
     a.err = err_dev;
     a.ntypes = num_types;
     a.kind = kind;
+    a.st_factor = b->steric_factor;
+    a.st_dr = b->steric_dr;
     a.nx = grid[0];
     a.ny = grid[1];
     a.nz = grid[2];
@@ -1916,6 +1948,20 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
     b->ss_temp_bytes = tb;
     b->ss_bits = bits;
     b->ss_on = true;
+    return FFH_OK;
+}
+
+int ffh_blob_set_steric(ffh_blob *b, double steric_factor, double steric_dr) {
+    if (!b) return fail(FFH_E_ARG, "null argument");
+    if (!(steric_dr > 0)) return fail(FFH_E_ARG, "steric_dr can only be > 0");
+    if (!std::isfinite(steric_factor)) return fail(FFH_E_ARG, "steric_factor must be finite");
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);  // the kernels' arguments change
+    b->graphs.clear();
+    b->steric_factor = steric_factor;
+    b->steric_dr = steric_dr;
+    b->ss.st_factor = steric_factor;
+    b->ss.st_dr = steric_dr;
     return FFH_OK;
 }
 
@@ -1955,7 +2001,10 @@ static int ssint_energy_impl(ffh_blob *b, double *energy, double *pe, int *pb, s
         pe = b->ss_pe;
         pb = b->ss_pb;
     }
-    hipLaunchKernelGGL(k_ss_pair_energy, P, 64, 0, b->stream, a, b->nblobs, (const int *)b->ss_off, pe, pb);
+    if (a.kind == FFH_SSINT_LENNARD_JONES)
+        hipLaunchKernelGGL(k_ss_pair_energy<false>, P, 64, 0, b->stream, a, b->nblobs, (const int *)b->ss_off, pe, pb);
+    else
+        hipLaunchKernelGGL(k_ss_pair_energy<true>, P, 64, 0, b->stream, a, b->nblobs, (const int *)b->ss_off, pe, pb);
     HIP_OR_FAIL(hipGetLastError());
     hipLaunchKernelGGL(k_ss_energy_sum, 1, 64, 0, b->stream, (int)npairs, (const double *)pe, (const int *)pb,
                        b->nblobs, b->ss_fe, b->ss_total);

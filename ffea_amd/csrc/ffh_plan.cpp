@@ -571,7 +571,7 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
     const size_t ns = in.spring_n0.size(), nc = in.const_node.size(), nf = in.faces.size() / 3;
     const size_t nr = in.rot_node.size();
     const size_t nset = in.set_off.empty() ? 0 : in.set_off.size() - 1;
-    const size_t nss = in.ssint_faces.size() / 3;
+    const size_t nss = in.ssint_faces.size() / 4;
     if (nss && nset)
         return "surface sets (CTF 3) together with surface-surface interactions are not on this path";
     if (in.spring_n1.size() != ns || in.spring_k.size() != ns || in.spring_l.size() != ns ||
@@ -664,7 +664,7 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
         }
     }
     // face groups: vertices first, then carrying midpoints; a group key is a set face, or
-    // 3 p + corner for interaction face p
+    // 4 p + k for node k of interaction face p (corners 0-2, opposite vertex 3)
     std::vector<std::vector<int32_t>> grp(N), mgrp;
     auto add_key = [&](int32_t v, int32_t key) {
         if (P.lin_of[v] >= 0) {
@@ -681,12 +681,13 @@ std::string build_forces(const Plan &P, const ForceInput &in, ForcePlan &F) {
     }
     F.ssint_row0 = (int32_t)in.set_faces.size();
     for (size_t p = 0; p < nss; p++)
-        for (int k = 0; k < 3; k++) {
-            const int32_t v = in.ssint_faces[3 * p + k];
+        for (int k = 0; k < 4; k++) {
+            const int32_t v = in.ssint_faces[4 * p + k];
+            if (k == 3 && v == -1) continue;  // no opposite vertex: no steric forces (lennard-jones)
             if (v < 0 || v >= NT) return "interaction face " + std::to_string(p) + ": node index out of range";
             if (P.lin_of[v] < 0 && end_a[v] < 0)
                 return "interaction face " + std::to_string(p) + ": node " + std::to_string(v) + " is in no element";
-            add_key(v, (int32_t)(3 * p + k));
+            add_key(v, (int32_t)(4 * p + k));
         }
     const int M = (int)(F.mid_ends.size() / 2);
     mpre.resize(M);

@@ -133,9 +133,10 @@ struct ForceInput {
     std::vector<int32_t> set_off;     // [S+1]
     std::vector<int32_t> set_faces;
     std::vector<double> set_force;    // [3S]
-    // surface-surface interactions (VdW_solver): the active faces' corners in pool order; each
-    // corner's Face::force is one value row the interaction kernel writes every step
-    std::vector<int32_t> ssint_faces; // [3P]
+    // surface-surface interactions (VdW_solver): the active faces' nodes Face::n[0..3] in pool
+    // order (the opposite vertex -1 where no steric forces reach it); each node's Face::force
+    // is one value row the interaction kernel writes every step
+    std::vector<int32_t> ssint_faces; // [4P]
 };
 
 // The same forces as per-node lists in the order the reference adds them to a node's force:
@@ -144,9 +145,9 @@ struct ForceInput {
 //     applies the springs, then update_internal_forces the constant forces, onto the zeroed
 //     force vector;
 //  2. after the element forces: one group per surface face holding the node (ascending face
-//     index), each the sum of the face's set tractions in set order, or the face corner's
-//     surface-surface interaction force (Face::force, gathered in aggregate_forces_and_solve,
-//     src/Blob.cpp:3842-3851);
+//     index), each the sum of the face's set tractions in set order, or the face node's
+//     surface-surface interaction force (Face::force[0..3], the opposite vertex included,
+//     gathered in aggregate_forces_and_solve, src/Blob.cpp:3842-3851);
 //  3. after the noise: the forces on edge midpoints (constant forces, then face groups, as in
 //     1 and 2), passed on to the edge ends element by element as linearise_force does
 //     (src/Blob.cpp:1066-1085): each (vertex, element) term is half the sum of the element's
@@ -165,7 +166,7 @@ struct ForcePlan {
     std::vector<double> rot;            // [8 * rotational]: axis point, direction, F, torque
     std::vector<int32_t> grp_off;       // [N+1] face groups of the vertices
     std::vector<int32_t> grp_ent_off;   // [groups + 1]
-    std::vector<int32_t> grp_ent;       // value rows: set entries, then 3 p + corner of interaction face p
+    std::vector<int32_t> grp_ent;       // value rows: set entries, then 4 p + k for node k of interaction face p
     int32_t ssint_row0 = 0;             // first interaction row (after the set entries)
     std::vector<int32_t> set_off;       // [S+1] entries of each set
     std::vector<int32_t> ent_ends;      // [6 * entries] per face corner its two edge ends (a vertex twice)

@@ -1,8 +1,9 @@
-// ffh_ssint.h -- surface-surface interactions (VdW_solver, src/VdW_solver.cpp:158-384,
-// 663-831): the lennard-jones type, ljsteric's Lennard-Jones with its soft core below Rmin, and
-// gensoft; face geometry and the lookup grid at es_update steps, and every step each active
-// face's Face::force summed in the reference's order.  The steric volume exclusion that
-// ljsteric and gensoft apply instead where face elements overlap is not built.
+// ffh_ssint.h -- surface-surface interactions (VdW_solver, src/VdW_solver.cpp:158-487,
+// 663-831): the lennard-jones type, ljsteric's Lennard-Jones with its soft core below Rmin,
+// gensoft, and the steric volume exclusion (ffh_steric.h) that steric applies alone and
+// ljsteric and gensoft apply instead of their potential where two face elements overlap; face
+// geometry and the lookup grid at es_update steps, and every step each active face's
+// Face::force[0..3] summed in the reference's order.
 // Part of ffh_kernels.hip: included inside its anonymous namespace, after the headers it
 // lists there; not a stand-alone header.
 #pragma once
@@ -27,8 +28,12 @@ struct SsArgs {
     const int32_t *type;   // [P]
     const double *emin, *rmin, *k0;  // [ntypes^2] meso, entry type1 * ntypes + type2
     int ntypes;
-    int kind;              // 1 lennard-jones, 2 ljsteric (its soft core below Rmin), 3 gensoft
-    const int32_t *nodes4; // [4P] corners and opposite element vertex (N_t ids), for kinds 2 and 3
+    int kind;              // 1 lennard-jones, 2 ljsteric (its soft core below Rmin), 3 gensoft,
+                           // 4 steric; 2-4 apply the steric forces where face elements overlap
+    const int32_t *nodes4; // [4P] corners and opposite element vertex (N_t ids), for kinds 2-4
+    const int32_t *opp;    // [P] the opposite vertex (compact id), for kinds 2-4
+    double st_factor, st_dr;  // steric_factor, steric_dr (meso)
+    int32_t *deferred;     // [P] kinds 2-4: 1 where some pair of the face overlaps (this step)
     double *geom;          // [8P] centroid, unit normal, area (Face::calc_area_normal_centroid)
     int32_t *cell;         // [P] lookup-grid cell of each face at the last refresh
     int32_t *order;        // [P] 0..P-1, the sort's values
@@ -38,8 +43,9 @@ struct SsArgs {
     double h;
     int inc_self, P, ncell;
     SsCtl *ctl;
-    int *err;              // host-visible: a face centroid fell outside the grid
-    double *out;           // [9P] Face::force of the three corners of every pool face
+    int *err;              // host-visible: 1 a face centroid fell outside the grid, 2 a face
+                           // element too degenerate for the steric volume (the reference throws)
+    double *out;           // [12P] Face::force[0..3] of every pool face: corners, opposite vertex
 };
 
 // LinkedListCube::pbc (src/LinkedListCube.tpp:282-299): one period at most.
@@ -220,10 +226,87 @@ __device__ void ss_pair(const SsArgs &a, int f1, int f2, int side, double *acc, 
     }
 }
 
+// The tetrahedron of face f: its three corners and the opposite element vertex.
+__device__ __forceinline__ void ss_tet(const SsArgs &a, int f, double (*T)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double4 c = corner(a.pos, a.ends + 6 * (size_t)f + 2 * k);
+        T[k][0] = c.x;
+        T[k][1] = c.y;
+        T[k][2] = c.z;
+    }
+    const double4 o = a.pos[a.opp[f]];
+    T[3][0] = o.x;
+    T[3][1] = o.y;
+    T[3][2] = o.z;
+}
+
+// The interaction of the pair f1 < f2 that passed consider_interaction, as do_interaction of
+// each type (src/VdW_solver.cpp:327-330, src/Steric_solver.cpp:28-38,
+// src/LJSteric_solver.cpp:34-52, src/GenSoftSSINT_solver.cpp:29-47): for kinds 2-4, where the
+// face elements overlap (Face::checkTetraIntersection, src/Face.cpp:399-404) and their common
+// volume is not zero, the steric forces on all four nodes of face 1 (side 0) or face 2 (side 1)
+// and energy steric_factor * V; otherwise the pair potential (kinds 1-3; steric adds nothing).
+// acc [12]: Face::force[0..3] of that face.
+__device__ void ss_interact(const SsArgs &a, int f1, int f2, int side, double *acc, double *energy) {
+    if (a.kind != 1) {
+        double A[4][3], B[4][3];
+        ss_tet(a, f1, A);
+        ss_tet(a, f2, B);
+        if (st::tets_overlap(A, B)) {
+            double e, fA[12], fB[12];
+            int degenerate = 0;
+            const bool applied = st::pair_forces(A, B, a.st_dr, a.st_factor, &e, fA, fB, degenerate);
+            if (degenerate) *a.err = 2;
+            if (applied) {
+                if (energy) *energy = e;
+                if (acc)
+                    for (int k = 0; k < 12; k++) acc[k] += side == 0 ? fA[k] : fB[k];
+                return;
+            }
+        }
+        if (a.kind == 4) {
+            if (energy) *energy = 0.0;
+            return;
+        }
+    }
+    ss_pair(a, f1, f2, side, acc, energy);
+}
+
+// Whether the face elements of f1 and f2 overlap (kinds 2-4 screen every pair with it).
+__device__ __forceinline__ bool ss_overlap(const SsArgs &a, int f1, int f2) {
+    double A[4][3], B[4][3];
+    ss_tet(a, f1, A);
+    ss_tet(a, f2, B);
+    return st::tets_overlap(A, B);
+}
+
+// The force kernel's three modes: kSsLJ, lennard-jones (no steric test); kSsScreen, kinds 2-4
+// with the pair potential only, a face with an overlapping pair left to the next mode
+// (deferred); kSsFull, the deferred faces with the steric forces.  The volume's private
+// point list and its call stay out of the first two, which keep their registers (4 waves
+// per SIMD); overlapping face elements are rare, and only their faces pay for it.
+enum { kSsLJ = 0, kSsScreen = 1, kSsFull = 2 };
+
+// One pair's contribution in the given mode; returns true (kSsScreen) where the face elements
+// overlap and the face must be deferred.
+template <int kMode>
+__device__ __forceinline__ bool ss_contrib(const SsArgs &a, int f1, int f2, int side, double *acc) {
+    if (kMode == kSsFull) {
+        ss_interact(a, f1, f2, side, acc, nullptr);
+        return false;
+    }
+    if (kMode == kSsScreen && ss_overlap(a, f1, f2)) return true;
+    if (kMode == kSsLJ || a.kind != 4) ss_pair(a, f1, f2, side, acc, nullptr);
+    return false;
+}
+
 // Face f's pairs as face 2, (i, f) for i < f in ascending i (a cell reached through several
 // offsets counts that often), summed into acc by one thread: a merge of the 27 neighbour
 // ranges.  The fallback of k_ss_forces for faces with more candidates than its LDS list holds.
-__device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
+// Returns true (kSsScreen) where an overlapping pair defers the face.
+template <int kMode>
+__device__ bool ss_face2_serial(const SsArgs &a, int f, double *acc) {
     const int cf = a.cell[f];
     int cur[27], lim[27];
     for (int c = 0; c < 27; c++) {
@@ -243,8 +326,9 @@ __device__ void ss_face2_serial(const SsArgs &a, int f, double *acc) {
             }
         if (bc < 0 || best >= f) break;
         cur[bc]++;
-        if (ss_consider(a, best, f)) ss_pair(a, best, f, 1, acc, nullptr);
+        if (ss_consider(a, best, f) && ss_contrib<kMode>(a, best, f, 1, acc)) return true;
     }
+    return false;
 }
 
 constexpr int kSsCap = 1024;  // candidates below f a wave sorts (in registers, 16 per lane)
@@ -311,7 +395,8 @@ __device__ __forceinline__ void ss_sort_cand(int *cand, int na, int lane) {
 // interact stages +0.0, and adding +0.0 leaves every sum unchanged: a face force starts at +0.0
 // and never becomes -0.0 (in the reference either), so the loop needs no test and its loads
 // pipeline.
-__device__ __forceinline__ void ss_ordered_add(const double (*con)[9], int k, int e, double &acc) {
+template <int kN>
+__device__ __forceinline__ void ss_ordered_add(const double (*con)[kN], int k, int e, double &acc) {
     int q = 0;
     for (; q + 8 <= e; q += 8) {
         double v[8];
@@ -329,16 +414,20 @@ __device__ __forceinline__ void ss_ordered_add(const double (*con)[9], int k, in
 // the pairs (i, f) for i < f in ascending i, then, as face 1, its own pairs (f, j) in its
 // visiting order.  The pairs that pass consider_interaction join a queue in that order (a
 // ballot per 64 candidates); the lanes evaluate 64 queued pairs at a time, and the
-// contributions then go through LDS and are added one by one in queue order by lanes 0..8 (one
+// contributions then go through LDS and are added one by one in queue order by lanes 0..11 (one
 // force component each), so every sum has the reference's order.  Pairs that do not pass add
-// nothing in the reference and are left out here.
+// nothing in the reference and are left out here.  kMode: kSsLJ, kSsScreen or kSsFull (above).
+template <int kMode>
 __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     __shared__ int rs[27], re[27], pa[28], pb[28];
     __shared__ int cand[kSsCap];
     __shared__ int queue[128];  // pending pairs: the other face j for (f, j), ~i for (i, f)
-    __shared__ double con[64][9];
+    constexpr int kN = kMode == kSsLJ ? 9 : 12;  // force components: the opposite vertex's only take steric forces
+    __shared__ double con[64][kN];
+    __shared__ int defer;
     const int f = blockIdx.x, lane = threadIdx.x;
     if (f >= a.P) return;
+    if (kMode == kSsFull && !a.deferred[f]) return;  // the whole block: one face
     if (lane < 27) {
         // the cell's stack sorted[s0, s1) ascends by pool index: the entries below f, then f
         // itself (in its own cell), then the entries above f
@@ -367,7 +456,7 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     }
     __syncthreads();
     const int na = pa[27], nb = pb[27];
-    double acc = 0.0;  // lane k < 9: component k of the face force (corner k / 3, axis k % 3)
+    double acc = 0.0;  // lane k < kN: component k of the face force (node k / 3, axis k % 3)
     int nq = 0;        // queued pairs (the same in every lane)
     // (a) as face 2: the candidates below f (the bottoms of the stacks), sorted ascending
     if (na <= kSsCap) {
@@ -377,14 +466,16 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
         }
         __syncthreads();
     }
+    bool deferred = false;  // the same in every lane
     if (na > kSsCap) {  // too many to sort here: one lane merges
-        double acc9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double acc12[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (lane == 0) {
-            ss_face2_serial(a, f, acc9);
-            for (int k = 0; k < 9; k++) con[0][k] = acc9[k];
+            defer = ss_face2_serial<kMode>(a, f, acc12);
+            for (int k = 0; k < kN; k++) con[0][k] = acc12[k];
         }
         __syncthreads();
-        if (lane < 9) acc = con[0][lane];
+        if (lane < kN) acc = con[0][lane];
+        deferred = defer;
         __syncthreads();
     } else if (na > 1) {
         int m = 64;
@@ -401,7 +492,7 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
     // the queue: part (a) chunks of 64 sorted candidates, then part (b) chunks of 64 visiting
     // positions, then one more pass that evaluates what is left
     const int chunks_a = na > kSsCap ? 0 : (na + 63) / 64, chunks = chunks_a + (nb + 63) / 64;
-    for (int ch = 0; ch <= chunks; ch++) {
+    for (int ch = 0; ch <= chunks && !deferred; ch++) {
         int i1 = -1, i2 = -1;  // the pair (i1 < i2) this lane queues, if it passes
         if (ch < chunks_a) {
             const int t = ch * 64 + lane;
@@ -423,15 +514,20 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
         __syncthreads();
         const int e = ch == chunks ? nq : (nq >= 64 ? 64 : 0);
         if (e > 0) {  // evaluate the first e queued pairs, one per lane; add them in queue order
-            double c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double c12[kN] = {};
+            bool ov = false;
             if (lane < e) {
                 const int o = queue[lane];
                 const bool second = o < 0;
-                ss_pair(a, second ? ~o : f, second ? f : o, second ? 1 : 0, c9, nullptr);
+                ov = ss_contrib<kMode>(a, second ? ~o : f, second ? f : o, second ? 1 : 0, c12);
             }
-            for (int k = 0; k < 9; k++) con[lane][k] = c9[k];
+            if (kMode == kSsScreen && __ballot(ov)) {  // wave-uniform: leave the face to kSsFull
+                deferred = true;
+                break;
+            }
+            for (int k = 0; k < kN; k++) con[lane][k] = c12[k];
             __syncthreads();
-            if (lane < 9) ss_ordered_add(con, lane, e, acc);
+            if (lane < kN) ss_ordered_add<kN>(con, lane, e, acc);
             nq -= e;
             const int carry = lane < nq ? queue[64 + lane] : 0;
             __syncthreads();
@@ -439,7 +535,8 @@ __global__ __launch_bounds__(64) void k_ss_forces(SsArgs a) {
             __syncthreads();
         }
     }
-    if (lane < 9) a.out[9 * (size_t)f + lane] = acc;
+    if (kMode == kSsScreen && lane == 0) a.deferred[f] = deferred ? 1 : 0;
+    if (!deferred && lane < kN) a.out[12 * (size_t)f + lane] = acc;
 }
 
 // The energy of the last solve (VdW_solver::get_field_energy(-1, -1),
@@ -481,6 +578,7 @@ __global__ __launch_bounds__(64) void k_ss_pair_count(SsArgs a, int *cnt) {
     if (lane == 0) cnt[f] = k;
 }
 
+template <bool kSteric>  // kinds 2-4: the steric energy where face elements overlap
 __global__ __launch_bounds__(64) void k_ss_pair_energy(SsArgs a, int nblobs, const int *off, double *pe, int *pb) {
     __shared__ int rs[27], re[27], pre[28];
     const int f = blockIdx.x, lane = threadIdx.x;
@@ -500,7 +598,10 @@ __global__ __launch_bounds__(64) void k_ss_pair_energy(SsArgs a, int nblobs, con
         if (ok) {
             const int r = __popcll(m & ((1ull << lane) - 1ull));
             double e;
-            ss_pair(a, f, j, 0, nullptr, &e);
+            if (kSteric)
+                ss_interact(a, f, j, 0, nullptr, &e);
+            else
+                ss_pair(a, f, j, 0, nullptr, &e);
             pe[w + r] = e;
             pb[w + r] = a.blob[f] * nblobs + a.blob[j];
         }

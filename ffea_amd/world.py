@@ -342,6 +342,15 @@ class World:
         self.batch = Batch(self.meshes, self.params, self.scales, rng_state=st, precision=precision)
         self.batch.set_positions(np.concatenate(self.pos0))
         self.batch.set_solver(self.solver)
+        # <velocity = (vx, vy, vz)>: every node of the blob (m/s -> meso; Blob::velocity_all,
+        # src/World.cpp:2577-2584, 2707-2709); the massless solve overwrites it in its first step
+        if self.mass_in_system and any("velocity" in b for b in s["blobs"]):
+            off = self.batch.node_off
+            vel = np.zeros((int(off[-1]), 3))
+            for k, b in enumerate(s["blobs"]):
+                if "velocity" in b:
+                    vel[off[k]:off[k + 1]] = np.array(_vec3(b["velocity"])) / U.MESO_VELOCITY
+            self.batch.set_velocities(vel)
         # walls: periodic ones wrap a blob whose centroid leaves the box at the start of every
         # step (src/World.cpp:1850-1901).  A hard wall zeroes outward velocities of surface nodes
         # (Blob::enforce_box_boundaries, src/Blob.cpp:2272-2330), which the massless solve then
@@ -415,11 +424,12 @@ class World:
     def _surface_interactions(self, s):
         """Surface-surface interactions as World::init sets them up (src/World.cpp:273-280,
         896-970; SimulationParams checks, src/SimulationParams.cpp:626-660): the faces of the
-        blobs' .vdw / .ssint files that are active (type != -1).  The SurfSurfEnergy column is written whenever calc_ssint or calc_steric is on (the
-        defaults: ljsteric with steric on).  lennard-jones, ljsteric and gensoft run on the
-        device; the steric volume exclusion (alone, or where ljsteric / gensoft face elements
-        overlap) is not computed.  Periodic blob corrections (force_pbc) and sticky walls are
-        not on this path."""
+        blobs' .vdw / .ssint files that are active (type != -1).  The SurfSurfEnergy column is
+        written whenever calc_ssint or calc_steric is on (the defaults: ljsteric with steric
+        on).  lennard-jones, ljsteric, gensoft and the steric volume exclusion alone (steric,
+        or calc_ssint 0 with calc_steric 1) run on the device, with steric_factor and steric_dr
+        (src/SimulationParams.cpp:412-418).  Periodic blob corrections (force_pbc) and sticky
+        walls are not on this path."""
         p = self.raw
         calc_ssint = int(float(p.get("calc_ssint", p.get("calc_vdw", 1))))
         calc_steric = int(float(p.get("calc_steric", 1)))
@@ -459,7 +469,7 @@ class World:
                 raise F.FormatError(f"blob {b}: surface-surface interactions need its surface and vdw files")
             surf = F.read_surface(blob["surface"])
             fc = surf[:, 1:4]
-            if calc_ssint == 1 and kind != "lennard-jones":  # Face::n[3], read by their pair tests
+            if not (calc_ssint == 1 and kind == "lennard-jones"):  # Face::n[3]: pair tests, steric
                 opposite.append(F.surface_opposite(surf, self.meshes[b].topo) + off[b])
             ty = F.read_ssint_types(path, fc.shape[0])
             if calc_ssint == 1:
@@ -473,19 +483,15 @@ class World:
         if not (types != -1).any():
             return
         if calc_ssint != 1:
-            # steric volume exclusion alone (the reference's default with calc_ssint 0): its
-            # forces and energy are zero unless surface elements overlap, and it is not built
-            print("warning: steric surface interactions are not computed on this path (zero unless surface "
-                  "elements overlap); SurfSurfEnergy is written as 0", file=sys.stderr)
-            return
-        if kind != "lennard-jones":
-            print(f"warning: {kind}: the steric volume exclusion it applies where face elements overlap is not "
-                  "computed on this path; elsewhere the interaction is the reference's", file=sys.stderr)
+            kind = "steric"  # the volume exclusion alone (Steric_solver, or calc_ssint 0)
         if int(float(p.get("force_pbc", 0))) == 1:
             raise F.FormatError("force_pbc is not on this path")
         cutoff = float(p.get("ssint_cutoff", p.get("vdw_cutoff", 3e-9))) / U.MESO_LENGTH
         inc_self = int(float(p.get("inc_self_ssint", p.get("inc_self_vdw", 1))))
         es_update = int(float(p.get("es_update", 10)))
+        if kind != "lennard-jones":
+            self.batch.set_steric(float(p.get("steric_factor", p.get("vdw_steric_factor", 1))),
+                                  float(p.get("steric_dr", p.get("vdw_steric_dr", 5e-3))))
         self.batch.set_ssint(np.concatenate(faces), types, E, R, self.es_N, cutoff, inc_self, es_update, kind=kind,
                              opposite=np.concatenate(opposite) if opposite else None, k0=K)
 

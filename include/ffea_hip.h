@@ -249,25 +249,32 @@ int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_node
  * src/VdW_solver.cpp:158-384, 536-831; SSINT_matrix, src/LJ_matrix.cpp:122-199; the lookup grid
  * of NearestNeighbourLinkedListCube, src/NearestNeighbourLinkedListCube.cpp:26-77).
  * kind: FFH_SSINT_LENNARD_JONES, FFH_SSINT_LJSTERIC (Lennard-Jones with the soft core below
- * Rmin, calc_ljinterpolated_force_pair_matrix) or FFH_SSINT_GENSOFT (rmin = the cutoff, k0).
+ * Rmin, calc_ljinterpolated_force_pair_matrix), FFH_SSINT_GENSOFT (rmin = the cutoff, k0) or
+ * FFH_SSINT_STERIC (the volume exclusion alone, Steric_solver; emin, rmin, k0 unread, may be
+ * null).
  * face_nodes [3*num_faces]: the surface faces' corners (N_t ids of the handle, blobs in order,
  * each blob's faces in .surf order); face_opposite [num_faces]: Face::n[3], the element vertex
- * opposite each face (read by the ljsteric / gensoft pair tests; may be null for
- * lennard-jones); face_type [num_faces]: the .vdw / .ssint type, -1 for an inactive face;
+ * opposite each face (for every kind but lennard-jones, for which it may be null);
+ * face_type [num_faces]: the .vdw / .ssint type, -1 for an inactive face;
  * emin, rmin, k0 [num_types^2] in meso units (Emin * area^2 / Energy, Rmin / length,
  * k0 * area / Energy), entry type1 * num_types + type2; grid [3] = es_N_x/y/z and cell_size =
  * ssint_cutoff (meso) of the lookup grid; inc_self_ssint; es_update (face geometry and grid
  * refreshed on the first step and every es_update steps, as World::run does).  Every step
  * then starts with the solve: each pair of active faces in neighbouring cells (i < j in pool
- * order, other blob unless inc_self, normals not facing the same way; for ljsteric and gensoft
- * also faces in front of each other and no shared node within a blob) adds its
- * three-point-quadrature forces to both faces, which reach the nodes with the other face
- * forces after the element forces (src/Blob.cpp:3842-3851).  The steric volume exclusion that
- * ljsteric and gensoft apply instead where the two face elements overlap
- * (Face::checkTetraIntersection) is not on this path.  All faces inactive (or num_faces 0)
- * removes the interaction.  Linear surface forces (CTF 3) together with it are refused.  A
- * face centroid outside the grid after one period fails the step (FFH_E_ARG), as
- * LinkedListCube::add_node_to_stack does.
+ * order, other blob unless inc_self, normals not facing the same way; for the kinds other than
+ * lennard-jones also faces in front of each other and no shared node within a blob) interacts.
+ * For ljsteric, gensoft and steric, where the two face elements (corners and opposite vertex)
+ * overlap (Face::checkTetraIntersection, src/CheckTetrahedraOverlap.cpp) with a nonzero common
+ * volume V (src/VolumeIntersection.cpp), the pair gets the steric forces instead
+ * (VdW_solver::do_steric_interaction, src/VdW_solver.cpp:458-487; Face.cpp:473-514): energy
+ * steric_factor * V and forces steric_factor * phi_j dV/dr on the four nodes of each face
+ * (ffh_blob_set_steric); otherwise the three-point-quadrature forces of its potential (none for
+ * steric) on the corners.  The faces' forces reach the nodes with the other face forces after
+ * the element forces (src/Blob.cpp:3842-3851).  All faces inactive (or num_faces 0) removes
+ * the interaction.  Linear surface forces (CTF 3) together with it are refused.  A face
+ * centroid outside the grid after one period fails the step (FFH_E_ARG), as
+ * LinkedListCube::add_node_to_stack does; so does a face element whose geometry makes the
+ * reference throw (arr3Normalise2 of a zero vector).
  * ffh_blob_ssint_energy: VdW_solver::get_field_energy(-1, -1) of the last solve (meso energy),
  * summed in the solve's order; call it between ffh_blob_update_internal_forces and
  * ffh_blob_update_positions, as World::make_measurements does.
@@ -275,9 +282,18 @@ int ffh_blob_set_surface_forces(ffh_blob *b, int num_faces, const int *face_node
 #define FFH_SSINT_LENNARD_JONES 1
 #define FFH_SSINT_LJSTERIC 2
 #define FFH_SSINT_GENSOFT 3
+#define FFH_SSINT_STERIC 4
 int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nodes, const int *face_opposite,
                        const int *face_type, int num_types, const double *emin, const double *rmin, const double *k0,
                        const int *grid, double cell_size, int inc_self, int es_update);
+
+/*
+ * The steric parameters SimulationParams reads as steric_factor and steric_dr (aliases
+ * vdw_steric_factor, vdw_steric_dr; src/SimulationParams.cpp:412-418, 749-755), used as given
+ * (meso units, no conversion).  Defaults 1 and 5e-3 (src/SimulationParams.cpp:44-45).
+ * steric_dr must be > 0 (FFH_E_ARG otherwise, as the reference refuses it).
+ */
+int ffh_blob_set_steric(ffh_blob *b, double steric_factor, double steric_dr);
 int ffh_blob_ssint_energy(ffh_blob *b, double *energy);
 
 /*

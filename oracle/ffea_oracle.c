@@ -504,7 +504,7 @@ typedef struct {
     int *face_nodes;       /* [3 n_faces] */
     int *set_off, *set_faces;
     double *set_force;     /* [3 n_sets] */
-    double *face_force;    /* [9 n_faces]: the three face nodes' forces (Face::force[0..2]) */
+    double *face_force;    /* [12 n_faces]: Face::force[0..3], the three corners then the opposite vertex */
     /* surface-surface interactions: per surface face its type (-1: inactive) and the
      * geometry Face::calc_area_normal_centroid last left (centroid, normal, area) */
     int *ssint_type;       /* [n_faces] or NULL */
@@ -748,7 +748,7 @@ void orc_blob_set_positions(orc_blob *b, const double *pos) {
 static void blob_zero_forces(orc_blob *b) {
     memset(b->node_force, 0, sizeof(double) * 30 * (size_t)b->num_elements);
     memset(b->force, 0, sizeof(double) * 3 * (size_t)b->num_nodes);
-    if (b->face_force) memset(b->face_force, 0, sizeof(double) * 9 * (size_t)b->n_faces);
+    if (b->face_force) memset(b->face_force, 0, sizeof(double) * 12 * (size_t)b->n_faces);
 }
 
 /* ---- forces from outside the element loop ----------------------------------- */
@@ -852,7 +852,7 @@ void orc_blob_set_surface_forces(orc_blob *b, int n_faces, const int *faces, int
     b->set_off = xcalloc((size_t)n_sets + 1, sizeof(int));
     b->set_faces = xcalloc(n_sets > 0 && set_off[n_sets] > 0 ? set_off[n_sets] : 1, sizeof(int));
     b->set_force = xcalloc(3 * (size_t)(n_sets > 0 ? n_sets : 1), sizeof(double));
-    b->face_force = xcalloc(9 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
+    b->face_force = xcalloc(12 * (size_t)(n_faces > 0 ? n_faces : 1), sizeof(double));
     free(b->ssint_type);
     free(b->ssint_opp);
     free(b->face_geom);
@@ -903,7 +903,7 @@ static void blob_apply_ctforces(orc_blob *b) {
         for (int j = j0; j < j1; j++) {
             double t[3], f = areas[j - j0] / (3 * total);
             for (int k = 0; k < 3; k++) t[k] = f * b->set_force[3 * s + k];
-            double *ff = &b->face_force[9 * b->set_faces[j]];
+            double *ff = &b->face_force[12 * b->set_faces[j]];
             for (int v = 0; v < 3; v++)
                 for (int k = 0; k < 3; k++) ff[3 * v + k] += t[k];
         }
@@ -1122,11 +1122,14 @@ static int blob_aggregate_forces(orc_blob *b, orc_stream *g) {
             b->force[3 * n + 1] += c[1];
             b->force[3 * n + 2] += c[2];
         }
-    /* surface face forces (src/Blob.cpp:3842-3851), face by face; the fourth node's slot only
-     * carries interaction forces, zero here */
-    for (int n = 0; n < b->n_faces; n++)
+    /* surface face forces (src/Blob.cpp:3842-3851), face by face, Face::n[0..3]; the opposite
+     * vertex n[3] (known once interaction types are set) only carries steric forces */
+    for (int n = 0; n < b->n_faces; n++) {
         for (int i = 0; i < 3; i++)
-            for (int k = 0; k < 3; k++) b->force[3 * b->face_nodes[3 * n + i] + k] += b->face_force[9 * n + 3 * i + k];
+            for (int k = 0; k < 3; k++) b->force[3 * b->face_nodes[3 * n + i] + k] += b->face_force[12 * n + 3 * i + k];
+        if (b->ssint_opp && b->ssint_opp[n] >= 0)
+            for (int k = 0; k < 3; k++) b->force[3 * b->ssint_opp[n] + k] += b->face_force[12 * n + 9 + k];
+    }
     if (b->calc_stokes == 1 && b->linear_solver != 3) {
         /* mass-ful branch (src/Blob.cpp:3854-3877): drag on the node velocity, then noise */
         for (int i = 0; i < N; i++) {
@@ -1455,7 +1458,8 @@ static const int g_adj[27][3] = {{-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0,
 
 /* SSINT_matrix in meso units (LJI(i, j) = i * ntypes + j), the lookup grid and the solver's
  * switches (World::init, src/World.cpp:896-970; VdW_solver::init). ntypes 0: off.  kind: 1
- * lennard-jones, 2 ljsteric (its soft-core branch below Rmin), 3 gensoft. */
+ * lennard-jones, 2 ljsteric (its soft-core branch below Rmin), 3 gensoft, 4 steric (volume
+ * exclusion only); 2-4 apply the steric forces where face elements overlap. */
 static int g_ss_ntypes, g_ss_inc_self, g_ss_es_update, g_ss_es_count, g_ss_N[3], g_ss_kind = 1;
 static double *g_ss_emin, *g_ss_rmin, *g_ss_k0, g_ss_h;
 static double *g_ss_energy; /* fieldenergy [nblobs * nblobs] of the last solve */
@@ -1662,9 +1666,484 @@ static void lj_pair(orc_blob *b1, int f1, orc_blob *b2, int f2, double *energy_o
                 for (int x = 0; x < 3; x++) F1[x] += c * fpm[k][l][x];
                 for (int x = 0; x < 3; x++) F2[x] -= d * fpm[l][k][x];
             }
-        for (int x = 0; x < 3; x++) b1->face_force[9 * f1 + 3 * j + x] += F1[x] * ApAq;
-        for (int x = 0; x < 3; x++) b2->face_force[9 * f2 + 3 * j + x] += F2[x] * ApAq;
+        for (int x = 0; x < 3; x++) b1->face_force[12 * f1 + 3 * j + x] += F1[x] * ApAq;
+        for (int x = 0; x < 3; x++) b2->face_force[12 * f2 + 3 * j + x] += F2[x] * ApAq;
     }
+}
+
+/* ---- steric volume exclusion: overlap test, intersection volume, its gradient ------------ */
+
+/* Steric parameters (SimulationParams steric_factor and steric_dr, src/SimulationParams.cpp:412-
+ * 418, used as given, in meso units) and a flag for the geometry the reference cannot handle
+ * (arr3Normalise2 throws on a zero vector, src/mat_vec_fns_II.cpp:164-177). */
+static double g_st_factor = 1.0, g_st_dr = 5e-3;
+static int g_st_throw;
+
+void orc_world_set_steric(double factor, double dr) {
+    g_st_factor = factor;
+    g_st_dr = dr;
+}
+
+/* The vector helpers of src/mat_vec_fns_II.cpp, each rounding as the reference's loops do:
+ * dot and norm sums start from 0.0 and add the terms in index order. */
+static double vx_dot(const double *a, const double *b) { /* arr3arr3DotProduct :138-147 */
+    double r = 0.0;
+    r += a[0] * b[0];
+    r += a[1] * b[1];
+    r += a[2] * b[2];
+    return r;
+}
+static void vx_sub(const double *a, const double *b, double *r) { /* arr3arr3Substract :117-122 */
+    for (int i = 0; i < 3; i++) r[i] = a[i] - b[i];
+}
+static void vx_add(const double *a, const double *b, double *r) { /* arr3arr3Add :105-110 */
+    for (int i = 0; i < 3; i++) r[i] = a[i] + b[i];
+}
+static void vx_neg(double *u) { /* arr3Resize(mOne, u) :179-184 */
+    for (int i = 0; i < 3; i++) u[i] = -1.0 * u[i];
+}
+static void vx_cross(const double *u, const double *v, double *w) { /* arr3arr3VectorProduct :127-132 */
+    w[0] = u[1] * v[2] - v[1] * u[2];
+    w[1] = -u[0] * v[2] + v[0] * u[2];
+    w[2] = u[0] * v[1] - v[0] * u[1];
+}
+static void vx_unit_cross(const double *u, const double *v, double *w) { /* getUnitNormal :311-322 */
+    vx_cross(u, v, w);
+    const double l = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int i = 0; i < 3; i++) w[i] /= l;
+}
+static void vx_tangent(const double *a, const double *b, double *t) { /* tangent :295-306 */
+    double w = 0;
+    for (int i = 0; i < 3; i++) {
+        t[i] = a[i] - b[i];
+        w += t[i] * t[i];
+    }
+    w = sqrt(w);
+    for (int i = 0; i < 3; i++) t[i] /= w;
+}
+static void vx_normalise_into(const double *e, double *n) { /* arr3Normalise2 :164-177 */
+    double s = 0.0;
+    for (int i = 0; i < 3; i++) s += e[i] * e[i];
+    if (s == 0.0) g_st_throw = 1;
+    s = sqrt(s);
+    for (int i = 0; i < 3; i++) n[i] = e[i] / s;
+}
+static double vx_distance(const double *a, const double *b) { /* arr3arr3Distance :218-224 */
+    double d = 0.0;
+    for (int i = 0; i < 3; i++) d += (a[i] - b[i]) * (a[i] - b[i]);
+    return sqrt(d);
+}
+static int vx_same_sign(double a, double b) { return a * b >= 0; } /* sameSign :34-38 */
+
+/* getNormalInwards(f0, f1, f2, p3, n) (:361-377): the unit normal of the plane f0 f1 f2,
+ * turned to the side of p3. */
+static void vx_normal_inwards(const double *f0, const double *f1, const double *f2, const double *p3, double *n) {
+    double pl1[3], pl2[3], aux[3];
+    vx_sub(f1, f0, pl1);
+    vx_sub(f2, f0, pl2);
+    vx_cross(pl1, pl2, aux);
+    vx_normalise_into(aux, n);
+    vx_add(aux, f0, aux);
+    const double d = -vx_dot(n, f0);
+    const double t1 = vx_dot(p3, n) + d, t2 = vx_dot(aux, n) + d;
+    if (!vx_same_sign(t1, t2)) vx_neg(n);
+}
+
+/* sameSidePlane (:383-398): vec and test on one side of the plane p1 p2 p3 (or on it). */
+static int vx_same_side_plane(const double *vec, const double *test, const double *p1, const double *p2,
+                              const double *p3) {
+    double pl1[3], pl2[3], n[3];
+    vx_sub(p2, p1, pl1);
+    vx_sub(p3, p1, pl2);
+    vx_cross(pl1, pl2, n);
+    double s = 0.0;  /* arr3Normalise (:149-158), no zero test */
+    for (int i = 0; i < 3; i++) s += n[i] * n[i];
+    s = sqrt(s);
+    for (int i = 0; i < 3; i++) n[i] /= s;
+    const double d = -vx_dot(n, p1);
+    return vx_same_sign(vx_dot(vec, n) + d, vx_dot(test, n) + d);
+}
+
+/* sameSideLine (:405-418): e and p1 on one side of the line p2 p3 within their plane. */
+static int vx_same_side_line(const double *e, const double *p1, const double *p2, const double *p3) {
+    double pl23[3], pl21[3], pl2e[3], v1[3], ve[3];
+    vx_sub(p3, p2, pl23);
+    vx_sub(p1, p2, pl21);
+    vx_sub(e, p2, pl2e);
+    vx_cross(pl21, pl23, v1);
+    vx_cross(pl2e, pl23, ve);
+    return vx_dot(v1, ve) >= 0;
+}
+
+/* nodeInTet (:434-441): on the inner side of all four face planes. */
+static int vx_node_in_tet(const double *v, const double *t0, const double *t1, const double *t2, const double *t3) {
+    return vx_same_side_plane(v, t0, t1, t2, t3) && vx_same_side_plane(v, t1, t2, t3, t0) &&
+           vx_same_side_plane(v, t2, t3, t0, t1) && vx_same_side_plane(v, t3, t0, t1, t2);
+}
+
+/* intersectionPoint(ip, e1, e2, f1, f2, f3) (:573-588) with linePlaneIntersectionPoint
+ * (:450-470) and isPointInFace (:531-538): where the edge e1 e2 crosses the triangle. */
+static int vx_edge_face_point(double *ip, const double *e1, const double *e2, const double *f1, const double *f2,
+                              const double *f3) {
+    if (vx_same_side_plane(e1, e2, f1, f2, f3)) return 0;
+    double v[3], pl1[3], pl2[3], pn[3];
+    vx_sub(e2, e1, v);
+    vx_sub(f2, f1, pl1);
+    vx_sub(f3, f2, pl2);
+    vx_unit_cross(pl1, pl2, pn);
+    const double d = -vx_dot(pn, f1);
+    const double t = -(vx_dot(pn, e1) + d) / vx_dot(pn, v);
+    for (int i = 0; i < 3; i++) v[i] = t * v[i];
+    vx_add(e1, v, ip);
+    return vx_same_side_line(ip, f1, f2, f3) && vx_same_side_line(ip, f3, f1, f2) && vx_same_side_line(ip, f2, f3, f1);
+}
+
+/* getBAndN(f0, f1, f2, p3, t, b, n) (src/VolumeIntersection.cpp:179-204): for the edge leaving
+ * f0 along t, b the inward unit normal of its face f0 f1 f2 and n the unit vector in that face
+ * normal to t, towards f2. */
+static void vx_face_frame(const double *f0, const double *f1, const double *f2, const double *p3, const double *t,
+                          double *b, double *n) {
+    double pl2[3], aux[3];
+    vx_sub(f2, f0, pl2);
+    vx_cross(t, pl2, aux);
+    vx_normalise_into(aux, b);
+    vx_add(aux, f0, aux);
+    const double d = -vx_dot(b, f0);
+    const double t1 = vx_dot(p3, b) + d, t2 = vx_dot(aux, b) + d;
+    if (!vx_same_sign(t1, t2)) vx_neg(b);
+    vx_unit_cross(b, t, n);
+    vx_add(f0, n, aux);
+    if (!vx_same_side_line(aux, f2, f1, f0)) vx_neg(n);
+}
+
+/* volumeForNode(tet0, tet1, tet2, tetN) (:259-298): vertex tetN's term of the polyhedron
+ * volume sum, over its three edges and the two faces at each. */
+static double vx_vertex_term(const double *a0, const double *a1, const double *a2, const double *aN) {
+    const double *far[3] = {a0, a1, a2};
+    static const int other[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+    double vol = 0.0, t[3], b[3], n[3];
+    for (int e = 0; e < 3; e++) {
+        vx_tangent(far[e], aN, t);
+        const double pt = vx_dot(aN, t);
+        for (int s = 0; s < 2; s++) {
+            const double *q1 = far[other[e][s]], *q2 = far[other[e][1 - s]];
+            vx_face_frame(aN, far[e], q1, q2, t, b, n);
+            vol += pt * vx_dot(aN, n) * vx_dot(aN, b);
+        }
+    }
+    return vol;
+}
+
+/* volumeForIntPointII (:547-655): the term of an intersection point ip of edge e1 e2 (of the
+ * tetrahedron e1 e2 e3 e4) with the face f1 f2 f3 (of the tetrahedron f1 f2 f3 f4). */
+static double vx_crossing_term(const double *ip, const double *e1, const double *e2, const double *e3,
+                               const double *e4, const double *f1, const double *f2, const double *f3,
+                               const double *f4) {
+    double B[3][3], T[3][3], aux[3], n[3];
+    vx_normal_inwards(f1, f2, f3, f4, B[0]);
+    vx_tangent(e2, e1, T[0]);
+    if (vx_same_side_plane(e1, f4, f1, f2, f3)) vx_neg(T[0]);
+    vx_normal_inwards(e1, e2, e3, e4, B[1]);
+    vx_unit_cross(B[0], B[1], T[1]);
+    vx_add(ip, T[1], aux);
+    if (!vx_same_side_plane(aux, e3, e4, e1, e2)) vx_neg(T[1]);
+    vx_normal_inwards(e1, e2, e4, e3, B[2]);
+    vx_unit_cross(B[0], B[2], T[2]);
+    vx_add(ip, T[2], aux);
+    if (!vx_same_side_plane(aux, e4, e3, e1, e2)) vx_neg(T[2]);
+    double ipT[3], ipB[3];
+    for (int i = 0; i < 3; i++) {
+        ipT[i] = vx_dot(ip, T[i]);
+        ipB[i] = vx_dot(ip, B[i]);
+    }
+    double vol = 0;
+    /* the edge e1 e2 with the faces e1 e2 e3 and e1 e2 e4 */
+    const double *side[2] = {e3, e4};
+    for (int s = 0; s < 2; s++) {
+        vx_unit_cross(B[1 + s], T[0], n);
+        vx_add(ip, n, aux);
+        if (!vx_same_side_line(aux, side[s], e2, e1)) vx_neg(n);
+        vol += ipT[0] * vx_dot(ip, n) * ipB[1 + s];
+    }
+    /* the two new edges, each with the face f1 f2 f3 and its own face of e1 e2 e3 e4 */
+    double V[3][3];
+    vx_add(ip, T[1], V[0]);
+    vx_add(ip, T[2], V[1]);
+    for (int i = 0; i < 3; i++) V[2][i] = V[0][i];
+    for (int i = 0; i < 2; i++) {
+        vx_unit_cross(B[0], T[i + 1], n);
+        vx_add(ip, n, aux);
+        if (!vx_same_side_line(aux, V[i + 1], V[i], ip)) vx_neg(n);
+        vol += ipT[i + 1] * vx_dot(ip, n) * ipB[0];
+        vx_unit_cross(B[i + 1], T[i + 1], n);
+        if (vx_dot(n, B[0]) < 0) vx_neg(n);
+        vol += ipT[i + 1] * vx_dot(ip, n) * ipB[i + 1];
+    }
+    return vol;
+}
+
+/* The intersection polyhedron's vertices as they are found (at most 8 + 48), in the
+ * reference's list W[56] and count ips. */
+typedef struct {
+    double w[56][3];
+    int n;
+} vx_points;
+
+/* exists (:32-43): a point within 3 machine epsilons of one already listed. */
+static int vx_listed(const vx_points *P, const double *p) {
+    for (int i = 0; i < P->n; i++)
+        if (vx_distance(p, P->w[i]) < 3.0 * DBL_EPSILON) return 1;
+    return 0;
+}
+
+/* contribVolForIntersections (:929-949): edge ai aj of the tetrahedron ai aj a3 a4 against the
+ * four faces of b0 b1 b2 b3, taken as (b0 b1 b2), (b1 b2 b3), (b2 b3 b0), (b3 b0 b1). */
+static void vx_edge_terms(double *vol, vx_points *P, const double *ai, const double *aj, const double *a3,
+                          const double *a4, const double *const *b) {
+    double ip[3];
+    for (int f = 0; f < 4; f++) {
+        const double *q0 = b[f], *q1 = b[(f + 1) & 3], *q2 = b[(f + 2) & 3], *q3 = b[(f + 3) & 3];
+        if (vx_edge_face_point(ip, ai, aj, q0, q1, q2) && !vx_listed(P, ip)) {
+            memcpy(P->w[P->n++], ip, sizeof(ip));
+            *vol += vx_crossing_term(ip, ai, aj, a3, a4, q0, q1, q2, q3);
+        }
+    }
+}
+
+/* volumeIntersectionII (:951-1024) with maxVolume (:48-66) and findCM (:95-104): the volume
+ * common to tetrahedra A and B as -1/6 of the sum of the vertex terms of the intersection
+ * polyhedron (vertices of either inside the other, then edge-face crossings, edge pairs in
+ * the order (0,1), (0,2), (0,3), (1,2), (1,3), (2,3), A's edge before B's), zero for three
+ * points or fewer, and replaced by 8 d^3 where it is negative or above that bound (d: the
+ * distance from the points' mean to the last point listed, as the reference's loop leaves
+ * it); cm, if asked for, the mean of the points. */
+static double vx_intersection(const double *const *A, const double *const *B, int want_cm, double *cm) {
+    vx_points P;
+    P.n = 0;
+    double vol = 0.0;
+    static const int rest[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int side = 0; side < 2; side++) {
+        const double *const *X = side ? B : A, *const *Y = side ? A : B;
+        for (int v = 0; v < 4; v++)
+            if (!vx_listed(&P, X[v]) && vx_node_in_tet(X[v], Y[0], Y[1], Y[2], Y[3])) {
+                memcpy(P.w[P.n++], X[v], 3 * sizeof(double));
+                vol += vx_vertex_term(X[rest[v][0]], X[rest[v][1]], X[rest[v][2]], X[v]);
+            }
+    }
+    static const int edge[6][4] = {{0, 1, 2, 3}, {0, 2, 1, 3}, {0, 3, 1, 2}, {1, 2, 0, 3}, {1, 3, 0, 2}, {2, 3, 0, 1}};
+    for (int e = 0; e < 6; e++) {
+        const int *k = edge[e];
+        vx_edge_terms(&vol, &P, A[k[0]], A[k[1]], A[k[2]], A[k[3]], B);
+        vx_edge_terms(&vol, &P, B[k[0]], B[k[1]], B[k[2]], B[k[3]], A);
+    }
+    vol *= -0.166666666666666667;
+    if (P.n <= 3) return 0.;
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < P.n; i++) vx_add(c, P.w[i], c);
+    const double s = 1. / P.n;
+    for (int i = 0; i < 3; i++) c[i] = s * c[i];
+    double d0 = 0.0, di = 0.0;
+    for (int i = 0; i < P.n; i++) {
+        di = vx_distance(c, P.w[i]);
+        if (di > d0) d0 = di;
+    }
+    const double w = 8.0 * di * di * di;
+    if (fabs(vol) > w || vol < 0) vol = w;
+    if (want_cm) {
+        for (int i = 0; i < 3; i++) cm[i] = 0.0;
+        for (int i = 0; i < P.n; i++) vx_add(cm, P.w[i], cm);
+        for (int i = 0; i < 3; i++) cm[i] = s * cm[i];
+    }
+    return vol;
+}
+
+/* getTetrahedraVolume (src/mat_vec_fns_II.cpp:708-714): six times the signed volume. */
+static double vx_tet6(const double *p0, const double *p1, const double *p2, const double *p3) {
+    return (p1[0] - p0[0]) * ((p1[1] - p2[1]) * (p2[2] - p3[2]) - (p2[1] - p3[1]) * (p1[2] - p2[2])) +
+           (p2[0] - p1[0]) * ((p2[1] - p3[1]) * (p0[2] - p1[2]) - (p0[1] - p1[1]) * (p2[2] - p3[2])) +
+           (p3[0] - p2[0]) * ((p0[1] - p1[1]) * (p1[2] - p2[2]) - (p1[1] - p2[1]) * (p0[2] - p1[2]));
+}
+
+/* getLocalCoordinatesForLinTet (:725-733): the barycentric coordinates of p in t0..t3. */
+static void vx_local_coords(const double *const *t, const double *p, double *phi) {
+    phi[0] = vx_tet6(p, t[1], t[2], t[3]);
+    phi[1] = -vx_tet6(p, t[0], t[2], t[3]);
+    phi[2] = vx_tet6(p, t[0], t[1], t[3]);
+    phi[3] = -vx_tet6(p, t[0], t[1], t[2]);
+    const double v = vx_tet6(t[0], t[1], t[2], t[3]);
+    for (int i = 0; i < 4; i++) phi[i] /= v;
+}
+
+/* tet_a_tetII (src/CheckTetrahedraOverlap.cpp:208-337): the separating-axis overlap test of
+ * Ganovelli, Ponchio and Rocchini (JGT 2002) -- A's four face planes (with the edge planes at
+ * each pair of them), then, unless some vertex of B lies inside all of A's faces, B's four. */
+static int vx_side_mask(const double *c) {
+    int m = 0;
+    for (int k = 0; k < 4; k++)
+        if (c[k] > 0.0) m |= 1 << k;
+    return m;
+}
+static int vx_edge_separates(const double *c0, const double *c1, int m0, int m1) {
+    if ((m0 | m1) != 15) return 0;
+    m0 &= (m0 ^ m1);
+    m1 &= (m0 ^ m1);
+    static const int pr[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    for (int q = 0; q < 6; q++) {
+        const int i = pr[q][0], j = pr[q][1];
+        const double det = c0[j] * c1[i] - c0[i] * c1[j];
+        if ((m0 & (1 << i)) && (m1 & (1 << j)) && det > 0.0) return 0;
+        if ((m0 & (1 << j)) && (m1 & (1 << i)) && det < 0.0) return 0;
+    }
+    return 1;
+}
+static int vx_tets_overlap(const double *const *A, const double *const *B) {
+    double PA[4][3], e[5][3], n[3], C[4][4];
+    int mask[4];
+    for (int k = 0; k < 4; k++) vx_sub(B[k], A[0], PA[k]);
+    vx_sub(A[1], A[0], e[0]);
+    vx_sub(A[2], A[0], e[1]);
+    vx_sub(A[3], A[0], e[2]);
+    /* A's faces 0..2 through A0, then face 3 through A1 */
+    for (int f = 0; f < 4; f++) {
+        if (f == 0) vx_cross(e[0], e[1], n);
+        else if (f == 1) vx_cross(e[2], e[0], n);
+        else if (f == 2) vx_cross(e[1], e[2], n);
+        else {
+            vx_sub(A[3], A[1], e[4]);
+            vx_sub(A[2], A[1], e[3]);
+            vx_cross(e[4], e[3], n);
+        }
+        for (int k = 0; k < 4; k++) {
+            if (f < 3)
+                C[f][k] = PA[k][0] * n[0] + PA[k][1] * n[1] + PA[k][2] * n[2];
+            else
+                C[f][k] = (B[k][0] - A[1][0]) * n[0] + (B[k][1] - A[1][1]) * n[1] + (B[k][2] - A[1][2]) * n[2];
+        }
+        mask[f] = vx_side_mask(C[f]);
+        if (mask[f] == 15) return 0;
+        for (int g = 0; g < f; g++)
+            if (vx_edge_separates(C[g], C[f], mask[g], mask[f])) return 0;
+    }
+    if ((mask[0] | mask[1] | mask[2] | mask[3]) != 15) return 1;
+    double PB[4][3], d[5][3];
+    for (int k = 0; k < 4; k++) vx_sub(A[k], B[0], PB[k]);
+    vx_sub(B[1], B[0], d[0]);
+    vx_sub(B[2], B[0], d[1]);
+    for (int f = 0; f < 4; f++) {
+        if (f == 0) vx_cross(d[0], d[1], n);
+        else if (f == 1) {
+            vx_sub(B[3], B[0], d[2]);
+            vx_cross(d[2], d[0], n);
+        } else if (f == 2) vx_cross(d[1], d[2], n);
+        else {
+            vx_sub(B[3], B[1], d[4]);
+            vx_sub(B[2], B[1], d[3]);
+            vx_cross(d[4], d[3], n);
+        }
+        int all = 1;
+        for (int k = 0; k < 4 && all; k++) {
+            const double s = f < 3 ? PB[k][0] * n[0] + PB[k][1] * n[1] + PB[k][2] * n[2]
+                                   : (A[k][0] - B[1][0]) * n[0] + (A[k][1] - B[1][1]) * n[1] + (A[k][2] - B[1][2]) * n[2];
+            all = s > 0.0;
+        }
+        if (all) return 0;
+    }
+    return 1;
+}
+
+/* Face::getTetraIntersectionVolumeTotalGradientAndShapeFunctions (src/Face.cpp:473-514) and
+ * VdW_solver::do_steric_interaction (src/VdW_solver.cpp:458-487) for the tetrahedra A (face 1:
+ * its corners and opposite vertex) and B (face 2): the common volume V with the points' mean
+ * cm, the barycentric coordinates of cm in each, and the one-sided differences
+ *   dV/dr_d = (V(A, B + dr e_d) - V) / dr - (V(B, A + dr e_d) - V) / dr;
+ * then V and the gradient scaled by the steric factor, V the pair's energy, phi1_j dV/dr the
+ * force on A's node j and -phi2_j dV/dr on B's.  Returns 0 (nothing applied) where V is 0. */
+static int steric_forces(const double *const *A, const double *const *B, double dr, double factor, double *energy,
+                         double *fA, double *fB) {
+    double cm[3], phi1[4], phi2[4], g[3];
+    double vol = vx_intersection(A, B, 1, cm);
+    if (vol == 0) return 0;
+    vx_local_coords(A, cm, phi1);
+    vx_local_coords(B, cm, phi2);
+    for (int dir = 0; dir < 3; dir++) {
+        double dx[3] = {0, 0, 0}, sA[4][3], sB[4][3];
+        dx[dir] = dr;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 3; j++) {
+                sA[i][j] = A[i][j] + dx[j];
+                sB[i][j] = B[i][j] + dx[j];
+            }
+        const double *SA[4] = {sA[0], sA[1], sA[2], sA[3]}, *SB[4] = {sB[0], sB[1], sB[2], sB[3]};
+        double vm = vx_intersection(A, SB, 0, NULL);
+        g[dir] = (vm - vol) / dr;
+        vm = vx_intersection(B, SA, 0, NULL);
+        g[dir] -= (vm - vol) / dr;
+    }
+    vol *= factor;
+    for (int i = 0; i < 3; i++) g[i] = factor * g[i];
+    *energy = vol;
+    for (int j = 0; j < 4; j++)
+        for (int x = 0; x < 3; x++) {
+            fA[3 * j + x] = phi1[j] * g[x];
+            fB[3 * j + x] = (-1.0 * phi2[j]) * g[x];
+        }
+    return 1;
+}
+
+/* Face 1 of blob b1 against face 2 of blob b2 (steric, ljsteric and gensoft): the overlap test
+ * of Face::checkTetraIntersection (src/Face.cpp:399-404), then the steric forces added to both
+ * faces' Face::force[0..3]; returns 1 where they applied. */
+static int steric_pair(orc_blob *b1, int f1, orc_blob *b2, int f2, double *energy) {
+    const double *A[4], *B[4];
+    for (int k = 0; k < 3; k++) {
+        A[k] = &b1->pos[3 * b1->face_nodes[3 * f1 + k]];
+        B[k] = &b2->pos[3 * b2->face_nodes[3 * f2 + k]];
+    }
+    if (b1->ssint_opp[f1] < 0 || b2->ssint_opp[f2] < 0) { /* Face::n[3] is always set in the reference */
+        g_st_throw = 2;
+        return 0;
+    }
+    A[3] = &b1->pos[3 * b1->ssint_opp[f1]];
+    B[3] = &b2->pos[3 * b2->ssint_opp[f2]];
+    if (!vx_tets_overlap(A, B)) return 0;
+    double fA[12], fB[12];
+    if (!steric_forces(A, B, g_st_dr, g_st_factor, energy, fA, fB)) return 0;
+    for (int k = 0; k < 12; k++) {
+        b1->face_force[12 * f1 + k] += fA[k];
+        b2->face_force[12 * f2 + k] += fB[k];
+    }
+    return 1;
+}
+
+/* Test hooks: tetrahedra as [4][3] vertex arrays. */
+static void tet_ptrs(const double *t, const double **p) {
+    for (int k = 0; k < 4; k++) p[k] = t + 3 * k;
+}
+int orc_tets_overlap(const double *a, const double *b) {
+    const double *A[4], *B[4];
+    tet_ptrs(a, A);
+    tet_ptrs(b, B);
+    return vx_tets_overlap(A, B);
+}
+double orc_tet_intersection(const double *a, const double *b, int want_cm, double *cm) {
+    const double *A[4], *B[4];
+    tet_ptrs(a, A);
+    tet_ptrs(b, B);
+    return vx_intersection(A, B, want_cm, cm);
+}
+void orc_tet_local_coords(const double *t, const double *p, double *phi) {
+    const double *T[4];
+    tet_ptrs(t, T);
+    vx_local_coords(T, p, phi);
+}
+/* out: [0] the energy, [1..12] A's node forces, [13..24] B's; returns whether they apply. */
+int orc_steric_forces(const double *a, const double *b, double dr, double factor, double *out) {
+    const double *A[4], *B[4];
+    tet_ptrs(a, A);
+    tet_ptrs(b, B);
+    g_st_throw = 0;
+    const int r = steric_forces(A, B, dr, factor, &out[0], &out[1], &out[13]);
+    return g_st_throw ? -1 : r;
 }
 
 /* consider_interaction's further tests for the types other than lennard-jones
@@ -1689,10 +2168,10 @@ static int ssint_steric_checks(const orc_blob *b1, int f1, const orc_blob *b2, i
     return 1;
 }
 
-/* VdW_solver::solve (src/VdW_solver.cpp:158-210) with consider_interaction (:332-384), Lennard-
- * Jones type: each pool entry i against the stacks of its cell and the 26 around it (top
- * down), pairs with i < j, of different blobs unless inc_self_ssint, whose normals do not face
- * the same way (dot product not above zero). */
+/* VdW_solver::solve (src/VdW_solver.cpp:158-210) with consider_interaction (:332-384): each
+ * pool entry i against the stacks of its cell and the 26 around it (top down), pairs with
+ * i < j, of different blobs unless inc_self_ssint, whose normals do not face the same way (dot
+ * product not above zero), and for the types other than lennard-jones the further tests. */
 static void ssint_solve(orc_blob **blobs, int nblobs) {
     free(g_ss_energy);
     g_ss_energy = xcalloc((size_t)nblobs * nblobs, sizeof(double));
@@ -1711,9 +2190,17 @@ static void ssint_solve(orc_blob **blobs, int nblobs) {
                 const double *nj = &blobs[bj]->face_geom[7 * fj + 3];
                 if (ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2] > 0.0) continue;
                 if (g_ss_kind != 1 && !ssint_steric_checks(blobs[bi], fi, blobs[bj], fj, bi == bj)) continue;
+                /* Steric_solver, LJSteric_solver and GenSoftSSINT_solver::do_interaction
+                 * (src/Steric_solver.cpp:28-38, src/LJSteric_solver.cpp:34-52,
+                 * src/GenSoftSSINT_solver.cpp:29-47): the steric forces where the face elements
+                 * overlap with a nonzero volume, otherwise the pair potential (none for steric) */
                 double e;
-                lj_pair(blobs[bi], fi, blobs[bj], fj, &e);
-                g_ss_energy[bi * nblobs + bj] += e;
+                if (g_ss_kind != 1 && steric_pair(blobs[bi], fi, blobs[bj], fj, &e))
+                    g_ss_energy[bi * nblobs + bj] += e;
+                else if (g_ss_kind != 4) {
+                    lj_pair(blobs[bi], fi, blobs[bj], fj, &e);
+                    g_ss_energy[bi * nblobs + bj] += e;
+                }
             }
         }
     }
@@ -1742,7 +2229,13 @@ int orc_world_step(orc_blob **blobs, int nblobs, orc_stream *g) {
         }
     }
     world_apply_springs(blobs); /* src/World.cpp:1933 */
-    if (g_ss_ntypes > 0) ssint_solve(blobs, nblobs); /* src/World.cpp:2002-2004 */
+    if (g_ss_ntypes > 0) { /* src/World.cpp:2002-2004 */
+        g_st_throw = 0;
+        ssint_solve(blobs, nblobs);
+        if (g_st_throw) return g_st_throw == 2 ? -4 : -3; /* -3: a zero vector normalised, where the
+                                                          * reference throws; -4: a face without its
+                                                          * opposite vertex */
+    }
     for (int i = 0; i < nblobs; i++) inv += orc_blob_update_internal_forces(blobs[i], g);
     if (inv > 0) return inv;
     for (int i = 0; i < nblobs; i++)
@@ -1809,7 +2302,7 @@ void orc_blob_get(const orc_blob *b, const char *what, double *out) {
     else if (!strcmp(what, "stokes_drag")) memcpy(out, b->stokes_drag, sizeof(double) * N);
     else if (!strcmp(what, "last_det")) memcpy(out, b->last_det, sizeof(double) * E);
     else if (!strcmp(what, "face_force") && b->face_force)
-        memcpy(out, b->face_force, sizeof(double) * 9 * (size_t)b->n_faces);
+        memcpy(out, b->face_force, sizeof(double) * 12 * (size_t)b->n_faces);
     else if (!strcmp(what, "material")) {
         for (size_t e = 0; e < E; e++) {
             out[5 * e] = b->rho[e];
@@ -1835,4 +2328,31 @@ double orc_blob_strain_energy(const orc_blob *b) {
         se += b->vol_0[n] * (b->G[n] * (dc - 3) + 0.5 * C * (t * t - 1) - (C + 2 * b->G[n]) * log(t));
     }
     return 0.5 * se;
+}
+
+/* For the tests' intersection counts (the squidgy-spheres check of the reference's
+ * tests/physics/squidgy_steric/checkSquidgySpheres.cpp): the number of pairs of tetrahedra, one
+ * from each list ([n][4][3] vertex arrays), with a nonzero common volume.  Pairs whose bounding
+ * boxes are apart have none and are skipped. */
+long orc_count_tet_intersections(int na, const double *ta, int nb, const double *tb) {
+    long n = 0;
+    for (int i = 0; i < na; i++) {
+        const double *a = ta + 12 * (size_t)i;
+        double alo[3], ahi[3];
+        for (int k = 0; k < 3; k++) {
+            alo[k] = fmin(fmin(a[k], a[3 + k]), fmin(a[6 + k], a[9 + k]));
+            ahi[k] = fmax(fmax(a[k], a[3 + k]), fmax(a[6 + k], a[9 + k]));
+        }
+        for (int j = 0; j < nb; j++) {
+            const double *b = tb + 12 * (size_t)j;
+            int apart = 0;
+            for (int k = 0; k < 3 && !apart; k++) {
+                const double lo = fmin(fmin(b[k], b[3 + k]), fmin(b[6 + k], b[9 + k]));
+                const double hi = fmax(fmax(b[k], b[3 + k]), fmax(b[6 + k], b[9 + k]));
+                apart = hi < alo[k] || lo > ahi[k];
+            }
+            if (!apart && orc_tet_intersection(a, b, 0, NULL) != 0) n++;
+        }
+    }
+    return n;
 }

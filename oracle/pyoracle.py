@@ -82,6 +82,41 @@ class _Lib:
     def fn(self, name):
         return getattr(self.lib, self.prefix + name)
 
+    # ---- steric geometry (tetrahedra as [4, 3] vertex arrays, meso units) ----
+    def _tet(self, t):
+        t = np.ascontiguousarray(t, np.float64)
+        assert t.shape == (4, 3)
+        return t
+
+    def tets_overlap(self, a, b) -> bool:
+        f = self.fn("tets_overlap")
+        f.argtypes, f.restype = [_dp, _dp], C.c_int
+        return bool(f(self._tet(a), self._tet(b)))
+
+    def tet_intersection(self, a, b, want_cm=False):
+        """(volume common to a and b, the intersection points' mean or None)."""
+        f = self.fn("tet_intersection")
+        f.argtypes, f.restype = [_dp, _dp, C.c_int, _dp], C.c_double
+        cm = np.zeros(3)
+        v = f(self._tet(a), self._tet(b), int(want_cm), cm)
+        return v, (cm if want_cm else None)
+
+    def tet_local_coords(self, t, p) -> np.ndarray:
+        f = self.fn("tet_local_coords")
+        f.argtypes = [_dp, _dp, _dp]
+        out = np.zeros(4)
+        f(self._tet(t), np.ascontiguousarray(p, np.float64), out)
+        return out
+
+    def steric_forces(self, a, b, dr=5e-3, factor=1.0):
+        """(status 1 applied / 0 zero volume / -1 degenerate, energy, forces on a's nodes [4, 3],
+        forces on b's [4, 3])."""
+        f = self.fn("steric_forces")
+        f.argtypes, f.restype = [_dp, _dp, C.c_double, C.c_double, _dp], C.c_int
+        out = np.zeros(25)
+        rc = f(self._tet(a), self._tet(b), float(dr), float(factor), out)
+        return rc, out[0], out[1:13].reshape(4, 3), out[13:25].reshape(4, 3)
+
     # ---- RNG ---------------------------------------------------------------
     def package_seed(self, rng_seed: int) -> np.ndarray:
         out = np.zeros(6, np.uint64)
@@ -183,7 +218,7 @@ class Blob:
                  "dpsi": 12 * self.E, "vol": self.E, "vol_0": self.E, "J_inv_0": 9 * self.E,
                  "F_ij": 9 * self.E, "visc": 144 * self.E, "node_force": 30 * self.E,
                  "stress_mag": self.E, "stokes_drag": self.N, "material": 5 * self.E,
-                 "last_det": self.E, "face_force": 9 * getattr(self, "n_faces", 0)}
+                 "last_det": self.E, "face_force": 12 * getattr(self, "n_faces", 0)}
         out = np.zeros(sizes[what], np.float64)
         self.lib.fn("blob_get")(self.h, what.encode(), out)
         return out
@@ -288,13 +323,17 @@ class Oracle(_Lib):
     def stream(self) -> "OracleStream":
         return OracleStream(self)
 
-    SSINT_KINDS = {"lennard-jones": 1, "ljsteric": 2, "gensoft": 3}
+    SSINT_KINDS = {"lennard-jones": 1, "ljsteric": 2, "gensoft": 3, "steric": 4}
 
     def set_ssint(self, emin=None, rmin=None, grid=(1, 1, 1), h=1.0, inc_self=1, es_update=1, kind="lennard-jones",
-                  k0=None):
+                  k0=None, steric_factor=1.0, steric_dr=5e-3):
         """Surface-surface interactions for the next world_step calls: emin, rmin (and k0 for
         gensoft) [ntypes, ntypes] meso, the lookup grid's cell counts and cell size (meso),
-        inc_self_ssint, es_update and the type.  emin None switches them off."""
+        inc_self_ssint, es_update and the type; for the types other than lennard-jones the
+        steric factor and difference step (meso, as SimulationParams holds them).  emin None
+        switches them off."""
+        self.lib.orc_world_set_steric.argtypes = [C.c_double, C.c_double]
+        self.lib.orc_world_set_steric(float(steric_factor), float(steric_dr))
         if emin is None:
             self.lib.orc_world_set_ssint(1, 0, np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(3, np.int32), 1.0, 0, 1)
             return
@@ -303,6 +342,13 @@ class Oracle(_Lib):
         self.lib.orc_world_set_ssint(self.SSINT_KINDS[kind], e.shape[0], e.ravel(),
                                      np.ascontiguousarray(rmin, np.float64).ravel(), k.ravel(),
                                      np.ascontiguousarray(grid, np.int32), float(h), int(inc_self), int(es_update))
+
+    def count_tet_intersections(self, ta, tb) -> int:
+        """Pairs (one tetrahedron of ta [n, 4, 3], one of tb) with a nonzero common volume."""
+        f = self.lib.orc_count_tet_intersections
+        f.argtypes, f.restype = [C.c_int, _dp, C.c_int, _dp], C.c_long
+        ta, tb = (np.ascontiguousarray(t, np.float64).reshape(-1, 4, 3) for t in (ta, tb))
+        return int(f(ta.shape[0], ta.ravel(), tb.shape[0], tb.ravel()))
 
     def ssint_factors(self, kind, r, Emin, Rmin, k0=0.0):
         """(force magnitude, energy) of one Gauss-point pair at distance r (meso)."""

@@ -2,7 +2,7 @@
  * ref_harness.cpp -- drives the reference's OWN compiled hot-path objects
  * (tetra_element_linear, NoMassCGSolver, ConjugateGradientSolver, MassLumpedSolver,
  * SparsityPattern, SparseMatrixFixedPattern, SparseMatrixTypes, mat_vec_fns, mat_vec_fns_II,
- * RngStream, mesh_node) from /root/reference/src.
+ * RngStream, mesh_node, VolumeIntersection, CheckTetrahedraOverlap) from /root/reference/src.
  *
  * TEST INFRASTRUCTURE ONLY (the checker of the checker).  Built by oracle/Makefile
  * into oracle/_ref/libffea_ref.so; nothing in the product links it.
@@ -25,7 +25,9 @@
 #include <omp.h>
 #endif
 
+#include "CheckTetrahedraOverlap.h"
 #include "RngStream.h"
+#include "VolumeIntersection.h"
 #include "mat_vec_fns.h"
 #include "mat_vec_fns_II.h"
 #include "mesh_node.h"
@@ -564,6 +566,86 @@ void ref_blob_get(ref_blob *b, const char *what, double *out) {
             out[5 * e + 4] = b->elem[e].E;
         }
     }
+}
+
+/* ---- steric geometry: the reference's tet_a_tetII (src/CheckTetrahedraOverlap.cpp:208-337),
+ * volumeIntersectionII (src/VolumeIntersection.cpp:951-1024) and getLocalCoordinatesForLinTet
+ * (src/mat_vec_fns_II.cpp:725-733).  Face.cpp needs Blob (Eigen), so the gradient loop of
+ * Face::getTetraIntersectionVolumeTotalGradientAndShapeFunctions (src/Face.cpp:473-514) and
+ * the scaling and spreading of VdW_solver::do_steric_interaction (src/VdW_solver.cpp:458-487)
+ * are restated over those three.  Tetrahedra come as [4][3] vertex arrays. */
+static void ref_tet(const double *t, grr3 *T) {
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 3; j++) T[i][j] = t[3 * i + j];
+}
+
+int ref_tets_overlap(const double *a, const double *b) {
+    arr3 A[4], B[4];
+    ref_tet(a, A);
+    ref_tet(b, B);
+    return tet_a_tetII(A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3]) ? 1 : 0;
+}
+
+double ref_tet_intersection(const double *a, const double *b, int want_cm, double *cm) {
+    grr3 A[4], B[4], c = {0, 0, 0};
+    ref_tet(a, A);
+    ref_tet(b, B);
+    const double v = volumeIntersectionII<geoscalar, grr3>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3], want_cm != 0, c);
+    if (want_cm)
+        for (int k = 0; k < 3; k++) cm[k] = c[k];
+    return v;
+}
+
+void ref_tet_local_coords(const double *t, const double *p, double *phi) {
+    grr3 T[4], P = {p[0], p[1], p[2]};
+    grr4 f;
+    ref_tet(t, T);
+    getLocalCoordinatesForLinTet<geoscalar, grr3, grr4>(T[0], T[1], T[2], T[3], P, f);
+    for (int k = 0; k < 4; k++) phi[k] = f[k];
+}
+
+/* out: [0] the energy (steric_factor V), [1..12] the forces on A's nodes, [13..24] on B's;
+ * returns 1 where they apply, 0 where V is 0, -1 where the reference throws. */
+int ref_steric_forces(const double *a, const double *b, double dr, double factor, double *out) {
+    grr3 A[4], B[4], cm, dVdr, tA[4], tB[4];
+    grr4 phi1, phi2;
+    ref_tet(a, A);
+    ref_tet(b, B);
+    try {
+        geoscalar vol = volumeIntersectionII<geoscalar, grr3>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3], true, cm);
+        if (vol == 0) return 0;
+        getLocalCoordinatesForLinTet<geoscalar, grr3, grr4>(A[0], A[1], A[2], A[3], cm, phi1);
+        getLocalCoordinatesForLinTet<geoscalar, grr3, grr4>(B[0], B[1], B[2], B[3], cm, phi2);
+        for (int dir = 0; dir < 3; dir++) {
+            grr3 dx;
+            arr3Initialise<grr3>(dx);
+            dx[dir] = dr;
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 3; j++) {
+                    tA[i][j] = A[i][j] + dx[j];
+                    tB[i][j] = B[i][j] + dx[j];
+                }
+            geoscalar vm = volumeIntersectionII<geoscalar, grr3>(A[0], A[1], A[2], A[3], tB[0], tB[1], tB[2], tB[3], false, cm);
+            dVdr[dir] = (vm - vol) / dr;
+            vm = volumeIntersectionII<geoscalar, grr3>(B[0], B[1], B[2], B[3], tA[0], tA[1], tA[2], tA[3], false, cm);
+            dVdr[dir] -= (vm - vol) / dr;
+        }
+        vol *= factor;
+        arr3Resize<geoscalar, grr3>(factor, dVdr);
+        out[0] = vol;
+        for (int j = 0; j < 4; j++) {
+            grr3 f1, f2;
+            arr3Resize2<geoscalar, grr3>(phi1[j], dVdr, f1);
+            arr3Resize2<geoscalar, grr3>(ffea_const::mOne * phi2[j], dVdr, f2);
+            for (int k = 0; k < 3; k++) {
+                out[1 + 3 * j + k] = f1[k];
+                out[13 + 3 * j + k] = f2[k];
+            }
+        }
+    } catch (int) {
+        return -1;
+    }
+    return 1;
 }
 
 }  // extern "C"

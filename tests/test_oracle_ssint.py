@@ -38,9 +38,10 @@ def cubes_world(oracle):
     for b, m, sc, p in zip(s["blobs"], meshes, scales, pos):
         o = oracle.blob(m, prm, sc)
         o.set_positions(p.ravel())
-        faces = F.read_surface(b["surface"])[:, 1:4]
+        surf = F.read_surface(b["surface"])
+        faces = surf[:, 1:4]
         o.set_surface_forces(faces, [], np.zeros((0, 3)))
-        o.set_ssint_types(F.read_ssint_types(b["vdw"], faces.shape[0]))
+        o.set_ssint_types(F.read_ssint_types(b["vdw"], faces.shape[0]), F.surface_opposite(surf, m.topo))
         blobs.append(o)
     lj = os.path.join(os.path.dirname(CUBES), s["params"]["vdw_forcefield_params"])
     E, R, _ = F.read_ssint_params(lj, "lennard-jones")
@@ -219,18 +220,21 @@ def test_face_pair_force_spread_moments(oracle, ssint_off, kind):
     a = np.flatnonzero(tops[0] >= 0)[17]
     cand = np.flatnonzero(tops[1] >= 0)
     b = cand[np.argmin(np.linalg.norm(cen[1][cand] - cen[0][a], axis=1))]
-    for o, t, keep in ((blobs[0], tops[0], a), (blobs[1], tops[1], b)):
+    opps = [F.surface_opposite(F.read_surface(x["surface"]), F.load_blob(x["nodes"], x["topology"], x["material"],
+                                                                      x["stokes"]).topo) for x in s["blobs"]]
+    for o, t, keep, op in ((blobs[0], tops[0], a, opps[0]), (blobs[1], tops[1], b, opps[1])):
         one = np.full_like(t, -1)
         one[keep] = t[keep]
-        o.set_ssint_types(one)
+        o.set_ssint_types(one, op)
     k0 = None
     if kind == "gensoft":  # Rmin is the cutoff
         R, k0 = np.full_like(R, h), np.full_like(E, 0.02)
     oracle.set_ssint(E, R, (10, 10, 10), h, inc_self=0, es_update=1, kind=kind, k0=k0)
     oracle.set_package_seed(np.arange(1, 7, dtype=np.uint64))
     assert pyoracle.world_step(oracle, blobs, oracle.stream()) == 0
-    f1 = blobs[0].get("face_force").reshape(-1, 3, 3)[a]
-    f2 = blobs[1].get("face_force").reshape(-1, 3, 3)[b]
+    f1, f2 = (o.get("face_force").reshape(-1, 4, 3)[k] for o, k in ((blobs[0], a), (blobs[1], b)))
+    assert not f1[3].any() and not f2[3].any()  # the opposite vertices: steric forces only
+    f1, f2 = f1[:3], f2[:3]
     x1, x2 = pos[0][faces[0][a]], pos[1][faces[1][b]]
     eta = np.full((3, 3), 1 / 6) + np.eye(3) / 2
     p, q = eta @ x1, eta @ x2
