@@ -48,14 +48,10 @@ __device__ __forceinline__ R visc_entry(const R *dpsi, R A, R B, R vol, int Rr, 
     return v * vol;
 }
 
-// calculate_jacobian (src/tetra_element_linear.cpp:193-205), from FP64 positions, and
-// calc_shape_function_derivatives_and_volume (:223-293): J, det and the shape gradients
-// dpsi[a + 4 dim] (x-major).  Shared by the element kernel and the assembly, which recomputes
-// them per contribution: the same code rounds the same way, so the matrix is unchanged.
+// The element geometry from its four vertex positions: see element_geometry.
 template <typename R>
-__device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos, const int4 n, R J[3][3],
-                                                 R dpsi[12], R &det) {
-    const double4 p0 = pos[n.x], p1 = pos[n.y], p2 = pos[n.z], p3 = pos[n.w];
+__device__ __forceinline__ void element_geometry_p(const double4 p0, const double4 p1, const double4 p2, const double4 p3,
+                                                   R J[3][3], R dpsi[12], R &det) {
     const double4 pr[3] = {p1, p2, p3};
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -86,6 +82,16 @@ __device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos
     dpsi[0] = -(dpsi[1] + dpsi[2] + dpsi[3]);
     dpsi[4] = -(dpsi[5] + dpsi[6] + dpsi[7]);
     dpsi[8] = -(dpsi[9] + dpsi[10] + dpsi[11]);
+}
+
+// calculate_jacobian (src/tetra_element_linear.cpp:193-205), from FP64 positions, and
+// calc_shape_function_derivatives_and_volume (:223-293): J, det and the shape gradients
+// dpsi[a + 4 dim] (x-major).  Shared by the element kernel and the assembly, which recomputes
+// them for every slice that needs the element: the same code rounds the same way.
+template <typename R>
+__device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos, const int4 n, R J[3][3],
+                                                 R dpsi[12], R &det) {
+    element_geometry_p<R>(pos[n.x], pos[n.y], pos[n.z], pos[n.w], J, dpsi, det);
 }
 
 template <typename R, typename T, bool kMass>
@@ -208,12 +214,14 @@ struct AsmArgs {
     const int4 *enodes;
     const double *A, *B;
     const int64_t *eoff;      // [slices + 1] into elem
-    const int32_t *elem;      // per slice, the distinct elements its slots sum over, ascending
-    const int32_t *width;     // [slices] tiles per slice
-    const int64_t *slice_off; // [slices + 1] first slot; slot = slice_off[s] + 64 t + lane
-    const int64_t *toff;      // [tiles] first code of each 64-slot tile
+    const int32_t *elem;      // per slice, the distinct elements its rows touch, ascending
+    const int32_t *woff;      // [slices + 1] work tiles of each slice (the first: its diagonal tile)
+    const int32_t *wtile;     // [work tiles] SELL tile position of an upper tile, -1 for the diagonal tile
+    const int64_t *wcoff;     // [work tiles + 1] first code of each work tile, 128-byte aligned
     const uint16_t *code;     // slice-local element << 4 | a << 2 | b, tile-major [k][64]
-    const uint8_t *cnt;       // [slots] contributions
+    const uint8_t *cnt;       // [work tiles * 64] contributions
+    const int32_t *aux;       // [work tiles * 64] diagonal SELL position / mirror slot, -1: none
+    const int64_t *slice_off; // [slices + 1] first slot; slot = slice_off[s] + 64 t + lane
     const uint8_t *flags;     // [slots] ffh::kSlotDiag / kSlotIdentity / kSlotPad
     const double *zeta;
     void *val;  // T tiles
@@ -227,25 +235,28 @@ struct AsmArgs {
 // calc_inverse_diagonal (:325-333).
 //
 // Element block (a, b) (row vertex a, column vertex b), entry (i, j) = V[a+4i][b+4j] with
-// g_a = grad psi_a, g_b = grad psi_b (dpsi[4i+a] = g_a[i]); these are visc_entry's expressions
-// with every product written in an order that rounds identically:
-//   i == j : ((g_a[i] g_b[i]) (A+B) + (g_a.g_b) A) vol
-//   i != j : a == b ? ((B+A)(g_a[i] g_a[j])) vol : (B (g_a[i] g_b[j]) + A (g_b[i] g_a[j])) vol
-// Block (b, a) of the same element at entry (j, i) multiplies the same pairs of factors (IEEE
-// products commute), so every lower block equals the transposed upper one bit for bit and each
-// slot is computed where it is stored.
+// g_a = grad psi_a, g_b = grad psi_b (dpsi[4i+a] = g_a[i]) and P_ij = g_a[i] g_b[j]; these are
+// visc_entry's expressions, every product written in an order that rounds identically:
+//   i == j : (P_ii (A+B) + ((P_00 + P_11) + P_22) A) vol
+//   i != j : a == b ? ((B+A) P_ij) vol : (B P_ij + A P_ji) vol
+// a == b exactly on the diagonal blocks.  Block (b, a) of the same element at entry (j, i)
+// multiplies the same pairs of factors (IEEE products and sums of two commute), so a lower block
+// (column < row), summed over the same elements in the same order, is its upper mirror's
+// transpose bit for bit: only the diagonal and the upper blocks are summed, and each upper block
+// is stored twice.
 //
 // One workgroup per slice.  The slice's distinct elements are evaluated once each, kThreads at
 // a time, into LDS (shape gradients, volume, A, B: element_geometry, the element kernel's own
-// code, so they round as there); then the thread of every slot adds its contributions of that
-// chunk in order.  Wave w owns the slice's tiles w, w + kWaves, ... (kTpt of them per pass).
-// kLdsCodes: the slice's contribution codes (at most kAsmCodeLds) are first copied to LDS with
-// 16-byte loads, and each thread finds where its list leaves the chunk by binary search, so the
-// summation loop has no data-dependent exit and its LDS reads pipeline.
+// code, so they round as there; element ids are loaded one chunk ahead and nodes are gathered
+// while the previous chunk's geometry is computed).  Wave 0 sums the 64 diagonal blocks (six
+// distinct entries each: a diagonal block is symmetric); waves 1.. sum the upper tiles, kTpt
+// each per pass.  kLdsCodes: the slice's contribution codes (at most kAsmCodeLds) are first
+// copied to LDS with 16-byte loads, and each thread finds where its list leaves the chunk by
+// binary search, so the summation loop has no data-dependent exit and its LDS reads pipeline.
 constexpr int kAsmCodeLds = 9216;
 
 template <typename T, int kThreads>
-__device__ __forceinline__ void asm_add(const T (&lds)[15][kThreads], uint32_t code, int le, T acc[9]) {
+__device__ __forceinline__ void asm_add_upper(const T (&lds)[15][kThreads], uint32_t code, int le, T acc[9]) {
     const int a = (code >> 2) & 3, b = code & 3;
     T ga[3], gb[3];
 #pragma unroll
@@ -254,127 +265,198 @@ __device__ __forceinline__ void asm_add(const T (&lds)[15][kThreads], uint32_t c
         gb[i] = lds[b + 4 * i][le];
     }
     const T vol = lds[12][le], A = lds[13][le], B = lds[14][le];
-    const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    T P[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int jj = 0; jj < 3; jj++) {
-            T v;
-            if (i == jj) {
-                v = ga[i] * gb[i] * (A + B);
-                v += del2 * A;
-            } else if (a == b) {
-                v = (B + A) * (ga[i] * gb[jj]);
-            } else {
-                v = B * (ga[i] * gb[jj]) + A * (gb[i] * ga[jj]);
-            }
-            acc[3 * i + jj] += v * vol;
+        for (int j = 0; j < 3; j++) P[i][j] = ga[i] * gb[j];
+    const T del2 = P[0][0] + P[1][1] + P[2][2];
+    const T AB = A + B, dA = del2 * A;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const T v = i == j ? P[i][i] * AB + dA : B * P[i][j] + A * P[j][i];
+            acc[3 * i + j] += v * vol;
         }
+}
+
+// Diagonal block (a == b): entries 00, 11, 22, 01, 02, 12 (the block is symmetric).
+template <typename T, int kThreads>
+__device__ __forceinline__ void asm_add_diag(const T (&lds)[15][kThreads], uint32_t code, int le, T d[6]) {
+    const int a = (code >> 2) & 3;
+    const T g0 = lds[a][le], g1 = lds[a + 4][le], g2 = lds[a + 8][le];
+    const T vol = lds[12][le], A = lds[13][le], B = lds[14][le];
+    const T P00 = g0 * g0, P11 = g1 * g1, P22 = g2 * g2;
+    const T del2 = P00 + P11 + P22;
+    const T AB = A + B, dA = del2 * A;
+    d[0] += (P00 * AB + dA) * vol;
+    d[1] += (P11 * AB + dA) * vol;
+    d[2] += (P22 * AB + dA) * vol;
+    d[3] += (AB * (g0 * g1)) * vol;
+    d[4] += (AB * (g0 * g2)) * vol;
+    d[5] += (AB * (g1 * g2)) * vol;
+}
+
+// Position after the last of a lane's codes [k, cnt) that lies before `lim` (codes ascend).
+__device__ __forceinline__ int asm_chunk_end(const uint16_t *lcode, int64_t co, int k, int cnt, uint32_t lim) {
+    int lo = k, hi = cnt;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((uint32_t)lcode[co + 64 * mid] < lim)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 template <typename T, int kWaves, int kTpt, bool kLdsCodes>
 __global__ __launch_bounds__(64 * kWaves) void k_assemble_slice(AsmArgs g) {
+    static_assert(kWaves >= 2, "wave 0 sums the diagonal blocks, the others the upper tiles");
     constexpr int kThreads = 64 * kWaves;
+    constexpr int kUp = (kWaves - 1) * kTpt;  // upper tiles per pass
     __shared__ T lds[15][kThreads];
     __shared__ __align__(16) uint16_t lcode[kLdsCodes ? kAsmCodeLds : 8];
     const int s = xcd_block(g.nslices);
     if (s >= g.nslices) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int w = g.width[s];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t e0 = g.eoff[s];
     const int ne = (int)(g.eoff[s + 1] - e0);
+    const int g0 = g.woff[s], nw = g.woff[s + 1] - g0;
     const int64_t slot0 = g.slice_off[s];
-    const int64_t cb = g.toff[slot0 >> 6];  // the slice's codes: [cb, toff[tile0 + w]), 128-byte aligned
+    const int64_t cb = g.wcoff[g0];  // the slice's codes: [cb, wcoff[g0 + nw]), 128-byte aligned
     if constexpr (kLdsCodes) {
-        const int n8 = (int)((g.toff[(slot0 >> 6) + w] - cb) >> 3);
+        const int n8 = (int)((g.wcoff[g0 + nw] - cb) >> 3);
         const uint4 *src = reinterpret_cast<const uint4 *>(g.code + cb);
         uint4 *dst = reinterpret_cast<uint4 *>(lcode);
-        for (int i = threadIdx.x; i < n8; i += kThreads) dst[i] = src[i];  // visible after the first barrier
+        for (int i = tid; i < n8; i += kThreads) dst[i] = src[i];  // visible after the first barrier
     }
-    for (int tg = 0; tg < w; tg += kWaves * kTpt) {  // one pass unless the slice is very wide
-        T acc[kTpt][9];
-        int k[kTpt], cnt[kTpt];
+    auto code_at = [&](int64_t i) -> uint32_t {
+        if constexpr (kLdsCodes)
+            return lcode[i];
+        else
+            return g.code[i];
+    };
+    T *val = reinterpret_cast<T *>(g.val);
+    const int npass = nw > 1 ? (nw - 1 + kUp - 1) / kUp : 1;
+    for (int pass = 0; pass < npass; pass++) {
+        // this wave's work tiles: wave 0 the diagonal tile (first pass), wave w > 0 the upper
+        // tiles 1 + pass kUp + (w - 1) + (kWaves - 1) j
+        int gj[kTpt], k[kTpt], cnt[kTpt];
         int64_t co[kTpt];  // this lane's first code: in LDS relative to cb, else in global memory
+        T acc[kTpt][9];
 #pragma unroll
         for (int j = 0; j < kTpt; j++) {
-            const int t = tg + wave + kWaves * j;
-            const int64_t slot = slot0 + (int64_t)t * 64 + lane;
-            cnt[j] = t < w ? (int)g.cnt[slot] : 0;
-            co[j] = (t < w ? g.toff[slot >> 6] - (kLdsCodes ? cb : 0) : 0) + lane;
+            const int r = wave == 0 ? (pass == 0 && j == 0 ? 0 : nw) : 1 + pass * kUp + (wave - 1) + (kWaves - 1) * j;
+            gj[j] = r < nw ? g0 + r : -1;
+            cnt[j] = gj[j] >= 0 ? (int)g.cnt[(int64_t)gj[j] * 64 + lane] : 0;
+            co[j] = (gj[j] >= 0 ? g.wcoff[gj[j]] - (kLdsCodes ? cb : 0) : 0) + lane;
             k[j] = 0;
 #pragma unroll
             for (int q = 0; q < 9; q++) acc[j][q] = (T)0.0;
         }
+        // Every lane loads an element each chunk (past the end, the slice's last one again:
+        // loaded, not used), so no load sits in a branch and none is waited for early.
+        auto eid = [&](int i) { return g.elem[e0 + min(i, ne - 1)]; };
+        int id_c = 0, id_n = 0;
+        int4 en_c = make_int4(0, 0, 0, 0);
+        if (ne > 0) {
+            id_c = eid(tid);
+            id_n = eid(kThreads + tid);
+            en_c = g.enodes[id_c];
+        }
         for (int c0 = 0; c0 < ne; c0 += kThreads) {
-            if (c0 + (int)threadIdx.x < ne) {
-                const int e = g.elem[e0 + c0 + threadIdx.x];
+            const double4 p0 = g.pos[en_c.x], p1 = g.pos[en_c.y], p2 = g.pos[en_c.z], p3 = g.pos[en_c.w];
+            const double Ae = g.A[id_c], Be = g.B[id_c];
+            // the next chunk's nodes and the one after's ids, in flight while this one is summed
+            const int4 en_n = g.enodes[id_n];
+            const int id_nn = eid(c0 + 2 * kThreads + tid);
+            if (c0 + tid < ne) {
                 T J[3][3], dpsi[12], det;
-                element_geometry<T>(g.pos, g.enodes[e], J, dpsi, det);
+                element_geometry_p<T>(p0, p1, p2, p3, J, dpsi, det);
 #pragma unroll
-                for (int q = 0; q < 12; q++) lds[q][threadIdx.x] = dpsi[q];
-                lds[12][threadIdx.x] = (T)(1.0 / 6.0) * fabs(det);
-                lds[13][threadIdx.x] = (T)g.A[e];
-                lds[14][threadIdx.x] = (T)g.B[e];
+                for (int q = 0; q < 12; q++) lds[q][tid] = dpsi[q];
+                lds[12][tid] = (T)(1.0 / 6.0) * fabs(det);
+                lds[13][tid] = (T)Ae;
+                lds[14][tid] = (T)Be;
             }
             __syncthreads();
+            const uint32_t lim = (uint32_t)(c0 + kThreads) << 4;
 #pragma unroll
             for (int j = 0; j < kTpt; j++) {
+                if (wave == 0 && j > 0) break;
+                int lo;
                 if constexpr (kLdsCodes) {
-                    // codes of this lane's list are ascending: the first one at or past the chunk end
-                    const uint32_t lim = (uint32_t)(c0 + kThreads) << 4;
-                    int lo = k[j], hi = cnt[j];
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if ((uint32_t)lcode[co[j] + 64 * mid] < lim)
-                            lo = mid + 1;
-                        else
-                            hi = mid;
-                    }
-#pragma unroll 2
-                    for (int kk = k[j]; kk < lo; kk++) {
-                        const uint32_t code = lcode[co[j] + 64 * kk];
-                        asm_add<T, kThreads>(lds, code, (int)(code >> 4) - c0, acc[j]);
-                    }
-                    k[j] = lo;
+                    lo = asm_chunk_end(lcode, co[j], k[j], cnt[j], lim);
                 } else {
-                    while (k[j] < cnt[j]) {
-                        const uint32_t code = g.code[co[j] + 64 * k[j]];
-                        const int le = (int)(code >> 4) - c0;
-                        if (le >= kThreads) break;  // from a later chunk
-                        asm_add<T, kThreads>(lds, code, le, acc[j]);
-                        k[j]++;
+                    lo = k[j];
+                    while (lo < cnt[j] && code_at(co[j] + 64 * lo) < lim) lo++;
+                }
+                // the next code is read while this contribution is summed
+                if (k[j] < lo) {
+                    uint32_t code = code_at(co[j] + 64 * k[j]);
+                    if (wave == 0) {
+                        for (int kk = k[j]; kk < lo; kk++) {
+                            const uint32_t nxt = code_at(co[j] + 64 * min(kk + 1, lo - 1));
+                            asm_add_diag<T, kThreads>(lds, code, (int)(code >> 4) - c0, acc[0]);
+                            code = nxt;
+                        }
+                    } else {
+                        for (int kk = k[j]; kk < lo; kk++) {
+                            const uint32_t nxt = code_at(co[j] + 64 * min(kk + 1, lo - 1));
+                            asm_add_upper<T, kThreads>(lds, code, (int)(code >> 4) - c0, acc[j]);
+                            code = nxt;
+                        }
                     }
                 }
+                k[j] = lo;
             }
             __syncthreads();
+            id_c = id_n;
+            id_n = id_nn;
+            en_c = en_n;
         }
-        T *val = reinterpret_cast<T *>(g.val);
+        if (wave == 0) {
+            if (gj[0] < 0) continue;
+            const int td = g.aux[(int64_t)gj[0] * 64 + lane];
+            if (td < 0) continue;  // a padding row past the last node
+            const int64_t tile = slot0 + (int64_t)td * 64;
+            const uint8_t fl = g.flags[tile + lane];
+            if (fl & ffh::kSlotPad) continue;
+            const int node = s * 64 + lane;
+            const T *d = acc[0];
+            T b[9] = {d[0], d[3], d[4], d[3], d[1], d[5], d[4], d[5], d[2]};
+            if (fl & ffh::kSlotIdentity) {
 #pragma unroll
-        for (int j = 0; j < kTpt; j++) {
-            const int t = tg + wave + kWaves * j;
-            if (t >= w) continue;
-            const int64_t slot = slot0 + (int64_t)t * 64 + lane;
-            const uint8_t fl = g.flags[slot];
-            if (fl & ffh::kSlotPad) continue;  // zero for good
-            if (fl & ffh::kSlotDiag) {
-                const int node = s * 64 + lane;
-                if (fl & ffh::kSlotIdentity) {
-#pragma unroll
-                    for (int q = 0; q < 9; q++) acc[j][q] = (T)0.0;
-                    acc[j][0] = acc[j][4] = acc[j][8] = (T)1.0;
-                } else {
-                    const T z = (T)g.zeta[node];
-                    acc[j][0] += z;
-                    acc[j][4] += z;
-                    acc[j][8] += z;
-                }
-                g.dinv[3 * (size_t)node + 0] = 1.0 / (double)acc[j][0];
-                g.dinv[3 * (size_t)node + 1] = 1.0 / (double)acc[j][4];
-                g.dinv[3 * (size_t)node + 2] = 1.0 / (double)acc[j][8];
+                for (int q = 0; q < 9; q++) b[q] = (T)0.0;
+                b[0] = b[4] = b[8] = (T)1.0;
+            } else {
+                const T z = (T)g.zeta[node];
+                b[0] += z;
+                b[4] += z;
+                b[8] += z;
             }
-            const int64_t tile = slot & ~(int64_t)63;
+            g.dinv[3 * (size_t)node + 0] = 1.0 / (double)b[0];
+            g.dinv[3 * (size_t)node + 1] = 1.0 / (double)b[4];
+            g.dinv[3 * (size_t)node + 2] = 1.0 / (double)b[8];
 #pragma unroll
-            for (int q = 0; q < 9; q++) val[tile * 9 + q * 64 + lane] = acc[j][q];
+            for (int q = 0; q < 9; q++) val[tile * 9 + q * 64 + lane] = b[q];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kTpt; j++) {
+                if (gj[j] < 0) continue;
+                const int m = g.aux[(int64_t)gj[j] * 64 + lane];
+                if (m < 0) continue;  // no upper block in this lane
+                const int64_t tile = slot0 + (int64_t)g.wtile[gj[j]] * 64;
+#pragma unroll
+                for (int q = 0; q < 9; q++) val[tile * 9 + q * 64 + lane] = acc[j][q];
+                const int64_t mt = (int64_t)(m & ~63), ml = m & 63;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int jj = 0; jj < 3; jj++) val[mt * 9 + (3 * jj + i) * 64 + ml] = acc[j][3 * i + jj];
+            }
         }
     }
 }

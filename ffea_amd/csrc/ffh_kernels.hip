@@ -112,11 +112,11 @@ struct ffh_blob {
     int64_t *slice_off = nullptr;
     uint8_t *pinned = nullptr;
     // slice assembly (k_assemble_slice): element lists, tiled contribution codes, slot tables
-    int64_t *sa_eoff = nullptr, *sa_toff = nullptr;
-    int32_t *sa_elem = nullptr;
+    int64_t *sa_eoff = nullptr, *sa_wcoff = nullptr;
+    int32_t *sa_elem = nullptr, *sa_woff = nullptr, *sa_wtile = nullptr, *sa_aux = nullptr;
     uint16_t *sa_code = nullptr;
     uint8_t *sa_cnt = nullptr, *slot_flags = nullptr;
-    int max_width = 0;      // widest slice, in tiles
+    int max_upper = 0;      // most upper work tiles of a slice
     bool codes_lds = true;  // every slice's assembly codes fit the kernel's LDS area
     double *zeta = nullptr, *dinv = nullptr;
     double *f = nullptr, *r = nullptr, *x = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *vel = nullptr;
@@ -449,11 +449,13 @@ struct ffh_blob {
         a.B = B;
         a.eoff = sa_eoff;
         a.elem = sa_elem;
-        a.width = slice_width;
-        a.slice_off = slice_off;
-        a.toff = sa_toff;
+        a.woff = sa_woff;
+        a.wtile = sa_wtile;
+        a.wcoff = sa_wcoff;
         a.code = sa_code;
         a.cnt = sa_cnt;
+        a.aux = sa_aux;
+        a.slice_off = slice_off;
         a.flags = slot_flags;
         a.zeta = zeta;
         a.val = val;
@@ -466,26 +468,33 @@ struct ffh_blob {
                 launch_assemble<float>(a, e0, e1);
         });
     }
-    // asm_block threads (4 or 8 waves) per slice, asm_tiles tiles per thread and pass; the
-    // slices' codes staged in LDS when the largest fits
+    // asm_block threads (4, 8 or 16 waves) per slice, asm_tiles upper tiles per wave and pass;
+    // the slices' codes staged in LDS when the largest fits
+    template <typename T, int kWaves, bool kLds>
+    void launch_assemble_w(const AsmArgs &a, hipEvent_t e0, hipEvent_t e1) {
+        const int gr = P_slice;
+        if (asm_tiles == 1)
+            hipExtLaunchKernelGGL((k_assemble_slice<T, kWaves, 1, kLds>), gr, 64 * kWaves, 0, stream, e0, e1, 0, a);
+        else if (asm_tiles == 2 || kWaves == 16)  // 16 waves: at most 2 (30 upper tiles per pass)
+            hipExtLaunchKernelGGL((k_assemble_slice<T, kWaves, 2, kLds>), gr, 64 * kWaves, 0, stream, e0, e1, 0, a);
+        else if constexpr (kWaves < 16)
+            hipExtLaunchKernelGGL((k_assemble_slice<T, kWaves, 4, kLds>), gr, 64 * kWaves, 0, stream, e0, e1, 0, a);
+    }
+    template <typename T, bool kLds>
+    void launch_assemble_l(const AsmArgs &a, hipEvent_t e0, hipEvent_t e1) {
+        if (asm_block == 1024)
+            launch_assemble_w<T, 16, kLds>(a, e0, e1);
+        else if (asm_block == 512)
+            launch_assemble_w<T, 8, kLds>(a, e0, e1);
+        else
+            launch_assemble_w<T, 4, kLds>(a, e0, e1);
+    }
     template <typename T>
     void launch_assemble(const AsmArgs &a, hipEvent_t e0, hipEvent_t e1) {
-        const int gr = P_slice;
-        const int sel = (asm_block == 512 ? 0 : 3) + (asm_tiles == 1 ? 0 : asm_tiles == 2 ? 1 : 2) + (codes_lds ? 0 : 6);
-        switch (sel) {
-        case 0: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 1, true>), gr, 512, 0, stream, e0, e1, 0, a); break;
-        case 1: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 2, true>), gr, 512, 0, stream, e0, e1, 0, a); break;
-        case 2: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 4, true>), gr, 512, 0, stream, e0, e1, 0, a); break;
-        case 3: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 1, true>), gr, 256, 0, stream, e0, e1, 0, a); break;
-        case 4: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 2, true>), gr, 256, 0, stream, e0, e1, 0, a); break;
-        case 5: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 4, true>), gr, 256, 0, stream, e0, e1, 0, a); break;
-        case 6: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 1, false>), gr, 512, 0, stream, e0, e1, 0, a); break;
-        case 7: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 2, false>), gr, 512, 0, stream, e0, e1, 0, a); break;
-        case 8: hipExtLaunchKernelGGL((k_assemble_slice<T, 8, 4, false>), gr, 512, 0, stream, e0, e1, 0, a); break;
-        case 9: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 1, false>), gr, 256, 0, stream, e0, e1, 0, a); break;
-        case 10: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 2, false>), gr, 256, 0, stream, e0, e1, 0, a); break;
-        default: hipExtLaunchKernelGGL((k_assemble_slice<T, 4, 4, false>), gr, 256, 0, stream, e0, e1, 0, a); break;
-        }
+        if (codes_lds)
+            launch_assemble_l<T, true>(a, e0, e1);
+        else
+            launch_assemble_l<T, false>(a, e0, e1);
     }
     RhsArgs rhs_args(bool noise, const double *f_in) const {
         RhsArgs g;
@@ -944,9 +953,10 @@ struct ffh_blob {
         best = 1e30f;
         int pick_u = asm_tiles;
         pick = asm_block;
-        for (int bs : {256, 512})
+        for (int bs : {256, 512, 1024})
             for (int un : {1, 2, 4}) {
-                if (un > 1 && (bs / 64) * (un / 2) >= max_width) continue;  // a smaller shape covers the slice
+                if (un > 1 && (bs / 64 - 1) * (un / 2) >= max_upper) continue;  // a smaller shape covers the slice
+                if (bs == 1024 && un == 4) continue;
                 asm_block = bs;
                 asm_tiles = un;
                 if ((rc = time_ms([&] { return run_assemble(); }, 5, ms))) return rc;
@@ -1260,14 +1270,17 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     }
     rc |= b->upload(b->sa_eoff, P.sa_eoff);
     rc |= b->upload(b->sa_elem, P.sa_elem);
-    rc |= b->upload(b->sa_toff, P.sa_toff);
+    rc |= b->upload(b->sa_woff, P.sa_woff);
+    rc |= b->upload(b->sa_wtile, P.sa_wtile);
+    rc |= b->upload(b->sa_wcoff, P.sa_wcoff);
     rc |= b->upload(b->sa_code, P.sa_code);
     rc |= b->upload(b->sa_cnt, P.sa_cnt);
+    rc |= b->upload(b->sa_aux, P.sa_aux);
     rc |= b->upload(b->slot_flags, P.slot_flags);
     for (int s = 0; s < P.num_slices; s++) {
-        b->max_width = std::max(b->max_width, P.slice_width[s]);
-        const int64_t t0 = P.slice_off[s] / ffh::kSlice;
-        if (P.sa_toff[t0 + P.slice_width[s]] - P.sa_toff[t0] > kAsmCodeLds) b->codes_lds = false;
+        const int w0 = P.sa_woff[s], w1 = P.sa_woff[s + 1];
+        b->max_upper = std::max(b->max_upper, w1 - w0 - 1);
+        if (P.sa_wcoff[w1] - P.sa_wcoff[w0] > kAsmCodeLds) b->codes_lds = false;
     }
     rc |= b->upload(b->nt_off, P.ntile.off);
     rc |= b->upload(b->nt_cnt, P.ntile.cnt);
@@ -1487,11 +1500,12 @@ int ffh_blob_set_tuning(ffh_blob *b, int element_block, int assembly_block, int 
                         int spmv_waves_per_slice, int spmv_nontemporal) {
     if (!b) return fail(FFH_E_ARG, "null blob");
     auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
-    if (!ok(element_block) || (assembly_block != 256 && assembly_block != 512) ||
+    if (!ok(element_block) || (assembly_block != 256 && assembly_block != 512 && assembly_block != 1024) ||
         (spmv_waves_per_slice != 1 && spmv_waves_per_slice != 3) ||
-        (assembly_tiles != 1 && assembly_tiles != 2 && assembly_tiles != 4))
-        return fail(FFH_E_ARG, "the element block size must be 64, 128 or 256, the assembly workgroup 256 or 512 "
-                               "threads with 1, 2 or 4 tiles per thread, and SpMV waves per slice 1 or 3");
+        (assembly_tiles != 1 && assembly_tiles != 2 && assembly_tiles != 4) ||
+        (assembly_block == 1024 && assembly_tiles == 4))
+        return fail(FFH_E_ARG, "the element block size must be 64, 128 or 256, the assembly workgroup 256, 512 or "
+                               "1024 threads with 1, 2 or 4 upper tiles per wave (1 or 2 with 1024), and SpMV waves per slice 1 or 3");
     b->elem_block = element_block;
     b->asm_block = assembly_block;
     b->asm_tiles = assembly_tiles;

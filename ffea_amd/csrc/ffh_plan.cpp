@@ -392,21 +392,35 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
 
     // --- slice assembly lists (SparseMatrixFixedPattern::build's source lists,
     // src/SparseMatrixFixedPattern.cpp:85-93): per slice the distinct elements its rows touch,
-    // ascending; per slot its contributions (element, local row vertex a, local column vertex b)
-    // in ascending element order, as codes local-element << 4 | a << 2 | b, tiled [k][64] ---
+    // ascending, and its work tiles.  A lower block (column < row) is the transpose of the upper
+    // block (row, column swapped) bit for bit -- the same element blocks, the same products in
+    // the same order -- so only the diagonal and the upper blocks are summed; each upper block is
+    // also written, transposed, to its lower mirror.  Work tile 0 of a slice holds the 64 rows'
+    // diagonal blocks; then one work tile per SELL tile that holds an upper block.  Per lane its
+    // contributions (element, local row vertex a, local column vertex b) in ascending element
+    // order, as codes local-element << 4 | a << 2 | b, tiled [k][64]; aux: the diagonal's SELL
+    // position (tile 0) or the mirror's global slot (upper tiles), -1 where the lane has none ---
     struct SliceLists {
         std::vector<int32_t> elem;
-        std::vector<uint16_t> width;  // per tile of the slice: max contributions of its slots
-        std::vector<uint16_t> code;   // the slice's tiles, each [width][64]
-        std::vector<uint8_t> cnt;     // [w * 64]
+        std::vector<int32_t> wtile;   // SELL tile of each work tile, -1 for the diagonal tile
+        std::vector<uint16_t> width;  // per work tile: max contributions of its lanes
+        std::vector<uint16_t> code;   // the work tiles, each [width][64]
+        std::vector<uint8_t> cnt;     // [work tiles * 64]
+        std::vector<int32_t> aux;     // [work tiles * 64]
         std::string err;
     };
     std::vector<SliceLists> sl(P.num_slices);
+    auto slot_of = [&](int r, int c) -> int64_t {  // global slot of block (r, c), which exists
+        const int32_t *cb = bcol.data() + row_off[r], *ce = bcol.data() + row_off[r + 1];
+        const int t = (int)(std::lower_bound(cb, ce, c) - cb);
+        return P.slice_off[r / kSlice] + (int64_t)t * kSlice + r % kSlice;
+    };
     parallel_for(P.num_slices, [&](int64_t s, std::vector<int32_t> &pos_k) {
         SliceLists &o = sl[s];
         const int w = P.slice_width[s];
-        // each row's contributions, element by element (ascending, the node lists' order), then
-        // column vertex b: for a fixed slot they arrive in ascending element order
+        // each row's diagonal and upper contributions, element by element (ascending, the node
+        // lists' order), then column vertex b: for a fixed block they arrive in ascending
+        // element order.  fn(SELL position t, lane, element, a << 2 | b)
         auto each = [&](auto &&fn) {
             for (int l = 0; l < kSlice; l++) {
                 const int r = (int)s * kSlice + l;
@@ -416,17 +430,34 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
                     const uint32_t e = P.nctb[m] >> 2, a = P.nctb[m] & 3u;
                     for (uint32_t b = 0; b < 4; b++) {
                         const int c = P.enodes[4 * (size_t)e + b];
-                        if (P.pinned[c]) continue;
-                        fn((int)(std::lower_bound(cb, ce, c) - cb) * kSlice + l, e, a << 2 | b);
+                        if (P.pinned[c] || c < r) continue;
+                        fn((int)(std::lower_bound(cb, ce, c) - cb), l, e, a << 2 | b, c == r);
                     }
                 }
             }
         };
-        o.cnt.assign((size_t)w * kSlice, 0);
+        // work tiles: the diagonal tile, then the SELL tiles holding an upper block
+        std::vector<int> wt_of(w, -1);
+        o.wtile.push_back(-1);
+        {
+            std::vector<char> up(w, 0);
+            each([&](int t, int, uint32_t, uint32_t, bool diag) {
+                if (!diag) up[t] = 1;
+            });
+            for (int t = 0; t < w; t++)
+                if (up[t]) {
+                    wt_of[t] = (int)o.wtile.size();
+                    o.wtile.push_back(t);
+                }
+        }
+        const int nw = (int)o.wtile.size();
+        o.cnt.assign((size_t)nw * kSlice, 0);
+        o.aux.assign((size_t)nw * kSlice, -1);
         bool over = false;
-        each([&](int slot, uint32_t e, uint32_t) {
-            if (o.cnt[slot] == 255) over = true;
-            else o.cnt[slot]++;
+        each([&](int t, int l, uint32_t e, uint32_t, bool diag) {
+            uint8_t &c = o.cnt[(size_t)(diag ? 0 : wt_of[t]) * kSlice + l];
+            if (c == 255) over = true;
+            else c++;
             o.elem.push_back((int32_t)e);
         });
         if (over) {
@@ -440,36 +471,56 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
             o.err = "a 64-row slice touches more than 4096 elements";
             return;
         }
-        o.width.assign(w, 0);
-        std::vector<size_t> base(w + 1, 0);
-        for (int t = 0; t < w; t++) {
-            for (int l = 0; l < kSlice; l++) o.width[t] = std::max<uint16_t>(o.width[t], o.cnt[(size_t)t * kSlice + l]);
-            base[t + 1] = base[t] + (size_t)o.width[t] * kSlice;
+        // aux: the diagonal's SELL position of every row that has one; an upper block's mirror
+        for (int l = 0; l < kSlice; l++) {
+            const int r = (int)s * kSlice + l;
+            if (r >= N) continue;
+            const int32_t *cb = bcol.data() + row_off[r], *ce = bcol.data() + row_off[r + 1];
+            o.aux[l] = (int32_t)(std::lower_bound(cb, ce, r) - cb);
+            if (P.pinned[r]) continue;
+            for (const int32_t *c = cb; c < ce; c++)
+                if (*c > r) o.aux[(size_t)wt_of[c - cb] * kSlice + l] = (int32_t)slot_of(*c, r);
         }
-        o.code.assign(base[w], 0);
-        pos_k.assign((size_t)w * kSlice, 0);
-        each([&](int slot, uint32_t e, uint32_t ab) {
+        o.width.assign(nw, 0);
+        std::vector<size_t> base(nw + 1, 0);
+        for (int j = 0; j < nw; j++) {
+            for (int l = 0; l < kSlice; l++) o.width[j] = std::max<uint16_t>(o.width[j], o.cnt[(size_t)j * kSlice + l]);
+            base[j + 1] = base[j] + (size_t)o.width[j] * kSlice;
+        }
+        o.code.assign(base[nw], 0);
+        pos_k.assign((size_t)nw * kSlice, 0);
+        each([&](int t, int l, uint32_t e, uint32_t ab, bool diag) {
             const uint32_t le = (uint32_t)(std::lower_bound(o.elem.begin(), o.elem.end(), (int32_t)e) - o.elem.begin());
-            const int t = slot / kSlice, l = slot % kSlice;
-            o.code[base[t] + (size_t)pos_k[slot]++ * kSlice + l] = (uint16_t)(le << 4 | ab);
+            const int j = diag ? 0 : wt_of[t];
+            o.code[base[j] + (size_t)pos_k[(size_t)j * kSlice + l]++ * kSlice + l] = (uint16_t)(le << 4 | ab);
         });
     });
     P.sa_eoff.assign((size_t)P.num_slices + 1, 0);
-    P.sa_toff.assign((size_t)(P.num_slots / kSlice) + 1, 0);
+    P.sa_woff.assign((size_t)P.num_slices + 1, 0);
     for (int s = 0; s < P.num_slices; s++) {
         if (!sl[s].err.empty()) return sl[s].err;
         P.sa_eoff[s + 1] = P.sa_eoff[s] + (int64_t)sl[s].elem.size();
-        const int64_t tile0 = P.slice_off[s] / kSlice;
-        for (int t = 0; t < P.slice_width[s]; t++)
-            P.sa_toff[tile0 + t + 1] = P.sa_toff[tile0 + t] + (int64_t)sl[s].width[t] * kSlice;
+        P.sa_woff[s + 1] = P.sa_woff[s] + (int32_t)sl[s].wtile.size();
     }
+    const int64_t NW = P.sa_woff[P.num_slices];
+    P.sa_wtile.resize(NW);
+    P.sa_wcoff.assign((size_t)NW + 1, 0);
+    for (int s = 0; s < P.num_slices; s++)
+        for (size_t j = 0; j < sl[s].wtile.size(); j++) {
+            const int64_t g = P.sa_woff[s] + (int64_t)j;
+            P.sa_wtile[g] = sl[s].wtile[j];
+            P.sa_wcoff[g + 1] = P.sa_wcoff[g] + (int64_t)sl[s].width[j] * kSlice;
+        }
     P.sa_elem.resize(P.sa_eoff[P.num_slices]);
-    P.sa_code.resize(P.sa_toff.back());
-    P.sa_cnt.resize(P.num_slots);
+    P.sa_code.resize(P.sa_wcoff.back());
+    P.sa_cnt.resize((size_t)NW * kSlice);
+    P.sa_aux.resize((size_t)NW * kSlice);
     parallel_for(P.num_slices, [&](int64_t s, std::vector<int32_t> &) {
+        const int64_t g0 = P.sa_woff[s];
         std::copy(sl[s].elem.begin(), sl[s].elem.end(), P.sa_elem.begin() + P.sa_eoff[s]);
-        std::copy(sl[s].code.begin(), sl[s].code.end(), P.sa_code.begin() + P.sa_toff[P.slice_off[s] / kSlice]);
-        std::copy(sl[s].cnt.begin(), sl[s].cnt.end(), P.sa_cnt.begin() + P.slice_off[s]);
+        std::copy(sl[s].code.begin(), sl[s].code.end(), P.sa_code.begin() + P.sa_wcoff[g0]);
+        std::copy(sl[s].cnt.begin(), sl[s].cnt.end(), P.sa_cnt.begin() + g0 * kSlice);
+        std::copy(sl[s].aux.begin(), sl[s].aux.end(), P.sa_aux.begin() + g0 * kSlice);
         SliceLists().elem.swap(sl[s].elem);
         std::vector<uint16_t>().swap(sl[s].code);
     });

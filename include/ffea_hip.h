@@ -137,9 +137,10 @@ int ffh_blob_step(ffh_blob *b, int nsteps);
 
 /* Launch shapes, tuned at creation by timing this blob's own kernels (set FFH_AUTOTUNE=0 in
  * the environment to keep the defaults 256 / 512 / 2 / 3 / 0): threads per workgroup of the
- * element kernel (64, 128 or 256); threads per workgroup of the assembly (256 or 512: one
- * workgroup per 64-row slice) and the 64-slot tiles each assembly thread sums per pass (1, 2 or
- * 4); waves per 64-row SpMV slice (3: one per component; 1: one wave computes all three) and
+ * element kernel (64, 128 or 256); threads per workgroup of the assembly (256, 512 or 1024: one
+ * workgroup per 64-row slice, whose first wave sums the diagonal blocks) and the 64-slot tiles of
+ * upper blocks each other assembly wave sums per pass (1, 2 or 4; 1 or 2 with 1024 threads);
+ * waves per 64-row SpMV slice (3: one per component; 1: one wave computes all three) and
  * non-temporal matrix loads in the SpMV (1: the streamed matrix leaves the caches to the
  * gathered vector; wins when it streams from HBM, loses when it fits the Infinity Cache).
  * Results are bit-identical for every choice. */

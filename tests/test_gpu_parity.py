@@ -237,7 +237,7 @@ def test_tuned_launch_shapes_give_identical_results(precision, batch):
             return gpu_blob(m, prm, sc, seed, x0, precision)
     ref = None
     for shape in ((256, 512, 2, 3, 0), (64, 256, 1, 1, 1), (128, 512, 4, 3, 1), (256, 256, 4, 1, 0),
-                  (128, 512, 1, 1, 1), (64, 256, 2, 3, 0)):
+                  (128, 512, 1, 1, 1), (64, 256, 2, 3, 0), (64, 1024, 1, 3, 0), (128, 1024, 2, 1, 1)):
         b = make()
         b.set_tuning(*shape)
         assert b.tuning() == dict(zip(b.TUNING_KEYS, shape))
