@@ -119,11 +119,24 @@ __device__ __forceinline__ void block_partial_at(double v, double *__restrict__ 
     if (threadIdx.x == 0) parts[idx] = t;
 }
 
+// s = ((0 + p[i]) + p[i + stride]) + ... over [i, i1): a fixed order, with eight loads in
+// flight instead of one dependent load per term.
+__device__ __forceinline__ double strided_sum(const double *__restrict__ p, int i, int i1, int stride) {
+    double s = 0.0;
+    for (; i + 7 * stride < i1; i += 8 * stride) {
+        double a[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = p[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += a[k];
+    }
+    for (; i < i1; i += stride) s += p[i];
+    return s;
+}
+
 // Fixed-order total of the partials [i0, i1): one blob's slices.
 __device__ __forceinline__ double parts_range(const double *__restrict__ parts, int i0, int i1, double *sh) {
-    double s = 0.0;
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) s += parts[i];
-    return block_sum_all(s, sh);
+    return block_sum_all(strided_sum(parts, i0 + (int)threadIdx.x, i1, (int)blockDim.x), sh);
 }
 
 // Per-blob totals.  A blob of many slices would make every consumer block re-add thousands of
@@ -178,6 +191,48 @@ __device__ __forceinline__ X stream_load(const X *p) {
         return __builtin_nontemporal_load(p);
     else
         return *p;
+}
+
+// A 64-slot tile of the block matrix holds 9 * 64 values.  FP64 tiles are entry-major (entry
+// q = 3 i + j of lane l at q * 64 + l: a wave reads one entry of its 64 blocks with one 512-byte
+// load).  FP32 tiles keep a lane's entries 0-3 and 4-7 as two 16-byte groups ([2][64][4]) and
+// entry 8 after them ([64]): a lane reads its whole block with two 16-byte loads and one 4-byte
+// load, each of them one contiguous run over the wave.
+template <typename T>
+__host__ __device__ __forceinline__ int tile_entry(int q, int lane) {
+    if constexpr (sizeof(T) == 8)
+        return q * 64 + lane;
+    else
+        return q < 8 ? (q >> 2) * 256 + lane * 4 + (q & 3) : 512 + lane;
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Lane `lane`'s block (entries 0..8) from the tile at `t` (9 * 64 values, 16-byte aligned).
+template <bool kNt, typename T>
+__device__ __forceinline__ void load_block(const T *__restrict__ t, int lane, T v[9]) {
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) v[q] = stream_load<kNt>(t + q * 64 + lane);
+    } else {
+        const f32x4 a = stream_load<kNt>(reinterpret_cast<const f32x4 *>(t) + lane);
+        const f32x4 b = stream_load<kNt>(reinterpret_cast<const f32x4 *>(t) + 64 + lane);
+        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+        v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+        v[8] = stream_load<kNt>(t + 512 + lane);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_block(T *__restrict__ t, int lane, const T v[9]) {
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) t[q * 64 + lane] = v[q];
+    } else {
+        reinterpret_cast<f32x4 *>(t)[lane] = f32x4{v[0], v[1], v[2], v[3]};
+        reinterpret_cast<f32x4 *>(t)[64 + lane] = f32x4{v[4], v[5], v[6], v[7]};
+        t[512 + lane] = v[8];
+    }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The gathering

@@ -440,8 +440,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble_slice(AsmArgs g) {
             g.dinv[3 * (size_t)node + 0] = 1.0 / (double)b[0];
             g.dinv[3 * (size_t)node + 1] = 1.0 / (double)b[4];
             g.dinv[3 * (size_t)node + 2] = 1.0 / (double)b[8];
-#pragma unroll
-            for (int q = 0; q < 9; q++) val[tile * 9 + q * 64 + lane] = b[q];
+            store_block<T>(val + tile * 9, lane, b);
         } else {
 #pragma unroll
             for (int j = 0; j < kTpt; j++) {
@@ -449,13 +448,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble_slice(AsmArgs g) {
                 const int m = g.aux[(int64_t)gj[j] * 64 + lane];
                 if (m < 0) continue;  // no upper block in this lane
                 const int64_t tile = slot0 + (int64_t)g.wtile[gj[j]] * 64;
-#pragma unroll
-                for (int q = 0; q < 9; q++) val[tile * 9 + q * 64 + lane] = acc[j][q];
-                const int64_t mt = (int64_t)(m & ~63), ml = m & 63;
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int jj = 0; jj < 3; jj++) val[mt * 9 + (3 * jj + i) * 64 + ml] = acc[j][3 * i + jj];
+                store_block<T>(val + tile * 9, lane, acc[j]);
+                const T tr[9] = {acc[j][0], acc[j][3], acc[j][6], acc[j][1], acc[j][4], acc[j][7], acc[j][2], acc[j][5], acc[j][8]};
+                store_block<T>(val + (int64_t)(m & ~63) * 9, m & 63, tr);
             }
         }
     }

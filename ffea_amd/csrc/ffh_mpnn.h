@@ -90,8 +90,9 @@ __global__ __launch_bounds__(256) void k_mpnn_edges(SpmvArgs m, int nslices, con
     for (int t = 0; t < w; t++) {
         const int64_t slot = base + (int64_t)t * 64 + lane;
         const int c = m.col[slot];
-        const T *vt = val + (base + (int64_t)t * 64) * 9 + lane;
-        const double tr = (double)vt[0] + (double)vt[4 * 64] + (double)vt[8 * 64];
+        const T *vt = val + (base + (int64_t)t * 64) * 9;
+        const double tr = (double)vt[tile_entry<T>(0, lane)] + (double)vt[tile_entry<T>(4, lane)] +
+                          (double)vt[tile_entry<T>(8, lane)];
         const double trj = 1.0 / dinv[3 * c] + 1.0 / dinv[3 * c + 1] + 1.0 / dinv[3 * c + 2];
         aw[slot] = (float)(tr / sqrt(tri * trj));
     }

@@ -293,8 +293,10 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
         for (int t = 0; t < w; t++) {
             const int64_t tile = base + (int64_t)t * 64;
             const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
-            const double v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
+            const T *vt = val + tile * 9;
+            const double v0 = stream_load<kNt>(vt + tile_entry<T>(3 * ci, lane)),
+                         v1 = stream_load<kNt>(vt + tile_entry<T>(3 * ci + 1, lane)),
+                         v2 = stream_load<kNt>(vt + tile_entry<T>(3 * ci + 2, lane));
             acc += v0 * (double)z[3 * c + 0];
             acc += v1 * (double)z[3 * c + 1];
             acc += v2 * (double)z[3 * c + 2];
@@ -303,15 +305,32 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             accx += v2 * x[3 * c + 2];
         }
     } else {
-#pragma unroll 4
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9 + (3 * ci) * 64 + lane;
-            const T v0 = stream_load<kNt>(vt), v1 = stream_load<kNt>(vt + 64), v2 = stream_load<kNt>(vt + 128);
-            acc += (double)v0 * (double)z[3 * c + 0];
-            acc += (double)v1 * (double)z[3 * c + 1];
-            acc += (double)v2 * (double)z[3 * c + 2];
+        // kG tiles at a time, loads first, sums in tile order (as in k_cg_spmv1)
+        constexpr int kG = 8;
+        for (int t0 = 0; t0 < w; t0 += kG) {
+            const int n = min(kG, w - t0);
+            int c[kG];
+            T vv[kG][3];
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                const int64_t tile = base + (int64_t)min(t0 + u, w - 1) * 64;
+                c[u] = stream_load<kNt>(m.col + tile + lane);
+                const T *vt = val + tile * 9;
+#pragma unroll
+                for (int j = 0; j < 3; j++) vv[u][j] = stream_load<kNt>(vt + tile_entry<T>(3 * ci + j, lane));
+            }
+            V zz[kG][3];
+#pragma unroll
+            for (int u = 0; u < kG; u++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) zz[u][j] = z[3 * c[u] + j];
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                if (u >= n) break;
+                acc += (double)vv[u][0] * (double)zz[u][0];
+                acc += (double)vv[u][1] * (double)zz[u][1];
+                acc += (double)vv[u][2] * (double)zz[u][2];
+            }
         }
     }
     const size_t k = 3 * (size_t)row + ci;
@@ -334,11 +353,7 @@ __device__ __forceinline__ double parts_range_as192(const double *__restrict__ p
     const int t = threadIdx.x & 63;
     double tot = 0.0;
 #pragma unroll
-    for (int w = 0; w < 3; w++) {
-        double s = 0.0;
-        for (int i = i0 + t + 64 * w; i < i1; i += 192) s += parts[i];
-        tot += __shfl(wave_sum(s), 0, 64);
-    }
+    for (int w = 0; w < 3; w++) tot += __shfl(wave_sum(strided_sum(parts, i0 + t + 64 * w, i1, 192)), 0, 64);
     return tot;
 }
 
@@ -389,10 +404,11 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
         for (int t = 0; t < w; t++) {
             const int64_t tile = base + (int64_t)t * 64;
             const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9 + lane;
+            T vt[9];
+            load_block<kNt>(val + tile * 9, lane, vt);
             double vv[9];
 #pragma unroll
-            for (int kk = 0; kk < 9; kk++) vv[kk] = (double)stream_load<kNt>(vt + 64 * kk);
+            for (int kk = 0; kk < 9; kk++) vv[kk] = (double)vt[kk];
             const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
             const double y0 = x[3 * c + 0], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
             a0 += vv[0] * z0;
@@ -415,24 +431,40 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             x2 += vv[8] * y2;
         }
     } else {
-#pragma unroll 2
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9 + lane;
-            T vv[9];
+        // kG tiles at a time: their column indices and blocks are loaded first, then the gathers
+        // of z, then the sums in tile order -- the order of a one-tile loop, with kG dependent
+        // load pairs in flight.  The last group loads the slice's last tile again in place of
+        // the missing ones and does not add them.
+        constexpr int kG = 8;
+        for (int t0 = 0; t0 < w; t0 += kG) {
+            const int n = min(kG, w - t0);
+            int c[kG];
+            T vv[kG][9];
 #pragma unroll
-            for (int kk = 0; kk < 9; kk++) vv[kk] = stream_load<kNt>(vt + 64 * kk);
-            const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-            a0 += (double)vv[0] * z0;
-            a0 += (double)vv[1] * z1;
-            a0 += (double)vv[2] * z2;
-            a1 += (double)vv[3] * z0;
-            a1 += (double)vv[4] * z1;
-            a1 += (double)vv[5] * z2;
-            a2 += (double)vv[6] * z0;
-            a2 += (double)vv[7] * z1;
-            a2 += (double)vv[8] * z2;
+            for (int u = 0; u < kG; u++) {
+                const int64_t tile = base + (int64_t)min(t0 + u, w - 1) * 64;
+                c[u] = stream_load<kNt>(m.col + tile + lane);
+                load_block<kNt>(val + tile * 9, lane, vv[u]);
+            }
+            V zz[kG][3];
+#pragma unroll
+            for (int u = 0; u < kG; u++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) zz[u][j] = z[3 * c[u] + j];
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                if (u >= n) break;
+                const double z0 = zz[u][0], z1 = zz[u][1], z2 = zz[u][2];
+                a0 += (double)vv[u][0] * z0;
+                a0 += (double)vv[u][1] * z1;
+                a0 += (double)vv[u][2] * z2;
+                a1 += (double)vv[u][3] * z0;
+                a1 += (double)vv[u][4] * z1;
+                a1 += (double)vv[u][5] * z2;
+                a2 += (double)vv[u][6] * z0;
+                a2 += (double)vv[u][7] * z1;
+                a2 += (double)vv[u][8] * z2;
+            }
         }
     }
     const size_t k = 3 * (size_t)row;
@@ -554,11 +586,12 @@ __global__ __launch_bounds__(kBlock) void k_apply(SpmvArgs m, int N, const doubl
     for (int t = 0; t < w; t++) {
         const int64_t tile = base + (int64_t)t * 64;
         const int c = m.col[tile + lane];
-        const T *vt = val + tile * 9 + lane;
+        T vt[9];
+        load_block<false>(val + tile * 9, lane, vt);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) acc[i] += (double)vt[(3 * i + j) * 64] * in[3 * c + j];
+            for (int j = 0; j < 3; j++) acc[i] += (double)vt[3 * i + j] * in[3 * c + j];
     }
     out[3 * row] = acc[0];
     out[3 * row + 1] = acc[1];
