@@ -216,11 +216,11 @@ struct AsmArgs {
     const int64_t *eoff;      // [slices + 1] into elem
     const int32_t *elem;      // per slice, the distinct elements its rows touch, ascending
     const int32_t *woff;      // [slices + 1] work tiles of each slice (the first: its diagonal tile)
-    const int32_t *wtile;     // [work tiles] SELL tile position of an upper tile, -1 for the diagonal tile
     const int64_t *wcoff;     // [work tiles + 1] first code of each work tile, 128-byte aligned
     const uint16_t *code;     // slice-local element << 4 | a << 2 | b, tile-major [k][64]
     const uint8_t *cnt;       // [work tiles * 64] contributions
     const int32_t *aux;       // [work tiles * 64] diagonal SELL position / mirror slot, -1: none
+    const uint8_t *wpos;      // [work tiles * 64] SELL position of an upper block
     const int64_t *slice_off; // [slices + 1] first slot; slot = slice_off[s] + 64 t + lane
     const uint8_t *flags;     // [slots] ffh::kSlotDiag / kSlotIdentity / kSlotPad
     const double *zeta;
@@ -249,10 +249,11 @@ struct AsmArgs {
 // a time, into LDS (shape gradients, volume, A, B: element_geometry, the element kernel's own
 // code, so they round as there; element ids are loaded one chunk ahead and nodes are gathered
 // while the previous chunk's geometry is computed).  Wave 0 sums the 64 diagonal blocks (six
-// distinct entries each: a diagonal block is symmetric); waves 1.. sum the upper tiles, kTpt
-// each per pass.  kLdsCodes: the slice's contribution codes (at most kAsmCodeLds) are first
-// copied to LDS with 16-byte loads, and each thread finds where its list leaves the chunk by
-// binary search, so the summation loop has no data-dependent exit and its LDS reads pipeline.
+// distinct entries each: a diagonal block is symmetric); waves 1.. sum the upper work tiles
+// (the upper blocks of SELL tiles whose lanes do not meet share one), kTpt each per pass.  A
+// lane adds the contributions of its list that fall in the chunk, reading the next code while it
+// sums the current one.  kLdsCodes: the slice's contribution codes (at most kAsmCodeLds) are
+// first copied to LDS with 16-byte loads.
 constexpr int kAsmCodeLds = 9216;
 
 template <typename T, int kThreads>
@@ -296,19 +297,6 @@ __device__ __forceinline__ void asm_add_diag(const T (&lds)[15][kThreads], uint3
     d[3] += (AB * (g0 * g1)) * vol;
     d[4] += (AB * (g0 * g2)) * vol;
     d[5] += (AB * (g1 * g2)) * vol;
-}
-
-// Position after the last of a lane's codes [k, cnt) that lies before `lim` (codes ascend).
-__device__ __forceinline__ int asm_chunk_end(const uint16_t *lcode, int64_t co, int k, int cnt, uint32_t lim) {
-    int lo = k, hi = cnt;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((uint32_t)lcode[co + 64 * mid] < lim)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
 }
 
 template <typename T, int kWaves, int kTpt, bool kLdsCodes>
@@ -386,31 +374,29 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble_slice(AsmArgs g) {
 #pragma unroll
             for (int j = 0; j < kTpt; j++) {
                 if (wave == 0 && j > 0) break;
-                int lo;
-                if constexpr (kLdsCodes) {
-                    lo = asm_chunk_end(lcode, co[j], k[j], cnt[j], lim);
-                } else {
-                    lo = k[j];
-                    while (lo < cnt[j] && code_at(co[j] + 64 * lo) < lim) lo++;
-                }
-                // the next code is read while this contribution is summed
-                if (k[j] < lo) {
-                    uint32_t code = code_at(co[j] + 64 * k[j]);
+                // this lane's codes from k[j] on while they lie in the chunk (codes ascend); the
+                // next code is read while the current contribution is summed
+                int kk = k[j];
+                if (kk < cnt[j]) {
+                    const int last = cnt[j] - 1;
+                    uint32_t code = code_at(co[j] + 64 * kk);
                     if (wave == 0) {
-                        for (int kk = k[j]; kk < lo; kk++) {
-                            const uint32_t nxt = code_at(co[j] + 64 * min(kk + 1, lo - 1));
+                        while (kk <= last && code < lim) {
+                            const uint32_t nxt = code_at(co[j] + 64 * min(kk + 1, last));
                             asm_add_diag<T, kThreads>(lds, code, (int)(code >> 4) - c0, acc[0]);
                             code = nxt;
+                            kk++;
                         }
                     } else {
-                        for (int kk = k[j]; kk < lo; kk++) {
-                            const uint32_t nxt = code_at(co[j] + 64 * min(kk + 1, lo - 1));
+                        while (kk <= last && code < lim) {
+                            const uint32_t nxt = code_at(co[j] + 64 * min(kk + 1, last));
                             asm_add_upper<T, kThreads>(lds, code, (int)(code >> 4) - c0, acc[j]);
                             code = nxt;
+                            kk++;
                         }
                     }
                 }
-                k[j] = lo;
+                k[j] = kk;
             }
             __syncthreads();
             id_c = id_n;
@@ -447,7 +433,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_assemble_slice(AsmArgs g) {
                 if (gj[j] < 0) continue;
                 const int m = g.aux[(int64_t)gj[j] * 64 + lane];
                 if (m < 0) continue;  // no upper block in this lane
-                const int64_t tile = slot0 + (int64_t)g.wtile[gj[j]] * 64;
+                const int64_t tile = slot0 + (int64_t)g.wpos[(int64_t)gj[j] * 64 + lane] * 64;
                 store_block<T>(val + tile * 9, lane, acc[j]);
                 const T tr[9] = {acc[j][0], acc[j][3], acc[j][6], acc[j][1], acc[j][4], acc[j][7], acc[j][2], acc[j][5], acc[j][8]};
                 store_block<T>(val + (int64_t)(m & ~63) * 9, m & 63, tr);

@@ -113,9 +113,9 @@ struct ffh_blob {
     uint8_t *pinned = nullptr;
     // slice assembly (k_assemble_slice): element lists, tiled contribution codes, slot tables
     int64_t *sa_eoff = nullptr, *sa_wcoff = nullptr;
-    int32_t *sa_elem = nullptr, *sa_woff = nullptr, *sa_wtile = nullptr, *sa_aux = nullptr;
+    int32_t *sa_elem = nullptr, *sa_woff = nullptr, *sa_aux = nullptr;
     uint16_t *sa_code = nullptr;
-    uint8_t *sa_cnt = nullptr, *slot_flags = nullptr;
+    uint8_t *sa_cnt = nullptr, *sa_pos = nullptr, *slot_flags = nullptr;
     int max_upper = 0;      // most upper work tiles of a slice
     bool codes_lds = true;  // every slice's assembly codes fit the kernel's LDS area
     double *zeta = nullptr, *dinv = nullptr;
@@ -450,11 +450,11 @@ struct ffh_blob {
         a.eoff = sa_eoff;
         a.elem = sa_elem;
         a.woff = sa_woff;
-        a.wtile = sa_wtile;
         a.wcoff = sa_wcoff;
         a.code = sa_code;
         a.cnt = sa_cnt;
         a.aux = sa_aux;
+        a.wpos = sa_pos;
         a.slice_off = slice_off;
         a.flags = slot_flags;
         a.zeta = zeta;
@@ -1271,11 +1271,11 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->upload(b->sa_eoff, P.sa_eoff);
     rc |= b->upload(b->sa_elem, P.sa_elem);
     rc |= b->upload(b->sa_woff, P.sa_woff);
-    rc |= b->upload(b->sa_wtile, P.sa_wtile);
     rc |= b->upload(b->sa_wcoff, P.sa_wcoff);
     rc |= b->upload(b->sa_code, P.sa_code);
     rc |= b->upload(b->sa_cnt, P.sa_cnt);
     rc |= b->upload(b->sa_aux, P.sa_aux);
+    rc |= b->upload(b->sa_pos, P.sa_pos);
     rc |= b->upload(b->slot_flags, P.slot_flags);
     for (int s = 0; s < P.num_slices; s++) {
         const int w0 = P.sa_woff[s], w1 = P.sa_woff[s + 1];

@@ -369,6 +369,8 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
         P.slice_off[s + 1] = P.slice_off[s] + (int64_t)w * kSlice;
     }
     P.num_slots = P.slice_off[P.num_slices];
+    for (int s = 0; s < P.num_slices; s++)
+        if (P.slice_width[s] > 254) return "a node has more than 253 neighbours";
     P.slice_blob.assign(P.num_slices, 0);
     for (int b = 0; b < num_blobs; b++)
         for (int sl = P.blob_slice_off[b]; sl < P.blob_slice_off[b + 1]; sl++) P.slice_blob[sl] = b;
@@ -402,11 +404,11 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     // position (tile 0) or the mirror's global slot (upper tiles), -1 where the lane has none ---
     struct SliceLists {
         std::vector<int32_t> elem;
-        std::vector<int32_t> wtile;   // SELL tile of each work tile, -1 for the diagonal tile
         std::vector<uint16_t> width;  // per work tile: max contributions of its lanes
         std::vector<uint16_t> code;   // the work tiles, each [width][64]
         std::vector<uint8_t> cnt;     // [work tiles * 64]
         std::vector<int32_t> aux;     // [work tiles * 64]
+        std::vector<uint8_t> pos;     // [work tiles * 64] SELL position of the lane's block, 255: none
         std::string err;
     };
     std::vector<SliceLists> sl(P.num_slices);
@@ -436,23 +438,33 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
                 }
             }
         };
-        // work tiles: the diagonal tile, then the SELL tiles holding an upper block
+        // work tiles: the diagonal tile, then the SELL tiles holding an upper block, merged where
+        // their lanes do not meet (densest first, each into the first work tile it fits): a work
+        // lane then holds one upper block of its row, at SELL position pos
         std::vector<int> wt_of(w, -1);
-        o.wtile.push_back(-1);
+        std::vector<uint64_t> wmask(1, 0);
         {
-            std::vector<char> up(w, 0);
-            each([&](int t, int, uint32_t, uint32_t, bool diag) {
-                if (!diag) up[t] = 1;
+            std::vector<uint64_t> lanes(w, 0);
+            each([&](int t, int l, uint32_t, uint32_t, bool diag) {
+                if (!diag) lanes[t] |= 1ull << l;
             });
+            std::vector<int> order;
             for (int t = 0; t < w; t++)
-                if (up[t]) {
-                    wt_of[t] = (int)o.wtile.size();
-                    o.wtile.push_back(t);
-                }
+                if (lanes[t]) order.push_back(t);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return __builtin_popcountll(lanes[x]) > __builtin_popcountll(lanes[y]); });
+            for (int t : order) {
+                size_t j = 1;
+                while (j < wmask.size() && (wmask[j] & lanes[t])) j++;
+                if (j == wmask.size()) wmask.push_back(0);
+                wmask[j] |= lanes[t];
+                wt_of[t] = (int)j;
+            }
         }
-        const int nw = (int)o.wtile.size();
+        const int nw = (int)wmask.size();
         o.cnt.assign((size_t)nw * kSlice, 0);
         o.aux.assign((size_t)nw * kSlice, -1);
+        o.pos.assign((size_t)nw * kSlice, 255);
         bool over = false;
         each([&](int t, int l, uint32_t e, uint32_t, bool diag) {
             uint8_t &c = o.cnt[(size_t)(diag ? 0 : wt_of[t]) * kSlice + l];
@@ -477,9 +489,13 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
             if (r >= N) continue;
             const int32_t *cb = bcol.data() + row_off[r], *ce = bcol.data() + row_off[r + 1];
             o.aux[l] = (int32_t)(std::lower_bound(cb, ce, r) - cb);
+            o.pos[l] = (uint8_t)o.aux[l];
             if (P.pinned[r]) continue;
             for (const int32_t *c = cb; c < ce; c++)
-                if (*c > r) o.aux[(size_t)wt_of[c - cb] * kSlice + l] = (int32_t)slot_of(*c, r);
+                if (*c > r) {
+                    o.aux[(size_t)wt_of[c - cb] * kSlice + l] = (int32_t)slot_of(*c, r);
+                    o.pos[(size_t)wt_of[c - cb] * kSlice + l] = (uint8_t)(c - cb);
+                }
         }
         o.width.assign(nw, 0);
         std::vector<size_t> base(nw + 1, 0);
@@ -500,27 +516,27 @@ std::string build_plan(Plan &P, int num_nodes, int num_elements, const double *p
     for (int s = 0; s < P.num_slices; s++) {
         if (!sl[s].err.empty()) return sl[s].err;
         P.sa_eoff[s + 1] = P.sa_eoff[s] + (int64_t)sl[s].elem.size();
-        P.sa_woff[s + 1] = P.sa_woff[s] + (int32_t)sl[s].wtile.size();
+        P.sa_woff[s + 1] = P.sa_woff[s] + (int32_t)sl[s].width.size();
     }
     const int64_t NW = P.sa_woff[P.num_slices];
-    P.sa_wtile.resize(NW);
     P.sa_wcoff.assign((size_t)NW + 1, 0);
     for (int s = 0; s < P.num_slices; s++)
-        for (size_t j = 0; j < sl[s].wtile.size(); j++) {
+        for (size_t j = 0; j < sl[s].width.size(); j++) {
             const int64_t g = P.sa_woff[s] + (int64_t)j;
-            P.sa_wtile[g] = sl[s].wtile[j];
             P.sa_wcoff[g + 1] = P.sa_wcoff[g] + (int64_t)sl[s].width[j] * kSlice;
         }
     P.sa_elem.resize(P.sa_eoff[P.num_slices]);
     P.sa_code.resize(P.sa_wcoff.back());
     P.sa_cnt.resize((size_t)NW * kSlice);
     P.sa_aux.resize((size_t)NW * kSlice);
+    P.sa_pos.resize((size_t)NW * kSlice);
     parallel_for(P.num_slices, [&](int64_t s, std::vector<int32_t> &) {
         const int64_t g0 = P.sa_woff[s];
         std::copy(sl[s].elem.begin(), sl[s].elem.end(), P.sa_elem.begin() + P.sa_eoff[s]);
         std::copy(sl[s].code.begin(), sl[s].code.end(), P.sa_code.begin() + P.sa_wcoff[g0]);
         std::copy(sl[s].cnt.begin(), sl[s].cnt.end(), P.sa_cnt.begin() + g0 * kSlice);
         std::copy(sl[s].aux.begin(), sl[s].aux.end(), P.sa_aux.begin() + g0 * kSlice);
+        std::copy(sl[s].pos.begin(), sl[s].pos.end(), P.sa_pos.begin() + g0 * kSlice);
         SliceLists().elem.swap(sl[s].elem);
         std::vector<uint16_t>().swap(sl[s].code);
     });

@@ -72,20 +72,20 @@ struct Plan {
     // pinned row's diagonal: identity), kSlotPad (padding: zero for good)
     std::vector<uint8_t> slot_flags;  // [num_slots]
     // slice assembly: per slice its distinct elements (ascending) and its work tiles: the
-    // diagonal tile (lane l: row 64 s + l's diagonal block), then one per SELL tile holding an
-    // upper block (column > row; a lower block is written as its upper mirror's transpose).
-    // Per work-tile lane its contributions (slice-local element << 4 | local row vertex a << 2 |
-    // local column vertex b, ascending element), tiled [k][64] from sa_wcoff[g]; sa_aux: the
-    // row's diagonal SELL position (diagonal tile) or the mirror's global slot (upper tiles), -1
-    // where the lane has nothing
+    // diagonal tile (lane l: row 64 s + l's diagonal block), then the upper blocks (column > row;
+    // a lower block is written as its upper mirror's transpose) of SELL tiles whose lanes do not
+    // meet, merged into one work tile.  Per work-tile lane its contributions (slice-local element
+    // << 4 | local row vertex a << 2 | local column vertex b, ascending element), tiled [k][64]
+    // from sa_wcoff[g]; sa_pos: the block's SELL position (255: none); sa_aux: for an upper block
+    // its mirror's global slot, -1 where the lane has none
     std::vector<int64_t> sa_eoff;   // [num_slices + 1]
     std::vector<int32_t> sa_elem;   // global element ids
     std::vector<int32_t> sa_woff;   // [num_slices + 1] work tiles of each slice
-    std::vector<int32_t> sa_wtile;  // [work tiles] SELL tile position, -1 for the diagonal tile
     std::vector<int64_t> sa_wcoff;  // [work tiles + 1]
     std::vector<uint16_t> sa_code;
     std::vector<uint8_t> sa_cnt;    // [work tiles * 64]
     std::vector<int32_t> sa_aux;    // [work tiles * 64]
+    std::vector<uint8_t> sa_pos;    // [work tiles * 64]
     // RNG jump tables: tbl[((d*256 + v)*2 + comp)] = A_comp^(v * 256^d), d=0..3
     std::vector<Mat3u> jump;
     Mat3u step_jump[2];        // A^(draws per step)
