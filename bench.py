@@ -111,10 +111,13 @@ def algorithmic_bytes(kernel, sizes, bm, bv=8):
     return {
         # enodes 16, J0^-1 72, vol0 8, A B G E 32, last_det r+w 16; writes the 12 nodal forces
         "element": E * (16 + 72 + 8 + 32 + 16 + 12 * bm) + v,
-        # enodes + A, B once (32E), element lists (4E, each element once), contribution codes
-        # (16 per element, 2 B each), slot counts and flags (2S), positions (32N), drag (8N);
-        # writes every block once and D^-1 (24N)
-        "assemble": 68 * E + 2 * S + 64 * N + 9 * bm * B,
+        # per slice its element list (4 B per slice-element pair, L pairs), enodes + A, B once
+        # (32E), contribution codes (2 B per contribution, C of them: the diagonal and upper
+        # blocks' element terms), per summed block (N diagonal + (B - N) / 2 upper) its count,
+        # SELL position and mirror slot (6 B), the diagonal slots' flags (N), positions (32N),
+        # drag (8N); writes every block once (9 bm B) and D^-1 (24N)
+        "assemble": 4 * sizes.get("slice_elements", E) + 32 * E + 2 * sizes.get("contributions", 10 * E)
+                    + 6 * (N + (B - N) // 2) + 65 * N + 9 * bm * B,
         # element forces + codes + per-node data; writes f, x (FP64) and r, p, q, z
         "rhs": 12 * bm * E + 16 * E + 39 * N + 2 * v + 4 * w,
         # matrix + column indices, z gathered once, p q read and written
@@ -123,6 +126,25 @@ def algorithmic_bytes(kernel, sizes, bm, bv=8):
         "cg_update": 3 * v + 5 * w,
         "integrate": 4 * v,
     }.get(kernel)
+
+
+def assembly_lists(meshes):
+    """Sizes of the assembly's lists for these meshes (blobs numbered on from slice boundaries):
+    L, the distinct (element, 64-row slice) pairs, and C, the element terms of the summed blocks
+    (each element's unpinned vertices on the diagonal, its unpinned vertex pairs above it)."""
+    L = C = 0
+    for m in meshes:
+        t = np.asarray(m.topo)[:, :4].astype(np.int64)
+        E = t.shape[0]
+        sl = t // 64
+        pairs = np.unique(np.concatenate([np.arange(E, dtype=np.int64) * (1 << 32) + sl[:, k] for k in range(4)]))
+        L += int(pairs.size)
+        free = np.ones(m.num_linear, bool)
+        free[np.asarray(m.pinned, np.int64)[np.asarray(m.pinned) < m.num_linear]] = False
+        f = free[t]
+        nf = f.sum(axis=1)
+        C += int(nf.sum() + (nf * (nf - 1) // 2).sum())
+    return {"slice_elements": L, "contributions": C}
 
 
 def true_residual_bytes(sizes):
@@ -168,6 +190,7 @@ def run_gpu(ms, prm, sc, xs, precision, steps, warmup):
     kt = b.kernel_times()
     b.set_profiling(False)
     sizes = b.sizes()
+    sizes.update(assembly_lists(ms))
     sizes["tuning"] = b.tuning()
     log(f"[{precision}] {steps} steps in {dt:.3f}s = {steps / dt:.1f} steps/s (with per-kernel events "
         f"{steps / dt_prof:.1f} steps/s), CG iterations mean {np.mean(iters):.2f}")
