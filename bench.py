@@ -842,6 +842,47 @@ def cg_phase(b, steps, kind):
             "kernels_us": {k: v[0] / v[1] * 1e3 for k, v in kt.items() if v[1]}}
 
 
+def kappa_jacobi(mesh, x0, prm, sc, dims):
+    """Extreme eigenvalues of the Jacobi-preconditioned viscosity matrix D^-1 K (the operator
+    Jacobi-PCG iterates on) of one box mesh, after one step from x0: the scalar diagonal of K by
+    probing (nodes (i, j, k) of one residue class mod 3 share no element, so 27 x 3 applies of K
+    give it exactly), then Lanczos (scipy eigsh) on D^-1/2 K D^-1/2, whose eigenvalues are D^-1 K's,
+    with K applied on the GPU through Solver::apply_matrix.  Also the CG iteration bound for the
+    relative tolerance eps at that condition number, ln(2 / eps) / ln((sqrt k + 1) / (sqrt k - 1))."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    from ffea_amd.blob import Blob, package_seed
+    b = Blob(mesh, prm, sc, rng_state=package_seed(prm.rng_seed), precision="fp64", pos=x0)
+    b.step(1)
+    nl, nt = mesh.num_linear, mesh.num_nodes
+    nx, ny, nz = dims
+    ids = np.arange(nl)
+    i, j, k = ids % (nx + 1), (ids // (nx + 1)) % (ny + 1), ids // ((nx + 1) * (ny + 1))
+    color = (i % 3) + 3 * (j % 3) + 9 * (k % 3)
+
+    def apply(v_lin):
+        full = np.zeros(3 * nt)
+        full[:3 * nl] = v_lin
+        return b.apply_matrix(full)[:3 * nl]
+
+    diag = np.zeros(3 * nl)
+    for col in range(27):
+        sel = np.nonzero(color == col)[0]
+        for c in range(3):
+            v = np.zeros(3 * nl)
+            v[3 * sel + c] = 1.0
+            diag[3 * sel + c] = apply(v)[3 * sel + c]
+    dm = 1.0 / np.sqrt(diag)
+    op = LinearOperator((3 * nl, 3 * nl), matvec=lambda v: dm * apply(dm * np.asarray(v).ravel()), dtype=np.float64)
+    lmax = float(eigsh(op, k=1, which="LA", tol=1e-4, return_eigenvectors=False)[0])
+    lmin = float(eigsh(op, k=1, which="SA", tol=1e-4, return_eigenvectors=False)[0])
+    b.close()
+    kappa = lmax / lmin
+    rho = (np.sqrt(kappa) - 1) / (np.sqrt(kappa) + 1)
+    bound = int(np.ceil(np.log(2.0 / np.sqrt(prm.epsilon2)) / -np.log(rho)))
+    return {"cells": [int(nx), int(ny), int(nz)], "linear_nodes": int(nl), "lambda_min": lmin, "lambda_max": lmax,
+            "kappa": kappa, "cg_iteration_bound": bound}
+
+
 def run_c3(a):
     """Learned preconditioner: harvest, train, evaluate on held-out meshes."""
     from ffea_amd import units as U
@@ -885,6 +926,18 @@ def run_c3(a):
         sizes = b.sizes()
         b.close()
     base = res["fp64_jacobi"]["cg_ms_per_solve"]
+    # condition numbers of D^-1 K on the smallest held-out meshes (the bound on what any
+    # preconditioner can save over Jacobi-PCG)
+    dims = np.random.default_rng(31).integers(24, 65, size=(64, 3))
+    order = np.argsort(np.prod(dims + 1, axis=1))[:a.kappa_meshes]
+    kappas = []
+    for m_i in order:
+        t0 = time.time()
+        kk = kappa_jacobi(ms[m_i], xs[m_i], prm, sc, dims[m_i])
+        kk["blob"] = int(m_i)
+        kappas.append(kk)
+        log(f"[c3 kappa] blob {m_i} {kk['cells']}: lambda {kk['lambda_min']:.4f} .. {kk['lambda_max']:.4f}, "
+            f"kappa {kk['kappa']:.3f}, CG bound {kk['cg_iteration_bound']} iterations ({time.time() - t0:.1f}s)")
     out = {
         "metric": "c3: CG time-to-solution at eps=1e-6, learned-preconditioned mixed flexible CG vs plain FP64 "
                   "Jacobi-PCG (ratio of CG ms per solve)",
@@ -901,6 +954,7 @@ def run_c3(a):
                      "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1])},
         "solvers": res,
         "speedup_mixed_jacobi_vs_fp64_jacobi": base / res["mixed_jacobi"]["cg_ms_per_solve"],
+        "jacobi_condition_numbers": kappas,
     }
     print(json.dumps(out), flush=True)
 
@@ -915,6 +969,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--epochs", type=int, default=20, help="c3: training epochs")
+    ap.add_argument("--kappa-meshes", type=int, default=6,
+                    help="c3: held-out meshes (smallest first) whose D^-1 K condition number is measured")
     ap.add_argument("--parity-steps", type=int, default=None,
                     help="noise-off steps compared with the reference's strict objects (0: skip; default 10 "
                          "for c0 and c1, 0 above: their setup alone takes minutes on the host)")

@@ -152,6 +152,40 @@ __global__ __launch_bounds__(256) void k_mpnn_embed(MpnnArgs a, const U *__restr
     }
 }
 
+// acc[k] += A_rt h[c_t][k] over row rloc's slots t in order (k = 0..7, the 8 bf16 channels at
+// src + c ld): the slots' column indices and couplings are loaded eight at a time, then their
+// eight 16-byte gathers, then the sums in slot order -- the order of a one-slot loop, with eight
+// dependent load pairs in flight.  The last group loads the row's last slot again in place of the
+// missing ones and does not add them.
+__device__ __forceinline__ void message_rows(const MpnnArgs &a, int64_t base, int w, int rloc,
+                                             const bf16_t *__restrict__ src, float acc[8], int ld = kHid) {
+    constexpr int kG = 8;
+    for (int t0 = 0; t0 < w; t0 += kG) {
+        const int n = min(kG, w - t0);
+        int c[kG];
+        float wg[kG];
+#pragma unroll
+        for (int u = 0; u < kG; u++) {
+            const int64_t slot = base + (int64_t)min(t0 + u, w - 1) * 64 + rloc;
+            c[u] = a.col[slot];
+            wg[u] = a.aw[slot];
+        }
+        uint4 v[kG];
+#pragma unroll
+        for (int u = 0; u < kG; u++) v[u] = *reinterpret_cast<const uint4 *>(src + (size_t)c[u] * ld);
+#pragma unroll
+        for (int u = 0; u < kG; u++) {
+            if (u >= n) break;
+            const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                acc[2 * k] += wg[u] * bf2f((bf16_t)(vv[k] & 0xffffu));
+                acc[2 * k + 1] += wg[u] * bf2f((bf16_t)(vv[k] >> 16));
+            }
+        }
+    }
+}
+
 // g = A h (one slice per block, 4 threads per row, 8 channels each).  Input [N][ld] of TI,
 // columns off..off+31; output bf16 [N][32].  With mask (backward), the output is
 // (in0[:, 0:32] + A in[:, off:off+32]) * (mask > 0) in fp32 instead.
@@ -165,29 +199,39 @@ __global__ __launch_bounds__(256) void k_mpnn_spmm(MpnnArgs a, const TI *__restr
     const int w = a.slice_width[slice];
     const int64_t base = a.slice_off[slice];
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int t = 0; t < w; t++) {
-        const int64_t slot = base + (int64_t)t * 64 + rloc;
-        const int c = a.col[slot];
-        const float wgt = a.aw[slot];
-        const TI *src = in + (size_t)c * ld + off + 8 * q;
-        if constexpr (sizeof(TI) == 2) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(src);
-            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
-            for (int k = 0; k < 4; k++) {
-                acc[2 * k] += wgt * bf2f((bf16_t)(vv[k] & 0xffffu));
-                acc[2 * k + 1] += wgt * bf2f((bf16_t)(vv[k] >> 16));
+    if constexpr (sizeof(TI) == 2) {
+        message_rows(a, base, w, rloc, reinterpret_cast<const bf16_t *>(in) + off + 8 * q, acc, ld);
+    } else {
+        constexpr int kG = 8;
+        for (int t0 = 0; t0 < w; t0 += kG) {
+            const int n = min(kG, w - t0);
+            int c[kG];
+            float wg[kG];
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                const int64_t slot = base + (int64_t)min(t0 + u, w - 1) * 64 + rloc;
+                c[u] = a.col[slot];
+                wg[u] = a.aw[slot];
             }
-        } else {
-            const float4 v0 = *reinterpret_cast<const float4 *>(src);
-            const float4 v1 = *reinterpret_cast<const float4 *>(src + 4);
-            acc[0] += wgt * v0.x;
-            acc[1] += wgt * v0.y;
-            acc[2] += wgt * v0.z;
-            acc[3] += wgt * v0.w;
-            acc[4] += wgt * v1.x;
-            acc[5] += wgt * v1.y;
-            acc[6] += wgt * v1.z;
-            acc[7] += wgt * v1.w;
+            float4 v0[kG], v1[kG];
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                const TI *src = in + (size_t)c[u] * ld + off + 8 * q;
+                v0[u] = *reinterpret_cast<const float4 *>(src);
+                v1[u] = *reinterpret_cast<const float4 *>(src + 4);
+            }
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                if (u >= n) break;
+                acc[0] += wg[u] * v0[u].x;
+                acc[1] += wg[u] * v0[u].y;
+                acc[2] += wg[u] * v0[u].z;
+                acc[3] += wg[u] * v0[u].w;
+                acc[4] += wg[u] * v1[u].x;
+                acc[5] += wg[u] * v1[u].y;
+                acc[6] += wg[u] * v1[u].z;
+                acc[7] += wg[u] * v1[u].w;
+            }
         }
     }
     if constexpr (kBwd) {
@@ -249,17 +293,7 @@ __global__ __launch_bounds__(256) void k_mpnn_layer(MpnnArgs a, int layer, const
         const int w = a.slice_width[slice];
         const int64_t base = a.slice_off[slice];
         float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int t = 0; t < w; t++) {
-            const int64_t slot = base + (int64_t)t * 64 + rloc;
-            const int c = a.col[slot];
-            const float wgt = a.aw[slot];
-            const uint4 v = *reinterpret_cast<const uint4 *>(H + (size_t)c * kHid + 8 * q);
-            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
-            for (int k = 0; k < 4; k++) {
-                acc[2 * k] += wgt * bf2f((bf16_t)(vv[k] & 0xffffu));
-                acc[2 * k + 1] += wgt * bf2f((bf16_t)(vv[k] >> 16));
-            }
-        }
+        message_rows(a, base, w, rloc, H + 8 * q, acc);
         for (int k = 0; k < 8; k++) gs[rloc][8 * q + k] = f2bf(acc[k]);
     }
     __syncthreads();
