@@ -163,7 +163,7 @@ struct ffh_blob {
     void *ss_temp = nullptr;
     size_t ss_temp_bytes = 0;
     int ss_bits = 1;
-    int *ss_err_host = nullptr;
+    int *ss_err_host = nullptr, *ss_err_dev = nullptr;
     double *ss_fe = nullptr, *ss_total = nullptr;
     // the energy's pair list: per face count and offset, the pairs' energies and blob pairs
     int *ss_cnt = nullptr, *ss_off = nullptr, *ss_pb = nullptr;
@@ -180,6 +180,7 @@ struct ffh_blob {
         ss_pair_cap = 0;
         if (ss_err_host) (void)hipHostFree(ss_err_host);
         ss_err_host = nullptr;
+        ss_err_dev = nullptr;
         ss_on = false;
         ss = SsArgs{};
         ss_temp = nullptr;
@@ -903,14 +904,22 @@ struct ffh_blob {
                                       " iterations" + (nblobs > 1 ? " (blob " + std::to_string(failed) + ")" : ""));
         return FFH_OK;
     }
-    int run_integrate() {
+    // gated: inside a step graph, after k_step_gate (the kernel returns at once when the gate is 0)
+    int run_integrate(bool gated = false) {
         CgArgs v = cg_args();
         int adv = plan.draws_per_step > 0 ? 1 : 0;
         Mat3u j1 = plan.step_jump[0], j2 = plan.step_jump[1];
+        const int *gt = gated ? step_gate : nullptr;
         return launch(5, [&](hipEvent_t e0, hipEvent_t e1) {
             hipExtLaunchKernelGGL(k_integrate, grid(N), kBlock, 0, stream, e0, e1, 0, v, pos, dt, rng_state, j1, j2, adv,
-                                  massful() ? 1 : 0);
+                                  massful() ? 1 : 0, gt);
         });
+    }
+    int run_step_gate() {
+        hipLaunchKernelGGL(k_step_gate, 1, 64, 0, stream, (const CgCtl *)ctl, nblobs, lumped() ? 0 : 1,
+                           (const int *)(ss_on ? ss_err_dev : nullptr), step_gate);
+        HIP_OR_FAIL(hipGetLastError());
+        return FFH_OK;
     }
     // ---- autotuning of launch shapes (results do not depend on them) ----
     // Best of `reps` separately timed runs after one warm-up run: the minimum is the least
@@ -1090,6 +1099,8 @@ struct ffh_blob {
     // and per-kernel profiling take the ordinary launch path.
     std::vector<std::pair<int, hipGraphExec_t>> graphs;  // by K
     bool use_graphs = true;
+    int *step_gate = nullptr;  // device flag of k_step_gate
+    bool integrated = false;   // the last step's graph has integrated it
     int graph_for(int K, hipGraphExec_t &ge) {
         for (auto &g : graphs)
             if (g.first == K) {
@@ -1107,6 +1118,8 @@ struct ffh_blob {
         CgRun c = cg_setup();
         if (!rc && !lumped()) rc = cg_begin(c);
         if (!rc && !lumped()) rc = cg_iterations(c, K);
+        if (!rc) rc = run_step_gate();
+        if (!rc) rc = run_integrate(true);
         hipError_t e = hipStreamEndCapture(stream, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
@@ -1131,6 +1144,7 @@ struct ffh_blob {
     // One step through a graph replay; returns like the ordinary step (inversion, CG status).
     int step_eager() {
         int rc;
+        integrated = false;
         if ((rc = run_wrap())) return rc;
         if ((rc = run_ssint())) return rc;
         if ((rc = reset_inversions())) return rc;
@@ -1149,12 +1163,21 @@ struct ffh_blob {
             return step_eager();
         }
         HIP_OR_FAIL(hipGraphLaunch(ge, stream));
-        if (lumped()) return lumped_finish();
+        // the graph ends with the gated integration: it ran if the polled state passes the gate
+        auto gate_passed = [&](bool all_done) {
+            return all_done && ctl_host[0].inversions == 0 && !(ss_on && *ss_err_host);
+        };
+        if (lumped()) {
+            rc = lumped_finish();
+            integrated = rc == FFH_OK && gate_passed(true);
+            return rc;
+        }
         CgRun c = cg_setup();
         c.it = K;
         c.updated = mixed_cg();
         bool all_done = false;
         if ((rc = cg_poll(all_done))) return rc;
+        integrated = gate_passed(all_done);
         return cg_run_rest(c, true, all_done);
     }
     int read_inversions(int *ninv) {
@@ -1292,6 +1315,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->upload(b->pinned, P.pinned);
     rc |= b->upload(b->zeta, P.zeta);
     rc |= b->alloc(b->dinv, 3 * (size_t)N);
+    rc |= b->alloc(b->step_gate, 1);
     rc |= b->alloc(b->f, 3 * (size_t)N);
     rc |= b->alloc(b->r, 3 * (size_t)N);
     rc |= b->alloc(b->x, 3 * (size_t)N);
@@ -1478,7 +1502,7 @@ int ffh_blob_step(ffh_blob *b, int nsteps) {
             int ninv = 0;
             return b->read_inversions(&ninv);
         }
-        if ((rc = b->run_integrate())) return rc;
+        if (!b->integrated && (rc = b->run_integrate())) return rc;
         b->steps_done++;
         if (cg) return cg;
     }
@@ -1946,6 +1970,7 @@ int ffh_blob_set_ssint(ffh_blob *b, int kind, int num_faces, const int *face_nod
         return rc;
     }
     *b->ss_err_host = 0;
+    b->ss_err_dev = err_dev;
     a.err = err_dev;
     a.ntypes = num_types;
     a.kind = kind;

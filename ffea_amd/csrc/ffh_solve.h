@@ -542,7 +542,8 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
 // skips it.  Thread 0 advances the stream by one step's draws.
 __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, double dt,
                                                      uint32_t *rng_state, Mat3u j1, Mat3u j2, int advance,
-                                                     int massful) {
+                                                     int massful, const int *gate) {
+    if (gate && *gate == 0) return;  // in a step graph: the host finishes this step itself
     const int n = blockIdx.x * kBlock + threadIdx.x;
     if (n == 0 && advance) {
         uint32_t s[6];
@@ -566,6 +567,18 @@ __global__ __launch_bounds__(kBlock) void k_integrate(CgArgs v, double4 *pos, do
     p.y += x1 * dt;
     p.z += x2 * dt;
     pos[n] = p;
+}
+
+// The step graph integrates only a step that the host will not continue: every blob's solve has
+// stopped (converged or failed), no element has inverted (the reference stops before the
+// position phase, src/World.cpp:2025-2028) and the surface-surface interaction raised no error.
+// The host polls the same control blocks after the graph and decides the same way.
+__global__ void k_step_gate(const CgCtl *ctl, int nblobs, int need_cg, const int *ss_err, int *gate) {
+    if (threadIdx.x != 0) return;
+    int ok = ctl[0].inversions == 0 && (!ss_err || *ss_err == 0);
+    if (need_cg)
+        for (int b = 0; b < nblobs; b++) ok = ok && ctl[b].done != 0;
+    *gate = ok;
 }
 
 __global__ void k_reset_inversions(CgCtl *ctl) {
