@@ -257,6 +257,29 @@ __device__ __forceinline__ bool cg_head(const CgArgs &v, CgCtl *ctl, int it, int
     return true;
 }
 
+// The SpMV visits a slice's w tiles in groups, group(size, first tile, tiles used): eight at a
+// time, the last group clamped to the slice (it loads the last tile again in place of the
+// missing ones, a cache hit); with non-temporal loads the re-read would go to HBM again, so the
+// rest is split exactly into 4, 2 and 1.
+template <bool kNt, typename F>
+__device__ __forceinline__ void spmv_groups(int w, F &&group) {
+    int t = 0;
+    if constexpr (kNt) {
+        for (; t + 8 <= w; t += 8) group(std::integral_constant<int, 8>(), t, 8);
+        if (t + 4 <= w) {
+            group(std::integral_constant<int, 4>(), t, 4);
+            t += 4;
+        }
+        if (t + 2 <= w) {
+            group(std::integral_constant<int, 2>(), t, 2);
+            t += 2;
+        }
+        if (t < w) group(std::integral_constant<int, 1>(), t, 1);
+    } else {
+        for (; t < w; t += 8) group(std::integral_constant<int, 8>(), t, min(8, w - t));
+    }
+}
+
 template <typename T, bool kNt, bool kMass>
 __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
     using V = typename CgVec<T, kMass>::V;
@@ -305,33 +328,33 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             accx += v2 * x[3 * c + 2];
         }
     } else {
-        // kG tiles at a time, loads first, sums in tile order (as in k_cg_spmv1)
-        constexpr int kG = 8;
-        for (int t0 = 0; t0 < w; t0 += kG) {
-            const int n = min(kG, w - t0);
-            int c[kG];
-            T vv[kG][3];
+        // tiles in groups, loads first, sums in tile order (as in k_cg_spmv1)
+        auto group = [&](auto g_, int t0, int n) {
+            constexpr int G = decltype(g_)::value;
+            int c[G];
+            T vv[G][3];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
+            for (int u = 0; u < G; u++) {
                 const int64_t tile = base + (int64_t)min(t0 + u, w - 1) * 64;
                 c[u] = stream_load<kNt>(m.col + tile + lane);
                 const T *vt = val + tile * 9;
 #pragma unroll
                 for (int j = 0; j < 3; j++) vv[u][j] = stream_load<kNt>(vt + tile_entry<T>(3 * ci + j, lane));
             }
-            V zz[kG][3];
+            V zz[G][3];
 #pragma unroll
-            for (int u = 0; u < kG; u++)
+            for (int u = 0; u < G; u++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) zz[u][j] = z[3 * c[u] + j];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
+            for (int u = 0; u < G; u++) {
                 if (u >= n) break;
                 acc += (double)vv[u][0] * (double)zz[u][0];
                 acc += (double)vv[u][1] * (double)zz[u][1];
                 acc += (double)vv[u][2] * (double)zz[u][2];
             }
-        }
+        };
+        spmv_groups<kNt>(w, group);
     }
     const size_t k = 3 * (size_t)row + ci;
     V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
@@ -431,28 +454,26 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             x2 += vv[8] * y2;
         }
     } else {
-        // kG tiles at a time: their column indices and blocks are loaded first, then the gathers
-        // of z, then the sums in tile order -- the order of a one-tile loop, with kG dependent
-        // load pairs in flight.  The last group loads the slice's last tile again in place of
-        // the missing ones and does not add them.
-        constexpr int kG = 8;
-        for (int t0 = 0; t0 < w; t0 += kG) {
-            const int n = min(kG, w - t0);
-            int c[kG];
-            T vv[kG][9];
+        // tiles in groups: their column indices and blocks are loaded first, then the gathers of
+        // z, then the sums in tile order -- the order of a one-tile loop, with a group's
+        // dependent load pairs in flight
+        auto group = [&](auto g_, int t0, int n) {
+            constexpr int G = decltype(g_)::value;
+            int c[G];
+            T vv[G][9];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
+            for (int u = 0; u < G; u++) {
                 const int64_t tile = base + (int64_t)min(t0 + u, w - 1) * 64;
                 c[u] = stream_load<kNt>(m.col + tile + lane);
                 load_block<kNt>(val + tile * 9, lane, vv[u]);
             }
-            V zz[kG][3];
+            V zz[G][3];
 #pragma unroll
-            for (int u = 0; u < kG; u++)
+            for (int u = 0; u < G; u++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) zz[u][j] = z[3 * c[u] + j];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
+            for (int u = 0; u < G; u++) {
                 if (u >= n) break;
                 const double z0 = zz[u][0], z1 = zz[u][1], z2 = zz[u][2];
                 a0 += (double)vv[u][0] * z0;
@@ -465,7 +486,8 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
                 a2 += (double)vv[u][7] * z1;
                 a2 += (double)vv[u][8] * z2;
             }
-        }
+        };
+        spmv_groups<kNt>(w, group);
     }
     const size_t k = 3 * (size_t)row;
     const double acc[3] = {a0, a1, a2}, accx[3] = {x0, x1, x2};
