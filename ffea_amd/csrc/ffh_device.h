@@ -139,6 +139,52 @@ __device__ __forceinline__ double parts_range(const double *__restrict__ parts, 
     return block_sum_all(strided_sum(parts, i0 + (int)threadIdx.x, i1, (int)blockDim.x), sh);
 }
 
+// The totals of two partial arrays over [i0, i1) in one pass and one block tree; each equals
+// parts_range's total of that array bit for bit.  sh: >= 2 blockDim.x/64 doubles.
+__device__ __forceinline__ void strided_sum2(const double *__restrict__ pa, const double *__restrict__ pb, int i, int i1,
+                                             int stride, double &sa, double &sb) {
+    sa = 0.0;
+    sb = 0.0;
+    for (; i + 7 * stride < i1; i += 8 * stride) {
+        double a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            a[k] = pa[i + k * stride];
+            b[k] = pb[i + k * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            sa += a[k];
+            sb += b[k];
+        }
+    }
+    for (; i < i1; i += stride) {
+        sa += pa[i];
+        sb += pb[i];
+    }
+}
+
+__device__ __forceinline__ void parts_range2(const double *pa, const double *pb, int i0, int i1, double *sh, double &ta,
+                                             double &tb) {
+    double sa, sb;
+    strided_sum2(pa, pb, i0 + (int)threadIdx.x, i1, (int)blockDim.x, sa, sb);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    sa = wave_sum(sa);
+    sb = wave_sum(sb);
+    __syncthreads();
+    if (lane == 0) {
+        sh[w] = sa;
+        sh[nw + w] = sb;
+    }
+    __syncthreads();
+    ta = 0.0;
+    tb = 0.0;
+    for (int k = 0; k < nw; k++) {
+        ta += sh[k];
+        tb += sh[nw + k];
+    }
+}
+
 // Per-blob totals.  A blob of many slices would make every consumer block re-add thousands of
 // partials (O(slices^2) work in all); above kTotSlicesMin slices per blob a small kernel adds
 // each blob's partials once (same fixed order) into tot[blob][slot] and consumers read that.
@@ -150,6 +196,16 @@ __device__ __forceinline__ double blob_total(const double *__restrict__ parts, c
                                              int b, int s0, int s1, double *sh) {
     if (tot) return tot[b * kTotSlots + slot];
     return parts_range(parts, s0, s1, sh);
+}
+
+__device__ __forceinline__ void blob_total2(const double *pa, const double *pb, const double *__restrict__ tot, int slot_a,
+                                            int slot_b, int b, int s0, int s1, double *sh, double &ta, double &tb) {
+    if (tot) {
+        ta = tot[b * kTotSlots + slot_a];
+        tb = tot[b * kTotSlots + slot_b];
+        return;
+    }
+    parts_range2(pa, pb, s0, s1, sh, ta, tb);
 }
 
 struct TotArgs {

@@ -99,10 +99,22 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         const T *ef = reinterpret_cast<const T *>(g.eforce);
         const int cnt = g.cnt[n];
         const uint32_t *tl = g.ctb + g.toff[blk] + lane;
-#pragma unroll 4
-        for (int k = 0; k < cnt; k++) {
-            const uint32_t code = tl[64 * k];
-            f += (double)ef[(size_t)(3 * (code & 3u) + i) * g.E + (code >> 2)];
+        // eight terms at a time: their codes, then their element forces, then the additions in
+        // order (a last partial group loads the node's last term again and does not add it)
+        constexpr int kG = 8;
+        for (int k0 = 0; k0 < cnt; k0 += kG) {
+            const int m = min(kG, cnt - k0);
+            uint32_t code[kG];
+            T fe[kG];
+#pragma unroll
+            for (int u = 0; u < kG; u++) code[u] = tl[64 * min(k0 + u, cnt - 1)];
+#pragma unroll
+            for (int u = 0; u < kG; u++) fe[u] = ef[(size_t)(3 * (code[u] & 3u) + i) * g.E + (code[u] >> 2)];
+#pragma unroll
+            for (int u = 0; u < kG; u++) {
+                if (u >= m) break;
+                f += (double)fe[u];
+            }
         }
         // surface face forces, face by face, after the element forces (src/Blob.cpp:3842-3851)
         if (g.fa.grp_off) f = face_groups(g.fa, g.fa.grp_off[n], g.fa.grp_off[n + 1], i, f);
@@ -198,14 +210,14 @@ __device__ __forceinline__ bool cg_stop(double res, int it, const CgArgs &v, boo
 // residual passes the stopping test -- the update kernel then stops only if the true residual
 // passes it too (the final epsilon check of the FP64 path is on the recurrence residual, as
 // NoMassCGSolver::solve's is).
-template <bool kMass, bool kMixed, typename Tot>
+template <bool kMass, bool kMixed, typename Tot, typename Tot2>
 __device__ __forceinline__ bool cg_head(const CgArgs &v, CgCtl *ctl, int it, int mode, bool decider, Tot &&tot,
-                                        double &beta, bool &check) {
+                                        Tot2 &&tot2, double &beta, bool &check) {
     beta = 0.0;
     check = false;
     if (it == 0) {
-        const double f2 = tot(v.parts_ff, kTotFF);
-        const double d0 = tot(v.parts_rz, kTotRZ);
+        double f2, d0;
+        tot2(v.parts_ff, kTotFF, v.parts_rz, kTotRZ, f2, d0);
         const bool stop = cg_stop_first<kMass>(f2, v.eps2);
         if (decider) {
             ctl->f2 = f2;
@@ -220,7 +232,8 @@ __device__ __forceinline__ bool cg_head(const CgArgs &v, CgCtl *ctl, int it, int
         return !stop;
     }
     const double f2 = ctl->f2;
-    const double r2 = tot(v.parts_rr, kTotRR);
+    double r2, dnew;
+    tot2(v.parts_rr, kTotRR, v.parts_rz, kTotRZ, r2, dnew);
     const double res = r2 / f2;
     if constexpr (kMixed) {
         check = res < v.eps2 || (v.rr_period > 0 && it % v.rr_period == 0);
@@ -245,7 +258,6 @@ __device__ __forceinline__ bool cg_head(const CgArgs &v, CgCtl *ctl, int it, int
             return false;
         }
     }
-    const double dnew = tot(v.parts_rz, kTotRZ);
     if (mode == 0)
         beta = dnew / ctl->delta[(it - 1) & 1];
     else if (mode == 1)
@@ -284,7 +296,7 @@ template <typename T, bool kNt, bool kMass>
 __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
     using V = typename CgVec<T, kMass>::V;
     constexpr bool kMixed = CgVec<T, kMass>::kMixed;
-    __shared__ double sh[4];
+    __shared__ double sh[8];  // two totals at a time (parts_range2)
     const int slice = xcd_block(v.nslices);
     if (slice >= v.nslices) return;
     const int b = v.slice_blob[slice];
@@ -294,7 +306,10 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     double beta;
     bool check;
     auto tot = [&](const double *parts, int slot) { return blob_total(parts, v.tot, slot, b, s0, s1, sh); };
-    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, beta, check)) return;
+    auto tot2 = [&](const double *pa, int sa, const double *pb, int sb, double &ta, double &tb) {
+        blob_total2(pa, pb, v.tot, sa, sb, b, s0, s1, sh, ta, tb);
+    };
+    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
@@ -380,6 +395,21 @@ __device__ __forceinline__ double parts_range_as192(const double *__restrict__ p
     return tot;
 }
 
+// parts_range2's totals as parts_range_as192 forms each of them.
+__device__ __forceinline__ void parts_range2_as192(const double *__restrict__ pa, const double *__restrict__ pb, int i0,
+                                                   int i1, double &ta, double &tb) {
+    const int t = threadIdx.x & 63;
+    ta = 0.0;
+    tb = 0.0;
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        double sa, sb;
+        strided_sum2(pa, pb, i0 + t + 64 * w, i1, 192, sa, sb);
+        ta += __shfl(wave_sum(sa), 0, 64);
+        tb += __shfl(wave_sum(sb), 0, 64);
+    }
+}
+
 __device__ __forceinline__ double blob_total_as192(const double *parts, const double *tot, int slot, int b, int s0, int s1) {
     if (tot) return tot[b * kTotSlots + slot];
     return parts_range_as192(parts, s0, s1);
@@ -402,7 +432,15 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
     double beta;
     bool check;
     auto tot = [&](const double *parts, int slot) { return blob_total_as192(parts, v.tot, slot, b, s0, s1); };
-    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, beta, check)) return;
+    auto tot2 = [&](const double *pa, int sa, const double *pb, int sb, double &ta, double &tb) {
+        if (v.tot) {
+            ta = v.tot[b * kTotSlots + sa];
+            tb = v.tot[b * kTotSlots + sb];
+        } else {
+            parts_range2_as192(pa, pb, s0, s1, ta, tb);
+        }
+    };
+    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
     const int lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
