@@ -153,29 +153,26 @@ __global__ __launch_bounds__(256) void k_mpnn_embed(MpnnArgs a, const U *__restr
 }
 
 // acc[k] += A_rt h[c_t][k] over row rloc's slots t in order (k = 0..7, the 8 bf16 channels at
-// src + c ld): the slots' column indices and couplings are loaded eight at a time, then their
-// eight 16-byte gathers, then the sums in slot order -- the order of a one-slot loop, with eight
-// dependent load pairs in flight.  The last group loads the row's last slot again in place of the
-// missing ones and does not add them.
+// src + c ld): the slots' column indices and couplings are loaded up to eight at a time, then
+// their 16-byte gathers, then the sums in slot order -- the order of a one-slot loop, with a
+// group's dependent load pairs in flight.
 __device__ __forceinline__ void message_rows(const MpnnArgs &a, int64_t base, int w, int rloc,
                                              const bf16_t *__restrict__ src, float acc[8], int ld = kHid) {
-    constexpr int kG = 8;
-    for (int t0 = 0; t0 < w; t0 += kG) {
-        const int n = min(kG, w - t0);
-        int c[kG];
-        float wg[kG];
+    exact_groups(0, w, [&](auto g_, int t0) {
+        constexpr int G = decltype(g_)::value;
+        int c[G];
+        float wg[G];
 #pragma unroll
-        for (int u = 0; u < kG; u++) {
-            const int64_t slot = base + (int64_t)min(t0 + u, w - 1) * 64 + rloc;
+        for (int u = 0; u < G; u++) {
+            const int64_t slot = base + (int64_t)(t0 + u) * 64 + rloc;
             c[u] = a.col[slot];
             wg[u] = a.aw[slot];
         }
-        uint4 v[kG];
+        uint4 v[G];
 #pragma unroll
-        for (int u = 0; u < kG; u++) v[u] = *reinterpret_cast<const uint4 *>(src + (size_t)c[u] * ld);
+        for (int u = 0; u < G; u++) v[u] = *reinterpret_cast<const uint4 *>(src + (size_t)c[u] * ld);
 #pragma unroll
-        for (int u = 0; u < kG; u++) {
-            if (u >= n) break;
+        for (int u = 0; u < G; u++) {
             const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -183,7 +180,7 @@ __device__ __forceinline__ void message_rows(const MpnnArgs &a, int64_t base, in
                 acc[2 * k + 1] += wg[u] * bf2f((bf16_t)(vv[k] >> 16));
             }
         }
-    }
+    });
 }
 
 // g = A h (one slice per block, 4 threads per row, 8 channels each).  Input [N][ld] of TI,
@@ -202,27 +199,25 @@ __global__ __launch_bounds__(256) void k_mpnn_spmm(MpnnArgs a, const TI *__restr
     if constexpr (sizeof(TI) == 2) {
         message_rows(a, base, w, rloc, reinterpret_cast<const bf16_t *>(in) + off + 8 * q, acc, ld);
     } else {
-        constexpr int kG = 8;
-        for (int t0 = 0; t0 < w; t0 += kG) {
-            const int n = min(kG, w - t0);
-            int c[kG];
-            float wg[kG];
+        exact_groups(0, w, [&](auto g_, int t0) {
+            constexpr int G = decltype(g_)::value;
+            int c[G];
+            float wg[G];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
-                const int64_t slot = base + (int64_t)min(t0 + u, w - 1) * 64 + rloc;
+            for (int u = 0; u < G; u++) {
+                const int64_t slot = base + (int64_t)(t0 + u) * 64 + rloc;
                 c[u] = a.col[slot];
                 wg[u] = a.aw[slot];
             }
-            float4 v0[kG], v1[kG];
+            float4 v0[G], v1[G];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
+            for (int u = 0; u < G; u++) {
                 const TI *src = in + (size_t)c[u] * ld + off + 8 * q;
                 v0[u] = *reinterpret_cast<const float4 *>(src);
                 v1[u] = *reinterpret_cast<const float4 *>(src + 4);
             }
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
-                if (u >= n) break;
+            for (int u = 0; u < G; u++) {
                 acc[0] += wg[u] * v0[u].x;
                 acc[1] += wg[u] * v0[u].y;
                 acc[2] += wg[u] * v0[u].z;
@@ -232,7 +227,7 @@ __global__ __launch_bounds__(256) void k_mpnn_spmm(MpnnArgs a, const TI *__restr
                 acc[6] += wg[u] * v1[u].z;
                 acc[7] += wg[u] * v1[u].w;
             }
-        }
+        });
     }
     if constexpr (kBwd) {
         const float *own = reinterpret_cast<const float *>(in) + (size_t)row * ld + 8 * q;

@@ -99,23 +99,19 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         const T *ef = reinterpret_cast<const T *>(g.eforce);
         const int cnt = g.cnt[n];
         const uint32_t *tl = g.ctb + g.toff[blk] + lane;
-        // eight terms at a time: their codes, then their element forces, then the additions in
-        // order (a last partial group loads the node's last term again and does not add it)
-        constexpr int kG = 8;
-        for (int k0 = 0; k0 < cnt; k0 += kG) {
-            const int m = min(kG, cnt - k0);
-            uint32_t code[kG];
-            T fe[kG];
+        // terms in groups of up to eight: their codes, then their element forces, then the
+        // additions in order
+        exact_groups(0, cnt, [&](auto g_, int k0) {
+            constexpr int G = decltype(g_)::value;
+            uint32_t code[G];
+            T fe[G];
 #pragma unroll
-            for (int u = 0; u < kG; u++) code[u] = tl[64 * min(k0 + u, cnt - 1)];
+            for (int u = 0; u < G; u++) code[u] = tl[64 * (k0 + u)];
 #pragma unroll
-            for (int u = 0; u < kG; u++) fe[u] = ef[(size_t)(3 * (code[u] & 3u) + i) * g.E + (code[u] >> 2)];
+            for (int u = 0; u < G; u++) fe[u] = ef[(size_t)(3 * (code[u] & 3u) + i) * g.E + (code[u] >> 2)];
 #pragma unroll
-            for (int u = 0; u < kG; u++) {
-                if (u >= m) break;
-                f += (double)fe[u];
-            }
-        }
+            for (int u = 0; u < G; u++) f += (double)fe[u];
+        });
         // surface face forces, face by face, after the element forces (src/Blob.cpp:3842-3851)
         if (g.fa.grp_off) f = face_groups(g.fa, g.fa.grp_off[n], g.fa.grp_off[n + 1], i, f);
         // external forces come after the element forces and before the Stokes noise
@@ -269,29 +265,6 @@ __device__ __forceinline__ bool cg_head(const CgArgs &v, CgCtl *ctl, int it, int
     return true;
 }
 
-// The SpMV visits a slice's w tiles in groups, group(size, first tile, tiles used): eight at a
-// time, the last group clamped to the slice (it loads the last tile again in place of the
-// missing ones, a cache hit); with non-temporal loads the re-read would go to HBM again, so the
-// rest is split exactly into 4, 2 and 1.
-template <bool kNt, typename F>
-__device__ __forceinline__ void spmv_groups(int w, F &&group) {
-    int t = 0;
-    if constexpr (kNt) {
-        for (; t + 8 <= w; t += 8) group(std::integral_constant<int, 8>(), t, 8);
-        if (t + 4 <= w) {
-            group(std::integral_constant<int, 4>(), t, 4);
-            t += 4;
-        }
-        if (t + 2 <= w) {
-            group(std::integral_constant<int, 2>(), t, 2);
-            t += 2;
-        }
-        if (t < w) group(std::integral_constant<int, 1>(), t, 1);
-    } else {
-        for (; t < w; t += 8) group(std::integral_constant<int, 8>(), t, min(8, w - t));
-    }
-}
-
 template <typename T, bool kNt, bool kMass>
 __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, int mode) {
     using V = typename CgVec<T, kMass>::V;
@@ -344,13 +317,13 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
         }
     } else {
         // tiles in groups, loads first, sums in tile order (as in k_cg_spmv1)
-        auto group = [&](auto g_, int t0, int n) {
+        auto group = [&](auto g_, int t0) {
             constexpr int G = decltype(g_)::value;
             int c[G];
             T vv[G][3];
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                const int64_t tile = base + (int64_t)min(t0 + u, w - 1) * 64;
+                const int64_t tile = base + (int64_t)(t0 + u) * 64;
                 c[u] = stream_load<kNt>(m.col + tile + lane);
                 const T *vt = val + tile * 9;
 #pragma unroll
@@ -363,13 +336,12 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
                 for (int j = 0; j < 3; j++) zz[u][j] = z[3 * c[u] + j];
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                if (u >= n) break;
                 acc += (double)vv[u][0] * (double)zz[u][0];
                 acc += (double)vv[u][1] * (double)zz[u][1];
                 acc += (double)vv[u][2] * (double)zz[u][2];
             }
         };
-        spmv_groups<kNt>(w, group);
+        exact_groups(0, w, group);
     }
     const size_t k = 3 * (size_t)row + ci;
     V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
@@ -495,13 +467,13 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
         // tiles in groups: their column indices and blocks are loaded first, then the gathers of
         // z, then the sums in tile order -- the order of a one-tile loop, with a group's
         // dependent load pairs in flight
-        auto group = [&](auto g_, int t0, int n) {
+        auto group = [&](auto g_, int t0) {
             constexpr int G = decltype(g_)::value;
             int c[G];
             T vv[G][9];
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                const int64_t tile = base + (int64_t)min(t0 + u, w - 1) * 64;
+                const int64_t tile = base + (int64_t)(t0 + u) * 64;
                 c[u] = stream_load<kNt>(m.col + tile + lane);
                 load_block<kNt>(val + tile * 9, lane, vv[u]);
             }
@@ -512,7 +484,6 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
                 for (int j = 0; j < 3; j++) zz[u][j] = z[3 * c[u] + j];
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                if (u >= n) break;
                 const double z0 = zz[u][0], z1 = zz[u][1], z2 = zz[u][2];
                 a0 += (double)vv[u][0] * z0;
                 a0 += (double)vv[u][1] * z1;
@@ -525,7 +496,7 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
                 a2 += (double)vv[u][8] * z2;
             }
         };
-        spmv_groups<kNt>(w, group);
+        exact_groups(0, w, group);
     }
     const size_t k = 3 * (size_t)row;
     const double acc[3] = {a0, a1, a2}, accx[3] = {x0, x1, x2};
