@@ -299,22 +299,39 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             const int c = stream_load<kNt>(m.col + tile + lane);
             acc += stream_load<kNt>(mv + tile + lane) * z[3 * c + ci];
         }
-    } else if (kMixed && check) {  // K z and K x from one pass over the matrix
-#pragma unroll 4
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            const T *vt = val + tile * 9;
-            const double v0 = stream_load<kNt>(vt + tile_entry<T>(3 * ci, lane)),
-                         v1 = stream_load<kNt>(vt + tile_entry<T>(3 * ci + 1, lane)),
-                         v2 = stream_load<kNt>(vt + tile_entry<T>(3 * ci + 2, lane));
-            acc += v0 * (double)z[3 * c + 0];
-            acc += v1 * (double)z[3 * c + 1];
-            acc += v2 * (double)z[3 * c + 2];
-            accx += v0 * x[3 * c + 0];
-            accx += v1 * x[3 * c + 1];
-            accx += v2 * x[3 * c + 2];
-        }
+    } else if (kMixed && check) {  // K z and K x from one pass over the matrix, tiles in groups
+        exact_groups(0, w, [&](auto g_, int t0) {
+            constexpr int G = decltype(g_)::value;
+            int c[G];
+            T vv[G][3];
+#pragma unroll
+            for (int u = 0; u < G; u++) {
+                const int64_t tile = base + (int64_t)(t0 + u) * 64;
+                c[u] = stream_load<kNt>(m.col + tile + lane);
+                const T *vt = val + tile * 9;
+#pragma unroll
+                for (int j = 0; j < 3; j++) vv[u][j] = stream_load<kNt>(vt + tile_entry<T>(3 * ci + j, lane));
+            }
+            V zz[G][3];
+            double xx[G][3];
+#pragma unroll
+            for (int u = 0; u < G; u++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    zz[u][j] = z[3 * c[u] + j];
+                    xx[u][j] = x[3 * c[u] + j];
+                }
+#pragma unroll
+            for (int u = 0; u < G; u++) {
+                const double v0 = vv[u][0], v1 = vv[u][1], v2 = vv[u][2];
+                acc += v0 * (double)zz[u][0];
+                acc += v1 * (double)zz[u][1];
+                acc += v2 * (double)zz[u][2];
+                accx += v0 * xx[u][0];
+                accx += v1 * xx[u][1];
+                accx += v2 * xx[u][2];
+            }
+        });
     } else {
         // tiles in groups, loads first, sums in tile order (as in k_cg_spmv1)
         auto group = [&](auto g_, int t0) {
@@ -432,37 +449,53 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             a1 += a * z[3 * c + 1];
             a2 += a * z[3 * c + 2];
         }
-    } else if (kMixed && check) {  // K z and K x from one pass over the matrix
-#pragma unroll 2
-        for (int t = 0; t < w; t++) {
-            const int64_t tile = base + (int64_t)t * 64;
-            const int c = stream_load<kNt>(m.col + tile + lane);
-            T vt[9];
-            load_block<kNt>(val + tile * 9, lane, vt);
-            double vv[9];
+    } else if (kMixed && check) {  // K z and K x from one pass over the matrix, tiles in groups
+        exact_groups(0, w, [&](auto g_, int t0) {
+            constexpr int G = decltype(g_)::value;
+            int c[G];
+            T vt[G][9];
 #pragma unroll
-            for (int kk = 0; kk < 9; kk++) vv[kk] = (double)vt[kk];
-            const double z0 = z[3 * c + 0], z1 = z[3 * c + 1], z2 = z[3 * c + 2];
-            const double y0 = x[3 * c + 0], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
-            a0 += vv[0] * z0;
-            a0 += vv[1] * z1;
-            a0 += vv[2] * z2;
-            a1 += vv[3] * z0;
-            a1 += vv[4] * z1;
-            a1 += vv[5] * z2;
-            a2 += vv[6] * z0;
-            a2 += vv[7] * z1;
-            a2 += vv[8] * z2;
-            x0 += vv[0] * y0;
-            x0 += vv[1] * y1;
-            x0 += vv[2] * y2;
-            x1 += vv[3] * y0;
-            x1 += vv[4] * y1;
-            x1 += vv[5] * y2;
-            x2 += vv[6] * y0;
-            x2 += vv[7] * y1;
-            x2 += vv[8] * y2;
-        }
+            for (int u = 0; u < G; u++) {
+                const int64_t tile = base + (int64_t)(t0 + u) * 64;
+                c[u] = stream_load<kNt>(m.col + tile + lane);
+                load_block<kNt>(val + tile * 9, lane, vt[u]);
+            }
+            V zz[G][3];
+            double xx[G][3];
+#pragma unroll
+            for (int u = 0; u < G; u++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    zz[u][j] = z[3 * c[u] + j];
+                    xx[u][j] = x[3 * c[u] + j];
+                }
+#pragma unroll
+            for (int u = 0; u < G; u++) {
+                double vv[9];
+#pragma unroll
+                for (int kk = 0; kk < 9; kk++) vv[kk] = (double)vt[u][kk];
+                const double z0 = zz[u][0], z1 = zz[u][1], z2 = zz[u][2];
+                const double y0 = xx[u][0], y1 = xx[u][1], y2 = xx[u][2];
+                a0 += vv[0] * z0;
+                a0 += vv[1] * z1;
+                a0 += vv[2] * z2;
+                a1 += vv[3] * z0;
+                a1 += vv[4] * z1;
+                a1 += vv[5] * z2;
+                a2 += vv[6] * z0;
+                a2 += vv[7] * z1;
+                a2 += vv[8] * z2;
+                x0 += vv[0] * y0;
+                x0 += vv[1] * y1;
+                x0 += vv[2] * y2;
+                x1 += vv[3] * y0;
+                x1 += vv[4] * y1;
+                x1 += vv[5] * y2;
+                x2 += vv[6] * y0;
+                x2 += vv[7] * y1;
+                x2 += vv[8] * y2;
+            }
+        });
     } else {
         // tiles in groups: their column indices and blocks are loaded first, then the gathers of
         // z, then the sums in tile order -- the order of a one-tile loop, with a group's
