@@ -109,8 +109,9 @@ def algorithmic_bytes(kernel, sizes, bm, bv=8):
     v = 24 * N  # one FP64 3-vector over the linear nodes
     w = 3 * bv * N  # one work vector
     return {
-        # enodes 16, J0^-1 72, vol0 8, A B G E 32, last_det r+w 16; writes the 12 nodal forces
-        "element": E * (16 + 72 + 8 + 32 + 16 + 12 * bm) + v,
+        # enodes 16, J0^-1, vol0, A B G E (14 constants: FP64, or their FP32 copies on the mixed
+        # path), last_det r+w 16; writes the 12 nodal forces
+        "element": E * (16 + 14 * bm + 16 + 12 * bm) + v,
         # per slice its element list (4 B per slice-element pair, L pairs), enodes + A, B once
         # (32E), contribution codes (2 B per contribution, C of them: the diagonal and upper
         # blocks' element terms), per summed block (N diagonal + (B - N) / 2 upper) its count,
@@ -839,7 +840,8 @@ def cg_phase(b, steps, kind):
     return {"cg_ms_per_solve": cg_ms, "ms_per_step_with_events": 1e3 * wall,
             "iterations_mean_per_blob": float(its.mean()), "iterations_max": int(its.max()),
             "launched_iterations_mean": float(its.max(axis=1).mean()),
-            "kernels_us": {k: v[0] / v[1] * 1e3 for k, v in kt.items() if v[1]}}
+            "kernels_us": {k: v[0] / v[1] * 1e3 for k, v in kt.items() if v[1]},
+            "launches_per_step": {k: v[1] / steps for k, v in kt.items() if v[1]}}
 
 
 def kappa_jacobi(mesh, x0, prm, sc, dims):

@@ -119,26 +119,26 @@ __device__ __forceinline__ void block_partial_at(double v, double *__restrict__ 
     if (threadIdx.x == 0) parts[idx] = t;
 }
 
-// Visits [t0, t1) in groups: as many of exactly eight as fit, then one group of the remaining
-// 1-7 -- group(std::integral_constant<int, G>(), first).  Every index a group touches is in
-// range and used, so the compiler issues a group's loads together ahead of their uses (with a
+// Visits [t0, t1) in groups: as many of exactly kMax as fit, then one group of the remaining
+// 1 .. kMax-1 -- group(std::integral_constant<int, G>(), first).  Every index a group touches is
+// in range and used, so the compiler issues a group's loads together ahead of their uses (with a
 // clamped last group whose extra members are skipped by a branch, it sinks each load to its use
 // and the loads go out one at a time), and a run of 15 -- a row of the tetrahedral meshes --
 // takes two dependent rounds of loads, not four.
-template <typename F>
+template <int R, typename F>
+__device__ __forceinline__ void rest_group(int r, int t, F &&group) {
+    if constexpr (R > 0) {
+        if (r == R)
+            group(std::integral_constant<int, R>(), t);
+        else
+            rest_group<R - 1>(r, t, group);
+    }
+}
+template <int kMax = 8, typename F>
 __device__ __forceinline__ void exact_groups(int t0, int t1, F &&group) {
     int t = t0;
-    for (; t + 8 <= t1; t += 8) group(std::integral_constant<int, 8>(), t);
-    switch (t1 - t) {
-        case 7: group(std::integral_constant<int, 7>(), t); break;
-        case 6: group(std::integral_constant<int, 6>(), t); break;
-        case 5: group(std::integral_constant<int, 5>(), t); break;
-        case 4: group(std::integral_constant<int, 4>(), t); break;
-        case 3: group(std::integral_constant<int, 3>(), t); break;
-        case 2: group(std::integral_constant<int, 2>(), t); break;
-        case 1: group(std::integral_constant<int, 1>(), t); break;
-        default: break;
-    }
+    for (; t + kMax <= t1; t += kMax) group(std::integral_constant<int, kMax>(), t);
+    rest_group<kMax - 1>(t1 - t, t, group);
 }
 
 // s = ((0 + p[i]) + p[i + stride]) + ... over [i, i1): a fixed order, with eight loads in

@@ -11,6 +11,9 @@ struct ElemArgs {
     const int4 *enodes;
     const double *jinv0;  // SoA [9][E]
     const double *vol0, *A, *B, *G, *Em;
+    // the same per-element constants rounded to float, SoA [14][E] (jinv0 0-8, vol0, A, B, G,
+    // Em): what the FP32 element kernel reads (half the bytes); nullptr on the FP64 path
+    const float *elem32;
     double *last_det;
     void *eforce;  // T SoA [12][E]: node_force[a][i] at (3a+i)*E + e
     int E;
@@ -19,6 +22,7 @@ struct ElemArgs {
     const uint32_t *rng_state;
     const Mat3u *jump;
     uint32_t elem_base;
+    const uint32_t *wave_state;  // [6 per 64 elements]: the stream at each wave's first element
     CgCtl *ctl;
     const double *vel;  // [3N] node velocities (mass-ful solvers: the viscous force K_e u_e)
     double *stress_mag;  // [E] internal_stress_mag per element, or nullptr (not computed)
@@ -94,6 +98,20 @@ __device__ __forceinline__ void element_geometry(const double4 *__restrict__ pos
     element_geometry_p<R>(pos[n.x], pos[n.y], pos[n.z], pos[n.w], J, dpsi, det);
 }
 
+// The stream state at the first element of every 64-element wave, one thread per wave: the
+// element kernel's lanes step on from it with the A^(7 lane) table, so no wave evaluates the
+// four-digit jump itself (as uniform scalar code that jump had been a third of the element
+// kernel's time at c1: the scalar unit is one per compute unit).
+__global__ __launch_bounds__(kBlock) void k_elem_wave_states(const uint32_t *__restrict__ rng_state,
+                                                            const Mat3u *__restrict__ jump, uint32_t elem_base,
+                                                            int nwaves, uint32_t *__restrict__ out) {
+    const int w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= nwaves) return;
+    const Stream6 st = rng_jump(rng_state, jump, elem_base + 7u * 64u * (uint32_t)w);
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[6 * (size_t)w + k] = st.s[k];
+}
+
 template <typename R, typename T, bool kMass>
 __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     const int bd = (int)blockDim.x;  // 64, 128 or 256 (tuned); waves stay 64-aligned
@@ -106,7 +124,26 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     g.last_det[e] = (double)det;
     const R vol = (R)(1.0 / 6.0) * fabs(det);
 
-    const R A = (R)g.A[e], B = (R)g.B[e], G = (R)g.G[e], Em = (R)g.Em[e], vol0 = (R)g.vol0[e];
+    R A, B, G, Em, vol0, Ji0[9];  // Ji0[3 j + k] = jinv0[j][k]
+    if constexpr (sizeof(R) == 4) {
+        const float *c = g.elem32 + e;
+        const size_t E = (size_t)g.E;
+#pragma unroll
+        for (int q = 0; q < 9; q++) Ji0[q] = c[q * E];
+        vol0 = c[9 * E];
+        A = c[10 * E];
+        B = c[11 * E];
+        G = c[12 * E];
+        Em = c[13 * E];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) Ji0[q] = (R)g.jinv0[(size_t)q * g.E + e];
+        vol0 = (R)g.vol0[e];
+        A = (R)g.A[e];
+        B = (R)g.B[e];
+        G = (R)g.G[e];
+        Em = (R)g.Em[e];
+    }
 
     // add_shear_elastic_stress (:378-396): F_ij = sum_k J[k][i] Jinv0[j][k]; S = F F^T
     R F[3][3], S[3][3];
@@ -116,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
         for (int j = 0; j < 3; j++) {
             R f = (R)0;
 #pragma unroll
-            for (int k = 0; k < 3; k++) f += J[k][i] * (R)g.jinv0[(size_t)(3 * j + k) * g.E + e];
+            for (int k = 0; k < 3; k++) f += J[k][i] * Ji0[3 * j + k];
             F[i][j] = f;
         }
 #pragma unroll
@@ -146,10 +183,12 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     }
     // add_fluctuating_stress (:417-436): 7 draws at this element's stream offset
     if (g.calc_noise) {
-        // the wave's first element's state (uniform), then A^(7 lane) for this lane
+        // the wave's first element's state (k_elem_wave_states), then A^(7 lane) for this lane
         const int lane = threadIdx.x & 63;
-        const uint32_t e0 = __builtin_amdgcn_readfirstlane((uint32_t)(e - lane));
-        Stream6 st = rng_jump(g.rng_state, g.jump, g.elem_base + 7u * e0);
+        const uint32_t w = __builtin_amdgcn_readfirstlane((uint32_t)(e >> 6));
+        Stream6 st;
+#pragma unroll
+        for (int k = 0; k < 6; k++) st.s[k] = g.wave_state[6 * w + k];
         rng_apply(st, g.jump[kLaneTable + 2 * lane], g.jump[kLaneTable + 2 * lane + 1]);
         const R c = sqrt(((R)24 * (R)g.kT) / (vol * (R)g.dt));
         const R bf = sqrt(B) * (R)rand_half(st);

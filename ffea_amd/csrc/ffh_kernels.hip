@@ -104,6 +104,7 @@ struct ffh_blob {
     int4 *enodes = nullptr;
     double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
            *last_det = nullptr;
+    float *elem32 = nullptr;  // FP32 element constants (mixed precision only)
     void *eforce = nullptr, *val = nullptr;
     uint32_t *nctb = nullptr, *rng_off = nullptr;
     int64_t *nt_off = nullptr;
@@ -229,6 +230,7 @@ struct ffh_blob {
     CgCtl *ctl_host = nullptr;  // pinned host mirror
     uint32_t *rng_state = nullptr;
     Mat3u *jump = nullptr;
+    uint32_t *wave_state = nullptr;  // [6 per 64 elements], rewritten before every element pass
     int P_slice = 0, P_elem = 0;
     // profiling
     bool profiling = false;
@@ -408,6 +410,7 @@ struct ffh_blob {
         g.B = B;
         g.G = G;
         g.Em = Em;
+        g.elem32 = elem32;
         g.last_det = last_det;
         g.eforce = eforce;
         g.E = E;
@@ -417,6 +420,7 @@ struct ffh_blob {
         g.rng_state = rng_state;
         g.jump = jump;
         g.elem_base = 0;
+        g.wave_state = wave_state;
         g.ctl = ctl;
         g.vel = vel;
         g.stress_mag = stress_mag;
@@ -427,6 +431,12 @@ struct ffh_blob {
     int run_element() {
         ElemArgs g = elem_args();
         const int gr = xcd_grid(grid(E, elem_block));
+        if (calc_noise) {
+            const int nw = (E + 63) / 64;
+            hipLaunchKernelGGL(k_elem_wave_states, grid(nw), kBlock, 0, stream, (const uint32_t *)rng_state,
+                               (const Mat3u *)jump, g.elem_base, nw, wave_state);
+            HIP_OR_FAIL(hipGetLastError());
+        }
         return launch(0, [&](hipEvent_t e0, hipEvent_t e1) {
             if (precision == FFH_PRECISION_FP64) {
                 if (massful())
@@ -1281,6 +1291,14 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->upload(b->B, P.matB);
     rc |= b->upload(b->G, P.matG);
     rc |= b->upload(b->Em, P.matE);
+    if (b->tsize == 4) {
+        std::vector<float> c32(14 * (size_t)E);
+        const std::vector<double> *cols[5] = {&P.vol0, &P.matA, &P.matB, &P.matG, &P.matE};
+        for (size_t i = 0; i < 9 * (size_t)E; i++) c32[i] = (float)P.jinv0[i];
+        for (int r = 0; r < 5; r++)
+            for (int e = 0; e < E; e++) c32[(9 + r) * (size_t)E + e] = (float)(*cols[r])[e];
+        rc |= b->upload(b->elem32, c32);
+    }
     rc |= b->upload(b->last_det, P.last_det);
     {
         char *ef = nullptr, *vv = nullptr;
@@ -1331,6 +1349,7 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
     rc |= b->alloc(b->ctl, (size_t)b->nblobs);
     rc |= b->alloc(b->tot, (size_t)b->nblobs * kTotSlots);
     rc |= b->alloc(b->rng_state, 6);
+    rc |= b->alloc(b->wave_state, 6 * (((size_t)E + 63) / 64) + 6);
     rc |= b->upload(b->jump, P.jump);
     if (rc) {
         if (g_err.empty()) fail(FFH_E_HIP, "device allocation or upload failed");

@@ -282,11 +282,17 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     auto tot2 = [&](const double *pa, int sa, const double *pb, int sb, double &ta, double &tb) {
         blob_total2(pa, pb, v.tot, sa, sb, b, s0, s1, sh, ta, tb);
     };
-    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
+    // the slice's shape and this row's z, p, q (and f for a true-residual pass) are loaded
+    // before the head, so they arrive while it reduces
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
     const int64_t base = m.slice_off[slice];
+    const size_t k = 3 * (size_t)row + ci;
+    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
+    const double zk = (double)static_cast<const V *>(v.z)[k], pk = (double)p[k], qk = (double)q[k];
+    const double fk = kMixed ? v.f[k] : 0.0;
+    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
     const T *val = reinterpret_cast<const T *>(m.val);
     const V *__restrict__ z = static_cast<const V *>(v.z);
     const double *__restrict__ x = v.x;
@@ -300,7 +306,7 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
             acc += stream_load<kNt>(mv + tile + lane) * z[3 * c + ci];
         }
     } else if (kMixed && check) {  // K z and K x from one pass over the matrix, tiles in groups
-        exact_groups(0, w, [&](auto g_, int t0) {
+        exact_groups<4>(0, w, [&](auto g_, int t0) {
             constexpr int G = decltype(g_)::value;
             int c[G];
             T vv[G][3];
@@ -360,15 +366,13 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
         };
         exact_groups(0, w, group);
     }
-    const size_t k = 3 * (size_t)row + ci;
-    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
-    const double pn = (double)(V)((double)z[k] + beta * (double)p[k]);
-    const double qn = (double)(V)(acc + beta * (double)q[k]);
+    const double pn = (double)(V)(zk + beta * pk);
+    const double qn = (double)(V)(acc + beta * qk);
     p[k] = (V)pn;
     q[k] = (V)qn;
     block_partial_at(pn * qn, v.parts_pq, slice, sh);
     if (kMixed && check) {
-        const double rt = v.f[k] - accx;
+        const double rt = fk - accx;
         static_cast<V *>(v.r)[k] = (V)rt;
         block_partial_at(rt * rt, v.parts_tr, slice, sh);
     }
@@ -384,7 +388,6 @@ __device__ __forceinline__ double parts_range_as192(const double *__restrict__ p
     return tot;
 }
 
-// parts_range2's totals as parts_range_as192 forms each of them.
 __device__ __forceinline__ void parts_range2_as192(const double *__restrict__ pa, const double *__restrict__ pb, int i0,
                                                    int i1, double &ta, double &tb) {
     const int t = threadIdx.x & 63;
@@ -429,11 +432,23 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             parts_range2_as192(pa, pb, s0, s1, ta, tb);
         }
     };
-    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
+    // the slice's shape and this row's z, p, q (and f for a true-residual pass) are loaded
+    // before the head, so they arrive while it reduces
     const int lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
     const int64_t base = m.slice_off[slice];
+    const size_t k = 3 * (size_t)row;
+    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
+    double zk[3], pk[3], qk[3], fk[3];
+#pragma unroll
+    for (int ci = 0; ci < 3; ci++) {
+        zk[ci] = (double)static_cast<const V *>(v.z)[k + ci];
+        pk[ci] = (double)p[k + ci];
+        qk[ci] = (double)q[k + ci];
+        fk[ci] = kMixed ? v.f[k + ci] : 0.0;
+    }
+    if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
     const T *val = reinterpret_cast<const T *>(m.val);
     const V *__restrict__ z = static_cast<const V *>(v.z);
     const double *__restrict__ x = v.x;
@@ -450,7 +465,7 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             a2 += a * z[3 * c + 2];
         }
     } else if (kMixed && check) {  // K z and K x from one pass over the matrix, tiles in groups
-        exact_groups(0, w, [&](auto g_, int t0) {
+        exact_groups<4>(0, w, [&](auto g_, int t0) {
             constexpr int G = decltype(g_)::value;
             int c[G];
             T vt[G][9];
@@ -531,19 +546,17 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
         };
         exact_groups(0, w, group);
     }
-    const size_t k = 3 * (size_t)row;
     const double acc[3] = {a0, a1, a2}, accx[3] = {x0, x1, x2};
-    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
     double part = 0.0, tpart = 0.0;
 #pragma unroll
     for (int ci = 0; ci < 3; ci++) {
-        const double pn = (double)(V)((double)z[k + ci] + beta * (double)p[k + ci]);
-        const double qn = (double)(V)(acc[ci] + beta * (double)q[k + ci]);
+        const double pn = (double)(V)(zk[ci] + beta * pk[ci]);
+        const double qn = (double)(V)(acc[ci] + beta * qk[ci]);
         p[k + ci] = (V)pn;
         q[k + ci] = (V)qn;
         part += __shfl(wave_sum(pn * qn), 0, 64);  // ((0 + W0) + W1) + W2, as the 3-wave block
         if (kMixed && check) {
-            const double rt = v.f[k + ci] - accx[ci];
+            const double rt = fk[ci] - accx[ci];
             static_cast<V *>(v.r)[k + ci] = (V)rt;
             tpart += __shfl(wave_sum(rt * rt), 0, 64);
         }
@@ -567,6 +580,10 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     if (slice >= v.nslices) return;
     const int b = v.slice_blob[slice];
     CgCtl *ctl = &v.ctl[b];
+    // this row's x, p, r, q and D^-1 are loaded before the head, so they arrive while it reduces
+    const size_t k = (size_t)slice * 192 + threadIdx.x;
+    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q), *r = static_cast<V *>(v.r), *z = static_cast<V *>(v.z);
+    const double xk = v.x[k], pk = (double)p[k], r0 = (double)r[k], qk = (double)q[k], dk = v.dinv[k];
     if (ctl->done) return;
     const int s0 = v.blob_slice_off[b], s1 = v.blob_slice_off[b + 1];
     if constexpr (kMixed) {
@@ -585,13 +602,10 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
         }
     }
     const double alpha = ctl->delta[it & 1] / blob_total(v.parts_pq, v.tot, kTotPQ, b, s0, s1, sh);
-    const size_t k = (size_t)slice * 192 + threadIdx.x;
-    V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q), *r = static_cast<V *>(v.r), *z = static_cast<V *>(v.z);
-    v.x[k] += alpha * (double)p[k];
-    const double r0 = (double)r[k];
-    const double rn = (double)(V)(r0 + (-alpha) * (double)q[k]);
+    v.x[k] = xk + alpha * pk;
+    const double rn = (double)(V)(r0 + (-alpha) * qk);
     r[k] = (V)rn;
-    const double zn = (double)(V)(v.dinv[k] * rn);
+    const double zn = (double)(V)(dk * rn);
     z[k] = (V)zn;
     block_partial_at(rn * rn, v.parts_rr, slice, sh);
     block_partial_at(rn * zn, v.parts_rz, slice, sh);
