@@ -110,8 +110,8 @@ def algorithmic_bytes(kernel, sizes, bm, bv=8):
     w = 3 * bv * N  # one work vector
     return {
         # enodes 16, J0^-1, vol0, A B G E (14 constants: FP64, or their FP32 copies on the mixed
-        # path), last_det r+w 16; writes the 12 nodal forces
-        "element": E * (16 + 14 * bm + 16 + 12 * bm) + v,
+        # path), the determinant's sign read and written (2); writes the 12 nodal forces
+        "element": E * (16 + 14 * bm + 2 + 12 * bm) + v,
         # per slice its element list (4 B per slice-element pair, L pairs), enodes + A, B once
         # (32E), contribution codes (2 B per contribution, C of them: the diagonal and upper
         # blocks' element terms), per summed block (N diagonal + (B - N) / 2 upper) its count,

@@ -14,7 +14,7 @@ struct ElemArgs {
     // the same per-element constants rounded to float, SoA [14][E] (jinv0 0-8, vol0, A, B, G,
     // Em): what the FP32 element kernel reads (half the bytes); nullptr on the FP64 path
     const float *elem32;
-    double *last_det;
+    int8_t *last_sign;  // [E] sign of the element's determinant at the last evaluation
     void *eforce;  // T SoA [12][E]: node_force[a][i] at (3a+i)*E + e
     int E;
     int calc_noise;
@@ -119,9 +119,11 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs g) {
     if (e >= g.E) return;
     R J[3][3], dpsi[12], det;
     element_geometry<R>(g.pos, g.enodes[e], J, dpsi, det);
-    const double last = g.last_det[e];
-    if (last * (double)det < 0) atomicAdd(&g.ctl->inversions, 1);
-    g.last_det[e] = (double)det;
+    // an inversion is a change of the determinant's sign (last_det * det < 0), so its sign is
+    // all that is kept
+    const int sg = det > (R)0 ? 1 : (det < (R)0 ? -1 : 0);
+    if ((int)g.last_sign[e] * sg < 0) atomicAdd(&g.ctl->inversions, 1);
+    g.last_sign[e] = (int8_t)sg;
     const R vol = (R)(1.0 / 6.0) * fabs(det);
 
     R A, B, G, Em, vol0, Ji0[9];  // Ji0[3 j + k] = jinv0[j][k]

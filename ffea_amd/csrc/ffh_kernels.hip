@@ -102,8 +102,8 @@ struct ffh_blob {
     // device arrays
     double4 *pos = nullptr;
     int4 *enodes = nullptr;
-    double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr,
-           *last_det = nullptr;
+    double *jinv0 = nullptr, *vol0 = nullptr, *A = nullptr, *B = nullptr, *G = nullptr, *Em = nullptr;
+    int8_t *last_sign = nullptr;
     float *elem32 = nullptr;  // FP32 element constants (mixed precision only)
     void *eforce = nullptr, *val = nullptr;
     uint32_t *nctb = nullptr, *rng_off = nullptr;
@@ -411,7 +411,7 @@ struct ffh_blob {
         g.G = G;
         g.Em = Em;
         g.elem32 = elem32;
-        g.last_det = last_det;
+        g.last_sign = last_sign;
         g.eforce = eforce;
         g.E = E;
         g.calc_noise = calc_noise;
@@ -1299,7 +1299,11 @@ static ffh_blob *create_impl(int num_nodes, int num_elements, const double *pos,
             for (int e = 0; e < E; e++) c32[(9 + r) * (size_t)E + e] = (float)(*cols[r])[e];
         rc |= b->upload(b->elem32, c32);
     }
-    rc |= b->upload(b->last_det, P.last_det);
+    {
+        std::vector<int8_t> sg(E);
+        for (int e = 0; e < E; e++) sg[e] = P.last_det[e] > 0.0 ? 1 : (P.last_det[e] < 0.0 ? -1 : 0);
+        rc |= b->upload(b->last_sign, sg);
+    }
     {
         char *ef = nullptr, *vv = nullptr;
         rc |= b->alloc(ef, 12 * (size_t)E * b->tsize);
