@@ -282,8 +282,8 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     auto tot2 = [&](const double *pa, int sa, const double *pb, int sb, double &ta, double &tb) {
         blob_total2(pa, pb, v.tot, sa, sb, b, s0, s1, sh, ta, tb);
     };
-    // the slice's shape and this row's z, p, q (and f for a true-residual pass) are loaded
-    // before the head, so they arrive while it reduces
+    // the slice's shape and this row's z, p, q are loaded before the head, so they arrive while
+    // it reduces (f, which only a true-residual pass reads, is loaded there)
     const int ci = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
@@ -291,7 +291,6 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     const size_t k = 3 * (size_t)row + ci;
     V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
     const double zk = (double)static_cast<const V *>(v.z)[k], pk = (double)p[k], qk = (double)q[k];
-    const double fk = kMixed ? v.f[k] : 0.0;
     if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
     const T *val = reinterpret_cast<const T *>(m.val);
     const V *__restrict__ z = static_cast<const V *>(v.z);
@@ -372,7 +371,7 @@ __global__ __launch_bounds__(192) void k_cg_spmv(SpmvArgs m, CgArgs v, int it, i
     q[k] = (V)qn;
     block_partial_at(pn * qn, v.parts_pq, slice, sh);
     if (kMixed && check) {
-        const double rt = fk - accx;
+        const double rt = v.f[k] - accx;
         static_cast<V *>(v.r)[k] = (V)rt;
         block_partial_at(rt * rt, v.parts_tr, slice, sh);
     }
@@ -432,21 +431,20 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
             parts_range2_as192(pa, pb, s0, s1, ta, tb);
         }
     };
-    // the slice's shape and this row's z, p, q (and f for a true-residual pass) are loaded
-    // before the head, so they arrive while it reduces
+    // the slice's shape and this row's z, p, q are loaded before the head, so they arrive while
+    // it reduces (f, which only a true-residual pass reads, is loaded there)
     const int lane = threadIdx.x & 63;
     const int row = slice * 64 + lane;
     const int w = m.slice_width[slice];
     const int64_t base = m.slice_off[slice];
     const size_t k = 3 * (size_t)row;
     V *p = static_cast<V *>(v.p), *q = static_cast<V *>(v.q);
-    double zk[3], pk[3], qk[3], fk[3];
+    double zk[3], pk[3], qk[3];
 #pragma unroll
     for (int ci = 0; ci < 3; ci++) {
         zk[ci] = (double)static_cast<const V *>(v.z)[k + ci];
         pk[ci] = (double)p[k + ci];
         qk[ci] = (double)q[k + ci];
-        fk[ci] = kMixed ? v.f[k + ci] : 0.0;
     }
     if (!cg_head<kMass, kMixed>(v, ctl, it, mode, slice == s0 && threadIdx.x == 0, tot, tot2, beta, check)) return;
     const T *val = reinterpret_cast<const T *>(m.val);
@@ -556,7 +554,7 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
         q[k + ci] = (V)qn;
         part += __shfl(wave_sum(pn * qn), 0, 64);  // ((0 + W0) + W1) + W2, as the 3-wave block
         if (kMixed && check) {
-            const double rt = fk[ci] - accx[ci];
+            const double rt = v.f[k + ci] - accx[ci];
             static_cast<V *>(v.r)[k + ci] = (V)rt;
             tpart += __shfl(wave_sum(rt * rt), 0, 64);
         }
