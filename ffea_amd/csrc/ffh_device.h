@@ -119,6 +119,33 @@ __device__ __forceinline__ void block_partial_at(double v, double *__restrict__ 
     if (threadIdx.x == 0) parts[idx] = t;
 }
 
+// Writes this block's sums of a and b to pa[idx] and pb[idx], each bitwise block_partial_at's,
+// with the two wave trees interleaved and one pair of barriers.  sh: >= 2 * blockDim.x/64 doubles.
+__device__ __forceinline__ void block_partial2_at(double a, double *__restrict__ pa, double b,
+                                                  double *__restrict__ pb, int idx, double *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        sh[w] = a;
+        sh[nw + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+        for (int i = 0; i < nw; i++) {
+            ta += sh[i];
+            tb += sh[nw + i];
+        }
+        pa[idx] = ta;
+        pb[idx] = tb;
+    }
+}
+
 // Visits [t0, t1) in groups: as many of exactly kMax as fit, then one group of the remaining
 // 1 .. kMax-1 -- group(std::integral_constant<int, G>(), first).  Every index a group touches is
 // in range and used, so the compiler issues a group's loads together ahead of their uses (with a

@@ -64,7 +64,7 @@ struct RhsArgs {
 
 template <typename T, typename V>
 __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
-    __shared__ double sh[4];
+    __shared__ double sh[8];
     __shared__ uint32_t rs[6][64];
     const int i = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int blk = xcd_block(v.nslices);
@@ -151,8 +151,7 @@ __global__ __launch_bounds__(192) void k_rhs(RhsArgs g, CgArgs v) {
         static_cast<V *>(v.p)[k] = (V)0.0;
         static_cast<V *>(v.q)[k] = (V)0.0;
         static_cast<V *>(v.z)[k] = (V)z;
-        block_partial_at(r * z, v.parts_rz, blk, sh);
-        block_partial_at(f * f, v.parts_ff, blk, sh);
+        block_partial2_at(r * z, v.parts_rz, f * f, v.parts_ff, blk, sh);
         if (v.parts_uu) block_partial_at(z * z, v.parts_uu, blk, sh);
     }
     const int b = v.slice_blob[blk];
@@ -573,7 +572,7 @@ __global__ __launch_bounds__(64) void k_cg_spmv1(SpmvArgs m, CgArgs v, int it, i
 // there (residual replacement).
 template <typename V, bool kMixed>
 __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
-    __shared__ double sh[4];
+    __shared__ double sh[8];
     const int slice = xcd_block(v.nslices);
     if (slice >= v.nslices) return;
     const int b = v.slice_blob[slice];
@@ -619,8 +618,7 @@ __global__ __launch_bounds__(192) void k_cg_update(CgArgs v, int it) {
     r[k] = (V)rn;
     const double zn = (double)(V)(dk * rn);
     z[k] = (V)zn;
-    block_partial_at(rn * rn, v.parts_rr, slice, sh);
-    block_partial_at(rn * zn, v.parts_rz, slice, sh);
+    block_partial2_at(rn * rn, v.parts_rr, rn * zn, v.parts_rz, slice, sh);
     if (v.parts_uu) {  // learned preconditioner follows: z holds u = D^-1 r
         static_cast<V *>(v.r_prev)[k] = (V)r0;
         block_partial_at(zn * zn, v.parts_uu, slice, sh);
